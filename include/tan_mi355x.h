@@ -1,0 +1,181 @@
+/*
+ * tan_mi355x.h -- C ABI of the MI355X (gfx950) TAN convolution library.
+ *
+ * The library implements TAN's time-varying multichannel convolution path
+ * (uniformly partitioned FFT convolution with a frequency-domain delay line
+ * and IR crossfade) and TANFFT on one GPU.  Every entry point takes plain
+ * pointers and sizes and returns an int32_t equal to an AMF_RESULT value
+ * (amf/public/include/core/Result.h:43-122).  Objects are thread-safe per
+ * object, as the reference's are (locks at ConvolutionImpl.h:210-216).
+ *
+ * Reference paths below are relative to the TAN checkout.
+ */
+#ifndef TAN_MI355X_H
+#define TAN_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tanx_ctx tanx_ctx;
+typedef struct tanx_conv tanx_conv;
+
+/* AMF_RESULT values used here (Result.h:43-122). */
+enum {
+    TANX_OK = 0,
+    TANX_FAIL = 1,
+    TANX_UNEXPECTED = 2,
+    TANX_INVALID_ARG = 4,
+    TANX_OUT_OF_MEMORY = 6,
+    TANX_INVALID_POINTER = 7,
+    TANX_NOT_IMPLEMENTED = 9,
+    TANX_NOT_SUPPORTED = 10,
+    TANX_NOT_FOUND = 11,
+    TANX_ALREADY_INITIALIZED = 12,
+    TANX_NOT_INITIALIZED = 13,
+    TANX_WRONG_STATE = 15,
+    TANX_NO_DEVICE = 17,
+    TANX_TAN_UNSUPPORTED_VERSION = 43
+};
+
+/* TAN_CONVOLUTION_METHOD (tan/tanlibrary/include/TrueAudioNext.h:81-98). */
+enum {
+    TANX_METHOD_FFT_OVERLAP_ADD = 0,
+    TANX_METHOD_FFT_PARTITIONED_UNIFORM = 1,
+    TANX_METHOD_FFT_PARTITIONED_NONUNIFORM = 2,
+    TANX_METHOD_FFT_UNIFORM_PARTITIONED = 3,
+    TANX_METHOD_FHT_UNIFORM_PARTITIONED = 4,
+    TANX_METHOD_FFT_UNIFORM_HEAD_TAIL = 5,
+    TANX_METHOD_FHT_UNIFORM_HEAD_TAIL = 6,
+    TANX_METHOD_TIME_DOMAIN = 7,
+    TANX_METHOD_FHT_NONUNIFORM_PARTITIONED = 8,
+    TANX_METHOD_FFT_NONUNIFORM_PARTITIONED = 9,
+    TANX_METHOD_USE_PROCESS_FINALIZE = 0x8000,
+    TANX_METHOD_USE_PROCESS_TAILTHREAD = 0xC000
+};
+
+/* TAN_CONVOLUTION_CHANNEL_FLAG / TAN_CONVOLUTION_OPERATION_FLAG
+ * (TrueAudioNext.h:105-120). */
+enum {
+    TANX_CHANNEL_FLAG_PROCESS = 0,
+    TANX_CHANNEL_FLAG_STOP_INPUT = 0x01,
+    TANX_CHANNEL_FLAG_FLUSH_STREAM = 0x02,
+    TANX_OPERATION_FLAG_NONE = 0,
+    TANX_OPERATION_FLAG_BLOCK_UNTIL_READY = 0x01
+};
+
+/* TAN_FFT_TRANSFORM_DIRECTION (TrueAudioNext.h:493-501). */
+enum {
+    TANX_FFT_FORWARD = 0,
+    TANX_FFT_BACKWARD = 1,
+    TANX_FFT_R2C_FORWARD = 2,
+    TANX_FFT_C2R_BACKWARD = 3,
+    TANX_FFT_R2C_PLANAR_FORWARD = 4,
+    TANX_FFT_C2R_PLANAR_BACKWARD = 5
+};
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces TANCreateContext + TANContext::InitOpenCL
+ * (core/TANContextImpl.cpp:87-132, 264-328): binds HIP device `device` and
+ * creates the two streams that stand in for the general (IR update) and
+ * convolution (process) queues.  `version` must have major 1
+ * (TANContextImpl.cpp:114-120), else TANX_TAN_UNSUPPORTED_VERSION. */
+int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out);
+/* TANContext::Terminate (TANContextImpl.cpp:158-179). */
+int32_t tanx_ctx_destroy(tanx_ctx *ctx);
+int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
+
+/* ---- convolution ----------------------------------------------------------- */
+
+/* TANCreateConvolution + TANConvolution::Init/InitGpu
+ * (convolution/ConvolutionImpl.cpp:66-80, 179-305, 1113-1259).  The uniform
+ * methods (1, 3, 4, 5, 6) and overlap-add (0) map to the one uniformly
+ * partitioned engine; bit 0x8000 selects ProcessFinalize mode.  B must be a
+ * power of two in [64, 2048] (the reference GPU path requires a power of
+ * two, ConvolutionImpl.cpp:1137-1146).  The IR length is rounded up to a
+ * power of two as the reference does (ConvolutionImpl.cpp:1230-1235). */
+int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
+                         uint32_t block, uint32_t channels, tanx_conv **out);
+/* TANConvolution::Terminate (ConvolutionImpl.cpp:307-355). */
+int32_t tanx_conv_destroy(tanx_conv *conv);
+
+/* UpdateResponseTD(float* ppBuffer[], ...) (ConvolutionImpl.cpp:357-382,
+ * 407-695): ir[c] holds n <= ir_len host floats, zero padded to the rounded
+ * length.  flags may be NULL; channels with STOP_INPUT keep their IR; FLUSH
+ * flushes the channel first.  The new IR is crossfaded in two Process calls
+ * later, as in the reference when its update thread keeps up. */
+int32_t tanx_conv_update_td(tanx_conv *conv, const float *const *ir, size_t n,
+                            const uint32_t *flags, uint32_t opflags);
+/* Device-memory variant (stands in for UpdateResponseTD(cl_mem[])):
+ * channel c's IR at d_ir + c * stride floats, device resident. */
+int32_t tanx_conv_update_td_device(tanx_conv *conv, const float *d_ir, size_t stride,
+                                   size_t n, const uint32_t *flags, uint32_t opflags);
+
+/* Process(float* in[], float* out[], ...) (ConvolutionImpl.cpp:809-833,
+ * 848-1043).  Processes one block of `block` samples per channel when
+ * n >= block (processed = block); n < block processes nothing.  Outputs are
+ * complete on return. */
+int32_t tanx_conv_process(tanx_conv *conv, const float *const *in, float *const *out,
+                          size_t n, const uint32_t *flags, size_t *processed);
+/* Device-memory variant (stands in for Process(cl_mem[], cl_mem[])):
+ * channel c reads d_in + c * in_stride and writes d_out + c * out_stride.
+ * Asynchronous: ordered on the library's process stream; use
+ * tanx_conv_synchronize before reading d_out on the host. */
+int32_t tanx_conv_process_device(tanx_conv *conv, const float *d_in, size_t in_stride,
+                                 float *d_out, size_t out_stride, size_t n,
+                                 const uint32_t *flags, size_t *processed);
+/* ProcessFinalize (ConvolutionImpl.cpp:1046-1079): enqueues the tail
+ * multiply-accumulate for the next block.  Implicit after each Process unless
+ * the object was created with the 0x8000 bit. */
+int32_t tanx_conv_finalize(tanx_conv *conv);
+/* Flush (ConvolutionImpl.cpp:1261-1325): zero one channel's history. */
+int32_t tanx_conv_flush(tanx_conv *conv, uint32_t channel);
+/* GetNextFreeChannel (ConvolutionImpl.cpp:1084-1110). */
+int32_t tanx_conv_get_next_free_channel(tanx_conv *conv, uint32_t *channel,
+                                        const uint32_t *flags);
+/* Waits for all work of this object (process and update streams). */
+int32_t tanx_conv_synchronize(tanx_conv *conv);
+
+typedef struct tanx_conv_info {
+    uint32_t ir_len;       /* rounded IR length */
+    uint32_t block;        /* B */
+    uint32_t channels;     /* C */
+    uint32_t partitions;   /* P = ir_len / B */
+    uint32_t tail_chunks;  /* partition chunks per channel in the tail kernel */
+    uint32_t finalize_mode;
+    uint64_t device_bytes; /* HBM held by the object */
+    uint64_t ticks;        /* blocks processed */
+    uint64_t crossfades;   /* crossfade blocks processed */
+} tanx_conv_info;
+int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
+/* Sets the partition chunking of the tail kernel (0 = automatic). */
+int32_t tanx_conv_set_tail_chunks(tanx_conv *conv, uint32_t chunks);
+/* 1 if the last processed block was a crossfade block. */
+int32_t tanx_conv_last_was_crossfade(const tanx_conv *conv);
+/* Stream on which process work is ordered (a hipStream_t). */
+void *tanx_conv_process_stream(tanx_conv *conv);
+
+/* ---- FFT ------------------------------------------------------------------- */
+
+/* TANFFT::Transform(direction, log2len, channels, float* in[], float* out[])
+ * (fft/FFTImpl.cpp:507-575): each channel buffer holds 2^(log2len+1) floats;
+ * complex data interleaved; backward transforms scaled by 1/N; planar
+ * layouts keep im at N/2+8.  log2len in [1, 12] (planar: [4, 12]). */
+int32_t tanx_fft_transform(tanx_ctx *ctx, uint32_t direction, uint32_t log2len,
+                           uint32_t channels, const float *const *in, float *const *out);
+/* Device-memory variant (stands in for Transform(cl_mem[]) and
+ * TransformBatchGPU): channel c at d_in/d_out + c * stride floats. */
+int32_t tanx_fft_transform_device(tanx_ctx *ctx, uint32_t direction, uint32_t log2len,
+                                  uint32_t channels, const float *d_in, float *d_out,
+                                  size_t stride);
+
+const char *tanx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

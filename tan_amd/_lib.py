@@ -1,0 +1,73 @@
+"""Loader for the in-tree HIP library (tan_amd/lib/libtan_mi355x.so).
+
+The library is the product: there is no CPU fallback.  A missing or
+unloadable library raises immediately.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtan_mi355x.so")
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_u32 = ctypes.c_uint32
+_sz = ctypes.c_size_t
+
+
+class ConvInfo(ctypes.Structure):
+    _fields_ = [("ir_len", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("channels", ctypes.c_uint32), ("partitions", ctypes.c_uint32),
+                ("tail_chunks", ctypes.c_uint32), ("finalize_mode", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64),
+                ("crossfades", ctypes.c_uint64)]
+
+
+_SIGS = {
+    "tanx_ctx_create": (_i32, [ctypes.c_uint64, _i32, ctypes.POINTER(_vp)]),
+    "tanx_ctx_destroy": (_i32, [_vp]),
+    "tanx_ctx_device": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "tanx_conv_create": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "tanx_conv_destroy": (_i32, [_vp]),
+    "tanx_conv_update_td": (_i32, [_vp, ctypes.POINTER(_f32p), _sz, _u32p, _u32]),
+    "tanx_conv_update_td_device": (_i32, [_vp, _vp, _sz, _sz, _u32p, _u32]),
+    "tanx_conv_process": (_i32, [_vp, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p), _sz, _u32p,
+                                 ctypes.POINTER(_sz)]),
+    "tanx_conv_process_device": (_i32, [_vp, _vp, _sz, _vp, _sz, _sz, _u32p, ctypes.POINTER(_sz)]),
+    "tanx_conv_finalize": (_i32, [_vp]),
+    "tanx_conv_flush": (_i32, [_vp, _u32]),
+    "tanx_conv_get_next_free_channel": (_i32, [_vp, _u32p, _u32p]),
+    "tanx_conv_synchronize": (_i32, [_vp]),
+    "tanx_conv_get_info": (_i32, [_vp, ctypes.POINTER(ConvInfo)]),
+    "tanx_conv_set_tail_chunks": (_i32, [_vp, _u32]),
+    "tanx_conv_last_was_crossfade": (_i32, [_vp]),
+    "tanx_conv_process_stream": (_vp, [_vp]),
+    "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),
+    "tanx_fft_transform_device": (_i32, [_vp, _u32, _u32, _u32, _vp, _vp, _sz]),
+    "tanx_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load():
+    """Load and type the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"tan_amd: HIP library not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)

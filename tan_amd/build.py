@@ -1,0 +1,43 @@
+"""Build recipe for the HIP library (gfx950) and the C++ interface shim.
+
+Invoked by __graft_entry__.build(); compiles every time it is called.
+"""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "tan_amd", "csrc")
+LIBDIR = os.path.join(ROOT, "tan_amd", "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# packed FP32 instructions stay off (only the compiler's scalar/vector FP32)
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + os.path.join(ROOT, "include")]
+
+SOURCES = ["tan_kernels.hip", "tan_engine.cpp", "tan_api.cpp"]
+
+
+def build_library(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libtan_mi355x.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle(verbose=False):
+    cmd = ["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+if __name__ == "__main__":
+    build_library(verbose=True)
+    build_oracle(verbose=True)

@@ -1,0 +1,828 @@
+// Host engine of the MI355X TAN convolution library: contexts (device +
+// process/update streams), convolution objects (device-resident IR slots,
+// FDL and tail partial sums; the IR-switch state machine of the reference's
+// TANConvolutionImpl::Process) and the C ABI declared in tan_mi355x.h.
+#include "tan_internal.h"
+#include "../../include/tan_mi355x.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+using namespace tanx;
+
+namespace {
+
+int32_t map_hip(hipError_t e)
+{
+    if (e == hipSuccess) return TANX_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return TANX_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return TANX_NO_DEVICE;
+    if (e == hipErrorInvalidValue) return TANX_INVALID_ARG;
+    return TANX_UNEXPECTED;
+}
+
+#define TANX_CHECK(expr)                                          \
+    do {                                                          \
+        hipError_t _e = (expr);                                   \
+        if (_e != hipSuccess) {                                   \
+            std::fprintf(stderr, "tan_mi355x: %s failed: %s (%s:%d)\n", #expr, \
+                         hipGetErrorString(_e), __FILE__, __LINE__); \
+            return map_hip(_e);                                   \
+        }                                                         \
+    } while (0)
+
+// Makes `dev` current for the scope and restores the caller's device.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+inline bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct tanx_ctx {
+    int device = 0;
+    hipStream_t proc = nullptr;  // convolution queue equivalent
+    hipStream_t upd = nullptr;   // general queue equivalent (IR updates, FFT)
+    float2 *tw = nullptr;        // twiddle tables, see tw_table()
+    std::mutex fft_mtx;
+    float *fft_dev = nullptr;
+    size_t fft_dev_floats = 0;
+    float *fft_host = nullptr;
+    size_t fft_host_floats = 0;
+};
+
+static int32_t ctx_upload_twiddles(tanx_ctx *ctx)
+{
+    const size_t total = (size_t(1) << (kMaxTwLog2 + 1)) - 1;
+    std::vector<float2> h(total);
+    for (int l = 0; l <= kMaxTwLog2; l++) {
+        const size_t n = size_t(1) << l, off = n - 1;
+        for (size_t t = 0; t < n; t++) {
+            // exp(-2 pi i t/n) in FP64, rounded once to FP32
+            const double ang = -2.0 * M_PI * (double)t / (double)n;
+            h[off + t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+    }
+    TANX_CHECK(hipMalloc(&ctx->tw, total * sizeof(float2)));
+    TANX_CHECK(hipMemcpy(ctx->tw, h.data(), total * sizeof(float2), hipMemcpyHostToDevice));
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out)
+{
+    if (!out) return TANX_INVALID_POINTER;
+    *out = nullptr;
+    if ((version >> 48) != 1) return TANX_TAN_UNSUPPORTED_VERSION;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TANX_NO_DEVICE;
+    if (device < 0 || device >= ndev) return TANX_INVALID_ARG;
+    DeviceGuard g(device);
+    if (!g.ok) return TANX_NO_DEVICE;
+    tanx_ctx *ctx = new (std::nothrow) tanx_ctx();
+    if (!ctx) return TANX_OUT_OF_MEMORY;
+    ctx->device = device;
+    int32_t rc = TANX_OK;
+    // default-priority streams only
+    if (hipStreamCreateWithFlags(&ctx->proc, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->upd, hipStreamNonBlocking) != hipSuccess) {
+        rc = TANX_UNEXPECTED;
+    }
+    if (rc == TANX_OK) rc = ctx_upload_twiddles(ctx);
+    if (rc != TANX_OK) {
+        tanx_ctx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
+{
+    if (!ctx) return TANX_INVALID_POINTER;
+    {
+        DeviceGuard g(ctx->device);
+        if (ctx->proc) { (void)hipStreamSynchronize(ctx->proc); (void)hipStreamDestroy(ctx->proc); }
+        if (ctx->upd) { (void)hipStreamSynchronize(ctx->upd); (void)hipStreamDestroy(ctx->upd); }
+        if (ctx->tw) (void)hipFree(ctx->tw);
+        if (ctx->fft_dev) (void)hipFree(ctx->fft_dev);
+        if (ctx->fft_host) (void)hipHostFree(ctx->fft_host);
+    }
+    delete ctx;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device)
+{
+    if (!ctx || !device) return TANX_INVALID_POINTER;
+    *device = ctx->device;
+    return TANX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// convolution object
+// ---------------------------------------------------------------------------
+static constexpr int kMaxChunks = 64;
+static constexpr int kRunRing = 8;
+
+struct tanx_conv {
+    tanx_ctx *ctx = nullptr;
+    std::mutex mtx;
+    uint32_t method = 0;
+    bool fmode = false;
+    int L = 0, B = 0, C = 0, len = 0, P = 0;
+    int nchunk = 1, pc = 1, maxchunk = 1;
+    int tail_nchunk = 1;  // chunking of the partial sums the next head reads
+    bool fixed_chunks = false;
+
+    // device memory (one allocation carved into pieces)
+    void *dmem = nullptr;
+    size_t dbytes = 0;
+    float2 *H = nullptr;     // [3][C][P][B]
+    float2 *F = nullptr;     // [C][P][B]
+    float2 *T = nullptr;     // [2][C][maxchunk][B]
+    float *xprev = nullptr;  // [C][B]
+    float *din = nullptr;    // [C][B]
+    float *dout = nullptr;   // [C][B]
+    float *dir = nullptr;    // [C][len]
+    int *dchan = nullptr;    // [C]
+    uint8_t *drun = nullptr; // [C]
+
+    // pinned host staging
+    float *hin = nullptr, *hout = nullptr, *hir = nullptr;
+    int *hchan = nullptr;
+    uint8_t *hrun = nullptr;  // [kRunRing][C]
+    hipEvent_t run_ev[kRunRing] = {};
+    int run_ring = 0;
+
+    hipEvent_t ev_upd = nullptr;   // last IR update finished
+    hipEvent_t ev_proc = nullptr;  // last process work enqueued
+    hipEvent_t ev_out = nullptr;   // host-path output copied
+
+    // IR switch state (ConvolutionImpl.cpp:115-117, 869-927)
+    int cur = 1, prev = 0, upd = 2;
+    bool pending = false, ready = false, delayed = false, crossfading = false, last_xfade = false;
+    int pos = 0;  // FDL ring slot of the next block
+    std::vector<uint8_t> staged;  // channel staged since the last hand-off
+
+    // channel flags (ConvolutionImpl.cpp:2868-2907)
+    std::vector<uint8_t> available, flushed, run_now, run_dev;
+    std::vector<int> tail_left;
+
+    uint64_t ticks = 0, xfades = 0;
+
+    size_t slot_elems() const { return (size_t)C * P * B; }
+    float2 *slot_ptr(int s) const { return H + (size_t)s * slot_elems(); }
+    float2 *tail_ptr(int f) const { return T + (size_t)f * C * maxchunk * B; }
+};
+
+static void conv_free(tanx_conv *cv)
+{
+    if (!cv) return;
+    if (cv->ctx) {
+        DeviceGuard g(cv->ctx->device);
+        (void)hipStreamSynchronize(cv->ctx->proc);
+        (void)hipStreamSynchronize(cv->ctx->upd);
+        if (cv->dmem) (void)hipFree(cv->dmem);
+        if (cv->hin) (void)hipHostFree(cv->hin);
+        if (cv->hout) (void)hipHostFree(cv->hout);
+        if (cv->hir) (void)hipHostFree(cv->hir);
+        if (cv->hchan) (void)hipHostFree(cv->hchan);
+        if (cv->hrun) (void)hipHostFree(cv->hrun);
+        for (auto &e : cv->run_ev) if (e) (void)hipEventDestroy(e);
+        if (cv->ev_upd) (void)hipEventDestroy(cv->ev_upd);
+        if (cv->ev_proc) (void)hipEventDestroy(cv->ev_proc);
+        if (cv->ev_out) (void)hipEventDestroy(cv->ev_out);
+    }
+    delete cv;
+}
+
+static int auto_chunks(int C, int P)
+{
+    // enough workgroups to cover 256 CUs several times at small channel
+    // counts, while the partial sums stay a small fraction of the streamed
+    // bytes
+    int want = (1024 + C - 1) / C;
+    want = std::max(1, std::min(want, 16));
+    return std::min(want, std::max(1, P - 1));
+}
+
+static void set_chunks(tanx_conv *cv, int n)
+{
+    const int parts = std::max(1, cv->P - 1);
+    n = std::max(1, std::min(n, std::min(cv->maxchunk, parts)));
+    cv->pc = (parts + n - 1) / n;
+    cv->nchunk = (parts + cv->pc - 1) / cv->pc;
+}
+
+extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
+                                    uint32_t block, uint32_t channels, tanx_conv **out)
+{
+    if (!ctx || !out) return TANX_INVALID_POINTER;
+    *out = nullptr;
+    if (ir_len == 0 || block == 0 || channels == 0) return TANX_INVALID_ARG;
+    if (!is_pow2(block) || block < 64 || block > 1024) return TANX_INVALID_ARG;
+    const uint32_t base = method & ~uint32_t(TANX_METHOD_USE_PROCESS_TAILTHREAD);
+    switch (base) {
+    case TANX_METHOD_FFT_OVERLAP_ADD:
+    case TANX_METHOD_FFT_PARTITIONED_UNIFORM:
+    case TANX_METHOD_FFT_UNIFORM_PARTITIONED:
+    case TANX_METHOD_FHT_UNIFORM_PARTITIONED:
+    case TANX_METHOD_FFT_UNIFORM_HEAD_TAIL:
+    case TANX_METHOD_FHT_UNIFORM_HEAD_TAIL:
+        break;
+    default:
+        return TANX_NOT_SUPPORTED;
+    }
+    uint32_t len = 1;
+    while (len < ir_len) len <<= 1;
+    if (len < block) len = block;
+    if (len > (1u << 24)) return TANX_INVALID_ARG;
+
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return TANX_NO_DEVICE;
+    tanx_conv *cv = new (std::nothrow) tanx_conv();
+    if (!cv) return TANX_OUT_OF_MEMORY;
+    cv->ctx = ctx;
+    cv->method = method;
+    cv->fmode = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
+    cv->L = (int)ir_len;
+    cv->B = (int)block;
+    cv->C = (int)channels;
+    cv->len = (int)len;
+    cv->P = (int)(len / block);
+    cv->maxchunk = std::max(1, std::min(kMaxChunks, cv->P - 1));
+    set_chunks(cv, auto_chunks(cv->C, cv->P));
+    cv->tail_nchunk = cv->nchunk;
+
+    const size_t C = cv->C, P = cv->P, B = cv->B;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t bH = al(3 * C * P * B * sizeof(float2));
+    const size_t bF = al(C * P * B * sizeof(float2));
+    const size_t bT = al(2 * C * (size_t)cv->maxchunk * B * sizeof(float2));
+    const size_t bX = al(C * B * sizeof(float));
+    const size_t bIR = al(C * (size_t)len * sizeof(float));
+    const size_t bCh = al(C * sizeof(int));
+    const size_t bRun = al(C);
+    cv->dbytes = bH + bF + bT + 3 * bX + bIR + bCh + bRun;
+    int32_t rc = TANX_OK;
+    auto fail = [&](int32_t r) { conv_free(cv); return r; };
+    hipError_t e = hipMalloc(&cv->dmem, cv->dbytes);
+    if (e != hipSuccess) { cv->dmem = nullptr; return fail(map_hip(e)); }
+    char *p = static_cast<char *>(cv->dmem);
+    cv->H = reinterpret_cast<float2 *>(p); p += bH;
+    cv->F = reinterpret_cast<float2 *>(p); p += bF;
+    cv->T = reinterpret_cast<float2 *>(p); p += bT;
+    cv->xprev = reinterpret_cast<float *>(p); p += bX;
+    cv->din = reinterpret_cast<float *>(p); p += bX;
+    cv->dout = reinterpret_cast<float *>(p); p += bX;
+    cv->dir = reinterpret_cast<float *>(p); p += bIR;
+    cv->dchan = reinterpret_cast<int *>(p); p += bCh;
+    cv->drun = reinterpret_cast<uint8_t *>(p); p += bRun;
+
+    if (hipHostMalloc(&cv->hin, C * B * sizeof(float)) != hipSuccess ||
+        hipHostMalloc(&cv->hout, C * B * sizeof(float)) != hipSuccess ||
+        hipHostMalloc(&cv->hir, C * (size_t)len * sizeof(float)) != hipSuccess ||
+        hipHostMalloc(&cv->hchan, C * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&cv->hrun, kRunRing * C) != hipSuccess)
+        return fail(TANX_OUT_OF_MEMORY);
+    for (auto &ev : cv->run_ev)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(TANX_UNEXPECTED);
+    if (hipEventCreateWithFlags(&cv->ev_upd, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&cv->ev_proc, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&cv->ev_out, hipEventDisableTiming) != hipSuccess)
+        return fail(TANX_UNEXPECTED);
+
+    // all state starts at zero: the initial IR slot is silent
+    // (ConvolutionImpl.cpp:115-117, 1640-1650)
+    e = launch_zero(cv->dmem, bH + bF + bT + 3 * bX, ctx->proc);
+    if (e == hipSuccess) {
+        std::memset(cv->hrun, 1, C);
+        e = hipMemcpyAsync(cv->drun, cv->hrun, C, hipMemcpyHostToDevice, ctx->proc);
+    }
+    if (e == hipSuccess) e = hipEventRecord(cv->ev_upd, ctx->upd);
+    if (e == hipSuccess) e = hipEventRecord(cv->ev_proc, ctx->proc);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->proc);
+    if (e != hipSuccess) return fail(map_hip(e));
+
+    cv->staged.assign(C, 0);
+    cv->available.assign(C, 0);
+    cv->flushed.assign(C, 0);
+    cv->run_now.assign(C, 1);
+    cv->run_dev.assign(C, 1);
+    cv->tail_left.assign(C, 0);
+    (void)rc;
+    *out = cv;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_destroy(tanx_conv *cv)
+{
+    if (!cv) return TANX_INVALID_POINTER;
+    conv_free(cv);
+    return TANX_OK;
+}
+
+// zero one channel's history: FDL ring, previous block, tail partial sums
+static int32_t flush_channel(tanx_conv *cv, int ch)
+{
+    hipStream_t s = cv->ctx->proc;
+    const size_t B = cv->B;
+    hipError_t e = launch_zero(cv->F + (size_t)ch * cv->P * B, (size_t)cv->P * B * sizeof(float2), s);
+    if (e == hipSuccess) e = launch_zero(cv->xprev + (size_t)ch * B, B * sizeof(float), s);
+    // partial sums are laid out [C][tail_nchunk][B]: the next head reads them
+    for (int f = 0; f < 2 && e == hipSuccess; f++)
+        e = launch_zero(cv->tail_ptr(f) + (size_t)ch * cv->tail_nchunk * B,
+                        (size_t)cv->tail_nchunk * B * sizeof(float2), s);
+    return map_hip(e);
+}
+
+// mirrors the channel flag bookkeeping of ProcessInternal
+// (ConvolutionImpl.cpp:2868-2907) and returns the running count
+static int32_t channel_flags(tanx_conv *cv, const uint32_t *flags, size_t n, int *nrun)
+{
+    int running = 0;
+    for (int ch = 0; ch < cv->C; ch++) {
+        bool do_flush = false;
+        if (!flags) {
+            cv->available[ch] = 0;
+            cv->tail_left[ch] = cv->len;
+        } else if (flags[ch] & TANX_CHANNEL_FLAG_STOP_INPUT) {
+            if (flags[ch] & TANX_CHANNEL_FLAG_FLUSH_STREAM) {
+                cv->available[ch] = 1;
+                cv->tail_left[ch] = 0;
+            } else if (cv->tail_left[ch] > 0) {
+                cv->available[ch] = 0;
+                cv->tail_left[ch] -= (int)n;
+                if (cv->tail_left[ch] <= 0) do_flush = true;
+            } else {
+                cv->available[ch] = 1;
+                cv->tail_left[ch] = 0;
+            }
+        } else {
+            cv->available[ch] = 0;
+            cv->tail_left[ch] = cv->len;
+        }
+        if (flags && (flags[ch] & TANX_CHANNEL_FLAG_FLUSH_STREAM)) {
+            if (!cv->flushed[ch]) do_flush = true;
+            cv->flushed[ch] = 1;
+        } else {
+            cv->flushed[ch] = 0;
+        }
+        if (do_flush) {
+            int32_t rc = flush_channel(cv, ch);
+            if (rc != TANX_OK) return rc;
+        }
+        cv->run_now[ch] = cv->available[ch] ? 0 : 1;
+        running += cv->run_now[ch];
+    }
+    *nrun = running;
+    return TANX_OK;
+}
+
+static int32_t sync_run_mask(tanx_conv *cv)
+{
+    if (cv->run_now == cv->run_dev) return TANX_OK;
+    const int slot = cv->run_ring;
+    cv->run_ring = (cv->run_ring + 1) % kRunRing;
+    TANX_CHECK(hipEventSynchronize(cv->run_ev[slot]));  // pinned entry free again
+    uint8_t *h = cv->hrun + (size_t)slot * cv->C;
+    std::memcpy(h, cv->run_now.data(), cv->C);
+    TANX_CHECK(hipMemcpyAsync(cv->drun, h, cv->C, hipMemcpyHostToDevice, cv->ctx->proc));
+    TANX_CHECK(hipEventRecord(cv->run_ev[slot], cv->ctx->proc));
+    cv->run_dev = cv->run_now;
+    return TANX_OK;
+}
+
+// ProcessFinalize: the tail for the next block, with both filters when the
+// next block is the crossfade block; then the update hand-off that the
+// reference's update thread performs (ConvolutionImpl.cpp:1046-1079,
+// 2004-2199).
+static int32_t enqueue_tail(tanx_conv *cv)
+{
+    tanx_ctx *ctx = cv->ctx;
+    const int nf = cv->delayed ? 2 : 1;
+    if (nf == 2) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
+    TailArgs a{};
+    a.fdl = cv->F;
+    a.P = cv->P;
+    a.base = cv->pos;
+    a.Ha = cv->slot_ptr(cv->cur);
+    a.Hb = nf == 2 ? cv->slot_ptr(cv->upd) : nullptr;
+    a.Ta = cv->tail_ptr(0);
+    a.Tb = nf == 2 ? cv->tail_ptr(1) : nullptr;
+    a.nchunk = cv->nchunk;
+    a.pc = cv->pc;
+    a.run = cv->drun;
+    TANX_CHECK(launch_tail(cv->B, cv->C, nf, a, ctx->proc));
+    cv->tail_nchunk = cv->nchunk;
+    TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
+    if (!cv->crossfading && cv->pending) {
+        cv->pending = false;
+        cv->ready = true;
+        std::fill(cv->staged.begin(), cv->staged.end(), 0);
+    }
+    return TANX_OK;
+}
+
+// Process state machine (ConvolutionImpl.cpp:848-1043) and the head launch.
+static int32_t process_head(tanx_conv *cv, const float *din, long long in_stride, float *dout,
+                            long long out_stride, size_t n, const uint32_t *flags)
+{
+    tanx_ctx *ctx = cv->ctx;
+    int nrun = 0;
+    int32_t rc = channel_flags(cv, flags, n, &nrun);
+    if (rc != TANX_OK) return rc;
+    if (nrun == 0) return TANX_WRONG_STATE;
+    rc = sync_run_mask(cv);
+    if (rc != TANX_OK) return rc;
+
+    bool xfade = false;
+    if (!cv->pending && cv->ready && !cv->delayed) {
+        cv->ready = false;
+        cv->delayed = true;
+    } else {
+        xfade = cv->delayed;
+    }
+    if (xfade) {
+        cv->prev = cv->cur;
+        cv->cur = (cv->cur + 1) % 3;
+        cv->upd = (cv->upd + 1) % 3;
+        TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
+    }
+    HeadArgs a{};
+    a.xin = din;
+    a.in_stride = in_stride;
+    a.yout = dout;
+    a.out_stride = out_stride;
+    a.xprev = cv->xprev;
+    a.fdl = cv->F;
+    a.P = cv->P;
+    a.slot = cv->pos;
+    a.Ha = cv->slot_ptr(xfade ? cv->prev : cv->cur);
+    a.Hb = xfade ? cv->slot_ptr(cv->cur) : nullptr;
+    a.Ta = cv->tail_ptr(0);
+    a.Tb = xfade ? cv->tail_ptr(1) : nullptr;
+    a.nchunk = cv->tail_nchunk;
+    a.xfade = xfade ? 1 : 0;
+    a.run = cv->drun;
+    const int lB = __builtin_ctz((unsigned)cv->B);
+    a.twM = tw_table(ctx->tw, lB);
+    a.twN = tw_table(ctx->tw, lB + 1);
+    TANX_CHECK(launch_head(cv->B, cv->C, a, ctx->proc));
+    TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
+    cv->pos = (cv->pos + 1) % cv->P;
+    cv->crossfading = xfade;
+    cv->last_xfade = xfade;
+    if (xfade) {
+        cv->delayed = false;
+        cv->xfades++;
+    }
+    cv->ticks++;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_process(tanx_conv *cv, const float *const *in, float *const *out,
+                                     size_t n, const uint32_t *flags, size_t *processed)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (!in || !out) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    if (processed) *processed = 0;
+    if (n < (size_t)cv->B) return TANX_OK;
+    DeviceGuard g(cv->ctx->device);
+    const size_t B = cv->B;
+    for (int ch = 0; ch < cv->C; ch++) {
+        if (!in[ch]) return TANX_INVALID_ARG;
+        std::memcpy(cv->hin + ch * B, in[ch], B * sizeof(float));
+    }
+    hipStream_t s = cv->ctx->proc;
+    TANX_CHECK(hipMemcpyAsync(cv->din, cv->hin, cv->C * B * sizeof(float), hipMemcpyHostToDevice, s));
+    int32_t rc = process_head(cv, cv->din, (long long)B, cv->dout, (long long)B, n, flags);
+    if (rc != TANX_OK) return rc;
+    TANX_CHECK(hipMemcpyAsync(cv->hout, cv->dout, cv->C * B * sizeof(float), hipMemcpyDeviceToHost, s));
+    TANX_CHECK(hipEventRecord(cv->ev_out, s));
+    if (!cv->fmode) {
+        rc = enqueue_tail(cv);
+        if (rc != TANX_OK) return rc;
+    }
+    TANX_CHECK(hipEventSynchronize(cv->ev_out));
+    for (int ch = 0; ch < cv->C; ch++)
+        if (cv->run_now[ch] && out[ch]) std::memcpy(out[ch], cv->hout + ch * B, B * sizeof(float));
+    if (processed) *processed = B;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, size_t in_stride,
+                                            float *d_out, size_t out_stride, size_t n,
+                                            const uint32_t *flags, size_t *processed)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (!d_in || !d_out) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    if (processed) *processed = 0;
+    if (n < (size_t)cv->B) return TANX_OK;
+    if (in_stride < (size_t)cv->B || out_stride < (size_t)cv->B) return TANX_INVALID_ARG;
+    DeviceGuard g(cv->ctx->device);
+    int32_t rc = process_head(cv, d_in, (long long)in_stride, d_out, (long long)out_stride, n, flags);
+    if (rc != TANX_OK) return rc;
+    if (!cv->fmode) {
+        rc = enqueue_tail(cv);
+        if (rc != TANX_OK) return rc;
+    }
+    if (processed) *processed = (size_t)cv->B;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_finalize(tanx_conv *cv)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    return enqueue_tail(cv);
+}
+
+// Stages an IR update.  Slot choice: while a crossfade is armed (delayed),
+// the reference blocks the caller until the crossfade block has run
+// (ConvolutionImpl.cpp:425-428); here the update is written into the slot
+// that becomes the update slot after that crossfade, and handed off by the
+// first regular block after it, which is the order the reference produces
+// for a caller on another thread.
+static int32_t update_common(tanx_conv *cv, const float *const *hir, const float *dir, size_t dstride,
+                             size_t n, const uint32_t *flags, uint32_t opflags)
+{
+    tanx_ctx *ctx = cv->ctx;
+    if (n > (size_t)cv->L) return TANX_INVALID_ARG;
+    TANX_CHECK(hipEventSynchronize(cv->ev_upd));  // staging buffers free
+    for (int ch = 0; ch < cv->C; ch++) {
+        if (flags && (flags[ch] & TANX_CHANNEL_FLAG_FLUSH_STREAM)) {
+            int32_t rc = flush_channel(cv, ch);
+            if (rc != TANX_OK) return rc;
+        }
+    }
+    const int target = cv->delayed ? cv->prev : cv->upd;
+    const int keep = cv->delayed ? cv->upd : cv->cur;
+    int nupd = 0;
+    std::vector<int> keep_list;
+    for (int ch = 0; ch < cv->C; ch++) {
+        if (flags && (flags[ch] & TANX_CHANNEL_FLAG_STOP_INPUT)) {
+            if (!cv->staged[ch]) keep_list.push_back(ch);
+            continue;
+        }
+        cv->hchan[nupd++] = ch;
+    }
+    // the update slot must not be overwritten while queued process work reads it
+    TANX_CHECK(hipStreamWaitEvent(ctx->upd, cv->ev_proc, 0));
+    const size_t slotElems = (size_t)cv->P * cv->B;
+    if (nupd > 0) {
+        const float *src = dir;
+        long long stride = (long long)dstride;
+        if (hir) {
+            for (int k = 0; k < nupd; k++) {
+                const int ch = cv->hchan[k];
+                if (!hir[ch]) return TANX_INVALID_ARG;
+                std::memcpy(cv->hir + (size_t)ch * cv->len, hir[ch], n * sizeof(float));
+                TANX_CHECK(hipMemcpyAsync(cv->dir + (size_t)ch * cv->len, cv->hir + (size_t)ch * cv->len,
+                                          n * sizeof(float), hipMemcpyHostToDevice, ctx->upd));
+            }
+            src = cv->dir;
+            stride = cv->len;
+        }
+        TANX_CHECK(hipMemcpyAsync(cv->dchan, cv->hchan, nupd * sizeof(int), hipMemcpyHostToDevice, ctx->upd));
+        IrArgs a{};
+        a.ir = src;
+        a.stride = stride;
+        a.n = (long long)n;
+        a.chan = cv->dchan;
+        a.H = cv->slot_ptr(target);
+        a.P = cv->P;
+        const int lB = __builtin_ctz((unsigned)cv->B);
+        a.twM = tw_table(ctx->tw, lB);
+        a.twN = tw_table(ctx->tw, lB + 1);
+        TANX_CHECK(launch_ir_transform(cv->B, nupd, a, ctx->upd));
+        for (int k = 0; k < nupd; k++) cv->staged[cv->hchan[k]] = 1;
+    }
+    // channels not updated keep the IR that is current when the update is
+    // handed off (ConvolutionImpl.cpp:2149-2158)
+    for (int ch : keep_list) {
+        TANX_CHECK(hipMemcpyAsync(cv->slot_ptr(target) + (size_t)ch * slotElems,
+                                  cv->slot_ptr(keep) + (size_t)ch * slotElems, slotElems * sizeof(float2),
+                                  hipMemcpyDeviceToDevice, ctx->upd));
+    }
+    TANX_CHECK(hipEventRecord(cv->ev_upd, ctx->upd));
+    cv->pending = true;
+    if (opflags & TANX_OPERATION_FLAG_BLOCK_UNTIL_READY) TANX_CHECK(hipEventSynchronize(cv->ev_upd));
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_update_td(tanx_conv *cv, const float *const *ir, size_t n,
+                                       const uint32_t *flags, uint32_t opflags)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (!ir) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    return update_common(cv, ir, nullptr, 0, n, flags, opflags);
+}
+
+extern "C" int32_t tanx_conv_update_td_device(tanx_conv *cv, const float *d_ir, size_t stride, size_t n,
+                                              const uint32_t *flags, uint32_t opflags)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (!d_ir || stride < n) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    return update_common(cv, nullptr, d_ir, stride, n, flags, opflags);
+}
+
+extern "C" int32_t tanx_conv_flush(tanx_conv *cv, uint32_t channel)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (channel >= (uint32_t)cv->C) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    int32_t rc = flush_channel(cv, (int)channel);
+    if (rc == TANX_OK) TANX_CHECK(hipEventRecord(cv->ev_proc, cv->ctx->proc));
+    return rc;
+}
+
+extern "C" int32_t tanx_conv_get_next_free_channel(tanx_conv *cv, uint32_t *channel, const uint32_t *flags)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (!channel) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    *channel = (uint32_t)-1;
+    if (!flags) return TANX_NOT_FOUND;
+    for (int ch = 0; ch < cv->C; ch++) {
+        if ((flags[ch] & TANX_CHANNEL_FLAG_STOP_INPUT) && cv->tail_left[ch] <= 0) {
+            *channel = (uint32_t)ch;
+            return TANX_OK;
+        }
+    }
+    return TANX_NOT_FOUND;
+}
+
+extern "C" int32_t tanx_conv_synchronize(tanx_conv *cv)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    DeviceGuard g(cv->ctx->device);
+    TANX_CHECK(hipStreamSynchronize(cv->ctx->proc));
+    TANX_CHECK(hipStreamSynchronize(cv->ctx->upd));
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_get_info(const tanx_conv *cv, tanx_conv_info *info)
+{
+    if (!cv || !info) return TANX_INVALID_POINTER;
+    info->ir_len = (uint32_t)cv->len;
+    info->block = (uint32_t)cv->B;
+    info->channels = (uint32_t)cv->C;
+    info->partitions = (uint32_t)cv->P;
+    info->tail_chunks = (uint32_t)cv->nchunk;
+    info->finalize_mode = cv->fmode ? 1u : 0u;
+    info->device_bytes = cv->dbytes;
+    info->ticks = cv->ticks;
+    info->crossfades = cv->xfades;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_set_tail_chunks(tanx_conv *cv, uint32_t chunks)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    set_chunks(cv, chunks == 0 ? auto_chunks(cv->C, cv->P) : (int)chunks);
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_last_was_crossfade(const tanx_conv *cv)
+{
+    return cv && cv->last_xfade ? 1 : 0;
+}
+
+extern "C" void *tanx_conv_process_stream(tanx_conv *cv)
+{
+    return cv ? (void *)cv->ctx->proc : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// TANFFT
+// ---------------------------------------------------------------------------
+static int32_t fft_check(uint32_t dir, uint32_t log2len, uint32_t channels)
+{
+    if (channels == 0 || log2len == 0) return TANX_INVALID_ARG;
+    if (dir > TANX_FFT_C2R_PLANAR_BACKWARD) return TANX_INVALID_ARG;
+    if (dir <= TANX_FFT_BACKWARD && log2len > 11) return TANX_NOT_SUPPORTED;
+    if (dir >= TANX_FFT_R2C_FORWARD && log2len > 12) return TANX_NOT_SUPPORTED;
+    // planar layout ends at N + 9 floats of a 2N-float channel buffer
+    if (dir >= TANX_FFT_R2C_PLANAR_FORWARD && log2len < 4) return TANX_INVALID_ARG;
+    return TANX_OK;
+}
+
+static FftArgs fft_args(tanx_ctx *ctx, uint32_t dir, uint32_t log2len, const float *in, float *out, size_t stride)
+{
+    FftArgs a{};
+    a.in = in;
+    a.out = out;
+    a.stride = (long long)stride;
+    a.mode = (int)dir;
+    a.log2len = (int)log2len;
+    const int lM = dir >= TANX_FFT_R2C_FORWARD ? (int)log2len - 1 : (int)log2len;
+    a.twM = tw_table(ctx->tw, lM);
+    a.twN = tw_table(ctx->tw, lM + 1);
+    return a;
+}
+
+extern "C" int32_t tanx_fft_transform_device(tanx_ctx *ctx, uint32_t direction, uint32_t log2len,
+                                             uint32_t channels, const float *d_in, float *d_out, size_t stride)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    int32_t rc = fft_check(direction, log2len, channels);
+    if (rc != TANX_OK) return rc;
+    if (!d_in || !d_out || stride < (size_t(2) << log2len)) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->fft_mtx);
+    DeviceGuard g(ctx->device);
+    TANX_CHECK(launch_fft((int)channels, fft_args(ctx, direction, log2len, d_in, d_out, stride), ctx->upd));
+    TANX_CHECK(hipStreamSynchronize(ctx->upd));
+    return TANX_OK;
+}
+
+// output ranges (floats) a transform defines in each channel buffer
+static void fft_out_ranges(uint32_t dir, uint32_t log2len, size_t r[2][2])
+{
+    const size_t n = size_t(1) << log2len, m = n / 2;
+    r[1][0] = r[1][1] = 0;
+    switch (dir) {
+    case TANX_FFT_FORWARD:
+    case TANX_FFT_BACKWARD: r[0][0] = 0; r[0][1] = 2 * n; break;
+    case TANX_FFT_R2C_FORWARD: r[0][0] = 0; r[0][1] = n + 2; break;
+    case TANX_FFT_R2C_PLANAR_FORWARD:
+        r[0][0] = 0; r[0][1] = m + 1;
+        r[1][0] = m + 8; r[1][1] = 2 * m + 9;
+        break;
+    default: r[0][0] = 0; r[0][1] = n; break;
+    }
+}
+
+extern "C" int32_t tanx_fft_transform(tanx_ctx *ctx, uint32_t direction, uint32_t log2len,
+                                      uint32_t channels, const float *const *in, float *const *out)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    if (!in || !out) return TANX_INVALID_ARG;
+    int32_t rc = fft_check(direction, log2len, channels);
+    if (rc != TANX_OK) return rc;
+    std::lock_guard<std::mutex> lk(ctx->fft_mtx);
+    DeviceGuard g(ctx->device);
+    const size_t stride = size_t(2) << log2len;
+    const size_t need = stride * channels;
+    if (ctx->fft_dev_floats < need) {
+        if (ctx->fft_dev) (void)hipFree(ctx->fft_dev);
+        ctx->fft_dev = nullptr;
+        ctx->fft_dev_floats = 0;
+        TANX_CHECK(hipMalloc(&ctx->fft_dev, need * sizeof(float)));
+        ctx->fft_dev_floats = need;
+    }
+    if (ctx->fft_host_floats < need) {
+        if (ctx->fft_host) (void)hipHostFree(ctx->fft_host);
+        ctx->fft_host = nullptr;
+        ctx->fft_host_floats = 0;
+        TANX_CHECK(hipHostMalloc(&ctx->fft_host, need * sizeof(float)));
+        ctx->fft_host_floats = need;
+    }
+    for (uint32_t c = 0; c < channels; c++) {
+        if (!in[c] || !out[c]) return TANX_INVALID_ARG;
+        std::memcpy(ctx->fft_host + c * stride, in[c], stride * sizeof(float));
+    }
+    hipStream_t s = ctx->upd;
+    TANX_CHECK(hipMemcpyAsync(ctx->fft_dev, ctx->fft_host, need * sizeof(float), hipMemcpyHostToDevice, s));
+    TANX_CHECK(launch_fft((int)channels, fft_args(ctx, direction, log2len, ctx->fft_dev, ctx->fft_dev, stride), s));
+    TANX_CHECK(hipMemcpyAsync(ctx->fft_host, ctx->fft_dev, need * sizeof(float), hipMemcpyDeviceToHost, s));
+    TANX_CHECK(hipStreamSynchronize(s));
+    size_t r[2][2];
+    fft_out_ranges(direction, log2len, r);
+    for (uint32_t c = 0; c < channels; c++)
+        for (int k = 0; k < 2; k++)
+            if (r[k][1] > r[k][0])
+                std::memcpy(out[c] + r[k][0], ctx->fft_host + c * stride + r[k][0],
+                            (r[k][1] - r[k][0]) * sizeof(float));
+    return TANX_OK;
+}
+
+extern "C" const char *tanx_version(void) { return "tan_mi355x 0.1 (gfx950)"; }

@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace tanx {
+
+// Spectra are stored "packed": a 2B-point real FFT of a frame of 2B reals
+// has B+1 bins whose DC and Nyquist bins are real; element 0 holds
+// (DC.re, Nyquist.re) and elements 1..B-1 hold bins 1..B-1.  Every spectrum
+// row is therefore exactly B complex floats (B*8 bytes).
+
+struct HeadArgs {
+    const float *xin;      // channel c block at xin + c * in_stride
+    long long in_stride;
+    float *yout;           // channel c output at yout + c * out_stride
+    long long out_stride;
+    float *xprev;          // [C][B] previous input block (overlap-save frame)
+    float2 *fdl;           // [C][P][B] ring of input spectra
+    int P;
+    int slot;              // ring slot receiving this block's spectrum
+    const float2 *Ha;      // filter A slot base [C][P][B] (partition 0 used)
+    const float2 *Hb;      // filter B slot base (crossfade) or nullptr
+    const float2 *Ta;      // tail partial sums for filter A [C][nchunk][B]
+    const float2 *Tb;      // tail partial sums for filter B or nullptr
+    int nchunk;
+    int xfade;             // 0: y = y_A; 1: y = y_B * j/F + y_A * (F-j)/F
+    const uint8_t *run;    // [C] 1 = channel running
+    const float2 *twM;     // W_B^t, t < B
+    const float2 *twN;     // W_2B^t, t < 2B
+};
+
+struct TailArgs {
+    const float2 *fdl;     // [C][P][B]
+    int P;
+    int base;              // ring slot of the block the tail is for
+    const float2 *Ha;      // [C][P][B]
+    const float2 *Hb;      // nullptr for one filter
+    float2 *Ta;            // [C][nchunk][B]
+    float2 *Tb;
+    int nchunk;
+    int pc;                // partitions per chunk
+    const uint8_t *run;
+};
+
+struct IrArgs {
+    const float *ir;       // channel ch IR at ir + ch * stride, n valid samples
+    long long stride;
+    long long n;
+    const int *chan;       // list of channels to transform (blockIdx.y indexes it)
+    float2 *H;             // destination slot base [C][P][B]
+    int P;
+    const float2 *twM;
+    const float2 *twN;
+};
+
+struct FftArgs {
+    const float *in;
+    float *out;
+    long long stride;      // floats between channels
+    int mode;              // TAN_FFT_TRANSFORM_DIRECTION
+    int log2len;
+    const float2 *twM;     // W_M table for the complex FFT size M
+    const float2 *twN;     // W_2M table (real transforms)
+};
+
+// Launchers (return hipError_t).  B is the block size (power of two).
+hipError_t launch_head(int B, int C, const HeadArgs &a, hipStream_t s);
+hipError_t launch_tail(int B, int C, int nf, const TailArgs &a, hipStream_t s);
+hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);
+hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
+
+// Twiddle tables: tw + (2^l - 1) holds W_{2^l}^t = exp(-2 pi i t / 2^l), t < 2^l,
+// for l = 0..kMaxTwLog2.
+constexpr int kMaxTwLog2 = 13;
+inline const float2 *tw_table(const float2 *tw, int log2) { return tw + ((1 << log2) - 1); }
+
+}  // namespace tanx

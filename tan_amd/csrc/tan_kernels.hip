@@ -1,0 +1,468 @@
+// gfx950 kernels of the uniformly partitioned FFT convolution path.
+//
+// Method: overlap-save with a frequency-domain delay line (FDL).
+//   X_t    = FFT_2B([x_{t-1}, x_t])                       (head)
+//   A_f(t) = sum_{i=0}^{P-1} X_{t-i} H_{f,i}              (tail: i >= 1, head: i = 0)
+//   y_f(t) = IFFT_2B(A_f(t))[B .. 2B)                     (head)
+// with H_{f,i} = FFT_2B([h_f[iB .. iB+B), 0_B]).  The reference CPU path
+// (ovlNUPProcessCPU/ovlNUPProcessTail, ConvolutionImpl.cpp:2380-2643) is the
+// overlap-add form of the same sum; overlap-save keeps no filter-dependent
+// output state, so switching IRs (Process crossfade, ConvolutionImpl.cpp:
+// 893-991) evaluates both filters exactly on one input history.
+//
+// FFTs: the 2B-point real transforms run as B-point complex Stockham
+// radix-4 (+ one radix-2) FFTs staged in LDS, with the standard real/complex
+// split post- and pre-processing.  Twiddles come from tables computed in
+// FP64 on the host and rounded to FP32.
+#include "tan_internal.h"
+
+namespace tanx {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)  // a * conj(b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// Stockham autosort FFT of M points held in LDS buffer `a`; `b` is scratch of
+// the same size.  Returns the buffer that holds the result (natural order).
+// Radix-4 stages with a trailing radix-2 stage when log2(M) is odd.  Each
+// stage: butterfly q reads a[q + m*M/R], m < R, multiplies by W_{pR}^{m*k}
+// (k = q mod p), and writes b[(q - k)*R + k + m*p].  INV selects the
+// unscaled inverse.  All threads of the block must call it.
+template <int M, bool INV>
+__device__ float2 *fft_lds(float2 *a, float2 *b, const float2 *__restrict__ tw)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int p = 1;
+#pragma unroll 1
+    for (; p * 4 <= M; p *= 4) {
+        const int tstride = M / (4 * p);
+        for (int q = tid; q < M / 4; q += nt) {
+            const int k = q & (p - 1);
+            float2 x0 = a[q], x1 = a[q + M / 4], x2 = a[q + M / 2], x3 = a[q + 3 * M / 4];
+            if (p > 1) {
+                const int t = k * tstride;
+                float2 w1 = tw[t], w2 = tw[2 * t], w3 = tw[3 * t];
+                if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+                x1 = cmul(x1, w1);
+                x2 = cmul(x2, w2);
+                x3 = cmul(x3, w3);
+            }
+            const float2 s02 = cadd(x0, x2), d02 = csub(x0, x2);
+            const float2 s13 = cadd(x1, x3), d13 = csub(x1, x3);
+            // forward: W_4 = -i ; inverse: +i
+            const float2 jd13 = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+            const int j = (q - k) * 4 + k;
+            b[j] = cadd(s02, s13);
+            b[j + p] = cadd(d02, jd13);
+            b[j + 2 * p] = csub(s02, s13);
+            b[j + 3 * p] = csub(d02, jd13);
+        }
+        __syncthreads();
+        float2 *t = a; a = b; b = t;
+    }
+    if (p < M) {  // radix-2, p = M/2
+        for (int q = tid; q < M / 2; q += nt) {
+            float2 x0 = a[q], x1 = a[q + M / 2];
+            float2 w = tw[q];
+            if (INV) w.y = -w.y;
+            x1 = cmul(x1, w);
+            b[q] = cadd(x0, x1);
+            b[q + M / 2] = csub(x0, x1);
+        }
+        __syncthreads();
+        a = b;
+    }
+    return a;
+}
+
+// Real-FFT post-processing: Z = FFT_M(z), z[n] = x[2n] + i x[2n+1].
+// Returns packed bin k of the 2M-point real FFT (k = 0: (X[0], X[M])).
+template <int M>
+__device__ __forceinline__ float2 r2c_post(const float2 *Z, int k, const float2 *__restrict__ twN)
+{
+    const float2 z0 = Z[k];
+    if (k == 0) return make_float2(z0.x + z0.y, z0.x - z0.y);
+    const float2 zc = Z[M - k];                       // conj taken below
+    const float er = 0.5f * (z0.x + zc.x), ei = 0.5f * (z0.y - zc.y);
+    const float dr = z0.x - zc.x, di = z0.y + zc.y;   // Z[k] - conj(Z[M-k])
+    const float orr = 0.5f * di, oi = -0.5f * dr;     // (Z - conj)/(2i)
+    const float2 w = twN[k];
+    return make_float2(er + (w.x * orr - w.y * oi), ei + (w.x * oi + w.y * orr));
+}
+
+// Inverse pre-processing from a packed spectrum A (element 0 = (DC, Nyquist)):
+// returns Z''[k] = 2 (E[k] + i O[k]) so that IFFT_M(Z'')/(2M) interleaves the
+// even/odd output samples.
+template <int M>
+__device__ __forceinline__ float2 c2r_pre(const float2 *A, int k, const float2 *__restrict__ twN)
+{
+    const float2 a = A[k];
+    if (k == 0) return make_float2(a.x + a.y, a.x - a.y);
+    const float2 c = A[M - k];                        // conj(c) used
+    const float er = a.x + c.x, ei = a.y - c.y;
+    const float dr = a.x - c.x, di = a.y + c.y;       // X[k] - conj(X[M-k])
+    const float2 w = twN[k];                          // multiply by conj(w)
+    const float orr = dr * w.x + di * w.y, oi = di * w.x - dr * w.y;
+    return make_float2(er - oi, ei + orr);
+}
+
+// ---------------------------------------------------------------------------
+// Head: one workgroup per channel.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_head(HeadArgs a)
+{
+    constexpr float kScale = 1.0f / (2.0f * (float)M);
+    __shared__ float2 s0[M], s1[M], sx[M], sa[M];
+    __shared__ float ya[M];
+    const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (!a.run[c]) return;
+
+    float *xp = a.xprev + (size_t)c * M;
+    const float *xi = a.xin + (size_t)c * a.in_stride;
+    // frame [x_{t-1}, x_t] as M complex points
+    for (int n = tid; n < M; n += nt) {
+        const float *src = n < M / 2 ? xp + 2 * n : xi + 2 * (n - M / 2);
+        s0[n] = make_float2(src[0], src[1]);
+    }
+    __syncthreads();
+    for (int j = tid; j < M; j += nt) xp[j] = xi[j];
+
+    const float2 *Z = fft_lds<M, false>(s0, s1, a.twM);
+    float2 *row = a.fdl + ((size_t)c * a.P + a.slot) * M;
+    for (int k = tid; k < M; k += nt) {
+        const float2 X = r2c_post<M>(Z, k, a.twN);
+        sx[k] = X;
+        row[k] = X;
+    }
+    __syncthreads();
+
+    const int nf = a.xfade ? 2 : 1;
+    for (int f = 0; f < nf; f++) {
+        const float2 *H0 = (f == 0 ? a.Ha : a.Hb) + (size_t)c * a.P * M;
+        const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.nchunk * M;
+        for (int k = tid; k < M; k += nt) {
+            float2 acc = make_float2(0.f, 0.f);
+            for (int j = 0; j < a.nchunk; j++) {
+                const float2 t = T[(size_t)j * M + k];
+                acc.x += t.x;
+                acc.y += t.y;
+            }
+            const float2 x = sx[k], h = H0[k];
+            if (k == 0) {
+                acc.x = fmaf(x.x, h.x, acc.x);
+                acc.y = fmaf(x.y, h.y, acc.y);
+            } else {
+                acc.x = fmaf(x.x, h.x, acc.x);
+                acc.x = fmaf(-x.y, h.y, acc.x);
+                acc.y = fmaf(x.x, h.y, acc.y);
+                acc.y = fmaf(x.y, h.x, acc.y);
+            }
+            sa[k] = acc;
+        }
+        __syncthreads();
+        for (int k = tid; k < M; k += nt) s0[k] = c2r_pre<M>(sa, k, a.twN);
+        __syncthreads();
+        const float2 *z = fft_lds<M, true>(s0, s1, a.twM);
+        // second half of the 2M output samples: z[M/2 .. M)
+        if (f == 0) {
+            for (int n = tid; n < M / 2; n += nt) {
+                const float2 v = z[M / 2 + n];
+                ya[2 * n] = v.x * kScale;
+                ya[2 * n + 1] = v.y * kScale;
+            }
+        } else {
+            const float step = (float)(1.0 / (double)M);
+            for (int n = tid; n < M / 2; n += nt) {
+                const float2 v = z[M / 2 + n];
+                const int j0 = 2 * n, j1 = 2 * n + 1;
+                ya[j0] = (v.x * kScale) * (step * (float)j0) + ya[j0] * (step * (float)(M - j0));
+                ya[j1] = (v.y * kScale) * (step * (float)j1) + ya[j1] * (step * (float)(M - j1));
+            }
+        }
+        __syncthreads();
+    }
+    float *yo = a.yout + (size_t)c * a.out_stride;
+    for (int j = tid; j < M; j += nt) yo[j] = ya[j];
+}
+
+// ---------------------------------------------------------------------------
+// Tail: sum over partitions [1 + j*pc, 1 + (j+1)*pc) of X_{base-i} H_i for
+// one channel and chunk j.  256 threads; each row of B complex is read as
+// float4 (two complex) per lane; G rows are in flight per iteration.
+// ---------------------------------------------------------------------------
+template <int NF>
+__device__ __forceinline__ void mac4(float4 &acc, float4 x, float4 h, float s, bool dc)
+{
+    // complex 0 (packed DC/Nyquist when dc)
+    acc.x = fmaf(x.x, h.x, acc.x);
+    acc.x = fmaf(-s * x.y, h.y, acc.x);
+    acc.y = fmaf(s * x.x, h.y, acc.y);
+    acc.y = fmaf(x.y, dc ? h.y : h.x, acc.y);
+    // complex 1
+    acc.z = fmaf(x.z, h.z, acc.z);
+    acc.z = fmaf(-x.w, h.w, acc.z);
+    acc.w = fmaf(x.z, h.w, acc.w);
+    acc.w = fmaf(x.w, h.z, acc.w);
+}
+
+template <int M, int NF>
+__global__ __launch_bounds__(256) void k_tail(TailArgs a)
+{
+    constexpr int H = M / 2;                     // float4 per row
+    constexpr int V = H > 256 ? H / 256 : 1;     // float4 per lane per row
+    constexpr int R = H / V;                     // lanes per row
+    constexpr int G = 256 / R;                   // rows in flight
+    constexpr int U = 4;                         // unroll (partitions per lane)
+    const int c = blockIdx.y, j = blockIdx.x, tid = threadIdx.x;
+    if (!a.run[c]) return;
+    const int g = tid / R, r = tid % R;
+    const int i0 = 1 + j * a.pc;
+    const int i1 = min(a.P, i0 + a.pc);
+    const int mask = a.P - 1;
+
+    const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
+    const float4 *HA = reinterpret_cast<const float4 *>(a.Ha) + (size_t)c * a.P * H;
+    const float4 *HB = NF == 2 ? reinterpret_cast<const float4 *>(a.Hb) + (size_t)c * a.P * H : nullptr;
+
+    float4 accA[V], accB[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        accA[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        accB[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const bool dc0 = (r == 0);
+    const float s0 = dc0 ? 0.f : 1.f;
+
+    int i = i0 + g;
+    for (; i + (U - 1) * G < i1; i += U * G) {
+        float4 xf[U][V], ha[U][V], hb[U][V];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u * G;
+            const int s = (a.base - ii) & mask;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                xf[u][v] = F[(size_t)s * H + v * R + r];
+                ha[u][v] = HA[(size_t)ii * H + v * R + r];
+                if (NF == 2) hb[u][v] = HB[(size_t)ii * H + v * R + r];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const bool dc = dc0 && v == 0;
+                const float s = v == 0 ? s0 : 1.f;
+                mac4<NF>(accA[v], xf[u][v], ha[u][v], s, dc);
+                if (NF == 2) mac4<NF>(accB[v], xf[u][v], hb[u][v], s, dc);
+            }
+        }
+    }
+    for (; i < i1; i += G) {
+        const int s = (a.base - i) & mask;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const bool dc = dc0 && v == 0;
+            const float sv = v == 0 ? s0 : 1.f;
+            const float4 x = F[(size_t)s * H + v * R + r];
+            mac4<NF>(accA[v], x, HA[(size_t)i * H + v * R + r], sv, dc);
+            if (NF == 2) mac4<NF>(accB[v], x, HB[(size_t)i * H + v * R + r], sv, dc);
+        }
+    }
+
+    float4 *TA = reinterpret_cast<float4 *>(a.Ta) + ((size_t)c * a.nchunk + j) * H;
+    float4 *TB = NF == 2 ? reinterpret_cast<float4 *>(a.Tb) + ((size_t)c * a.nchunk + j) * H : nullptr;
+    if constexpr (G == 1) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            TA[v * R + r] = accA[v];
+            if (NF == 2) TB[v * R + r] = accB[v];
+        }
+    } else {
+        __shared__ float4 red[NF][G][H];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            red[0][g][v * R + r] = accA[v];
+            if (NF == 2) red[NF - 1][g][v * R + r] = accB[v];
+        }
+        __syncthreads();
+        for (int e = tid; e < NF * H; e += 256) {
+            const int f = e / H, q = e % H;
+            float4 sum = red[f][0][q];
+#pragma unroll
+            for (int gg = 1; gg < G; gg++) {
+                const float4 t = red[f][gg][q];
+                sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
+            }
+            (f == 0 ? TA : TB)[q] = sum;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// IR partition transform: one workgroup per (partition i, listed channel).
+// Partition i of channel ch becomes the packed spectrum of [h[iB..iB+B), 0_B].
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs a)
+{
+    __shared__ float2 s0[M], s1[M];
+    const int i = blockIdx.x, ch = a.chan[blockIdx.y], tid = threadIdx.x, nt = blockDim.x;
+    const float *h = a.ir + (size_t)ch * a.stride;
+    const long long base = (long long)i * M;
+    for (int n = tid; n < M; n += nt) {
+        float2 v = make_float2(0.f, 0.f);
+        if (n < M / 2) {
+            const long long k = base + 2 * n;
+            v.x = k < a.n ? h[k] : 0.f;
+            v.y = k + 1 < a.n ? h[k + 1] : 0.f;
+        }
+        s0[n] = v;
+    }
+    __syncthreads();
+    const float2 *Z = fft_lds<M, false>(s0, s1, a.twM);
+    float2 *row = a.H + ((size_t)ch * a.P + i) * M;
+    for (int k = tid; k < M; k += nt) row[k] = r2c_post<M>(Z, k, a.twN);
+}
+
+// ---------------------------------------------------------------------------
+// TANFFT transforms, one workgroup per channel.  M is the complex FFT size
+// (N for complex directions, N/2 for real ones).
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(M / 4 > 64 ? (M / 4 < 1024 ? M / 4 : 1024) : 64) void k_fft(FftArgs a)
+{
+    __shared__ float2 s0[M > 1 ? M : 2], s1[M > 1 ? M : 2], s2[M > 1 ? M : 2];
+    const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const float *in = a.in + (size_t)c * a.stride;
+    float *out = a.out + (size_t)c * a.stride;
+    const int mode = a.mode;
+    if (mode == 0 || mode == 1) {
+        for (int n = tid; n < M; n += nt) s0[n] = make_float2(in[2 * n], in[2 * n + 1]);
+        __syncthreads();
+        const float2 *z = mode == 0 ? fft_lds<M, false>(s0, s1, a.twM) : fft_lds<M, true>(s0, s1, a.twM);
+        const float sc = mode == 1 ? 1.0f / (float)M : 1.0f;
+        for (int k = tid; k < M; k += nt) {
+            out[2 * k] = z[k].x * sc;
+            out[2 * k + 1] = z[k].y * sc;
+        }
+    } else if (mode == 2 || mode == 4) {
+        for (int n = tid; n < M; n += nt) s0[n] = make_float2(in[2 * n], in[2 * n + 1]);
+        __syncthreads();
+        const float2 *Z = fft_lds<M, false>(s0, s1, a.twM);
+        const int imoff = M + 8;
+        for (int k = tid; k <= M; k += nt) {
+            float2 X;
+            if (k == 0) X = make_float2(Z[0].x + Z[0].y, 0.f);
+            else if (k == M) X = make_float2(Z[0].x - Z[0].y, 0.f);
+            else X = r2c_post<M>(Z, k, a.twN);
+            if (mode == 2) { out[2 * k] = X.x; out[2 * k + 1] = X.y; }
+            else { out[k] = X.x; out[imoff + k] = X.y; }
+        }
+    } else {
+        const int imoff = M + 8;
+        // packed spectrum in s2: element 0 = (DC.re, Nyquist.re); imag parts of
+        // DC and Nyquist are ignored as a c2r transform does
+        for (int k = tid; k < M; k += nt) {
+            float2 v;
+            if (k == 0) v = mode == 3 ? make_float2(in[0], in[2 * M]) : make_float2(in[0], in[M]);
+            else v = mode == 3 ? make_float2(in[2 * k], in[2 * k + 1]) : make_float2(in[k], in[imoff + k]);
+            s2[k] = v;
+        }
+        __syncthreads();
+        for (int k = tid; k < M; k += nt) s0[k] = c2r_pre<M>(s2, k, a.twN);
+        __syncthreads();
+        const float2 *z = fft_lds<M, true>(s0, s1, a.twM);
+        const float sc = 1.0f / (2.0f * (float)M);
+        for (int n = tid; n < M; n += nt) {
+            out[2 * n] = z[n].x * sc;
+            out[2 * n + 1] = z[n].y * sc;
+        }
+    }
+}
+
+__global__ void k_zero(float4 *p, size_t n4)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline int fft_threads(int M)
+{
+    int t = M / 4;
+    if (t < 64) t = 64;
+    if (t > 1024) t = 1024;
+    return t;
+}
+
+#define TANX_SWITCH_B(B, CALL)          \
+    switch (B) {                        \
+    case 64: { constexpr int M = 64; CALL; } break;     \
+    case 128: { constexpr int M = 128; CALL; } break;   \
+    case 256: { constexpr int M = 256; CALL; } break;   \
+    case 512: { constexpr int M = 512; CALL; } break;   \
+    case 1024: { constexpr int M = 1024; CALL; } break; \
+    default: return hipErrorInvalidValue;               \
+    }
+
+hipError_t launch_head(int B, int C, const HeadArgs &a, hipStream_t s)
+{
+    TANX_SWITCH_B(B, hipLaunchKernelGGL(k_head<M>, dim3(C), dim3(fft_threads(M)), 0, s, a));
+    return hipGetLastError();
+}
+
+hipError_t launch_tail(int B, int C, int nf, const TailArgs &a, hipStream_t s)
+{
+    if (nf == 2) {
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tail<M, 2>), dim3(a.nchunk, C), dim3(256), 0, s, a));
+    } else {
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tail<M, 1>), dim3(a.nchunk, C), dim3(256), 0, s, a));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
+{
+    TANX_SWITCH_B(B, hipLaunchKernelGGL(k_ir_transform<M>, dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+    return hipGetLastError();
+}
+
+hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s)
+{
+    const bool real = a.mode >= 2;
+    const int M = real ? (1 << (a.log2len - 1)) : (1 << a.log2len);
+    switch (M) {
+#define TANX_FFT_CASE(V) case V: hipLaunchKernelGGL(k_fft<V>, dim3(channels), dim3(fft_threads(V)), 0, s, a); break;
+        TANX_FFT_CASE(1) TANX_FFT_CASE(2) TANX_FFT_CASE(4) TANX_FFT_CASE(8) TANX_FFT_CASE(16)
+        TANX_FFT_CASE(32) TANX_FFT_CASE(64) TANX_FFT_CASE(128) TANX_FFT_CASE(256) TANX_FFT_CASE(512)
+        TANX_FFT_CASE(1024) TANX_FFT_CASE(2048)
+#undef TANX_FFT_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return hipSuccess;
+    if (bytes % 16 != 0 || (reinterpret_cast<uintptr_t>(p) & 15)) return hipErrorInvalidValue;
+    const size_t n4 = bytes / 16;
+    size_t blocks = (n4 + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<float4 *>(p), n4);
+    return hipGetLastError();
+}
+
+}  // namespace tanx

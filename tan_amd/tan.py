@@ -1,0 +1,309 @@
+"""Python mirror of TAN's TANContext / TANConvolution / TANFFT interfaces.
+
+Same class and method names, argument meaning and AMF_RESULT return codes as
+tan/tanlibrary/include/TrueAudioNext.h:125-250 (TANConvolution), 493-541
+(TANFFT) and 580-597 (TANContext).  Every call goes through the C ABI of the
+HIP library (include/tan_mi355x.h); host arrays are numpy float32, device
+arrays are torch tensors on the GPU (the stand-ins for the reference's
+cl_mem overloads).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+# AMF_RESULT (amf/public/include/core/Result.h:43-122)
+AMF_OK = 0
+AMF_FAIL = 1
+AMF_UNEXPECTED = 2
+AMF_INVALID_ARG = 4
+AMF_OUT_OF_MEMORY = 6
+AMF_INVALID_POINTER = 7
+AMF_NOT_IMPLEMENTED = 9
+AMF_NOT_SUPPORTED = 10
+AMF_NOT_FOUND = 11
+AMF_ALREADY_INITIALIZED = 12
+AMF_NOT_INITIALIZED = 13
+AMF_WRONG_STATE = 15
+AMF_NO_DEVICE = 17
+AMF_TAN_UNSUPPORTED_VERSION = 43
+
+# TAN_CONVOLUTION_METHOD (TrueAudioNext.h:81-98)
+TAN_CONVOLUTION_METHOD_FFT_OVERLAP_ADD = 0
+TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_UNIFORM = 1
+TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM = 2
+TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED = 3
+TAN_CONVOLUTION_METHOD_FHT_UNIFORM_PARTITIONED = 4
+TAN_CONVOLUTION_METHOD_FFT_UNIFORM_HEAD_TAIL = 5
+TAN_CONVOLUTION_METHOD_FHT_UNIFORM_HEAD_TAIL = 6
+TAN_CONVOLUTION_METHOD_TIME_DOMAIN = 7
+TAN_CONVOLUTION_METHOD_FHT_NONUNIFORM_PARTITIONED = 8
+TAN_CONVOLUTION_METHOD_FFT_NONUNIFORM_PARTITIONED = 9
+TAN_CONVOLUTION_METHOD_USE_PROCESS_FINALIZE = 0x8000
+TAN_CONVOLUTION_METHOD_USE_PROCESS_TAILTHREAD = 0xC000
+
+TAN_CONVOLUTION_CHANNEL_FLAG_PROCESS = 0
+TAN_CONVOLUTION_CHANNEL_FLAG_STOP_INPUT = 0x01
+TAN_CONVOLUTION_CHANNEL_FLAG_FLUSH_STREAM = 0x02
+TAN_CONVOLUTION_OPERATION_FLAG_NONE = 0
+TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY = 0x01
+
+# TAN_FFT_TRANSFORM_DIRECTION (TrueAudioNext.h:493-501)
+TAN_FFT_TRANSFORM_DIRECTION_FORWARD = 0
+TAN_FFT_TRANSFORM_DIRECTION_BACKWARD = 1
+TAN_FFT_R2C_TRANSFORM_DIRECTION_FORWARD = 2
+TAN_FFT_C2R_TRANSFORM_DIRECTION_BACKWARD = 3
+TAN_FFT_R2C_PLANAR_TRANSFORM_DIRECTION_FORWARD = 4
+TAN_FFT_C2R_PLANAR_TRANSFORM_DIRECTION_BACKWARD = 5
+
+TAN_VERSION_MAJOR, TAN_VERSION_MINOR, TAN_VERSION_RELEASE, TAN_VERSION_BUILD = 1, 3, 1, 1
+TAN_FULL_VERSION = ((TAN_VERSION_MAJOR << 48) | (TAN_VERSION_MINOR << 32)
+                    | (TAN_VERSION_RELEASE << 16) | TAN_VERSION_BUILD)
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _is_device_tensor(x):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+def _host_rows(arrs, channels, length=None, writable=False):
+    """Row-pointer array for host float32 data: a 2-D array or a list of 1-D arrays.
+
+    Returns (pointer array, keepalive list)."""
+    if isinstance(arrs, np.ndarray) and arrs.ndim == 2:
+        rows = [arrs[i] for i in range(arrs.shape[0])]
+    else:
+        rows = list(arrs)
+    if len(rows) < channels:
+        raise ValueError("fewer channel arrays than channels")
+    keep = []
+    ptrs = (_f32p * channels)()
+    for i in range(channels):
+        r = rows[i]
+        if not (isinstance(r, np.ndarray) and r.dtype == np.float32 and r.flags.c_contiguous):
+            if writable:
+                raise ValueError("output rows must be C-contiguous float32 numpy arrays")
+            r = np.ascontiguousarray(r, dtype=np.float32)
+        if length is not None and r.size < length:
+            raise ValueError("channel array shorter than the requested length")
+        keep.append(r)
+        ptrs[i] = r.ctypes.data_as(_f32p)
+    return ptrs, keep
+
+
+def _flags_ptr(flags, channels):
+    if flags is None:
+        return None, None
+    arr = np.ascontiguousarray(flags, dtype=np.uint32)
+    if arr.size < channels:
+        raise ValueError("flag array shorter than channel count")
+    return arr.ctypes.data_as(_u32p), arr
+
+
+class TANContext:
+    """TANCreateContext + InitOpenCL equivalent: a HIP device with a process
+    stream and an update stream (TANContextImpl.cpp:87-132, 264-328)."""
+
+    def __init__(self, device=0, version=TAN_FULL_VERSION):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.tanx_ctx_create(version, device, ctypes.byref(h))
+        if rc != AMF_OK:
+            raise RuntimeError(f"TANCreateContext failed: AMF_RESULT {rc}")
+        self.handle = h
+        self.device = device
+
+    def Terminate(self):
+        if self.handle:
+            rc = self._lib.tanx_ctx_destroy(self.handle)
+            self.handle = None
+            return rc
+        return AMF_OK
+
+    def __del__(self):
+        try:
+            self.Terminate()
+        except Exception:
+            pass
+
+
+class TANConvolution:
+    """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
+
+    def __init__(self, context):
+        self._ctx = context
+        self._lib = context._lib
+        self.handle = None
+        self.channels = 0
+        self.block = 0
+
+    # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
+    def Init(self, method, response_length, buffer_size, channels):
+        return self.InitGpu(method, response_length, buffer_size, channels)
+
+    def InitGpu(self, method, response_length, buffer_size, channels):
+        if self.handle:
+            return AMF_ALREADY_INITIALIZED
+        h = ctypes.c_void_p()
+        rc = self._lib.tanx_conv_create(self._ctx.handle, method, response_length, buffer_size,
+                                        channels, ctypes.byref(h))
+        if rc == AMF_OK:
+            self.handle = h
+            self.channels = channels
+            self.block = buffer_size
+        return rc
+
+    def InitCpu(self, method, response_length, buffer_size, channels):
+        # the CPU engine of the reference is not part of this library
+        return AMF_NOT_SUPPORTED
+
+    def Terminate(self):
+        if self.handle:
+            rc = self._lib.tanx_conv_destroy(self.handle)
+            self.handle = None
+            return rc
+        return AMF_OK
+
+    def GetContext(self):
+        return self._ctx
+
+    def __del__(self):
+        try:
+            self.Terminate()
+        except Exception:
+            pass
+
+    # -- responses --
+    def UpdateResponseTD(self, buffers, num_samples=None, flag_masks=None, operation_flags=0):
+        if not self.handle:
+            return AMF_NOT_INITIALIZED
+        fp, _fk = _flags_ptr(flag_masks, self.channels)
+        if _is_device_tensor(buffers):
+            t = buffers
+            if num_samples is None:
+                num_samples = t.shape[1]
+            return self._lib.tanx_conv_update_td_device(self.handle, t.data_ptr(), t.stride(0),
+                                                        num_samples, fp, operation_flags)
+        if num_samples is None:
+            num_samples = (buffers.shape[1] if isinstance(buffers, np.ndarray)
+                           else min(len(b) for b in buffers))
+        ptrs, _keep = _host_rows(buffers, self.channels, num_samples)
+        return self._lib.tanx_conv_update_td(self.handle, ptrs, num_samples, fp, operation_flags)
+
+    def UpdateResponseFD(self, *args, **kwargs):
+        # not implemented in the reference either (ConvolutionImpl.cpp:697-717)
+        return AMF_NOT_SUPPORTED if self.handle else AMF_NOT_INITIALIZED
+
+    # -- processing --
+    def Process(self, inputs, outputs, num_samples=None, flag_masks=None):
+        """Returns (AMF_RESULT, samples processed)."""
+        if not self.handle:
+            return AMF_NOT_INITIALIZED, 0
+        fp, _fk = _flags_ptr(flag_masks, self.channels)
+        done = ctypes.c_size_t(0)
+        if _is_device_tensor(inputs):
+            if num_samples is None:
+                num_samples = inputs.shape[1]
+            rc = self._lib.tanx_conv_process_device(
+                self.handle, inputs.data_ptr(), inputs.stride(0), outputs.data_ptr(),
+                outputs.stride(0), num_samples, fp, ctypes.byref(done))
+            return rc, done.value
+        if num_samples is None:
+            num_samples = (inputs.shape[1] if isinstance(inputs, np.ndarray)
+                           else min(len(b) for b in inputs))
+        pin, _ki = _host_rows(inputs, self.channels, min(num_samples, self.block))
+        pout, _ko = _host_rows(outputs, self.channels, min(num_samples, self.block), writable=True)
+        rc = self._lib.tanx_conv_process(self.handle, pin, pout, num_samples, fp, ctypes.byref(done))
+        return rc, done.value
+
+    def ProcessFinalize(self):
+        if not self.handle:
+            return AMF_NOT_INITIALIZED
+        return self._lib.tanx_conv_finalize(self.handle)
+
+    def GetNextFreeChannel(self, flag_masks):
+        """Returns (AMF_RESULT, channel index)."""
+        if not self.handle:
+            return AMF_NOT_INITIALIZED, None
+        fp, _fk = _flags_ptr(flag_masks, self.channels)
+        idx = ctypes.c_uint32(0)
+        rc = self._lib.tanx_conv_get_next_free_channel(self.handle, ctypes.byref(idx), fp)
+        return rc, (idx.value if rc == AMF_OK else None)
+
+    # -- extras of this implementation --
+    def Flush(self, channel):
+        return self._lib.tanx_conv_flush(self.handle, channel)
+
+    def Synchronize(self):
+        return self._lib.tanx_conv_synchronize(self.handle)
+
+    def Info(self):
+        info = _lib.ConvInfo()
+        self._lib.tanx_conv_get_info(self.handle, ctypes.byref(info))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def SetTailChunks(self, chunks):
+        return self._lib.tanx_conv_set_tail_chunks(self.handle, chunks)
+
+    def LastWasCrossfade(self):
+        return bool(self._lib.tanx_conv_last_was_crossfade(self.handle))
+
+    def ProcessStream(self):
+        return self._lib.tanx_conv_process_stream(self.handle)
+
+
+class TANFFT:
+    """TANFFT on the GPU (FFTImpl.cpp:507-575)."""
+
+    def __init__(self, context):
+        self._ctx = context
+        self._lib = context._lib
+        self._init = False
+
+    def Init(self):
+        self._init = True
+        return AMF_OK
+
+    def Terminate(self):
+        self._init = False
+        return AMF_OK
+
+    def GetContext(self):
+        return self._ctx
+
+    def Transform(self, direction, log2len, channels, inputs, outputs):
+        if not self._init:
+            return AMF_NOT_INITIALIZED
+        if _is_device_tensor(inputs):
+            return self._lib.tanx_fft_transform_device(self._ctx.handle, direction, log2len, channels,
+                                                       inputs.data_ptr(), outputs.data_ptr(),
+                                                       inputs.stride(0))
+        n = 2 << log2len
+        pin, _ki = _host_rows(inputs, channels, n)
+        pout, _ko = _host_rows(outputs, channels, n, writable=True)
+        return self._lib.tanx_fft_transform(self._ctx.handle, direction, log2len, channels, pin, pout)
+
+
+def TANCreateContext(version=TAN_FULL_VERSION, device=0):
+    """Returns (AMF_RESULT, TANContext or None), as TANCreateContext does."""
+    try:
+        return AMF_OK, TANContext(device=device, version=version)
+    except RuntimeError as e:
+        msg = str(e)
+        code = int(msg.rsplit(" ", 1)[-1]) if msg[-1].isdigit() else AMF_FAIL
+        return code, None
+
+
+def TANCreateConvolution(context):
+    return AMF_OK, TANConvolution(context)
+
+
+def TANCreateFFT(context):
+    return AMF_OK, TANFFT(context)
