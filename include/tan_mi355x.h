@@ -95,7 +95,7 @@ int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
  * (convolution/ConvolutionImpl.cpp:66-80, 179-305, 1113-1259).  The uniform
  * methods (1, 3, 4, 5, 6) and overlap-add (0) map to the one uniformly
  * partitioned engine; bit 0x8000 selects ProcessFinalize mode.  B must be a
- * power of two in [64, 2048] (the reference GPU path requires a power of
+ * power of two in [64, 1024] (the reference GPU path requires a power of
  * two, ConvolutionImpl.cpp:1137-1146).  The IR length is rounded up to a
  * power of two as the reference does (ConvolutionImpl.cpp:1230-1235). */
 int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
@@ -116,15 +116,17 @@ int32_t tanx_conv_update_td_device(tanx_conv *conv, const float *d_ir, size_t st
                                    size_t n, const uint32_t *flags, uint32_t opflags);
 
 /* Process(float* in[], float* out[], ...) (ConvolutionImpl.cpp:809-833,
- * 848-1043).  Processes one block of `block` samples per channel when
- * n >= block (processed = block); n < block processes nothing.  Outputs are
- * complete on return. */
+ * 848-1043).  Processes every whole block of n samples per channel, each as
+ * one Process step of the reference, and sets processed to n rounded down to
+ * a multiple of the block size (TrueAudioNext.h:208-210); n < block
+ * processes nothing.  Outputs are complete on return. */
 int32_t tanx_conv_process(tanx_conv *conv, const float *const *in, float *const *out,
                           size_t n, const uint32_t *flags, size_t *processed);
 /* Device-memory variant (stands in for Process(cl_mem[], cl_mem[])):
- * channel c reads d_in + c * in_stride and writes d_out + c * out_stride.
- * Asynchronous: ordered on the library's process stream; use
- * tanx_conv_synchronize before reading d_out on the host. */
+ * channel c reads block b at d_in + c * in_stride + b * block and writes
+ * d_out + c * out_stride + b * block.  Asynchronous: ordered on the
+ * library's process stream; use tanx_conv_synchronize before reading d_out
+ * on the host. */
 int32_t tanx_conv_process_device(tanx_conv *conv, const float *d_in, size_t in_stride,
                                  float *d_out, size_t out_stride, size_t n,
                                  const uint32_t *flags, size_t *processed);
@@ -154,6 +156,21 @@ typedef struct tanx_conv_info {
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */
 int32_t tanx_conv_set_tail_chunks(tanx_conv *conv, uint32_t chunks);
+/* Kernel timing: when enabled, HIP events are recorded on the process
+ * stream around every head and tail launch.  read_timing synchronizes and
+ * returns summed durations and the algorithmic bytes of the timed launches
+ * (tail: FDL + IR rows of partitions 1..P-1 per channel; head: x_{t-1}, x_t,
+ * y, the new FDL row and partition 0 of each filter). */
+typedef struct tanx_kernel_timing {
+    double head_ms;
+    double tail_ms;
+    uint64_t head_launches;
+    uint64_t tail_launches;
+    uint64_t head_alg_bytes;
+    uint64_t tail_alg_bytes;
+} tanx_kernel_timing;
+int32_t tanx_conv_enable_timing(tanx_conv *conv, uint32_t enable);
+int32_t tanx_conv_read_timing(tanx_conv *conv, tanx_kernel_timing *out, uint32_t reset);
 /* 1 if the last processed block was a crossfade block. */
 int32_t tanx_conv_last_was_crossfade(const tanx_conv *conv);
 /* Stream on which process work is ordered (a hipStream_t). */

@@ -594,10 +594,14 @@ int orc_conv_update_td(orc_conv *c, const float *const *ir, size_t n,
     return AMF_OK;
 }
 
-/* channel flag bookkeeping (ConvolutionImpl.cpp:2868-2907) */
+/* channel flag bookkeeping (ConvolutionImpl.cpp:2868-2907).  A channel that
+ * returns from the stopped (available) state starts from silence: the
+ * reference flushes a draining channel before its last drain block and would
+ * otherwise resume with that block's state. */
 static void channel_flags(orc_conv *c, const uint32_t *flags, size_t n)
 {
     for (int ch = 0; ch < c->C; ch++) {
+        const unsigned char was_available = c->available[ch];
         if (!flags) {
             c->available[ch] = 0;
             c->tailLeft[ch] = c->len;
@@ -623,6 +627,7 @@ static void channel_flags(orc_conv *c, const uint32_t *flags, size_t n)
         } else {
             c->flushed[ch] = 0;
         }
+        if (was_available && !c->available[ch]) flush_channel(c, ch);
     }
 }
 

@@ -25,6 +25,12 @@ class ConvInfo(ctypes.Structure):
                 ("crossfades", ctypes.c_uint64)]
 
 
+class KernelTiming(ctypes.Structure):
+    _fields_ = [("head_ms", ctypes.c_double), ("tail_ms", ctypes.c_double),
+                ("head_launches", ctypes.c_uint64), ("tail_launches", ctypes.c_uint64),
+                ("head_alg_bytes", ctypes.c_uint64), ("tail_alg_bytes", ctypes.c_uint64)]
+
+
 _SIGS = {
     "tanx_ctx_create": (_i32, [ctypes.c_uint64, _i32, ctypes.POINTER(_vp)]),
     "tanx_ctx_destroy": (_i32, [_vp]),
@@ -43,6 +49,8 @@ _SIGS = {
     "tanx_conv_get_info": (_i32, [_vp, ctypes.POINTER(ConvInfo)]),
     "tanx_conv_set_tail_chunks": (_i32, [_vp, _u32]),
     "tanx_conv_last_was_crossfade": (_i32, [_vp]),
+    "tanx_conv_enable_timing": (_i32, [_vp, _u32]),
+    "tanx_conv_read_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_conv_process_stream": (_vp, [_vp]),
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),
     "tanx_fft_transform_device": (_i32, [_vp, _u32, _u32, _u32, _vp, _vp, _sz]),
@@ -52,11 +60,26 @@ _SIGS = {
 _lib = None
 
 
+def _bind_process_hip_runtime():
+    """PyTorch-ROCm wheels carry their own libamdhip64/libhsa-runtime64 and
+    request them by unversioned name; this library requests the versioned
+    sonames (libamdhip64.so.7, libhsa-runtime64.so.1), which those copies
+    also carry.  Importing torch first makes the dynamic loader bind this
+    library to the runtime torch already loaded, so one HIP runtime serves
+    the process; loading this library first would make torch load a second
+    runtime that then finds no devices."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Load and type the library; raises if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    _bind_process_hip_runtime()
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"tan_amd: HIP library not built ({LIB_PATH}); run __graft_entry__.build()")

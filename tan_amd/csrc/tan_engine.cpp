@@ -142,6 +142,46 @@ extern "C" int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device)
 // ---------------------------------------------------------------------------
 static constexpr int kMaxChunks = 64;
 static constexpr int kRunRing = 8;
+static constexpr int kTimerRing = 1024;
+
+// Per-kernel durations measured with HIP events recorded on the process
+// stream around each launch; drained into running sums when the ring fills
+// or when read.
+struct KernelTimer {
+    std::vector<hipEvent_t> start, stop;
+    int n = 0;
+    double ms = 0.0;
+    uint64_t count = 0;
+    bool init()
+    {
+        start.resize(kTimerRing);
+        stop.resize(kTimerRing);
+        for (int i = 0; i < kTimerRing; i++)
+            if (hipEventCreate(&start[i]) != hipSuccess || hipEventCreate(&stop[i]) != hipSuccess) return false;
+        return true;
+    }
+    void destroy()
+    {
+        for (auto e : start) if (e) (void)hipEventDestroy(e);
+        for (auto e : stop) if (e) (void)hipEventDestroy(e);
+        start.clear();
+        stop.clear();
+    }
+    hipError_t drain()
+    {
+        for (int i = 0; i < n; i++) {
+            hipError_t e = hipEventSynchronize(stop[i]);
+            if (e != hipSuccess) return e;
+            float t = 0.f;
+            e = hipEventElapsedTime(&t, start[i], stop[i]);
+            if (e != hipSuccess) return e;
+            ms += t;
+            count++;
+        }
+        n = 0;
+        return hipSuccess;
+    }
+};
 
 struct tanx_conv {
     tanx_ctx *ctx = nullptr;
@@ -162,7 +202,7 @@ struct tanx_conv {
     float *xprev = nullptr;  // [C][B]
     float *din = nullptr;    // [C][B]
     float *dout = nullptr;   // [C][B]
-    float *dir = nullptr;    // [C][len]
+    float *dir = nullptr;    // [C][len] host-IR staging, allocated on first host update
     int *dchan = nullptr;    // [C]
     uint8_t *drun = nullptr; // [C]
 
@@ -189,6 +229,10 @@ struct tanx_conv {
 
     uint64_t ticks = 0, xfades = 0;
 
+    bool timing = false;
+    KernelTimer t_head, t_tail;
+    uint64_t tail_alg_bytes = 0, head_alg_bytes = 0;
+
     size_t slot_elems() const { return (size_t)C * P * B; }
     float2 *slot_ptr(int s) const { return H + (size_t)s * slot_elems(); }
     float2 *tail_ptr(int f) const { return T + (size_t)f * C * maxchunk * B; }
@@ -202,6 +246,7 @@ static void conv_free(tanx_conv *cv)
         (void)hipStreamSynchronize(cv->ctx->proc);
         (void)hipStreamSynchronize(cv->ctx->upd);
         if (cv->dmem) (void)hipFree(cv->dmem);
+        if (cv->dir) (void)hipFree(cv->dir);
         if (cv->hin) (void)hipHostFree(cv->hin);
         if (cv->hout) (void)hipHostFree(cv->hout);
         if (cv->hir) (void)hipHostFree(cv->hir);
@@ -211,6 +256,8 @@ static void conv_free(tanx_conv *cv)
         if (cv->ev_upd) (void)hipEventDestroy(cv->ev_upd);
         if (cv->ev_proc) (void)hipEventDestroy(cv->ev_proc);
         if (cv->ev_out) (void)hipEventDestroy(cv->ev_out);
+        cv->t_head.destroy();
+        cv->t_tail.destroy();
     }
     delete cv;
 }
@@ -269,7 +316,8 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
     cv->C = (int)channels;
     cv->len = (int)len;
     cv->P = (int)(len / block);
-    cv->maxchunk = std::max(1, std::min(kMaxChunks, cv->P - 1));
+    // partial-sum rows are only worth it at small channel counts
+    cv->maxchunk = std::max(1, std::min(std::min(kMaxChunks, cv->P - 1), 16384 / cv->C));
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
 
@@ -279,10 +327,9 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
     const size_t bF = al(C * P * B * sizeof(float2));
     const size_t bT = al(2 * C * (size_t)cv->maxchunk * B * sizeof(float2));
     const size_t bX = al(C * B * sizeof(float));
-    const size_t bIR = al(C * (size_t)len * sizeof(float));
     const size_t bCh = al(C * sizeof(int));
     const size_t bRun = al(C);
-    cv->dbytes = bH + bF + bT + 3 * bX + bIR + bCh + bRun;
+    cv->dbytes = bH + bF + bT + 3 * bX + bCh + bRun;
     int32_t rc = TANX_OK;
     auto fail = [&](int32_t r) { conv_free(cv); return r; };
     hipError_t e = hipMalloc(&cv->dmem, cv->dbytes);
@@ -294,13 +341,11 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
     cv->xprev = reinterpret_cast<float *>(p); p += bX;
     cv->din = reinterpret_cast<float *>(p); p += bX;
     cv->dout = reinterpret_cast<float *>(p); p += bX;
-    cv->dir = reinterpret_cast<float *>(p); p += bIR;
     cv->dchan = reinterpret_cast<int *>(p); p += bCh;
     cv->drun = reinterpret_cast<uint8_t *>(p); p += bRun;
 
     if (hipHostMalloc(&cv->hin, C * B * sizeof(float)) != hipSuccess ||
         hipHostMalloc(&cv->hout, C * B * sizeof(float)) != hipSuccess ||
-        hipHostMalloc(&cv->hir, C * (size_t)len * sizeof(float)) != hipSuccess ||
         hipHostMalloc(&cv->hchan, C * sizeof(int)) != hipSuccess ||
         hipHostMalloc(&cv->hrun, kRunRing * C) != hipSuccess)
         return fail(TANX_OUT_OF_MEMORY);
@@ -356,12 +401,16 @@ static int32_t flush_channel(tanx_conv *cv, int ch)
 }
 
 // mirrors the channel flag bookkeeping of ProcessInternal
-// (ConvolutionImpl.cpp:2868-2907) and returns the running count
+// (ConvolutionImpl.cpp:2868-2907) and returns the running count.  A channel
+// that returns from the stopped (available) state starts from silence: the
+// reference flushes a draining channel before its last drain block and would
+// otherwise resume with that block's state.
 static int32_t channel_flags(tanx_conv *cv, const uint32_t *flags, size_t n, int *nrun)
 {
     int running = 0;
     for (int ch = 0; ch < cv->C; ch++) {
         bool do_flush = false;
+        const bool was_available = cv->available[ch] != 0;
         if (!flags) {
             cv->available[ch] = 0;
             cv->tail_left[ch] = cv->len;
@@ -387,6 +436,7 @@ static int32_t channel_flags(tanx_conv *cv, const uint32_t *flags, size_t n, int
         } else {
             cv->flushed[ch] = 0;
         }
+        if (was_available && !cv->available[ch]) do_flush = true;
         if (do_flush) {
             int32_t rc = flush_channel(cv, ch);
             if (rc != TANX_OK) return rc;
@@ -432,7 +482,18 @@ static int32_t enqueue_tail(tanx_conv *cv)
     a.nchunk = cv->nchunk;
     a.pc = cv->pc;
     a.run = cv->drun;
+    KernelTimer &kt = cv->t_tail;
+    if (cv->timing) {
+        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
+        TANX_CHECK(hipEventRecord(kt.start[kt.n], ctx->proc));
+    }
     TANX_CHECK(launch_tail(cv->B, cv->C, nf, a, ctx->proc));
+    if (cv->timing) {
+        TANX_CHECK(hipEventRecord(kt.stop[kt.n], ctx->proc));
+        kt.n++;
+        // algorithmic bytes: FDL rows and nf IR rows of partitions 1..P-1
+        cv->tail_alg_bytes += (uint64_t)cv->C * (uint64_t)(cv->P - 1) * cv->B * sizeof(float2) * (1 + nf);
+    }
     cv->tail_nchunk = cv->nchunk;
     TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
     if (!cv->crossfading && cv->pending) {
@@ -487,7 +548,19 @@ static int32_t process_head(tanx_conv *cv, const float *din, long long in_stride
     const int lB = __builtin_ctz((unsigned)cv->B);
     a.twM = tw_table(ctx->tw, lB);
     a.twN = tw_table(ctx->tw, lB + 1);
+    KernelTimer &kt = cv->t_head;
+    if (cv->timing) {
+        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
+        TANX_CHECK(hipEventRecord(kt.start[kt.n], ctx->proc));
+    }
     TANX_CHECK(launch_head(cv->B, cv->C, a, ctx->proc));
+    if (cv->timing) {
+        TANX_CHECK(hipEventRecord(kt.stop[kt.n], ctx->proc));
+        kt.n++;
+        // x_{t-1}, x_t, y, the new FDL row and partition 0 of each filter
+        const uint64_t nf = xfade ? 2 : 1;
+        cv->head_alg_bytes += (uint64_t)cv->C * cv->B * (3 * sizeof(float) + sizeof(float2) * (1 + nf));
+    }
     TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
     cv->pos = (cv->pos + 1) % cv->P;
     cv->crossfading = xfade;
@@ -500,20 +573,12 @@ static int32_t process_head(tanx_conv *cv, const float *din, long long in_stride
     return TANX_OK;
 }
 
-extern "C" int32_t tanx_conv_process(tanx_conv *cv, const float *const *in, float *const *out,
-                                     size_t n, const uint32_t *flags, size_t *processed)
+// One block of the host-memory path: inputs at in[ch] + off, outputs at out[ch] + off.
+static int32_t process_host_block(tanx_conv *cv, const float *const *in, float *const *out, size_t off,
+                                  size_t n, const uint32_t *flags)
 {
-    if (!cv) return TANX_NOT_INITIALIZED;
-    if (!in || !out) return TANX_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(cv->mtx);
-    if (processed) *processed = 0;
-    if (n < (size_t)cv->B) return TANX_OK;
-    DeviceGuard g(cv->ctx->device);
     const size_t B = cv->B;
-    for (int ch = 0; ch < cv->C; ch++) {
-        if (!in[ch]) return TANX_INVALID_ARG;
-        std::memcpy(cv->hin + ch * B, in[ch], B * sizeof(float));
-    }
+    for (int ch = 0; ch < cv->C; ch++) std::memcpy(cv->hin + ch * B, in[ch] + off, B * sizeof(float));
     hipStream_t s = cv->ctx->proc;
     TANX_CHECK(hipMemcpyAsync(cv->din, cv->hin, cv->C * B * sizeof(float), hipMemcpyHostToDevice, s));
     int32_t rc = process_head(cv, cv->din, (long long)B, cv->dout, (long long)B, n, flags);
@@ -526,8 +591,30 @@ extern "C" int32_t tanx_conv_process(tanx_conv *cv, const float *const *in, floa
     }
     TANX_CHECK(hipEventSynchronize(cv->ev_out));
     for (int ch = 0; ch < cv->C; ch++)
-        if (cv->run_now[ch] && out[ch]) std::memcpy(out[ch], cv->hout + ch * B, B * sizeof(float));
-    if (processed) *processed = B;
+        if (cv->run_now[ch]) std::memcpy(out[ch] + off, cv->hout + ch * B, B * sizeof(float));
+    return TANX_OK;
+}
+
+// Processes every whole block of n samples, one Process step per block
+// (TrueAudioNext.h:208-210: the processed count is n rounded down to a
+// multiple of the block size).
+extern "C" int32_t tanx_conv_process(tanx_conv *cv, const float *const *in, float *const *out,
+                                     size_t n, const uint32_t *flags, size_t *processed)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (!in || !out) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    if (processed) *processed = 0;
+    const size_t B = cv->B, nb = n / B;
+    if (nb == 0) return TANX_OK;
+    for (int ch = 0; ch < cv->C; ch++)
+        if (!in[ch] || !out[ch]) return TANX_INVALID_ARG;
+    DeviceGuard g(cv->ctx->device);
+    for (size_t b = 0; b < nb; b++) {
+        int32_t rc = process_host_block(cv, in, out, b * B, B, flags);
+        if (rc != TANX_OK) return rc;
+        if (processed) *processed = (b + 1) * B;
+    }
     return TANX_OK;
 }
 
@@ -539,16 +626,20 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (!d_in || !d_out) return TANX_INVALID_ARG;
     std::lock_guard<std::mutex> lk(cv->mtx);
     if (processed) *processed = 0;
-    if (n < (size_t)cv->B) return TANX_OK;
-    if (in_stride < (size_t)cv->B || out_stride < (size_t)cv->B) return TANX_INVALID_ARG;
+    const size_t B = cv->B, nb = n / B;
+    if (nb == 0) return TANX_OK;
+    if (in_stride < nb * B || out_stride < nb * B) return TANX_INVALID_ARG;
     DeviceGuard g(cv->ctx->device);
-    int32_t rc = process_head(cv, d_in, (long long)in_stride, d_out, (long long)out_stride, n, flags);
-    if (rc != TANX_OK) return rc;
-    if (!cv->fmode) {
-        rc = enqueue_tail(cv);
+    for (size_t b = 0; b < nb; b++) {
+        int32_t rc = process_head(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
+                                  (long long)out_stride, B, flags);
         if (rc != TANX_OK) return rc;
+        if (!cv->fmode) {
+            rc = enqueue_tail(cv);
+            if (rc != TANX_OK) return rc;
+        }
+        if (processed) *processed = (b + 1) * B;
     }
-    if (processed) *processed = (size_t)cv->B;
     return TANX_OK;
 }
 
@@ -596,6 +687,19 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         const float *src = dir;
         long long stride = (long long)dstride;
         if (hir) {
+            if (!cv->dir || !cv->hir) {
+                const size_t bytes = (size_t)cv->C * cv->len * sizeof(float);
+                float *d = nullptr, *h = nullptr;
+                hipError_t e1 = hipMalloc(&d, bytes);
+                hipError_t e2 = e1 == hipSuccess ? hipHostMalloc(&h, bytes) : e1;
+                if (e2 != hipSuccess) {
+                    if (d) (void)hipFree(d);
+                    return map_hip(e2);
+                }
+                cv->dir = d;
+                cv->hir = h;
+                cv->dbytes += bytes;
+            }
             for (int k = 0; k < nupd; k++) {
                 const int ch = cv->hchan[k];
                 if (!hir[ch]) return TANX_INVALID_ARG;
@@ -709,6 +813,43 @@ extern "C" int32_t tanx_conv_set_tail_chunks(tanx_conv *cv, uint32_t chunks)
     if (!cv) return TANX_NOT_INITIALIZED;
     std::lock_guard<std::mutex> lk(cv->mtx);
     set_chunks(cv, chunks == 0 ? auto_chunks(cv->C, cv->P) : (int)chunks);
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_enable_timing(tanx_conv *cv, uint32_t enable)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    if (enable && cv->t_head.start.empty()) {
+        if (!cv->t_head.init() || !cv->t_tail.init()) return TANX_UNEXPECTED;
+    }
+    if (!enable && cv->timing) {
+        TANX_CHECK(cv->t_head.drain());
+        TANX_CHECK(cv->t_tail.drain());
+    }
+    cv->timing = enable != 0;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_read_timing(tanx_conv *cv, tanx_kernel_timing *out, uint32_t reset)
+{
+    if (!cv || !out) return TANX_INVALID_POINTER;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    TANX_CHECK(cv->t_head.drain());
+    TANX_CHECK(cv->t_tail.drain());
+    out->head_ms = cv->t_head.ms;
+    out->head_launches = cv->t_head.count;
+    out->head_alg_bytes = cv->head_alg_bytes;
+    out->tail_ms = cv->t_tail.ms;
+    out->tail_launches = cv->t_tail.count;
+    out->tail_alg_bytes = cv->tail_alg_bytes;
+    if (reset) {
+        cv->t_head.ms = cv->t_tail.ms = 0.0;
+        cv->t_head.count = cv->t_tail.count = 0;
+        cv->head_alg_bytes = cv->tail_alg_bytes = 0;
+    }
     return TANX_OK;
 }
 

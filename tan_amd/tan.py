@@ -252,6 +252,16 @@ class TANConvolution:
     def SetTailChunks(self, chunks):
         return self._lib.tanx_conv_set_tail_chunks(self.handle, chunks)
 
+    def EnableTiming(self, enable=True):
+        return self._lib.tanx_conv_enable_timing(self.handle, int(bool(enable)))
+
+    def ReadTiming(self, reset=False):
+        t = _lib.KernelTiming()
+        rc = self._lib.tanx_conv_read_timing(self.handle, ctypes.byref(t), int(bool(reset)))
+        if rc != AMF_OK:
+            raise RuntimeError(f"read_timing failed: {rc}")
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def LastWasCrossfade(self):
         return bool(self._lib.tanx_conv_last_was_crossfade(self.handle))
 
