@@ -156,11 +156,17 @@ typedef struct tanx_conv_info {
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */
 int32_t tanx_conv_set_tail_chunks(tanx_conv *conv, uint32_t chunks);
+/* Device-memory Process runs each block's head and the next block's tail
+ * multiply-accumulate in one launch (default 1); 0 launches them separately
+ * as the host-memory path does.  Results are identical either way. */
+int32_t tanx_conv_set_fused(tanx_conv *conv, uint32_t fused);
 /* Kernel timing: when enabled, HIP events are recorded on the process
- * stream around every head and tail launch.  read_timing synchronizes and
- * returns summed durations and the algorithmic bytes of the timed launches
- * (tail: FDL + IR rows of partitions 1..P-1 per channel; head: x_{t-1}, x_t,
- * y, the new FDL row and partition 0 of each filter). */
+ * stream around every launch.  "head" counts launches that carry the head
+ * role (with the next block's tail when fused); "tail" counts separate tail
+ * launches.  read_timing synchronizes and returns summed durations and the
+ * algorithmic bytes of the timed launches (tail role: FDL + IR rows of
+ * partitions 2..P-1 per channel and filter; head role: x_{t-1}, x_t, y, the
+ * new and previous FDL rows, partitions 0 and 1 of each filter). */
 typedef struct tanx_kernel_timing {
     double head_ms;
     double tail_ms;

@@ -50,6 +50,7 @@ _SIGS = {
     "tanx_conv_set_tail_chunks": (_i32, [_vp, _u32]),
     "tanx_conv_last_was_crossfade": (_i32, [_vp]),
     "tanx_conv_enable_timing": (_i32, [_vp, _u32]),
+    "tanx_conv_set_fused": (_i32, [_vp, _u32]),
     "tanx_conv_read_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_conv_process_stream": (_vp, [_vp]),
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),

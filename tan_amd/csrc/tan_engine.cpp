@@ -191,6 +191,9 @@ struct tanx_conv {
     int L = 0, B = 0, C = 0, len = 0, P = 0;
     int nchunk = 1, pc = 1, maxchunk = 1;
     int tail_nchunk = 1;  // chunking of the partial sums the next head reads
+    int tread = 0;        // partial-sum buffer the next head reads
+    bool tail_due = false;  // a head ran since the last tail
+    bool fused = true;    // device path: head and next tail in one launch
     bool fixed_chunks = false;
 
     // device memory (one allocation carved into pieces)
@@ -198,7 +201,7 @@ struct tanx_conv {
     size_t dbytes = 0;
     float2 *H = nullptr;     // [3][C][P][B]
     float2 *F = nullptr;     // [C][P][B]
-    float2 *T = nullptr;     // [2][C][maxchunk][B]
+    float2 *T = nullptr;     // [2 parity][2 filter][C][maxchunk][B] tail partial sums
     float *xprev = nullptr;  // [C][B]
     float *din = nullptr;    // [C][B]
     float *dout = nullptr;   // [C][B]
@@ -235,7 +238,8 @@ struct tanx_conv {
 
     size_t slot_elems() const { return (size_t)C * P * B; }
     float2 *slot_ptr(int s) const { return H + (size_t)s * slot_elems(); }
-    float2 *tail_ptr(int f) const { return T + (size_t)f * C * maxchunk * B; }
+    // partial sums of buffer `par` for filter f (0 = current, 1 = next)
+    float2 *tail_ptr(int par, int f) const { return T + ((size_t)par * 2 + f) * C * maxchunk * B; }
 };
 
 static void conv_free(tanx_conv *cv)
@@ -269,12 +273,12 @@ static int auto_chunks(int C, int P)
     // bytes
     int want = (1024 + C - 1) / C;
     want = std::max(1, std::min(want, 16));
-    return std::min(want, std::max(1, P - 1));
+    return std::min(want, std::max(1, P - 2));
 }
 
 static void set_chunks(tanx_conv *cv, int n)
 {
-    const int parts = std::max(1, cv->P - 1);
+    const int parts = std::max(1, cv->P - 2);  // the tail covers partitions 2..P-1
     n = std::max(1, std::min(n, std::min(cv->maxchunk, parts)));
     cv->pc = (parts + n - 1) / n;
     cv->nchunk = (parts + cv->pc - 1) / cv->pc;
@@ -317,7 +321,7 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
     cv->len = (int)len;
     cv->P = (int)(len / block);
     // partial-sum rows are only worth it at small channel counts
-    cv->maxchunk = std::max(1, std::min(std::min(kMaxChunks, cv->P - 1), 16384 / cv->C));
+    cv->maxchunk = std::max(1, std::min(std::min(kMaxChunks, cv->P - 2), 16384 / cv->C));
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
 
@@ -325,7 +329,7 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t bH = al(3 * C * P * B * sizeof(float2));
     const size_t bF = al(C * P * B * sizeof(float2));
-    const size_t bT = al(2 * C * (size_t)cv->maxchunk * B * sizeof(float2));
+    const size_t bT = al(4 * C * (size_t)cv->maxchunk * B * sizeof(float2));
     const size_t bX = al(C * B * sizeof(float));
     const size_t bCh = al(C * sizeof(int));
     const size_t bRun = al(C);
@@ -395,7 +399,7 @@ static int32_t flush_channel(tanx_conv *cv, int ch)
     if (e == hipSuccess) e = launch_zero(cv->xprev + (size_t)ch * B, B * sizeof(float), s);
     // partial sums are laid out [C][tail_nchunk][B]: the next head reads them
     for (int f = 0; f < 2 && e == hipSuccess; f++)
-        e = launch_zero(cv->tail_ptr(f) + (size_t)ch * cv->tail_nchunk * B,
+        e = launch_zero(cv->tail_ptr(cv->tread, f) + (size_t)ch * cv->tail_nchunk * B,
                         (size_t)cv->tail_nchunk * B * sizeof(float2), s);
     return map_hip(e);
 }
@@ -462,51 +466,98 @@ static int32_t sync_run_mask(tanx_conv *cv)
     return TANX_OK;
 }
 
-// ProcessFinalize: the tail for the next block, with both filters when the
-// next block is the crossfade block; then the update hand-off that the
-// reference's update thread performs (ConvolutionImpl.cpp:1046-1079,
-// 2004-2199).
-static int32_t enqueue_tail(tanx_conv *cv)
+static TickArgs tick_common(tanx_conv *cv)
 {
-    tanx_ctx *ctx = cv->ctx;
-    const int nf = cv->delayed ? 2 : 1;
-    if (nf == 2) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
-    TailArgs a{};
-    a.fdl = cv->F;
+    TickArgs a{};
+    a.C = cv->C;
     a.P = cv->P;
-    a.base = cv->pos;
-    a.Ha = cv->slot_ptr(cv->cur);
-    a.Hb = nf == 2 ? cv->slot_ptr(cv->upd) : nullptr;
-    a.Ta = cv->tail_ptr(0);
-    a.Tb = nf == 2 ? cv->tail_ptr(1) : nullptr;
+    a.run = cv->drun;
+    const int lB = __builtin_ctz((unsigned)cv->B);
+    a.twM = tw_table(cv->ctx->tw, lB);
+    a.twN = tw_table(cv->ctx->tw, lB + 1);
+    a.fdl = cv->F;
+    return a;
+}
+
+// Tail role arguments for the block after the current one: partitions
+// 2..P-1, with both filters when that block is the crossfade block.
+static int fill_tail(tanx_conv *cv, TickArgs &a)
+{
+    const int nf = cv->delayed ? 2 : 1;
     a.nchunk = cv->nchunk;
     a.pc = cv->pc;
-    a.run = cv->drun;
-    KernelTimer &kt = cv->t_tail;
-    if (cv->timing) {
-        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
-        TANX_CHECK(hipEventRecord(kt.start[kt.n], ctx->proc));
-    }
-    TANX_CHECK(launch_tail(cv->B, cv->C, nf, a, ctx->proc));
-    if (cv->timing) {
-        TANX_CHECK(hipEventRecord(kt.stop[kt.n], ctx->proc));
-        kt.n++;
-        // algorithmic bytes: FDL rows and nf IR rows of partitions 1..P-1
-        cv->tail_alg_bytes += (uint64_t)cv->C * (uint64_t)(cv->P - 1) * cv->B * sizeof(float2) * (1 + nf);
-    }
+    a.tbase = cv->pos;  // pos already advanced past the current block
+    a.tHa = cv->slot_ptr(cv->cur);
+    a.tHb = nf == 2 ? cv->slot_ptr(cv->upd) : nullptr;
+    a.tTa = cv->tail_ptr(cv->tread ^ 1, 0);
+    a.tTb = nf == 2 ? cv->tail_ptr(cv->tread ^ 1, 1) : nullptr;
+    return nf;
+}
+
+// After a tail has been enqueued: the next head reads its buffer, and the
+// update hand-off the reference's update thread performs
+// (ConvolutionImpl.cpp:1046-1079, 2004-2199).
+static void tail_done(tanx_conv *cv)
+{
+    cv->tread ^= 1;
     cv->tail_nchunk = cv->nchunk;
-    TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
+    cv->tail_due = false;
     if (!cv->crossfading && cv->pending) {
         cv->pending = false;
         cv->ready = true;
         std::fill(cv->staged.begin(), cv->staged.end(), 0);
     }
+}
+
+static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const TickArgs &a,
+                            uint64_t alg_bytes, uint64_t &alg_sum)
+{
+    hipStream_t s = cv->ctx->proc;
+    if (cv->timing) {
+        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
+        TANX_CHECK(hipEventRecord(kt.start[kt.n], s));
+    }
+    TANX_CHECK(launch_tick(cv->B, tail_nf, a, s));
+    if (cv->timing) {
+        TANX_CHECK(hipEventRecord(kt.stop[kt.n], s));
+        kt.n++;
+        alg_sum += alg_bytes;
+    }
     return TANX_OK;
 }
 
-// Process state machine (ConvolutionImpl.cpp:848-1043) and the head launch.
-static int32_t process_head(tanx_conv *cv, const float *din, long long in_stride, float *dout,
-                            long long out_stride, size_t n, const uint32_t *flags)
+// algorithmic bytes of the roles (regular blocks count one filter)
+static uint64_t head_bytes(const tanx_conv *cv, int nf)
+{
+    // x_{t-1}, x_t, y, the new FDL row, the previous FDL row, and per filter
+    // partitions 0 and 1
+    return (uint64_t)cv->C * cv->B * (3 * sizeof(float) + 2 * sizeof(float2) + 2 * sizeof(float2) * nf);
+}
+static uint64_t tail_bytes(const tanx_conv *cv, int nf)
+{
+    // FDL rows and nf IR rows of partitions 2..P-1
+    const uint64_t parts = cv->P > 2 ? (uint64_t)(cv->P - 2) : 0;
+    return (uint64_t)cv->C * parts * cv->B * sizeof(float2) * (1 + nf);
+}
+
+// ProcessFinalize: the tail for the next block as its own launch.
+static int32_t enqueue_tail(tanx_conv *cv)
+{
+    if (!cv->tail_due) return TANX_OK;
+    TickArgs a = tick_common(cv);
+    a.nhead = 0;
+    const int nf = fill_tail(cv, a);
+    if (nf == 2) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
+    int32_t rc = launch_timed(cv, cv->t_tail, nf, a, tail_bytes(cv, nf), cv->tail_alg_bytes);
+    if (rc != TANX_OK) return rc;
+    tail_done(cv);
+    return TANX_OK;
+}
+
+// Process state machine (ConvolutionImpl.cpp:848-1043) for one block and the
+// head launch; with_tail also runs the next block's tail in the same launch.
+static int32_t process_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
+                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
 {
     tanx_ctx *ctx = cv->ctx;
     int nrun = 0;
@@ -527,41 +578,22 @@ static int32_t process_head(tanx_conv *cv, const float *din, long long in_stride
         cv->prev = cv->cur;
         cv->cur = (cv->cur + 1) % 3;
         cv->upd = (cv->upd + 1) % 3;
-        TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
     }
-    HeadArgs a{};
+    TickArgs a = tick_common(cv);
+    a.nhead = cv->C;
     a.xin = din;
     a.in_stride = in_stride;
     a.yout = dout;
     a.out_stride = out_stride;
     a.xprev = cv->xprev;
-    a.fdl = cv->F;
-    a.P = cv->P;
     a.slot = cv->pos;
     a.Ha = cv->slot_ptr(xfade ? cv->prev : cv->cur);
     a.Hb = xfade ? cv->slot_ptr(cv->cur) : nullptr;
-    a.Ta = cv->tail_ptr(0);
-    a.Tb = xfade ? cv->tail_ptr(1) : nullptr;
-    a.nchunk = cv->tail_nchunk;
+    a.Ta = cv->tail_ptr(cv->tread, 0);
+    a.Tb = xfade ? cv->tail_ptr(cv->tread, 1) : cv->tail_ptr(cv->tread, 0);
+    a.tchunk = cv->tail_nchunk;
     a.xfade = xfade ? 1 : 0;
-    a.run = cv->drun;
-    const int lB = __builtin_ctz((unsigned)cv->B);
-    a.twM = tw_table(ctx->tw, lB);
-    a.twN = tw_table(ctx->tw, lB + 1);
-    KernelTimer &kt = cv->t_head;
-    if (cv->timing) {
-        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
-        TANX_CHECK(hipEventRecord(kt.start[kt.n], ctx->proc));
-    }
-    TANX_CHECK(launch_head(cv->B, cv->C, a, ctx->proc));
-    if (cv->timing) {
-        TANX_CHECK(hipEventRecord(kt.stop[kt.n], ctx->proc));
-        kt.n++;
-        // x_{t-1}, x_t, y, the new FDL row and partition 0 of each filter
-        const uint64_t nf = xfade ? 2 : 1;
-        cv->head_alg_bytes += (uint64_t)cv->C * cv->B * (3 * sizeof(float) + sizeof(float2) * (1 + nf));
-    }
-    TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
+
     cv->pos = (cv->pos + 1) % cv->P;
     cv->crossfading = xfade;
     cv->last_xfade = xfade;
@@ -570,6 +602,20 @@ static int32_t process_head(tanx_conv *cv, const float *din, long long in_stride
         cv->xfades++;
     }
     cv->ticks++;
+    cv->tail_due = true;
+
+    int tail_nf = 1;
+    if (with_tail) tail_nf = fill_tail(cv, a);
+    if (xfade || tail_nf == 2) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
+    const uint64_t hb = head_bytes(cv, xfade ? 2 : 1);
+    if (with_tail) {
+        rc = launch_timed(cv, cv->t_head, tail_nf, a, hb + tail_bytes(cv, tail_nf), cv->head_alg_bytes);
+        if (rc != TANX_OK) return rc;
+        tail_done(cv);
+    } else {
+        rc = launch_timed(cv, cv->t_head, 1, a, hb, cv->head_alg_bytes);
+        if (rc != TANX_OK) return rc;
+    }
     return TANX_OK;
 }
 
@@ -581,7 +627,7 @@ static int32_t process_host_block(tanx_conv *cv, const float *const *in, float *
     for (int ch = 0; ch < cv->C; ch++) std::memcpy(cv->hin + ch * B, in[ch] + off, B * sizeof(float));
     hipStream_t s = cv->ctx->proc;
     TANX_CHECK(hipMemcpyAsync(cv->din, cv->hin, cv->C * B * sizeof(float), hipMemcpyHostToDevice, s));
-    int32_t rc = process_head(cv, cv->din, (long long)B, cv->dout, (long long)B, n, flags);
+    int32_t rc = process_block(cv, cv->din, (long long)B, cv->dout, (long long)B, n, flags, false);
     if (rc != TANX_OK) return rc;
     TANX_CHECK(hipMemcpyAsync(cv->hout, cv->dout, cv->C * B * sizeof(float), hipMemcpyDeviceToHost, s));
     TANX_CHECK(hipEventRecord(cv->ev_out, s));
@@ -631,10 +677,11 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (in_stride < nb * B || out_stride < nb * B) return TANX_INVALID_ARG;
     DeviceGuard g(cv->ctx->device);
     for (size_t b = 0; b < nb; b++) {
-        int32_t rc = process_head(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
-                                  (long long)out_stride, B, flags);
+        const bool fuse = cv->fused && !cv->fmode;
+        int32_t rc = process_block(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
+                                   (long long)out_stride, B, flags, fuse);
         if (rc != TANX_OK) return rc;
-        if (!cv->fmode) {
+        if (!cv->fmode && !fuse) {
             rc = enqueue_tail(cv);
             if (rc != TANX_OK) return rc;
         }
@@ -681,6 +728,7 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         cv->hchan[nupd++] = ch;
     }
     // the update slot must not be overwritten while queued process work reads it
+    TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
     TANX_CHECK(hipStreamWaitEvent(ctx->upd, cv->ev_proc, 0));
     const size_t slotElems = (size_t)cv->P * cv->B;
     if (nupd > 0) {
@@ -763,9 +811,7 @@ extern "C" int32_t tanx_conv_flush(tanx_conv *cv, uint32_t channel)
     if (channel >= (uint32_t)cv->C) return TANX_INVALID_ARG;
     std::lock_guard<std::mutex> lk(cv->mtx);
     DeviceGuard g(cv->ctx->device);
-    int32_t rc = flush_channel(cv, (int)channel);
-    if (rc == TANX_OK) TANX_CHECK(hipEventRecord(cv->ev_proc, cv->ctx->proc));
-    return rc;
+    return flush_channel(cv, (int)channel);
 }
 
 extern "C" int32_t tanx_conv_get_next_free_channel(tanx_conv *cv, uint32_t *channel, const uint32_t *flags)
@@ -850,6 +896,14 @@ extern "C" int32_t tanx_conv_read_timing(tanx_conv *cv, tanx_kernel_timing *out,
         cv->t_head.count = cv->t_tail.count = 0;
         cv->head_alg_bytes = cv->tail_alg_bytes = 0;
     }
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_conv_set_fused(tanx_conv *cv, uint32_t fused)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    cv->fused = fused != 0;
     return TANX_OK;
 }
 

@@ -12,37 +12,44 @@ namespace tanx {
 // (DC.re, Nyquist.re) and elements 1..B-1 hold bins 1..B-1.  Every spectrum
 // row is therefore exactly B complex floats (B*8 bytes).
 
-struct HeadArgs {
+// One launch of k_tick runs up to two roles:
+//   head (blocks [0, nhead)): channel c = block; R2C of [x_{t-1}, x_t], FDL
+//     insert, A_f = sum of tail partials + X_{t-1} H_{f,1} + X_t H_{f,0},
+//     C2R, crossfade, output;
+//   tail (blocks [nhead, nhead + C * nchunk)): partial sums over partitions
+//     [2 + j*pc, 2 + (j+1)*pc) of X_{t+1-i} H_i for the NEXT block.
+// The two roles touch disjoint data, so they may share a launch.
+struct TickArgs {
+    int C;
+    int P;
+    const uint8_t *run;    // [C] 1 = channel running
+    const float2 *twM;     // W_B^t, t < B
+    const float2 *twN;     // W_2B^t, t < 2B
+    float2 *fdl;           // [C][P][B] ring of input spectra
+
+    // head role
+    int nhead;             // C or 0
     const float *xin;      // channel c block at xin + c * in_stride
     long long in_stride;
     float *yout;           // channel c output at yout + c * out_stride
     long long out_stride;
     float *xprev;          // [C][B] previous input block (overlap-save frame)
-    float2 *fdl;           // [C][P][B] ring of input spectra
-    int P;
-    int slot;              // ring slot receiving this block's spectrum
-    const float2 *Ha;      // filter A slot base [C][P][B] (partition 0 used)
-    const float2 *Hb;      // filter B slot base (crossfade) or nullptr
-    const float2 *Ta;      // tail partial sums for filter A [C][nchunk][B]
-    const float2 *Tb;      // tail partial sums for filter B or nullptr
-    int nchunk;
+    int slot;              // ring slot receiving X_t
+    const float2 *Ha;      // filter slot bases [C][P][B]
+    const float2 *Hb;      // crossfade: new filter, else nullptr
+    const float2 *Ta;      // partial sums read by the head [C][tchunk][B]
+    const float2 *Tb;
+    int tchunk;
     int xfade;             // 0: y = y_A; 1: y = y_B * j/F + y_A * (F-j)/F
-    const uint8_t *run;    // [C] 1 = channel running
-    const float2 *twM;     // W_B^t, t < B
-    const float2 *twN;     // W_2B^t, t < 2B
-};
 
-struct TailArgs {
-    const float2 *fdl;     // [C][P][B]
-    int P;
-    int base;              // ring slot of the block the tail is for
-    const float2 *Ha;      // [C][P][B]
-    const float2 *Hb;      // nullptr for one filter
-    float2 *Ta;            // [C][nchunk][B]
-    float2 *Tb;
-    int nchunk;
+    // tail role
+    int nchunk;            // chunks per channel (0 = no tail blocks)
     int pc;                // partitions per chunk
-    const uint8_t *run;
+    int tbase;             // ring slot of block t+1
+    const float2 *tHa;
+    const float2 *tHb;     // second filter (next block crossfades) or nullptr
+    float2 *tTa;           // partial sums written [C][nchunk][B]
+    float2 *tTb;
 };
 
 struct IrArgs {
@@ -67,8 +74,7 @@ struct FftArgs {
 };
 
 // Launchers (return hipError_t).  B is the block size (power of two).
-hipError_t launch_head(int B, int C, const HeadArgs &a, hipStream_t s);
-hipError_t launch_tail(int B, int C, int nf, const TailArgs &a, hipStream_t s);
+hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s);
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);

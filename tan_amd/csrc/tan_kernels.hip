@@ -114,94 +114,36 @@ __device__ __forceinline__ float2 c2r_pre(const float2 *A, int k, const float2 *
 }
 
 // ---------------------------------------------------------------------------
-// Head: one workgroup per channel.
+// Tick kernel: head and tail roles (see TickArgs).  256 threads per block.
 // ---------------------------------------------------------------------------
-template <int M>
-__global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_head(HeadArgs a)
+constexpr int kTickThreads = 256;
+constexpr int kHeadChunkBatch = 16;  // partial-sum rows loaded together by the head
+
+// Streamed rows (IR spectra, FDL) are read once per block: nontemporal loads
+// keep them from displacing reused data in the caches.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_stream(const float4 *p)
 {
-    constexpr float kScale = 1.0f / (2.0f * (float)M);
-    __shared__ float2 s0[M], s1[M], sx[M], sa[M];
-    __shared__ float ya[M];
-    const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    if (!a.run[c]) return;
-
-    float *xp = a.xprev + (size_t)c * M;
-    const float *xi = a.xin + (size_t)c * a.in_stride;
-    // frame [x_{t-1}, x_t] as M complex points
-    for (int n = tid; n < M; n += nt) {
-        const float *src = n < M / 2 ? xp + 2 * n : xi + 2 * (n - M / 2);
-        s0[n] = make_float2(src[0], src[1]);
-    }
-    __syncthreads();
-    for (int j = tid; j < M; j += nt) xp[j] = xi[j];
-
-    const float2 *Z = fft_lds<M, false>(s0, s1, a.twM);
-    float2 *row = a.fdl + ((size_t)c * a.P + a.slot) * M;
-    for (int k = tid; k < M; k += nt) {
-        const float2 X = r2c_post<M>(Z, k, a.twN);
-        sx[k] = X;
-        row[k] = X;
-    }
-    __syncthreads();
-
-    const int nf = a.xfade ? 2 : 1;
-    for (int f = 0; f < nf; f++) {
-        const float2 *H0 = (f == 0 ? a.Ha : a.Hb) + (size_t)c * a.P * M;
-        const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.nchunk * M;
-        for (int k = tid; k < M; k += nt) {
-            float2 acc = make_float2(0.f, 0.f);
-            for (int j = 0; j < a.nchunk; j++) {
-                const float2 t = T[(size_t)j * M + k];
-                acc.x += t.x;
-                acc.y += t.y;
-            }
-            const float2 x = sx[k], h = H0[k];
-            if (k == 0) {
-                acc.x = fmaf(x.x, h.x, acc.x);
-                acc.y = fmaf(x.y, h.y, acc.y);
-            } else {
-                acc.x = fmaf(x.x, h.x, acc.x);
-                acc.x = fmaf(-x.y, h.y, acc.x);
-                acc.y = fmaf(x.x, h.y, acc.y);
-                acc.y = fmaf(x.y, h.x, acc.y);
-            }
-            sa[k] = acc;
-        }
-        __syncthreads();
-        for (int k = tid; k < M; k += nt) s0[k] = c2r_pre<M>(sa, k, a.twN);
-        __syncthreads();
-        const float2 *z = fft_lds<M, true>(s0, s1, a.twM);
-        // second half of the 2M output samples: z[M/2 .. M)
-        if (f == 0) {
-            for (int n = tid; n < M / 2; n += nt) {
-                const float2 v = z[M / 2 + n];
-                ya[2 * n] = v.x * kScale;
-                ya[2 * n + 1] = v.y * kScale;
-            }
-        } else {
-            const float step = (float)(1.0 / (double)M);
-            for (int n = tid; n < M / 2; n += nt) {
-                const float2 v = z[M / 2 + n];
-                const int j0 = 2 * n, j1 = 2 * n + 1;
-                ya[j0] = (v.x * kScale) * (step * (float)j0) + ya[j0] * (step * (float)(M - j0));
-                ya[j1] = (v.y * kScale) * (step * (float)j1) + ya[j1] * (step * (float)(M - j1));
-            }
-        }
-        __syncthreads();
-    }
-    float *yo = a.yout + (size_t)c * a.out_stride;
-    for (int j = tid; j < M; j += nt) yo[j] = ya[j];
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 
-// ---------------------------------------------------------------------------
-// Tail: sum over partitions [1 + j*pc, 1 + (j+1)*pc) of X_{base-i} H_i for
-// one channel and chunk j.  256 threads; each row of B complex is read as
-// float4 (two complex) per lane; G rows are in flight per iteration.
-// ---------------------------------------------------------------------------
-template <int NF>
+__device__ __forceinline__ void mac2(float2 &acc, float2 x, float2 h, bool dc)
+{
+    // packed DC/Nyquist element: two real products; otherwise complex
+    acc.x = fmaf(x.x, h.x, acc.x);
+    if (dc) {
+        acc.y = fmaf(x.y, h.y, acc.y);
+    } else {
+        acc.x = fmaf(-x.y, h.y, acc.x);
+        acc.y = fmaf(x.x, h.y, acc.y);
+        acc.y = fmaf(x.y, h.x, acc.y);
+    }
+}
+
 __device__ __forceinline__ void mac4(float4 &acc, float4 x, float4 h, float s, bool dc)
 {
-    // complex 0 (packed DC/Nyquist when dc)
+    // complex 0 (packed DC/Nyquist when dc; s = 0 then)
     acc.x = fmaf(x.x, h.x, acc.x);
     acc.x = fmaf(-s * x.y, h.y, acc.x);
     acc.y = fmaf(s * x.x, h.y, acc.y);
@@ -213,24 +155,135 @@ __device__ __forceinline__ void mac4(float4 &acc, float4 x, float4 h, float s, b
     acc.w = fmaf(x.w, h.z, acc.w);
 }
 
+template <int M>
+struct HeadSmem {
+    float2 s0[M], s1[M], sx[M], sa[M], tw[M], twn[M];
+    float ya[M];
+};
+
+template <int M>
+__device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
+{
+    constexpr int NT = kTickThreads;
+    constexpr int KPT = M >= NT ? M / NT : 1;  // bins per thread
+    constexpr float kScale = 1.0f / (2.0f * (float)M);
+    const int tid = threadIdx.x;
+    const int nf = a.xfade ? 2 : 1;
+
+    // ---- issue every global load first: frame, twiddles, H0/H1, partials
+    const float *xp = a.xprev + (size_t)c * M;
+    const float *xi = a.xin + (size_t)c * a.in_stride;
+    for (int n = tid; n < M; n += NT) {
+        const float *src = n < M / 2 ? xp + 2 * n : xi + 2 * (n - M / 2);
+        sm.s0[n] = make_float2(src[0], src[1]);
+        sm.tw[n] = a.twM[n];
+        sm.twn[n] = a.twN[n];
+    }
+    const int prevslot = (a.slot - 1) & (a.P - 1);
+    const float2 *Fprev = a.fdl + ((size_t)c * a.P + prevslot) * M;
+    float2 acc[2][KPT], h0[2][KPT];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        if (f >= nf) continue;
+        const float2 *H = (f == 0 ? a.Ha : a.Hb) + (size_t)c * a.P * M;
+        const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.tchunk * M;
+#pragma unroll
+        for (int q = 0; q < KPT; q++) {
+            const int k = tid + q * NT;
+            float2 s = make_float2(0.f, 0.f);
+            if (k < M) {
+                // every partial-sum load in flight at once (one memory round trip)
+                int j = 0;
+                for (; j < a.tchunk; j += kHeadChunkBatch) {
+                    float2 t[kHeadChunkBatch];
+#pragma unroll
+                    for (int u = 0; u < kHeadChunkBatch; u++)
+                        t[u] = (j + u < a.tchunk) ? T[(size_t)(j + u) * M + k] : make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int u = 0; u < kHeadChunkBatch; u++) {
+                        s.x += t[u].x;
+                        s.y += t[u].y;
+                    }
+                }
+                if (a.P > 1) mac2(s, Fprev[k], H[M + k], k == 0);
+                h0[f][q] = H[k];
+            }
+            acc[f][q] = s;
+        }
+    }
+    __syncthreads();
+    if (tid < M) {
+        // this block becomes the previous one of the next frame
+        float *xpw = a.xprev + (size_t)c * M;
+        for (int j = tid; j < M; j += NT) xpw[j] = xi[j];
+    }
+
+    const float2 *Z = fft_lds<M, false>(sm.s0, sm.s1, sm.tw);
+    float2 *row = a.fdl + ((size_t)c * a.P + a.slot) * M;
+    for (int k = tid; k < M; k += NT) {
+        const float2 X = r2c_post<M>(Z, k, sm.twn);
+        sm.sx[k] = X;
+        row[k] = X;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        if (f >= nf) continue;
+#pragma unroll
+        for (int q = 0; q < KPT; q++) {
+            const int k = tid + q * NT;
+            if (k < M) {
+                float2 v = acc[f][q];
+                mac2(v, sm.sx[k], h0[f][q], k == 0);
+                sm.sa[k] = v;
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < M; k += NT) sm.s0[k] = c2r_pre<M>(sm.sa, k, sm.twn);
+        __syncthreads();
+        const float2 *z = fft_lds<M, true>(sm.s0, sm.s1, sm.tw);
+        // second half of the 2M output samples: z[M/2 .. M)
+        if (f == 0) {
+            for (int n = tid; n < M / 2; n += NT) {
+                const float2 v = z[M / 2 + n];
+                sm.ya[2 * n] = v.x * kScale;
+                sm.ya[2 * n + 1] = v.y * kScale;
+            }
+        } else {
+            const float step = (float)(1.0 / (double)M);
+            for (int n = tid; n < M / 2; n += NT) {
+                const float2 v = z[M / 2 + n];
+                const int j0 = 2 * n, j1 = 2 * n + 1;
+                sm.ya[j0] = (v.x * kScale) * (step * (float)j0) + sm.ya[j0] * (step * (float)(M - j0));
+                sm.ya[j1] = (v.y * kScale) * (step * (float)j1) + sm.ya[j1] * (step * (float)(M - j1));
+            }
+        }
+        __syncthreads();
+    }
+    float *yo = a.yout + (size_t)c * a.out_stride;
+    for (int j = tid; j < M; j += NT) yo[j] = sm.ya[j];
+}
+
+// Partial sums over partitions [i0, i1) for one channel; 256 threads; a row
+// of B complex is read as float4 (two complex) per lane, G rows in flight.
 template <int M, int NF>
-__global__ __launch_bounds__(256) void k_tail(TailArgs a)
+__device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
 {
     constexpr int H = M / 2;                     // float4 per row
     constexpr int V = H > 256 ? H / 256 : 1;     // float4 per lane per row
     constexpr int R = H / V;                     // lanes per row
-    constexpr int G = 256 / R;                   // rows in flight
-    constexpr int U = 4;                         // unroll (partitions per lane)
-    const int c = blockIdx.y, j = blockIdx.x, tid = threadIdx.x;
-    if (!a.run[c]) return;
+    constexpr int G = kTickThreads / R;          // rows in flight
+    constexpr int U = 8;                         // unroll (partitions per lane)
+    const int tid = threadIdx.x;
     const int g = tid / R, r = tid % R;
-    const int i0 = 1 + j * a.pc;
+    const int i0 = 2 + j * a.pc;
     const int i1 = min(a.P, i0 + a.pc);
     const int mask = a.P - 1;
 
     const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
-    const float4 *HA = reinterpret_cast<const float4 *>(a.Ha) + (size_t)c * a.P * H;
-    const float4 *HB = NF == 2 ? reinterpret_cast<const float4 *>(a.Hb) + (size_t)c * a.P * H : nullptr;
+    const float4 *HA = reinterpret_cast<const float4 *>(a.tHa) + (size_t)c * a.P * H;
+    const float4 *HB = NF == 2 ? reinterpret_cast<const float4 *>(a.tHb) + (size_t)c * a.P * H : nullptr;
 
     float4 accA[V], accB[V];
 #pragma unroll
@@ -247,12 +300,12 @@ __global__ __launch_bounds__(256) void k_tail(TailArgs a)
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int ii = i + u * G;
-            const int s = (a.base - ii) & mask;
+            const int s = (a.tbase - ii) & mask;
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                xf[u][v] = F[(size_t)s * H + v * R + r];
-                ha[u][v] = HA[(size_t)ii * H + v * R + r];
-                if (NF == 2) hb[u][v] = HB[(size_t)ii * H + v * R + r];
+                xf[u][v] = ld_stream(F + (size_t)s * H + v * R + r);
+                ha[u][v] = ld_stream(HA + (size_t)ii * H + v * R + r);
+                if (NF == 2) hb[u][v] = ld_stream(HB + (size_t)ii * H + v * R + r);
             }
         }
 #pragma unroll
@@ -261,25 +314,25 @@ __global__ __launch_bounds__(256) void k_tail(TailArgs a)
             for (int v = 0; v < V; v++) {
                 const bool dc = dc0 && v == 0;
                 const float s = v == 0 ? s0 : 1.f;
-                mac4<NF>(accA[v], xf[u][v], ha[u][v], s, dc);
-                if (NF == 2) mac4<NF>(accB[v], xf[u][v], hb[u][v], s, dc);
+                mac4(accA[v], xf[u][v], ha[u][v], s, dc);
+                if (NF == 2) mac4(accB[v], xf[u][v], hb[u][v], s, dc);
             }
         }
     }
     for (; i < i1; i += G) {
-        const int s = (a.base - i) & mask;
+        const int s = (a.tbase - i) & mask;
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const bool dc = dc0 && v == 0;
             const float sv = v == 0 ? s0 : 1.f;
-            const float4 x = F[(size_t)s * H + v * R + r];
-            mac4<NF>(accA[v], x, HA[(size_t)i * H + v * R + r], sv, dc);
-            if (NF == 2) mac4<NF>(accB[v], x, HB[(size_t)i * H + v * R + r], sv, dc);
+            const float4 x = ld_stream(F + (size_t)s * H + v * R + r);
+            mac4(accA[v], x, ld_stream(HA + (size_t)i * H + v * R + r), sv, dc);
+            if (NF == 2) mac4(accB[v], x, ld_stream(HB + (size_t)i * H + v * R + r), sv, dc);
         }
     }
 
-    float4 *TA = reinterpret_cast<float4 *>(a.Ta) + ((size_t)c * a.nchunk + j) * H;
-    float4 *TB = NF == 2 ? reinterpret_cast<float4 *>(a.Tb) + ((size_t)c * a.nchunk + j) * H : nullptr;
+    float4 *TA = reinterpret_cast<float4 *>(a.tTa) + ((size_t)c * a.nchunk + j) * H;
+    float4 *TB = NF == 2 ? reinterpret_cast<float4 *>(a.tTb) + ((size_t)c * a.nchunk + j) * H : nullptr;
     if constexpr (G == 1) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -287,23 +340,42 @@ __global__ __launch_bounds__(256) void k_tail(TailArgs a)
             if (NF == 2) TB[v * R + r] = accB[v];
         }
     } else {
-        __shared__ float4 red[NF][G][H];
+        // red: [NF][G][H] float4
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            red[0][g][v * R + r] = accA[v];
-            if (NF == 2) red[NF - 1][g][v * R + r] = accB[v];
+            red[(0 * G + g) * H + v * R + r] = accA[v];
+            if (NF == 2) red[(1 * G + g) * H + v * R + r] = accB[v];
         }
         __syncthreads();
-        for (int e = tid; e < NF * H; e += 256) {
+        for (int e = tid; e < NF * H; e += kTickThreads) {
             const int f = e / H, q = e % H;
-            float4 sum = red[f][0][q];
+            float4 sum = red[(f * G + 0) * H + q];
 #pragma unroll
             for (int gg = 1; gg < G; gg++) {
-                const float4 t = red[f][gg][q];
+                const float4 t = red[(f * G + gg) * H + q];
                 sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
             }
             (f == 0 ? TA : TB)[q] = sum;
         }
+    }
+}
+
+template <int M, int NF>
+__global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
+{
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
+    constexpr int kHeadBytes = sizeof(HeadSmem<M>);
+    constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
+    __shared__ float4 smem[(kBytes + 15) / 16];
+    const int b = blockIdx.x;
+    if (b < a.nhead) {
+        if (!a.run[b]) return;
+        head_role<M>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
+    } else {
+        const int t = b - a.nhead;
+        const int c = t / a.nchunk, j = t % a.nchunk;
+        if (!a.run[c]) return;
+        tail_role<M, NF>(a, c, j, smem);
     }
 }
 
@@ -416,18 +488,14 @@ static inline int fft_threads(int M)
     default: return hipErrorInvalidValue;               \
     }
 
-hipError_t launch_head(int B, int C, const HeadArgs &a, hipStream_t s)
+hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s)
 {
-    TANX_SWITCH_B(B, hipLaunchKernelGGL(k_head<M>, dim3(C), dim3(fft_threads(M)), 0, s, a));
-    return hipGetLastError();
-}
-
-hipError_t launch_tail(int B, int C, int nf, const TailArgs &a, hipStream_t s)
-{
-    if (nf == 2) {
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tail<M, 2>), dim3(a.nchunk, C), dim3(256), 0, s, a));
+    const unsigned blocks = (unsigned)(a.nhead + a.C * a.nchunk);
+    if (blocks == 0) return hipSuccess;
+    if (tail_nf == 2) {
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 2>), dim3(blocks), dim3(kTickThreads), 0, s, a));
     } else {
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tail<M, 1>), dim3(a.nchunk, C), dim3(256), 0, s, a));
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 1>), dim3(blocks), dim3(kTickThreads), 0, s, a));
     }
     return hipGetLastError();
 }

@@ -252,6 +252,9 @@ class TANConvolution:
     def SetTailChunks(self, chunks):
         return self._lib.tanx_conv_set_tail_chunks(self.handle, chunks)
 
+    def SetFused(self, fused=True):
+        return self._lib.tanx_conv_set_fused(self.handle, int(bool(fused)))
+
     def EnableTiming(self, enable=True):
         return self._lib.tanx_conv_enable_timing(self.handle, int(bool(enable)))
 

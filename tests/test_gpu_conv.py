@@ -236,9 +236,11 @@ def test_tail_chunking_invariant(gpu_ctx, chunks):
     conv.Terminate()
 
 
-def test_device_memory_variants(gpu_ctx):
+@pytest.mark.parametrize("fused", [True, False])
+def test_device_memory_variants(gpu_ctx, fused):
     """Process/UpdateResponseTD on device buffers (the cl_mem stand-ins)
-    equal the host-memory path."""
+    equal the host-memory path bit for bit, with the head and the next
+    block's tail in one launch (fused) or in two."""
     import torch
     L, B, C, nb = 8192, 256, 3, 24
     ir = _irs(C, L, 0)
@@ -247,6 +249,7 @@ def test_device_memory_variants(gpu_ctx):
     conv_d = T.TANConvolution(gpu_ctx)
     conv_h.InitGpu(UNIFORM, L, B, C)
     conv_d.InitGpu(UNIFORM, L, B, C)
+    conv_d.SetFused(fused)
     conv_h.UpdateResponseTD(ir, L, None, BLOCKING)
     dir_ = torch.from_numpy(ir).cuda()
     torch.cuda.synchronize()
@@ -265,3 +268,35 @@ def test_device_memory_variants(gpu_ctx):
     np.testing.assert_array_equal(dy.cpu().numpy(), yh)
     conv_h.Terminate()
     conv_d.Terminate()
+
+
+def test_multiblock_device_process_with_updates(gpu_ctx):
+    """One device Process call over many blocks equals block-by-block calls,
+    including a crossfade inside the call."""
+    import torch
+    L, B, C, nb = 4096, 128, 2, 40
+    x = T.signals.input_bank(C, nb * B)
+    dx = torch.from_numpy(x).cuda()
+    outs = []
+    for per_call in (1, 20):
+        conv = T.TANConvolution(gpu_ctx)
+        conv.InitGpu(UNIFORM, L, B, C)
+        dy = torch.zeros_like(dx)
+        torch.cuda.synchronize()
+        for b0 in range(0, nb, per_call):
+            if b0 in (0, 20):
+                conv.UpdateResponseTD(_irs(C, L, b0), L, None, BLOCKING)
+            rc, done = conv.Process(dx[:, b0 * B:], dy[:, b0 * B:], per_call * B)
+            assert rc == T.AMF_OK and done == per_call * B
+        conv.Synchronize()
+        outs.append(dy.cpu().numpy())
+        conv.Terminate()
+    np.testing.assert_array_equal(outs[0], outs[1])
+    orc = O.OracleConvolution(L, B, C)
+    yo = np.zeros_like(x)
+    for b in range(nb):
+        if b in (0, 20):
+            orc.update_td(_irs(C, L, b))
+        _, o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
+        yo[:, b * B:(b + 1) * B] = o
+    assert_parity(outs[0], yo, "multiblock")
