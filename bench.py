@@ -186,6 +186,20 @@ def bench_sweep(T, torch, ctx, counts, ticks=96):
     return best, rows
 
 
+def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1>"):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_k_tick.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    for k in data.get("kernels", []):
+        if k["kernel"].startswith(kernel_prefix) and k["grid_threads"] == grid_threads:
+            return float(k["traffic_bytes"])
+    return None
+
+
 def cpu_baseline(seconds):
     """The oracle's restatement of the reference CPU path, timed on this host."""
     path = os.path.join(ROOT, "oracle", "build", "libtan_oracle.so")
@@ -214,6 +228,34 @@ def cpu_baseline(seconds):
             "mean_block_ms": float(arr.mean() * 1e3)}
 
 
+def profile_run(T, torch, ctx):
+    """Short fixed workload for counter collection (rocprofv3 --pmc): 100
+    blocks of the headline configuration, 10 blocks at 8,192 channels
+    (HBM-bound), and a plain device copy of 512 MiB (3 launches) whose
+    FETCH_SIZE/WRITE_SIZE calibrate the counters against known bytes."""
+    bench_throughput(T, torch, ctx, 100, 10, timing=False)
+    ch = 8192
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, ch) == T.AMF_OK
+    h = ir_bank_device(torch, ch, L, 77)
+    torch.cuda.synchronize()
+    conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY)
+    del h
+    x = noise_device(torch, ch, 16 * B, 5)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    conv.Process(x, y, 10 * B)
+    conv.Synchronize()
+    conv.Terminate()
+    del x, y
+    a = torch.ones(512 << 18, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("profile run done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +266,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,32768,40960")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (channels and "
@@ -234,6 +278,10 @@ def main():
     import tan_amd as T
     T.load_library()
     ctx = T.TANContext(device=0)
+    if args.profile_run:
+        profile_run(T, torch, ctx)
+        ctx.Terminate()
+        return
 
     log("throughput pass")
     dt, _, info, finite = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=False)
@@ -247,8 +295,11 @@ def main():
     # head and the next block's tail multiply-accumulate
     tick_us = kt["head_ms"] / kt["head_launches"] * 1e3
     achieved = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9
+    # 64 head blocks + 64 x 16 tail blocks of 256 threads
+    traffic = pmc_traffic((C + C * info2["tail_chunks"]) * 256)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "k_tick (head + next tail)",
+                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": "k_tick (head + next tail)",
+                "traffic_source": "profiles/pmc_traffic_k_tick.json" if traffic else None,
                 "alg_bytes_per_launch": kt["head_alg_bytes"] / kt["head_launches"],
                 "avg_launch_us": tick_us, "launches": kt["head_launches"],
                 "note": ("64-channel working set (~67 MB) is resident in the 256 MB Infinity "

@@ -31,6 +31,22 @@ def build_library(verbose=False):
     return out
 
 
+def build_harness(verbose=False):
+    """C++ driver restating TALibTestConvolution against include/TrueAudioNext.h
+    (test infrastructure; links the library and the oracle)."""
+    out_dir = os.path.join(ROOT, "tests", "cpp", "build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "talib_harness")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "oracle"),
+           os.path.join(ROOT, "tests", "cpp", "talib_harness.cpp"), "-o", out,
+           "-L" + LIBDIR, "-ltan_mi355x", "-L" + os.path.join(ROOT, "oracle", "build"), "-ltan_oracle",
+           "-Wl,-rpath,$ORIGIN/../../../tan_amd/lib:$ORIGIN/../../../oracle/build"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
+
+
 def build_oracle(verbose=False):
     cmd = ["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")]
     if verbose:
@@ -41,3 +57,4 @@ def build_oracle(verbose=False):
 if __name__ == "__main__":
     build_library(verbose=True)
     build_oracle(verbose=True)
+    build_harness(verbose=True)

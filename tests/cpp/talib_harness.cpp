@@ -1,0 +1,174 @@
+// Restatement of the reference's convolution driver
+// (tan/tests/src/TALibTestConvolution/TALibTestConvolution.cpp:66-321) against
+// this library's C++ interface (include/TrueAudioNext.h), on the driver's own
+// input files, with the check the reference leaves out: every output block
+// is compared with the CPU oracle (oracle/tan_oracle.h) driven through the
+// same calls.  Also exercises TANFFT through the C++ interface.
+//
+// usage: talib_harness <dir with pulsetrainS_0p5s.wav, testresponse.wav, testresponse2.wav>
+#include "TrueAudioNext.h"
+#include "tan_oracle.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace amf;
+
+namespace {
+
+// PCM16 WAV reader; samples scaled by 1/32768 as the reference's reader does
+// (tan/common/wav.cpp:235).
+bool read_wav(const std::string &path, int &channels, std::vector<std::vector<float>> &data)
+{
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::vector<unsigned char> buf;
+    unsigned char tmp[65536];
+    size_t n;
+    while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    std::fclose(f);
+    if (buf.size() < 12 || std::memcmp(buf.data(), "RIFF", 4) || std::memcmp(buf.data() + 8, "WAVE", 4)) return false;
+    size_t pos = 12;
+    int bits = 0;
+    channels = 0;
+    while (pos + 8 <= buf.size()) {
+        uint32_t sz;
+        std::memcpy(&sz, buf.data() + pos + 4, 4);
+        const unsigned char *body = buf.data() + pos + 8;
+        if (!std::memcmp(buf.data() + pos, "fmt ", 4)) {
+            uint16_t ch, bps;
+            std::memcpy(&ch, body + 2, 2);
+            std::memcpy(&bps, body + 14, 2);
+            channels = ch;
+            bits = bps;
+        } else if (!std::memcmp(buf.data() + pos, "data", 4)) {
+            if (bits != 16 || channels <= 0) return false;
+            const size_t frames = sz / (2 * channels);
+            data.assign(channels, std::vector<float>(frames));
+            const int16_t *s = reinterpret_cast<const int16_t *>(body);
+            for (size_t i = 0; i < frames; i++)
+                for (int c = 0; c < channels; c++) data[c][i] = (float)s[i * channels + c] / 32768.0f;
+            return true;
+        }
+        pos += 8 + sz + (sz & 1);
+    }
+    return false;
+}
+
+int fail(const char *what, int rc)
+{
+    std::printf("FAIL: %s (AMF_RESULT %d)\n", what, rc);
+    return 1;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const std::string dir = argc > 1 ? argv[1] : "tests/golden";
+    int nch = 0, nres_ch = 0, nres2_ch = 0;
+    std::vector<std::vector<float>> samples, resp1, resp2;
+    if (!read_wav(dir + "/pulsetrainS_0p5s.wav", nch, samples) || !read_wav(dir + "/testresponse.wav", nres_ch, resp1) ||
+        !read_wav(dir + "/testresponse2.wav", nres2_ch, resp2) || nch != nres_ch || nch != nres2_ch) {
+        std::printf("FAIL: cannot read the input files in %s\n", dir.c_str());
+        return 1;
+    }
+    const int n_samples = 128;  // TALibTestConvolution.cpp:92
+    const amf_uint32 NSamples = (amf_uint32)samples[0].size();
+    const amf_uint32 NResSamples = (amf_uint32)resp1[0].size();
+
+    TANContextPtr ctx;
+    AMF_RESULT r = TANCreateContext(TAN_FULL_VERSION, &ctx);
+    if (r != AMF_OK) return fail("TANCreateContext", r);
+    if ((r = ctx->InitHIP(0)) != AMF_OK) return fail("InitHIP", r);
+    TANConvolutionPtr conv;
+    if ((r = TANCreateConvolution(ctx, &conv)) != AMF_OK) return fail("TANCreateConvolution", r);
+    // the driver's GPU-UN method (TALibTestConvolution.cpp:78-85)
+    if ((r = conv->InitGpu(TAN_CONVOLUTION_METHOD_FHT_UNIFORM_PARTITIONED, NResSamples, n_samples, nch)) != AMF_OK)
+        return fail("InitGpu", r);
+    if (conv->InitCpu(TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_UNIFORM, NResSamples, n_samples, nch) != AMF_NOT_SUPPORTED)
+        return fail("InitCpu should be unsupported", -1);
+
+    orc_conv *orc = orc_conv_create((int)NResSamples, n_samples, nch, ORC_XFADE_INTENDED, 0);
+    if (!orc) return fail("oracle", -1);
+
+    std::vector<float *> pr1(nch), pr2(nch);
+    std::vector<std::vector<float>> out(nch, std::vector<float>(NSamples, 0.f)), oref = out;
+    for (int c = 0; c < nch; c++) {
+        pr1[c] = resp1[c].data();
+        pr2[c] = resp2[c].data();
+    }
+    amf_uint32 flags[] = {0, 0};
+    const amf_uint32 blocking = TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY;
+    if ((r = conv->UpdateResponseTD(pr1.data(), NResSamples, nullptr, blocking)) != AMF_OK)
+        return fail("UpdateResponseTD", r);
+    orc_conv_update_td(orc, pr1.data(), NResSamples, nullptr, blocking);
+
+    const int n_blocks = (int)((NSamples + n_samples - 1) / n_samples);
+    const int n_IR_updates = 10;
+    const int update_interv = n_blocks / n_IR_updates;
+    std::vector<float *> in(nch), o(nch), oo(nch);
+    int b = 0, xfades = 0, expected_xfades = 0;
+    const int full_blocks = (int)(NSamples / n_samples);
+    for (amf_uint32 ns = 0; ns < NSamples; ns += n_samples, b++) {
+        if (b % update_interv == 0) {
+            float **resp = ((b & 1) == 0) ? pr1.data() : pr2.data();
+            if ((r = conv->UpdateResponseTD(resp, NResSamples, nullptr, blocking)) != AMF_OK)
+                return fail("UpdateResponseTD", r);
+            orc_conv_update_td(orc, resp, NResSamples, nullptr, blocking);
+            if (b + 2 < full_blocks) expected_xfades++;  // crossfade two blocks later
+        }
+        const amf_size actual = (NSamples - ns >= (amf_uint32)n_samples) ? n_samples : NSamples - ns;
+        for (int c = 0; c < nch; c++) {
+            in[c] = samples[c].data() + ns;
+            o[c] = out[c].data() + ns;
+            oo[c] = oref[c].data() + ns;
+        }
+        amf_size done = 0, done_o = 0;
+        if ((r = conv->Process(in.data(), o.data(), actual, flags, &done)) != AMF_OK) return fail("Process", r);
+        orc_conv_process(orc, in.data(), oo.data(), actual, flags, &done_o);
+        if (done != done_o) return fail("processed count differs from the oracle", (int)done);
+        xfades += orc_conv_last_was_crossfade(orc);
+    }
+    double maxerr = 0.0, es = 0.0, ee = 0.0;
+    for (int c = 0; c < nch; c++)
+        for (amf_uint32 i = 0; i < NSamples; i++) {
+            const double d = (double)out[c][i] - (double)oref[c][i];
+            maxerr = std::fmax(maxerr, std::fabs(d));
+            es += (double)oref[c][i] * oref[c][i];
+            ee += d * d;
+        }
+    const double snr = ee > 0 ? 10.0 * std::log10(es / ee) : 999.0;
+    std::printf("convolution: %d blocks, %d crossfades, max|gpu-oracle| = %.3e, SNR = %.1f dB\n", b, xfades, maxerr,
+                snr);
+
+    // TANFFT through the C++ interface: R2C planar then C2R planar round trip
+    TANFFTPtr fft;
+    if ((r = TANCreateFFT(ctx, &fft)) != AMF_OK) return fail("TANCreateFFT", r);
+    if ((r = fft->Init()) != AMF_OK) return fail("TANFFT::Init", r);
+    const int log2n = 9, n = 1 << log2n;
+    std::vector<float> fin(2 * n, 0.f), fspec(2 * n, 0.f), fback(2 * n, 0.f), ospec(2 * n, 0.f);
+    for (int i = 0; i < n; i++) fin[i] = samples[0][i + 2000];
+    float *pi[] = {fin.data()}, *ps[] = {fspec.data()}, *pb[] = {fback.data()}, *pos[] = {ospec.data()};
+    if ((r = fft->Transform(TAN_FFT_R2C_PLANAR_TRANSFORM_DIRECTION_FORWARD, log2n, 1, pi, ps)) != AMF_OK)
+        return fail("Transform R2C planar", r);
+    orc_fft_transform(ORC_FFT_R2C_PLANAR_FORWARD, log2n, 1, pi, pos);
+    if ((r = fft->Transform(TAN_FFT_C2R_PLANAR_TRANSFORM_DIRECTION_BACKWARD, log2n, 1, ps, pb)) != AMF_OK)
+        return fail("Transform C2R planar", r);
+    double fe = 0.0, re = 0.0;
+    for (int k = 0; k <= n / 2; k++) {
+        fe = std::fmax(fe, std::fabs((double)fspec[k] - ospec[k]));
+        fe = std::fmax(fe, std::fabs((double)fspec[n / 2 + 8 + k] - ospec[n / 2 + 8 + k]));
+    }
+    for (int i = 0; i < n; i++) re = std::fmax(re, std::fabs((double)fback[i] - fin[i]));
+    std::printf("fft: max|gpu-oracle| spectrum = %.3e, round trip = %.3e\n", fe, re);
+
+    orc_conv_destroy(orc);
+    ctx->Terminate();  // while conv/fft are still alive, as the driver does
+    const bool ok = maxerr <= 1e-5 && snr >= 120.0 && xfades == expected_xfades && fe <= 1e-4 && re <= 1e-5;
+    std::printf("%s\n", ok ? "PASS" : "FAIL");
+    return ok ? 0 : 1;
+}
