@@ -83,6 +83,9 @@ def run_blocks(T, conv, x, y, banks, t0, t1):
 def bench_throughput(T, torch, ctx, steps, warmup, timing):
     conv = T.TANConvolution(ctx)
     assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, C) == T.AMF_OK
+    rc, chunks, us = conv.Autotune()
+    assert rc == T.AMF_OK, rc
+    log(f"  autotune: {chunks} tail chunks, {us:.2f} us per block")
     total = warmup + steps
     nbank = total // UPDATE_EVERY + 1
     banks = [ir_bank_device(torch, C, L, 1000 + k) for k in range(nbank)]

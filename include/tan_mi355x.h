@@ -100,6 +100,18 @@ int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
  * power of two as the reference does (ConvolutionImpl.cpp:1230-1235). */
 int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
                          uint32_t block, uint32_t channels, tanx_conv **out);
+/* Storage precision of the IR spectra.  FP16 and BF16 rows are scaled by
+ * their max |re|,|im| (one FP32 scale per partition row); every product and
+ * sum is FP32.  FP32 is the reference's precision and the default. */
+enum { TANX_IR_FP32 = 0, TANX_IR_FP16 = 1, TANX_IR_BF16 = 2 };
+typedef struct tanx_conv_options {
+    uint32_t ir_precision;  /* TANX_IR_* */
+    uint32_t reserved[7];   /* zero */
+} tanx_conv_options;
+/* tanx_conv_create with options (no counterpart in the reference, which
+ * stores FP32 spectra only); opt may be NULL. */
+int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t ir_len, uint32_t block,
+                            uint32_t channels, const tanx_conv_options *opt, tanx_conv **out);
 /* TANConvolution::Terminate (ConvolutionImpl.cpp:307-355). */
 int32_t tanx_conv_destroy(tanx_conv *conv);
 
@@ -152,10 +164,17 @@ typedef struct tanx_conv_info {
     uint64_t device_bytes; /* HBM held by the object */
     uint64_t ticks;        /* blocks processed */
     uint64_t crossfades;   /* crossfade blocks processed */
+    uint32_t ir_precision; /* TANX_IR_* */
+    uint32_t reserved;
 } tanx_conv_info;
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */
 int32_t tanx_conv_set_tail_chunks(tanx_conv *conv, uint32_t chunks);
+/* Autotuner: on an object that has processed no block yet, times the fused
+ * tick kernel for each power-of-two tail chunking up to the object's maximum
+ * and keeps the fastest (the dry runs leave all state unchanged).  Returns
+ * the chosen chunk count and its time per launch in microseconds. */
+int32_t tanx_conv_autotune(tanx_conv *conv, uint32_t *chunks, double *us_per_block);
 /* Device-memory Process runs each block's head and the next block's tail
  * multiply-accumulate in one launch (default 1); 0 launches them separately
  * as the host-memory path does.  Results are identical either way. */

@@ -22,7 +22,12 @@ class ConvInfo(ctypes.Structure):
                 ("channels", ctypes.c_uint32), ("partitions", ctypes.c_uint32),
                 ("tail_chunks", ctypes.c_uint32), ("finalize_mode", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64),
-                ("crossfades", ctypes.c_uint64)]
+                ("crossfades", ctypes.c_uint64), ("ir_precision", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class ConvOptions(ctypes.Structure):
+    _fields_ = [("ir_precision", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
 
 
 class KernelTiming(ctypes.Structure):
@@ -36,6 +41,7 @@ _SIGS = {
     "tanx_ctx_destroy": (_i32, [_vp]),
     "tanx_ctx_device": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "tanx_conv_create": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "tanx_conv_create_ex": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "tanx_conv_destroy": (_i32, [_vp]),
     "tanx_conv_update_td": (_i32, [_vp, ctypes.POINTER(_f32p), _sz, _u32p, _u32]),
     "tanx_conv_update_td_device": (_i32, [_vp, _vp, _sz, _sz, _u32p, _u32]),
@@ -51,6 +57,7 @@ _SIGS = {
     "tanx_conv_last_was_crossfade": (_i32, [_vp]),
     "tanx_conv_enable_timing": (_i32, [_vp, _u32]),
     "tanx_conv_set_fused": (_i32, [_vp, _u32]),
+    "tanx_conv_autotune": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_double)]),
     "tanx_conv_read_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_conv_process_stream": (_vp, [_vp]),
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),

@@ -199,7 +199,9 @@ struct tanx_conv {
     // device memory (one allocation carved into pieces)
     void *dmem = nullptr;
     size_t dbytes = 0;
-    float2 *H = nullptr;     // [3][C][P][B]
+    char *H = nullptr;       // [3][C][P][B] IR spectra of irtype
+    float *S = nullptr;      // [3][C][P] row scales (reduced-precision spectra)
+    int irtype = IR_F32;
     float2 *F = nullptr;     // [C][P][B]
     float2 *T = nullptr;     // [2 parity][2 filter][C][maxchunk][B] tail partial sums
     float *xprev = nullptr;  // [C][B]
@@ -236,8 +238,10 @@ struct tanx_conv {
     KernelTimer t_head, t_tail;
     uint64_t tail_alg_bytes = 0, head_alg_bytes = 0;
 
-    size_t slot_elems() const { return (size_t)C * P * B; }
-    float2 *slot_ptr(int s) const { return H + (size_t)s * slot_elems(); }
+    size_t row_bytes() const { return (size_t)B * ir_elem_bytes(irtype); }
+    size_t slot_bytes() const { return (size_t)C * P * row_bytes(); }
+    void *slot_ptr(int s) const { return H + (size_t)s * slot_bytes(); }
+    float *scale_ptr(int s) const { return S + (size_t)s * C * P; }
     // partial sums of buffer `par` for filter f (0 = current, 1 = next)
     float2 *tail_ptr(int par, int f) const { return T + ((size_t)par * 2 + f) * C * maxchunk * B; }
 };
@@ -287,8 +291,16 @@ static void set_chunks(tanx_conv *cv, int n)
 extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
                                     uint32_t block, uint32_t channels, tanx_conv **out)
 {
+    return tanx_conv_create_ex(ctx, method, ir_len, block, channels, nullptr, out);
+}
+
+extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t ir_len, uint32_t block,
+                                       uint32_t channels, const tanx_conv_options *opt, tanx_conv **out)
+{
     if (!ctx || !out) return TANX_INVALID_POINTER;
     *out = nullptr;
+    const uint32_t prec = opt ? opt->ir_precision : TANX_IR_FP32;
+    if (prec > TANX_IR_BF16) return TANX_INVALID_ARG;
     if (ir_len == 0 || block == 0 || channels == 0) return TANX_INVALID_ARG;
     if (!is_pow2(block) || block < 64 || block > 1024) return TANX_INVALID_ARG;
     const uint32_t base = method & ~uint32_t(TANX_METHOD_USE_PROCESS_TAILTHREAD);
@@ -315,6 +327,7 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
     cv->ctx = ctx;
     cv->method = method;
     cv->fmode = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
+    cv->irtype = prec == TANX_IR_FP16 ? IR_F16 : prec == TANX_IR_BF16 ? IR_BF16 : IR_F32;
     cv->L = (int)ir_len;
     cv->B = (int)block;
     cv->C = (int)channels;
@@ -327,19 +340,21 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
 
     const size_t C = cv->C, P = cv->P, B = cv->B;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t bH = al(3 * C * P * B * sizeof(float2));
+    const size_t bH = al(3 * C * P * B * (size_t)ir_elem_bytes(cv->irtype));
+    const size_t bS = al(3 * C * P * sizeof(float));
     const size_t bF = al(C * P * B * sizeof(float2));
     const size_t bT = al(4 * C * (size_t)cv->maxchunk * B * sizeof(float2));
     const size_t bX = al(C * B * sizeof(float));
     const size_t bCh = al(C * sizeof(int));
     const size_t bRun = al(C);
-    cv->dbytes = bH + bF + bT + 3 * bX + bCh + bRun;
+    cv->dbytes = bH + bS + bF + bT + 3 * bX + bCh + bRun;
     int32_t rc = TANX_OK;
     auto fail = [&](int32_t r) { conv_free(cv); return r; };
     hipError_t e = hipMalloc(&cv->dmem, cv->dbytes);
     if (e != hipSuccess) { cv->dmem = nullptr; return fail(map_hip(e)); }
     char *p = static_cast<char *>(cv->dmem);
-    cv->H = reinterpret_cast<float2 *>(p); p += bH;
+    cv->H = p; p += bH;
+    cv->S = reinterpret_cast<float *>(p); p += bS;
     cv->F = reinterpret_cast<float2 *>(p); p += bF;
     cv->T = reinterpret_cast<float2 *>(p); p += bT;
     cv->xprev = reinterpret_cast<float *>(p); p += bX;
@@ -362,7 +377,7 @@ extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_
 
     // all state starts at zero: the initial IR slot is silent
     // (ConvolutionImpl.cpp:115-117, 1640-1650)
-    e = launch_zero(cv->dmem, bH + bF + bT + 3 * bX, ctx->proc);
+    e = launch_zero(cv->dmem, bH + bS + bF + bT + 3 * bX, ctx->proc);
     if (e == hipSuccess) {
         std::memset(cv->hrun, 1, C);
         e = hipMemcpyAsync(cv->drun, cv->hrun, C, hipMemcpyHostToDevice, ctx->proc);
@@ -471,6 +486,7 @@ static TickArgs tick_common(tanx_conv *cv)
     TickArgs a{};
     a.C = cv->C;
     a.P = cv->P;
+    a.irtype = cv->irtype;
     a.run = cv->drun;
     const int lB = __builtin_ctz((unsigned)cv->B);
     a.twM = tw_table(cv->ctx->tw, lB);
@@ -489,6 +505,8 @@ static int fill_tail(tanx_conv *cv, TickArgs &a)
     a.tbase = cv->pos;  // pos already advanced past the current block
     a.tHa = cv->slot_ptr(cv->cur);
     a.tHb = nf == 2 ? cv->slot_ptr(cv->upd) : nullptr;
+    a.tSa = cv->scale_ptr(cv->cur);
+    a.tSb = nf == 2 ? cv->scale_ptr(cv->upd) : nullptr;
     a.tTa = cv->tail_ptr(cv->tread ^ 1, 0);
     a.tTb = nf == 2 ? cv->tail_ptr(cv->tread ^ 1, 1) : nullptr;
     return nf;
@@ -531,13 +549,14 @@ static uint64_t head_bytes(const tanx_conv *cv, int nf)
 {
     // x_{t-1}, x_t, y, the new FDL row, the previous FDL row, and per filter
     // partitions 0 and 1
-    return (uint64_t)cv->C * cv->B * (3 * sizeof(float) + 2 * sizeof(float2) + 2 * sizeof(float2) * nf);
+    return (uint64_t)cv->C * cv->B * (3 * sizeof(float) + 2 * sizeof(float2) + 2 * ir_elem_bytes(cv->irtype) * nf);
 }
 static uint64_t tail_bytes(const tanx_conv *cv, int nf)
 {
-    // FDL rows and nf IR rows of partitions 2..P-1
+    // FDL rows and nf IR rows (plus their scales) of partitions 2..P-1
     const uint64_t parts = cv->P > 2 ? (uint64_t)(cv->P - 2) : 0;
-    return (uint64_t)cv->C * parts * cv->B * sizeof(float2) * (1 + nf);
+    const uint64_t scales = cv->irtype == IR_F32 ? 0 : sizeof(float) * nf;
+    return (uint64_t)cv->C * parts * (cv->B * (sizeof(float2) + (uint64_t)ir_elem_bytes(cv->irtype) * nf) + scales);
 }
 
 // ProcessFinalize: the tail for the next block as its own launch.
@@ -589,6 +608,8 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
     a.slot = cv->pos;
     a.Ha = cv->slot_ptr(xfade ? cv->prev : cv->cur);
     a.Hb = xfade ? cv->slot_ptr(cv->cur) : nullptr;
+    a.Sa = cv->scale_ptr(xfade ? cv->prev : cv->cur);
+    a.Sb = xfade ? cv->scale_ptr(cv->cur) : nullptr;
     a.Ta = cv->tail_ptr(cv->tread, 0);
     a.Tb = xfade ? cv->tail_ptr(cv->tread, 1) : cv->tail_ptr(cv->tread, 0);
     a.tchunk = cv->tail_nchunk;
@@ -730,7 +751,7 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
     // the update slot must not be overwritten while queued process work reads it
     TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
     TANX_CHECK(hipStreamWaitEvent(ctx->upd, cv->ev_proc, 0));
-    const size_t slotElems = (size_t)cv->P * cv->B;
+    const size_t chanBytes = (size_t)cv->P * cv->row_bytes();
     if (nupd > 0) {
         const float *src = dir;
         long long stride = (long long)dstride;
@@ -765,6 +786,8 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         a.n = (long long)n;
         a.chan = cv->dchan;
         a.H = cv->slot_ptr(target);
+        a.S = cv->scale_ptr(target);
+        a.irtype = cv->irtype;
         a.P = cv->P;
         const int lB = __builtin_ctz((unsigned)cv->B);
         a.twM = tw_table(ctx->tw, lB);
@@ -775,9 +798,11 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
     // channels not updated keep the IR that is current when the update is
     // handed off (ConvolutionImpl.cpp:2149-2158)
     for (int ch : keep_list) {
-        TANX_CHECK(hipMemcpyAsync(cv->slot_ptr(target) + (size_t)ch * slotElems,
-                                  cv->slot_ptr(keep) + (size_t)ch * slotElems, slotElems * sizeof(float2),
+        TANX_CHECK(hipMemcpyAsync(static_cast<char *>(cv->slot_ptr(target)) + (size_t)ch * chanBytes,
+                                  static_cast<char *>(cv->slot_ptr(keep)) + (size_t)ch * chanBytes, chanBytes,
                                   hipMemcpyDeviceToDevice, ctx->upd));
+        TANX_CHECK(hipMemcpyAsync(cv->scale_ptr(target) + (size_t)ch * cv->P, cv->scale_ptr(keep) + (size_t)ch * cv->P,
+                                  (size_t)cv->P * sizeof(float), hipMemcpyDeviceToDevice, ctx->upd));
     }
     TANX_CHECK(hipEventRecord(cv->ev_upd, ctx->upd));
     cv->pending = true;
@@ -848,6 +873,7 @@ extern "C" int32_t tanx_conv_get_info(const tanx_conv *cv, tanx_conv_info *info)
     info->partitions = (uint32_t)cv->P;
     info->tail_chunks = (uint32_t)cv->nchunk;
     info->finalize_mode = cv->fmode ? 1u : 0u;
+    info->ir_precision = cv->irtype == IR_F16 ? TANX_IR_FP16 : cv->irtype == IR_BF16 ? TANX_IR_BF16 : TANX_IR_FP32;
     info->device_bytes = cv->dbytes;
     info->ticks = cv->ticks;
     info->crossfades = cv->xfades;
@@ -897,6 +923,76 @@ extern "C" int32_t tanx_conv_read_timing(tanx_conv *cv, tanx_kernel_timing *out,
         cv->head_alg_bytes = cv->tail_alg_bytes = 0;
     }
     return TANX_OK;
+}
+
+// Picks the tail chunking by timing dry-run fused launches.  Only on a fresh
+// object (no block processed yet): the FDL, previous block and partial sums
+// are all zero, the dry run feeds a zero block, and so every buffer it writes
+// keeps its zero contents.
+extern "C" int32_t tanx_conv_autotune(tanx_conv *cv, uint32_t *chosen, double *best_us)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    if (cv->ticks != 0) return TANX_WRONG_STATE;
+    DeviceGuard g(cv->ctx->device);
+    hipStream_t s = cv->ctx->proc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TANX_CHECK(hipEventCreate(&e0));
+    TANX_CHECK(hipEventCreate(&e1));
+    const int keep = cv->nchunk;
+    double best = 1e30;
+    int best_n = keep;
+    int32_t rc = TANX_OK;
+    // zero input block (a host Process that failed may have staged data in din)
+    rc = map_hip(launch_zero(cv->din, (size_t)cv->C * cv->B * sizeof(float), s));
+    for (int n = 1; n <= cv->maxchunk && rc == TANX_OK; n *= 2) {
+        set_chunks(cv, n);
+        if (cv->nchunk != n) continue;
+        TickArgs a = tick_common(cv);
+        a.nhead = cv->C;
+        a.xin = cv->din;
+        a.in_stride = cv->B;
+        a.yout = cv->dout;
+        a.out_stride = cv->B;
+        a.xprev = cv->xprev;
+        a.slot = 0;
+        a.Ha = a.Hb = cv->slot_ptr(cv->cur);
+        a.Sa = a.Sb = cv->scale_ptr(cv->cur);
+        a.Ta = a.Tb = cv->tail_ptr(0, 0);
+        a.tchunk = cv->nchunk;
+        a.xfade = 0;
+        a.nchunk = cv->nchunk;
+        a.pc = cv->pc;
+        a.tbase = 1 % cv->P;
+        a.tHa = cv->slot_ptr(cv->cur);
+        a.tHb = nullptr;
+        a.tSa = cv->scale_ptr(cv->cur);
+        a.tSb = nullptr;
+        a.tTa = cv->tail_ptr(1, 0);
+        a.tTb = nullptr;
+        const int reps = cv->C >= 4096 ? 3 : 12;
+        for (int w = 0; w < 2 && rc == TANX_OK; w++) rc = map_hip(launch_tick(cv->B, 1, a, s));
+        if (rc != TANX_OK) break;
+        if ((rc = map_hip(hipEventRecord(e0, s))) != TANX_OK) break;
+        for (int r = 0; r < reps && rc == TANX_OK; r++) rc = map_hip(launch_tick(cv->B, 1, a, s));
+        if (rc != TANX_OK) break;
+        if ((rc = map_hip(hipEventRecord(e1, s))) != TANX_OK) break;
+        if ((rc = map_hip(hipEventSynchronize(e1))) != TANX_OK) break;
+        float ms = 0.f;
+        if ((rc = map_hip(hipEventElapsedTime(&ms, e0, e1))) != TANX_OK) break;
+        const double us = 1e3 * ms / reps;
+        if (us < best) {
+            best = us;
+            best_n = n;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_chunks(cv, rc == TANX_OK ? best_n : keep);
+    cv->tail_nchunk = cv->nchunk;
+    if (chosen) *chosen = (uint32_t)cv->nchunk;
+    if (best_us) *best_us = best;
+    return rc;
 }
 
 extern "C" int32_t tanx_conv_set_fused(tanx_conv *cv, uint32_t fused)

@@ -19,9 +19,16 @@ namespace tanx {
 //   tail (blocks [nhead, nhead + C * nchunk)): partial sums over partitions
 //     [2 + j*pc, 2 + (j+1)*pc) of X_{t+1-i} H_i for the NEXT block.
 // The two roles touch disjoint data, so they may share a launch.
+// IR spectrum storage: FP32 complex, or FP16 / BF16 complex scaled per
+// partition row (value = stored * S[c * P + i], S = the row's max |re|,|im|),
+// always accumulated in FP32.
+enum IrType { IR_F32 = 0, IR_F16 = 1, IR_BF16 = 2 };
+inline int ir_elem_bytes(int t) { return t == IR_F32 ? 8 : 4; }
+
 struct TickArgs {
     int C;
     int P;
+    int irtype;            // IrType of Ha/Hb/tHa/tHb
     const uint8_t *run;    // [C] 1 = channel running
     const float2 *twM;     // W_B^t, t < B
     const float2 *twN;     // W_2B^t, t < 2B
@@ -35,8 +42,10 @@ struct TickArgs {
     long long out_stride;
     float *xprev;          // [C][B] previous input block (overlap-save frame)
     int slot;              // ring slot receiving X_t
-    const float2 *Ha;      // filter slot bases [C][P][B]
-    const float2 *Hb;      // crossfade: new filter, else nullptr
+    const void *Ha;        // filter slot bases [C][P][B] of IrType
+    const void *Hb;        // crossfade: new filter, else nullptr
+    const float *Sa;       // row scales [C][P] (reduced precision)
+    const float *Sb;
     const float2 *Ta;      // partial sums read by the head [C][tchunk][B]
     const float2 *Tb;
     int tchunk;
@@ -46,8 +55,10 @@ struct TickArgs {
     int nchunk;            // chunks per channel (0 = no tail blocks)
     int pc;                // partitions per chunk
     int tbase;             // ring slot of block t+1
-    const float2 *tHa;
-    const float2 *tHb;     // second filter (next block crossfades) or nullptr
+    const void *tHa;
+    const void *tHb;       // second filter (next block crossfades) or nullptr
+    const float *tSa;
+    const float *tSb;
     float2 *tTa;           // partial sums written [C][nchunk][B]
     float2 *tTb;
 };
@@ -57,7 +68,9 @@ struct IrArgs {
     long long stride;
     long long n;
     const int *chan;       // list of channels to transform (blockIdx.y indexes it)
-    float2 *H;             // destination slot base [C][P][B]
+    void *H;               // destination slot base [C][P][B] of IrType
+    float *S;              // destination row scales [C][P]
+    int irtype;
     int P;
     const float2 *twM;
     const float2 *twN;

@@ -16,6 +16,8 @@
 // FP64 on the host and rounded to FP32.
 #include "tan_internal.h"
 
+#include <hip/hip_fp16.h>
+
 namespace tanx {
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
@@ -128,6 +130,74 @@ __device__ __forceinline__ float4 ld_stream(const float4 *p)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float half_lo(unsigned int w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
+__device__ __forceinline__ float half_hi(unsigned int w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
+__device__ __forceinline__ float bf_lo(unsigned int w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Access to one IR spectrum row (B complex) of a storage type.
+template <int HT>
+struct IrRow;
+
+template <>
+struct IrRow<IR_F32> {
+    static constexpr int kBytes = 8;
+    __device__ static float2 get1(const void *row, int k) { return reinterpret_cast<const float2 *>(row)[k]; }
+    // complex 2q and 2q+1, streamed
+    __device__ static float4 get2(const void *row, int q) { return ld_stream(reinterpret_cast<const float4 *>(row) + q); }
+};
+
+template <>
+struct IrRow<IR_F16> {
+    static constexpr int kBytes = 4;
+    __device__ static float2 get1(const void *row, int k)
+    {
+        const unsigned int w = reinterpret_cast<const unsigned int *>(row)[k];
+        return make_float2(half_lo(w), half_hi(w));
+    }
+    __device__ static float4 get2(const void *row, int q)
+    {
+        const v2u w = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(row) + q);
+        return make_float4(half_lo(w.x), half_hi(w.x), half_lo(w.y), half_hi(w.y));
+    }
+};
+
+template <>
+struct IrRow<IR_BF16> {
+    static constexpr int kBytes = 4;
+    __device__ static float2 get1(const void *row, int k)
+    {
+        const unsigned int w = reinterpret_cast<const unsigned int *>(row)[k];
+        return make_float2(bf_lo(w), bf_hi(w));
+    }
+    __device__ static float4 get2(const void *row, int q)
+    {
+        const v2u w = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(row) + q);
+        return make_float4(bf_lo(w.x), bf_hi(w.x), bf_lo(w.y), bf_hi(w.y));
+    }
+};
+
+// base address of row (c, i) of a slot
+template <int HT>
+__device__ __forceinline__ const void *ir_row(const void *base, int c, int P, int i, int M)
+{
+    return reinterpret_cast<const char *>(base) + ((size_t)c * P + i) * (size_t)M * IrRow<HT>::kBytes;
+}
+
+template <int HT>
+__device__ __forceinline__ float ir_scale(const float *S, int c, int P, int i)
+{
+    if constexpr (HT == IR_F32) {
+        return 1.0f;
+    } else {
+        return S[(size_t)c * P + i];
+    }
+}
+
+__device__ __forceinline__ float4 scale4(float4 v, float s) { return make_float4(v.x * s, v.y * s, v.z * s, v.w * s); }
+
 __device__ __forceinline__ void mac2(float2 &acc, float2 x, float2 h, bool dc)
 {
     // packed DC/Nyquist element: two real products; otherwise complex
@@ -161,7 +231,7 @@ struct HeadSmem {
     float ya[M];
 };
 
-template <int M>
+template <int M, int HT>
 __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
 {
     constexpr int NT = kTickThreads;
@@ -185,7 +255,12 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
 #pragma unroll
     for (int f = 0; f < 2; f++) {
         if (f >= nf) continue;
-        const float2 *H = (f == 0 ? a.Ha : a.Hb) + (size_t)c * a.P * M;
+        const void *Hbase = f == 0 ? a.Ha : a.Hb;
+        const float *Sbase = f == 0 ? a.Sa : a.Sb;
+        const void *H0row = ir_row<HT>(Hbase, c, a.P, 0, M);
+        const void *H1row = a.P > 1 ? ir_row<HT>(Hbase, c, a.P, 1, M) : H0row;
+        const float s0 = ir_scale<HT>(Sbase, c, a.P, 0);
+        const float s1 = a.P > 1 ? ir_scale<HT>(Sbase, c, a.P, 1) : 0.f;
         const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.tchunk * M;
 #pragma unroll
         for (int q = 0; q < KPT; q++) {
@@ -205,8 +280,12 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                         s.y += t[u].y;
                     }
                 }
-                if (a.P > 1) mac2(s, Fprev[k], H[M + k], k == 0);
-                h0[f][q] = H[k];
+                if (a.P > 1) {
+                    const float2 h1 = IrRow<HT>::get1(H1row, k);
+                    mac2(s, Fprev[k], make_float2(h1.x * s1, h1.y * s1), k == 0);
+                }
+                const float2 h0v = IrRow<HT>::get1(H0row, k);
+                h0[f][q] = make_float2(h0v.x * s0, h0v.y * s0);
             }
             acc[f][q] = s;
         }
@@ -267,7 +346,7 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
 
 // Partial sums over partitions [i0, i1) for one channel; 256 threads; a row
 // of B complex is read as float4 (two complex) per lane, G rows in flight.
-template <int M, int NF>
+template <int M, int NF, int HT>
 __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
 {
     constexpr int H = M / 2;                     // float4 per row
@@ -282,8 +361,12 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
     const int mask = a.P - 1;
 
     const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
-    const float4 *HA = reinterpret_cast<const float4 *>(a.tHa) + (size_t)c * a.P * H;
-    const float4 *HB = NF == 2 ? reinterpret_cast<const float4 *>(a.tHb) + (size_t)c * a.P * H : nullptr;
+    const void *HA = ir_row<HT>(a.tHa, c, a.P, 0, M);
+    const void *HB = NF == 2 ? ir_row<HT>(a.tHb, c, a.P, 0, M) : nullptr;
+    constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
+    auto hrow = [&](const void *base, int i) {
+        return reinterpret_cast<const void *>(reinterpret_cast<const char *>(base) + (size_t)i * kRowBytes);
+    };
 
     float4 accA[V], accB[V];
 #pragma unroll
@@ -297,15 +380,18 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
     int i = i0 + g;
     for (; i + (U - 1) * G < i1; i += U * G) {
         float4 xf[U][V], ha[U][V], hb[U][V];
+        float sa[U], sb[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int ii = i + u * G;
             const int s = (a.tbase - ii) & mask;
+            sa[u] = ir_scale<HT>(a.tSa, c, a.P, ii);
+            if (NF == 2) sb[u] = ir_scale<HT>(a.tSb, c, a.P, ii);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 xf[u][v] = ld_stream(F + (size_t)s * H + v * R + r);
-                ha[u][v] = ld_stream(HA + (size_t)ii * H + v * R + r);
-                if (NF == 2) hb[u][v] = ld_stream(HB + (size_t)ii * H + v * R + r);
+                ha[u][v] = IrRow<HT>::get2(hrow(HA, ii), v * R + r);
+                if (NF == 2) hb[u][v] = IrRow<HT>::get2(hrow(HB, ii), v * R + r);
             }
         }
 #pragma unroll
@@ -314,20 +400,26 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
             for (int v = 0; v < V; v++) {
                 const bool dc = dc0 && v == 0;
                 const float s = v == 0 ? s0 : 1.f;
-                mac4(accA[v], xf[u][v], ha[u][v], s, dc);
-                if (NF == 2) mac4(accB[v], xf[u][v], hb[u][v], s, dc);
+                mac4(accA[v], xf[u][v], HT == IR_F32 ? ha[u][v] : scale4(ha[u][v], sa[u]), s, dc);
+                if (NF == 2) mac4(accB[v], xf[u][v], HT == IR_F32 ? hb[u][v] : scale4(hb[u][v], sb[u]), s, dc);
             }
         }
     }
     for (; i < i1; i += G) {
         const int s = (a.tbase - i) & mask;
+        const float sai = ir_scale<HT>(a.tSa, c, a.P, i);
+        const float sbi = NF == 2 ? ir_scale<HT>(a.tSb, c, a.P, i) : 0.f;
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const bool dc = dc0 && v == 0;
             const float sv = v == 0 ? s0 : 1.f;
             const float4 x = ld_stream(F + (size_t)s * H + v * R + r);
-            mac4(accA[v], x, ld_stream(HA + (size_t)i * H + v * R + r), sv, dc);
-            if (NF == 2) mac4(accB[v], x, ld_stream(HB + (size_t)i * H + v * R + r), sv, dc);
+            const float4 h1 = IrRow<HT>::get2(hrow(HA, i), v * R + r);
+            mac4(accA[v], x, HT == IR_F32 ? h1 : scale4(h1, sai), sv, dc);
+            if (NF == 2) {
+                const float4 h2 = IrRow<HT>::get2(hrow(HB, i), v * R + r);
+                mac4(accB[v], x, HT == IR_F32 ? h2 : scale4(h2, sbi), sv, dc);
+            }
         }
     }
 
@@ -360,7 +452,7 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
     }
 }
 
-template <int M, int NF>
+template <int M, int NF, int HT>
 __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
 {
     constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
@@ -370,12 +462,12 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
     const int b = blockIdx.x;
     if (b < a.nhead) {
         if (!a.run[b]) return;
-        head_role<M>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
+        head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
     } else {
         const int t = b - a.nhead;
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        tail_role<M, NF>(a, c, j, smem);
+        tail_role<M, NF, HT>(a, c, j, smem);
     }
 }
 
@@ -383,10 +475,17 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
 // IR partition transform: one workgroup per (partition i, listed channel).
 // Partition i of channel ch becomes the packed spectrum of [h[iB..iB+B), 0_B].
 // ---------------------------------------------------------------------------
-template <int M>
+__device__ __forceinline__ unsigned short f2bf_rne(float f)
+{
+    const unsigned int u = __float_as_uint(f);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+template <int M, int HT>
 __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs a)
 {
     __shared__ float2 s0[M], s1[M];
+    __shared__ float smax[32];
     const int i = blockIdx.x, ch = a.chan[blockIdx.y], tid = threadIdx.x, nt = blockDim.x;
     const float *h = a.ir + (size_t)ch * a.stride;
     const long long base = (long long)i * M;
@@ -401,8 +500,40 @@ __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs
     }
     __syncthreads();
     const float2 *Z = fft_lds<M, false>(s0, s1, a.twM);
-    float2 *row = a.H + ((size_t)ch * a.P + i) * M;
-    for (int k = tid; k < M; k += nt) row[k] = r2c_post<M>(Z, k, a.twN);
+    if constexpr (HT == IR_F32) {
+        float2 *row = reinterpret_cast<float2 *>(a.H) + ((size_t)ch * a.P + i) * M;
+        for (int k = tid; k < M; k += nt) row[k] = r2c_post<M>(Z, k, a.twN);
+    } else {
+        // spectrum into the free buffer, then the row's max |re|,|im|
+        float2 *X = (Z == s0) ? s1 : s0;
+        float m = 0.f;
+        for (int k = tid; k < M; k += nt) {
+            const float2 v = r2c_post<M>(Z, k, a.twN);
+            X[k] = v;
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((tid & 63) == 0) smax[tid >> 6] = m;
+        __syncthreads();
+        float sc = 0.f;
+        for (int w = 0; w < (nt + 63) / 64; w++) sc = fmaxf(sc, smax[w]);
+        if (sc == 0.f) sc = 1.f;
+        const float inv = 1.0f / sc;
+        unsigned int *row = reinterpret_cast<unsigned int *>(a.H) + ((size_t)ch * a.P + i) * M;
+        for (int k = tid; k < M; k += nt) {
+            const float2 v = X[k];
+            unsigned int lo, hi;
+            if constexpr (HT == IR_F16) {
+                lo = __half_as_ushort(__float2half(v.x * inv));
+                hi = __half_as_ushort(__float2half(v.y * inv));
+            } else {
+                lo = f2bf_rne(v.x * inv);
+                hi = f2bf_rne(v.y * inv);
+            }
+            row[k] = lo | (hi << 16);
+        }
+        if (tid == 0) a.S[(size_t)ch * a.P + i] = sc;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -488,21 +619,44 @@ static inline int fft_threads(int M)
     default: return hipErrorInvalidValue;               \
     }
 
-hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s)
+template <int HT>
+static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream_t s, unsigned blocks)
 {
-    const unsigned blocks = (unsigned)(a.nhead + a.C * a.nchunk);
-    if (blocks == 0) return hipSuccess;
     if (tail_nf == 2) {
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 2>), dim3(blocks), dim3(kTickThreads), 0, s, a));
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 2, HT>), dim3(blocks), dim3(kTickThreads), 0, s, a));
     } else {
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 1>), dim3(blocks), dim3(kTickThreads), 0, s, a));
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 1, HT>), dim3(blocks), dim3(kTickThreads), 0, s, a));
     }
     return hipGetLastError();
 }
 
+hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)(a.nhead + a.C * a.nchunk);
+    if (blocks == 0) return hipSuccess;
+    switch (a.irtype) {
+    case IR_F32: return launch_tick_t<IR_F32>(B, tail_nf, a, s, blocks);
+    case IR_F16: return launch_tick_t<IR_F16>(B, tail_nf, a, s, blocks);
+    case IR_BF16: return launch_tick_t<IR_BF16>(B, tail_nf, a, s, blocks);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
 {
-    TANX_SWITCH_B(B, hipLaunchKernelGGL(k_ir_transform<M>, dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+    switch (a.irtype) {
+    case IR_F32:
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F32>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        break;
+    case IR_F16:
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        break;
+    case IR_BF16:
+        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_ir_transform<M, IR_BF16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

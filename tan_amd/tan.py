@@ -57,6 +57,11 @@ TAN_FFT_C2R_TRANSFORM_DIRECTION_BACKWARD = 3
 TAN_FFT_R2C_PLANAR_TRANSFORM_DIRECTION_FORWARD = 4
 TAN_FFT_C2R_PLANAR_TRANSFORM_DIRECTION_BACKWARD = 5
 
+# IR spectrum storage precision (an option of this library)
+TAN_IR_PRECISION_FP32 = 0
+TAN_IR_PRECISION_FP16 = 1
+TAN_IR_PRECISION_BF16 = 2
+
 TAN_VERSION_MAJOR, TAN_VERSION_MINOR, TAN_VERSION_RELEASE, TAN_VERSION_BUILD = 1, 3, 1, 1
 TAN_FULL_VERSION = ((TAN_VERSION_MAJOR << 48) | (TAN_VERSION_MINOR << 32)
                     | (TAN_VERSION_RELEASE << 16) | TAN_VERSION_BUILD)
@@ -137,12 +142,13 @@ class TANContext:
 class TANConvolution:
     """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
 
-    def __init__(self, context):
+    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32):
         self._ctx = context
         self._lib = context._lib
         self.handle = None
         self.channels = 0
         self.block = 0
+        self.ir_precision = ir_precision
 
     # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
     def Init(self, method, response_length, buffer_size, channels):
@@ -152,8 +158,10 @@ class TANConvolution:
         if self.handle:
             return AMF_ALREADY_INITIALIZED
         h = ctypes.c_void_p()
-        rc = self._lib.tanx_conv_create(self._ctx.handle, method, response_length, buffer_size,
-                                        channels, ctypes.byref(h))
+        opt = _lib.ConvOptions()
+        opt.ir_precision = self.ir_precision
+        rc = self._lib.tanx_conv_create_ex(self._ctx.handle, method, response_length, buffer_size,
+                                           channels, ctypes.addressof(opt), ctypes.byref(h))
         if rc == AMF_OK:
             self.handle = h
             self.channels = channels
@@ -251,6 +259,13 @@ class TANConvolution:
 
     def SetTailChunks(self, chunks):
         return self._lib.tanx_conv_set_tail_chunks(self.handle, chunks)
+
+    def Autotune(self):
+        """Returns (AMF_RESULT, chosen tail chunks, microseconds per block)."""
+        n = ctypes.c_uint32(0)
+        us = ctypes.c_double(0.0)
+        rc = self._lib.tanx_conv_autotune(self.handle, ctypes.byref(n), ctypes.byref(us))
+        return rc, n.value, us.value
 
     def SetFused(self, fused=True):
         return self._lib.tanx_conv_set_fused(self.handle, int(bool(fused)))

@@ -300,3 +300,26 @@ def test_multiblock_device_process_with_updates(gpu_ctx):
         _, o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
         yo[:, b * B:(b + 1) * B] = o
     assert_parity(outs[0], yo, "multiblock")
+
+
+def test_autotune_keeps_state_and_parity(gpu_ctx):
+    """The autotuner's dry runs leave a fresh object unchanged; it refuses
+    once blocks have been processed."""
+    L, B, C, nb = 65536, 256, 8, 12
+    conv = T.TANConvolution(gpu_ctx)
+    conv.InitGpu(UNIFORM, L, B, C)
+    ir = _irs(C, L, 0)
+    conv.UpdateResponseTD(ir, L, None, BLOCKING)
+    rc, chunks, us = conv.Autotune()
+    assert rc == T.AMF_OK and chunks >= 1 and us > 0
+    orc = O.OracleConvolution(L, B, C)
+    orc.update_td(ir)
+    x = T.signals.input_bank(C, nb * B)
+    for b in range(nb):
+        blk = np.ascontiguousarray(x[:, b * B:(b + 1) * B])
+        out = np.zeros((C, B), np.float32)
+        conv.Process(blk, out)
+        _, oo, _ = orc.process(blk)
+        assert np.abs(out - oo).max() <= MAX_ABS
+    assert conv.Autotune()[0] == T.AMF_WRONG_STATE
+    conv.Terminate()
