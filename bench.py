@@ -100,9 +100,11 @@ def bench_throughput(T, torch, ctx, steps, warmup, timing):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     run_blocks(T, conv, x, y, banks, warmup, total)
+    t_enq = time.perf_counter() - t0
     conv.Synchronize()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    log(f"  host enqueue {t_enq * 1e3:.1f} ms of {dt * 1e3:.1f} ms")
     kt = conv.ReadTiming() if timing else None
     info = conv.Info()
     ok = bool(torch.isfinite(y[:, warmup * B:]).all().item())
@@ -189,6 +191,61 @@ def bench_sweep(T, torch, ctx, counts, ticks=96):
     return best, rows
 
 
+def bench_precision(T, torch, ctx, channels=1024, taps=131072, warm=8, blocks=32):
+    """Reduced-precision IR spectra on the precision workload (1,024 channels,
+    B=256, 131,072 taps): block time, algorithmic GB/s, real-time channels
+    and SNR of channel 0 against FP64 and against the FP32 path."""
+    precs = [("fp32", T.TAN_IR_PRECISION_FP32), ("fp16", T.TAN_IR_PRECISION_FP16),
+             ("bf16", T.TAN_IR_PRECISION_BF16)]
+    h = ir_bank_device(torch, channels, taps, 4242)
+    x = noise_device(torch, channels, (warm + blocks) * B, 4243)
+    n = (warm + blocks) * B
+    ref = None
+    rows = []
+    outs = {}
+    for name, prec in precs:
+        conv = T.TANConvolution(ctx, ir_precision=prec)
+        assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, taps, B, channels) == T.AMF_OK
+        conv.Autotune()
+        torch.cuda.synchronize()
+        assert conv.UpdateResponseTD(h, taps, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY) == 0
+        y = torch.zeros_like(x)
+        conv.Process(x, y, warm * B)
+        conv.Synchronize()
+        conv.EnableTiming(True)
+        conv.ReadTiming(reset=True)
+        conv.Process(x[:, warm * B:], y[:, warm * B:], blocks * B)
+        kt = conv.ReadTiming()
+        conv.Terminate()
+        tick_us = kt["head_ms"] / kt["head_launches"] * 1e3
+        gbps = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9
+        outs[name] = y[:2].cpu().numpy().astype(np.float64)
+        rows.append({"precision": name, "tick_us": tick_us, "alg_GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
+                     "rt_channels": channels * DEADLINE_MS * 1e3 / tick_us})
+        del y
+    hx = x[:2].cpu().numpy().astype(np.float64)
+    hh = h[:2].cpu().numpy().astype(np.float64)
+    nfft = 1 << (n + taps - 1).bit_length()
+    for row in rows:
+        s64, s32 = [], []
+        for c in range(2):
+            full = np.fft.irfft(np.fft.rfft(hx[c], nfft) * np.fft.rfft(hh[c], nfft), nfft)[:n]
+            # the IR fades in over block 2 (two blocks after the update)
+            w = np.concatenate([np.zeros(2 * B), np.arange(B) / B, np.ones(n - 3 * B)])
+            ref = full * w
+            got = outs[row["precision"]][c]
+            s64.append(10 * np.log10(np.sum(ref ** 2) / np.sum((got - ref) ** 2)))
+            r32 = outs["fp32"][c]
+            e32 = np.sum((got - r32) ** 2)
+            s32.append(10 * np.log10(np.sum(r32 ** 2) / e32) if e32 > 0 else float("inf"))
+        row["snr_vs_fp64_db"] = float(min(s64))
+        row["snr_vs_fp32_db"] = float(min(s32))
+        log(f"precision: {row}")
+    del h, x
+    torch.cuda.empty_cache()
+    return rows
+
+
 def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1>"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
@@ -268,7 +325,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
-    ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,32768,40960")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
@@ -318,6 +376,11 @@ def main():
         log("deadline sweep")
         best, rows = bench_sweep(T, torch, ctx, [int(v) for v in args.sweep.split(",") if v])
 
+    prec_rows = []
+    if not args.no_precision:
+        log("precision pass")
+        prec_rows = bench_precision(T, torch, ctx)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -341,6 +404,7 @@ def main():
         "sweep": rows,
         "roofline_hbm": ({"channels": big["channels"], "achieved": big["tick_GBps"],
                           "frac": big["tick_GBps"] / HBM_PEAK_GBPS} if big else None),
+        "precision": prec_rows,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

@@ -142,6 +142,17 @@ int32_t tanx_conv_process(tanx_conv *conv, const float *const *in, float *const 
 int32_t tanx_conv_process_device(tanx_conv *conv, const float *d_in, size_t in_stride,
                                  float *d_out, size_t out_stride, size_t n,
                                  const uint32_t *flags, size_t *processed);
+/* One block for each of n objects in ONE kernel launch (no counterpart in
+ * the reference, which runs one engine per object).  Object i reads
+ * d_in[i] + c * in_stride[i] and writes d_out[i] + c * out_stride[i] for
+ * channel c, with flags[i] (flags or flags[i] may be NULL), exactly as
+ * tanx_conv_process_device would for one block; objects keep independent
+ * state, so each result equals the object processed alone.  All objects
+ * must share the context and IR precision, have B <= 512 and not use
+ * ProcessFinalize mode.  Asynchronous on the context's process stream. */
+int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
+                                  const size_t *in_stride, float *const *d_out, const size_t *out_stride,
+                                  const uint32_t *const *flags);
 /* ProcessFinalize (ConvolutionImpl.cpp:1046-1079): enqueues the tail
  * multiply-accumulate for the next block.  Implicit after each Process unless
  * the object was created with the 0x8000 bit. */
@@ -179,6 +190,12 @@ int32_t tanx_conv_autotune(tanx_conv *conv, uint32_t *chunks, double *us_per_blo
  * multiply-accumulate in one launch (default 1); 0 launches them separately
  * as the host-memory path does.  Results are identical either way. */
 int32_t tanx_conv_set_fused(tanx_conv *conv, uint32_t fused);
+/* Device-memory Process over two or more blocks (no channel flags, fused,
+ * timing off) replays a captured hipGraph of one tick launch per block,
+ * whose arguments come from a descriptor array copied once per batch of up
+ * to 64 blocks (default 1); 0 launches each block directly.  Results are
+ * identical either way. */
+int32_t tanx_conv_set_graph(tanx_conv *conv, uint32_t use_graph);
 /* Kernel timing: when enabled, HIP events are recorded on the process
  * stream around every launch.  "head" counts launches that carry the head
  * role (with the next block's tail when fused); "tail" counts separate tail

@@ -57,9 +57,13 @@ _SIGS = {
     "tanx_conv_last_was_crossfade": (_i32, [_vp]),
     "tanx_conv_enable_timing": (_i32, [_vp, _u32]),
     "tanx_conv_set_fused": (_i32, [_vp, _u32]),
+    "tanx_conv_set_graph": (_i32, [_vp, _u32]),
     "tanx_conv_autotune": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_double)]),
     "tanx_conv_read_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_conv_process_stream": (_vp, [_vp]),
+    "tanx_group_process_device": (_i32, [_u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                         ctypes.POINTER(_sz), ctypes.POINTER(_vp), ctypes.POINTER(_sz),
+                                         ctypes.c_void_p]),
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),
     "tanx_fft_transform_device": (_i32, [_vp, _u32, _u32, _u32, _vp, _vp, _sz]),
     "tanx_version": (ctypes.c_char_p, []),

@@ -60,6 +60,13 @@ inline bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
 // ---------------------------------------------------------------------------
 struct tanx_ctx {
     int device = 0;
+    // group launches: descriptors + block offsets, pinned ring and device copy
+    std::mutex group_mtx;
+    char *g_host = nullptr;     // [kGroupRing][g_cap]
+    char *g_dev = nullptr;      // [g_cap]
+    size_t g_cap = 0;
+    hipEvent_t g_ev[4] = {};
+    int g_ring = 0;
     hipStream_t proc = nullptr;  // convolution queue equivalent
     hipStream_t upd = nullptr;   // general queue equivalent (IR updates, FFT)
     float2 *tw = nullptr;        // twiddle tables, see tw_table()
@@ -125,6 +132,9 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
         if (ctx->tw) (void)hipFree(ctx->tw);
         if (ctx->fft_dev) (void)hipFree(ctx->fft_dev);
         if (ctx->fft_host) (void)hipHostFree(ctx->fft_host);
+        if (ctx->g_host) (void)hipHostFree(ctx->g_host);
+        if (ctx->g_dev) (void)hipFree(ctx->g_dev);
+        for (auto &e : ctx->g_ev) if (e) (void)hipEventDestroy(e);
     }
     delete ctx;
     return TANX_OK;
@@ -143,6 +153,8 @@ extern "C" int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device)
 static constexpr int kMaxChunks = 64;
 static constexpr int kRunRing = 8;
 static constexpr int kTimerRing = 1024;
+static constexpr int kGraphBlocks = 64;
+static constexpr int kDescRing = 4;
 
 // Per-kernel durations measured with HIP events recorded on the process
 // stream around each launch; drained into running sums when the ring fills
@@ -236,6 +248,19 @@ struct tanx_conv {
 
     bool timing = false;
     KernelTimer t_head, t_tail;
+
+    // graph replay of device-memory blocks
+    bool use_graph = true;
+    struct GraphEntry {
+        int k;
+        unsigned blocks;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    TickArgs *d_desc = nullptr;   // [kGraphBlocks] launch descriptors (device)
+    TickArgs *h_desc = nullptr;   // [kDescRing][kGraphBlocks] pinned staging
+    hipEvent_t desc_ev[kDescRing] = {};
+    int desc_ring = 0;
     uint64_t tail_alg_bytes = 0, head_alg_bytes = 0;
 
     size_t row_bytes() const { return (size_t)B * ir_elem_bytes(irtype); }
@@ -266,6 +291,10 @@ static void conv_free(tanx_conv *cv)
         if (cv->ev_out) (void)hipEventDestroy(cv->ev_out);
         cv->t_head.destroy();
         cv->t_tail.destroy();
+        for (auto &g : cv->graphs) (void)hipGraphExecDestroy(g.exec);
+        if (cv->d_desc) (void)hipFree(cv->d_desc);
+        if (cv->h_desc) (void)hipHostFree(cv->h_desc);
+        for (auto &e : cv->desc_ev) if (e) (void)hipEventDestroy(e);
     }
     delete cv;
 }
@@ -370,6 +399,13 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
         return fail(TANX_OUT_OF_MEMORY);
     for (auto &ev : cv->run_ev)
         if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(TANX_UNEXPECTED);
+    if (hipMalloc(&cv->d_desc, sizeof(TickArgs) * kGraphBlocks) != hipSuccess ||
+        hipHostMalloc(&cv->h_desc, sizeof(TickArgs) * kGraphBlocks * kDescRing) != hipSuccess)
+        return fail(TANX_OUT_OF_MEMORY);
+    for (auto &ev : cv->desc_ev)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(TANX_UNEXPECTED);
+    for (auto &ev : cv->desc_ev)
+        if (hipEventRecord(ev, ctx->proc) != hipSuccess) return fail(TANX_UNEXPECTED);
     if (hipEventCreateWithFlags(&cv->ev_upd, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&cv->ev_proc, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&cv->ev_out, hipEventDisableTiming) != hipSuccess)
@@ -484,6 +520,7 @@ static int32_t sync_run_mask(tanx_conv *cv)
 static TickArgs tick_common(tanx_conv *cv)
 {
     TickArgs a{};
+    a.B = cv->B;
     a.C = cv->C;
     a.P = cv->P;
     a.irtype = cv->irtype;
@@ -573,12 +610,14 @@ static int32_t enqueue_tail(tanx_conv *cv)
     return TANX_OK;
 }
 
-// Process state machine (ConvolutionImpl.cpp:848-1043) for one block and the
-// head launch; with_tail also runs the next block's tail in the same launch.
-static int32_t process_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
-                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
+// Process state machine (ConvolutionImpl.cpp:848-1043) for one block: flag
+// bookkeeping, switch schedule, slot rotation, and the launch arguments of
+// the block's head (and, with_tail, the next block's tail).  Sets *tail_nf
+// and *needs_update when the launch reads a freshly transformed IR slot.
+static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
+                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail,
+                             TickArgs &a, int *tail_nf, bool *needs_update, bool *xfade_out)
 {
-    tanx_ctx *ctx = cv->ctx;
     int nrun = 0;
     int32_t rc = channel_flags(cv, flags, n, &nrun);
     if (rc != TANX_OK) return rc;
@@ -598,7 +637,7 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
         cv->cur = (cv->cur + 1) % 3;
         cv->upd = (cv->upd + 1) % 3;
     }
-    TickArgs a = tick_common(cv);
+    a = tick_common(cv);
     a.nhead = cv->C;
     a.xin = din;
     a.in_stride = in_stride;
@@ -625,9 +664,26 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
     cv->ticks++;
     cv->tail_due = true;
 
+    int nf = 1;
+    if (with_tail) nf = fill_tail(cv, a);
+    *tail_nf = nf;
+    *needs_update = xfade || nf == 2;
+    *xfade_out = xfade;
+    return TANX_OK;
+}
+
+// One block launched directly; with_tail also runs the next block's tail in
+// the same launch.
+static int32_t process_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
+                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
+{
+    TickArgs a;
     int tail_nf = 1;
-    if (with_tail) tail_nf = fill_tail(cv, a);
-    if (xfade || tail_nf == 2) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
+    bool needs_update = false, xfade = false;
+    int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, with_tail, a, &tail_nf,
+                               &needs_update, &xfade);
+    if (rc != TANX_OK) return rc;
+    if (needs_update) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     const uint64_t hb = head_bytes(cv, xfade ? 2 : 1);
     if (with_tail) {
         rc = launch_timed(cv, cv->t_head, tail_nf, a, hb + tail_bytes(cv, tail_nf), cv->head_alg_bytes);
@@ -636,6 +692,72 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
     } else {
         rc = launch_timed(cv, cv->t_head, 1, a, hb, cv->head_alg_bytes);
         if (rc != TANX_OK) return rc;
+    }
+    return TANX_OK;
+}
+
+// Instantiated graph of k descriptor-driven tick launches (one per block).
+static int32_t graph_for(tanx_conv *cv, int k, hipGraphExec_t *out)
+{
+    const unsigned blocks = (unsigned)(cv->C + cv->C * cv->nchunk);
+    for (auto &g : cv->graphs)
+        if (g.k == k && g.blocks == blocks) {
+            *out = g.exec;
+            return TANX_OK;
+        }
+    hipStream_t s = cv->ctx->proc;
+    hipGraph_t graph = nullptr;
+    TANX_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < k && e == hipSuccess; j++) e = launch_tick_desc(cv->B, cv->irtype, cv->d_desc, j, blocks, s);
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return map_hip(e != hipSuccess ? e : e2);
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return map_hip(e);
+    cv->graphs.push_back({k, blocks, exec});
+    *out = exec;
+    return TANX_OK;
+}
+
+// Device-memory blocks [0, nb) replayed through graphs of up to
+// kGraphBlocks launches: the host runs the state machine for the batch,
+// copies the batch's launch descriptors to the device, and launches the graph.
+static int32_t process_device_graph(tanx_conv *cv, const float *d_in, size_t in_stride, float *d_out,
+                                    size_t out_stride, size_t nb, size_t *processed)
+{
+    hipStream_t s = cv->ctx->proc;
+    const size_t B = cv->B;
+    size_t b = 0;
+    while (b < nb) {
+        const int k = (int)std::min<size_t>(kGraphBlocks, nb - b);
+        const int slot = cv->desc_ring;
+        cv->desc_ring = (cv->desc_ring + 1) % kDescRing;
+        TANX_CHECK(hipEventSynchronize(cv->desc_ev[slot]));  // pinned entry free
+        TickArgs *h = cv->h_desc + (size_t)slot * kGraphBlocks;
+        bool wait_update = false;
+        for (int j = 0; j < k; j++) {
+            int nf = 1;
+            bool nu = false, xf = false;
+            int32_t rc = prepare_block(cv, d_in + (b + j) * B, (long long)in_stride, d_out + (b + j) * B,
+                                       (long long)out_stride, B, nullptr, true, h[j], &nf, &nu, &xf);
+            if (rc != TANX_OK) return rc;
+            wait_update = wait_update || nu;
+            tail_done(cv);
+        }
+        hipGraphExec_t exec = nullptr;
+        int32_t rc = graph_for(cv, k, &exec);
+        if (rc != TANX_OK) return rc;
+        if (wait_update) TANX_CHECK(hipStreamWaitEvent(s, cv->ev_upd, 0));
+        TANX_CHECK(hipMemcpyAsync(cv->d_desc, h, sizeof(TickArgs) * k, hipMemcpyHostToDevice, s));
+        TANX_CHECK(hipEventRecord(cv->desc_ev[slot], s));
+        TANX_CHECK(hipGraphLaunch(exec, s));
+        b += k;
+        if (processed) *processed = b * B;
     }
     return TANX_OK;
 }
@@ -697,6 +819,8 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (nb == 0) return TANX_OK;
     if (in_stride < nb * B || out_stride < nb * B) return TANX_INVALID_ARG;
     DeviceGuard g(cv->ctx->device);
+    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !flags && nb >= 2)
+        return process_device_graph(cv, d_in, in_stride, d_out, out_stride, nb, processed);
     for (size_t b = 0; b < nb; b++) {
         const bool fuse = cv->fused && !cv->fmode;
         int32_t rc = process_block(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
@@ -808,6 +932,100 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
     cv->pending = true;
     if (opflags & TANX_OPERATION_FLAG_BLOCK_UNTIL_READY) TANX_CHECK(hipEventSynchronize(cv->ev_upd));
     return TANX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// group launches
+// ---------------------------------------------------------------------------
+static int32_t group_staging(tanx_ctx *ctx, size_t bytes, char **host, int *slot)
+{
+    if (!ctx->g_ev[0]) {
+        for (auto &e : ctx->g_ev) {
+            TANX_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            TANX_CHECK(hipEventRecord(e, ctx->proc));
+        }
+    }
+    if (ctx->g_cap < bytes) {
+        // queued group kernels may still read the device copy
+        TANX_CHECK(hipStreamSynchronize(ctx->proc));
+        if (ctx->g_host) (void)hipHostFree(ctx->g_host);
+        if (ctx->g_dev) (void)hipFree(ctx->g_dev);
+        ctx->g_host = nullptr;
+        ctx->g_dev = nullptr;
+        ctx->g_cap = 0;
+        size_t cap = 4096;
+        while (cap < bytes) cap *= 2;
+        TANX_CHECK(hipHostMalloc(&ctx->g_host, cap * 4));
+        TANX_CHECK(hipMalloc(&ctx->g_dev, cap));
+        ctx->g_cap = cap;
+    }
+    *slot = ctx->g_ring;
+    ctx->g_ring = (ctx->g_ring + 1) % 4;
+    TANX_CHECK(hipEventSynchronize(ctx->g_ev[*slot]));
+    *host = ctx->g_host + (size_t)(*slot) * ctx->g_cap;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
+                                             const size_t *in_stride, float *const *d_out,
+                                             const size_t *out_stride, const uint32_t *const *flags)
+{
+    if (n == 0) return TANX_OK;
+    if (!convs || !d_in || !d_out || !in_stride || !out_stride) return TANX_INVALID_ARG;
+    tanx_ctx *ctx = convs[0] ? convs[0]->ctx : nullptr;
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    const int irtype = convs[0]->irtype;
+    for (uint32_t i = 0; i < n; i++) {
+        tanx_conv *cv = convs[i];
+        if (!cv) return TANX_NOT_INITIALIZED;
+        if (cv->ctx != ctx || cv->irtype != irtype || cv->B > kGroupMaxB || cv->fmode) return TANX_NOT_SUPPORTED;
+        if (!d_in[i] || !d_out[i] || in_stride[i] < (size_t)cv->B || out_stride[i] < (size_t)cv->B)
+            return TANX_INVALID_ARG;
+    }
+    // lock every object (address order) and the context's group staging
+    std::vector<tanx_conv *> order(convs, convs + n);
+    std::sort(order.begin(), order.end());
+    if (std::adjacent_find(order.begin(), order.end()) != order.end()) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(n);
+    for (tanx_conv *cv : order) locks.emplace_back(cv->mtx);
+    DeviceGuard g(ctx->device);
+
+    const size_t desc_bytes = sizeof(TickArgs) * n, start_bytes = sizeof(int) * (n + 1);
+    char *host = nullptr;
+    int slot = 0;
+    int32_t rc = group_staging(ctx, desc_bytes + start_bytes, &host, &slot);
+    if (rc != TANX_OK) return rc;
+    TickArgs *descs = reinterpret_cast<TickArgs *>(host);
+    int *starts = reinterpret_cast<int *>(host + desc_bytes);
+    uint32_t done = 0;
+    int total = 0;
+    for (; done < n; done++) {
+        tanx_conv *cv = convs[done];
+        int nf = 1;
+        bool nu = false, xf = false;
+        rc = prepare_block(cv, d_in[done], (long long)in_stride[done], d_out[done], (long long)out_stride[done],
+                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &nu, &xf);
+        if (rc != TANX_OK) break;
+        if (nu) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
+        tail_done(cv);
+        starts[done] = total;
+        total += cv->C + cv->C * descs[done].nchunk;
+    }
+    if (done > 0) {
+        // objects already advanced are launched even if a later one failed
+        const size_t nb = sizeof(TickArgs) * done;
+        std::memmove(host + nb, starts, sizeof(int) * done);
+        TANX_CHECK(hipMemcpyAsync(ctx->g_dev, host, nb + sizeof(int) * done, hipMemcpyHostToDevice, ctx->proc));
+        TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
+        TANX_CHECK(launch_group_tick(irtype, reinterpret_cast<const TickArgs *>(ctx->g_dev),
+                                     reinterpret_cast<const int *>(ctx->g_dev + nb), (int)done, (unsigned)total,
+                                     ctx->proc));
+    } else {
+        TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
+    }
+    return rc;
 }
 
 extern "C" int32_t tanx_conv_update_td(tanx_conv *cv, const float *const *ir, size_t n,
@@ -993,6 +1211,14 @@ extern "C" int32_t tanx_conv_autotune(tanx_conv *cv, uint32_t *chosen, double *b
     if (chosen) *chosen = (uint32_t)cv->nchunk;
     if (best_us) *best_us = best;
     return rc;
+}
+
+extern "C" int32_t tanx_conv_set_graph(tanx_conv *cv, uint32_t use_graph)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    cv->use_graph = use_graph != 0;
+    return TANX_OK;
 }
 
 extern "C" int32_t tanx_conv_set_fused(tanx_conv *cv, uint32_t fused)

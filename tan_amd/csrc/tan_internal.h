@@ -26,6 +26,7 @@ enum IrType { IR_F32 = 0, IR_F16 = 1, IR_BF16 = 2 };
 inline int ir_elem_bytes(int t) { return t == IR_F32 ? 8 : 4; }
 
 struct TickArgs {
+    int B;                 // block size (group launches dispatch on it)
     int C;
     int P;
     int irtype;            // IrType of Ha/Hb/tHa/tHb
@@ -88,6 +89,14 @@ struct FftArgs {
 
 // Launchers (return hipError_t).  B is the block size (power of two).
 hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s);
+// One launch for several objects: block b belongs to the object o with
+// starts[o] <= b < starts[o + 1] and runs its roles from d_descs[o].  Every
+// object must use irtype and a block size in {64, 128, 256, 512}.
+hipError_t launch_group_tick(int irtype, const TickArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
+                             hipStream_t s);
+constexpr int kGroupMaxB = 512;
+// k_tick reading its arguments from d_descs[k] (graph replay)
+hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, unsigned blocks, hipStream_t s);
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);

@@ -471,6 +471,65 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
     }
 }
 
+// Roles of one tick for block index b of one object's launch.
+template <int M, int HT>
+__device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem)
+{
+    if (b < a.nhead) {
+        if (!a.run[b]) return;
+        head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
+    } else {
+        const int t = b - a.nhead;
+        if (t >= a.C * a.nchunk) return;
+        const int c = t / a.nchunk, j = t % a.nchunk;
+        if (!a.run[c]) return;
+        if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
+        else tail_role<M, 1, HT>(a, c, j, smem);
+    }
+}
+
+// Graph-replayed variant: the arguments of launch k come from a descriptor
+// array in device memory (written by one copy per batch), so a captured
+// graph of k nodes can be replayed with new arguments; the tail's filter
+// count is read from the descriptor.
+template <int M, int HT>
+__global__ __launch_bounds__(kTickThreads) void k_tick_desc(const TickArgs *__restrict__ descs, int k)
+{
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
+    constexpr int kHeadBytes = sizeof(HeadSmem<M>);
+    constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
+    __shared__ float4 smem[(kBytes + 15) / 16];
+    const TickArgs a = descs[k];
+    tick_body<M, HT>(a, blockIdx.x, smem);
+}
+
+template <int HT>
+__global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__restrict__ descs,
+                                                             const int *__restrict__ starts, int nobj)
+{
+    constexpr int M = kGroupMaxB;
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);
+    constexpr int kHeadBytes = sizeof(HeadSmem<M>);
+    constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
+    __shared__ float4 smem[(kBytes + 15) / 16];
+    const int b = blockIdx.x;
+    int lo = 0, hi = nobj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TickArgs a = descs[lo];
+    const int lb = b - starts[lo];
+    switch (a.B) {
+    case 64: tick_body<64, HT>(a, lb, smem); break;
+    case 128: tick_body<128, HT>(a, lb, smem); break;
+    case 256: tick_body<256, HT>(a, lb, smem); break;
+    case 512: tick_body<512, HT>(a, lb, smem); break;
+    default: break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // IR partition transform: one workgroup per (partition i, listed channel).
 // Partition i of channel ch becomes the packed spectrum of [h[iB..iB+B), 0_B].
@@ -640,6 +699,36 @@ hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s)
     case IR_BF16: return launch_tick_t<IR_BF16>(B, tail_nf, a, s, blocks);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int HT>
+static hipError_t launch_desc_t(int B, const TickArgs *d, int k, unsigned blocks, hipStream_t s)
+{
+    TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick_desc<M, HT>), dim3(blocks), dim3(kTickThreads), 0, s, d, k));
+    return hipGetLastError();
+}
+
+hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, unsigned blocks, hipStream_t s)
+{
+    switch (irtype) {
+    case IR_F32: return launch_desc_t<IR_F32>(B, d_descs, k, blocks, s);
+    case IR_F16: return launch_desc_t<IR_F16>(B, d_descs, k, blocks, s);
+    case IR_BF16: return launch_desc_t<IR_BF16>(B, d_descs, k, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_group_tick(int irtype, const TickArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
+                             hipStream_t s)
+{
+    if (blocks == 0) return hipSuccess;
+    switch (irtype) {
+    case IR_F32: hipLaunchKernelGGL((k_group_tick<IR_F32>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_F16: hipLaunchKernelGGL((k_group_tick<IR_F16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_BF16: hipLaunchKernelGGL((k_group_tick<IR_BF16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)

@@ -267,6 +267,9 @@ class TANConvolution:
         rc = self._lib.tanx_conv_autotune(self.handle, ctypes.byref(n), ctypes.byref(us))
         return rc, n.value, us.value
 
+    def SetGraph(self, use_graph=True):
+        return self._lib.tanx_conv_set_graph(self.handle, int(bool(use_graph)))
+
     def SetFused(self, fused=True):
         return self._lib.tanx_conv_set_fused(self.handle, int(bool(fused)))
 
@@ -335,3 +338,32 @@ def TANCreateConvolution(context):
 
 def TANCreateFFT(context):
     return AMF_OK, TANFFT(context)
+
+
+def GroupProcess(convs, inputs, outputs, flag_masks=None):
+    """One block for each convolution object in one kernel launch.
+
+    convs: TANConvolution objects of one context; inputs/outputs: torch CUDA
+    tensors (channels, >= block) per object; flag_masks: None or a list of
+    per-object flag arrays (None entries allowed).  Asynchronous on the
+    context's process stream, like device-memory Process."""
+    n = len(convs)
+    if n == 0:
+        return AMF_OK
+    lib = convs[0]._lib
+    hs = (ctypes.c_void_p * n)(*[c.handle for c in convs])
+    ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in inputs])
+    outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in outputs])
+    ist = (ctypes.c_size_t * n)(*[t.stride(0) for t in inputs])
+    ost = (ctypes.c_size_t * n)(*[t.stride(0) for t in outputs])
+    fptr = None
+    keep = []
+    if flag_masks is not None:
+        arr = (ctypes.c_void_p * n)()
+        for i, f in enumerate(flag_masks):
+            if f is not None:
+                a = np.ascontiguousarray(f, dtype=np.uint32)
+                keep.append(a)
+                arr[i] = a.ctypes.data
+        fptr = ctypes.cast(arr, ctypes.c_void_p)
+    return lib.tanx_group_process_device(n, hs, ins, ist, outs, ost, fptr)

@@ -323,3 +323,28 @@ def test_autotune_keeps_state_and_parity(gpu_ctx):
         assert np.abs(out - oo).max() <= MAX_ABS
     assert conv.Autotune()[0] == T.AMF_WRONG_STATE
     conv.Terminate()
+
+
+def test_graph_replay_matches_direct_launches(gpu_ctx):
+    """Device Process over many blocks replays a captured graph of tick
+    launches; results equal direct launches bit for bit, across batch
+    boundaries (64 blocks) and crossfades."""
+    import torch
+    L, B, C, nb = 16384, 256, 4, 150
+    x = T.signals.input_bank(C, nb * B)
+    dx = torch.from_numpy(x).cuda()
+    outs = []
+    for graph in (True, False):
+        conv = T.TANConvolution(gpu_ctx)
+        conv.InitGpu(UNIFORM, L, B, C)
+        conv.SetGraph(graph)
+        dy = torch.zeros_like(dx)
+        torch.cuda.synchronize()
+        for b0, n in ((0, 70), (70, 5), (75, 75)):
+            conv.UpdateResponseTD(_irs(C, L, b0), L, None, BLOCKING)
+            rc, done = conv.Process(dx[:, b0 * B:], dy[:, b0 * B:], n * B)
+            assert rc == T.AMF_OK and done == n * B
+        conv.Synchronize()
+        outs.append(dy.cpu().numpy())
+        conv.Terminate()
+    np.testing.assert_array_equal(outs[0], outs[1])
