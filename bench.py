@@ -246,6 +246,144 @@ def bench_precision(T, torch, ctx, channels=1024, taps=131072, warm=8, blocks=32
     return rows
 
 
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else None
+
+
+def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
+    """Uneven streams (IR log-uniform 4,096-262,144 taps, B in {128, 256, 512},
+    one channel each) in one group launch per 128-sample base tick, with the
+    tail chunks tuned per (block, rounded length) bucket."""
+    from tan_amd import batching as Bt
+    rows = []
+    for n in counts:
+        specs = Bt.draw_streams(n, seed=seed)
+        batch = Bt.StreamBatch(ctx, specs)
+        period = batch.base_period_ms
+        cyc = batch.cycle
+        before = batch.time_ticks(cyc * 64) / (cyc * 64)
+        t0 = time.perf_counter()
+        chosen = batch.tune(cycles=8)
+        tune_s = time.perf_counter() - t0
+        after = batch.time_ticks(cyc * 64) / (cyc * 64)
+        # steady stream of ticks: device time over the whole run
+        total_ms = batch.time_ticks(ticks)
+        per = batch.time_ticks(min(ticks, 1000), per_tick=True)
+        gb = sum(batch.bytes_per_tick) / cyc / (total_ms / ticks * 1e-3) / 1e9
+        row = {"streams": n, "base_block": batch.base, "deadline_ms": period,
+               "objects_per_launch": [len(d) for d in batch.due],
+               "tick_ms_default_chunks": before, "tick_ms_tuned": after, "tune_seconds": tune_s,
+               "tuned_chunks": {f"B{b}_L{ln}": c for (b, ln), c in sorted(chosen.items())},
+               "mean_tick_ms": total_ms / ticks, "p50_tick_ms": _pct(per, 50), "p99_tick_ms": _pct(per, 99),
+               "max_tick_ms": float(max(per)), "deadline_misses": int(sum(1 for v in per if v > period)),
+               "alg_GBps": gb, "hbm_frac": gb / HBM_PEAK_GBPS,
+               "realtime_factor": period / (total_ms / ticks)}
+        log(f"  batching {n} streams: tick {row['mean_tick_ms'] * 1e3:.1f} us (default chunks "
+            f"{before * 1e3:.1f} us), p99 {row['p99_tick_ms'] * 1e3:.1f} us, {gb:.0f} GB/s, "
+            f"tuned {row['tuned_chunks']}")
+        rows.append(row)
+        for c in batch.convs:
+            c.Terminate()
+        del batch
+        torch.cuda.synchronize()
+    return rows
+
+
+def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precision=None):
+    """Poisson session arrivals (20/s, mean life 30 s, 1-16 channels, IR
+    log-uniform 8,192-524,288 taps, B = 256, IR update every U{10..100}
+    blocks) served with HBM-budget admission and one group launch per tick.
+
+    The run starts from a stationary population.  Session control (admission,
+    IR updates, departures) runs between ticks on the same thread; block
+    latency is the device time from the tick's start marker to the end of
+    its kernel (host enqueue included)."""
+    from tan_amd import serving as S
+    prec = T.TAN_IR_PRECISION_FP32 if precision is None else precision
+    wl = S.PoissonWorkload(block=B, seed=seed)
+    nb_bank = 4
+    bank = torch.empty(nb_bank, 16, wl.max_len, device="cuda")
+    env = torch.exp(-torch.arange(wl.max_len, device="cuda", dtype=torch.float32) / (0.25 * FS))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    for k in range(nb_bank):
+        bank[k] = torch.randn(16, wl.max_len, device="cuda", generator=g) * env
+    rows_cap = 16384
+    nx = 8
+    noise = torch.empty(nx, rows_cap, B, device="cuda").uniform_(-1.0, 1.0, generator=g)
+    torch.cuda.synchronize()
+    srv = S.SessionServer(ctx, block=B, hbm_budget=int(budget_gib * (1 << 30)), ir_precision=prec,
+                          io_rows=rows_cap)
+    meta = {}
+
+    def ir_for(sid, k, c, ln):
+        return bank[(sid + k) % nb_bank, :c, :ln]
+
+    def submit(d):
+        meta[d["sid"]] = d
+        d["updates"] = 0
+        srv.submit(d["sid"], d["channels"], d["ir_len"], ir_for(d["sid"], 0, d["channels"], d["ir_len"]))
+
+    t0 = time.perf_counter()
+    for d in wl.initial_population():
+        submit(d)
+    srv.admit()
+    setup_s = time.perf_counter() - t0
+    ticks = int(seconds * FS / B)
+    lat, enq, ctl, chans, sess, byts, waits = [], [], [], [], [], [], []
+    admitted = departed = updates = 0
+    pend = None
+    wall0 = time.perf_counter()
+    for t in range(ticks):
+        c0 = time.perf_counter()
+        for sid in [sid for sid, d in meta.items() if d["end_tick"] <= t]:
+            srv.release(sid)
+            del meta[sid]
+            departed += 1
+        for d in wl.arrivals(t):
+            submit(d)
+        for s in srv.admit():
+            admitted += 1
+            waits.append(t - s.arrived_tick)
+        for sid, s in srv.active.items():
+            d = meta[sid]
+            if s.blocks and s.blocks % d["update_every"] == 0:
+                d["updates"] += 1
+                srv.update(sid, ir_for(sid, d["updates"], d["channels"], d["ir_len"]))
+                updates += 1
+        srv.in_arena.copy_(noise[t % nx])
+        ctl.append((time.perf_counter() - c0) * 1e3)
+        pend = srv.tick()
+        rec = S.SessionServer.finish(pend)
+        lat.append(rec.latency_ms)
+        enq.append(rec.enqueue_ms)
+        chans.append(rec.channels)
+        sess.append(rec.sessions)
+        byts.append(rec.alg_bytes)
+    wall = time.perf_counter() - wall0
+    finite = bool(torch.isfinite(srv.out_arena).all().item())
+    queued = srv.admission.waiting()
+    srv.shutdown()
+    del bank, noise
+    torch.cuda.synchronize()
+    period = srv.period_ms
+    lat_a = np.asarray(lat)
+    gbps = np.asarray(byts) / (lat_a * 1e-3) / 1e9
+    return {"block": B, "deadline_ms": period, "audio_seconds": ticks * B / FS, "ticks": ticks,
+            "ir_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[prec],
+            "hbm_budget_GiB": budget_gib, "setup_seconds": setup_s,
+            "sessions_mean": float(np.mean(sess)), "sessions_max": int(max(sess)),
+            "channels_mean": float(np.mean(chans)), "channels_max": int(max(chans)),
+            "admitted_during_run": admitted, "departed": departed, "ir_updates": updates,
+            "queued_at_end": queued, "admission_wait_ticks_max": int(max(waits)) if waits else 0,
+            "p50_block_ms": _pct(lat, 50), "p99_block_ms": _pct(lat, 99), "max_block_ms": float(lat_a.max()),
+            "deadline_misses": int((lat_a > period).sum()), "miss_rate": float((lat_a > period).mean()),
+            "p50_enqueue_ms": _pct(enq, 50), "p99_enqueue_ms": _pct(enq, 99),
+            "p50_control_ms": _pct(ctl, 50), "p99_control_ms": _pct(ctl, 99),
+            "alg_GBps_p50": _pct(gbps, 50), "hbm_frac_p50": _pct(gbps, 50) / HBM_PEAK_GBPS,
+            "realtime_factor": ticks * B / FS / wall, "outputs_finite": finite}
+
+
 def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1>"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
@@ -326,6 +464,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batching-streams", type=str, default="512,2048")
+    ap.add_argument("--serving-seconds", type=float, default=20.0)
     ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
@@ -381,6 +523,17 @@ def main():
         log("precision pass")
         prec_rows = bench_precision(T, torch, ctx)
 
+    batching = []
+    if not args.no_batching:
+        log("batching pass")
+        batching = bench_batching(T, torch, ctx, [int(v) for v in args.batching_streams.split(",") if v])
+
+    serving = None
+    if not args.no_serving:
+        log("serving pass")
+        serving = bench_serving(T, torch, ctx, seconds=args.serving_seconds)
+        log(f"  {serving}")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -405,6 +558,8 @@ def main():
         "roofline_hbm": ({"channels": big["channels"], "achieved": big["tick_GBps"],
                           "frac": big["tick_GBps"] / HBM_PEAK_GBPS} if big else None),
         "precision": prec_rows,
+        "batching": batching,
+        "serving": serving,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

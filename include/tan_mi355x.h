@@ -88,6 +88,11 @@ int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out);
 /* TANContext::Terminate (TANContextImpl.cpp:158-179). */
 int32_t tanx_ctx_destroy(tanx_ctx *ctx);
 int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
+/* The context's process stream (a hipStream_t), on which every Process and
+ * group launch of its objects runs; for callers that order their own work or
+ * timing events against it (the reference's counterpart is
+ * TANContext::GetOpenCLConvQueue, TrueAudioNext.h:596). */
+void *tanx_ctx_process_stream(tanx_ctx *ctx);
 
 /* ---- convolution ----------------------------------------------------------- */
 
@@ -112,6 +117,11 @@ typedef struct tanx_conv_options {
  * stores FP32 spectra only); opt may be NULL. */
 int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t ir_len, uint32_t block,
                             uint32_t channels, const tanx_conv_options *opt, tanx_conv **out);
+/* HBM bytes an object of this shape will hold (for admission control; no
+ * counterpart in the reference).  Host-memory IR updates add C * rounded
+ * ir_len * 4 bytes of staging on first use. */
+int32_t tanx_conv_estimate_bytes(uint32_t ir_len, uint32_t block, uint32_t channels,
+                                 const tanx_conv_options *opt, uint64_t *bytes);
 /* TANConvolution::Terminate (ConvolutionImpl.cpp:307-355). */
 int32_t tanx_conv_destroy(tanx_conv *conv);
 

@@ -40,9 +40,11 @@ _SIGS = {
     "tanx_ctx_create": (_i32, [ctypes.c_uint64, _i32, ctypes.POINTER(_vp)]),
     "tanx_ctx_destroy": (_i32, [_vp]),
     "tanx_ctx_device": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "tanx_ctx_process_stream": (_vp, [_vp]),
     "tanx_conv_create": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
     "tanx_conv_create_ex": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "tanx_conv_destroy": (_i32, [_vp]),
+    "tanx_conv_estimate_bytes": (_i32, [_u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "tanx_conv_update_td": (_i32, [_vp, ctypes.POINTER(_f32p), _sz, _u32p, _u32]),
     "tanx_conv_update_td_device": (_i32, [_vp, _vp, _sz, _sz, _u32p, _u32]),
     "tanx_conv_process": (_i32, [_vp, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p), _sz, _u32p,

@@ -140,6 +140,8 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
     return TANX_OK;
 }
 
+extern "C" void *tanx_ctx_process_stream(tanx_ctx *ctx) { return ctx ? (void *)ctx->proc : nullptr; }
+
 extern "C" int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device)
 {
     if (!ctx || !device) return TANX_INVALID_POINTER;
@@ -317,6 +319,33 @@ static void set_chunks(tanx_conv *cv, int n)
     cv->nchunk = (parts + cv->pc - 1) / cv->pc;
 }
 
+// device bytes an object of this shape holds (the single allocation of
+// tanx_conv_create_ex, before any host-IR staging)
+static size_t conv_device_bytes(uint32_t len, uint32_t block, uint32_t channels, int irtype, int maxchunk)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t C = channels, B = block, P = len / block;
+    return al(3 * C * P * B * (size_t)ir_elem_bytes(irtype)) + al(3 * C * P * sizeof(float)) +
+           al(C * P * B * sizeof(float2)) + al(4 * C * (size_t)maxchunk * B * sizeof(float2)) +
+           3 * al(C * B * sizeof(float)) + al(C * sizeof(int)) + al(C) + sizeof(TickArgs) * kGraphBlocks;
+}
+
+static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(kMaxChunks, P - 2), 16384 / C)); }
+
+extern "C" int32_t tanx_conv_estimate_bytes(uint32_t ir_len, uint32_t block, uint32_t channels,
+                                            const tanx_conv_options *opt, uint64_t *bytes)
+{
+    if (!bytes) return TANX_INVALID_POINTER;
+    if (ir_len == 0 || block == 0 || channels == 0 || (block & (block - 1))) return TANX_INVALID_ARG;
+    uint32_t len = 1;
+    while (len < ir_len) len <<= 1;
+    if (len < block) len = block;
+    const uint32_t prec = opt ? opt->ir_precision : TANX_IR_FP32;
+    const int irtype = prec == TANX_IR_FP16 ? IR_F16 : prec == TANX_IR_BF16 ? IR_BF16 : IR_F32;
+    *bytes = conv_device_bytes(len, block, channels, irtype, max_chunks_for((int)channels, (int)(len / block)));
+    return TANX_OK;
+}
+
 extern "C" int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
                                     uint32_t block, uint32_t channels, tanx_conv **out)
 {
@@ -363,7 +392,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->len = (int)len;
     cv->P = (int)(len / block);
     // partial-sum rows are only worth it at small channel counts
-    cv->maxchunk = std::max(1, std::min(std::min(kMaxChunks, cv->P - 2), 16384 / cv->C));
+    cv->maxchunk = max_chunks_for(cv->C, cv->P);
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
 

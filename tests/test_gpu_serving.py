@@ -1,0 +1,139 @@
+"""Continuous batching: sessions that arrive, update their IR and leave while
+others run give the same output in the mix as processed alone, bit for bit,
+and match the oracle; HBM-budget admission holds a session back until
+another one leaves."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import tan_amd as T
+from helpers import snr_db
+from tan_amd import serving as S
+
+pytestmark = pytest.mark.gpu
+UNIFORM = T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+B = 256
+
+# sid: (channels, ir_len, arrive tick, leave tick, ticks of IR updates)
+PLAN = {
+    0: (2, 5000, 0, 60, (8, 30)),
+    1: (2, 20000, 3, 40, ()),
+    2: (3, 3000, 0, 60, (15,)),
+    3: (4, 12000, 10, 60, (20, 21, 44)),
+    4: (16, 2048, 25, 60, (33,)),
+}
+
+
+def _ir(sid, k, C, L):
+    return T.signals.ir_bank(C, L, k, base_seed=300 + sid)
+
+
+def _x(sid, tick, C):
+    rng = np.random.default_rng([sid, tick])
+    return rng.uniform(-1.0, 1.0, (C, B)).astype(np.float32)
+
+
+def _run_mix(ctx, budget, plan=PLAN, nticks=60):
+    """Drive a server through the plan; returns {sid: (admit tick, outputs)}."""
+    import torch
+    srv = S.SessionServer(ctx, block=B, hbm_budget=budget)
+    outs = {sid: [] for sid in plan}
+    admitted_at = {}
+    upd_count = {sid: 0 for sid in plan}
+    for t in range(nticks):
+        for sid, (C, L, t0, t1, ups) in plan.items():
+            if t == t1 and (sid in srv.active or sid in srv.pending):
+                srv.release(sid)
+            if t == t0:
+                assert srv.submit(sid, C, L, torch.from_numpy(_ir(sid, 0, C, L)).cuda())
+        for s in srv.admit():
+            admitted_at[s.sid] = t
+        for sid, (C, L, t0, t1, ups) in plan.items():
+            if sid in srv.active and (t - admitted_at[sid]) in ups:
+                upd_count[sid] += 1
+                srv.update(sid, torch.from_numpy(_ir(sid, upd_count[sid], C, L)).cuda())
+        for sid, s in srv.active.items():
+            s.inp.copy_(torch.from_numpy(_x(sid, t - admitted_at[sid], s.channels)))
+        rec = S.SessionServer.finish(srv.tick())
+        assert rec.sessions == len(srv.active)
+        for sid, s in srv.active.items():
+            outs[sid].append(s.out.cpu().numpy().copy())
+    srv.shutdown()
+    return {sid: (admitted_at[sid], np.concatenate(o, axis=1)) for sid, o in outs.items() if o}
+
+
+def _alone(ctx, sid, nblocks, plan=PLAN):
+    """The same session on its own object, one Process per block."""
+    import torch
+    C, L, t0, t1, ups = plan[sid]
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    y = np.zeros((C, nblocks * B), np.float32)
+    k = 0
+    assert conv.UpdateResponseTD(torch.from_numpy(_ir(sid, 0, C, L)).cuda(), L) == T.AMF_OK
+    yo = torch.zeros(C, B, device="cuda")
+    for b in range(nblocks):
+        if b in ups:
+            k += 1
+            assert conv.UpdateResponseTD(torch.from_numpy(_ir(sid, k, C, L)).cuda(), L) == T.AMF_OK
+        xb = torch.from_numpy(_x(sid, b, C)).cuda()
+        rc, done = conv.Process(xb, yo, B)
+        assert rc == T.AMF_OK and done == B
+        conv.Synchronize()
+        y[:, b * B:(b + 1) * B] = yo.cpu().numpy()
+    conv.Terminate()
+    return y
+
+
+def _oracle(sid, nblocks, plan=PLAN):
+    C, L, t0, t1, ups = plan[sid]
+    orc = O.OracleConvolution(L, B, C)
+    orc.update_td(_ir(sid, 0, C, L))
+    y = np.zeros((C, nblocks * B), np.float32)
+    k = 0
+    for b in range(nblocks):
+        if b in ups:
+            k += 1
+            orc.update_td(_ir(sid, k, C, L))
+        _, o, _ = orc.process(_x(sid, b, C))
+        y[:, b * B:(b + 1) * B] = o
+    return y
+
+
+def test_session_alone_equals_mix_and_oracle(gpu_ctx):
+    mix = _run_mix(gpu_ctx, budget=64 << 30)
+    for sid, (at, y) in mix.items():
+        C, L, t0, t1, ups = PLAN[sid]
+        assert at == t0
+        nb = y.shape[1] // B
+        assert nb == t1 - t0
+        ya = _alone(gpu_ctx, sid, nb)
+        np.testing.assert_array_equal(y, ya, err_msg=f"session {sid}")
+        yo = _oracle(sid, nb)
+        assert np.abs(y - yo).max() <= 1e-5, f"session {sid}"
+        live = [c for c in range(C) if np.any(yo[c])]
+        assert live and min(snr_db(yo[c], y[c]) for c in live) >= 120.0
+
+
+def test_budget_holds_session_until_release(gpu_ctx):
+    lib = gpu_ctx._lib
+    need = {sid: S.estimate_bytes(lib, L, B, C) for sid, (C, L, *_r) in PLAN.items()}
+    # 3 (arrives at 10) does not fit beside 0, 1 and 2; 4 (arrives at 25,
+    # smaller) does and goes first; 3 fits once 1 leaves at tick 40
+    assert need[1] > need[4] and need[3] >= need[1]
+    budget = need[0] + need[2] + need[3] + need[4]
+    mix = _run_mix(gpu_ctx, budget=budget)
+    at3, y3 = mix[3]
+    assert at3 == PLAN[1][3], "session 3 is admitted in the tick session 1 leaves"
+    nb = y3.shape[1] // B
+    np.testing.assert_array_equal(y3, _alone(gpu_ctx, 3, nb))
+    for sid in (0, 1, 2, 4):
+        assert mix[sid][0] == PLAN[sid][2]
+
+
+def test_admission_rejects_oversized(gpu_ctx):
+    import torch
+    srv = S.SessionServer(gpu_ctx, block=B, hbm_budget=1 << 20)
+    assert not srv.submit(0, 16, 524288, torch.zeros(16, 524288, device="cuda"))
+    assert srv.admission.rejected == 1 and not srv.pending
+    srv.shutdown()
