@@ -80,9 +80,13 @@ def run_blocks(T, conv, x, y, banks, t0, t1):
         t += nb
 
 
-def bench_throughput(T, torch, ctx, steps, warmup, timing):
+UNIFORM_METHOD = 3  # TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+NONUNIFORM_METHOD = 2  # TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
+
+
+def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD):
     conv = T.TANConvolution(ctx)
-    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, C) == T.AMF_OK
+    assert conv.InitGpu(method, L, B, C) == T.AMF_OK
     rc, chunks, us = conv.Autotune()
     assert rc == T.AMF_OK, rc
     log(f"  autotune: {chunks} tail chunks, {us:.2f} us per block")
@@ -141,18 +145,20 @@ def bench_latency(T, ctx, nblocks):
             "worst": [[int(i) + 8, float(lat[i])] for i in worst]}
 
 
-def bench_sweep(T, torch, ctx, counts, ticks=96):
+def bench_sweep(T, torch, ctx, counts, ticks=96, method=UNIFORM_METHOD):
     """Per-block latency with one block in flight, device-resident data."""
+    from tan_amd import serving as S
     rows = []
     best = 0
     for ch in counts:
         free, _ = torch.cuda.mem_get_info()
-        need = ch * (4 * (L // B) * B * 8 + L * 4 + 64 * B * 8) * 1.1
+        # the object, its IR staging and 2 x 64 blocks of input and output
+        need = (S.estimate_bytes(ctx._lib, L, B, ch, method=method) + ch * L * 4 + 2 * ch * 64 * B * 4) * 1.05
         if need > free:
             log(f"sweep: skip {ch} channels (needs {need / 2**30:.1f} GiB)")
             break
         conv = T.TANConvolution(ctx)
-        assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, ch) == T.AMF_OK
+        assert conv.InitGpu(method, L, B, ch) == T.AMF_OK
         h = ir_bank_device(torch, ch, L, 77)
         torch.cuda.synchronize()
         assert conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY) == 0
@@ -179,7 +185,8 @@ def bench_sweep(T, torch, ctx, counts, ticks=96):
         p99 = float(np.percentile(lat, 99))
         tick_us = kt["head_ms"] / max(1, kt["head_launches"]) * 1e3
         gbps = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9 if kt["head_ms"] > 0 else 0.0
-        row = {"channels": ch, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": p99,
+        row = {"channels": ch, "method": "nonuniform" if method == NONUNIFORM_METHOD else "uniform",
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": p99,
                "max_ms": float(lat.max()), "tick_kernel_us": tick_us,
                "tick_GBps": gbps, "meets_deadline": bool(lat.max() <= DEADLINE_MS)}
         log(f"sweep: {row}")
@@ -289,7 +296,7 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
     return rows
 
 
-def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precision=None):
+def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precision=None, method=UNIFORM_METHOD):
     """Poisson session arrivals (20/s, mean life 30 s, 1-16 channels, IR
     log-uniform 8,192-524,288 taps, B = 256, IR update every U{10..100}
     blocks) served with HBM-budget admission and one group launch per tick.
@@ -313,7 +320,7 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
     noise = torch.empty(nx, rows_cap, B, device="cuda").uniform_(-1.0, 1.0, generator=g)
     torch.cuda.synchronize()
     srv = S.SessionServer(ctx, block=B, hbm_budget=int(budget_gib * (1 << 30)), ir_precision=prec,
-                          io_rows=rows_cap)
+                          io_rows=rows_cap, method=method)
     meta = {}
 
     def ir_for(sid, k, c, ln):
@@ -369,7 +376,8 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
     period = srv.period_ms
     lat_a = np.asarray(lat)
     gbps = np.asarray(byts) / (lat_a * 1e-3) / 1e9
-    return {"block": B, "deadline_ms": period, "audio_seconds": ticks * B / FS, "ticks": ticks,
+    return {"method": "nonuniform" if method == NONUNIFORM_METHOD else "uniform",
+            "block": B, "deadline_ms": period, "audio_seconds": ticks * B / FS, "ticks": ticks,
             "ir_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[prec],
             "hbm_budget_GiB": budget_gib, "setup_seconds": setup_s,
             "sessions_mean": float(np.mean(sess)), "sessions_max": int(max(sess)),
@@ -467,7 +475,9 @@ def main():
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batching-streams", type=str, default="512,2048")
-    ap.add_argument("--serving-seconds", type=float, default=20.0)
+    ap.add_argument("--serving-seconds", type=float, default=15.0)
+    ap.add_argument("--no-nonuniform", action="store_true")
+    ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304")
     ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
@@ -523,16 +533,35 @@ def main():
         log("precision pass")
         prec_rows = bench_precision(T, torch, ctx)
 
+    nonuniform = None
+    if not args.no_nonuniform:
+        log("non-uniform pass")
+        dtn, _, infon, finite_n = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=False,
+                                                   method=NONUNIFORM_METHOD)
+        assert finite_n, "non-finite output (non-uniform)"
+        nu_value = C * args.steps * B / FS / dtn
+        log(f"  non-uniform (M = {infon['nu_multiple']}): {nu_value:.0f} real-time channels")
+        nu_best, nu_rows = (0, [])
+        if not args.no_sweep:
+            nu_best, nu_rows = bench_sweep(T, torch, ctx, [int(v) for v in args.nu_sweep.split(",") if v],
+                                           method=NONUNIFORM_METHOD)
+        nonuniform = {"method": "TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM",
+                      "nu_multiple": infon["nu_multiple"], "value": nu_value,
+                      "ms_per_step": dtn / args.steps * 1e3, "rt_channels_at_deadline": nu_best,
+                      "sweep": nu_rows}
+
     batching = []
     if not args.no_batching:
         log("batching pass")
         batching = bench_batching(T, torch, ctx, [int(v) for v in args.batching_streams.split(",") if v])
 
-    serving = None
+    serving = []
     if not args.no_serving:
-        log("serving pass")
-        serving = bench_serving(T, torch, ctx, seconds=args.serving_seconds)
-        log(f"  {serving}")
+        for meth in ((UNIFORM_METHOD,) if args.no_nonuniform else (UNIFORM_METHOD, NONUNIFORM_METHOD)):
+            log(f"serving pass ({'non-uniform' if meth == NONUNIFORM_METHOD else 'uniform'})")
+            row = bench_serving(T, torch, ctx, seconds=args.serving_seconds, method=meth)
+            log(f"  {row}")
+            serving.append(row)
 
     cpu = None
     if not args.no_cpu:
@@ -558,6 +587,7 @@ def main():
         "roofline_hbm": ({"channels": big["channels"], "achieved": big["tick_GBps"],
                           "frac": big["tick_GBps"] / HBM_PEAK_GBPS} if big else None),
         "precision": prec_rows,
+        "nonuniform": nonuniform,
         "batching": batching,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -99,10 +99,18 @@ void *tanx_ctx_process_stream(tanx_ctx *ctx);
 /* TANCreateConvolution + TANConvolution::Init/InitGpu
  * (convolution/ConvolutionImpl.cpp:66-80, 179-305, 1113-1259).  The uniform
  * methods (1, 3, 4, 5, 6) and overlap-add (0) map to the one uniformly
- * partitioned engine; bit 0x8000 selects ProcessFinalize mode.  B must be a
- * power of two in [64, 1024] (the reference GPU path requires a power of
- * two, ConvolutionImpl.cpp:1137-1146).  The IR length is rounded up to a
- * power of two as the reference does (ConvolutionImpl.cpp:1230-1235). */
+ * partitioned engine; bit 0x8000 selects ProcessFinalize mode.  The
+ * non-uniform methods (2, 8, 9; ConvolutionImpl.cpp:201-262, 2380-2643) split
+ * the IR into a head of one frame of M blocks, run as uniform partitions of
+ * B, and frame partitions of M * B whose multiply-accumulate is spread over
+ * the M blocks of a frame; the output is the same convolution with the same
+ * B-sample latency, and an IR switch crossfades at the first frame boundary
+ * at least M + 1 blocks after the switch would happen in a uniform object
+ * (the reference's non-uniform mode also waits for a frame boundary,
+ * ConvolutionImpl.cpp:899-905).  B must be a power of two in [64, 1024]
+ * (the reference GPU path requires a power of two, ConvolutionImpl.cpp:
+ * 1137-1146).  The IR length is rounded up to a power of two as the
+ * reference does (ConvolutionImpl.cpp:1230-1235). */
 int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
                          uint32_t block, uint32_t channels, tanx_conv **out);
 /* Storage precision of the IR spectra.  FP16 and BF16 rows are scaled by
@@ -111,7 +119,10 @@ int32_t tanx_conv_create(tanx_ctx *ctx, uint32_t method, uint32_t ir_len,
 enum { TANX_IR_FP32 = 0, TANX_IR_FP16 = 1, TANX_IR_BF16 = 2 };
 typedef struct tanx_conv_options {
     uint32_t ir_precision;  /* TANX_IR_* */
-    uint32_t reserved[7];   /* zero */
+    uint32_t nu_multiple;   /* non-uniform methods: blocks per frame (a power
+                               of two >= 2, frame <= 4096 samples); 0 = chosen
+                               by the library, 1 = uniform partitions */
+    uint32_t reserved[6];   /* zero */
 } tanx_conv_options;
 /* tanx_conv_create with options (no counterpart in the reference, which
  * stores FP32 spectra only); opt may be NULL. */
@@ -120,7 +131,7 @@ int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t ir_len, uin
 /* HBM bytes an object of this shape will hold (for admission control; no
  * counterpart in the reference).  Host-memory IR updates add C * rounded
  * ir_len * 4 bytes of staging on first use. */
-int32_t tanx_conv_estimate_bytes(uint32_t ir_len, uint32_t block, uint32_t channels,
+int32_t tanx_conv_estimate_bytes(uint32_t method, uint32_t ir_len, uint32_t block, uint32_t channels,
                                  const tanx_conv_options *opt, uint64_t *bytes);
 /* TANConvolution::Terminate (ConvolutionImpl.cpp:307-355). */
 int32_t tanx_conv_destroy(tanx_conv *conv);
@@ -186,7 +197,7 @@ typedef struct tanx_conv_info {
     uint64_t ticks;        /* blocks processed */
     uint64_t crossfades;   /* crossfade blocks processed */
     uint32_t ir_precision; /* TANX_IR_* */
-    uint32_t reserved;
+    uint32_t nu_multiple;  /* blocks per frame of non-uniform objects, 1 = uniform */
 } tanx_conv_info;
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */

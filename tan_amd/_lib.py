@@ -23,11 +23,12 @@ class ConvInfo(ctypes.Structure):
                 ("tail_chunks", ctypes.c_uint32), ("finalize_mode", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64),
                 ("crossfades", ctypes.c_uint64), ("ir_precision", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("nu_multiple", ctypes.c_uint32)]
 
 
 class ConvOptions(ctypes.Structure):
-    _fields_ = [("ir_precision", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+    _fields_ = [("ir_precision", ctypes.c_uint32), ("nu_multiple", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
 
 
 class KernelTiming(ctypes.Structure):
@@ -44,7 +45,7 @@ _SIGS = {
     "tanx_conv_create": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
     "tanx_conv_create_ex": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "tanx_conv_destroy": (_i32, [_vp]),
-    "tanx_conv_estimate_bytes": (_i32, [_u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "tanx_conv_estimate_bytes": (_i32, [_u32, _u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "tanx_conv_update_td": (_i32, [_vp, ctypes.POINTER(_f32p), _sz, _u32p, _u32]),
     "tanx_conv_update_td_device": (_i32, [_vp, _vp, _sz, _sz, _u32p, _u32]),
     "tanx_conv_process": (_i32, [_vp, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p), _sz, _u32p,

@@ -100,7 +100,8 @@ class StreamBatch:
 
     specs: list of (ir_len, block[, channels]) per stream."""
 
-    def __init__(self, ctx, specs, ir_precision=T.TAN_IR_PRECISION_FP32, device="cuda", ir_seed=11):
+    def __init__(self, ctx, specs, ir_precision=T.TAN_IR_PRECISION_FP32, device="cuda", ir_seed=11,
+                 method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED):
         import torch
 
         self.ctx = ctx
@@ -120,7 +121,7 @@ class StreamBatch:
         gen.manual_seed(ir_seed)
         for j, (ln, b, c) in enumerate(self.specs):
             conv = T.TANConvolution(ctx, ir_precision)
-            rc = conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, ln, b, c)
+            rc = conv.InitGpu(method, ln, b, c)
             if rc != T.AMF_OK:
                 raise RuntimeError(f"InitGpu(L={ln}, B={b}, C={c}) failed: AMF_RESULT {rc}")
             # decaying-noise IR (exp(-t / 0.25 s)), unit L2, generated on the device

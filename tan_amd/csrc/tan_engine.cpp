@@ -252,7 +252,9 @@ struct tanx_conv {
     KernelTimer t_head, t_tail;
 
     // graph replay of device-memory blocks
-    bool use_graph = true;
+    // opt-in: on this runtime a 64-node graph launch costs more host time
+    // than 64 direct launches (about 22 us against 3 us per block)
+    bool use_graph = false;
     struct GraphEntry {
         int k;
         unsigned blocks;
@@ -264,6 +266,27 @@ struct tanx_conv {
     hipEvent_t desc_ev[kDescRing] = {};
     int desc_ring = 0;
     uint64_t tail_alg_bytes = 0, head_alg_bytes = 0;
+
+    // non-uniform objects: frames of M blocks (N = M * B samples); the
+    // uniform fields above then describe the first frame of taps (P = M)
+    bool nu = false;
+    int M = 1, N = 0, PB = 0, RB = 0;
+    int nchB = 0, pcB = 1;
+    char *HBm = nullptr;     // [3][C][PB][N] frame-partition spectra of irtype
+    float *SBm = nullptr;    // [3][C][PB] row scales
+    float2 *FB = nullptr;    // [C][RB][N] frame spectra ring
+    float *XR = nullptr;     // [C][2M][B] input ring
+    float2 *AC = nullptr;    // [2 parity][2 filter][C][nchB][N] frame partial sums
+    float *ZB = nullptr;     // [2 parity][2 filter][C][N] frame outputs
+    uint64_t switch_at = 0;  // block index of the armed crossfade
+    long long zsel_frame = -1;  // frame whose current-filter output is in ZB[.][1]
+
+    size_t brow_bytes() const { return (size_t)N * ir_elem_bytes(irtype); }
+    size_t bslot_bytes() const { return (size_t)C * PB * brow_bytes(); }
+    void *bslot_ptr(int s) const { return HBm + (size_t)s * bslot_bytes(); }
+    float *bscale_ptr(int s) const { return SBm + (size_t)s * C * PB; }
+    float2 *acc_ptr(int par, int f) const { return AC + ((size_t)par * 2 + f) * C * nchB * N; }
+    float *zb_ptr(int par, int f) const { return ZB + ((size_t)par * 2 + f) * C * N; }
 
     size_t row_bytes() const { return (size_t)B * ir_elem_bytes(irtype); }
     size_t slot_bytes() const { return (size_t)C * P * row_bytes(); }
@@ -319,30 +342,137 @@ static void set_chunks(tanx_conv *cv, int n)
     cv->nchunk = (parts + cv->pc - 1) / cv->pc;
 }
 
-// device bytes an object of this shape holds (the single allocation of
-// tanx_conv_create_ex, before any host-IR staging)
-static size_t conv_device_bytes(uint32_t len, uint32_t block, uint32_t channels, int irtype, int maxchunk)
-{
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t C = channels, B = block, P = len / block;
-    return al(3 * C * P * B * (size_t)ir_elem_bytes(irtype)) + al(3 * C * P * sizeof(float)) +
-           al(C * P * B * sizeof(float2)) + al(4 * C * (size_t)maxchunk * B * sizeof(float2)) +
-           3 * al(C * B * sizeof(float)) + al(C * sizeof(int)) + al(C) + sizeof(TickArgs) * kGraphBlocks;
-}
-
 static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(kMaxChunks, P - 2), 16384 / C)); }
 
-extern "C" int32_t tanx_conv_estimate_bytes(uint32_t ir_len, uint32_t block, uint32_t channels,
-                                            const tanx_conv_options *opt, uint64_t *bytes)
+// Shape of an object: uniform partitions of B over the first `head` taps
+// (P = head / B), then (non-uniform objects) PB frame partitions of N = M * B.
+struct ConvShape {
+    int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1;
+    int irtype = IR_F32;
+};
+
+static bool is_nu_method(uint32_t base)
 {
-    if (!bytes) return TANX_INVALID_POINTER;
-    if (ir_len == 0 || block == 0 || channels == 0 || (block & (block - 1))) return TANX_INVALID_ARG;
+    return base == TANX_METHOD_FFT_PARTITIONED_NONUNIFORM || base == TANX_METHOD_FHT_NONUNIFORM_PARTITIONED ||
+           base == TANX_METHOD_FFT_NONUNIFORM_PARTITIONED;
+}
+
+// Frame chunks: enough workgroups at small channel counts, as for the tail.
+static int frame_chunks(int C, int PB) { return std::max(0, std::min(std::min(16, (1024 + C - 1) / C), PB - 2)); }
+
+// Modeled HBM bytes per block and channel of a frame multiple M (M = 1:
+// uniform).  The reference picks its multiple from a CPU cost model of FFT
+// and multiply-accumulate time (bestNUMultiple, ConvolutionImpl.cpp:201-246);
+// on this device the multiply-accumulate streams HBM, so the model counts
+// bytes: the first frame of taps as uniform partitions, the frame partitions
+// 2..PB-1 once per frame (one slice per block), and the frame kernel's
+// traffic spread over the frame's M blocks.
+static double nu_model_bytes(int len, int B, int C, int M, int elem)
+{
+    const double row = 8.0 + elem;  // FDL + IR bytes per bin
+    if (M == 1) return row * B * (len / B);
+    const int N = M * B, PB = len / N - 1;
+    const int nch = frame_chunks(C, PB);
+    const double head = row * B * M;
+    const double mac = PB > 2 ? row * B * (PB - 2) + 8.0 * B * nch : 0.0;
+    const double frame = 2.0 * N * 4 + 2.0 * N * 8 + 2.0 * N * elem + (double)nch * N * 8 + 4.0 * N;
+    return head + mac + frame / M;
+}
+
+static int choose_multiple(int len, int B, int C, int elem)
+{
+    int best = 1;
+    double bytes = nu_model_bytes(len, B, C, 1, elem);
+    for (int M = 2; M * B <= kMaxFrame && 2 * M * B <= len; M *= 2) {
+        const double b = nu_model_bytes(len, B, C, M, elem);
+        if (b < bytes) {
+            bytes = b;
+            best = M;
+        }
+    }
+    return best;
+}
+
+static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint32_t channels,
+                          const tanx_conv_options *opt, ConvShape &sh)
+{
+    const uint32_t prec = opt ? opt->ir_precision : TANX_IR_FP32;
+    if (prec > TANX_IR_BF16) return TANX_INVALID_ARG;
+    if (ir_len == 0 || block == 0 || channels == 0) return TANX_INVALID_ARG;
+    if (!is_pow2(block) || block < 64 || block > 1024) return TANX_INVALID_ARG;
     uint32_t len = 1;
     while (len < ir_len) len <<= 1;
     if (len < block) len = block;
-    const uint32_t prec = opt ? opt->ir_precision : TANX_IR_FP32;
-    const int irtype = prec == TANX_IR_FP16 ? IR_F16 : prec == TANX_IR_BF16 ? IR_BF16 : IR_F32;
-    *bytes = conv_device_bytes(len, block, channels, irtype, max_chunks_for((int)channels, (int)(len / block)));
+    if (len > (1u << 24)) return TANX_INVALID_ARG;
+    sh.len = (int)len;
+    sh.B = (int)block;
+    sh.C = (int)channels;
+    sh.irtype = prec == TANX_IR_FP16 ? IR_F16 : prec == TANX_IR_BF16 ? IR_BF16 : IR_F32;
+    const uint32_t base = method & ~uint32_t(TANX_METHOD_USE_PROCESS_TAILTHREAD);
+    int M = 1;
+    if (is_nu_method(base)) {
+        const uint32_t want = opt ? opt->nu_multiple : 0;
+        if (want == 0) {
+            M = choose_multiple(sh.len, sh.B, sh.C, ir_elem_bytes(sh.irtype));
+        } else if (want > 1) {
+            if (!is_pow2(want) || (uint64_t)want * block > (uint64_t)kMaxFrame || 2ull * want * block > len)
+                return TANX_INVALID_ARG;
+            M = (int)want;
+        }
+    }
+    sh.M = M;
+    if (M > 1) {
+        sh.N = M * sh.B;
+        sh.P = M;
+        sh.RB = sh.len / sh.N;
+        sh.PB = sh.RB - 1;
+        sh.nchB = frame_chunks(sh.C, sh.PB);
+        sh.pcB = sh.nchB > 0 ? (sh.PB - 2 + sh.nchB - 1) / sh.nchB : 1;
+        if (sh.nchB > 0) sh.nchB = (sh.PB - 2 + sh.pcB - 1) / sh.pcB;
+    } else {
+        sh.P = sh.len / sh.B;
+    }
+    sh.maxchunk = max_chunks_for(sh.C, sh.P);
+    return TANX_OK;
+}
+
+// Sizes of the pieces of an object's single device allocation, in order.
+struct ConvLayout {
+    size_t H, S, F, T, X, Ch, Run, HB, SB, FB, XR, AC, ZB;
+    size_t total() const { return H + S + F + T + 3 * X + Ch + Run + HB + SB + FB + XR + AC + ZB; }
+    size_t zeroed() const { return total() - Ch - Run; }
+};
+
+static ConvLayout conv_layout(const ConvShape &sh)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t C = sh.C, B = sh.B, P = sh.P, N = sh.N, PB = sh.PB, elem = ir_elem_bytes(sh.irtype);
+    ConvLayout l{};
+    l.H = al(3 * C * P * B * elem);
+    l.S = al(3 * C * P * sizeof(float));
+    l.F = al(C * P * B * sizeof(float2));
+    l.T = al(4 * C * (size_t)sh.maxchunk * B * sizeof(float2));
+    l.X = al(C * B * sizeof(float));
+    l.HB = al(3 * C * PB * N * elem);
+    l.SB = al(3 * C * PB * sizeof(float));
+    l.FB = al(C * (size_t)sh.RB * N * sizeof(float2));
+    l.XR = al(C * 2 * N * sizeof(float));
+    l.AC = al(4 * C * (size_t)sh.nchB * N * sizeof(float2));
+    l.ZB = al(4 * C * N * sizeof(float));
+    // channel list and run mask last: they are not cleared with the state
+    l.Ch = al(C * sizeof(int));
+    l.Run = al(C);
+    return l;
+}
+
+extern "C" int32_t tanx_conv_estimate_bytes(uint32_t method, uint32_t ir_len, uint32_t block, uint32_t channels,
+                                            const tanx_conv_options *opt, uint64_t *bytes)
+{
+    if (!bytes) return TANX_INVALID_POINTER;
+    ConvShape sh;
+    const int32_t rc = conv_shape(method, ir_len, block, channels, opt, sh);
+    if (rc != TANX_OK) return rc;
+    *bytes = conv_layout(sh).total() + sizeof(TickArgs) * kGraphBlocks;
     return TANX_OK;
 }
 
@@ -357,10 +487,6 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
 {
     if (!ctx || !out) return TANX_INVALID_POINTER;
     *out = nullptr;
-    const uint32_t prec = opt ? opt->ir_precision : TANX_IR_FP32;
-    if (prec > TANX_IR_BF16) return TANX_INVALID_ARG;
-    if (ir_len == 0 || block == 0 || channels == 0) return TANX_INVALID_ARG;
-    if (!is_pow2(block) || block < 64 || block > 1024) return TANX_INVALID_ARG;
     const uint32_t base = method & ~uint32_t(TANX_METHOD_USE_PROCESS_TAILTHREAD);
     switch (base) {
     case TANX_METHOD_FFT_OVERLAP_ADD:
@@ -369,14 +495,16 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     case TANX_METHOD_FHT_UNIFORM_PARTITIONED:
     case TANX_METHOD_FFT_UNIFORM_HEAD_TAIL:
     case TANX_METHOD_FHT_UNIFORM_HEAD_TAIL:
+    case TANX_METHOD_FFT_PARTITIONED_NONUNIFORM:
+    case TANX_METHOD_FHT_NONUNIFORM_PARTITIONED:
+    case TANX_METHOD_FFT_NONUNIFORM_PARTITIONED:
         break;
     default:
         return TANX_NOT_SUPPORTED;
     }
-    uint32_t len = 1;
-    while (len < ir_len) len <<= 1;
-    if (len < block) len = block;
-    if (len > (1u << 24)) return TANX_INVALID_ARG;
+    ConvShape sh;
+    int32_t rc = conv_shape(method, ir_len, block, channels, opt, sh);
+    if (rc != TANX_OK) return rc;
 
     DeviceGuard g(ctx->device);
     if (!g.ok) return TANX_NO_DEVICE;
@@ -385,41 +513,47 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->ctx = ctx;
     cv->method = method;
     cv->fmode = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
-    cv->irtype = prec == TANX_IR_FP16 ? IR_F16 : prec == TANX_IR_BF16 ? IR_BF16 : IR_F32;
+    cv->irtype = sh.irtype;
     cv->L = (int)ir_len;
-    cv->B = (int)block;
-    cv->C = (int)channels;
-    cv->len = (int)len;
-    cv->P = (int)(len / block);
+    cv->B = sh.B;
+    cv->C = sh.C;
+    cv->len = sh.len;
+    cv->P = sh.P;
+    cv->nu = sh.M > 1;
+    cv->M = sh.M;
+    cv->N = sh.N;
+    cv->PB = sh.PB;
+    cv->RB = sh.RB;
+    cv->nchB = sh.nchB;
+    cv->pcB = sh.pcB;
     // partial-sum rows are only worth it at small channel counts
-    cv->maxchunk = max_chunks_for(cv->C, cv->P);
+    cv->maxchunk = sh.maxchunk;
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
 
-    const size_t C = cv->C, P = cv->P, B = cv->B;
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t bH = al(3 * C * P * B * (size_t)ir_elem_bytes(cv->irtype));
-    const size_t bS = al(3 * C * P * sizeof(float));
-    const size_t bF = al(C * P * B * sizeof(float2));
-    const size_t bT = al(4 * C * (size_t)cv->maxchunk * B * sizeof(float2));
-    const size_t bX = al(C * B * sizeof(float));
-    const size_t bCh = al(C * sizeof(int));
-    const size_t bRun = al(C);
-    cv->dbytes = bH + bS + bF + bT + 3 * bX + bCh + bRun;
-    int32_t rc = TANX_OK;
+    const size_t C = cv->C, B = cv->B;
+    const ConvLayout l = conv_layout(sh);
+    const size_t bX = l.X;
+    cv->dbytes = l.total();
     auto fail = [&](int32_t r) { conv_free(cv); return r; };
     hipError_t e = hipMalloc(&cv->dmem, cv->dbytes);
     if (e != hipSuccess) { cv->dmem = nullptr; return fail(map_hip(e)); }
     char *p = static_cast<char *>(cv->dmem);
-    cv->H = p; p += bH;
-    cv->S = reinterpret_cast<float *>(p); p += bS;
-    cv->F = reinterpret_cast<float2 *>(p); p += bF;
-    cv->T = reinterpret_cast<float2 *>(p); p += bT;
+    cv->H = p; p += l.H;
+    cv->S = reinterpret_cast<float *>(p); p += l.S;
+    cv->F = reinterpret_cast<float2 *>(p); p += l.F;
+    cv->T = reinterpret_cast<float2 *>(p); p += l.T;
     cv->xprev = reinterpret_cast<float *>(p); p += bX;
     cv->din = reinterpret_cast<float *>(p); p += bX;
     cv->dout = reinterpret_cast<float *>(p); p += bX;
-    cv->dchan = reinterpret_cast<int *>(p); p += bCh;
-    cv->drun = reinterpret_cast<uint8_t *>(p); p += bRun;
+    cv->HBm = p; p += l.HB;
+    cv->SBm = reinterpret_cast<float *>(p); p += l.SB;
+    cv->FB = reinterpret_cast<float2 *>(p); p += l.FB;
+    cv->XR = reinterpret_cast<float *>(p); p += l.XR;
+    cv->AC = reinterpret_cast<float2 *>(p); p += l.AC;
+    cv->ZB = reinterpret_cast<float *>(p); p += l.ZB;
+    cv->dchan = reinterpret_cast<int *>(p); p += l.Ch;
+    cv->drun = reinterpret_cast<uint8_t *>(p); p += l.Run;
 
     if (hipHostMalloc(&cv->hin, C * B * sizeof(float)) != hipSuccess ||
         hipHostMalloc(&cv->hout, C * B * sizeof(float)) != hipSuccess ||
@@ -442,7 +576,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
 
     // all state starts at zero: the initial IR slot is silent
     // (ConvolutionImpl.cpp:115-117, 1640-1650)
-    e = launch_zero(cv->dmem, bH + bS + bF + bT + 3 * bX, ctx->proc);
+    e = launch_zero(cv->dmem, l.zeroed(), ctx->proc);
     if (e == hipSuccess) {
         std::memset(cv->hrun, 1, C);
         e = hipMemcpyAsync(cv->drun, cv->hrun, C, hipMemcpyHostToDevice, ctx->proc);
@@ -458,7 +592,6 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->run_now.assign(C, 1);
     cv->run_dev.assign(C, 1);
     cv->tail_left.assign(C, 0);
-    (void)rc;
     *out = cv;
     return TANX_OK;
 }
@@ -481,6 +614,19 @@ static int32_t flush_channel(tanx_conv *cv, int ch)
     for (int f = 0; f < 2 && e == hipSuccess; f++)
         e = launch_zero(cv->tail_ptr(cv->tread, f) + (size_t)ch * cv->tail_nchunk * B,
                         (size_t)cv->tail_nchunk * B * sizeof(float2), s);
+    if (cv->nu) {
+        // frame ring, input ring, frame partials and outputs of the channel
+        const size_t N = cv->N;
+        if (e == hipSuccess) e = launch_zero(cv->FB + (size_t)ch * cv->RB * N, (size_t)cv->RB * N * sizeof(float2), s);
+        if (e == hipSuccess) e = launch_zero(cv->XR + (size_t)ch * 2 * N, 2 * N * sizeof(float), s);
+        for (int par = 0; par < 2; par++)
+            for (int f = 0; f < 2; f++) {
+                if (e == hipSuccess && cv->nchB > 0)
+                    e = launch_zero(cv->acc_ptr(par, f) + (size_t)ch * cv->nchB * N,
+                                    (size_t)cv->nchB * N * sizeof(float2), s);
+                if (e == hipSuccess) e = launch_zero(cv->zb_ptr(par, f) + (size_t)ch * N, N * sizeof(float), s);
+            }
+    }
     return map_hip(e);
 }
 
@@ -565,7 +711,8 @@ static TickArgs tick_common(tanx_conv *cv)
 // 2..P-1, with both filters when that block is the crossfade block.
 static int fill_tail(tanx_conv *cv, TickArgs &a)
 {
-    const int nf = cv->delayed ? 2 : 1;
+    // cv->ticks is the index of the block this tail is for
+    const int nf = (cv->delayed && cv->ticks == cv->switch_at) ? 2 : 1;
     a.nchunk = cv->nchunk;
     a.pc = cv->pc;
     a.tbase = cv->pos;  // pos already advanced past the current block
@@ -594,7 +741,7 @@ static void tail_done(tanx_conv *cv)
 }
 
 static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const TickArgs &a,
-                            uint64_t alg_bytes, uint64_t &alg_sum)
+                            uint64_t alg_bytes, uint64_t &alg_sum, const FrameArgs *fa = nullptr)
 {
     hipStream_t s = cv->ctx->proc;
     if (cv->timing) {
@@ -602,6 +749,7 @@ static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const T
         TANX_CHECK(hipEventRecord(kt.start[kt.n], s));
     }
     TANX_CHECK(launch_tick(cv->B, tail_nf, a, s));
+    if (fa) TANX_CHECK(launch_frame(cv->irtype, *fa, s));
     if (cv->timing) {
         TANX_CHECK(hipEventRecord(kt.stop[kt.n], s));
         kt.n++;
@@ -625,6 +773,25 @@ static uint64_t tail_bytes(const tanx_conv *cv, int nf)
     return (uint64_t)cv->C * parts * (cv->B * (sizeof(float2) + (uint64_t)ir_elem_bytes(cv->irtype) * nf) + scales);
 }
 
+// frame roles of a non-uniform block: the multiply-accumulate slice (nf
+// filters) and, at the end of a frame, the frame kernel
+static uint64_t frame_mac_bytes(const tanx_conv *cv, int nf)
+{
+    if (!cv->nu || cv->nchB == 0) return 0;
+    const uint64_t scales = cv->irtype == IR_F32 ? 0 : sizeof(float) * nf;
+    return (uint64_t)cv->C * (cv->PB - 2) * (cv->B * (sizeof(float2) + (uint64_t)ir_elem_bytes(cv->irtype) * nf) + scales) +
+           (uint64_t)cv->C * cv->nchB * cv->B * sizeof(float2) * nf;
+}
+static uint64_t frame_kernel_bytes(const tanx_conv *cv, int nf)
+{
+    // 2N input samples, X_g written, X_{g-1} read, per filter two spectrum
+    // rows, the partials and the N output samples
+    const uint64_t N = cv->N;
+    return (uint64_t)cv->C * (2 * N * sizeof(float) + 2 * N * sizeof(float2) +
+                              nf * (2 * N * (uint64_t)ir_elem_bytes(cv->irtype) + cv->nchB * N * sizeof(float2) +
+                                    N * sizeof(float)));
+}
+
 // ProcessFinalize: the tail for the next block as its own launch.
 static int32_t enqueue_tail(tanx_conv *cv)
 {
@@ -643,9 +810,90 @@ static int32_t enqueue_tail(tanx_conv *cv)
 // bookkeeping, switch schedule, slot rotation, and the launch arguments of
 // the block's head (and, with_tail, the next block's tail).  Sets *tail_nf
 // and *needs_update when the launch reads a freshly transformed IR slot.
+// Frame roles of block b of a non-uniform object (after this block's slot
+// rotation).  The slice for frame gm = (b+1)/M feeds blocks (gm+1)M.. and
+// uses the filters in effect there: both while a switch at (gm+1)M is
+// armed, the armed one alone at the block before an armed switch (gm's
+// blocks all follow it), else the current one.  The frame kernel at the last
+// block of frame g feeds block b+1.  Returns whether the update slot is read.
+static bool nu_block_args(tanx_conv *cv, uint64_t b, bool xfade, TickArgs &a, FrameArgs &fa, bool *frame_due)
+{
+    const uint64_t M = (uint64_t)cv->M;
+    bool reads_upd = false;
+    a.nu = 1;
+    a.M = cv->M;
+    a.N = cv->N;
+    a.xring = cv->XR;
+    a.xslot = (int)(b % (2 * M));
+    a.zoff = (int)(b % M) * cv->B;
+    const long long fo = (long long)(b / M) - 1;  // frame whose output covers block b
+    a.zA = nullptr;
+    a.zB = nullptr;
+    if (fo >= 0) {
+        const int par = (int)(fo & 1);
+        a.zA = cv->zb_ptr(par, (!xfade && fo == cv->zsel_frame) ? 1 : 0);
+        a.zB = xfade ? cv->zb_ptr(par, 1) : nullptr;
+    }
+    a.PB = cv->PB;
+    a.RB = cv->RB;
+    a.fb = cv->FB;
+    if (cv->nchB > 0) {
+        const uint64_t gm = (b + 1) / M;
+        a.nchB = cv->nchB;
+        a.pcB = cv->pcB;
+        a.fbase = (int)(gm & (uint64_t)(cv->RB - 1));
+        a.bslice = (int)((b + 1) % M);
+        const bool both = cv->delayed && cv->switch_at == (gm + 1) * M;
+        const bool next_only = !both && cv->delayed && cv->switch_at == b + 1;
+        const int fa0 = next_only ? cv->upd : cv->cur;
+        a.bHa = cv->bslot_ptr(fa0);
+        a.bSa = cv->bscale_ptr(fa0);
+        a.bHb = both ? cv->bslot_ptr(cv->upd) : nullptr;
+        a.bSb = both ? cv->bscale_ptr(cv->upd) : nullptr;
+        a.bTa = cv->acc_ptr((int)(gm & 1), 0);
+        a.bTb = both ? cv->acc_ptr((int)(gm & 1), 1) : nullptr;
+        reads_upd = both || next_only;
+    }
+    *frame_due = ((b + 1) % M) == 0;
+    if (*frame_due) {
+        const uint64_t g = (b + 1) / M - 1;
+        const bool two = cv->delayed && cv->switch_at == b + 1;
+        fa = FrameArgs{};
+        fa.B = cv->B;
+        fa.M = cv->M;
+        fa.N = cv->N;
+        fa.C = cv->C;
+        fa.PB = cv->PB;
+        fa.RB = cv->RB;
+        fa.run = cv->drun;
+        fa.xring = cv->XR;
+        fa.xlast = (int)(b % (2 * M));
+        fa.fb = cv->FB;
+        fa.frow = (int)(g & (uint64_t)(cv->RB - 1));
+        fa.Ha = cv->bslot_ptr(cv->cur);
+        fa.Sa = cv->bscale_ptr(cv->cur);
+        fa.Hb = two ? cv->bslot_ptr(cv->upd) : nullptr;
+        fa.Sb = two ? cv->bscale_ptr(cv->upd) : nullptr;
+        fa.nch = cv->nchB;
+        fa.Ta = cv->nchB ? cv->acc_ptr((int)(g & 1), 0) : nullptr;
+        fa.Tb = (two && cv->nchB) ? cv->acc_ptr((int)(g & 1), 1) : nullptr;
+        fa.za = cv->zb_ptr((int)(g & 1), 0);
+        fa.zb = two ? cv->zb_ptr((int)(g & 1), 1) : nullptr;
+        const int lN = __builtin_ctz((unsigned)cv->N);
+        fa.twN = tw_table(cv->ctx->tw, lN);
+        fa.tw2N = tw_table(cv->ctx->tw, lN + 1);
+        if (two) {
+            cv->zsel_frame = (long long)g;
+            reads_upd = true;
+        }
+    }
+    return reads_upd;
+}
+
 static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
                              long long out_stride, size_t n, const uint32_t *flags, bool with_tail,
-                             TickArgs &a, int *tail_nf, bool *needs_update, bool *xfade_out)
+                             TickArgs &a, int *tail_nf, bool *needs_update, bool *xfade_out,
+                             FrameArgs *fa = nullptr, bool *frame_due = nullptr)
 {
     int nrun = 0;
     int32_t rc = channel_flags(cv, flags, n, &nrun);
@@ -654,12 +902,17 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     rc = sync_run_mask(cv);
     if (rc != TANX_OK) return rc;
 
+    const uint64_t b = cv->ticks;
     bool xfade = false;
     if (!cv->pending && cv->ready && !cv->delayed) {
         cv->ready = false;
         cv->delayed = true;
-    } else {
-        xfade = cv->delayed;
+        // uniform: the next block; non-uniform: a frame boundary late enough
+        // for the armed filter's frame partials (M + 1 blocks ahead)
+        const uint64_t M = (uint64_t)cv->M;
+        cv->switch_at = cv->nu ? ((b + M + 1 + M - 1) / M) * M : b + 1;
+    } else if (cv->delayed && b == cv->switch_at) {
+        xfade = true;
     }
     if (xfade) {
         cv->prev = cv->cur;
@@ -690,13 +943,19 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
         cv->delayed = false;
         cv->xfades++;
     }
+    bool nu_upd = false, fdue = false;
+    if (cv->nu) {
+        FrameArgs tmp;
+        nu_upd = nu_block_args(cv, b, xfade, a, fa ? *fa : tmp, &fdue);
+    }
+    if (frame_due) *frame_due = fdue;
     cv->ticks++;
     cv->tail_due = true;
 
     int nf = 1;
     if (with_tail) nf = fill_tail(cv, a);
     *tail_nf = nf;
-    *needs_update = xfade || nf == 2;
+    *needs_update = xfade || nf == 2 || nu_upd;
     *xfade_out = xfade;
     return TANX_OK;
 }
@@ -707,19 +966,22 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
                              long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
 {
     TickArgs a;
+    FrameArgs fa;
     int tail_nf = 1;
-    bool needs_update = false, xfade = false;
+    bool needs_update = false, xfade = false, frame_due = false;
     int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, with_tail, a, &tail_nf,
-                               &needs_update, &xfade);
+                               &needs_update, &xfade, &fa, &frame_due);
     if (rc != TANX_OK) return rc;
     if (needs_update) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
-    const uint64_t hb = head_bytes(cv, xfade ? 2 : 1);
+    uint64_t hb = head_bytes(cv, xfade ? 2 : 1) + frame_mac_bytes(cv, a.bHb ? 2 : 1);
+    if (frame_due) hb += frame_kernel_bytes(cv, fa.Hb ? 2 : 1);
+    const FrameArgs *fp = frame_due ? &fa : nullptr;
     if (with_tail) {
-        rc = launch_timed(cv, cv->t_head, tail_nf, a, hb + tail_bytes(cv, tail_nf), cv->head_alg_bytes);
+        rc = launch_timed(cv, cv->t_head, tail_nf, a, hb + tail_bytes(cv, tail_nf), cv->head_alg_bytes, fp);
         if (rc != TANX_OK) return rc;
         tail_done(cv);
     } else {
-        rc = launch_timed(cv, cv->t_head, 1, a, hb, cv->head_alg_bytes);
+        rc = launch_timed(cv, cv->t_head, 1, a, hb, cv->head_alg_bytes, fp);
         if (rc != TANX_OK) return rc;
     }
     return TANX_OK;
@@ -848,7 +1110,7 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (nb == 0) return TANX_OK;
     if (in_stride < nb * B || out_stride < nb * B) return TANX_INVALID_ARG;
     DeviceGuard g(cv->ctx->device);
-    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !flags && nb >= 2)
+    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !cv->nu && !flags && nb >= 2)
         return process_device_graph(cv, d_in, in_stride, d_out, out_stride, nb, processed);
     for (size_t b = 0; b < nb; b++) {
         const bool fuse = cv->fused && !cv->fmode;
@@ -946,6 +1208,19 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         a.twM = tw_table(ctx->tw, lB);
         a.twN = tw_table(ctx->tw, lB + 1);
         TANX_CHECK(launch_ir_transform(cv->B, nupd, a, ctx->upd));
+        if (cv->nu) {
+            // taps [N, len): frame partitions of N
+            IrArgs b = a;
+            b.ir = src + cv->N;
+            b.n = (long long)n > cv->N ? (long long)n - cv->N : 0;
+            b.H = cv->bslot_ptr(target);
+            b.S = cv->bscale_ptr(target);
+            b.P = cv->PB;
+            const int lN = __builtin_ctz((unsigned)cv->N);
+            b.twM = tw_table(ctx->tw, lN);
+            b.twN = tw_table(ctx->tw, lN + 1);
+            TANX_CHECK(launch_ir_transform(cv->N, nupd, b, ctx->upd));
+        }
         for (int k = 0; k < nupd; k++) cv->staged[cv->hchan[k]] = 1;
     }
     // channels not updated keep the IR that is current when the update is
@@ -956,6 +1231,15 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
                                   hipMemcpyDeviceToDevice, ctx->upd));
         TANX_CHECK(hipMemcpyAsync(cv->scale_ptr(target) + (size_t)ch * cv->P, cv->scale_ptr(keep) + (size_t)ch * cv->P,
                                   (size_t)cv->P * sizeof(float), hipMemcpyDeviceToDevice, ctx->upd));
+        if (cv->nu) {
+            const size_t cb = (size_t)cv->PB * cv->brow_bytes();
+            TANX_CHECK(hipMemcpyAsync(static_cast<char *>(cv->bslot_ptr(target)) + (size_t)ch * cb,
+                                      static_cast<char *>(cv->bslot_ptr(keep)) + (size_t)ch * cb, cb,
+                                      hipMemcpyDeviceToDevice, ctx->upd));
+            TANX_CHECK(hipMemcpyAsync(cv->bscale_ptr(target) + (size_t)ch * cv->PB,
+                                      cv->bscale_ptr(keep) + (size_t)ch * cv->PB, (size_t)cv->PB * sizeof(float),
+                                      hipMemcpyDeviceToDevice, ctx->upd));
+        }
     }
     TANX_CHECK(hipEventRecord(cv->ev_upd, ctx->upd));
     cv->pending = true;
@@ -1021,36 +1305,52 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     for (tanx_conv *cv : order) locks.emplace_back(cv->mtx);
     DeviceGuard g(ctx->device);
 
-    const size_t desc_bytes = sizeof(TickArgs) * n, start_bytes = sizeof(int) * (n + 1);
+    // staging: tick descriptors, their block offsets, frame descriptors and
+    // theirs (every piece 16-byte aligned)
+    auto al16 = [](size_t b) { return (b + 15) & ~size_t(15); };
+    const size_t desc_bytes = al16(sizeof(TickArgs) * n), start_bytes = al16(sizeof(int) * n);
+    const size_t fdesc_bytes = al16(sizeof(FrameArgs) * n), fstart_bytes = al16(sizeof(int) * n);
     char *host = nullptr;
     int slot = 0;
-    int32_t rc = group_staging(ctx, desc_bytes + start_bytes, &host, &slot);
+    int32_t rc = group_staging(ctx, desc_bytes + start_bytes + fdesc_bytes + fstart_bytes, &host, &slot);
     if (rc != TANX_OK) return rc;
     TickArgs *descs = reinterpret_cast<TickArgs *>(host);
     int *starts = reinterpret_cast<int *>(host + desc_bytes);
+    FrameArgs *fdescs = reinterpret_cast<FrameArgs *>(host + desc_bytes + start_bytes);
+    int *fstarts = reinterpret_cast<int *>(host + desc_bytes + start_bytes + fdesc_bytes);
     uint32_t done = 0;
-    int total = 0;
+    int total = 0, nframe = 0, ftotal = 0;
     for (; done < n; done++) {
         tanx_conv *cv = convs[done];
         int nf = 1;
-        bool nu = false, xf = false;
+        bool upd = false, xf = false, fdue = false;
         rc = prepare_block(cv, d_in[done], (long long)in_stride[done], d_out[done], (long long)out_stride[done],
-                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &nu, &xf);
+                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &upd, &xf,
+                           &fdescs[nframe], &fdue);
         if (rc != TANX_OK) break;
-        if (nu) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
+        if (upd) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
         tail_done(cv);
         starts[done] = total;
-        total += cv->C + cv->C * descs[done].nchunk;
+        total += cv->C + cv->C * descs[done].nchunk + cv->C * descs[done].nchB;
+        if (fdue) {
+            fstarts[nframe++] = ftotal;
+            ftotal += cv->C;
+        }
     }
     if (done > 0) {
-        // objects already advanced are launched even if a later one failed
-        const size_t nb = sizeof(TickArgs) * done;
-        std::memmove(host + nb, starts, sizeof(int) * done);
-        TANX_CHECK(hipMemcpyAsync(ctx->g_dev, host, nb + sizeof(int) * done, hipMemcpyHostToDevice, ctx->proc));
+        // objects already advanced are launched even if a later one failed;
+        // the staging pieces are copied in one transfer at their offsets
+        const size_t bytes = desc_bytes + start_bytes + fdesc_bytes + fstart_bytes;
+        TANX_CHECK(hipMemcpyAsync(ctx->g_dev, host, bytes, hipMemcpyHostToDevice, ctx->proc));
         TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
         TANX_CHECK(launch_group_tick(irtype, reinterpret_cast<const TickArgs *>(ctx->g_dev),
-                                     reinterpret_cast<const int *>(ctx->g_dev + nb), (int)done, (unsigned)total,
-                                     ctx->proc));
+                                     reinterpret_cast<const int *>(ctx->g_dev + desc_bytes), (int)done,
+                                     (unsigned)total, ctx->proc));
+        if (nframe > 0)
+            TANX_CHECK(launch_group_frame(
+                irtype, reinterpret_cast<const FrameArgs *>(ctx->g_dev + desc_bytes + start_bytes),
+                reinterpret_cast<const int *>(ctx->g_dev + desc_bytes + start_bytes + fdesc_bytes), nframe,
+                (unsigned)ftotal, ctx->proc));
     } else {
         TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
     }
@@ -1124,6 +1424,7 @@ extern "C" int32_t tanx_conv_get_info(const tanx_conv *cv, tanx_conv_info *info)
     info->device_bytes = cv->dbytes;
     info->ticks = cv->ticks;
     info->crossfades = cv->xfades;
+    info->nu_multiple = (uint32_t)cv->M;
     return TANX_OK;
 }
 

@@ -17,8 +17,21 @@ namespace tanx {
 //     insert, A_f = sum of tail partials + X_{t-1} H_{f,1} + X_t H_{f,0},
 //     C2R, crossfade, output;
 //   tail (blocks [nhead, nhead + C * nchunk)): partial sums over partitions
-//     [2 + j*pc, 2 + (j+1)*pc) of X_{t+1-i} H_i for the NEXT block.
-// The two roles touch disjoint data, so they may share a launch.
+//     [2 + j*pc, 2 + (j+1)*pc) of X_{t+1-i} H_i for the NEXT block;
+//   frame multiply-accumulate (blocks after those, non-uniform objects only,
+//     C * nchB of them): bins [j*B, (j+1)*B) of the frame segment's partial
+//     sums over frame partitions [2 + k*pcB, 2 + (k+1)*pcB).
+// The roles touch disjoint data, so they may share a launch.
+//
+// Non-uniform partitioned objects (frame N = M * B samples): taps [0, N)
+// run as the uniform engine above with P = M partitions of B; taps [N, len)
+// form PB = len / N - 1 partitions of N, convolved frame by frame with
+// 2N-point transforms.  Frame g (input samples [gN, (g+1)N)) is complete at
+// block gM + M - 1; its output covers blocks (g+1)M .. (g+1)M + M - 1.  The
+// sum over frame partitions 2..PB-1 only needs frames <= g - 2, so it is
+// spread over blocks gM - 1 .. gM + M - 2, one slice of B bins per block;
+// the frame kernel (FrameArgs) at block gM + M - 1 adds partitions 0 and 1
+// and the slices, and inverse-transforms the frame's output.
 // IR spectrum storage: FP32 complex, or FP16 / BF16 complex scaled per
 // partition row (value = stored * S[c * P + i], S = the row's max |re|,|im|),
 // always accumulated in FP32.
@@ -62,6 +75,56 @@ struct TickArgs {
     const float *tSb;
     float2 *tTa;           // partial sums written [C][nchunk][B]
     float2 *tTb;
+
+    // non-uniform objects (nu = 0 for uniform ones)
+    int nu;
+    int M;                 // blocks per frame
+    int N;                 // frame length M * B
+    float *xring;          // [C][2M][B] last 2M input blocks
+    int xslot;             // ring slot receiving x_t (t mod 2M)
+    const float *zA;       // frame output added to filter A's y: [C][N] at + zoff, or nullptr
+    const float *zB;       // the same for filter B (crossfade blocks)
+    int zoff;              // (t mod M) * B
+
+    // frame multiply-accumulate role
+    int nchB;              // chunks per channel (0 = none this block)
+    int pcB;               // frame partitions per chunk
+    int PB;                // frame partitions
+    int RB;                // frame ring rows (power of two > PB)
+    const float2 *fb;      // [C][RB][N] frame spectra ring
+    int fbase;             // ring row of the frame being summed for
+    int bslice;            // bin slice (bins bslice*B .. +B)
+    const void *bHa;       // frame-partition spectra [C][PB][N] of IrType
+    const void *bHb;       // second filter or nullptr
+    const float *bSa;      // row scales [C][PB]
+    const float *bSb;
+    float2 *bTa;           // partial sums written [C][nchB][N] (slice bins)
+    float2 *bTb;
+};
+
+// Frame kernel of a non-uniform object at the last block of frame g: one
+// workgroup per channel.  X_g = R2C_2N([x_{g-1}, x_g]) from the input ring
+// (the head of the same block has written x_t into it), into the frame ring;
+// for each filter f: A = X_g H_f0 + X_{g-1} H_f1 + sum of the frame partials,
+// z_f = C2R_2N(A)[N, 2N) / 2N, the frame's output.
+struct FrameArgs {
+    int B, M, N, C, PB, RB;
+    const uint8_t *run;
+    const float *xring;    // [C][2M][B]
+    int xlast;             // ring slot of the frame's last block
+    float2 *fb;            // [C][RB][N]
+    int frow;              // ring row of X_g
+    const void *Ha;        // frame-partition spectra [C][PB][N] of IrType
+    const void *Hb;        // second filter (the next block crossfades) or nullptr
+    const float *Sa;
+    const float *Sb;
+    const float2 *Ta;      // frame partials [C][nch][N] (nch may be 0)
+    const float2 *Tb;
+    int nch;
+    float *za;             // [C][N] outputs
+    float *zb;
+    const float2 *twN;     // W_N^t, t < N
+    const float2 *tw2N;    // W_2N^t, t < 2N
 };
 
 struct IrArgs {
@@ -98,6 +161,11 @@ constexpr int kGroupMaxB = 512;
 // k_tick reading its arguments from d_descs[k] (graph replay)
 hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, unsigned blocks, hipStream_t s);
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);
+// frame kernels: N in {128, ..., 4096}
+constexpr int kMaxFrame = 4096;
+hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s);
+hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
+                              hipStream_t s);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 

@@ -290,11 +290,28 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
             acc[f][q] = s;
         }
     }
+    // non-uniform objects: the frame outputs for this block (read early)
+    constexpr int ZP = (M / 2 + NT - 1) / NT;  // sample pairs per thread
+    float2 za[ZP], zb[ZP];
+#pragma unroll
+    for (int q = 0; q < ZP; q++) {
+        za[q] = make_float2(0.f, 0.f);
+        zb[q] = make_float2(0.f, 0.f);
+        const int n = tid + q * NT;
+        if (n < M / 2) {
+            if (a.zA) za[q] = reinterpret_cast<const float2 *>(a.zA + (size_t)c * a.N + a.zoff)[n];
+            if (a.zB) zb[q] = reinterpret_cast<const float2 *>(a.zB + (size_t)c * a.N + a.zoff)[n];
+        }
+    }
     __syncthreads();
     if (tid < M) {
         // this block becomes the previous one of the next frame
         float *xpw = a.xprev + (size_t)c * M;
         for (int j = tid; j < M; j += NT) xpw[j] = xi[j];
+        if (a.nu) {
+            float *xr = a.xring + ((size_t)c * 2 * a.M + a.xslot) * M;
+            for (int j = tid; j < M; j += NT) xr[j] = xi[j];
+        }
     }
 
     const float2 *Z = fft_lds<M, false>(sm.s0, sm.s1, sm.tw);
@@ -324,18 +341,27 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
         const float2 *z = fft_lds<M, true>(sm.s0, sm.s1, sm.tw);
         // second half of the 2M output samples: z[M/2 .. M)
         if (f == 0) {
-            for (int n = tid; n < M / 2; n += NT) {
-                const float2 v = z[M / 2 + n];
-                sm.ya[2 * n] = v.x * kScale;
-                sm.ya[2 * n + 1] = v.y * kScale;
+#pragma unroll
+            for (int q = 0; q < ZP; q++) {
+                const int n = tid + q * NT;
+                if (n < M / 2) {
+                    const float2 v = z[M / 2 + n];
+                    sm.ya[2 * n] = v.x * kScale + za[q].x;
+                    sm.ya[2 * n + 1] = v.y * kScale + za[q].y;
+                }
             }
         } else {
             const float step = (float)(1.0 / (double)M);
-            for (int n = tid; n < M / 2; n += NT) {
-                const float2 v = z[M / 2 + n];
-                const int j0 = 2 * n, j1 = 2 * n + 1;
-                sm.ya[j0] = (v.x * kScale) * (step * (float)j0) + sm.ya[j0] * (step * (float)(M - j0));
-                sm.ya[j1] = (v.y * kScale) * (step * (float)j1) + sm.ya[j1] * (step * (float)(M - j1));
+#pragma unroll
+            for (int q = 0; q < ZP; q++) {
+                const int n = tid + q * NT;
+                if (n < M / 2) {
+                    const float2 v = z[M / 2 + n];
+                    const int j0 = 2 * n, j1 = 2 * n + 1;
+                    const float y0 = v.x * kScale + zb[q].x, y1 = v.y * kScale + zb[q].y;
+                    sm.ya[j0] = y0 * (step * (float)j0) + sm.ya[j0] * (step * (float)(M - j0));
+                    sm.ya[j1] = y1 * (step * (float)j1) + sm.ya[j1] * (step * (float)(M - j1));
+                }
             }
         }
         __syncthreads();
@@ -344,10 +370,35 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
     for (int j = tid; j < M; j += NT) yo[j] = sm.ya[j];
 }
 
-// Partial sums over partitions [i0, i1) for one channel; 256 threads; a row
-// of B complex is read as float4 (two complex) per lane, G rows in flight.
+// Rows of one channel for a partial sum over partitions [i0, i1): partition
+// i multiplies FDL row (base - i) & mask with IR row i, over B bins starting
+// at the rows' slice offsets.
+struct MacView {
+    const float4 *F;       // FDL row 0 of the channel, at the slice (float4 units)
+    int fstride;           // float4 per FDL row
+    int base, mask;
+    const char *HA;        // IR row 0 of the channel, at the slice
+    const char *HB;        // second filter or nullptr
+    size_t hstride;        // bytes per IR row
+    const float *SA;       // scales of the channel's rows (reduced precision)
+    const float *SB;
+    int i0, i1;
+    bool dc;               // the slice starts with the packed (DC, Nyquist) element
+    float4 *TA;            // this chunk's output row (B complex)
+    float4 *TB;
+};
+
+template <int HT>
+__device__ __forceinline__ float row_scale(const float *S, int i)
+{
+    if constexpr (HT == IR_F32) return 1.f;
+    else return S[i];
+}
+
+// 256 threads; a row of B complex is read as float4 (two complex) per lane,
+// G rows in flight, U partitions unrolled per lane.
 template <int M, int NF, int HT>
-__device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
+__device__ void mac_rows(const MacView &w, float4 *red)
 {
     constexpr int H = M / 2;                     // float4 per row
     constexpr int V = H > 256 ? H / 256 : 1;     // float4 per lane per row
@@ -356,16 +407,8 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
     constexpr int U = 8;                         // unroll (partitions per lane)
     const int tid = threadIdx.x;
     const int g = tid / R, r = tid % R;
-    const int i0 = 2 + j * a.pc;
-    const int i1 = min(a.P, i0 + a.pc);
-    const int mask = a.P - 1;
-
-    const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
-    const void *HA = ir_row<HT>(a.tHa, c, a.P, 0, M);
-    const void *HB = NF == 2 ? ir_row<HT>(a.tHb, c, a.P, 0, M) : nullptr;
-    constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
-    auto hrow = [&](const void *base, int i) {
-        return reinterpret_cast<const void *>(reinterpret_cast<const char *>(base) + (size_t)i * kRowBytes);
+    auto hrow = [&](const char *base, int i) {
+        return reinterpret_cast<const void *>(base + (size_t)i * w.hstride);
     };
 
     float4 accA[V], accB[V];
@@ -374,24 +417,24 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
         accA[v] = make_float4(0.f, 0.f, 0.f, 0.f);
         accB[v] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    const bool dc0 = (r == 0);
+    const bool dc0 = w.dc && (r == 0);
     const float s0 = dc0 ? 0.f : 1.f;
 
-    int i = i0 + g;
-    for (; i + (U - 1) * G < i1; i += U * G) {
+    int i = w.i0 + g;
+    for (; i + (U - 1) * G < w.i1; i += U * G) {
         float4 xf[U][V], ha[U][V], hb[U][V];
         float sa[U], sb[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int ii = i + u * G;
-            const int s = (a.tbase - ii) & mask;
-            sa[u] = ir_scale<HT>(a.tSa, c, a.P, ii);
-            if (NF == 2) sb[u] = ir_scale<HT>(a.tSb, c, a.P, ii);
+            const int sl = (w.base - ii) & w.mask;
+            sa[u] = row_scale<HT>(w.SA, ii);
+            if (NF == 2) sb[u] = row_scale<HT>(w.SB, ii);
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                xf[u][v] = ld_stream(F + (size_t)s * H + v * R + r);
-                ha[u][v] = IrRow<HT>::get2(hrow(HA, ii), v * R + r);
-                if (NF == 2) hb[u][v] = IrRow<HT>::get2(hrow(HB, ii), v * R + r);
+                xf[u][v] = ld_stream(w.F + (size_t)sl * w.fstride + v * R + r);
+                ha[u][v] = IrRow<HT>::get2(hrow(w.HA, ii), v * R + r);
+                if (NF == 2) hb[u][v] = IrRow<HT>::get2(hrow(w.HB, ii), v * R + r);
             }
         }
 #pragma unroll
@@ -399,37 +442,35 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 const bool dc = dc0 && v == 0;
-                const float s = v == 0 ? s0 : 1.f;
-                mac4(accA[v], xf[u][v], HT == IR_F32 ? ha[u][v] : scale4(ha[u][v], sa[u]), s, dc);
-                if (NF == 2) mac4(accB[v], xf[u][v], HT == IR_F32 ? hb[u][v] : scale4(hb[u][v], sb[u]), s, dc);
+                const float sv = v == 0 ? s0 : 1.f;
+                mac4(accA[v], xf[u][v], HT == IR_F32 ? ha[u][v] : scale4(ha[u][v], sa[u]), sv, dc);
+                if (NF == 2) mac4(accB[v], xf[u][v], HT == IR_F32 ? hb[u][v] : scale4(hb[u][v], sb[u]), sv, dc);
             }
         }
     }
-    for (; i < i1; i += G) {
-        const int s = (a.tbase - i) & mask;
-        const float sai = ir_scale<HT>(a.tSa, c, a.P, i);
-        const float sbi = NF == 2 ? ir_scale<HT>(a.tSb, c, a.P, i) : 0.f;
+    for (; i < w.i1; i += G) {
+        const int sl = (w.base - i) & w.mask;
+        const float sai = row_scale<HT>(w.SA, i);
+        const float sbi = NF == 2 ? row_scale<HT>(w.SB, i) : 0.f;
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const bool dc = dc0 && v == 0;
             const float sv = v == 0 ? s0 : 1.f;
-            const float4 x = ld_stream(F + (size_t)s * H + v * R + r);
-            const float4 h1 = IrRow<HT>::get2(hrow(HA, i), v * R + r);
+            const float4 x = ld_stream(w.F + (size_t)sl * w.fstride + v * R + r);
+            const float4 h1 = IrRow<HT>::get2(hrow(w.HA, i), v * R + r);
             mac4(accA[v], x, HT == IR_F32 ? h1 : scale4(h1, sai), sv, dc);
             if (NF == 2) {
-                const float4 h2 = IrRow<HT>::get2(hrow(HB, i), v * R + r);
+                const float4 h2 = IrRow<HT>::get2(hrow(w.HB, i), v * R + r);
                 mac4(accB[v], x, HT == IR_F32 ? h2 : scale4(h2, sbi), sv, dc);
             }
         }
     }
 
-    float4 *TA = reinterpret_cast<float4 *>(a.tTa) + ((size_t)c * a.nchunk + j) * H;
-    float4 *TB = NF == 2 ? reinterpret_cast<float4 *>(a.tTb) + ((size_t)c * a.nchunk + j) * H : nullptr;
     if constexpr (G == 1) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            TA[v * R + r] = accA[v];
-            if (NF == 2) TB[v * R + r] = accB[v];
+            w.TA[v * R + r] = accA[v];
+            if (NF == 2) w.TB[v * R + r] = accB[v];
         }
     } else {
         // red: [NF][G][H] float4
@@ -447,9 +488,70 @@ __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
                 const float4 t = red[(f * G + gg) * H + q];
                 sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
             }
-            (f == 0 ? TA : TB)[q] = sum;
+            (f == 0 ? w.TA : w.TB)[q] = sum;
         }
     }
+}
+
+// Uniform tail: partitions [2 + j*pc, 2 + (j+1)*pc) of channel c for the next block.
+template <int M, int NF, int HT>
+__device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
+{
+    constexpr int H = M / 2;
+    constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
+    MacView w;
+    w.F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
+    w.fstride = H;
+    w.base = a.tbase;
+    w.mask = a.P - 1;
+    w.HA = reinterpret_cast<const char *>(a.tHa) + (size_t)c * a.P * kRowBytes;
+    w.HB = NF == 2 ? reinterpret_cast<const char *>(a.tHb) + (size_t)c * a.P * kRowBytes : nullptr;
+    w.hstride = kRowBytes;
+    w.SA = HT == IR_F32 ? nullptr : a.tSa + (size_t)c * a.P;
+    w.SB = (HT == IR_F32 || NF == 1) ? nullptr : a.tSb + (size_t)c * a.P;
+    w.i0 = 2 + j * a.pc;
+    w.i1 = min(a.P, w.i0 + a.pc);
+    w.dc = true;
+    w.TA = reinterpret_cast<float4 *>(a.tTa) + ((size_t)c * a.nchunk + j) * H;
+    w.TB = NF == 2 ? reinterpret_cast<float4 *>(a.tTb) + ((size_t)c * a.nchunk + j) * H : nullptr;
+    mac_rows<M, NF, HT>(w, red);
+}
+
+// Frame multiply-accumulate of a non-uniform object: bin slice a.bslice of
+// frame partitions [2 + k*pcB, 2 + (k+1)*pcB) for channel c.
+template <int M, int NF, int HT>
+__device__ void frame_mac_role(const TickArgs &a, int c, int k, float4 *red)
+{
+    const size_t rowB = (size_t)a.N * IrRow<HT>::kBytes;
+    const size_t off = (size_t)a.bslice * M;  // complex elements
+    MacView w;
+    w.F = reinterpret_cast<const float4 *>(a.fb + (size_t)c * a.RB * a.N + off);
+    w.fstride = a.N / 2;
+    w.base = a.fbase;
+    w.mask = a.RB - 1;
+    w.HA = reinterpret_cast<const char *>(a.bHa) + (size_t)c * a.PB * rowB + off * IrRow<HT>::kBytes;
+    w.HB = NF == 2 ? reinterpret_cast<const char *>(a.bHb) + (size_t)c * a.PB * rowB + off * IrRow<HT>::kBytes
+                   : nullptr;
+    w.hstride = rowB;
+    w.SA = HT == IR_F32 ? nullptr : a.bSa + (size_t)c * a.PB;
+    w.SB = (HT == IR_F32 || NF == 1) ? nullptr : a.bSb + (size_t)c * a.PB;
+    w.i0 = 2 + k * a.pcB;
+    w.i1 = min(a.PB, w.i0 + a.pcB);
+    w.dc = a.bslice == 0;
+    w.TA = reinterpret_cast<float4 *>(a.bTa + ((size_t)c * a.nchB + k) * a.N + off);
+    w.TB = NF == 2 ? reinterpret_cast<float4 *>(a.bTb + ((size_t)c * a.nchB + k) * a.N + off) : nullptr;
+    mac_rows<M, NF, HT>(w, red);
+}
+
+// frame multiply-accumulate blocks (after head and tail blocks)
+template <int M, int HT>
+__device__ __forceinline__ void frame_mac_block(const TickArgs &a, int t, float4 *smem)
+{
+    if (t >= a.C * a.nchB) return;
+    const int c = t / a.nchB, k = t % a.nchB;
+    if (!a.run[c]) return;
+    if (a.bHb) frame_mac_role<M, 2, HT>(a, c, k, smem);
+    else frame_mac_role<M, 1, HT>(a, c, k, smem);
 }
 
 template <int M, int NF, int HT>
@@ -463,11 +565,13 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
     if (b < a.nhead) {
         if (!a.run[b]) return;
         head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
-    } else {
+    } else if (b < a.nhead + a.C * a.nchunk) {
         const int t = b - a.nhead;
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
         tail_role<M, NF, HT>(a, c, j, smem);
+    } else {
+        frame_mac_block<M, HT>(a, b - a.nhead - a.C * a.nchunk, smem);
     }
 }
 
@@ -480,7 +584,10 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
         head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
     } else {
         const int t = b - a.nhead;
-        if (t >= a.C * a.nchunk) return;
+        if (t >= a.C * a.nchunk) {
+            frame_mac_block<M, HT>(a, t - a.C * a.nchunk, smem);
+            return;
+        }
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
         if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
@@ -526,6 +633,124 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
     case 128: tick_body<128, HT>(a, lb, smem); break;
     case 256: tick_body<256, HT>(a, lb, smem); break;
     case 512: tick_body<512, HT>(a, lb, smem); break;
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Frame kernel of non-uniform objects (see FrameArgs), 256 threads.
+// ---------------------------------------------------------------------------
+template <int N, int HT>
+__device__ void frame_role(const FrameArgs &a, int c, float2 *s0, float2 *s1)
+{
+    constexpr int NT = kTickThreads;
+    constexpr int KPT = N >= NT ? N / NT : 1;
+    constexpr float kScale = 1.0f / (2.0f * (float)N);
+    const int tid = threadIdx.x;
+    const int nf = a.Hb ? 2 : 1;
+    const int lB = __builtin_ctz((unsigned)a.B);
+    const int rmask = 2 * a.M - 1;
+
+    // [x_{g-1}, x_g]: the 2M ring blocks after the last one written, oldest first
+    for (int n = tid; n < N; n += NT) {
+        const int t = 2 * n;
+        const int kb = t >> lB, w = t & (a.B - 1);
+        const int sl = (a.xlast + 1 + kb) & rmask;
+        const float2 v = *reinterpret_cast<const float2 *>(a.xring + ((size_t)c * 2 * a.M + sl) * a.B + w);
+        s0[n] = v;
+    }
+    __syncthreads();
+    const float2 *Z = fft_lds<N, false>(s0, s1, a.twN);
+    float2 *zbuf = const_cast<float2 *>(Z);
+    float2 *fr = (Z == s0) ? s1 : s0;
+
+    float2 X[KPT], Xp[KPT];
+    float2 *row = a.fb + ((size_t)c * a.RB + a.frow) * N;
+    const float2 *prow = a.fb + ((size_t)c * a.RB + ((a.frow - 1) & (a.RB - 1))) * N;
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+        const int k = tid + q * NT;
+        X[q] = make_float2(0.f, 0.f);
+        Xp[q] = make_float2(0.f, 0.f);
+        if (k < N) {
+            X[q] = r2c_post<N>(Z, k, a.tw2N);
+            row[k] = X[q];
+            if (a.PB > 1) Xp[q] = prow[k];
+        }
+    }
+    constexpr size_t kRowBytes = (size_t)N * IrRow<HT>::kBytes;
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        if (f >= nf) continue;
+        const char *Hc = reinterpret_cast<const char *>(f == 0 ? a.Ha : a.Hb) + (size_t)c * a.PB * kRowBytes;
+        const float *Sc = HT == IR_F32 ? nullptr : (f == 0 ? a.Sa : a.Sb) + (size_t)c * a.PB;
+        const float sc0 = row_scale<HT>(Sc, 0);
+        const float sc1 = a.PB > 1 ? row_scale<HT>(Sc, 1) : 0.f;
+        const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.nch * N;
+#pragma unroll
+        for (int q = 0; q < KPT; q++) {
+            const int k = tid + q * NT;
+            if (k < N) {
+                float2 acc = make_float2(0.f, 0.f);
+                for (int j = 0; j < a.nch; j++) {
+                    const float2 t = T[(size_t)j * N + k];
+                    acc.x += t.x;
+                    acc.y += t.y;
+                }
+                if (a.PB > 1) {
+                    const float2 h1 = IrRow<HT>::get1(Hc + kRowBytes, k);
+                    mac2(acc, Xp[q], make_float2(h1.x * sc1, h1.y * sc1), k == 0);
+                }
+                const float2 h0 = IrRow<HT>::get1(Hc, k);
+                mac2(acc, X[q], make_float2(h0.x * sc0, h0.y * sc0), k == 0);
+                fr[k] = acc;
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < N; k += NT) zbuf[k] = c2r_pre<N>(fr, k, a.tw2N);
+        __syncthreads();
+        const float2 *z = fft_lds<N, true>(zbuf, fr, a.twN);
+        float *out = (f == 0 ? a.za : a.zb) + (size_t)c * N;
+        for (int n = tid; n < N / 2; n += NT) {
+            const float2 v = z[N / 2 + n];
+            out[2 * n] = v.x * kScale;
+            out[2 * n + 1] = v.y * kScale;
+        }
+        __syncthreads();
+    }
+}
+
+template <int N, int HT>
+__global__ __launch_bounds__(kTickThreads) void k_frame(FrameArgs a)
+{
+    __shared__ float2 s0[N], s1[N];
+    const int c = blockIdx.x;
+    if (!a.run[c]) return;
+    frame_role<N, HT>(a, c, s0, s1);
+}
+
+template <int HT>
+__global__ __launch_bounds__(kTickThreads) void k_group_frame(const FrameArgs *__restrict__ descs,
+                                                              const int *__restrict__ starts, int nobj)
+{
+    __shared__ float2 s0[kMaxFrame], s1[kMaxFrame];
+    const int b = blockIdx.x;
+    int lo = 0, hi = nobj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const FrameArgs a = descs[lo];
+    const int c = b - starts[lo];
+    if (c >= a.C || !a.run[c]) return;
+    switch (a.N) {
+    case 128: frame_role<128, HT>(a, c, s0, s1); break;
+    case 256: frame_role<256, HT>(a, c, s0, s1); break;
+    case 512: frame_role<512, HT>(a, c, s0, s1); break;
+    case 1024: frame_role<1024, HT>(a, c, s0, s1); break;
+    case 2048: frame_role<2048, HT>(a, c, s0, s1); break;
+    case 4096: frame_role<4096, HT>(a, c, s0, s1); break;
     default: break;
     }
 }
@@ -691,7 +916,7 @@ static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream
 
 hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)(a.nhead + a.C * a.nchunk);
+    const unsigned blocks = (unsigned)(a.nhead + a.C * a.nchunk + a.C * a.nchB);
     if (blocks == 0) return hipSuccess;
     switch (a.irtype) {
     case IR_F32: return launch_tick_t<IR_F32>(B, tail_nf, a, s, blocks);
@@ -731,20 +956,70 @@ hipError_t launch_group_tick(int irtype, const TickArgs *d_descs, const int *d_s
     return hipGetLastError();
 }
 
+// partition sizes of uniform blocks (64..1024) and of frames (128..4096)
+#define TANX_SWITCH_PART(B, CALL)                        \
+    switch (B) {                                         \
+    case 64: { constexpr int M = 64; CALL; } break;      \
+    case 128: { constexpr int M = 128; CALL; } break;    \
+    case 256: { constexpr int M = 256; CALL; } break;    \
+    case 512: { constexpr int M = 512; CALL; } break;    \
+    case 1024: { constexpr int M = 1024; CALL; } break;  \
+    case 2048: { constexpr int M = 2048; CALL; } break;  \
+    case 4096: { constexpr int M = 4096; CALL; } break;  \
+    default: return hipErrorInvalidValue;                \
+    }
+
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
 {
+    if (a.P == 0 || nchan == 0) return hipSuccess;
     switch (a.irtype) {
     case IR_F32:
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F32>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F32>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
         break;
     case IR_F16:
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
         break;
     case IR_BF16:
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_ir_transform<M, IR_BF16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform<M, IR_BF16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
         break;
     default:
         return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+#define TANX_SWITCH_FRAME(N, CALL)                       \
+    switch (N) {                                         \
+    case 128: { constexpr int F = 128; CALL; } break;    \
+    case 256: { constexpr int F = 256; CALL; } break;    \
+    case 512: { constexpr int F = 512; CALL; } break;    \
+    case 1024: { constexpr int F = 1024; CALL; } break;  \
+    case 2048: { constexpr int F = 2048; CALL; } break;  \
+    case 4096: { constexpr int F = 4096; CALL; } break;  \
+    default: return hipErrorInvalidValue;                \
+    }
+
+hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s)
+{
+    if (a.C == 0) return hipSuccess;
+    switch (irtype) {
+    case IR_F32: TANX_SWITCH_FRAME(a.N, hipLaunchKernelGGL((k_frame<F, IR_F32>), dim3(a.C), dim3(kTickThreads), 0, s, a)); break;
+    case IR_F16: TANX_SWITCH_FRAME(a.N, hipLaunchKernelGGL((k_frame<F, IR_F16>), dim3(a.C), dim3(kTickThreads), 0, s, a)); break;
+    case IR_BF16: TANX_SWITCH_FRAME(a.N, hipLaunchKernelGGL((k_frame<F, IR_BF16>), dim3(a.C), dim3(kTickThreads), 0, s, a)); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
+                              hipStream_t s)
+{
+    if (blocks == 0 || nobj == 0) return hipSuccess;
+    switch (irtype) {
+    case IR_F32: hipLaunchKernelGGL((k_group_frame<IR_F32>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_F16: hipLaunchKernelGGL((k_group_frame<IR_F16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_BF16: hipLaunchKernelGGL((k_group_frame<IR_BF16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

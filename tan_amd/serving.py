@@ -35,13 +35,15 @@ from .batching import GroupCall, alg_bytes_per_block
 SAMPLE_RATE = 48000
 
 
-def estimate_bytes(lib, ir_len, block, channels, ir_precision=T.TAN_IR_PRECISION_FP32):
+def estimate_bytes(lib, ir_len, block, channels, ir_precision=T.TAN_IR_PRECISION_FP32,
+                   method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, nu_multiple=0):
     """HBM bytes of one session: the object (tanx_conv_estimate_bytes) plus its
     input and output block."""
     opt = _lib.ConvOptions()
     opt.ir_precision = ir_precision
+    opt.nu_multiple = nu_multiple
     out = ctypes.c_uint64(0)
-    rc = lib.tanx_conv_estimate_bytes(ir_len, block, channels, ctypes.addressof(opt), ctypes.byref(out))
+    rc = lib.tanx_conv_estimate_bytes(method, ir_len, block, channels, ctypes.addressof(opt), ctypes.byref(out))
     if rc != T.AMF_OK:
         raise ValueError(f"estimate failed for L={ir_len}, B={block}, C={channels}: AMF_RESULT {rc}")
     return int(out.value) + 2 * channels * block * 4
@@ -175,13 +177,16 @@ class SessionServer:
     range gets tensors of its own."""
 
     def __init__(self, ctx, block=256, hbm_budget=32 << 30, ir_precision=T.TAN_IR_PRECISION_FP32,
-                 device="cuda", io_rows=0):
+                 device="cuda", io_rows=0, method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED,
+                 nu_multiple=0):
         if block & (block - 1) or block < 64 or block > 512:
             raise ValueError("block must be a power of two in [64, 512]")
         self.ctx = ctx
         self.lib = ctx._lib
         self.block = block
         self.ir_precision = ir_precision
+        self.method = method
+        self.nu_multiple = nu_multiple
         self.device = device
         self.admission = Admission(hbm_budget)
         self.pending = {}
@@ -208,7 +213,8 @@ class SessionServer:
         (kept until admission).  Returns False if it can never fit."""
         s = Session(sid, channels, ir_len)
         s.arrived_tick = self.ticks
-        nb = estimate_bytes(self.lib, ir_len, self.block, channels, self.ir_precision)
+        nb = estimate_bytes(self.lib, ir_len, self.block, channels, self.ir_precision, self.method,
+                            self.nu_multiple)
         if not self.admission.submit(sid, nb):
             return False
         self.pending[sid] = (s, first_ir)
@@ -221,8 +227,8 @@ class SessionServer:
         out = []
         for sid, nb in self.admission.admit():
             s, ir = self.pending.pop(sid)
-            conv = T.TANConvolution(self.ctx, self.ir_precision)
-            rc = conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, s.ir_len, self.block, s.channels)
+            conv = T.TANConvolution(self.ctx, self.ir_precision, self.nu_multiple)
+            rc = conv.InitGpu(self.method, s.ir_len, self.block, s.channels)
             if rc != T.AMF_OK:
                 self.admission.release(nb)
                 raise RuntimeError(f"InitGpu failed for session {sid}: AMF_RESULT {rc}")

@@ -142,13 +142,17 @@ class TANContext:
 class TANConvolution:
     """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
 
-    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32):
+    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0):
+        """ir_precision: storage of the IR spectra (TAN_IR_PRECISION_*);
+        nu_multiple: blocks per frame for the non-uniform methods (0: chosen
+        by the library, 1: uniform partitions)."""
         self._ctx = context
         self._lib = context._lib
         self.handle = None
         self.channels = 0
         self.block = 0
         self.ir_precision = ir_precision
+        self.nu_multiple = nu_multiple
 
     # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
     def Init(self, method, response_length, buffer_size, channels):
@@ -160,6 +164,7 @@ class TANConvolution:
         h = ctypes.c_void_p()
         opt = _lib.ConvOptions()
         opt.ir_precision = self.ir_precision
+        opt.nu_multiple = self.nu_multiple
         rc = self._lib.tanx_conv_create_ex(self._ctx.handle, method, response_length, buffer_size,
                                            channels, ctypes.addressof(opt), ctypes.byref(h))
         if rc == AMF_OK:

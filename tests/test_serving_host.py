@@ -139,3 +139,23 @@ def test_row_allocator_first_fit_and_coalescing():
     for st, n in held:
         b.release(st, n)
     assert b.free == [(0, 256)]
+
+
+def test_nonuniform_estimate_and_options():
+    lib = T._lib.load()
+    NU = T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
+    UNI = T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+    for L, B, C in [(524288, 256, 16), (65536, 128, 4), (8192, 64, 1)]:
+        u = S.estimate_bytes(lib, L, B, C, method=UNI)
+        n = S.estimate_bytes(lib, L, B, C, method=NU)
+        ln = Bt.rounded_length(L, B)
+        # both hold three IR slots and a history of the whole IR; the frame
+        # ring, input ring, partials and outputs come on top
+        assert n >= C * ln * (3 * 8 + 8)
+        assert n < 2 * u
+        # an explicit multiple of 1 is the uniform layout
+        assert S.estimate_bytes(lib, L, B, C, method=NU, nu_multiple=1) == u
+    # invalid multiples: not a power of two, frame above 4096, frame above half the IR
+    for mult, L, B in [(3, 65536, 256), (32, 524288, 256), (16, 4096, 256)]:
+        with pytest.raises(ValueError):
+            S.estimate_bytes(lib, L, B, 1, method=NU, nu_multiple=mult)
