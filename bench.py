@@ -331,10 +331,18 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
         d["updates"] = 0
         srv.submit(d["sid"], d["channels"], d["ir_len"], ir_for(d["sid"], 0, d["channels"], d["ir_len"]))
 
+    # a stationary population: sessions that started at random earlier
+    # times, admitted over a warm-up of 64 untimed ticks so that their block
+    # counters (and frame phases) are spread as in steady state
     t0 = time.perf_counter()
-    for d in wl.initial_population():
-        submit(d)
-    srv.admit()
+    init = wl.initial_population()
+    warm = 64
+    for w in range(warm):
+        for d in init[w::warm]:
+            submit(d)
+        srv.admit()
+        srv.in_arena.copy_(noise[w % nx])
+        srv.finish(srv.tick())
     setup_s = time.perf_counter() - t0
     ticks = int(seconds * FS / B)
     lat, enq, ctl, chans, sess, byts, waits = [], [], [], [], [], [], []
@@ -477,6 +485,7 @@ def main():
     ap.add_argument("--batching-streams", type=str, default="512,2048")
     ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--no-nonuniform", action="store_true")
+    ap.add_argument("--serving-methods", type=str, default="uniform,nonuniform")
     ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304")
     ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
     ap.add_argument("--profile-run", action="store_true",
@@ -557,7 +566,8 @@ def main():
 
     serving = []
     if not args.no_serving:
-        for meth in ((UNIFORM_METHOD,) if args.no_nonuniform else (UNIFORM_METHOD, NONUNIFORM_METHOD)):
+        names = {"uniform": UNIFORM_METHOD, "nonuniform": NONUNIFORM_METHOD}
+        for meth in [names[m] for m in args.serving_methods.split(",") if m]:
             log(f"serving pass ({'non-uniform' if meth == NONUNIFORM_METHOD else 'uniform'})")
             row = bench_serving(T, torch, ctx, seconds=args.serving_seconds, method=meth)
             log(f"  {row}")

@@ -216,9 +216,55 @@ public:
 };
 typedef TANInterfacePtr_T<TANFFT> TANFFTPtr;
 
+// TANConverter (TrueAudioNext.h:344-398 of the reference): conversions run
+// on the GPU; host overloads stage through device memory.
+class TANConverter : public TANInterface {
+public:
+    virtual AMF_RESULT AMF_STD_CALL Init() = 0;
+    virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
+    virtual TANContext *AMF_STD_CALL GetContext() = 0;
+    virtual AMF_RESULT AMF_STD_CALL Convert(short *inputBuffer, amf_size inputStep, amf_size numOfSamplesToProcess,
+                                            float *outputBuffer, amf_size outputStep, float conversionGain) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Convert(float *inputBuffer, amf_size inputStep, amf_size numOfSamplesToProcess,
+                                            short *outputBuffer, amf_size outputStep, float conversionGain,
+                                            bool *outputClipped = nullptr) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Convert(short **inputBuffers, amf_size inputStep, amf_size numOfSamplesToProcess,
+                                            float **outputBuffers, amf_size outputStep, float conversionGain,
+                                            int channels) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Convert(float **inputBuffers, amf_size inputStep, amf_size numOfSamplesToProcess,
+                                            short **outputBuffers, amf_size outputStep, float conversionGain,
+                                            int channels, bool *outputClipped = nullptr) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Convert(tan_hip_mem inputBuffer, amf_size inputStep, amf_size inputOffset,
+                                            TAN_SAMPLE_TYPE inputType, tan_hip_mem outputBuffer, amf_size outputStep,
+                                            amf_size outputOffset, TAN_SAMPLE_TYPE outputType,
+                                            amf_size numOfSamplesToProcess, float conversionGain,
+                                            bool *outputClipped = nullptr) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Convert(tan_hip_mem *inputBuffers, amf_size inputStep, amf_size *inputOffsets,
+                                            TAN_SAMPLE_TYPE inputType, tan_hip_mem *outputBuffers,
+                                            amf_size outputStep, amf_size *outputOffsets, TAN_SAMPLE_TYPE outputType,
+                                            amf_size numOfSamplesToProcess, float conversionGain, int count,
+                                            bool *outputClipped = nullptr) = 0;
+};
+typedef TANInterfacePtr_T<TANConverter> TANConverterPtr;
+
+// TANMixer (TrueAudioNext.h:611-635 of the reference).
+class TANMixer : public TANInterface {
+public:
+    virtual AMF_RESULT AMF_STD_CALL Init(amf_size buffer_size, int num_channels) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
+    virtual TANContext *AMF_STD_CALL GetContext() = 0;
+    virtual AMF_RESULT AMF_STD_CALL Mix(float *ppBufferInput[], float *ppBufferOutput) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Mix(tan_hip_mem pBufferInput[], tan_hip_mem pBufferOutput) = 0;
+    virtual AMF_RESULT AMF_STD_CALL Mix(tan_hip_mem pBufferInput, tan_hip_mem pBufferOutput,
+                                        amf_size inputStride) = 0;
+};
+typedef TANInterfacePtr_T<TANMixer> TANMixerPtr;
+
 }  // namespace amf
 
 extern "C" {
+TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateConverter(amf::TANContext *pContext, amf::TANConverter **ppConverter);
+TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateMixer(amf::TANContext *pContext, amf::TANMixer **ppMixer);
 TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateContext(amf_uint64 version, amf::TANContext **ppContext);
 TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateConvolution(amf::TANContext *pContext,
                                                             amf::TANConvolution **ppConvolution);

@@ -255,6 +255,50 @@ int32_t tanx_fft_transform_device(tanx_ctx *ctx, uint32_t direction, uint32_t lo
 
 const char *tanx_version(void);
 
+/* ---- output chain: mixer and sample converter -------------------------------- */
+
+/* TAN_SAMPLE_TYPE (TrueAudioNext.h:73-77). */
+enum { TANX_SAMPLE_FLOAT = 0, TANX_SAMPLE_SHORT = 1 };
+
+typedef struct tanx_mixer tanx_mixer;
+/* TANCreateMixer + TANMixer::Init(buffer_size, num_channels)
+ * (TrueAudioNext.h:611-635; mixer/MixerImpl.cpp).  The sum is taken in
+ * channel order in single precision, as Mixer.cl:26-42 and the CPU Mix do. */
+int32_t tanx_mixer_create(tanx_ctx *ctx, size_t buffer_size, int32_t channels, tanx_mixer **out);
+int32_t tanx_mixer_destroy(tanx_mixer *mixer);
+/* Mix(float* ppBufferInput[], float* ppBufferOutput): host buffers; returns
+ * with the output written. */
+int32_t tanx_mix(tanx_mixer *mixer, const float *const *in, float *out);
+/* Mix(cl_mem pBufferInput[], cl_mem pBufferOutput): device buffers (a host
+ * array of device pointers); asynchronous on the context's process stream. */
+int32_t tanx_mix_device(tanx_mixer *mixer, const float *const *d_in, float *d_out);
+/* Mix(cl_mem pBufferInput, cl_mem pBufferOutput, amf_size inputStride):
+ * channel c at d_in + c * in_stride; asynchronous. */
+int32_t tanx_mix_device_strided(tanx_mixer *mixer, const float *d_in, size_t in_stride, float *d_out);
+
+/* TANConverter::Convert on device buffers (cl_mem overloads,
+ * TrueAudioNext.h:375-398; ConvertGpu, converter/ConverterImpl.cpp:458-556;
+ * Converter.cl).  Element i is read at in[i * in_step + in_offset] and
+ * written at out[i * out_step + out_offset] (offsets and steps in samples).
+ * float -> short: x * 32767 * gain, clamped to [-32768, 32767], converted
+ * toward zero; short -> float: x * gain / 32767; same type: copy.
+ * gain > 0 and n > 0 (AMF_INVALID_ARG otherwise); other types:
+ * AMF_NOT_IMPLEMENTED.  Asynchronous on the process stream unless clipped is
+ * given: then it waits and sets *clipped to 1 if a float -> short conversion
+ * clamped a sample.  Clipping is reported through *clipped only; the call
+ * still returns AMF_OK, as the reference's does. */
+int32_t tanx_convert_device(tanx_ctx *ctx, const void *d_in, size_t in_step, size_t in_offset, int32_t in_type,
+                            void *d_out, size_t out_step, size_t out_offset, int32_t out_type, size_t n,
+                            float gain, int32_t *clipped);
+/* TANConverter::Convert on host buffers (TrueAudioNext.h:353-373;
+ * ConverterImpl.cpp:217-453): the device path's conversion through staging;
+ * returns with the output written.  The reference's own CPU conversion
+ * rounds groups of eight samples to nearest and the rest toward zero, and
+ * does not report clamping of the groups of eight; this one converts every
+ * sample like its GPU kernel. */
+int32_t tanx_convert(tanx_ctx *ctx, const void *in, size_t in_step, int32_t in_type, void *out, size_t out_step,
+                     int32_t out_type, size_t n, float gain, int32_t *clipped);
+
 #ifdef __cplusplus
 }
 #endif

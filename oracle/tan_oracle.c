@@ -788,3 +788,42 @@ double orc_bench_upols(int C, int L, int B, int nblocks, int warm, uint64_t seed
     orc_conv_destroy(c);
     return total;
 }
+
+/* ------------------------------------------------------------------------ */
+/* 4. I/O chain                                                              */
+/* ------------------------------------------------------------------------ */
+
+void orc_mix(const float *const *in, int channels, size_t n, float *out)
+{
+    for (size_t i = 0; i < n; i++) {
+        float acc = 0.0f;
+        for (int c = 0; c < channels; c++) acc = (c == 0) ? in[0][i] : acc + in[c][i];
+        out[i] = acc;
+    }
+}
+
+int orc_convert(const void *in, size_t in_step, size_t in_offset, int in_type, void *out, size_t out_step,
+                size_t out_offset, int out_type, size_t n, float gain, int *clipped)
+{
+    if ((in_type != 0 && in_type != 1) || (out_type != 0 && out_type != 1)) return -1;
+    int clip = 0;
+    for (size_t i = 0; i < n; i++) {
+        const size_t si = i * in_step + in_offset, so = i * out_step + out_offset;
+        if (in_type == 0 && out_type == 1) {
+            const float scale = 32767.0f * gain;
+            float f = ((const float *)in)[si] * scale;
+            if (f > 32767.0f) { f = 32767.0f; clip = 1; }
+            if (f < -32768.0f) { f = -32768.0f; clip = 1; }
+            ((int16_t *)out)[so] = (int16_t)f;  /* C conversion truncates toward zero */
+        } else if (in_type == 1 && out_type == 0) {
+            const float scale = gain / 32767.0f;
+            ((float *)out)[so] = (float)((const int16_t *)in)[si] * scale;
+        } else if (in_type == 0) {
+            ((float *)out)[so] = ((const float *)in)[si];
+        } else {
+            ((int16_t *)out)[so] = ((const int16_t *)in)[si];
+        }
+    }
+    if (clipped) *clipped = clip;
+    return 0;
+}

@@ -117,6 +117,24 @@ int orc_num_procs(void);
 double orc_bench_upols(int C, int L, int B, int nblocks, int warm,
                        uint64_t seed, double *times);
 
+/* 4. The device I/O chain of the reference GPU path.
+ *
+ * TANMixer::Mix: out[s] = in_0[s] + in_1[s] + ... in channel order, single
+ * precision, each partial sum rounded (mixer/Mixer.cl:26-42;
+ * MixerImpl.cpp Mix(float**, float*) adds in the same order). */
+void orc_mix(const float *const *in, int channels, size_t n, float *out);
+
+/* TANConverter on the GPU (converter/Converter.cl, ConvertGpu at
+ * ConverterImpl.cpp:458-556).  Types: 0 = float, 1 = short
+ * (TAN_SAMPLE_TYPE, TrueAudioNext.h:73-77).  float -> short:
+ * f = x * (32767 * gain), clamped to [-32768, 32767] (clamping sets
+ * *clipped), then converted toward zero (OpenCL convert_short);
+ * short -> float: x * (gain / 32767); same type: copy.  Element i is read
+ * at in[i * in_step + in_offset] and written at out[i * out_step +
+ * out_offset].  Returns 0, or -1 for an unknown type. */
+int orc_convert(const void *in, size_t in_step, size_t in_offset, int in_type, void *out, size_t out_step,
+                size_t out_offset, int out_type, size_t n, float gain, int *clipped);
+
 #ifdef __cplusplus
 }
 #endif

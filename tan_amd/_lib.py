@@ -42,6 +42,15 @@ _SIGS = {
     "tanx_ctx_destroy": (_i32, [_vp]),
     "tanx_ctx_device": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "tanx_ctx_process_stream": (_vp, [_vp]),
+    "tanx_mixer_create": (_i32, [_vp, ctypes.c_size_t, _i32, ctypes.POINTER(_vp)]),
+    "tanx_mixer_destroy": (_i32, [_vp]),
+    "tanx_mix": (_i32, [_vp, _vp, _vp]),
+    "tanx_mix_device": (_i32, [_vp, _vp, _vp]),
+    "tanx_mix_device_strided": (_i32, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "tanx_convert_device": (_i32, [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _i32, _vp, ctypes.c_size_t,
+                                   ctypes.c_size_t, _i32, ctypes.c_size_t, ctypes.c_float, ctypes.POINTER(_i32)]),
+    "tanx_convert": (_i32, [_vp, _vp, ctypes.c_size_t, _i32, _vp, ctypes.c_size_t, _i32, ctypes.c_size_t,
+                            ctypes.c_float, ctypes.POINTER(_i32)]),
     "tanx_conv_create": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
     "tanx_conv_create_ex": (_i32, [_vp, _u32, _u32, _u32, _u32, ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "tanx_conv_destroy": (_i32, [_vp]),

@@ -60,13 +60,27 @@ def rounded_length(ir_len, block):
     return max(n, block)
 
 
-def alg_bytes_per_block(ir_len, block, channels=1):
-    """Algorithmic HBM bytes of one block: IR + FDL reads of all partitions,
-    the new FDL row, input, output and overlap history (16*P*K + 8*K + 16*B per
-    channel, K = B + 1 packed bins counted as the reference's Hermitian bins)."""
-    p = rounded_length(ir_len, block) // block
+def alg_bytes_per_block(ir_len, block, channels=1, multiple=1):
+    """Algorithmic HBM bytes of one block (FP32 spectra).
+
+    Uniform (multiple = 1): IR + FDL reads of all partitions, the new FDL
+    row, input, output and overlap history: 16*P*K + 8*K + 16*B per channel,
+    K = B + 1 (the reference's Hermitian bins).
+    Non-uniform (frames of N = multiple * B): the first frame of taps as
+    uniform partitions (P = multiple), one slice of B bins of frame partitions
+    2..PB-1 per block (16*(PB-2)*B), and the frame kernel's input frame, frame
+    spectrum written and previous one read, two frame-partition spectra and
+    the frame output, spread over the frame's blocks."""
     k = block + 1
-    return channels * (16 * p * k + 8 * k + 16 * block)
+    if multiple <= 1:
+        p = rounded_length(ir_len, block) // block
+        return channels * (16 * p * k + 8 * k + 16 * block)
+    n = multiple * block
+    pb = rounded_length(ir_len, block) // n - 1
+    head = 16 * multiple * k + 8 * k + 16 * block
+    mac = 16 * (pb - 2) * block if pb > 2 else 0
+    frame = (2 * n * 4 + 2 * n * 8 + 2 * n * 8 + n * 4) // multiple
+    return channels * (head + mac + frame)
 
 
 def draw_streams(n, seed=7, min_len=4096, max_len=262144, blocks=(128, 256, 512)):

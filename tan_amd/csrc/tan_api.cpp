@@ -288,7 +288,164 @@ private:
     bool init_ = false;
 };
 
+class ConverterImpl : public RefCounted<TANConverter> {
+public:
+    explicit ConverterImpl(ContextImpl *ctx) : ctx_(ctx) { ctx_->Acquire(); }
+    ~ConverterImpl() override { ctx_->Release(); }
+    AMF_RESULT AMF_STD_CALL Init() override
+    {
+        init_ = ctx_->handle() != nullptr;
+        return init_ ? AMF_OK : AMF_NO_DEVICE;
+    }
+    AMF_RESULT AMF_STD_CALL Terminate() override
+    {
+        init_ = false;
+        return AMF_OK;
+    }
+    TANContext *AMF_STD_CALL GetContext() override { return ctx_; }
+    AMF_RESULT AMF_STD_CALL Convert(short *in, amf_size inStep, amf_size n, float *out, amf_size outStep,
+                                    float gain) override
+    {
+        if (!init_) return AMF_NOT_INITIALIZED;
+        return (AMF_RESULT)tanx_convert(ctx_->handle(), in, inStep, TANX_SAMPLE_SHORT, out, outStep,
+                                        TANX_SAMPLE_FLOAT, n, gain, nullptr);
+    }
+    AMF_RESULT AMF_STD_CALL Convert(float *in, amf_size inStep, amf_size n, short *out, amf_size outStep, float gain,
+                                    bool *clipped) override
+    {
+        if (!init_) return AMF_NOT_INITIALIZED;
+        int32_t c = 0;
+        const AMF_RESULT r = (AMF_RESULT)tanx_convert(ctx_->handle(), in, inStep, TANX_SAMPLE_FLOAT, out, outStep,
+                                                      TANX_SAMPLE_SHORT, n, gain, clipped ? &c : nullptr);
+        if (clipped) *clipped = c != 0;
+        return r;
+    }
+    // batch overloads: one conversion per buffer, clipping OR-ed
+    // (ConverterImpl.cpp:255-285, 420-456)
+    AMF_RESULT AMF_STD_CALL Convert(short **in, amf_size inStep, amf_size n, float **out, amf_size outStep,
+                                    float gain, int channels) override
+    {
+        if (!in || !out || channels <= 0) return AMF_INVALID_ARG;
+        for (int i = 0; i < channels; i++) {
+            const AMF_RESULT r = Convert(in[i], inStep, n, out[i], outStep, gain);
+            if (r != AMF_OK) return r;
+        }
+        return AMF_OK;
+    }
+    AMF_RESULT AMF_STD_CALL Convert(float **in, amf_size inStep, amf_size n, short **out, amf_size outStep,
+                                    float gain, int channels, bool *clipped) override
+    {
+        if (!in || !out || channels <= 0) return AMF_INVALID_ARG;
+        bool any = false;
+        for (int i = 0; i < channels; i++) {
+            bool c = false;
+            const AMF_RESULT r = Convert(in[i], inStep, n, out[i], outStep, gain, clipped ? &c : nullptr);
+            if (r != AMF_OK) return r;
+            any = any || c;
+        }
+        if (clipped) *clipped = any;
+        return AMF_OK;
+    }
+    AMF_RESULT AMF_STD_CALL Convert(tan_hip_mem in, amf_size inStep, amf_size inOff, TAN_SAMPLE_TYPE inType,
+                                    tan_hip_mem out, amf_size outStep, amf_size outOff, TAN_SAMPLE_TYPE outType,
+                                    amf_size n, float gain, bool *clipped) override
+    {
+        if (!init_) return AMF_NOT_INITIALIZED;
+        int32_t c = 0;
+        const AMF_RESULT r = (AMF_RESULT)tanx_convert_device(ctx_->handle(), in, inStep, inOff, (int32_t)inType, out,
+                                                             outStep, outOff, (int32_t)outType, n, gain,
+                                                             clipped ? &c : nullptr);
+        if (clipped) *clipped = c != 0;
+        return r;
+    }
+    AMF_RESULT AMF_STD_CALL Convert(tan_hip_mem *in, amf_size inStep, amf_size *inOff, TAN_SAMPLE_TYPE inType,
+                                    tan_hip_mem *out, amf_size outStep, amf_size *outOff, TAN_SAMPLE_TYPE outType,
+                                    amf_size n, float gain, int count, bool *clipped) override
+    {
+        if (!in || !out || !inOff || !outOff || count <= 0) return AMF_INVALID_ARG;
+        bool any = false;
+        for (int i = 0; i < count; i++) {
+            bool c = false;
+            const AMF_RESULT r = Convert(in[i], inStep, inOff[i], inType, out[i], outStep, outOff[i], outType, n,
+                                         gain, clipped ? &c : nullptr);
+            if (r != AMF_OK) return r;
+            any = any || c;
+        }
+        if (clipped) *clipped = any;
+        return AMF_OK;
+    }
+
+private:
+    ContextImpl *ctx_;
+    bool init_ = false;
+};
+
+class MixerImpl : public RefCounted<TANMixer> {
+public:
+    explicit MixerImpl(ContextImpl *ctx) : ctx_(ctx) { ctx_->Acquire(); }
+    ~MixerImpl() override
+    {
+        if (m_) tanx_mixer_destroy(m_);
+        ctx_->Release();
+    }
+    AMF_RESULT AMF_STD_CALL Init(amf_size bufferSize, int channels) override
+    {
+        if (m_) return AMF_ALREADY_INITIALIZED;
+        if (!ctx_->handle()) return AMF_NO_DEVICE;
+        return (AMF_RESULT)tanx_mixer_create(ctx_->handle(), bufferSize, channels, &m_);
+    }
+    AMF_RESULT AMF_STD_CALL Terminate() override
+    {
+        if (m_) tanx_mixer_destroy(m_);
+        m_ = nullptr;
+        return AMF_OK;
+    }
+    TANContext *AMF_STD_CALL GetContext() override { return ctx_; }
+    AMF_RESULT AMF_STD_CALL Mix(float *in[], float *out) override
+    {
+        if (!m_) return AMF_NOT_INITIALIZED;
+        return (AMF_RESULT)tanx_mix(m_, in, out);
+    }
+    AMF_RESULT AMF_STD_CALL Mix(tan_hip_mem in[], tan_hip_mem out) override
+    {
+        if (!m_) return AMF_NOT_INITIALIZED;
+        return (AMF_RESULT)tanx_mix_device(m_, reinterpret_cast<const float *const *>(in),
+                                           reinterpret_cast<float *>(out));
+    }
+    AMF_RESULT AMF_STD_CALL Mix(tan_hip_mem in, tan_hip_mem out, amf_size stride) override
+    {
+        if (!m_) return AMF_NOT_INITIALIZED;
+        return (AMF_RESULT)tanx_mix_device_strided(m_, reinterpret_cast<const float *>(in), stride,
+                                                   reinterpret_cast<float *>(out));
+    }
+
+private:
+    ContextImpl *ctx_;
+    tanx_mixer *m_ = nullptr;
+};
+
 }  // namespace
+
+extern "C" TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateConverter(amf::TANContext *pContext,
+                                                                     amf::TANConverter **ppConverter)
+{
+    if (!pContext || !ppConverter) return AMF_INVALID_POINTER;
+    ConverterImpl *c = new (std::nothrow) ConverterImpl(static_cast<ContextImpl *>(pContext));
+    if (!c) return AMF_OUT_OF_MEMORY;
+    c->Acquire();
+    *ppConverter = c;
+    return AMF_OK;
+}
+
+extern "C" TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateMixer(amf::TANContext *pContext, amf::TANMixer **ppMixer)
+{
+    if (!pContext || !ppMixer) return AMF_INVALID_POINTER;
+    MixerImpl *m = new (std::nothrow) MixerImpl(static_cast<ContextImpl *>(pContext));
+    if (!m) return AMF_OUT_OF_MEMORY;
+    m->Acquire();
+    *ppMixer = m;
+    return AMF_OK;
+}
 
 extern "C" TAN_SDK_LINK AMF_RESULT AMF_CDECL_CALL TANCreateContext(amf_uint64 version, amf::TANContext **ppContext)
 {

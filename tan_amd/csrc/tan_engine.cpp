@@ -75,6 +75,13 @@ struct tanx_ctx {
     size_t fft_dev_floats = 0;
     float *fft_host = nullptr;
     size_t fft_host_floats = 0;
+    // output chain: clip flag and host staging (converter, host mixer)
+    std::mutex io_mtx;
+    int *io_flag = nullptr;       // device
+    int *io_flag_host = nullptr;  // pinned
+    char *io_dev = nullptr;
+    char *io_host = nullptr;
+    size_t io_cap = 0;
 };
 
 static int32_t ctx_upload_twiddles(tanx_ctx *ctx)
@@ -135,6 +142,10 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
         if (ctx->g_host) (void)hipHostFree(ctx->g_host);
         if (ctx->g_dev) (void)hipFree(ctx->g_dev);
         for (auto &e : ctx->g_ev) if (e) (void)hipEventDestroy(e);
+        if (ctx->io_flag) (void)hipFree(ctx->io_flag);
+        if (ctx->io_flag_host) (void)hipHostFree(ctx->io_flag_host);
+        if (ctx->io_dev) (void)hipFree(ctx->io_dev);
+        if (ctx->io_host) (void)hipHostFree(ctx->io_host);
     }
     delete ctx;
     return TANX_OK;
@@ -327,11 +338,12 @@ static void conv_free(tanx_conv *cv)
 static int auto_chunks(int C, int P)
 {
     // enough workgroups to cover 256 CUs several times at small channel
-    // counts, while the partial sums stay a small fraction of the streamed
-    // bytes
+    // counts, while every chunk streams at least 8 partitions (a chunk of
+    // fewer rows is mostly launch and partial-sum overhead)
     int want = (1024 + C - 1) / C;
     want = std::max(1, std::min(want, 16));
-    return std::min(want, std::max(1, P - 2));
+    const int rows = std::max(1, (P - 2 + 7) / 8);
+    return std::min(want, rows);
 }
 
 static void set_chunks(tanx_conv *cv, int n)
@@ -358,7 +370,12 @@ static bool is_nu_method(uint32_t base)
 }
 
 // Frame chunks: enough workgroups at small channel counts, as for the tail.
-static int frame_chunks(int C, int PB) { return std::max(0, std::min(std::min(16, (1024 + C - 1) / C), PB - 2)); }
+static int frame_chunks(int C, int PB)
+{
+    if (PB <= 2) return 0;
+    const int by_rows = (PB - 2 + 7) / 8;  // at least eight partitions per chunk
+    return std::max(1, std::min(std::min(16, (1024 + C - 1) / C), by_rows));
+}
 
 // Modeled HBM bytes per block and channel of a frame multiple M (M = 1:
 // uniform).  The reference picks its multiple from a CPU cost model of FFT
@@ -746,15 +763,16 @@ static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const T
     hipStream_t s = cv->ctx->proc;
     if (cv->timing) {
         if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
-        TANX_CHECK(hipEventRecord(kt.start[kt.n], s));
+        // the launches stamp the events themselves: the time of the kernels alone
+        hipEvent_t e0 = kt.start[kt.n], e1 = kt.stop[kt.n];
+        TANX_CHECK(launch_tick(cv->B, tail_nf, a, s, e0, fa ? nullptr : e1));
+        if (fa) TANX_CHECK(launch_frame(cv->irtype, *fa, s, e1));
+        kt.n++;
+        alg_sum += alg_bytes;
+        return TANX_OK;
     }
     TANX_CHECK(launch_tick(cv->B, tail_nf, a, s));
     if (fa) TANX_CHECK(launch_frame(cv->irtype, *fa, s));
-    if (cv->timing) {
-        TANX_CHECK(hipEventRecord(kt.stop[kt.n], s));
-        kt.n++;
-        alg_sum += alg_bytes;
-    }
     return TANX_OK;
 }
 
@@ -1496,33 +1514,39 @@ extern "C" int32_t tanx_conv_autotune(tanx_conv *cv, uint32_t *chosen, double *b
     for (int n = 1; n <= cv->maxchunk && rc == TANX_OK; n *= 2) {
         set_chunks(cv, n);
         if (cv->nchunk != n) continue;
-        TickArgs a = tick_common(cv);
-        a.nhead = cv->C;
-        a.xin = cv->din;
-        a.in_stride = cv->B;
-        a.yout = cv->dout;
-        a.out_stride = cv->B;
-        a.xprev = cv->xprev;
-        a.slot = 0;
-        a.Ha = a.Hb = cv->slot_ptr(cv->cur);
-        a.Sa = a.Sb = cv->scale_ptr(cv->cur);
-        a.Ta = a.Tb = cv->tail_ptr(0, 0);
-        a.tchunk = cv->nchunk;
-        a.xfade = 0;
-        a.nchunk = cv->nchunk;
-        a.pc = cv->pc;
-        a.tbase = 1 % cv->P;
-        a.tHa = cv->slot_ptr(cv->cur);
-        a.tHb = nullptr;
-        a.tSa = cv->scale_ptr(cv->cur);
-        a.tSb = nullptr;
-        a.tTa = cv->tail_ptr(1, 0);
-        a.tTb = nullptr;
-        const int reps = cv->C >= 4096 ? 3 : 12;
-        for (int w = 0; w < 2 && rc == TANX_OK; w++) rc = map_hip(launch_tick(cv->B, 1, a, s));
+        // consecutive blocks as Process runs them (silent input): ring slots
+        // and partial-sum buffers rotate, so every launch streams the whole
+        // FDL and IR slot as a real block does
+        auto block_args = [&](int t) {
+            TickArgs a = tick_common(cv);
+            a.nhead = cv->C;
+            a.xin = cv->din;
+            a.in_stride = cv->B;
+            a.yout = cv->dout;
+            a.out_stride = cv->B;
+            a.xprev = cv->xprev;
+            a.slot = t % cv->P;
+            a.Ha = a.Hb = cv->slot_ptr(cv->cur);
+            a.Sa = a.Sb = cv->scale_ptr(cv->cur);
+            a.Ta = a.Tb = cv->tail_ptr(t & 1, 0);
+            a.tchunk = cv->nchunk;
+            a.xfade = 0;
+            a.nchunk = cv->nchunk;
+            a.pc = cv->pc;
+            a.tbase = (t + 1) % cv->P;
+            a.tHa = cv->slot_ptr(cv->cur);
+            a.tHb = nullptr;
+            a.tSa = cv->scale_ptr(cv->cur);
+            a.tSb = nullptr;
+            a.tTa = cv->tail_ptr((t + 1) & 1, 0);
+            a.tTb = nullptr;
+            return a;
+        };
+        const int reps = cv->C >= 4096 ? 4 : 32;
+        for (int w = 0; w < 4 && rc == TANX_OK; w++) rc = map_hip(launch_tick(cv->B, 1, block_args(w), s));
         if (rc != TANX_OK) break;
         if ((rc = map_hip(hipEventRecord(e0, s))) != TANX_OK) break;
-        for (int r = 0; r < reps && rc == TANX_OK; r++) rc = map_hip(launch_tick(cv->B, 1, a, s));
+        for (int r = 0; r < reps && rc == TANX_OK; r++) rc = map_hip(launch_tick(cv->B, 1, block_args(4 + r), s));
         if (rc != TANX_OK) break;
         if ((rc = map_hip(hipEventRecord(e1, s))) != TANX_OK) break;
         if ((rc = map_hip(hipEventSynchronize(e1))) != TANX_OK) break;
@@ -1673,3 +1697,220 @@ extern "C" int32_t tanx_fft_transform(tanx_ctx *ctx, uint32_t direction, uint32_
 }
 
 extern "C" const char *tanx_version(void) { return "tan_mi355x 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// output chain: TANMixer and TANConverter on the device
+// ---------------------------------------------------------------------------
+struct tanx_mixer {
+    tanx_ctx *ctx = nullptr;
+    std::mutex mtx;
+    size_t n = 0;
+    int C = 0;
+    // device copies of pointer lists (ring) and their pinned staging
+    const float **d_ptrs = nullptr;   // [kRunRing][C]
+    const float **h_ptrs = nullptr;   // [kRunRing][C]
+    hipEvent_t ev[kRunRing] = {};
+    int ring = 0;
+};
+
+// host staging of at least `bytes` (device and pinned); the caller holds io_mtx
+static int32_t io_staging(tanx_ctx *ctx, size_t bytes)
+{
+    if (!ctx->io_flag) {
+        TANX_CHECK(hipMalloc(&ctx->io_flag, 16));
+        TANX_CHECK(hipHostMalloc(&ctx->io_flag_host, 16));
+    }
+    if (ctx->io_cap >= bytes) return TANX_OK;
+    TANX_CHECK(hipStreamSynchronize(ctx->proc));
+    if (ctx->io_dev) (void)hipFree(ctx->io_dev);
+    if (ctx->io_host) (void)hipHostFree(ctx->io_host);
+    ctx->io_dev = nullptr;
+    ctx->io_host = nullptr;
+    ctx->io_cap = 0;
+    size_t cap = 4096;
+    while (cap < bytes) cap *= 2;
+    TANX_CHECK(hipMalloc(&ctx->io_dev, cap));
+    TANX_CHECK(hipHostMalloc(&ctx->io_host, cap));
+    ctx->io_cap = cap;
+    return TANX_OK;
+}
+
+/* TANMixer::Init (MixerImpl.cpp Init(buffer_size, num_channels)). */
+extern "C" int32_t tanx_mixer_create(tanx_ctx *ctx, size_t buffer_size, int32_t channels, tanx_mixer **out)
+{
+    if (!ctx || !out) return TANX_INVALID_POINTER;
+    *out = nullptr;
+    if (buffer_size == 0 || channels <= 0) return TANX_INVALID_ARG;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return TANX_NO_DEVICE;
+    tanx_mixer *m = new (std::nothrow) tanx_mixer();
+    if (!m) return TANX_OUT_OF_MEMORY;
+    m->ctx = ctx;
+    m->n = buffer_size;
+    m->C = channels;
+    hipError_t e = hipMalloc(&m->d_ptrs, sizeof(float *) * channels * kRunRing);
+    if (e == hipSuccess) e = hipHostMalloc(&m->h_ptrs, sizeof(float *) * channels * kRunRing);
+    for (auto &ev : m->ev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    for (auto &ev : m->ev)
+        if (e == hipSuccess) e = hipEventRecord(ev, ctx->proc);
+    if (e != hipSuccess) {
+        if (m->d_ptrs) (void)hipFree(m->d_ptrs);
+        if (m->h_ptrs) (void)hipHostFree(m->h_ptrs);
+        for (auto &ev : m->ev) if (ev) (void)hipEventDestroy(ev);
+        delete m;
+        return map_hip(e);
+    }
+    *out = m;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_mixer_destroy(tanx_mixer *m)
+{
+    if (!m) return TANX_INVALID_POINTER;
+    DeviceGuard g(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->proc);
+    if (m->d_ptrs) (void)hipFree(m->d_ptrs);
+    if (m->h_ptrs) (void)hipHostFree(m->h_ptrs);
+    for (auto &ev : m->ev) if (ev) (void)hipEventDestroy(ev);
+    delete m;
+    return TANX_OK;
+}
+
+/* TANMixer::Mix(cl_mem pBufferInput[], cl_mem pBufferOutput)
+ * (MixerImpl.cpp): d_in[c] are device buffers of buffer_size floats.
+ * Asynchronous on the context's process stream. */
+extern "C" int32_t tanx_mix_device(tanx_mixer *m, const float *const *d_in, float *d_out)
+{
+    if (!m) return TANX_NOT_INITIALIZED;
+    if (!d_in || !d_out) return TANX_INVALID_ARG;
+    for (int c = 0; c < m->C; c++)
+        if (!d_in[c]) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(m->mtx);
+    DeviceGuard g(m->ctx->device);
+    const int slot = m->ring;
+    m->ring = (m->ring + 1) % kRunRing;
+    TANX_CHECK(hipEventSynchronize(m->ev[slot]));
+    const float **h = m->h_ptrs + (size_t)slot * m->C;
+    const float **d = m->d_ptrs + (size_t)slot * m->C;
+    for (int c = 0; c < m->C; c++) h[c] = d_in[c];
+    hipStream_t s = m->ctx->proc;
+    TANX_CHECK(hipMemcpyAsync(d, h, sizeof(float *) * m->C, hipMemcpyHostToDevice, s));
+    TANX_CHECK(hipEventRecord(m->ev[slot], s));
+    TANX_CHECK(launch_mix(d, nullptr, 0, m->C, (long long)m->n, d_out, s));
+    return TANX_OK;
+}
+
+/* TANMixer::Mix(cl_mem pBufferInput, cl_mem pBufferOutput, amf_size inputStride)
+ * (MixerImpl.cpp): channel c at d_in + c * in_stride.  Asynchronous. */
+extern "C" int32_t tanx_mix_device_strided(tanx_mixer *m, const float *d_in, size_t in_stride, float *d_out)
+{
+    if (!m) return TANX_NOT_INITIALIZED;
+    if (!d_in || !d_out || (m->C > 1 && in_stride < m->n)) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(m->mtx);
+    DeviceGuard g(m->ctx->device);
+    TANX_CHECK(launch_mix(nullptr, d_in, (long long)in_stride, m->C, (long long)m->n, d_out, m->ctx->proc));
+    return TANX_OK;
+}
+
+/* TANMixer::Mix(float* ppBufferInput[], float* ppBufferOutput) (MixerImpl.cpp):
+ * host buffers; returns with the output written. */
+extern "C" int32_t tanx_mix(tanx_mixer *m, const float *const *in, float *out)
+{
+    if (!m) return TANX_NOT_INITIALIZED;
+    if (!in || !out) return TANX_INVALID_ARG;
+    for (int c = 0; c < m->C; c++)
+        if (!in[c]) return TANX_INVALID_ARG;
+    tanx_ctx *ctx = m->ctx;
+    std::lock_guard<std::mutex> lk(m->mtx);
+    std::lock_guard<std::mutex> lk2(ctx->io_mtx);
+    DeviceGuard g(ctx->device);
+    const size_t n = m->n, bytes = (size_t)(m->C + 1) * n * sizeof(float);
+    int32_t rc = io_staging(ctx, bytes);
+    if (rc != TANX_OK) return rc;
+    hipStream_t s = ctx->proc;
+    TANX_CHECK(hipStreamSynchronize(s));  // staging free
+    float *h = reinterpret_cast<float *>(ctx->io_host), *d = reinterpret_cast<float *>(ctx->io_dev);
+    for (int c = 0; c < m->C; c++) std::memcpy(h + (size_t)c * n, in[c], n * sizeof(float));
+    TANX_CHECK(hipMemcpyAsync(d, h, (size_t)m->C * n * sizeof(float), hipMemcpyHostToDevice, s));
+    TANX_CHECK(launch_mix(nullptr, d, (long long)n, m->C, (long long)n, d + (size_t)m->C * n, s));
+    TANX_CHECK(hipMemcpyAsync(h + (size_t)m->C * n, d + (size_t)m->C * n, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    TANX_CHECK(hipStreamSynchronize(s));
+    std::memcpy(out, h + (size_t)m->C * n, n * sizeof(float));
+    return TANX_OK;
+}
+
+static int sample_bytes(int32_t type) { return type == TANX_SAMPLE_SHORT ? 2 : 4; }
+
+/* TANConverter::Convert(cl_mem, inputStep, inputOffset, inputType, cl_mem,
+ * outputStep, outputOffset, outputType, n, gain, outputClipped)
+ * (ConverterImpl.cpp:458-576): element i read at in[i * in_step + in_offset],
+ * written at out[i * out_step + out_offset].  Asynchronous on the process
+ * stream unless clipped is given: then it waits and reports whether
+ * float -> short clamped any sample.  Returns AMF_OK when clipping happened,
+ * as the reference does. */
+extern "C" int32_t tanx_convert_device(tanx_ctx *ctx, const void *d_in, size_t in_step, size_t in_offset,
+                                       int32_t in_type, void *d_out, size_t out_step, size_t out_offset,
+                                       int32_t out_type, size_t n, float gain, int32_t *clipped)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    if (!d_in || !d_out || n == 0 || !(gain > 0.f)) return TANX_INVALID_ARG;
+    if ((in_type != TANX_SAMPLE_FLOAT && in_type != TANX_SAMPLE_SHORT) ||
+        (out_type != TANX_SAMPLE_FLOAT && out_type != TANX_SAMPLE_SHORT))
+        return TANX_NOT_IMPLEMENTED;
+    std::lock_guard<std::mutex> lk(ctx->io_mtx);
+    DeviceGuard g(ctx->device);
+    int32_t rc = io_staging(ctx, 0);
+    if (rc != TANX_OK) return rc;
+    hipStream_t s = ctx->proc;
+    const bool flag = clipped && in_type == TANX_SAMPLE_FLOAT && out_type == TANX_SAMPLE_SHORT;
+    if (flag) TANX_CHECK(launch_zero(ctx->io_flag, 16, s));
+    TANX_CHECK(launch_convert(d_in, (long long)in_step, (long long)in_offset, in_type, d_out, (long long)out_step,
+                              (long long)out_offset, out_type, (long long)n, gain, flag ? ctx->io_flag : nullptr, s));
+    if (clipped) {
+        *clipped = 0;
+        if (flag) {
+            TANX_CHECK(hipMemcpyAsync(ctx->io_flag_host, ctx->io_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            TANX_CHECK(hipStreamSynchronize(s));
+            *clipped = *ctx->io_flag_host ? 1 : 0;
+        }
+    }
+    return TANX_OK;
+}
+
+/* TANConverter::Convert on host buffers (ConverterImpl.cpp:217-453): the
+ * same conversion as the device path, run on the device through staging;
+ * returns with the output written. */
+extern "C" int32_t tanx_convert(tanx_ctx *ctx, const void *in, size_t in_step, int32_t in_type, void *out,
+                                size_t out_step, int32_t out_type, size_t n, float gain, int32_t *clipped)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    if (!in || !out || n == 0 || !(gain > 0.f) || in_step == 0 || out_step == 0) return TANX_INVALID_ARG;
+    if ((in_type != TANX_SAMPLE_FLOAT && in_type != TANX_SAMPLE_SHORT) ||
+        (out_type != TANX_SAMPLE_FLOAT && out_type != TANX_SAMPLE_SHORT))
+        return TANX_NOT_IMPLEMENTED;
+    const size_t ib = sample_bytes(in_type), ob = sample_bytes(out_type);
+    const size_t in_span = ((n - 1) * in_step + 1) * ib, out_span = ((n - 1) * out_step + 1) * ob;
+    const size_t out_off = (in_span + 15) & ~size_t(15);
+    std::lock_guard<std::mutex> lk(ctx->io_mtx);
+    DeviceGuard g(ctx->device);
+    int32_t rc = io_staging(ctx, out_off + out_span);
+    if (rc != TANX_OK) return rc;
+    hipStream_t s = ctx->proc;
+    TANX_CHECK(hipStreamSynchronize(s));  // staging free
+    char *h = ctx->io_host, *d = ctx->io_dev;
+    std::memcpy(h, in, in_span);
+    // strided outputs keep the samples between the written ones
+    std::memcpy(h + out_off, out, out_span);
+    TANX_CHECK(hipMemcpyAsync(d, h, out_off + out_span, hipMemcpyHostToDevice, s));
+    const bool flag = in_type == TANX_SAMPLE_FLOAT && out_type == TANX_SAMPLE_SHORT;
+    if (flag) TANX_CHECK(launch_zero(ctx->io_flag, 16, s));
+    TANX_CHECK(launch_convert(d, (long long)in_step, 0, in_type, d + out_off, (long long)out_step, 0, out_type,
+                              (long long)n, gain, flag ? ctx->io_flag : nullptr, s));
+    TANX_CHECK(hipMemcpyAsync(h + out_off, d + out_off, out_span, hipMemcpyDeviceToHost, s));
+    if (flag) TANX_CHECK(hipMemcpyAsync(ctx->io_flag_host, ctx->io_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    TANX_CHECK(hipStreamSynchronize(s));
+    std::memcpy(out, h + out_off, out_span);
+    if (clipped) *clipped = flag && *ctx->io_flag_host ? 1 : 0;
+    return TANX_OK;
+}

@@ -151,7 +151,9 @@ struct FftArgs {
 };
 
 // Launchers (return hipError_t).  B is the block size (power of two).
-hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s);
+// ev0/ev1 (optional): events stamped with the kernel's own start and end
+hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hipEvent_t ev0 = nullptr,
+                       hipEvent_t ev1 = nullptr);
 // One launch for several objects: block b belongs to the object o with
 // starts[o] <= b < starts[o + 1] and runs its roles from d_descs[o].  Every
 // object must use irtype and a block size in {64, 128, 256, 512}.
@@ -163,10 +165,16 @@ hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, u
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);
 // frame kernels: N in {128, ..., 4096}
 constexpr int kMaxFrame = 4096;
-hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s);
+hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s, hipEvent_t ev1 = nullptr);
 hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
                               hipStream_t s);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
+// output chain: channel sum, and sample conversion (types 0 = float, 1 = short)
+hipError_t launch_mix(const float *const *d_ins, const float *d_in, long long stride, int channels, long long n,
+                      float *d_out, hipStream_t s);
+hipError_t launch_convert(const void *in, long long in_step, long long in_off, int in_type, void *out,
+                          long long out_step, long long out_off, int out_type, long long n, float gain, int *d_flag,
+                          hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 
 // Twiddle tables: tw + (2^l - 1) holds W_{2^l}^t = exp(-2 pi i t / 2^l), t < 2^l,

@@ -16,6 +16,7 @@
 // FP64 on the host and rounded to FP32.
 #include "tan_internal.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 namespace tanx {
@@ -640,10 +641,14 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
 // ---------------------------------------------------------------------------
 // Frame kernel of non-uniform objects (see FrameArgs), 256 threads.
 // ---------------------------------------------------------------------------
-template <int N, int HT>
+// threads of a frame workgroup: one radix-4 butterfly per thread and stage
+template <int N>
+constexpr int frame_threads() { return N / 4 < 256 ? 256 : (N / 4 > 1024 ? 1024 : N / 4); }
+constexpr int kGroupFrameThreads = 1024;
+
+template <int N, int NT, int HT>
 __device__ void frame_role(const FrameArgs &a, int c, float2 *s0, float2 *s1)
 {
-    constexpr int NT = kTickThreads;
     constexpr int KPT = N >= NT ? N / NT : 1;
     constexpr float kScale = 1.0f / (2.0f * (float)N);
     const int tid = threadIdx.x;
@@ -721,17 +726,17 @@ __device__ void frame_role(const FrameArgs &a, int c, float2 *s0, float2 *s1)
 }
 
 template <int N, int HT>
-__global__ __launch_bounds__(kTickThreads) void k_frame(FrameArgs a)
+__global__ __launch_bounds__(frame_threads<N>()) void k_frame(FrameArgs a)
 {
     __shared__ float2 s0[N], s1[N];
     const int c = blockIdx.x;
     if (!a.run[c]) return;
-    frame_role<N, HT>(a, c, s0, s1);
+    frame_role<N, frame_threads<N>(), HT>(a, c, s0, s1);
 }
 
 template <int HT>
-__global__ __launch_bounds__(kTickThreads) void k_group_frame(const FrameArgs *__restrict__ descs,
-                                                              const int *__restrict__ starts, int nobj)
+__global__ __launch_bounds__(kGroupFrameThreads) void k_group_frame(const FrameArgs *__restrict__ descs,
+                                                                    const int *__restrict__ starts, int nobj)
 {
     __shared__ float2 s0[kMaxFrame], s1[kMaxFrame];
     const int b = blockIdx.x;
@@ -744,13 +749,14 @@ __global__ __launch_bounds__(kTickThreads) void k_group_frame(const FrameArgs *_
     const FrameArgs a = descs[lo];
     const int c = b - starts[lo];
     if (c >= a.C || !a.run[c]) return;
+    constexpr int NT = kGroupFrameThreads;
     switch (a.N) {
-    case 128: frame_role<128, HT>(a, c, s0, s1); break;
-    case 256: frame_role<256, HT>(a, c, s0, s1); break;
-    case 512: frame_role<512, HT>(a, c, s0, s1); break;
-    case 1024: frame_role<1024, HT>(a, c, s0, s1); break;
-    case 2048: frame_role<2048, HT>(a, c, s0, s1); break;
-    case 4096: frame_role<4096, HT>(a, c, s0, s1); break;
+    case 128: frame_role<128, NT, HT>(a, c, s0, s1); break;
+    case 256: frame_role<256, NT, HT>(a, c, s0, s1); break;
+    case 512: frame_role<512, NT, HT>(a, c, s0, s1); break;
+    case 1024: frame_role<1024, NT, HT>(a, c, s0, s1); break;
+    case 2048: frame_role<2048, NT, HT>(a, c, s0, s1); break;
+    case 4096: frame_role<4096, NT, HT>(a, c, s0, s1); break;
     default: break;
     }
 }
@@ -876,6 +882,50 @@ __global__ __launch_bounds__(M / 4 > 64 ? (M / 4 < 1024 ? M / 4 : 1024) : 64) vo
     }
 }
 
+// ---------------------------------------------------------------------------
+// Output chain: mixer and sample converter (grid-stride, one sample per lane).
+// ---------------------------------------------------------------------------
+// Sum of the channels in channel order (the reference's order), channel c at
+// ins[c] (device pointer list) or at in + c * stride.
+__global__ void k_mix(const float *const *__restrict__ ins, const float *__restrict__ in, long long stride,
+                      int channels, long long n, float *__restrict__ out)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+        for (int c = 0; c < channels; c++) {
+            const float v = ins ? ins[c][i] : in[(size_t)c * stride + i];
+            acc = c == 0 ? v : acc + v;
+        }
+        out[i] = acc;
+    }
+}
+
+// IT/OT: 0 = float, 1 = short.  float -> short scales by 32767 * gain,
+// clamps to [-32768, 32767] (setting *flag) and converts toward zero;
+// short -> float scales by gain / 32767.
+template <int IT, int OT>
+__global__ void k_convert(const void *__restrict__ in, long long in_step, long long in_off, void *__restrict__ out,
+                          long long out_step, long long out_off, long long n, float gain, int *__restrict__ flag)
+{
+    bool clip = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long si = i * in_step + in_off, so = i * out_step + out_off;
+        if constexpr (IT == 0 && OT == 1) {
+            float f = static_cast<const float *>(in)[si] * (32767.0f * gain);
+            if (f > 32767.0f) { f = 32767.0f; clip = true; }
+            if (f < -32768.0f) { f = -32768.0f; clip = true; }
+            static_cast<short *>(out)[so] = (short)(int)f;
+        } else if constexpr (IT == 1 && OT == 0) {
+            static_cast<float *>(out)[so] = (float)static_cast<const short *>(in)[si] * (gain / 32767.0f);
+        } else if constexpr (IT == 0) {
+            static_cast<float *>(out)[so] = static_cast<const float *>(in)[si];
+        } else {
+            static_cast<short *>(out)[so] = static_cast<const short *>(in)[si];
+        }
+    }
+    if (clip && flag) *flag = 1;
+}
+
 __global__ void k_zero(float4 *p, size_t n4)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
@@ -903,25 +953,28 @@ static inline int fft_threads(int M)
     default: return hipErrorInvalidValue;               \
     }
 
+// With ev0/ev1, the launch itself records the kernel's start and end (the
+// dispatch's own timestamps), so event timing measures the kernel alone.
 template <int HT>
-static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream_t s, unsigned blocks)
+static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream_t s, unsigned blocks,
+                                hipEvent_t ev0, hipEvent_t ev1)
 {
     if (tail_nf == 2) {
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 2, HT>), dim3(blocks), dim3(kTickThreads), 0, s, a));
+        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
     } else {
-        TANX_SWITCH_B(B, hipLaunchKernelGGL((k_tick<M, 1, HT>), dim3(blocks), dim3(kTickThreads), 0, s, a));
+        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
     }
     return hipGetLastError();
 }
 
-hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s)
+hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const unsigned blocks = (unsigned)(a.nhead + a.C * a.nchunk + a.C * a.nchB);
     if (blocks == 0) return hipSuccess;
     switch (a.irtype) {
-    case IR_F32: return launch_tick_t<IR_F32>(B, tail_nf, a, s, blocks);
-    case IR_F16: return launch_tick_t<IR_F16>(B, tail_nf, a, s, blocks);
-    case IR_BF16: return launch_tick_t<IR_BF16>(B, tail_nf, a, s, blocks);
+    case IR_F32: return launch_tick_t<IR_F32>(B, tail_nf, a, s, blocks, ev0, ev1);
+    case IR_F16: return launch_tick_t<IR_F16>(B, tail_nf, a, s, blocks, ev0, ev1);
+    case IR_BF16: return launch_tick_t<IR_BF16>(B, tail_nf, a, s, blocks, ev0, ev1);
     default: return hipErrorInvalidValue;
     }
 }
@@ -999,13 +1052,13 @@ hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
     default: return hipErrorInvalidValue;                \
     }
 
-hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s)
+hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s, hipEvent_t ev1)
 {
     if (a.C == 0) return hipSuccess;
     switch (irtype) {
-    case IR_F32: TANX_SWITCH_FRAME(a.N, hipLaunchKernelGGL((k_frame<F, IR_F32>), dim3(a.C), dim3(kTickThreads), 0, s, a)); break;
-    case IR_F16: TANX_SWITCH_FRAME(a.N, hipLaunchKernelGGL((k_frame<F, IR_F16>), dim3(a.C), dim3(kTickThreads), 0, s, a)); break;
-    case IR_BF16: TANX_SWITCH_FRAME(a.N, hipLaunchKernelGGL((k_frame<F, IR_BF16>), dim3(a.C), dim3(kTickThreads), 0, s, a)); break;
+    case IR_F32: TANX_SWITCH_FRAME(a.N, hipExtLaunchKernelGGL((k_frame<F, IR_F32>), dim3(a.C), dim3(frame_threads<F>()), 0, s, nullptr, ev1, 0, a)); break;
+    case IR_F16: TANX_SWITCH_FRAME(a.N, hipExtLaunchKernelGGL((k_frame<F, IR_F16>), dim3(a.C), dim3(frame_threads<F>()), 0, s, nullptr, ev1, 0, a)); break;
+    case IR_BF16: TANX_SWITCH_FRAME(a.N, hipExtLaunchKernelGGL((k_frame<F, IR_BF16>), dim3(a.C), dim3(frame_threads<F>()), 0, s, nullptr, ev1, 0, a)); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1016,9 +1069,9 @@ hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d
 {
     if (blocks == 0 || nobj == 0) return hipSuccess;
     switch (irtype) {
-    case IR_F32: hipLaunchKernelGGL((k_group_frame<IR_F32>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
-    case IR_F16: hipLaunchKernelGGL((k_group_frame<IR_F16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
-    case IR_BF16: hipLaunchKernelGGL((k_group_frame<IR_BF16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_F32: hipLaunchKernelGGL((k_group_frame<IR_F32>), dim3(blocks), dim3(kGroupFrameThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_F16: hipLaunchKernelGGL((k_group_frame<IR_F16>), dim3(blocks), dim3(kGroupFrameThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_BF16: hipLaunchKernelGGL((k_group_frame<IR_BF16>), dim3(blocks), dim3(kGroupFrameThreads), 0, s, d_descs, d_starts, nobj); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1037,6 +1090,40 @@ hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s)
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+static unsigned io_blocks(long long n)
+{
+    long long b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_mix(const float *const *d_ins, const float *d_in, long long stride, int channels, long long n,
+                      float *d_out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mix, dim3(io_blocks(n)), dim3(256), 0, s, d_ins, d_in, stride, channels, n, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const void *in, long long in_step, long long in_off, int in_type, void *out,
+                          long long out_step, long long out_off, int out_type, long long n, float gain, int *d_flag,
+                          hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 g(io_blocks(n)), t(256);
+    if (in_type == 0 && out_type == 1)
+        hipLaunchKernelGGL((k_convert<0, 1>), g, t, 0, s, in, in_step, in_off, out, out_step, out_off, n, gain, d_flag);
+    else if (in_type == 1 && out_type == 0)
+        hipLaunchKernelGGL((k_convert<1, 0>), g, t, 0, s, in, in_step, in_off, out, out_step, out_off, n, gain, d_flag);
+    else if (in_type == 0 && out_type == 0)
+        hipLaunchKernelGGL((k_convert<0, 0>), g, t, 0, s, in, in_step, in_off, out, out_step, out_off, n, gain, d_flag);
+    else if (in_type == 1 && out_type == 1)
+        hipLaunchKernelGGL((k_convert<1, 1>), g, t, 0, s, in, in_step, in_off, out, out_step, out_off, n, gain, d_flag);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -137,7 +137,7 @@ class RowAllocator:
 
 class Session:
     __slots__ = ("sid", "channels", "ir_len", "conv", "inp", "out", "row", "nbytes", "blocks", "arrived_tick",
-                 "admitted_tick", "updates")
+                 "admitted_tick", "updates", "multiple")
 
     def __init__(self, sid, channels, ir_len):
         self.sid = sid
@@ -147,6 +147,7 @@ class Session:
         self.inp = None
         self.out = None
         self.row = None
+        self.multiple = 1
         self.nbytes = 0
         self.blocks = 0
         self.arrived_tick = 0
@@ -241,6 +242,7 @@ class SessionServer:
                 self.admission.release(nb)
                 raise RuntimeError(f"UpdateResponseTD failed for session {sid}: AMF_RESULT {rc}")
             s.conv = conv
+            s.multiple = conv.Info()["nu_multiple"]
             s.nbytes = nb
             r = self.rows.alloc(s.channels) if self.rows else None
             if r is not None:
@@ -285,7 +287,7 @@ class SessionServer:
     def _rebuild(self):
         ss = list(self.active.values())
         self._call = GroupCall([s.conv for s in ss], [s.inp for s in ss], [s.out for s in ss])
-        self._bytes_tick = sum(alg_bytes_per_block(s.ir_len, self.block, s.channels) for s in ss)
+        self._bytes_tick = sum(alg_bytes_per_block(s.ir_len, self.block, s.channels, s.multiple) for s in ss)
         self._chan_tick = sum(s.channels for s in ss)
 
     def stream(self):

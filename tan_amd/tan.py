@@ -372,3 +372,119 @@ def GroupProcess(convs, inputs, outputs, flag_masks=None):
                 arr[i] = a.ctypes.data
         fptr = ctypes.cast(arr, ctypes.c_void_p)
     return lib.tanx_group_process_device(n, hs, ins, ist, outs, ost, fptr)
+
+
+# TAN_SAMPLE_TYPE (TrueAudioNext.h:73-77)
+TAN_SAMPLE_TYPE_FLOAT = 0
+TAN_SAMPLE_TYPE_SHORT = 1
+
+
+def _sample_type(dtype):
+    if dtype in (np.float32, "float32") or str(dtype).endswith("float32"):
+        return TAN_SAMPLE_TYPE_FLOAT
+    if dtype in (np.int16, "int16") or str(dtype).endswith("int16"):
+        return TAN_SAMPLE_TYPE_SHORT
+    raise ValueError(f"unsupported sample type {dtype}")
+
+
+class TANConverter:
+    """TANConverter (converter/ConverterImpl.cpp) on the GPU: float <-> short
+    with gain and clamping, strided and offset."""
+
+    def __init__(self, context):
+        self._ctx = context
+        self._lib = context._lib
+
+    def Init(self):
+        return AMF_OK
+
+    def Terminate(self):
+        return AMF_OK
+
+    def GetContext(self):
+        return self._ctx
+
+    def Convert(self, inputs, input_step, num_samples, outputs, output_step, gain,
+                input_offset=0, output_offset=0, want_clipped=True):
+        """Returns (AMF_RESULT, clipped).  numpy arrays (float32 / int16) use the
+        host overload; torch CUDA tensors the device overload (asynchronous
+        unless want_clipped)."""
+        clipped = ctypes.c_int32(0)
+        cp = ctypes.byref(clipped) if want_clipped else None
+        if _is_device_tensor(inputs):
+            rc = self._lib.tanx_convert_device(
+                self._ctx.handle, inputs.data_ptr(), input_step, input_offset, _sample_type(inputs.dtype),
+                outputs.data_ptr(), output_step, output_offset, _sample_type(outputs.dtype), num_samples,
+                float(gain), cp)
+            return rc, bool(clipped.value)
+        if not (inputs.flags.c_contiguous and outputs.flags.c_contiguous):
+            raise ValueError("host buffers must be C-contiguous")
+        rc = self._lib.tanx_convert(
+            self._ctx.handle, inputs.ctypes.data + input_offset * inputs.itemsize, input_step,
+            _sample_type(inputs.dtype), outputs.ctypes.data + output_offset * outputs.itemsize, output_step,
+            _sample_type(outputs.dtype), num_samples, float(gain), cp)
+        return rc, bool(clipped.value)
+
+
+class TANMixer:
+    """TANMixer (mixer/MixerImpl.cpp) on the GPU: the channel sum in channel
+    order."""
+
+    def __init__(self, context):
+        self._ctx = context
+        self._lib = context._lib
+        self.handle = None
+        self.buffer_size = 0
+        self.channels = 0
+
+    def Init(self, buffer_size, num_channels):
+        if self.handle:
+            return AMF_ALREADY_INITIALIZED
+        h = ctypes.c_void_p()
+        rc = self._lib.tanx_mixer_create(self._ctx.handle, buffer_size, num_channels, ctypes.byref(h))
+        if rc == AMF_OK:
+            self.handle = h
+            self.buffer_size = buffer_size
+            self.channels = num_channels
+        return rc
+
+    def Terminate(self):
+        if self.handle:
+            rc = self._lib.tanx_mixer_destroy(self.handle)
+            self.handle = None
+            return rc
+        return AMF_OK
+
+    def GetContext(self):
+        return self._ctx
+
+    def __del__(self):
+        try:
+            self.Terminate()
+        except Exception:
+            pass
+
+    def Mix(self, inputs, output, input_stride=None):
+        """inputs: a (channels, >= buffer_size) array or a list of per-channel
+        arrays; numpy -> host overload, torch CUDA -> device overloads (one
+        2-D tensor: the strided one; a list: the pointer-list one)."""
+        if not self.handle:
+            return AMF_NOT_INITIALIZED
+        if _is_device_tensor(output):
+            if _is_device_tensor(inputs):
+                stride = inputs.stride(0) if input_stride is None else input_stride
+                return self._lib.tanx_mix_device_strided(self.handle, inputs.data_ptr(), stride, output.data_ptr())
+            arr = (ctypes.c_void_p * self.channels)(*[t.data_ptr() for t in inputs])
+            return self._lib.tanx_mix_device(self.handle, arr, output.data_ptr())
+        ptrs, _keep = _host_rows(inputs, self.channels, self.buffer_size)
+        if not (isinstance(output, np.ndarray) and output.dtype == np.float32 and output.flags.c_contiguous):
+            raise ValueError("output must be a C-contiguous float32 numpy array")
+        return self._lib.tanx_mix(self.handle, ptrs, output.ctypes.data)
+
+
+def TANCreateConverter(context):
+    return AMF_OK, TANConverter(context)
+
+
+def TANCreateMixer(context):
+    return AMF_OK, TANMixer(context)

@@ -47,6 +47,11 @@ def _load():
     lib.orc_set_mutation.argtypes = [ctypes.c_int]
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_num_procs.argtypes = []
+    lib.orc_mix.argtypes = [ctypes.POINTER(_f32p), ctypes.c_int, ctypes.c_size_t, _f32p]
+    lib.orc_convert.restype = ctypes.c_int
+    lib.orc_convert.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t, ctypes.c_float,
+                                ctypes.POINTER(ctypes.c_int)]
     lib.orc_bench_upols.restype = ctypes.c_double
     lib.orc_bench_upols.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_uint64, _f64p]
@@ -150,3 +155,23 @@ def set_mutation(m):
 
 def set_threads(n):
     return lib.orc_set_threads(n)
+
+
+def mix(rows, n=None):
+    """orc_mix over the rows of a (C, >= n) float32 array."""
+    a = np.ascontiguousarray(rows, dtype=np.float32)
+    n = a.shape[1] if n is None else n
+    out = np.zeros(n, np.float32)
+    lib.orc_mix(_rows(a), a.shape[0], n, out.ctypes.data_as(_f32p))
+    return out
+
+
+def convert(inp, in_step, in_offset, out, out_step, out_offset, n, gain):
+    """orc_convert between numpy float32 / int16 arrays (out is written in
+    place); returns the clip flag."""
+    t = {np.dtype(np.float32): 0, np.dtype(np.int16): 1}
+    clipped = ctypes.c_int(0)
+    rc = lib.orc_convert(inp.ctypes.data, in_step, in_offset, t[inp.dtype], out.ctypes.data, out_step, out_offset,
+                         t[out.dtype], n, gain, ctypes.byref(clipped))
+    assert rc == 0
+    return bool(clipped.value)

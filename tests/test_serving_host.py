@@ -159,3 +159,14 @@ def test_nonuniform_estimate_and_options():
     for mult, L, B in [(3, 65536, 256), (32, 524288, 256), (16, 4096, 256)]:
         with pytest.raises(ValueError):
             S.estimate_bytes(lib, L, B, 1, method=NU, nu_multiple=mult)
+
+
+def test_nonuniform_byte_model():
+    # frames cut the per-block bytes of long IRs; one frame of taps is uniform
+    u = Bt.alg_bytes_per_block(524288, 256)
+    n = Bt.alg_bytes_per_block(524288, 256, 1, 16)
+    assert u == 16 * 2048 * 257 + 8 * 257 + 16 * 256
+    k, N, pb = 257, 4096, 127
+    assert n == 16 * 16 * k + 8 * k + 16 * 256 + 16 * (pb - 2) * 256 + (8 * N + 16 * N + 16 * N + 4 * N) // 16
+    assert u / n > 12
+    assert Bt.alg_bytes_per_block(65536, 256, 3, 1) == Bt.alg_bytes_per_block(65536, 256, 3)
