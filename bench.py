@@ -359,7 +359,7 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
             submit(d)
         for s in srv.admit():
             admitted += 1
-            waits.append(t - s.arrived_tick)
+            waits.append(srv.ticks - s.arrived_tick)
         for sid, s in srv.active.items():
             d = meta[sid]
             if s.blocks and s.blocks % d["update_every"] == 0:
@@ -400,7 +400,7 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
             "realtime_factor": ticks * B / FS / wall, "outputs_finite": finite}
 
 
-def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1>"):
+def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1, 0>"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_k_tick.json")
@@ -448,6 +448,21 @@ def profile_run(T, torch, ctx):
     (HBM-bound), and a plain device copy of 512 MiB (3 launches) whose
     FETCH_SIZE/WRITE_SIZE calibrate the counters against known bytes."""
     bench_throughput(T, torch, ctx, 100, 10, timing=False)
+    # the headline shape at every tail chunk count the autotuner may pick
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, C) == T.AMF_OK
+    h = ir_bank_device(torch, C, L, 78)
+    torch.cuda.synchronize()
+    conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY)
+    x = noise_device(torch, C, 20 * B, 6)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for chunks in (1, 2, 4, 8, 16, 32, 64):
+        conv.SetTailChunks(chunks)
+        conv.Process(x, y, 20 * B)
+    conv.Synchronize()
+    conv.Terminate()
+    del h, x, y
     ch = 8192
     conv = T.TANConvolution(ctx)
     assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, ch) == T.AMF_OK

@@ -174,6 +174,14 @@ int32_t tanx_conv_process_device(tanx_conv *conv, const float *d_in, size_t in_s
 int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
                                   const size_t *in_stride, float *const *d_out, const size_t *out_stride,
                                   const uint32_t *const *flags);
+/* ProcessDirect (ConvolutionImpl.cpp:719-762; TrueAudioNext.h ProcessDirect):
+ * time-domain objects (TANX_METHOD_TIME_DOMAIN, IR length <= 4096) only,
+ * AMF_NOT_SUPPORTED otherwise.  The IR of channel c is ir[c], taps
+ * [nz_first_last[2c], nz_first_last[2c+1]) (all taps when NULL); min(n, IR
+ * length) samples of every channel are convolved against its history, which
+ * advances.  Host buffers; returns with the outputs written. */
+int32_t tanx_conv_process_direct(tanx_conv *conv, const float *const *ir, const float *const *in,
+                                 float *const *out, size_t n, size_t *processed, const int32_t *nz_first_last);
 /* ProcessFinalize (ConvolutionImpl.cpp:1046-1079): enqueues the tail
  * multiply-accumulate for the next block.  Implicit after each Process unless
  * the object was created with the 0x8000 bit. */

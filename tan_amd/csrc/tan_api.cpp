@@ -215,9 +215,16 @@ public:
         if (processed) *processed = done;
         return r;
     }
-    AMF_RESULT AMF_STD_CALL ProcessDirect(float *[], float *[], float *[], amf_size, amf_size *, int *) override
+    AMF_RESULT AMF_STD_CALL ProcessDirect(float *ir[], float *in[], float *out[], amf_size n, amf_size *processed,
+                                          int *nzFirstLast) override
     {
-        return AMF_NOT_SUPPORTED;  // the TIME_DOMAIN method is not part of this library
+        // TIME_DOMAIN objects only, as in the reference (ConvolutionImpl.cpp:719-762)
+        if (processed) *processed = 0;
+        if (!conv_) return AMF_NOT_INITIALIZED;
+        size_t done = 0;
+        const AMF_RESULT r = (AMF_RESULT)tanx_conv_process_direct(conv_, ir, in, out, n, &done, nzFirstLast);
+        if (processed) *processed = done;
+        return r;
     }
     AMF_RESULT AMF_STD_CALL ProcessFinalize(void) override
     {

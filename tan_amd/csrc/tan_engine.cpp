@@ -278,6 +278,13 @@ struct tanx_conv {
     int desc_ring = 0;
     uint64_t tail_alg_bytes = 0, head_alg_bytes = 0;
 
+    // time-domain objects: IR slots [3][C][len] and history [C][len] floats
+    bool td = false;
+    int tdpos = 0;  // history position of the next input sample
+    float *td_slot(int s) const { return reinterpret_cast<float *>(H) + (size_t)s * C * len; }
+    float *td_hist() const { return reinterpret_cast<float *>(F); }
+    int *d_nz = nullptr;  // [C][2] ProcessDirect non-zero ranges (device)
+
     // non-uniform objects: frames of M blocks (N = M * B samples); the
     // uniform fields above then describe the first frame of taps (P = M)
     bool nu = false;
@@ -329,6 +336,7 @@ static void conv_free(tanx_conv *cv)
         cv->t_tail.destroy();
         for (auto &g : cv->graphs) (void)hipGraphExecDestroy(g.exec);
         if (cv->d_desc) (void)hipFree(cv->d_desc);
+        if (cv->d_nz) (void)hipFree(cv->d_nz);
         if (cv->h_desc) (void)hipHostFree(cv->h_desc);
         for (auto &e : cv->desc_ev) if (e) (void)hipEventDestroy(e);
     }
@@ -361,6 +369,7 @@ static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(k
 struct ConvShape {
     int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1;
     int irtype = IR_F32;
+    bool td = false;  // time-domain object: IR slots and history in samples
 };
 
 static bool is_nu_method(uint32_t base)
@@ -426,6 +435,14 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
     sh.C = (int)channels;
     sh.irtype = prec == TANX_IR_FP16 ? IR_F16 : prec == TANX_IR_BF16 ? IR_BF16 : IR_F32;
     const uint32_t base = method & ~uint32_t(TANX_METHOD_USE_PROCESS_TAILTHREAD);
+    if (base == TANX_METHOD_TIME_DOMAIN) {
+        // direct convolution for short (HRTF-length) responses, FP32
+        if (sh.len > kMaxTdLen || sh.irtype != IR_F32) return TANX_INVALID_ARG;
+        sh.td = true;
+        sh.P = 1;
+        sh.maxchunk = 1;
+        return TANX_OK;
+    }
     int M = 1;
     if (is_nu_method(base)) {
         const uint32_t want = opt ? opt->nu_multiple : 0;
@@ -465,6 +482,15 @@ static ConvLayout conv_layout(const ConvShape &sh)
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t C = sh.C, B = sh.B, P = sh.P, N = sh.N, PB = sh.PB, elem = ir_elem_bytes(sh.irtype);
     ConvLayout l{};
+    if (sh.td) {
+        // [3][C][len] IR slots, [C][len] history (in F), I/O blocks
+        l.H = al(3 * C * (size_t)sh.len * sizeof(float));
+        l.F = al(C * (size_t)sh.len * sizeof(float));
+        l.X = al(C * B * sizeof(float));
+        l.Ch = al(C * sizeof(int));
+        l.Run = al(C);
+        return l;
+    }
     l.H = al(3 * C * P * B * elem);
     l.S = al(3 * C * P * sizeof(float));
     l.F = al(C * P * B * sizeof(float2));
@@ -515,6 +541,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     case TANX_METHOD_FFT_PARTITIONED_NONUNIFORM:
     case TANX_METHOD_FHT_NONUNIFORM_PARTITIONED:
     case TANX_METHOD_FFT_NONUNIFORM_PARTITIONED:
+    case TANX_METHOD_TIME_DOMAIN:
         break;
     default:
         return TANX_NOT_SUPPORTED;
@@ -536,6 +563,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->C = sh.C;
     cv->len = sh.len;
     cv->P = sh.P;
+    cv->td = sh.td;
     cv->nu = sh.M > 1;
     cv->M = sh.M;
     cv->N = sh.N;
@@ -625,6 +653,7 @@ static int32_t flush_channel(tanx_conv *cv, int ch)
 {
     hipStream_t s = cv->ctx->proc;
     const size_t B = cv->B;
+    if (cv->td) return map_hip(launch_zero(cv->td_hist() + (size_t)ch * cv->len, (size_t)cv->len * sizeof(float), s));
     hipError_t e = launch_zero(cv->F + (size_t)ch * cv->P * B, (size_t)cv->P * B * sizeof(float2), s);
     if (e == hipSuccess) e = launch_zero(cv->xprev + (size_t)ch * B, B * sizeof(float), s);
     // partial sums are laid out [C][tail_nchunk][B]: the next head reads them
@@ -828,6 +857,30 @@ static int32_t enqueue_tail(tanx_conv *cv)
 // bookkeeping, switch schedule, slot rotation, and the launch arguments of
 // the block's head (and, with_tail, the next block's tail).  Sets *tail_nf
 // and *needs_update when the launch reads a freshly transformed IR slot.
+// Switch schedule of block b (ConvolutionImpl.cpp:869-927): arms a handed-off
+// update, and at the armed block crossfades and rotates the slots (current ->
+// previous, update -> current).  Returns whether block b crossfades.
+static bool advance_switch(tanx_conv *cv, uint64_t b)
+{
+    bool xfade = false;
+    if (!cv->pending && cv->ready && !cv->delayed) {
+        cv->ready = false;
+        cv->delayed = true;
+        // uniform: the next block; non-uniform: a frame boundary late enough
+        // for the armed filter's frame partials (M + 1 blocks ahead)
+        const uint64_t M = (uint64_t)cv->M;
+        cv->switch_at = cv->nu ? ((b + M + 1 + M - 1) / M) * M : b + 1;
+    } else if (cv->delayed && b == cv->switch_at) {
+        xfade = true;
+    }
+    if (xfade) {
+        cv->prev = cv->cur;
+        cv->cur = (cv->cur + 1) % 3;
+        cv->upd = (cv->upd + 1) % 3;
+    }
+    return xfade;
+}
+
 // Frame roles of block b of a non-uniform object (after this block's slot
 // rotation).  The slice for frame gm = (b+1)/M feeds blocks (gm+1)M.. and
 // uses the filters in effect there: both while a switch at (gm+1)M is
@@ -921,22 +974,7 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     if (rc != TANX_OK) return rc;
 
     const uint64_t b = cv->ticks;
-    bool xfade = false;
-    if (!cv->pending && cv->ready && !cv->delayed) {
-        cv->ready = false;
-        cv->delayed = true;
-        // uniform: the next block; non-uniform: a frame boundary late enough
-        // for the armed filter's frame partials (M + 1 blocks ahead)
-        const uint64_t M = (uint64_t)cv->M;
-        cv->switch_at = cv->nu ? ((b + M + 1 + M - 1) / M) * M : b + 1;
-    } else if (cv->delayed && b == cv->switch_at) {
-        xfade = true;
-    }
-    if (xfade) {
-        cv->prev = cv->cur;
-        cv->cur = (cv->cur + 1) % 3;
-        cv->upd = (cv->upd + 1) % 3;
-    }
+    const bool xfade = advance_switch(cv, b);
     a = tick_common(cv);
     a.nhead = cv->C;
     a.xin = din;
@@ -978,11 +1016,70 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     return TANX_OK;
 }
 
+static void td_common(const tanx_conv *cv, TdArgs &t)
+{
+    t = TdArgs{};
+    t.B = cv->B;
+    t.C = cv->C;
+    t.len = cv->len;
+    t.run = cv->drun;
+    t.hist = cv->td_hist();
+    t.h_stride = cv->len;
+}
+
+// One block of a time-domain object: the same switch schedule, one launch.
+static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
+                                long long out_stride, size_t n, const uint32_t *flags)
+{
+    int nrun = 0;
+    int32_t rc = channel_flags(cv, flags, n, &nrun);
+    if (rc != TANX_OK) return rc;
+    if (nrun == 0) return TANX_WRONG_STATE;
+    rc = sync_run_mask(cv);
+    if (rc != TANX_OK) return rc;
+    const bool xfade = advance_switch(cv, cv->ticks);
+    TdArgs t;
+    td_common(cv, t);
+    t.n = cv->B;
+    t.xin = din;
+    t.in_stride = in_stride;
+    t.yout = dout;
+    t.out_stride = out_stride;
+    t.pos = cv->tdpos;
+    t.Ha = cv->td_slot(xfade ? cv->prev : cv->cur);
+    t.Hb = xfade ? cv->td_slot(cv->cur) : nullptr;
+    t.xfade = xfade ? 1 : 0;
+    if (xfade) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
+    hipStream_t s = cv->ctx->proc;
+    KernelTimer &kt = cv->t_head;
+    if (cv->timing) {
+        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
+        TANX_CHECK(launch_td(t, s, kt.start[kt.n], kt.stop[kt.n]));
+        kt.n++;
+        // history window, filters, input and output
+        cv->head_alg_bytes += (uint64_t)cv->C * (cv->len * sizeof(float) * (xfade ? 3 : 2) + 3 * cv->B * sizeof(float));
+    } else {
+        TANX_CHECK(launch_td(t, s));
+    }
+    cv->tdpos = (cv->tdpos + cv->B) & (cv->len - 1);
+    cv->crossfading = xfade;
+    cv->last_xfade = xfade;
+    if (xfade) {
+        cv->delayed = false;
+        cv->xfades++;
+    }
+    cv->ticks++;
+    cv->tail_due = true;
+    tail_done(cv);  // no tail: hand off a staged update now
+    return TANX_OK;
+}
+
 // One block launched directly; with_tail also runs the next block's tail in
 // the same launch.
 static int32_t process_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
                              long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
 {
+    if (cv->td) return process_td_block(cv, din, in_stride, dout, out_stride, n, flags);
     TickArgs a;
     FrameArgs fa;
     int tail_nf = 1;
@@ -1128,7 +1225,7 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (nb == 0) return TANX_OK;
     if (in_stride < nb * B || out_stride < nb * B) return TANX_INVALID_ARG;
     DeviceGuard g(cv->ctx->device);
-    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !cv->nu && !flags && nb >= 2)
+    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !cv->nu && !cv->td && !flags && nb >= 2)
         return process_device_graph(cv, d_in, in_stride, d_out, out_stride, nb, processed);
     for (size_t b = 0; b < nb; b++) {
         const bool fuse = cv->fused && !cv->fmode;
@@ -1141,6 +1238,89 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
         }
         if (processed) *processed = (b + 1) * B;
     }
+    return TANX_OK;
+}
+
+/* ProcessDirect (ConvolutionImpl.cpp:719-762): time-domain objects only.
+ * The IR comes with the call (channel c: ir[c], taps [first, last) of
+ * nz_first_last[2c], [2c+1], or [0, len) when it is NULL); min(n, len)
+ * samples are convolved against the channel's history, in chunks of at most
+ * B samples, and the history advances; every channel is processed. */
+extern "C" int32_t tanx_conv_process_direct(tanx_conv *cv, const float *const *ir, const float *const *in,
+                                            float *const *out, size_t n, size_t *processed,
+                                            const int32_t *nz_first_last)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (processed) *processed = 0;
+    if (!cv->td) return TANX_NOT_SUPPORTED;
+    if (!ir || !in || !out) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    const size_t C = cv->C, len = cv->len, B = cv->B;
+    n = std::min(n, len);
+    for (size_t c = 0; c < C; c++)
+        if (!ir[c] || !in[c] || !out[c]) return TANX_INVALID_ARG;
+    hipStream_t s = cv->ctx->proc;
+    // staging for the IRs (the host-update buffers), the ranges and an
+    // all-running mask
+    if (!cv->dir || !cv->hir) {
+        const size_t bytes = C * len * sizeof(float);
+        float *d = nullptr, *h = nullptr;
+        hipError_t e1 = hipMalloc(&d, bytes);
+        hipError_t e2 = e1 == hipSuccess ? hipHostMalloc(&h, bytes) : e1;
+        if (e2 != hipSuccess) {
+            if (d) (void)hipFree(d);
+            return map_hip(e2);
+        }
+        cv->dir = d;
+        cv->hir = h;
+        cv->dbytes += bytes;
+    }
+    if (!cv->d_nz) {
+        TANX_CHECK(hipMalloc(&cv->d_nz, C * 2 * sizeof(int) + C));
+        cv->dbytes += C * 2 * sizeof(int) + C;
+    }
+    // the previous direct call's staging may still be read
+    TANX_CHECK(hipStreamSynchronize(s));
+    std::vector<int> nz(2 * C);
+    for (size_t c = 0; c < C; c++) {
+        int f0 = 0, f1 = (int)len;
+        if (nz_first_last) {
+            f0 = std::max(0, std::min((int)len, (int)nz_first_last[2 * c]));
+            f1 = std::max(f0, std::min((int)len, (int)nz_first_last[2 * c + 1]));
+        }
+        nz[2 * c] = f0;
+        nz[2 * c + 1] = f1;
+        std::memcpy(cv->hir + c * len + f0, ir[c] + f0, (size_t)(f1 - f0) * sizeof(float));
+        TANX_CHECK(hipMemcpyAsync(cv->dir + c * len + f0, cv->hir + c * len + f0, (size_t)(f1 - f0) * sizeof(float),
+                                  hipMemcpyHostToDevice, s));
+    }
+    std::vector<uint8_t> ones(C, 1);
+    uint8_t *d_ones = reinterpret_cast<uint8_t *>(cv->d_nz + 2 * C);
+    TANX_CHECK(hipMemcpyAsync(cv->d_nz, nz.data(), 2 * C * sizeof(int), hipMemcpyHostToDevice, s));
+    TANX_CHECK(hipMemcpyAsync(d_ones, ones.data(), C, hipMemcpyHostToDevice, s));
+    for (size_t off = 0; off < n; off += B) {
+        const size_t m = std::min(B, n - off);
+        for (size_t c = 0; c < C; c++) std::memcpy(cv->hin + c * B, in[c] + off, m * sizeof(float));
+        TANX_CHECK(hipMemcpyAsync(cv->din, cv->hin, C * B * sizeof(float), hipMemcpyHostToDevice, s));
+        TdArgs t;
+        td_common(cv, t);
+        t.run = d_ones;
+        t.n = (int)m;
+        t.xin = cv->din;
+        t.in_stride = (long long)B;
+        t.yout = cv->dout;
+        t.out_stride = (long long)B;
+        t.pos = cv->tdpos;
+        t.Ha = cv->dir;
+        t.nz = cv->d_nz;
+        TANX_CHECK(launch_td(t, s));
+        TANX_CHECK(hipMemcpyAsync(cv->hout, cv->dout, C * B * sizeof(float), hipMemcpyDeviceToHost, s));
+        TANX_CHECK(hipStreamSynchronize(s));
+        for (size_t c = 0; c < C; c++) std::memcpy(out[c] + off, cv->hout + c * B, m * sizeof(float));
+        cv->tdpos = (int)((cv->tdpos + m) & (len - 1));
+    }
+    if (processed) *processed = n;
     return TANX_OK;
 }
 
@@ -1213,6 +1393,19 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
             stride = cv->len;
         }
         TANX_CHECK(hipMemcpyAsync(cv->dchan, cv->hchan, nupd * sizeof(int), hipMemcpyHostToDevice, ctx->upd));
+        if (cv->td) {
+            TANX_CHECK(launch_td_load(src, stride, (long long)n, cv->dchan, nupd, cv->td_slot(target), cv->len,
+                                      ctx->upd));
+            for (int k = 0; k < nupd; k++) cv->staged[cv->hchan[k]] = 1;
+        }
+    }
+    if (nupd > 0 && !cv->td) {
+        const float *src = dir;
+        long long stride = (long long)dstride;
+        if (hir) {
+            src = cv->dir;
+            stride = cv->len;
+        }
         IrArgs a{};
         a.ir = src;
         a.stride = stride;
@@ -1244,6 +1437,11 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
     // channels not updated keep the IR that is current when the update is
     // handed off (ConvolutionImpl.cpp:2149-2158)
     for (int ch : keep_list) {
+        if (cv->td) {
+            TANX_CHECK(hipMemcpyAsync(cv->td_slot(target) + (size_t)ch * cv->len, cv->td_slot(keep) + (size_t)ch * cv->len,
+                                      (size_t)cv->len * sizeof(float), hipMemcpyDeviceToDevice, ctx->upd));
+            continue;
+        }
         TANX_CHECK(hipMemcpyAsync(static_cast<char *>(cv->slot_ptr(target)) + (size_t)ch * chanBytes,
                                   static_cast<char *>(cv->slot_ptr(keep)) + (size_t)ch * chanBytes, chanBytes,
                                   hipMemcpyDeviceToDevice, ctx->upd));
@@ -1309,7 +1507,8 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     for (uint32_t i = 0; i < n; i++) {
         tanx_conv *cv = convs[i];
         if (!cv) return TANX_NOT_INITIALIZED;
-        if (cv->ctx != ctx || cv->irtype != irtype || cv->B > kGroupMaxB || cv->fmode) return TANX_NOT_SUPPORTED;
+        if (cv->ctx != ctx || cv->irtype != irtype || cv->B > kGroupMaxB || cv->fmode || cv->td)
+            return TANX_NOT_SUPPORTED;
         if (!d_in[i] || !d_out[i] || in_stride[i] < (size_t)cv->B || out_stride[i] < (size_t)cv->B)
             return TANX_INVALID_ARG;
     }
@@ -1499,6 +1698,7 @@ extern "C" int32_t tanx_conv_autotune(tanx_conv *cv, uint32_t *chosen, double *b
 {
     if (!cv) return TANX_NOT_INITIALIZED;
     std::lock_guard<std::mutex> lk(cv->mtx);
+    if (cv->td) return TANX_NOT_SUPPORTED;  // no partitions to chunk
     if (cv->ticks != 0) return TANX_WRONG_STATE;
     DeviceGuard g(cv->ctx->device);
     hipStream_t s = cv->ctx->proc;

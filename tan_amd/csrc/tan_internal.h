@@ -102,6 +102,29 @@ struct TickArgs {
     float2 *bTb;
 };
 
+// Time-domain convolution (TAN_CONVOLUTION_METHOD_TIME_DOMAIN): one
+// workgroup per channel computes n <= B outputs
+// y[j] = sum_{k=first}^{last-1} hist[(pos + j - k) mod len] h[k] over the
+// channel's circular history (the new samples written at pos..pos+n-1), with
+// a one-block crossfade to the second filter when xfade is set, and then
+// writes the new samples into the history.
+constexpr int kMaxTdLen = 4096;  // window and two filters within 64 KB of LDS at B = 1024
+struct TdArgs {
+    int B, C, len, n;
+    const uint8_t *run;
+    const float *xin;
+    long long in_stride;
+    float *yout;
+    long long out_stride;
+    float *hist;           // [C][len]
+    int pos;               // history position of the first new sample
+    const float *Ha;       // filters: channel c at Ha + c * h_stride
+    const float *Hb;       // crossfade: new filter, else nullptr
+    long long h_stride;
+    const int *nz;         // [C][2] first / last+1 non-zero tap, or nullptr (0, len)
+    int xfade;
+};
+
 // Frame kernel of a non-uniform object at the last block of frame g: one
 // workgroup per channel.  X_g = R2C_2N([x_{g-1}, x_g]) from the input ring
 // (the head of the same block has written x_t into it), into the frame ring;
@@ -166,6 +189,10 @@ hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
 // frame kernels: N in {128, ..., 4096}
 constexpr int kMaxFrame = 4096;
 hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s, hipEvent_t ev1 = nullptr);
+hipError_t launch_td(const TdArgs &a, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// strided copy of IR rows into a [C][len] slot, zero past n (list of channels)
+hipError_t launch_td_load(const float *ir, long long stride, long long n, const int *chan, int nchan, float *H,
+                          int len, hipStream_t s);
 hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
                               hipStream_t s);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);

@@ -883,6 +883,64 @@ __global__ __launch_bounds__(M / 4 > 64 ? (M / 4 < 1024 ? M / 4 : 1024) : 64) vo
 }
 
 // ---------------------------------------------------------------------------
+// Time-domain convolution (see TdArgs), 256 threads, window and filters in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_td(TdArgs a)
+{
+    extern __shared__ float td_smem[];
+    const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (!a.run[c]) return;
+    const int len = a.len, n = a.n, mask = len - 1;
+    int f0 = 0, f1 = len;
+    if (a.nz) {
+        f0 = max(0, min(len, a.nz[2 * c]));
+        f1 = max(f0, min(len, a.nz[2 * c + 1]));
+    }
+    // window w[t] = sample (pos - (len - 1) + t) for t < len - 1 + n: the old
+    // history, then the n new samples
+    float *w = td_smem;                     // [len - 1 + n]
+    float *ha = td_smem + (len - 1 + a.B);  // [len]
+    float *hb = ha + len;                   // [len] (crossfade)
+    const float *hist = a.hist + (size_t)c * len;
+    const float *xi = a.xin + (size_t)c * a.in_stride;
+    for (int t = tid; t < len - 1; t += nt) w[t] = hist[(a.pos - (len - 1) + t) & mask];
+    for (int j = tid; j < n; j += nt) w[len - 1 + j] = xi[j];
+    const float *Ha = a.Ha + (size_t)c * a.h_stride;
+    for (int k = tid; k < len; k += nt) ha[k] = Ha[k];
+    if (a.Hb) {
+        const float *Hb = a.Hb + (size_t)c * a.h_stride;
+        for (int k = tid; k < len; k += nt) hb[k] = Hb[k];
+    }
+    __syncthreads();
+    float *yo = a.yout + (size_t)c * a.out_stride;
+    const float step = (float)(1.0 / (double)a.B);
+    for (int j = tid; j < n; j += nt) {
+        // y[j] = sum_k w[len - 1 + j - k] h[k], taps in ascending order
+        float ya = 0.f, yb = 0.f;
+        const float *wj = w + (len - 1 + j);
+        for (int k = f0; k < f1; k++) ya += wj[-k] * ha[k];
+        if (a.Hb) {
+            for (int k = f0; k < f1; k++) yb += wj[-k] * hb[k];
+            ya = yb * (step * (float)j) + ya * (step * (float)(a.B - j));
+        }
+        yo[j] = ya;
+    }
+    // the new samples enter the history (this workgroup alone owns channel c)
+    float *hw = a.hist + (size_t)c * len;
+    for (int j = tid; j < n; j += nt) hw[(a.pos + j) & mask] = xi[j];
+}
+
+__global__ void k_td_load(const float *__restrict__ ir, long long stride, long long n, const int *__restrict__ chan,
+                          float *__restrict__ H, int len)
+{
+    const int ch = chan[blockIdx.y];
+    const float *src = ir + (size_t)ch * stride;
+    float *dst = H + (size_t)ch * len;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < len; k += gridDim.x * blockDim.x)
+        dst[k] = k < n ? src[k] : 0.f;
+}
+
+// ---------------------------------------------------------------------------
 // Output chain: mixer and sample converter (grid-stride, one sample per lane).
 // ---------------------------------------------------------------------------
 // Sum of the channels in channel order (the reference's order), channel c at
@@ -1090,6 +1148,25 @@ hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s)
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_td(const TdArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (a.C == 0 || a.n == 0) return hipSuccess;
+    if (a.len > kMaxTdLen || a.n > a.B) return hipErrorInvalidValue;
+    const size_t smem = sizeof(float) * ((size_t)a.len - 1 + a.B + 2 * (size_t)a.len);
+    hipExtLaunchKernelGGL(k_td, dim3(a.C), dim3(256), smem, s, ev0, ev1, 0, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_td_load(const float *ir, long long stride, long long n, const int *chan, int nchan, float *H,
+                          int len, hipStream_t s)
+{
+    if (nchan == 0) return hipSuccess;
+    const int g = (len + 255) / 256;
+    const unsigned gx = (unsigned)(g > 64 ? 64 : g);
+    hipLaunchKernelGGL(k_td_load, dim3(gx, nchan), dim3(256), 0, s, ir, stride, n, chan, H, len);
     return hipGetLastError();
 }
 

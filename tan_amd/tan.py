@@ -236,6 +236,25 @@ class TANConvolution:
         rc = self._lib.tanx_conv_process(self.handle, pin, pout, num_samples, fp, ctypes.byref(done))
         return rc, done.value
 
+    def ProcessDirect(self, impulse_responses, inputs, outputs, num_samples=None, nz_first_last=None):
+        """ProcessDirect (ConvolutionImpl.cpp:719-762): TIME_DOMAIN objects
+        only.  Host arrays (channels, >= n); nz_first_last: None or (channels,
+        2) first / last+1 non-zero taps.  Returns (AMF_RESULT, processed)."""
+        if not self.handle:
+            return AMF_NOT_INITIALIZED, 0
+        if num_samples is None:
+            num_samples = inputs.shape[1] if isinstance(inputs, np.ndarray) else min(len(b) for b in inputs)
+        pir, _k1 = _host_rows(impulse_responses, self.channels)
+        pin, _k2 = _host_rows(inputs, self.channels, num_samples)
+        pout, _k3 = _host_rows(outputs, self.channels, num_samples, writable=True)
+        nz = None
+        if nz_first_last is not None:
+            nz = np.ascontiguousarray(nz_first_last, dtype=np.int32).reshape(-1)
+        done = ctypes.c_size_t(0)
+        rc = self._lib.tanx_conv_process_direct(self.handle, pir, pin, pout, num_samples, ctypes.byref(done),
+                                                nz.ctypes.data if nz is not None else None)
+        return rc, done.value
+
     def ProcessFinalize(self):
         if not self.handle:
             return AMF_NOT_INITIALIZED
