@@ -348,6 +348,10 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
     lat, enq, ctl, chans, sess, byts, waits = [], [], [], [], [], [], []
     admitted = departed = updates = 0
     pend = None
+    # a real-time control loop: no garbage-collector pauses inside ticks
+    import gc
+    gc.collect()
+    gc.disable()
     wall0 = time.perf_counter()
     for t in range(ticks):
         c0 = time.perf_counter()
@@ -376,6 +380,7 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
         sess.append(rec.sessions)
         byts.append(rec.alg_bytes)
     wall = time.perf_counter() - wall0
+    gc.enable()
     finite = bool(torch.isfinite(srv.out_arena).all().item())
     queued = srv.admission.waiting()
     srv.shutdown()

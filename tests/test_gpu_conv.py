@@ -156,7 +156,7 @@ def test_next_free_channel_and_errors(gpu_ctx):
     L, B, C = 1024, 64, 3
     conv = T.TANConvolution(gpu_ctx)
     assert conv.InitGpu(UNIFORM, L, 96, C) == T.AMF_INVALID_ARG  # not a power of two
-    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_TIME_DOMAIN, L, B, C) == T.AMF_NOT_SUPPORTED
+    assert conv.InitGpu(42, L, B, C) == T.AMF_NOT_SUPPORTED  # no such method
     assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
     assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_ALREADY_INITIALIZED
     assert conv.UpdateResponseTD(np.zeros((C, L + 1), np.float32)) == T.AMF_INVALID_ARG
