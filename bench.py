@@ -405,6 +405,59 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
             "realtime_factor": ticks * B / FS / wall, "outputs_finite": finite}
 
 
+def bench_dft_mfma(T, torch, ctx, channels=1024, log2n=9, reps=50):
+    """The block DFT on the matrix cores (precision track): 1,024 channels x
+    512-sample frames (the 2B-point transform at B = 256), plain bf16 and
+    split-bf16, against the FP32 FFT (TANFFT R2C) on the same frames."""
+    n = 1 << log2n
+    fft = T.TANFFT(ctx)
+    fft.Init()
+    s = torch.cuda.ExternalStream(T._lib.load().tanx_ctx_process_stream(ctx.handle))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    x = torch.empty(channels, 2 * n, device="cuda").uniform_(-1.0, 1.0, generator=g)
+    x[:, n:] = 0.0
+    torch.cuda.synchronize()
+    ref = np.fft.rfft(x[:64, :n].double().cpu().numpy(), axis=1)
+    refri = np.stack([ref.real, ref.imag], axis=2).reshape(64, n + 2)
+
+    def snr(a, b):
+        e = np.sum((a - b) ** 2)
+        return float(10 * np.log10(np.sum(a * a) / e)) if e > 0 else float("inf")
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    yf = torch.zeros(channels, 2 * n, device="cuda")
+    us_fft = timed(lambda: fft.Transform(T.TAN_FFT_R2C_TRANSFORM_DIRECTION_FORWARD, log2n, channels, x, yf))
+    torch.cuda.synchronize()
+    fp32 = yf[:64, :n + 2].double().cpu().numpy()
+    rows = []
+    for split in (1, 3):
+        y = torch.zeros(channels, 2 * n, device="cuda")
+        us = timed(lambda: fft.TransformR2CMatrixCores(log2n, channels, x, y, split))
+        torch.cuda.synchronize()
+        got = y[:64, :n + 2].double().cpu().numpy()
+        flops = 2.0 * channels * n * (n + 2) * split
+        tf = flops / (us * 1e-6) / 1e12
+        rows.append({"split": split, "channels": channels, "n": n, "us": us, "tflops": tf,
+                     "peak_tflops": 2500.0, "frac": tf / 2500.0, "fft_fp32_us": us_fft,
+                     "snr_vs_fp64_db": snr(refri, got), "snr_vs_fp32_fft_db": snr(fp32, got),
+                     "fft_fp32_snr_vs_fp64_db": snr(refri, fp32)})
+        log(f"  mfma dft split {split}: {us:.1f} us ({tf:.1f} TFLOP/s), FFT {us_fft:.1f} us, "
+            f"SNR {rows[-1]['snr_vs_fp64_db']:.1f} dB vs FP64")
+    del x, yf
+    return rows
+
+
 def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1, 0>"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
@@ -557,10 +610,11 @@ def main():
         log("deadline sweep")
         best, rows = bench_sweep(T, torch, ctx, [int(v) for v in args.sweep.split(",") if v])
 
-    prec_rows = []
+    prec_rows, dft_rows = [], []
     if not args.no_precision:
         log("precision pass")
         prec_rows = bench_precision(T, torch, ctx)
+        dft_rows = bench_dft_mfma(T, torch, ctx)
 
     nonuniform = None
     if not args.no_nonuniform:
@@ -617,6 +671,7 @@ def main():
         "roofline_hbm": ({"channels": big["channels"], "achieved": big["tick_GBps"],
                           "frac": big["tick_GBps"] / HBM_PEAK_GBPS} if big else None),
         "precision": prec_rows,
+        "mfma_dft": dft_rows,
         "nonuniform": nonuniform,
         "batching": batching,
         "serving": serving,

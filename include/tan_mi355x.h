@@ -307,6 +307,20 @@ int32_t tanx_convert_device(tanx_ctx *ctx, const void *d_in, size_t in_step, siz
 int32_t tanx_convert(tanx_ctx *ctx, const void *in, size_t in_step, int32_t in_type, void *out, size_t out_step,
                      int32_t out_type, size_t n, float gain, int32_t *clipped);
 
+/* ---- block DFT on the matrix cores (precision track) ------------------------- */
+
+/* Real-to-complex DFT of n = 2^log2len samples (5 <= log2len <= 11) per
+ * channel as a bf16 GEMM on the matrix cores with FP32 accumulation: split =
+ * 1 uses one product, split = 3 splits both operands into bf16 high and low
+ * parts and accumulates three products (hi hi, hi lo, lo hi).  The DFT
+ * tables are computed in FP64.  Channel c reads d_in + c * in_stride and
+ * writes bins 0..n/2 as (re, im) pairs at d_out + c * out_stride
+ * (out_stride >= n + 2), as TANFFT's R2C direction.  No counterpart in the
+ * reference (its transforms are FFTs, FFTImpl.cpp:996-1122).  Asynchronous
+ * on the context's process stream. */
+int32_t tanx_dft_r2c_mfma_device(tanx_ctx *ctx, uint32_t log2len, uint32_t channels, const float *d_in,
+                                 size_t in_stride, float *d_out, size_t out_stride, uint32_t split);
+
 #ifdef __cplusplus
 }
 #endif

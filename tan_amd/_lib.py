@@ -61,6 +61,7 @@ _SIGS = {
                                  ctypes.POINTER(_sz)]),
     "tanx_conv_process_device": (_i32, [_vp, _vp, _sz, _vp, _sz, _sz, _u32p, ctypes.POINTER(_sz)]),
     "tanx_conv_finalize": (_i32, [_vp]),
+    "tanx_dft_r2c_mfma_device": (_i32, [_vp, _u32, _u32, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _u32]),
     "tanx_conv_process_direct": (_i32, [_vp, _vp, _vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                         ctypes.c_void_p]),
     "tanx_conv_flush": (_i32, [_vp, _u32]),

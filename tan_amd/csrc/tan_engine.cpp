@@ -75,6 +75,10 @@ struct tanx_ctx {
     size_t fft_dev_floats = 0;
     float *fft_host = nullptr;
     size_t fft_host_floats = 0;
+    // matrix-core DFT tables per size: [n+2][n] bf16 high and low parts
+    std::mutex dft_mtx;
+    int dft_n = 0;
+    uint16_t *dft_hi = nullptr, *dft_lo = nullptr;
     // output chain: clip flag and host staging (converter, host mixer)
     std::mutex io_mtx;
     int *io_flag = nullptr;       // device
@@ -142,6 +146,8 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
         if (ctx->g_host) (void)hipHostFree(ctx->g_host);
         if (ctx->g_dev) (void)hipFree(ctx->g_dev);
         for (auto &e : ctx->g_ev) if (e) (void)hipEventDestroy(e);
+        if (ctx->dft_hi) (void)hipFree(ctx->dft_hi);
+        if (ctx->dft_lo) (void)hipFree(ctx->dft_lo);
         if (ctx->io_flag) (void)hipFree(ctx->io_flag);
         if (ctx->io_flag_host) (void)hipHostFree(ctx->io_flag_host);
         if (ctx->io_dev) (void)hipFree(ctx->io_dev);
@@ -2112,5 +2118,73 @@ extern "C" int32_t tanx_convert(tanx_ctx *ctx, const void *in, size_t in_step, i
     TANX_CHECK(hipStreamSynchronize(s));
     std::memcpy(out, h + out_off, out_span);
     if (clipped) *clipped = flag && *ctx->io_flag_host ? 1 : 0;
+    return TANX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block DFT on the matrix cores
+// ---------------------------------------------------------------------------
+static uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+static float bf16_to_f(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// tables of the n-point real DFT: column 2k = cos(2 pi k t / n), column 2k+1
+// = -sin(2 pi k t / n), values in FP64 split into bf16 high + low parts
+static int32_t dft_tables(tanx_ctx *ctx, int n)
+{
+    if (ctx->dft_n == n) return TANX_OK;
+    TANX_CHECK(hipStreamSynchronize(ctx->proc));
+    if (ctx->dft_hi) (void)hipFree(ctx->dft_hi);
+    if (ctx->dft_lo) (void)hipFree(ctx->dft_lo);
+    ctx->dft_hi = ctx->dft_lo = nullptr;
+    ctx->dft_n = 0;
+    const size_t cols = (size_t)n + 2, count = cols * n;
+    std::vector<uint16_t> hi(count), lo(count);
+    for (size_t col = 0; col < cols; col++) {
+        const size_t k = col / 2;
+        for (int t = 0; t < n; t++) {
+            const double ang = 2.0 * M_PI * (double)((k * (size_t)t) % (size_t)n) / (double)n;
+            const double v = (col & 1) ? -std::sin(ang) : std::cos(ang);
+            const uint16_t h = bf16_rne((float)v);
+            hi[col * n + t] = h;
+            lo[col * n + t] = bf16_rne((float)(v - (double)bf16_to_f(h)));
+        }
+    }
+    TANX_CHECK(hipMalloc(&ctx->dft_hi, count * 2));
+    TANX_CHECK(hipMalloc(&ctx->dft_lo, count * 2));
+    TANX_CHECK(hipMemcpy(ctx->dft_hi, hi.data(), count * 2, hipMemcpyHostToDevice));
+    TANX_CHECK(hipMemcpy(ctx->dft_lo, lo.data(), count * 2, hipMemcpyHostToDevice));
+    ctx->dft_n = n;
+    return TANX_OK;
+}
+
+/* Real-to-complex DFT of 2^log2len samples per channel on the matrix cores
+ * (the block DFT of the precision track; no counterpart in the reference,
+ * whose R2C transforms are FFTs: FFTImpl.cpp:996-1122).  Output as
+ * TANFFT's R2C direction: bins 0..n/2 interleaved (re, im). */
+extern "C" int32_t tanx_dft_r2c_mfma_device(tanx_ctx *ctx, uint32_t log2len, uint32_t channels, const float *d_in,
+                                            size_t in_stride, float *d_out, size_t out_stride, uint32_t split)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    if (!d_in || !d_out || channels == 0 || log2len < 5 || log2len > 11 || (split != 1 && split != 3))
+        return TANX_INVALID_ARG;
+    const int n = 1 << log2len;
+    if (in_stride < (size_t)n || out_stride < (size_t)n + 2) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->dft_mtx);
+    DeviceGuard g(ctx->device);
+    int32_t rc = dft_tables(ctx, n);
+    if (rc != TANX_OK) return rc;
+    TANX_CHECK(launch_dft_mfma(d_in, (long long)in_stride, d_out, (long long)out_stride, (int)channels, n,
+                               ctx->dft_hi, ctx->dft_lo, (int)split, ctx->proc));
     return TANX_OK;
 }

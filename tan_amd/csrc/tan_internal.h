@@ -196,6 +196,11 @@ hipError_t launch_td_load(const float *ir, long long stride, long long n, const 
 hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
                               hipStream_t s);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
+// batched real DFT of n-sample frames on the matrix cores (n a multiple of
+// 32): y[c][2k], y[c][2k+1] = Re, Im of bin k <= n/2; fh/fl: [n+2][n] bf16
+// tables (column-major in t); split 1 or 3 products
+hipError_t launch_dft_mfma(const float *x, long long in_stride, float *y, long long out_stride, int C, int n,
+                           const void *fh, const void *fl, int split, hipStream_t s);
 // output chain: channel sum, and sample conversion (types 0 = float, 1 = short)
 hipError_t launch_mix(const float *const *d_ins, const float *d_in, long long stride, int channels, long long n,
                       float *d_out, hipStream_t s);

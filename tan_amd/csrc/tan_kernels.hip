@@ -952,6 +952,70 @@ __global__ void k_td_load(const float *__restrict__ ir, long long stride, long l
 }
 
 // ---------------------------------------------------------------------------
+// Block DFT on the matrix cores: a batched real-to-complex DFT of frames of
+// n samples as the GEMM Y[c][col] = sum_t x[c][t] F[t][col], F[t][2k] =
+// cos(2 pi k t / n), F[t][2k+1] = -sin(2 pi k t / n), k = 0..n/2.  bf16
+// operands with FP32 accumulation; with split = 3 each operand is split into
+// bf16 high and low parts and three products (hi hi, hi lo, lo hi) are
+// accumulated, which recovers about 16 more bits of the products.
+// v_mfma_f32_16x16x32_bf16 lane layout: A (16 x 32): lane l holds row l % 16,
+// k = 8 (l / 16) + e; B (32 x 16): column l % 16, same k; D (16 x 16): column
+// l % 16, rows 4 (l / 16) + r.  One wave computes 16 channels x 64 columns.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_dft_mfma(const float *__restrict__ x, long long in_stride,
+                                                  float *__restrict__ y, long long out_stride, int C, int n,
+                                                  int cols, const bf16x8_t *__restrict__ fh,
+                                                  const bf16x8_t *__restrict__ fl, int split)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int strips = (cols + 63) / 64;
+    const int c0 = (wave / strips) * 16, col0 = (wave % strips) * 64;
+    if (c0 >= C) return;  // whole wave exits together
+    const int i = lane & 15, kq = lane >> 4;
+    const int c = min(c0 + i, C - 1);  // rows beyond C compute a copy and are not stored
+    const float *xr = x + (size_t)c * in_stride;
+    f32x4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    const int kv = n / 8;  // bf16x8 vectors per table row
+    for (int k0 = 0; k0 < n; k0 += 32) {
+        const int kb = k0 + 8 * kq;
+        bf16x8_t ah, al;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float v = xr[kb + e];
+            const __bf16 h = (__bf16)v;
+            ah[e] = h;
+            al[e] = (__bf16)(v - (float)h);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int col = min(col0 + 16 * q + i, cols - 1);
+            const bf16x8_t bh = fh[(size_t)col * kv + kb / 8];
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[q], 0, 0, 0);
+            if (split == 3) {
+                const bf16x8_t bl = fl[(size_t)col * kv + kb / 8];
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[q], 0, 0, 0);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[q], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int col = col0 + 16 * q + i;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = c0 + 4 * kq + r;
+            if (row < C && col < cols) y[(size_t)row * out_stride + col] = acc[q][r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Output chain: mixer and sample converter (grid-stride, one sample per lane).
 // ---------------------------------------------------------------------------
 // Sum of the channels in channel order (the reference's order), channel c at
@@ -1178,6 +1242,19 @@ hipError_t launch_td_load(const float *ir, long long stride, long long n, const 
     const int g = (len + 255) / 256;
     const unsigned gx = (unsigned)(g > 64 ? 64 : g);
     hipLaunchKernelGGL(k_td_load, dim3(gx, nchan), dim3(256), 0, s, ir, stride, n, chan, H, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_mfma(const float *x, long long in_stride, float *y, long long out_stride, int C, int n,
+                           const void *fh, const void *fl, int split, hipStream_t s)
+{
+    if (C <= 0) return hipSuccess;
+    if (n < 32 || (n & 31) || (split != 1 && split != 3)) return hipErrorInvalidValue;
+    const int cols = n + 2;
+    const long long waves = (long long)((C + 15) / 16) * ((cols + 63) / 64);
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    hipLaunchKernelGGL(k_dft_mfma, dim3(blocks), dim3(256), 0, s, x, in_stride, y, out_stride, C, n, cols,
+                       static_cast<const bf16x8_t *>(fh), static_cast<const bf16x8_t *>(fl), split);
     return hipGetLastError();
 }
 

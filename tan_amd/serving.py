@@ -305,6 +305,10 @@ class SessionServer:
         the tick, on an idle process stream) to the end of its kernel, so it
         includes the host enqueue time."""
         import torch
+        # membership changes rebuild the launch arguments before the tick
+        # starts: control work, as a server thread prepares it off the audio path
+        if self._call is None:
+            self._rebuild()
         t0 = time.perf_counter()
         s = self.stream()
         if timed:
@@ -316,8 +320,6 @@ class SessionServer:
             ev = torch.cuda.Event()
             ev.record()
             s.wait_event(ev)
-        if self._call is None:
-            self._rebuild()
         rc = self._call.launch()
         if rc != T.AMF_OK:
             raise RuntimeError(f"group launch failed: AMF_RESULT {rc}")

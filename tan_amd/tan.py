@@ -345,6 +345,14 @@ class TANFFT:
         pout, _ko = _host_rows(outputs, channels, n, writable=True)
         return self._lib.tanx_fft_transform(self._ctx.handle, direction, log2len, channels, pin, pout)
 
+    def TransformR2CMatrixCores(self, log2len, channels, inputs, outputs, split=3):
+        """Real-to-complex DFT of 2^log2len samples per channel as a bf16 GEMM on
+        the matrix cores (split = 3: compensated high/low products).  torch
+        CUDA tensors (channels, >= n) in and (channels, >= n + 2) out, bins as
+        (re, im) pairs; asynchronous on the context's process stream."""
+        return self._lib.tanx_dft_r2c_mfma_device(self._ctx.handle, log2len, channels, inputs.data_ptr(),
+                                                  inputs.stride(0), outputs.data_ptr(), outputs.stride(0), split)
+
 
 def TANCreateContext(version=TAN_FULL_VERSION, device=0):
     """Returns (AMF_RESULT, TANContext or None), as TANCreateContext does."""
