@@ -3,7 +3,8 @@
 // this library's C++ interface (include/TrueAudioNext.h), on the driver's own
 // input files, with the check the reference leaves out: every output block
 // is compared with the CPU oracle (oracle/tan_oracle.h) driven through the
-// same calls.  Also exercises TANFFT through the C++ interface.
+// same calls.  Also exercises TANFFT, TANMixer and TANConverter through the
+// C++ interface.
 //
 // usage: talib_harness <dir with pulsetrainS_0p5s.wav, testresponse.wav, testresponse2.wav>
 #include "TrueAudioNext.h"
@@ -166,9 +167,34 @@ int main(int argc, char **argv)
     for (int i = 0; i < n; i++) re = std::fmax(re, std::fabs((double)fback[i] - fin[i]));
     std::printf("fft: max|gpu-oracle| spectrum = %.3e, round trip = %.3e\n", fe, re);
 
+    // output chain through the C++ interface (the host overloads of
+    // TANMixer::Mix and TANConverter::Convert) on the convolution output:
+    // mono mix of all channels, then 16-bit PCM; against the oracle's
+    // restatement of Mixer.cl / Converter.cl
+    TANMixerPtr mixer;
+    TANConverterPtr converter;
+    if ((r = TANCreateMixer(ctx, &mixer)) != AMF_OK) return fail("TANCreateMixer", r);
+    if ((r = TANCreateConverter(ctx, &converter)) != AMF_OK) return fail("TANCreateConverter", r);
+    if ((r = mixer->Init(NSamples, nch)) != AMF_OK) return fail("TANMixer::Init", r);
+    if ((r = converter->Init()) != AMF_OK) return fail("TANConverter::Init", r);
+    std::vector<float *> outs(nch);
+    for (int c = 0; c < nch; c++) outs[c] = out[c].data();
+    std::vector<float> mono(NSamples), mono_ref(NSamples);
+    if ((r = mixer->Mix(outs.data(), mono.data())) != AMF_OK) return fail("TANMixer::Mix", r);
+    orc_mix(outs.data(), nch, NSamples, mono_ref.data());
+    std::vector<short> pcm(NSamples), pcm_ref(NSamples);
+    bool clipped = false;
+    int clipped_ref = 0;
+    if ((r = converter->Convert(mono.data(), 1, NSamples, pcm.data(), 1, 0.5f, &clipped)) != AMF_OK)
+        return fail("TANConverter::Convert", r);
+    orc_convert(mono_ref.data(), 1, 0, 0, pcm_ref.data(), 1, 0, 1, NSamples, 0.5f, &clipped_ref);
+    const bool chain_ok = mono == mono_ref && pcm == pcm_ref && clipped == (clipped_ref != 0);
+    std::printf("output chain: mix %s, convert %s, clipped %d\n", mono == mono_ref ? "bit-exact" : "DIFFERS",
+                pcm == pcm_ref ? "bit-exact" : "DIFFERS", (int)clipped);
+
     orc_conv_destroy(orc);
     ctx->Terminate();  // while conv/fft are still alive, as the driver does
-    const bool ok = maxerr <= 1e-5 && snr >= 120.0 && xfades == expected_xfades && fe <= 1e-4 && re <= 1e-5;
+    const bool ok = maxerr <= 1e-5 && snr >= 120.0 && xfades == expected_xfades && fe <= 1e-4 && re <= 1e-5 && chain_ok;
     std::printf("%s\n", ok ? "PASS" : "FAIL");
     return ok ? 0 : 1;
 }
