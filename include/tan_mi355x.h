@@ -315,7 +315,8 @@ int32_t tanx_convert(tanx_ctx *ctx, const void *in, size_t in_step, int32_t in_t
  * parts and accumulates three products (hi hi, hi lo, lo hi).  The DFT
  * tables are computed in FP64.  Channel c reads d_in + c * in_stride and
  * writes bins 0..n/2 as (re, im) pairs at d_out + c * out_stride
- * (out_stride >= n + 2), as TANFFT's R2C direction.  No counterpart in the
+ * (out_stride >= n + 2), as TANFFT's R2C direction; d_in must be 16-byte
+ * aligned and in_stride a multiple of 4.  No counterpart in the
  * reference (its transforms are FFTs, FFTImpl.cpp:996-1122).  Asynchronous
  * on the context's process stream. */
 int32_t tanx_dft_r2c_mfma_device(tanx_ctx *ctx, uint32_t log2len, uint32_t channels, const float *d_in,

@@ -2180,6 +2180,8 @@ extern "C" int32_t tanx_dft_r2c_mfma_device(tanx_ctx *ctx, uint32_t log2len, uin
         return TANX_INVALID_ARG;
     const int n = 1 << log2len;
     if (in_stride < (size_t)n || out_stride < (size_t)n + 2) return TANX_INVALID_ARG;
+    // the kernel reads 16-byte vectors of samples
+    if ((in_stride & 3) || (reinterpret_cast<uintptr_t>(d_in) & 15)) return TANX_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->dft_mtx);
     DeviceGuard g(ctx->device);
     int32_t rc = dft_tables(ctx, n);

@@ -960,59 +960,78 @@ __global__ void k_td_load(const float *__restrict__ ir, long long stride, long l
 // accumulated, which recovers about 16 more bits of the products.
 // v_mfma_f32_16x16x32_bf16 lane layout: A (16 x 32): lane l holds row l % 16,
 // k = 8 (l / 16) + e; B (32 x 16): column l % 16, same k; D (16 x 16): column
-// l % 16, rows 4 (l / 16) + r.  One wave computes 16 channels x 64 columns.
+// l % 16, rows 4 (l / 16) + r.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_dft_mfma(const float *__restrict__ x, long long in_stride,
-                                                  float *__restrict__ y, long long out_stride, int C, int n,
-                                                  int cols, const bf16x8_t *__restrict__ fh,
-                                                  const bf16x8_t *__restrict__ fl, int split)
+__global__ __launch_bounds__(64) void k_dft_mfma(const float *__restrict__ x, long long in_stride,
+                                                 float *__restrict__ y, long long out_stride, int C, int n, int cols,
+                                                 const bf16x8_t *__restrict__ fh, const bf16x8_t *__restrict__ fl,
+                                                 int split)
 {
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int strips = (cols + 63) / 64;
-    const int c0 = (wave / strips) * 16, col0 = (wave % strips) * 64;
-    if (c0 >= C) return;  // whole wave exits together
+    // one wave per workgroup: a 32-channel x 32-column tile as 2 x 2 MFMA tiles,
+    // so each k-step's operand fragments feed four (or twelve) products
+    const int lane = threadIdx.x;
+    const int tiles_c = (cols + 31) / 32;
+    const int c0 = (blockIdx.x / tiles_c) * 32, col0 = (blockIdx.x % tiles_c) * 32;
+    if (c0 >= C) return;
     const int i = lane & 15, kq = lane >> 4;
-    const int c = min(c0 + i, C - 1);  // rows beyond C compute a copy and are not stored
-    const float *xr = x + (size_t)c * in_stride;
-    f32x4_t acc[4];
+    const float *xr[2];
+    int colr[2];
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[q] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < 2; u++) {
+        xr[u] = x + (size_t)min(c0 + 16 * u + i, C - 1) * in_stride;  // rows past C compute a copy, not stored
+        colr[u] = min(col0 + 16 * u + i, cols - 1);
+    }
+    f32x4_t acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int v = 0; v < 2; v++) acc[u][v] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
     const int kv = n / 8;  // bf16x8 vectors per table row
     for (int k0 = 0; k0 < n; k0 += 32) {
         const int kb = k0 + 8 * kq;
-        bf16x8_t ah, al;
+        bf16x8_t ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const float v = xr[kb + e];
-            const __bf16 h = (__bf16)v;
-            ah[e] = h;
-            al[e] = (__bf16)(v - (float)h);
-        }
+        for (int u = 0; u < 2; u++) {
+            const float4 p0 = *reinterpret_cast<const float4 *>(xr[u] + kb);
+            const float4 p1 = *reinterpret_cast<const float4 *>(xr[u] + kb + 4);
+            const float v8[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int col = min(col0 + 16 * q + i, cols - 1);
-            const bf16x8_t bh = fh[(size_t)col * kv + kb / 8];
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[q], 0, 0, 0);
-            if (split == 3) {
-                const bf16x8_t bl = fl[(size_t)col * kv + kb / 8];
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[q], 0, 0, 0);
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[q], 0, 0, 0);
+            for (int e = 0; e < 8; e++) {
+                const __bf16 h = (__bf16)v8[e];
+                ah[u][e] = h;
+                al[u][e] = (__bf16)(v8[e] - (float)h);
             }
         }
-    }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int col = col0 + 16 * q + i;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = c0 + 4 * kq + r;
-            if (row < C && col < cols) y[(size_t)row * out_stride + col] = acc[q][r];
+        for (int v = 0; v < 2; v++) {
+            bh[v] = fh[(size_t)colr[v] * kv + kb / 8];
+            if (split == 3) bl[v] = fl[(size_t)colr[v] * kv + kb / 8];
         }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+                acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], bh[v], acc[u][v], 0, 0, 0);
+                if (split == 3) {
+                    acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], bl[v], acc[u][v], 0, 0, 0);
+                    acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[u], bh[v], acc[u][v], 0, 0, 0);
+                }
+            }
     }
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            const int col = col0 + 16 * v + i;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = c0 + 16 * u + 4 * kq + r;
+                if (row < C && col < cols) y[(size_t)row * out_stride + col] = acc[u][v][r];
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1251,9 +1270,8 @@ hipError_t launch_dft_mfma(const float *x, long long in_stride, float *y, long l
     if (C <= 0) return hipSuccess;
     if (n < 32 || (n & 31) || (split != 1 && split != 3)) return hipErrorInvalidValue;
     const int cols = n + 2;
-    const long long waves = (long long)((C + 15) / 16) * ((cols + 63) / 64);
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    hipLaunchKernelGGL(k_dft_mfma, dim3(blocks), dim3(256), 0, s, x, in_stride, y, out_stride, C, n, cols,
+    const unsigned blocks = (unsigned)(((C + 31) / 32) * ((cols + 31) / 32));
+    hipLaunchKernelGGL(k_dft_mfma, dim3(blocks), dim3(64), 0, s, x, in_stride, y, out_stride, C, n, cols,
                        static_cast<const bf16x8_t *>(fh), static_cast<const bf16x8_t *>(fl), split);
     return hipGetLastError();
 }
