@@ -198,6 +198,71 @@ def bench_sweep(T, torch, ctx, counts, ticks=96, method=UNIFORM_METHOD):
     return best, rows
 
 
+def bench_sweep_phased(T, torch, ctx, counts, ticks=96):
+    """Non-uniform deadline sweep with the channels split over M objects of
+    frame phases 0..M-1 (M: the library's frame multiple for the first
+    object), advanced together in one group launch per block: each block
+    carries the frame work of one object instead of all channels' frames
+    falling on the same block."""
+    from tan_amd import serving as S
+    rows = []
+    best = 0
+    for ch in counts:
+        free, _ = torch.cuda.mem_get_info()
+        need = (S.estimate_bytes(ctx._lib, L, B, ch, method=NONUNIFORM_METHOD) + ch * L * 4
+                + 2 * ch * 64 * B * 4) * 1.10
+        if need > free:
+            log(f"phased sweep: skip {ch} channels (needs {need / 2**30:.1f} GiB)")
+            break
+        first = T.TANConvolution(ctx)
+        assert first.InitGpu(NONUNIFORM_METHOD, L, B, ch) == T.AMF_OK
+        M = first.Info()["nu_multiple"]
+        first.Terminate()
+        K = max(1, M)
+        bounds = [ch * k // K for k in range(K + 1)]
+        objs = []
+        for k in range(K):
+            conv = T.TANConvolution(ctx, nu_multiple=M, nu_phase=k)
+            assert conv.InitGpu(NONUNIFORM_METHOD, L, B, bounds[k + 1] - bounds[k]) == T.AMF_OK
+            objs.append(conv)
+        for k, conv in enumerate(objs):
+            h = ir_bank_device(torch, bounds[k + 1] - bounds[k], L, 77 + k)
+            torch.cuda.synchronize()
+            assert conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY) == 0
+            del h
+        x = noise_device(torch, ch, 64 * B, 5)
+        y = torch.empty_like(x)
+        xs = [[x[bounds[k]:bounds[k + 1], t * B:] for k in range(K)] for t in range(64)]
+        ys = [[y[bounds[k]:bounds[k + 1], t * B:] for k in range(K)] for t in range(64)]
+        torch.cuda.synchronize()
+        for t in range(2 * M + 16):
+            assert T.GroupProcess(objs, xs[t % 64], ys[t % 64]) == T.AMF_OK
+        objs[0].Synchronize()
+        lat = []
+        for t in range(ticks):
+            t0 = time.perf_counter()
+            assert T.GroupProcess(objs, xs[t % 64], ys[t % 64]) == T.AMF_OK
+            objs[0].Synchronize()
+            lat.append((time.perf_counter() - t0) * 1e3)
+        finite = bool(torch.isfinite(y).all().item())
+        for conv in objs:
+            conv.Terminate()
+        del x, y, xs, ys
+        torch.cuda.empty_cache()
+        lat = np.array(lat)
+        row = {"channels": ch, "method": "nonuniform-phased", "objects": K, "nu_multiple": M,
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "max_ms": float(lat.max()), "outputs_finite": finite,
+               "meets_deadline": bool(lat.max() <= DEADLINE_MS)}
+        log(f"phased sweep: {row}")
+        rows.append(row)
+        if row["meets_deadline"]:
+            best = ch
+        else:
+            break
+    return best, rows
+
+
 def bench_precision(T, torch, ctx, channels=1024, taps=131072, warm=8, blocks=32):
     """Reduced-precision IR spectra on the precision workload (1,024 channels,
     B=256, 131,072 taps): block time, algorithmic GB/s, real-time channels
@@ -625,13 +690,16 @@ def main():
         nu_value = C * args.steps * B / FS / dtn
         log(f"  non-uniform (M = {infon['nu_multiple']}): {nu_value:.0f} real-time channels")
         nu_best, nu_rows = (0, [])
+        ph_best, ph_rows = (0, [])
         if not args.no_sweep:
-            nu_best, nu_rows = bench_sweep(T, torch, ctx, [int(v) for v in args.nu_sweep.split(",") if v],
-                                           method=NONUNIFORM_METHOD)
+            counts = [int(v) for v in args.nu_sweep.split(",") if v]
+            nu_best, nu_rows = bench_sweep(T, torch, ctx, counts, method=NONUNIFORM_METHOD)
+            ph_best, ph_rows = bench_sweep_phased(T, torch, ctx, counts)
         nonuniform = {"method": "TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM",
                       "nu_multiple": infon["nu_multiple"], "value": nu_value,
                       "ms_per_step": dtn / args.steps * 1e3, "rt_channels_at_deadline": nu_best,
-                      "sweep": nu_rows}
+                      "sweep": nu_rows, "rt_channels_at_deadline_phased": ph_best,
+                      "sweep_phased": ph_rows}
 
     batching = []
     if not args.no_batching:

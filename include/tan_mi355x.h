@@ -29,6 +29,7 @@ enum {
     TANX_FAIL = 1,
     TANX_UNEXPECTED = 2,
     TANX_INVALID_ARG = 4,
+    TANX_OUT_OF_RANGE = 5,
     TANX_OUT_OF_MEMORY = 6,
     TANX_INVALID_POINTER = 7,
     TANX_NOT_IMPLEMENTED = 9,
@@ -36,7 +37,9 @@ enum {
     TANX_NOT_FOUND = 11,
     TANX_ALREADY_INITIALIZED = 12,
     TANX_NOT_INITIALIZED = 13,
+    TANX_INVALID_FORMAT = 14,
     TANX_WRONG_STATE = 15,
+    TANX_FILE_NOT_OPEN = 16,
     TANX_NO_DEVICE = 17,
     TANX_TAN_UNSUPPORTED_VERSION = 43
 };
@@ -122,7 +125,12 @@ typedef struct tanx_conv_options {
     uint32_t nu_multiple;   /* non-uniform methods: blocks per frame (a power
                                of two >= 2, frame <= 4096 samples); 0 = chosen
                                by the library, 1 = uniform partitions */
-    uint32_t reserved[6];   /* zero */
+    uint32_t nu_phase;      /* non-uniform objects: frame phase in blocks,
+                               taken modulo M (ignored for uniform objects):
+                               the first frame ends at block M-1-(phase % M).
+                               Objects of one group with different phases
+                               spread their frame work over the ticks. */
+    uint32_t reserved[5];   /* zero */
 } tanx_conv_options;
 /* tanx_conv_create with options (no counterpart in the reference, which
  * stores FP32 spectra only); opt may be NULL. */
@@ -321,6 +329,40 @@ int32_t tanx_convert(tanx_ctx *ctx, const void *in, size_t in_step, int32_t in_t
  * on the context's process stream. */
 int32_t tanx_dft_r2c_mfma_device(tanx_ctx *ctx, uint32_t log2len, uint32_t channels, const float *d_in,
                                  size_t in_stride, float *d_out, size_t out_stride, uint32_t split);
+
+/* ---- IR sources: WAV files (host) ------------------------------------------- */
+
+/* Format of a WAV file: ReadWaveFile's outputs (tan/common/wav.cpp:87-98). */
+typedef struct tanx_wav_info {
+    uint32_t sample_rate;
+    uint16_t bits_per_sample; /* 8, 16 or 32 (32: IEEE float) */
+    uint16_t channels;
+    uint32_t samples;         /* per channel: data bytes * 8 / (bits * channels) */
+    uint16_t format_tag;
+    uint16_t reserved;
+} tanx_wav_info;
+
+/* ReadWaveFile's header walk (wav.cpp:107-171) without the samples.
+ * AMF_FILE_NOT_OPEN: no such file; AMF_INVALID_FORMAT: not RIFF/WAVE, no
+ * "fmt " or "data" chunk, or bits other than 8, 16, 32. */
+int32_t tanx_wav_read_info(const char *path, tanx_wav_info *info);
+/* ReadWaveFile (wav.cpp:87-238): de-interleaves the samples into out[c][0 ..
+ * min(samples, capacity)) as floats: 8-bit (u - 127) / 256, 16-bit
+ * s / 32768, 32-bit the stored float. */
+int32_t tanx_wav_read(const char *path, tanx_wav_info *info, float *const *out, size_t capacity);
+/* WavContent::ReadWaveFile (wav.cpp:403-505): the interleaved sample bytes
+ * (AMF_OUT_OF_RANGE if they exceed capacity_bytes). */
+int32_t tanx_wav_read_raw(const char *path, tanx_wav_info *info, void *data, size_t capacity_bytes);
+/* WriteWaveFileF (wav.cpp:240-318): planar floats to 8-, 16- or 32-bit
+ * samples; 8 and 16 bits clamp to [-1, 1], scale by 127 or 32767 and convert
+ * toward zero (8-bit samples are written as signed bytes, as the reference
+ * writes them).  Returns AMF_OK on success (the reference's returns false). */
+int32_t tanx_wav_write(const char *path, uint32_t sample_rate, uint16_t channels, uint16_t bits, uint32_t samples,
+                       const float *const *in);
+/* WriteWaveFileS (wav.cpp:320-401): interleaved 16-bit samples as they are;
+ * other widths AMF_NOT_IMPLEMENTED (the reference throws). */
+int32_t tanx_wav_write_short(const char *path, uint32_t sample_rate, uint16_t channels, uint16_t bits,
+                             uint32_t samples, const int16_t *interleaved);
 
 #ifdef __cplusplus
 }

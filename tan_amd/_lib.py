@@ -28,7 +28,13 @@ class ConvInfo(ctypes.Structure):
 
 class ConvOptions(ctypes.Structure):
     _fields_ = [("ir_precision", ctypes.c_uint32), ("nu_multiple", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+                ("nu_phase", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+
+
+class WavInfo(ctypes.Structure):
+    _fields_ = [("sample_rate", ctypes.c_uint32), ("bits_per_sample", ctypes.c_uint16),
+                ("channels", ctypes.c_uint16), ("samples", ctypes.c_uint32),
+                ("format_tag", ctypes.c_uint16), ("reserved", ctypes.c_uint16)]
 
 
 class KernelTiming(ctypes.Structure):
@@ -82,6 +88,12 @@ _SIGS = {
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),
     "tanx_fft_transform_device": (_i32, [_vp, _u32, _u32, _u32, _vp, _vp, _sz]),
     "tanx_version": (ctypes.c_char_p, []),
+    "tanx_wav_read_info": (_i32, [ctypes.c_char_p, ctypes.POINTER(WavInfo)]),
+    "tanx_wav_read": (_i32, [ctypes.c_char_p, ctypes.POINTER(WavInfo), ctypes.POINTER(_f32p), _sz]),
+    "tanx_wav_read_raw": (_i32, [ctypes.c_char_p, ctypes.POINTER(WavInfo), _vp, _sz]),
+    "tanx_wav_write": (_i32, [ctypes.c_char_p, _u32, ctypes.c_uint16, ctypes.c_uint16, _u32,
+                              ctypes.POINTER(_f32p)]),
+    "tanx_wav_write_short": (_i32, [ctypes.c_char_p, _u32, ctypes.c_uint16, ctypes.c_uint16, _u32, _vp]),
 }
 
 _lib = None

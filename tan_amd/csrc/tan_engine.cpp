@@ -303,6 +303,7 @@ struct tanx_conv {
     float2 *AC = nullptr;    // [2 parity][2 filter][C][nchB][N] frame partial sums
     float *ZB = nullptr;     // [2 parity][2 filter][C][N] frame outputs
     uint64_t switch_at = 0;  // block index of the armed crossfade
+    int nu_shift = 0;        // frame clock = block index + nu_shift (frame phase option)
     long long zsel_frame = -1;  // frame whose current-filter output is in ZB[.][1]
 
     size_t brow_bytes() const { return (size_t)N * ir_elem_bytes(irtype); }
@@ -373,7 +374,7 @@ static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(k
 // Shape of an object: uniform partitions of B over the first `head` taps
 // (P = head / B), then (non-uniform objects) PB frame partitions of N = M * B.
 struct ConvShape {
-    int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1;
+    int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1, shift = 0;
     int irtype = IR_F32;
     bool td = false;  // time-domain object: IR slots and history in samples
 };
@@ -462,6 +463,9 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
     }
     sh.M = M;
     if (M > 1) {
+        // frames end where (block + 1 + shift) % M == 0: block M - 1 - shift
+        // first (the frame clock runs `shift` blocks ahead of the block index)
+        sh.shift = (int)((opt ? opt->nu_phase : 0) % (uint32_t)M);
         sh.N = M * sh.B;
         sh.P = M;
         sh.RB = sh.len / sh.N;
@@ -577,6 +581,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->RB = sh.RB;
     cv->nchB = sh.nchB;
     cv->pcB = sh.pcB;
+    cv->nu_shift = sh.shift;
     // partial-sum rows are only worth it at small channel counts
     cv->maxchunk = sh.maxchunk;
     set_chunks(cv, auto_chunks(cv->C, cv->P));
@@ -874,8 +879,8 @@ static bool advance_switch(tanx_conv *cv, uint64_t b)
         cv->delayed = true;
         // uniform: the next block; non-uniform: a frame boundary late enough
         // for the armed filter's frame partials (M + 1 blocks ahead)
-        const uint64_t M = (uint64_t)cv->M;
-        cv->switch_at = cv->nu ? ((b + M + 1 + M - 1) / M) * M : b + 1;
+        const uint64_t M = (uint64_t)cv->M, d = (uint64_t)cv->nu_shift;
+        cv->switch_at = cv->nu ? ((b + d + M + 1 + M - 1) / M) * M - d : b + 1;
     } else if (cv->delayed && b == cv->switch_at) {
         xfade = true;
     }
@@ -893,9 +898,13 @@ static bool advance_switch(tanx_conv *cv, uint64_t b)
 // armed, the armed one alone at the block before an armed switch (gm's
 // blocks all follow it), else the current one.  The frame kernel at the last
 // block of frame g feeds block b+1.  Returns whether the update slot is read.
-static bool nu_block_args(tanx_conv *cv, uint64_t b, bool xfade, TickArgs &a, FrameArgs &fa, bool *frame_due)
+static bool nu_block_args(tanx_conv *cv, uint64_t b_obj, bool xfade, TickArgs &a, FrameArgs &fa, bool *frame_due)
 {
     const uint64_t M = (uint64_t)cv->M;
+    // the object's frame clock (a frame phase shifts it); b and the switch
+    // block are read in it below, ring slots and parities follow it
+    const uint64_t b = b_obj + (uint64_t)cv->nu_shift;
+    const uint64_t switch_at = cv->switch_at + (uint64_t)cv->nu_shift;
     bool reads_upd = false;
     a.nu = 1;
     a.M = cv->M;
@@ -920,8 +929,8 @@ static bool nu_block_args(tanx_conv *cv, uint64_t b, bool xfade, TickArgs &a, Fr
         a.pcB = cv->pcB;
         a.fbase = (int)(gm & (uint64_t)(cv->RB - 1));
         a.bslice = (int)((b + 1) % M);
-        const bool both = cv->delayed && cv->switch_at == (gm + 1) * M;
-        const bool next_only = !both && cv->delayed && cv->switch_at == b + 1;
+        const bool both = cv->delayed && switch_at == (gm + 1) * M;
+        const bool next_only = !both && cv->delayed && switch_at == b + 1;
         const int fa0 = next_only ? cv->upd : cv->cur;
         a.bHa = cv->bslot_ptr(fa0);
         a.bSa = cv->bscale_ptr(fa0);
@@ -934,7 +943,7 @@ static bool nu_block_args(tanx_conv *cv, uint64_t b, bool xfade, TickArgs &a, Fr
     *frame_due = ((b + 1) % M) == 0;
     if (*frame_due) {
         const uint64_t g = (b + 1) / M - 1;
-        const bool two = cv->delayed && cv->switch_at == b + 1;
+        const bool two = cv->delayed && switch_at == b + 1;
         fa = FrameArgs{};
         fa.B = cv->B;
         fa.M = cv->M;

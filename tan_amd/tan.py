@@ -142,10 +142,13 @@ class TANContext:
 class TANConvolution:
     """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
 
-    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0):
+    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0, nu_phase=0):
         """ir_precision: storage of the IR spectra (TAN_IR_PRECISION_*);
         nu_multiple: blocks per frame for the non-uniform methods (0: chosen
-        by the library, 1: uniform partitions)."""
+        by the library, 1: uniform partitions); nu_phase: frame phase in
+        blocks, modulo the multiple (the first frame ends at block
+        multiple - 1 - phase), so that objects advanced together can spread
+        their frame work over the ticks."""
         self._ctx = context
         self._lib = context._lib
         self.handle = None
@@ -153,6 +156,7 @@ class TANConvolution:
         self.block = 0
         self.ir_precision = ir_precision
         self.nu_multiple = nu_multiple
+        self.nu_phase = nu_phase
 
     # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
     def Init(self, method, response_length, buffer_size, channels):
@@ -165,6 +169,7 @@ class TANConvolution:
         opt = _lib.ConvOptions()
         opt.ir_precision = self.ir_precision
         opt.nu_multiple = self.nu_multiple
+        opt.nu_phase = self.nu_phase
         rc = self._lib.tanx_conv_create_ex(self._ctx.handle, method, response_length, buffer_size,
                                            channels, ctypes.addressof(opt), ctypes.byref(h))
         if rc == AMF_OK:

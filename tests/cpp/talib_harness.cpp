@@ -6,8 +6,12 @@
 // same calls.  Also exercises TANFFT, TANMixer and TANConverter through the
 // C++ interface.
 //
-// usage: talib_harness <dir with pulsetrainS_0p5s.wav, testresponse.wav, testresponse2.wav>
+// usage: talib_harness <dir with pulsetrainS_0p5s.wav, testresponse.wav, testresponse2.wav> [out.wav]
+//
+// Like the driver, it writes the convolution output with WriteWaveFileF
+// (tanx_wav_write, 16-bit) and checks the file read back.
 #include "TrueAudioNext.h"
+#include "tan_mi355x.h"
 #include "tan_oracle.h"
 
 #include <cmath>
@@ -20,43 +24,17 @@ using namespace amf;
 
 namespace {
 
-// PCM16 WAV reader; samples scaled by 1/32768 as the reference's reader does
-// (tan/common/wav.cpp:235).
+// ReadWaveFile (tan/common/wav.cpp:87-238) through the library
+// (tanx_wav_read): planar floats, 16-bit samples scaled by 1/32768.
 bool read_wav(const std::string &path, int &channels, std::vector<std::vector<float>> &data)
 {
-    FILE *f = std::fopen(path.c_str(), "rb");
-    if (!f) return false;
-    std::vector<unsigned char> buf;
-    unsigned char tmp[65536];
-    size_t n;
-    while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
-    std::fclose(f);
-    if (buf.size() < 12 || std::memcmp(buf.data(), "RIFF", 4) || std::memcmp(buf.data() + 8, "WAVE", 4)) return false;
-    size_t pos = 12;
-    int bits = 0;
-    channels = 0;
-    while (pos + 8 <= buf.size()) {
-        uint32_t sz;
-        std::memcpy(&sz, buf.data() + pos + 4, 4);
-        const unsigned char *body = buf.data() + pos + 8;
-        if (!std::memcmp(buf.data() + pos, "fmt ", 4)) {
-            uint16_t ch, bps;
-            std::memcpy(&ch, body + 2, 2);
-            std::memcpy(&bps, body + 14, 2);
-            channels = ch;
-            bits = bps;
-        } else if (!std::memcmp(buf.data() + pos, "data", 4)) {
-            if (bits != 16 || channels <= 0) return false;
-            const size_t frames = sz / (2 * channels);
-            data.assign(channels, std::vector<float>(frames));
-            const int16_t *s = reinterpret_cast<const int16_t *>(body);
-            for (size_t i = 0; i < frames; i++)
-                for (int c = 0; c < channels; c++) data[c][i] = (float)s[i * channels + c] / 32768.0f;
-            return true;
-        }
-        pos += 8 + sz + (sz & 1);
-    }
-    return false;
+    tanx_wav_info info{};
+    if (tanx_wav_read_info(path.c_str(), &info) != TANX_OK || info.channels == 0) return false;
+    channels = info.channels;
+    data.assign(channels, std::vector<float>(info.samples));
+    std::vector<float *> rows(channels);
+    for (int c = 0; c < channels; c++) rows[c] = data[c].data();
+    return tanx_wav_read(path.c_str(), &info, rows.data(), info.samples) == TANX_OK;
 }
 
 int fail(const char *what, int rc)
@@ -146,6 +124,26 @@ int main(int argc, char **argv)
     std::printf("convolution: %d blocks, %d crossfades, max|gpu-oracle| = %.3e, SNR = %.1f dB\n", b, xfades, maxerr,
                 snr);
 
+    // the driver's WriteWaveFileF of the output (TALibTestConvolution.cpp:290),
+    // read back: trunc(clamp(y, -1, 1) * 32767) / 32768 per sample
+    bool wav_ok = true;
+    if (argc > 2) {
+        std::vector<const float *> rows(nch);
+        for (int c = 0; c < nch; c++) rows[c] = out[c].data();
+        if ((r = (AMF_RESULT)tanx_wav_write(argv[2], 48000, (uint16_t)nch, 16, NSamples, rows.data())) != AMF_OK)
+            return fail("tanx_wav_write", r);
+        int bch = 0;
+        std::vector<std::vector<float>> back;
+        wav_ok = read_wav(argv[2], bch, back) && bch == nch && back[0].size() == NSamples;
+        for (int c = 0; wav_ok && c < nch; c++)
+            for (amf_uint32 i = 0; i < NSamples; i++) {
+                double v = out[c][i];
+                v = v > 1.0 ? 1.0 : v < -1.0 ? -1.0 : v;
+                if (back[c][i] != (float)(int16_t)(v * 32767.0) / 32768.0f) wav_ok = false;
+            }
+        std::printf("wav: output written and read back %s\n", wav_ok ? "exactly" : "WRONG");
+    }
+
     // TANFFT through the C++ interface: R2C planar then C2R planar round trip
     TANFFTPtr fft;
     if ((r = TANCreateFFT(ctx, &fft)) != AMF_OK) return fail("TANCreateFFT", r);
@@ -194,7 +192,8 @@ int main(int argc, char **argv)
 
     orc_conv_destroy(orc);
     ctx->Terminate();  // while conv/fft are still alive, as the driver does
-    const bool ok = maxerr <= 1e-5 && snr >= 120.0 && xfades == expected_xfades && fe <= 1e-4 && re <= 1e-5 && chain_ok;
+    const bool ok = maxerr <= 1e-5 && snr >= 120.0 && xfades == expected_xfades && fe <= 1e-4 && re <= 1e-5 && chain_ok &&
+                    wav_ok;
     std::printf("%s\n", ok ? "PASS" : "FAIL");
     return ok ? 0 : 1;
 }

@@ -17,22 +17,63 @@ UNIFORM = T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
 BLOCKING = T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY
 
 
-def nu_switch_block(update_before, M, last_switch=None):
+def nu_switch_block(update_before, M, last_switch=None, phase=0):
     """Block of the crossfade for an update issued before block
     `update_before`: armed one block later (or, when issued while another
-    switch is armed, at the block after that switch), then the first frame
-    boundary at least M + 1 blocks after arming."""
-    armed = update_before + 1
+    switch is armed, at the block after that switch; issued before the
+    crossfade block itself, a block later still), then the first frame
+    boundary at least M + 1 blocks after arming.  Frame boundaries lie where
+    (block + phase) % M == 0."""
+    handed = update_before
+    if last_switch is not None and update_before == last_switch:
+        handed += 1
+    armed = handed + 1
     if last_switch is not None and armed <= last_switch:
         armed = last_switch + 1
-    return -(-(armed + M + 1) // M) * M
+    return -(-(armed + phase + M + 1) // M) * M - phase
+
+
+def nu_schedule(updates, M, phase, nb):
+    """Crossfades of a non-uniform object as (block, update index) pairs: a
+    block-level model of the switch state machine.  An update issued before
+    block b is handed off at the end of the first block >= b that does not
+    crossfade and armed at the next block with no switch armed; an armed
+    switch falls on the first frame boundary ((block + phase) % M == 0) at
+    least M + 1 blocks after arming.  An update issued while a switch is
+    armed is held for the next one, and an update that has not been armed
+    yet is replaced by a later one."""
+    pending = ready = armed = False
+    waiting = held = None  # update in the slot to arm next / held while armed
+    switch_at = cur = None
+    out = []
+    k = 0
+    for b in range(nb):
+        while k < len(updates) and updates[k] == b:
+            if armed:
+                held = k
+            else:
+                waiting = k
+            pending = True
+            k += 1
+        xfade = False
+        if not pending and ready and not armed:
+            ready, armed, cur = False, True, waiting
+            switch_at = -(-(b + phase + M + 1) // M) * M - phase
+        elif armed and b == switch_at:
+            xfade = True
+            out.append((b, cur))
+            armed = False
+            waiting, held = held, None
+        if not xfade and pending:
+            pending, ready = False, True
+    return out
 
 
 def _run(ctx, method, L, B, C, x, irs, updates, multiple=0, precision=T.TAN_IR_PRECISION_FP32,
-         device=True):
+         device=True, phase=0):
     """Process x block by block; irs[k] is installed before block updates[k]."""
     import torch
-    conv = T.TANConvolution(ctx, ir_precision=precision, nu_multiple=multiple)
+    conv = T.TANConvolution(ctx, ir_precision=precision, nu_multiple=multiple, nu_phase=phase)
     assert conv.InitGpu(method, L, B, C) == T.AMF_OK
     info = conv.Info()
     nb = x.shape[1] // B
@@ -181,5 +222,62 @@ def test_group_launch_with_nonuniform_objects(gpu_ctx):
         c.Synchronize()
     for i in range(len(specs)):
         np.testing.assert_array_equal(yg[i].cpu().numpy(), ya[i].cpu().numpy(), err_msg=f"object {i}")
+    for c in objs + alone:
+        c.Terminate()
+
+
+@pytest.mark.parametrize("phase", [1, 3, 6, 7, 13])
+def test_frame_phase_shifts_frames_and_switches(gpu_ctx, phase):
+    """A frame phase moves the frame boundaries (and with them the switch
+    blocks) by `phase` blocks, modulo M; the output is the same time-varying
+    convolution at those switch blocks.  Updates 9 and 17 fall on or next to
+    the switch blocks of some phases."""
+    L, B, C, M = 32768, 128, 2, 8
+    nb = 100
+    x = T.signals.input_bank(C, nb * B, base_seed=130)
+    ups = [0, 9, 17, 40, 58]
+    irs = [T.signals.ir_bank(C, L, k, base_seed=131) for k in range(len(ups))]
+    y, info = _run(gpu_ctx, NU, L, B, C, x, irs, ups, multiple=M, phase=phase)
+    assert info["nu_multiple"] == M
+    sched = nu_schedule(ups, M, phase % M, nb)
+    sw = [s for s, _ in sched]
+    assert all((s + phase) % M == 0 for s in sw)
+    for c in range(C):
+        ref = crossfade_model(x[c], [irs[k][c] for _, k in sched], sw, B)
+        assert np.abs(y[c] - ref).max() <= 1e-5, f"channel {c}: switches {sw}"
+        assert snr_db(ref, y[c]) >= 120.0
+
+
+def test_phased_objects_in_one_group(gpu_ctx):
+    """M objects with phases 0..M-1 advanced in one group launch: each tick
+    runs the frame kernel of one object, and every object's output equals
+    the same object advanced alone."""
+    import torch
+    L, B, C, M = 65536, 256, 3, 4
+    nb = 40
+    objs, alone, xs, yg, ya = [], [], [], [], []
+    for p in range(M):
+        x = T.signals.input_bank(C, nb * B, base_seed=140 + p)
+        ir = T.signals.ir_bank(C, L, 0, base_seed=150 + p)
+        for lst in (objs, alone):
+            c = T.TANConvolution(gpu_ctx, nu_multiple=M, nu_phase=p)
+            assert c.InitGpu(NU, L, B, C) == T.AMF_OK
+            assert c.UpdateResponseTD(ir, L, None, BLOCKING) == T.AMF_OK
+            lst.append(c)
+        xs.append(torch.from_numpy(x).cuda())
+        yg.append(torch.zeros(C, nb * B, device="cuda"))
+        ya.append(torch.zeros(C, nb * B, device="cuda"))
+    for b in range(nb):
+        assert T.GroupProcess(objs, [xi[:, b * B:] for xi in xs], [yi[:, b * B:] for yi in yg]) == T.AMF_OK
+        for p in range(M):
+            rc, done = alone[p].Process(xs[p][:, b * B:], ya[p][:, b * B:], B)
+            assert rc == T.AMF_OK and done == B
+    for c in objs + alone:
+        c.Synchronize()
+    for p in range(M):
+        np.testing.assert_array_equal(yg[p].cpu().numpy(), ya[p].cpu().numpy(), err_msg=f"phase {p}")
+        s = nu_switch_block(0, M, phase=p)
+        ref = fft_conv_f64(xs[p].cpu().numpy()[0], T.signals.ir_bank(C, L, 0, base_seed=150 + p)[0])
+        assert np.abs(yg[p].cpu().numpy()[0, (s + 1) * B:] - ref[(s + 1) * B:]).max() <= 1e-5
     for c in objs + alone:
         c.Terminate()
