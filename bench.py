@@ -523,6 +523,77 @@ def bench_dft_mfma(T, torch, ctx, channels=1024, log2n=9, reps=50):
     return rows
 
 
+ROOM_SIZE = (10.0, 4.0, 8.0)  # m: width, height, length
+ROOM_LEN = 65536               # response taps: the headline IR length
+
+
+def _room_scene(R, count):
+    """`count` sources and listeners at random places in one 10 x 4 x 8 m
+    room with damping 0.9 (floor 0.6, ceiling 0.7)."""
+    room = R.make_room(*ROOM_SIZE, 0.9, damp_bottom=0.6, damp_top=0.7)
+    rng = np.random.default_rng(11)
+    srcs = [R.MonoSource(*rng.uniform([0.5, 0.5, 0.5], [9.5, 3.5, 7.5])) for _ in range(count)]
+    lsts = [R.make_listener(tuple(rng.uniform([1.0, 1.2, 1.0], [9.0, 2.0, 7.0])), 0.16,
+                            float(rng.uniform(-180, 180))) for _ in range(count)]
+    return room, srcs, lsts
+
+
+def _room_images(length):
+    n = [1 + length // int(v / 340.0 * FS) for v in ROOM_SIZE]
+    return int(np.prod([2 * v + 1 for v in n]))
+
+
+def bench_room(T, torch, ctx, counts=(1, 64), reps=5):
+    """Room responses by image sources (IR source for the convolution):
+    `count` listener/source pairs of 65,536 taps per launch, both ears."""
+    from tan_amd import room as R
+    s = torch.cuda.ExternalStream(T._lib.load().tanx_ctx_process_stream(ctx.handle))
+    rows = []
+    for count in counts:
+        room, srcs, lsts = _room_scene(R, count)
+        out = torch.empty((2 * count, ROOM_LEN), device="cuda")
+        rooms = [room] * count
+        assert R.GenerateRoomResponses(ctx, rooms, srcs, lsts, FS, ROOM_LEN, out) == 0
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            R.GenerateRoomResponses(ctx, rooms, srcs, lsts, FS, ROOM_LEN, out)
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        images = 2 * count * _room_images(ROOM_LEN)
+        row = {"responses": count, "taps": ROOM_LEN, "room_m": list(ROOM_SIZE), "ms_per_launch": ms,
+               "responses_per_s": count / (ms * 1e-3), "images_per_s": images / (ms * 1e-3),
+               "outputs_finite": bool(torch.isfinite(out).all().item())}
+        log(f"  room responses x{count}: {ms:.3f} ms per launch, {row['responses_per_s']:.0f} responses/s, "
+            f"{row['images_per_s'] / 1e9:.1f} G images/s")
+        rows.append(row)
+        del out
+    return rows
+
+
+def cpu_room_baseline():
+    """The oracle's restatement of the room generator (one response, both
+    ears, one thread), on the same scene."""
+    from tan_amd import room as R
+    path = os.path.join(ROOT, "oracle", "build", "libtan_oracle.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.orc_room_response.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                                              ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    room, srcs, lsts = _room_scene(R, 1)
+    left = np.zeros(ROOM_LEN, np.float32)
+    right = np.zeros(ROOM_LEN, np.float32)
+    t0 = time.perf_counter()
+    rc = lib.orc_room_response(ctypes.byref(room), ctypes.byref(srcs[0]), ctypes.byref(lsts[0]), FS, ROOM_LEN, 0, 0,
+                               left.ctypes.data, right.ctypes.data)
+    dt = time.perf_counter() - t0
+    return {"responses_per_s": 1.0 / dt, "cores": 1, "kind": "port", "rc": rc,
+            "sample": f"one {ROOM_LEN}-tap response (both ears) in the same room, scalar C"}
+
+
 def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1, 0>"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
@@ -623,8 +694,9 @@ def main():
     ap.add_argument("--batching-streams", type=str, default="512,2048")
     ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--no-nonuniform", action="store_true")
+    ap.add_argument("--no-room", action="store_true")
     ap.add_argument("--serving-methods", type=str, default="uniform,nonuniform")
-    ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304")
+    ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304,131072,163840")
     ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
@@ -701,6 +773,11 @@ def main():
                       "sweep": nu_rows, "rt_channels_at_deadline_phased": ph_best,
                       "sweep_phased": ph_rows}
 
+    room_rows = []
+    if not args.no_room:
+        log("room response pass")
+        room_rows = bench_room(T, torch, ctx)
+
     batching = []
     if not args.no_batching:
         log("batching pass")
@@ -720,6 +797,11 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(args.cpu_seconds)
         log(f"  {cpu}")
+        if room_rows:
+            cpu_room = cpu_room_baseline()
+            log(f"  room generator: {cpu_room}")
+            for r in room_rows:
+                r["cpu_responses_per_s"] = cpu_room["responses_per_s"] if cpu_room else None
     ctx.Terminate()
 
     big = max(rows, key=lambda r: r["channels"]) if rows else None
@@ -743,6 +825,7 @@ def main():
         "nonuniform": nonuniform,
         "batching": batching,
         "serving": serving,
+        "room_responses": room_rows,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

@@ -364,6 +364,64 @@ int32_t tanx_wav_write(const char *path, uint32_t sample_rate, uint16_t channels
 int32_t tanx_wav_write_short(const char *path, uint32_t sample_rate, uint16_t channels, uint16_t bits,
                              uint32_t samples, const int16_t *interleaved);
 
+/* ---- IR sources: room responses by image sources (device) ------------------ */
+
+/* generateRoomResponse flags (TrueAudioVR.h:54-56). */
+enum { TANX_ROOM_LIMIT_BOUNCES = 0x40, TANX_ROOM_SUPPRESS_DIRECT = 0x80 };
+#define TANX_HEAD_FILTER_MAX 64 /* SIMPLEHRTF_FFTLEN */
+
+/* RoomDefinition (TrueAudioVR.h:101-107), same layout: dimensions in meters
+ * (x: width, y: height, z: length) and the wall damping factors in [0, 1]
+ * (MaterialProperty::damp, in the order mLeft, mRight, mTop, mBottom,
+ * mFront, mBack). */
+typedef struct tanx_room {
+    float width, height, length;
+    float damp_left, damp_right, damp_top, damp_bottom, damp_front, damp_back;
+} tanx_room;
+/* MonoSource (TrueAudioVR.h:69-72): the source position. */
+typedef struct tanx_source {
+    float x, y, z;
+} tanx_source;
+/* StereoListener + HeadModel (TrueAudioVR.h:62-67, 76-81), same layout:
+ * head position, ear spacing (m), yaw / pitch / roll (degrees) and the head
+ * filter pair. */
+typedef struct tanx_listener {
+    float head_x, head_y, head_z;
+    float ear_spacing;
+    float yaw, pitch, roll;
+    int32_t filter_length;
+    float low_pass[TANX_HEAD_FILTER_MAX];
+    float high_pass[TANX_HEAD_FILTER_MAX];
+} tanx_listener;
+
+/* generateSimpleHeadRelatedTransform (TrueAudioVR.cpp:361-415): fills
+ * filter_length (64), low_pass (head shadow, corner 343 / (1.1 * spacing)
+ * Hz) and high_pass = impulse - low_pass.  Host only. */
+int32_t tanx_room_head_model(float ear_spacing, tanx_listener *listener);
+/* generateRoomResponse with GENROOM_USE_GPU_MEM (TrueAudioVR.cpp:659-778;
+ * kernels GenerateRoomResponse.cl:126-325, Fill.cl) for `count` rooms,
+ * sources and listeners at once: response k's left ear is written to
+ * d_out + 2k * out_stride and its right ear to d_out + (2k + 1) *
+ * out_stride, response_length floats each (out_stride >= response_length),
+ * ready to feed tanx_conv_update_td_device.  Images span -n .. n per axis,
+ * n = 1 + response_length / (size / 340 * sample_rate) samples, capped at
+ * max_bounces with TANX_ROOM_LIMIT_BOUNCES; TANX_ROOM_SUPPRESS_DIRECT drops
+ * the direct path (as the reference's CPU path does).  Contributions are
+ * summed in 2^26 fixed point, so the result does not depend on the order of
+ * the sums; the sums are read as signed integers (the reference's Fill.cl
+ * reads them as unsigned).  AMF_INVALID_ARG for non-positive sizes, damping
+ * outside [0, 1] or a room smaller than one sample per axis;
+ * AMF_OUT_OF_RANGE for more than 2^31 - 1 images per ear.  Asynchronous on
+ * the context's process stream. */
+int32_t tanx_room_response_device(tanx_ctx *ctx, uint32_t count, const tanx_room *rooms, const tanx_source *sources,
+                                  const tanx_listener *listeners, int32_t sample_rate, int32_t response_length,
+                                  uint32_t flags, int32_t max_bounces, float *d_out, size_t out_stride);
+/* generateRoomResponse into host memory (the GPU path's read back,
+ * TrueAudioVR.cpp:1127-1160): returns with left and right written. */
+int32_t tanx_room_response(tanx_ctx *ctx, const tanx_room *room, const tanx_source *source,
+                           const tanx_listener *listener, int32_t sample_rate, int32_t response_length,
+                           uint32_t flags, int32_t max_bounces, float *left, float *right);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,11 @@
  *      (ConvolutionImpl.cpp:2868-2931) and Flush (ConvolutionImpl.cpp:1261-1325).
  *      Pinned by the FP64 direct convolution and by closed-form crossfade
  *      weights.
+ *   4. The device I/O chain (mixer, sample converter).
+ *   5. Room impulse responses by image sources (tan_oracle_room.c), pinned
+ *      by a textbook image-source model built from reflection geometry and
+ *      by closed forms (direct path alone, one-axis echo trains, the head
+ *      filters against numpy's inverse FFT).
  */
 #ifndef TAN_ORACLE_H
 #define TAN_ORACLE_H
@@ -134,6 +139,52 @@ void orc_mix(const float *const *in, int channels, size_t n, float *out);
  * out_offset].  Returns 0, or -1 for an unknown type. */
 int orc_convert(const void *in, size_t in_step, size_t in_offset, int in_type, void *out, size_t out_step,
                 size_t out_offset, int out_type, size_t n, float gain, int *clipped);
+
+/* 5. Room impulse responses (tan_oracle_room.c): TrueAudioVR's GPU path
+ * (generateRoomResponse host set-up, TrueAudioVR.cpp:659-778; kernels
+ * GenerateRoomResponse.cl:204-325 and Fill.cl) restated in single
+ * precision without contraction, and the head filters of
+ * generateSimpleHeadRelatedTransform (TrueAudioVR.cpp:361-415).  The
+ * structs have the layouts of RoomDefinition, MonoSource and
+ * StereoListener (TrueAudioVR.h:62-107).  Return values are AMF_RESULT
+ * codes (0 ok, 4 invalid argument, 5 out of range, 6 out of memory). */
+#define ORC_HEAD_FILTER 64
+#define ORC_ROOM_LIMIT_BOUNCES 0x40
+#define ORC_ROOM_SUPPRESS_DIRECT 0x80
+typedef struct orc_room {
+    float width, height, length;
+    float damp_left, damp_right, damp_top, damp_bottom, damp_front, damp_back;
+} orc_room;
+typedef struct orc_source {
+    float x, y, z;
+} orc_source;
+typedef struct orc_listener {
+    float head_x, head_y, head_z;
+    float ear_spacing;
+    float yaw, pitch, roll;
+    int32_t filter_length;
+    float low_pass[ORC_HEAD_FILTER];
+    float high_pass[ORC_HEAD_FILTER];
+} orc_listener;
+/* Kernel arguments of one ear (GenerateRoomResponse's parameter list). */
+typedef struct orc_room_ear {
+    float src[3], head[3], ev[3], ev_norm, dim[3];
+    float damp[6]; /* right, left, top, bottom, front, back */
+    float max_gain, d_min;
+    int fs, len, hrtf_len, filt_len;
+    int n[3];
+    int suppress_direct;
+    float lp[ORC_HEAD_FILTER], hp[ORC_HEAD_FILTER];
+} orc_room_ear;
+int orc_room_ear_params(const orc_room *room, const orc_source *source, const orc_listener *ears, int fs, int len,
+                        unsigned flags, int max_bounces, orc_room_ear out[2], long long *images);
+/* One image: 0 = no contribution, 1 = one tap at *index of value
+ * (int)*scale, 2 = *taps head-filtered taps from *index of value
+ * (int)(*scale * (lp[i] + *hf * hp[i])). */
+int orc_room_image(const orc_room_ear *ear, int ix, int iy, int iz, int *index, int *taps, float *scale, float *hf);
+int orc_room_response(const orc_room *room, const orc_source *source, const orc_listener *listener, int fs,
+                      int len, unsigned flags, int max_bounces, float *left, float *right);
+int orc_room_head_model(float ear_spacing, orc_listener *listener);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,9 @@ _SIGS = {
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),
     "tanx_fft_transform_device": (_i32, [_vp, _u32, _u32, _u32, _vp, _vp, _sz]),
     "tanx_version": (ctypes.c_char_p, []),
+    "tanx_room_head_model": (_i32, [ctypes.c_float, _vp]),
+    "tanx_room_response_device": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _i32, _u32, _i32, _vp, _sz]),
+    "tanx_room_response": (_i32, [_vp, _vp, _vp, _vp, _i32, _i32, _u32, _i32, _f32p, _f32p]),
     "tanx_wav_read_info": (_i32, [ctypes.c_char_p, ctypes.POINTER(WavInfo)]),
     "tanx_wav_read": (_i32, [ctypes.c_char_p, ctypes.POINTER(WavInfo), ctypes.POINTER(_f32p), _sz]),
     "tanx_wav_read_raw": (_i32, [ctypes.c_char_p, ctypes.POINTER(WavInfo), _vp, _sz]),

@@ -15,7 +15,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include")]
 
-SOURCES = ["tan_kernels.hip", "tan_engine.cpp", "tan_api.cpp", "tan_wav.cpp"]
+SOURCES = ["tan_kernels.hip", "tan_room.hip", "tan_engine.cpp", "tan_api.cpp", "tan_wav.cpp"]
 
 
 def build_library(verbose=False):

@@ -86,6 +86,18 @@ struct tanx_ctx {
     char *io_dev = nullptr;
     char *io_host = nullptr;
     size_t io_cap = 0;
+    // room responses: fixed-point accumulator (kept zero between calls by
+    // the fill kernel), per-ear arguments (pinned staging + device copy) and
+    // the host path's output buffers
+    std::mutex room_mtx;
+    unsigned *room_acc = nullptr;
+    size_t room_acc_ints = 0;
+    RoomEar *room_ears_dev = nullptr, *room_ears_host = nullptr;
+    size_t room_ears_cap = 0;
+    hipEvent_t room_ev = nullptr;  // staging copied to the device
+    bool room_ev_live = false;
+    float *room_out_dev = nullptr, *room_out_host = nullptr;
+    size_t room_out_cap = 0;
 };
 
 static int32_t ctx_upload_twiddles(tanx_ctx *ctx)
@@ -152,6 +164,12 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
         if (ctx->io_flag_host) (void)hipHostFree(ctx->io_flag_host);
         if (ctx->io_dev) (void)hipFree(ctx->io_dev);
         if (ctx->io_host) (void)hipHostFree(ctx->io_host);
+        if (ctx->room_acc) (void)hipFree(ctx->room_acc);
+        if (ctx->room_ears_dev) (void)hipFree(ctx->room_ears_dev);
+        if (ctx->room_ears_host) (void)hipHostFree(ctx->room_ears_host);
+        if (ctx->room_ev) (void)hipEventDestroy(ctx->room_ev);
+        if (ctx->room_out_dev) (void)hipFree(ctx->room_out_dev);
+        if (ctx->room_out_host) (void)hipHostFree(ctx->room_out_host);
     }
     delete ctx;
     return TANX_OK;
@@ -2197,5 +2215,107 @@ extern "C" int32_t tanx_dft_r2c_mfma_device(tanx_ctx *ctx, uint32_t log2len, uin
     if (rc != TANX_OK) return rc;
     TANX_CHECK(launch_dft_mfma(d_in, (long long)in_stride, d_out, (long long)out_stride, (int)channels, n,
                                ctx->dft_hi, ctx->dft_lo, (int)split, ctx->proc));
+    return TANX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// IR sources: room responses by image sources
+// ---------------------------------------------------------------------------
+
+// Derives the ears of `count` responses, grows the context's buffers and
+// launches; the caller holds room_mtx and the device guard.
+static int32_t room_launch(tanx_ctx *ctx, uint32_t count, const tanx_room *rooms, const tanx_source *sources,
+                           const tanx_listener *listeners, int32_t fs, int32_t len, uint32_t flags, int32_t max_bounces,
+                           float *d_out, size_t out_stride)
+{
+    const size_t nears = 2 * (size_t)count;
+    std::vector<RoomEar> ears(nears);
+    long long max_images = 0;
+    for (uint32_t k = 0; k < count; k++) {
+        long long images = 0;
+        const int32_t rc = room_ears(&rooms[k], &sources[k], &listeners[k], fs, len, flags, max_bounces,
+                                     &ears[2 * k], &images);
+        if (rc != TANX_OK) return rc;
+        max_images = std::max(max_images, images);
+    }
+    const size_t ints = nears * (size_t)len;
+    if (ints > ctx->room_acc_ints || nears > ctx->room_ears_cap) {
+        // buffers in use by earlier calls on the stream: let them finish
+        TANX_CHECK(hipStreamSynchronize(ctx->proc));
+    }
+    if (ints > ctx->room_acc_ints) {
+        if (ctx->room_acc) (void)hipFree(ctx->room_acc);
+        ctx->room_acc = nullptr;
+        ctx->room_acc_ints = 0;
+        TANX_CHECK(hipMalloc(&ctx->room_acc, ints * sizeof(unsigned)));
+        ctx->room_acc_ints = ints;
+        TANX_CHECK(launch_zero(ctx->room_acc, ints * sizeof(unsigned), ctx->proc));
+    }
+    if (nears > ctx->room_ears_cap) {
+        if (ctx->room_ears_dev) (void)hipFree(ctx->room_ears_dev);
+        if (ctx->room_ears_host) (void)hipHostFree(ctx->room_ears_host);
+        ctx->room_ears_dev = ctx->room_ears_host = nullptr;
+        ctx->room_ears_cap = 0;
+        ctx->room_ev_live = false;
+        TANX_CHECK(hipMalloc(&ctx->room_ears_dev, nears * sizeof(RoomEar)));
+        TANX_CHECK(hipHostMalloc(&ctx->room_ears_host, nears * sizeof(RoomEar), hipHostMallocDefault));
+        ctx->room_ears_cap = nears;
+    }
+    if (!ctx->room_ev) TANX_CHECK(hipEventCreateWithFlags(&ctx->room_ev, hipEventDisableTiming));
+    // the previous call's staging copy must be done before it is rewritten
+    if (ctx->room_ev_live) TANX_CHECK(hipEventSynchronize(ctx->room_ev));
+    std::memcpy(ctx->room_ears_host, ears.data(), nears * sizeof(RoomEar));
+    TANX_CHECK(hipMemcpyAsync(ctx->room_ears_dev, ctx->room_ears_host, nears * sizeof(RoomEar),
+                              hipMemcpyHostToDevice, ctx->proc));
+    TANX_CHECK(hipEventRecord(ctx->room_ev, ctx->proc));
+    ctx->room_ev_live = true;
+    TANX_CHECK(launch_room(ctx->room_ears_dev, (int)nears, max_images, len, ctx->room_acc, d_out,
+                           (long long)out_stride, ctx->proc));
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_room_response_device(tanx_ctx *ctx, uint32_t count, const tanx_room *rooms,
+                                             const tanx_source *sources, const tanx_listener *listeners,
+                                             int32_t sample_rate, int32_t response_length, uint32_t flags,
+                                             int32_t max_bounces, float *d_out, size_t out_stride)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    if (!rooms || !sources || !listeners || !d_out) return TANX_INVALID_POINTER;
+    if (count == 0 || count > 32767 || response_length <= 0 || out_stride < (size_t)response_length)
+        return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->room_mtx);
+    DeviceGuard g(ctx->device);
+    return room_launch(ctx, count, rooms, sources, listeners, sample_rate, response_length, flags, max_bounces, d_out,
+                       out_stride);
+}
+
+extern "C" int32_t tanx_room_response(tanx_ctx *ctx, const tanx_room *room, const tanx_source *source,
+                                      const tanx_listener *listener, int32_t sample_rate, int32_t response_length,
+                                      uint32_t flags, int32_t max_bounces, float *left, float *right)
+{
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    if (!room || !source || !listener || !left || !right) return TANX_INVALID_POINTER;
+    if (response_length <= 0) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->room_mtx);
+    DeviceGuard g(ctx->device);
+    const size_t need = 2 * (size_t)response_length;
+    if (need > ctx->room_out_cap) {
+        TANX_CHECK(hipStreamSynchronize(ctx->proc));
+        if (ctx->room_out_dev) (void)hipFree(ctx->room_out_dev);
+        if (ctx->room_out_host) (void)hipHostFree(ctx->room_out_host);
+        ctx->room_out_dev = ctx->room_out_host = nullptr;
+        ctx->room_out_cap = 0;
+        TANX_CHECK(hipMalloc(&ctx->room_out_dev, need * sizeof(float)));
+        TANX_CHECK(hipHostMalloc(&ctx->room_out_host, need * sizeof(float), hipHostMallocDefault));
+        ctx->room_out_cap = need;
+    }
+    int32_t rc = room_launch(ctx, 1, room, source, listener, sample_rate, response_length, flags, max_bounces,
+                             ctx->room_out_dev, (size_t)response_length);
+    if (rc != TANX_OK) return rc;
+    TANX_CHECK(hipMemcpyAsync(ctx->room_out_host, ctx->room_out_dev, need * sizeof(float), hipMemcpyDeviceToHost,
+                              ctx->proc));
+    TANX_CHECK(hipStreamSynchronize(ctx->proc));
+    std::memcpy(left, ctx->room_out_host, (size_t)response_length * sizeof(float));
+    std::memcpy(right, ctx->room_out_host + response_length, (size_t)response_length * sizeof(float));
     return TANX_OK;
 }

@@ -5,6 +5,11 @@
 #include <cstddef>
 #include <cstdint>
 
+// public parameter structs of the room-response entry points (tan_mi355x.h)
+struct tanx_room;
+struct tanx_source;
+struct tanx_listener;
+
 namespace tanx {
 
 // Spectra are stored "packed": a 2B-point real FFT of a frame of 2B reals
@@ -208,6 +213,34 @@ hipError_t launch_convert(const void *in, long long in_step, long long in_off, i
                           long long out_step, long long out_off, int out_type, long long n, float gain, int *d_flag,
                           hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
+
+// Room responses by image sources (GenerateRoomResponse.cl).  One RoomEar
+// per response row: the kernel arguments the reference's host code derives
+// per ear (TrueAudioVR.cpp:659-778).  Images ix in [-n[0], n[0]] (x: width),
+// iy in [-n[1], n[1]] (y: height), iz in [-n[2], n[2]] (z: length).
+constexpr int kHeadFilterMax = 64;      // SIMPLEHRTF_FFTLEN
+constexpr float kRoomFixedOne = 67108864.0f;  // 2^26: one in the fixed-point accumulator
+struct RoomEar {
+    float src[3];      // source position, clamped into the room
+    float head[3];     // ear position
+    float ev[3];       // ear direction
+    float ev_norm;     // |ev|
+    float dim[3];      // width, height, length
+    float damp[6];     // right, left, top, bottom, front, back
+    float max_gain, d_min;
+    int fs, len, hrtf_len, filt_len;
+    int n[3];
+    int suppress_direct;
+    float lp[kHeadFilterMax], hp[kHeadFilterMax];
+};
+// RoomEar of the left (out[0]) and right (out[1]) ear; AMF_RESULT value.
+int32_t room_ears(const tanx_room *room, const tanx_source *source, const tanx_listener *ears, int fs, int len,
+                  uint32_t flags, int max_bounces, RoomEar out[2], long long *images);
+// Accumulates the images of ears[e] into acc[e * len ...] (wrapping 32-bit
+// integer sums of the fixed-point contributions), then converts each row to
+// float into out + e * out_stride and clears it.
+hipError_t launch_room(const RoomEar *d_ears, int nears, long long max_images, int len, unsigned *acc, float *out,
+                       long long out_stride, hipStream_t s);
 
 // Twiddle tables: tw + (2^l - 1) holds W_{2^l}^t = exp(-2 pi i t / 2^l), t < 2^l,
 // for l = 0..kMaxTwLog2.

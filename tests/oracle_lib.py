@@ -52,6 +52,15 @@ def _load():
     lib.orc_convert.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t, ctypes.c_float,
                                 ctypes.POINTER(ctypes.c_int)]
+    lib.orc_room_ear_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_longlong)]
+    lib.orc_room_image.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    lib.orc_room_response.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_uint, ctypes.c_int, _f32p, _f32p]
+    lib.orc_room_head_model.argtypes = [ctypes.c_float, ctypes.c_void_p]
     lib.orc_bench_upols.restype = ctypes.c_double
     lib.orc_bench_upols.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_uint64, _f64p]
@@ -175,3 +184,47 @@ def convert(inp, in_step, in_offset, out, out_step, out_offset, n, gain):
                          t[out.dtype], n, gain, ctypes.byref(clipped))
     assert rc == 0
     return bool(clipped.value)
+
+
+# ---- 5. room responses (tan_oracle_room.c) ----
+
+class RoomEar(ctypes.Structure):
+    """orc_room_ear: the kernel arguments of one ear."""
+    _fields_ = [("src", ctypes.c_float * 3), ("head", ctypes.c_float * 3), ("ev", ctypes.c_float * 3),
+                ("ev_norm", ctypes.c_float), ("dim", ctypes.c_float * 3), ("damp", ctypes.c_float * 6),
+                ("max_gain", ctypes.c_float), ("d_min", ctypes.c_float), ("fs", ctypes.c_int),
+                ("len", ctypes.c_int), ("hrtf_len", ctypes.c_int), ("filt_len", ctypes.c_int),
+                ("n", ctypes.c_int * 3), ("suppress_direct", ctypes.c_int),
+                ("lp", ctypes.c_float * 64), ("hp", ctypes.c_float * 64)]
+
+
+def room_head_model(ear_spacing, listener):
+    """Fills the listener struct's head filters (any struct with the
+    StereoListener layout)."""
+    return lib.orc_room_head_model(ear_spacing, ctypes.byref(listener))
+
+
+def room_ear_params(room, source, listener, fs, length, flags=0, max_bounces=0):
+    ears = (RoomEar * 2)()
+    images = ctypes.c_longlong(0)
+    rc = lib.orc_room_ear_params(ctypes.byref(room), ctypes.byref(source), ctypes.byref(listener), fs, length,
+                                 flags, max_bounces, ears, ctypes.byref(images))
+    return rc, ears, images.value
+
+
+def room_image(ear, ix, iy, iz):
+    """(kind, index, taps, scale, hf): kind 0 none, 1 one tap, 2 head-filtered."""
+    idx, taps = ctypes.c_int(), ctypes.c_int()
+    sc, hf = ctypes.c_float(), ctypes.c_float()
+    kind = lib.orc_room_image(ctypes.byref(ear), ix, iy, iz, ctypes.byref(idx), ctypes.byref(taps),
+                              ctypes.byref(sc), ctypes.byref(hf))
+    return kind, idx.value, taps.value, sc.value, hf.value
+
+
+def room_response(room, source, listener, fs, length, flags=0, max_bounces=0):
+    """(rc, left, right) float32."""
+    left = np.zeros(length, np.float32)
+    right = np.zeros(length, np.float32)
+    rc = lib.orc_room_response(ctypes.byref(room), ctypes.byref(source), ctypes.byref(listener), fs, length,
+                               flags, max_bounces, left.ctypes.data_as(_f32p), right.ctypes.data_as(_f32p))
+    return rc, left, right
