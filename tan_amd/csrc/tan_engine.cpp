@@ -268,7 +268,13 @@ struct tanx_conv {
     int run_ring = 0;
 
     hipEvent_t ev_upd = nullptr;   // last IR update finished
-    hipEvent_t ev_proc = nullptr;  // last process work enqueued
+    // Process work up to the latest crossfade: every launch that reads an IR
+    // slot other than the current one precedes it (crossfades read the
+    // previous and update slots, the blocks before a switch the armed one),
+    // so an update's transform, which writes a slot the current filter does
+    // not use, waits only for this point, not for all queued process work.
+    hipEvent_t ev_xf = nullptr;
+    bool xf_mark = false;  // a crossfade block was prepared: record ev_xf after its launch
     hipEvent_t ev_out = nullptr;   // host-path output copied
 
     // IR switch state (ConvolutionImpl.cpp:115-117, 869-927)
@@ -355,7 +361,7 @@ static void conv_free(tanx_conv *cv)
         if (cv->hrun) (void)hipHostFree(cv->hrun);
         for (auto &e : cv->run_ev) if (e) (void)hipEventDestroy(e);
         if (cv->ev_upd) (void)hipEventDestroy(cv->ev_upd);
-        if (cv->ev_proc) (void)hipEventDestroy(cv->ev_proc);
+        if (cv->ev_xf) (void)hipEventDestroy(cv->ev_xf);
         if (cv->ev_out) (void)hipEventDestroy(cv->ev_out);
         cv->t_head.destroy();
         cv->t_tail.destroy();
@@ -644,7 +650,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     for (auto &ev : cv->desc_ev)
         if (hipEventRecord(ev, ctx->proc) != hipSuccess) return fail(TANX_UNEXPECTED);
     if (hipEventCreateWithFlags(&cv->ev_upd, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&cv->ev_proc, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&cv->ev_xf, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&cv->ev_out, hipEventDisableTiming) != hipSuccess)
         return fail(TANX_UNEXPECTED);
 
@@ -656,7 +662,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
         e = hipMemcpyAsync(cv->drun, cv->hrun, C, hipMemcpyHostToDevice, ctx->proc);
     }
     if (e == hipSuccess) e = hipEventRecord(cv->ev_upd, ctx->upd);
-    if (e == hipSuccess) e = hipEventRecord(cv->ev_proc, ctx->proc);
+    if (e == hipSuccess) e = hipEventRecord(cv->ev_xf, ctx->proc);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->proc);
     if (e != hipSuccess) return fail(map_hip(e));
 
@@ -834,6 +840,15 @@ static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const T
     return TANX_OK;
 }
 
+// After the launch of a prepared crossfade block: ev_xf marks it.
+static int32_t mark_xfade(tanx_conv *cv)
+{
+    if (!cv->xf_mark) return TANX_OK;
+    TANX_CHECK(hipEventRecord(cv->ev_xf, cv->ctx->proc));
+    cv->xf_mark = false;
+    return TANX_OK;
+}
+
 // algorithmic bytes of the roles (regular blocks count one filter)
 static uint64_t head_bytes(const tanx_conv *cv, int nf)
 {
@@ -906,6 +921,7 @@ static bool advance_switch(tanx_conv *cv, uint64_t b)
         cv->prev = cv->cur;
         cv->cur = (cv->cur + 1) % 3;
         cv->upd = (cv->upd + 1) % 3;
+        cv->xf_mark = true;
     }
     return xfade;
 }
@@ -1094,6 +1110,8 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
     } else {
         TANX_CHECK(launch_td(t, s));
     }
+    rc = mark_xfade(cv);
+    if (rc != TANX_OK) return rc;
     cv->tdpos = (cv->tdpos + cv->B) & (cv->len - 1);
     cv->crossfading = xfade;
     cv->last_xfade = xfade;
@@ -1132,7 +1150,7 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
         rc = launch_timed(cv, cv->t_head, 1, a, hb, cv->head_alg_bytes, fp);
         if (rc != TANX_OK) return rc;
     }
-    return TANX_OK;
+    return mark_xfade(cv);
 }
 
 // Instantiated graph of k descriptor-driven tick launches (one per block).
@@ -1195,6 +1213,8 @@ static int32_t process_device_graph(tanx_conv *cv, const float *d_in, size_t in_
         TANX_CHECK(hipMemcpyAsync(cv->d_desc, h, sizeof(TickArgs) * k, hipMemcpyHostToDevice, s));
         TANX_CHECK(hipEventRecord(cv->desc_ev[slot], s));
         TANX_CHECK(hipGraphLaunch(exec, s));
+        int32_t mrc = mark_xfade(cv);
+        if (mrc != TANX_OK) return mrc;
         b += k;
         if (processed) *processed = b * B;
     }
@@ -1394,9 +1414,9 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         }
         cv->hchan[nupd++] = ch;
     }
-    // the update slot must not be overwritten while queued process work reads it
-    TANX_CHECK(hipEventRecord(cv->ev_proc, ctx->proc));
-    TANX_CHECK(hipStreamWaitEvent(ctx->upd, cv->ev_proc, 0));
+    // the target slot must not be overwritten while queued process work
+    // reads it: its last reader is at or before the latest crossfade
+    TANX_CHECK(hipStreamWaitEvent(ctx->upd, cv->ev_xf, 0));
     const size_t chanBytes = (size_t)cv->P * cv->row_bytes();
     if (nupd > 0) {
         const float *src = dir;
@@ -1601,6 +1621,10 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
                 irtype, reinterpret_cast<const FrameArgs *>(ctx->g_dev + desc_bytes + start_bytes),
                 reinterpret_cast<const int *>(ctx->g_dev + desc_bytes + start_bytes + fdesc_bytes), nframe,
                 (unsigned)ftotal, ctx->proc));
+        for (uint32_t i = 0; i < done; i++) {
+            const int32_t mrc = mark_xfade(convs[i]);
+            if (mrc != TANX_OK) return mrc;
+        }
     } else {
         TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
     }
@@ -2230,13 +2254,20 @@ static int32_t room_launch(tanx_ctx *ctx, uint32_t count, const tanx_room *rooms
 {
     const size_t nears = 2 * (size_t)count;
     std::vector<RoomEar> ears(nears);
-    long long max_images = 0;
+    long long max_early = 0;
+    int max_windows = 0;
     for (uint32_t k = 0; k < count; k++) {
         long long images = 0;
         const int32_t rc = room_ears(&rooms[k], &sources[k], &listeners[k], fs, len, flags, max_bounces,
                                      &ears[2 * k], &images);
         if (rc != TANX_OK) return rc;
-        max_images = std::max(max_images, images);
+        for (int e = 0; e < 2; e++) {
+            const RoomEar &E = ears[2 * k + e];
+            long long box = 1;
+            for (int a = 0; a < 3; a++) box *= E.hi[a] >= E.lo[a] ? (long long)(E.hi[a] - E.lo[a] + 1) : 0;
+            max_early = std::max(max_early, box);
+            max_windows = std::max(max_windows, (len - E.wstart + kRoomWindow - 1) / kRoomWindow);
+        }
     }
     const size_t ints = nears * (size_t)len;
     if (ints > ctx->room_acc_ints || nears > ctx->room_ears_cap) {
@@ -2269,7 +2300,7 @@ static int32_t room_launch(tanx_ctx *ctx, uint32_t count, const tanx_room *rooms
                               hipMemcpyHostToDevice, ctx->proc));
     TANX_CHECK(hipEventRecord(ctx->room_ev, ctx->proc));
     ctx->room_ev_live = true;
-    TANX_CHECK(launch_room(ctx->room_ears_dev, (int)nears, max_images, len, ctx->room_acc, d_out,
+    TANX_CHECK(launch_room(ctx->room_ears_dev, (int)nears, max_early, max_windows, len, ctx->room_acc, d_out,
                            (long long)out_stride, ctx->proc));
     return TANX_OK;
 }

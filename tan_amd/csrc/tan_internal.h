@@ -232,15 +232,23 @@ struct RoomEar {
     int n[3];
     int suppress_direct;
     float lp[kHeadFilterMax], hp[kHeadFilterMax];
+    // the box of image indices that can arrive before hrtf_len (the early,
+    // head-filtered images): lo[k] .. hi[k] per axis
+    int lo[3], hi[3];
+    int wstart;        // first output sample summed by windows: min(hrtf_len, len)
 };
+constexpr int kRoomWindow = 2048;  // output samples per late-image window (8 KB of LDS)
 // RoomEar of the left (out[0]) and right (out[1]) ear; AMF_RESULT value.
 int32_t room_ears(const tanx_room *room, const tanx_source *source, const tanx_listener *ears, int fs, int len,
                   uint32_t flags, int max_bounces, RoomEar out[2], long long *images);
-// Accumulates the images of ears[e] into acc[e * len ...] (wrapping 32-bit
-// integer sums of the fixed-point contributions), then converts each row to
-// float into out + e * out_stride and clears it.
-hipError_t launch_room(const RoomEar *d_ears, int nears, long long max_images, int len, unsigned *acc, float *out,
-                       long long out_stride, hipStream_t s);
+// Sums the images of ears[e] (wrapping 32-bit integer sums of the
+// fixed-point contributions) into row e of out (out + e * out_stride, len
+// floats).  Early images go into acc[e * len ...] with global atomics, late
+// ones through per-window LDS sums added into acc; acc is left cleared.
+// max_early: the largest early-box image count over the ears; max_windows:
+// the largest number of late windows.
+hipError_t launch_room(const RoomEar *d_ears, int nears, long long max_early, int max_windows, int len,
+                       unsigned *acc, float *out, long long out_stride, hipStream_t s);
 
 // Twiddle tables: tw + (2^l - 1) holds W_{2^l}^t = exp(-2 pi i t / 2^l), t < 2^l,
 // for l = 0..kMaxTwLog2.

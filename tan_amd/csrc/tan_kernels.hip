@@ -965,17 +965,22 @@ __global__ void k_td_load(const float *__restrict__ ir, long long stride, long l
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(64) void k_dft_mfma(const float *__restrict__ x, long long in_stride,
-                                                 float *__restrict__ y, long long out_stride, int C, int n, int cols,
-                                                 const bf16x8_t *__restrict__ fh, const bf16x8_t *__restrict__ fl,
-                                                 int split)
+constexpr int kDftWaves = 4;  // waves per workgroup, each over a quarter of the samples
+
+__global__ __launch_bounds__(64 * kDftWaves) void k_dft_mfma(const float *__restrict__ x, long long in_stride,
+                                                             float *__restrict__ y, long long out_stride, int C,
+                                                             int n, int cols, const bf16x8_t *__restrict__ fh,
+                                                             const bf16x8_t *__restrict__ fl, int split)
 {
-    // one wave per workgroup: a 32-channel x 32-column tile as 2 x 2 MFMA tiles,
-    // so each k-step's operand fragments feed four (or twelve) products
-    const int lane = threadIdx.x;
+    // a 32-channel x 32-column tile as 2 x 2 MFMA tiles per wave, so each
+    // k-step's operand fragments feed four (or twelve) products; the waves
+    // split the sum over t and their partial tiles are added in LDS in wave
+    // order (the result does not depend on scheduling)
+    __shared__ float part[kDftWaves][16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int tiles_c = (cols + 31) / 32;
     const int c0 = (blockIdx.x / tiles_c) * 32, col0 = (blockIdx.x % tiles_c) * 32;
-    if (c0 >= C) return;
+    if (c0 >= C) return;  // uniform over the workgroup
     const int i = lane & 15, kq = lane >> 4;
     const float *xr[2];
     int colr[2];
@@ -990,8 +995,10 @@ __global__ __launch_bounds__(64) void k_dft_mfma(const float *__restrict__ x, lo
 #pragma unroll
         for (int v = 0; v < 2; v++) acc[u][v] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
     const int kv = n / 8;  // bf16x8 vectors per table row
-    for (int k0 = 0; k0 < n; k0 += 32) {
-        const int kb = k0 + 8 * kq;
+    const int steps = n / 32, per = (steps + kDftWaves - 1) / kDftWaves;
+    const int s0 = w * per, s1 = min(steps, s0 + per);
+    for (int st = s0; st < s1; st++) {
+        const int kb = 32 * st + 8 * kq;
         bf16x8_t ah[2], al[2], bh[2], bl[2];
 #pragma unroll
         for (int u = 0; u < 2; u++) {
@@ -1024,14 +1031,20 @@ __global__ __launch_bounds__(64) void k_dft_mfma(const float *__restrict__ x, lo
 #pragma unroll
     for (int u = 0; u < 2; u++)
 #pragma unroll
-        for (int v = 0; v < 2; v++) {
-            const int col = col0 + 16 * v + i;
+        for (int v = 0; v < 2; v++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = c0 + 16 * u + 4 * kq + r;
-                if (row < C && col < cols) y[(size_t)row * out_stride + col] = acc[u][v][r];
-            }
-        }
+            for (int r = 0; r < 4; r++) part[w][(u * 2 + v) * 4 + r][lane] = acc[u][v][r];
+    __syncthreads();
+    // 1,024 outputs, four per thread: element e = (u, v, r) of lane l
+    for (int o = threadIdx.x; o < 16 * 64; o += 64 * kDftWaves) {
+        const int e = o >> 6, l = o & 63;
+        float sum = part[0][e][l];
+#pragma unroll
+        for (int ww = 1; ww < kDftWaves; ww++) sum += part[ww][e][l];
+        const int u = e >> 3, v = (e >> 2) & 1, r = e & 3;
+        const int row = c0 + 16 * u + 4 * (l >> 4) + r, col = col0 + 16 * v + (l & 15);
+        if (row < C && col < cols) y[(size_t)row * out_stride + col] = sum;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1271,7 +1284,7 @@ hipError_t launch_dft_mfma(const float *x, long long in_stride, float *y, long l
     if (n < 32 || (n & 31) || (split != 1 && split != 3)) return hipErrorInvalidValue;
     const int cols = n + 2;
     const unsigned blocks = (unsigned)(((C + 31) / 32) * ((cols + 31) / 32));
-    hipLaunchKernelGGL(k_dft_mfma, dim3(blocks), dim3(64), 0, s, x, in_stride, y, out_stride, C, n, cols,
+    hipLaunchKernelGGL(k_dft_mfma, dim3(blocks), dim3(64 * kDftWaves), 0, s, x, in_stride, y, out_stride, C, n, cols,
                        static_cast<const bf16x8_t *>(fh), static_cast<const bf16x8_t *>(fl), split);
     return hipGetLastError();
 }
