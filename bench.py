@@ -472,8 +472,9 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
 
 def bench_dft_mfma(T, torch, ctx, channels=1024, log2n=9, reps=50):
     """The block DFT on the matrix cores (precision track): 1,024 channels x
-    512-sample frames (the 2B-point transform at B = 256), plain bf16 and
-    split-bf16, against the FP32 FFT (TANFFT R2C) on the same frames."""
+    2^log2n-sample frames (256: the block; 512: the 2B-point transform at
+    B = 256), plain bf16 and split-bf16, against the FP32 FFT (TANFFT R2C) on
+    the same frames."""
     n = 1 << log2n
     fft = T.TANFFT(ctx)
     fft.Init()
@@ -517,7 +518,7 @@ def bench_dft_mfma(T, torch, ctx, channels=1024, log2n=9, reps=50):
                      "peak_tflops": 2500.0, "frac": tf / 2500.0, "fft_fp32_us": us_fft,
                      "snr_vs_fp64_db": snr(refri, got), "snr_vs_fp32_fft_db": snr(fp32, got),
                      "fft_fp32_snr_vs_fp64_db": snr(refri, fp32)})
-        log(f"  mfma dft split {split}: {us:.1f} us ({tf:.1f} TFLOP/s), FFT {us_fft:.1f} us, "
+        log(f"  mfma dft n={n} split {split}: {us:.1f} us ({tf:.1f} TFLOP/s), FFT {us_fft:.1f} us, "
             f"SNR {rows[-1]['snr_vs_fp64_db']:.1f} dB vs FP64")
     del x, yf
     return rows
@@ -696,7 +697,7 @@ def main():
     ap.add_argument("--no-nonuniform", action="store_true")
     ap.add_argument("--no-room", action="store_true")
     ap.add_argument("--serving-methods", type=str, default="uniform,nonuniform")
-    ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304,131072,163840")
+    ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304,131072")
     ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
@@ -751,7 +752,8 @@ def main():
     if not args.no_precision:
         log("precision pass")
         prec_rows = bench_precision(T, torch, ctx)
-        dft_rows = bench_dft_mfma(T, torch, ctx)
+        # 256 points (the configuration's block DFT) and 512 (the head's 2B-point frame)
+        dft_rows = bench_dft_mfma(T, torch, ctx, log2n=8) + bench_dft_mfma(T, torch, ctx, log2n=9)
 
     nonuniform = None
     if not args.no_nonuniform:
