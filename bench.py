@@ -183,8 +183,10 @@ def bench_sweep(T, torch, ctx, counts, ticks=96, method=UNIFORM_METHOD):
         torch.cuda.empty_cache()
         lat = np.array(lat)
         p99 = float(np.percentile(lat, 99))
-        tick_us = kt["head_ms"] / max(1, kt["head_launches"]) * 1e3
-        gbps = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9 if kt["head_ms"] > 0 else 0.0
+        # every launch of the device path (with passes: pass and head-only launches)
+        ms_all = kt["head_ms"] + kt["tail_ms"]
+        tick_us = ms_all / max(1, kt["head_launches"] + kt["tail_launches"]) * 1e3
+        gbps = (kt["head_alg_bytes"] + kt["tail_alg_bytes"]) / (ms_all * 1e-3) / 1e9 if ms_all > 0 else 0.0
         row = {"channels": ch, "method": "nonuniform" if method == NONUNIFORM_METHOD else "uniform",
                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": p99,
                "max_ms": float(lat.max()), "tick_kernel_us": tick_us,
@@ -595,7 +597,7 @@ def cpu_room_baseline():
             "sample": f"one {ROOM_LEN}-tap response (both ears) in the same room, scalar C"}
 
 
-def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1, 0>"):
+def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1, 0, 1>"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_k_tick.json")
@@ -724,20 +726,39 @@ def main():
 
     log("kernel timing pass")
     dt2, kt, info2, _ = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=True)
-    # device-memory Process runs one k_tick launch per block: the block's
-    # head and the next block's tail multiply-accumulate
-    tick_us = kt["head_ms"] / kt["head_launches"] * 1e3
-    achieved = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9
-    # 64 head blocks + 64 x 16 tail blocks of 256 threads
-    traffic = pmc_traffic((C + C * info2["tail_chunks"]) * 256)
+    kp = info2.get("pass_blocks", 1)
+    if kp > 1:
+        # one launch per block: every kp-th runs the block's head and a pass
+        # computing the next kp blocks' partial sums (the dominant kernel,
+        # k_tick<256, 1, 0, 2>); the others run the head alone
+        tick_us = kt["tail_ms"] / kt["tail_launches"] * 1e3
+        achieved = kt["tail_alg_bytes"] / (kt["tail_ms"] * 1e-3) / 1e9
+        alg_per_launch = kt["tail_alg_bytes"] / kt["tail_launches"]
+        launches = kt["tail_launches"]
+        kernel = f"k_tick (head + {kp}-block pass)"
+        head_only_us = kt["head_ms"] / max(1, kt["head_launches"]) * 1e3
+        prefix = "void tanx::k_tick<256, 1, 0, 2>"
+    else:
+        # one k_tick launch per block: the block's head and the next block's tail
+        tick_us = kt["head_ms"] / kt["head_launches"] * 1e3
+        achieved = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9
+        alg_per_launch = kt["head_alg_bytes"] / kt["head_launches"]
+        launches = kt["head_launches"]
+        kernel = "k_tick (head + next tail)"
+        head_only_us = None
+        prefix = "void tanx::k_tick<256, 1, 0, 1>"
+    per_block_us = (kt["head_ms"] + kt["tail_ms"]) * 1e3 / (kt["head_launches"] + kt["tail_launches"])
+    # the launch's grid: 64 head blocks + 64 x chunks tail blocks of 256 threads
+    traffic = pmc_traffic((C + C * info2["tail_chunks"]) * 256, prefix)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": "k_tick (head + next tail)",
+                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": kernel,
                 "traffic_source": "profiles/pmc_traffic_k_tick.json" if traffic else None,
-                "alg_bytes_per_launch": kt["head_alg_bytes"] / kt["head_launches"],
-                "avg_launch_us": tick_us, "launches": kt["head_launches"],
+                "alg_bytes_per_launch": alg_per_launch, "avg_launch_us": tick_us, "launches": launches,
+                "pass_blocks": kp, "head_only_launch_us": head_only_us, "avg_block_us": per_block_us,
                 "note": ("64-channel working set (~67 MB) is resident in the 256 MB Infinity "
                          "Cache; see sweep for HBM-bound channel counts")}
-    log(f"  tick kernel {tick_us:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
+    log(f"  tick kernel {tick_us:.2f} us/launch, {achieved:.0f} GB/s algorithmic; "
+        f"{per_block_us:.2f} us per block on average")
 
     log("latency pass")
     lat = bench_latency(T, ctx, args.latency_blocks)

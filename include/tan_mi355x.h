@@ -96,6 +96,11 @@ int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
  * timing events against it (the reference's counterpart is
  * TANContext::GetOpenCLConvQueue, TrueAudioNext.h:596). */
 void *tanx_ctx_process_stream(tanx_ctx *ctx);
+/* The context's update stream (a hipStream_t), on which IR updates read the
+ * caller's IR and transform it (TANContext::GetOpenCLGeneralQueue,
+ * TrueAudioNext.h:597).  A caller that writes a device IR on its own stream
+ * orders this stream after that write before UpdateResponseTD. */
+void *tanx_ctx_update_stream(tanx_ctx *ctx);
 
 /* ---- convolution ----------------------------------------------------------- */
 
@@ -130,7 +135,12 @@ typedef struct tanx_conv_options {
                                the first frame ends at block M-1-(phase % M).
                                Objects of one group with different phases
                                spread their frame work over the ticks. */
-    uint32_t reserved[5];   /* zero */
+    uint32_t pass_blocks;   /* uniform objects: blocks whose partial sums one
+                               tail pass computes (1..4; 1 = a tail every
+                               block; 0 = chosen by the library: 4 when the
+                               IR has 16 or more partitions and one pass
+                               streams at most 4.5 GB, else 1) */
+    uint32_t reserved[4];   /* zero */
 } tanx_conv_options;
 /* tanx_conv_create with options (no counterpart in the reference, which
  * stores FP32 spectra only); opt may be NULL. */
@@ -214,6 +224,7 @@ typedef struct tanx_conv_info {
     uint64_t crossfades;   /* crossfade blocks processed */
     uint32_t ir_precision; /* TANX_IR_* */
     uint32_t nu_multiple;  /* blocks per frame of non-uniform objects, 1 = uniform */
+    uint32_t pass_blocks;  /* blocks per tail pass (1 = a tail every block) */
 } tanx_conv_info;
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */
@@ -236,10 +247,13 @@ int32_t tanx_conv_set_graph(tanx_conv *conv, uint32_t use_graph);
 /* Kernel timing: when enabled, HIP events are recorded on the process
  * stream around every launch.  "head" counts launches that carry the head
  * role (with the next block's tail when fused); "tail" counts separate tail
- * launches.  read_timing synchronizes and returns summed durations and the
- * algorithmic bytes of the timed launches (tail role: FDL + IR rows of
- * partitions 2..P-1 per channel and filter; head role: x_{t-1}, x_t, y, the
- * new and previous FDL rows, partitions 0 and 1 of each filter). */
+ * launches, and with multi-block passes (tanx_conv_info.pass_blocks > 1) the
+ * fused launches that run a pass.  read_timing synchronizes and returns
+ * summed durations and the algorithmic bytes of the timed launches (tail
+ * role or pass: FDL + IR rows of partitions 2..P-1 per channel and filter;
+ * head role: x_{t-1}, x_t, y, the new and previous FDL rows, partitions 0
+ * and 1 of each filter, and with passes the FDL + IR rows of the further
+ * partitions the head adds). */
 typedef struct tanx_kernel_timing {
     double head_ms;
     double tail_ms;

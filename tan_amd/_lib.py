@@ -23,12 +23,13 @@ class ConvInfo(ctypes.Structure):
                 ("tail_chunks", ctypes.c_uint32), ("finalize_mode", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64),
                 ("crossfades", ctypes.c_uint64), ("ir_precision", ctypes.c_uint32),
-                ("nu_multiple", ctypes.c_uint32)]
+                ("nu_multiple", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32)]
 
 
 class ConvOptions(ctypes.Structure):
     _fields_ = [("ir_precision", ctypes.c_uint32), ("nu_multiple", ctypes.c_uint32),
-                ("nu_phase", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+                ("nu_phase", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class WavInfo(ctypes.Structure):
@@ -48,6 +49,7 @@ _SIGS = {
     "tanx_ctx_destroy": (_i32, [_vp]),
     "tanx_ctx_device": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "tanx_ctx_process_stream": (_vp, [_vp]),
+    "tanx_ctx_update_stream": (_vp, [_vp]),
     "tanx_mixer_create": (_i32, [_vp, ctypes.c_size_t, _i32, ctypes.POINTER(_vp)]),
     "tanx_mixer_destroy": (_i32, [_vp]),
     "tanx_mix": (_i32, [_vp, _vp, _vp]),

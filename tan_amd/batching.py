@@ -44,10 +44,12 @@ class GroupCall:
         self.ost = (ctypes.c_size_t * n)(*[t.stride(0) for t in outputs])
         # keep the tensors alive as long as the pointers are in use
         self._keep = (list(convs), list(inputs), list(outputs))
+        self._ctx = convs[0]._ctx if n else None
 
     def launch(self):
         if self.n == 0:
             return T.AMF_OK
+        self._ctx.order_after_torch()
         return self._lib.tanx_group_process_device(self.n, self.h, self.i, self.ist, self.o, self.ost, None)
 
 

@@ -113,7 +113,11 @@ static int32_t ctx_upload_twiddles(tanx_ctx *ctx)
         }
     }
     TANX_CHECK(hipMalloc(&ctx->tw, total * sizeof(float2)));
-    TANX_CHECK(hipMemcpy(ctx->tw, h.data(), total * sizeof(float2), hipMemcpyHostToDevice));
+    // on the process stream and waited for: a plain hipMemcpy from pageable
+    // memory may return before its DMA lands, unordered with the context's
+    // non-blocking streams
+    TANX_CHECK(hipMemcpyAsync(ctx->tw, h.data(), total * sizeof(float2), hipMemcpyHostToDevice, ctx->proc));
+    TANX_CHECK(hipStreamSynchronize(ctx->proc));
     return TANX_OK;
 }
 
@@ -176,6 +180,7 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
 }
 
 extern "C" void *tanx_ctx_process_stream(tanx_ctx *ctx) { return ctx ? (void *)ctx->proc : nullptr; }
+extern "C" void *tanx_ctx_update_stream(tanx_ctx *ctx) { return ctx ? (void *)ctx->upd : nullptr; }
 
 extern "C" int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device)
 {
@@ -242,6 +247,18 @@ struct tanx_conv {
     int tail_nchunk = 1;  // chunking of the partial sums the next head reads
     int tread = 0;        // partial-sum buffer the next head reads
     bool tail_due = false;  // a head ran since the last tail
+    // Multi-block tail passes (kp > 1): the launch of block wo computed the
+    // partial sums of blocks wo+1 .. wo+kp (the window, parity wpar), each
+    // over partitions k+1.. for block wo+k, and block wo+k's head adds
+    // partitions 0..k itself.  A switch armed inside a window gets a
+    // catch-up pass for its new filter; org[k-1][f] is the block whose
+    // launch computed window slot k's sums for filter slot f (0: the
+    // block's filter, 1: the crossfade's second filter).
+    int kp = 1;
+    long long wo = -1;
+    int wpar = 1;
+    int wchunk[2] = {1, 1}, wpc[2] = {1, 1};
+    long long org[kPassMax][2] = {};
     bool fused = true;    // device path: head and next tail in one launch
     bool fixed_chunks = false;
 
@@ -343,6 +360,10 @@ struct tanx_conv {
     float *scale_ptr(int s) const { return S + (size_t)s * C * P; }
     // partial sums of buffer `par` for filter f (0 = current, 1 = next)
     float2 *tail_ptr(int par, int f) const { return T + ((size_t)par * 2 + f) * C * maxchunk * B; }
+    float2 *tail_ptrk(int par, int k, int f) const
+    {
+        return T + (((size_t)par * kp + k) * 2 + f) * C * maxchunk * B;
+    }
 };
 
 static void conv_free(tanx_conv *cv)
@@ -393,12 +414,15 @@ static void set_chunks(tanx_conv *cv, int n)
     cv->nchunk = (parts + cv->pc - 1) / cv->pc;
 }
 
+constexpr double kPassDefaultBytes = 4.5e9;
+
 static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(kMaxChunks, P - 2), 16384 / C)); }
 
 // Shape of an object: uniform partitions of B over the first `head` taps
 // (P = head / B), then (non-uniform objects) PB frame partitions of N = M * B.
 struct ConvShape {
     int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1, shift = 0;
+    int kp = 1;  // blocks per tail pass
     int irtype = IR_F32;
     bool td = false;  // time-domain object: IR slots and history in samples
 };
@@ -501,6 +525,19 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
         sh.P = sh.len / sh.B;
     }
     sh.maxchunk = max_chunks_for(sh.C, sh.P);
+    // multi-block tail passes: uniform objects outside finalize mode.  A
+    // pass reads every partition once for kp blocks, but its launch takes
+    // longer than one block's tail; by default it is used while one pass
+    // streams at most kPassDefaultBytes (about 0.75 ms of HBM time), so the
+    // block carrying it stays far inside any audio deadline, and the IR has
+    // partitions enough to stream.
+    const uint32_t want_kp = opt ? opt->pass_blocks : 0;
+    if (want_kp > (uint32_t)kPassMax) return TANX_INVALID_ARG;
+    const bool finalize = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
+    if (M == 1 && !finalize) {
+        const double pass_bytes = (double)sh.C * (sh.P - 2) * sh.B * (sizeof(float2) + ir_elem_bytes(sh.irtype));
+        sh.kp = want_kp ? (int)want_kp : ((sh.P >= 16 && pass_bytes <= kPassDefaultBytes) ? kPassMax : 1);
+    }
     return TANX_OK;
 }
 
@@ -528,7 +565,7 @@ static ConvLayout conv_layout(const ConvShape &sh)
     l.H = al(3 * C * P * B * elem);
     l.S = al(3 * C * P * sizeof(float));
     l.F = al(C * P * B * sizeof(float2));
-    l.T = al(4 * C * (size_t)sh.maxchunk * B * sizeof(float2));
+    l.T = al(4 * (size_t)sh.kp * C * (size_t)sh.maxchunk * B * sizeof(float2));
     l.X = al(C * B * sizeof(float));
     l.HB = al(3 * C * PB * N * elem);
     l.SB = al(3 * C * PB * sizeof(float));
@@ -610,6 +647,9 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->maxchunk = sh.maxchunk;
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
+    cv->kp = sh.kp;
+    cv->wchunk[0] = cv->wchunk[1] = cv->nchunk;
+    cv->wpc[0] = cv->wpc[1] = cv->pc;
 
     const size_t C = cv->C, B = cv->B;
     const ConvLayout l = conv_layout(sh);
@@ -692,9 +732,18 @@ static int32_t flush_channel(tanx_conv *cv, int ch)
     hipError_t e = launch_zero(cv->F + (size_t)ch * cv->P * B, (size_t)cv->P * B * sizeof(float2), s);
     if (e == hipSuccess) e = launch_zero(cv->xprev + (size_t)ch * B, B * sizeof(float), s);
     // partial sums are laid out [C][tail_nchunk][B]: the next head reads them
-    for (int f = 0; f < 2 && e == hipSuccess; f++)
-        e = launch_zero(cv->tail_ptr(cv->tread, f) + (size_t)ch * cv->tail_nchunk * B,
-                        (size_t)cv->tail_nchunk * B * sizeof(float2), s);
+    if (cv->kp > 1) {
+        // every window slot: sums computed ahead hold the history being dropped
+        for (int par = 0; par < 2; par++)
+            for (int k = 0; k < cv->kp; k++)
+                for (int f = 0; f < 2 && e == hipSuccess; f++)
+                    e = launch_zero(cv->tail_ptrk(par, k, f) + (size_t)ch * cv->wchunk[par] * B,
+                                    (size_t)cv->wchunk[par] * B * sizeof(float2), s);
+    } else {
+        for (int f = 0; f < 2 && e == hipSuccess; f++)
+            e = launch_zero(cv->tail_ptr(cv->tread, f) + (size_t)ch * cv->tail_nchunk * B,
+                            (size_t)cv->tail_nchunk * B * sizeof(float2), s);
+    }
     if (cv->nu) {
         // frame ring, input ring, frame partials and outputs of the channel
         const size_t N = cv->N;
@@ -785,13 +834,18 @@ static TickArgs tick_common(tanx_conv *cv)
     a.twM = tw_table(cv->ctx->tw, lB);
     a.twN = tw_table(cv->ctx->tw, lB + 1);
     a.fdl = cv->F;
+    a.kn = 1;
+    a.dA = a.dB = 1;
     return a;
 }
 
 // Tail role arguments for the block after the current one: partitions
 // 2..P-1, with both filters when that block is the crossfade block.
+static int fill_tail_passes(tanx_conv *cv, TickArgs &a);
+
 static int fill_tail(tanx_conv *cv, TickArgs &a)
 {
+    if (cv->kp > 1) return fill_tail_passes(cv, a);
     // cv->ticks is the index of the block this tail is for
     const int nf = (cv->delayed && cv->ticks == cv->switch_at) ? 2 : 1;
     a.nchunk = cv->nchunk;
@@ -804,6 +858,83 @@ static int fill_tail(tanx_conv *cv, TickArgs &a)
     a.tTa = cv->tail_ptr(cv->tread ^ 1, 0);
     a.tTb = nf == 2 ? cv->tail_ptr(cv->tread ^ 1, 1) : nullptr;
     return nf;
+}
+
+// Tail role of a launch of an object with multi-block passes (cv->ticks is
+// the block after the one whose head the launch runs):
+//  - a pass when the current window is used up: the sums of the next kp
+//    blocks with the filter each of them will use (an armed switch inside
+//    the window: the old filter up to the crossfade block, the new one from
+//    it on, both for that block);
+//  - a catch-up when a switch was armed at this block inside the window:
+//    the new filter's sums for the window's remaining blocks (the
+//    crossfade block's second filter, and the following blocks' only one);
+//  - otherwise none (a head-only launch).
+// Returns the number of filters the launch's tail blocks read.
+static int fill_tail_passes(tanx_conv *cv, TickArgs &a)
+{
+    const long long hb = (long long)cv->ticks - 1;  // the block whose head this launch runs
+    const bool pass = cv->wo < 0 || hb == cv->wo + cv->kp;
+    const bool armed = cv->delayed && (long long)cv->switch_at == hb + 1;
+    a.tbase = cv->pos;
+    a.tHb = nullptr;
+    a.tSb = nullptr;
+    for (int k = 0; k < kPassMax; k++) a.kTA[k] = a.kTB[k] = nullptr;
+    if (!pass && !armed) {
+        a.nchunk = 0;
+        a.kn = 1;
+        return 1;
+    }
+    if (pass) {
+        cv->wo = hb;
+        cv->wpar ^= 1;
+        cv->wchunk[cv->wpar] = cv->nchunk;
+        cv->wpc[cv->wpar] = cv->pc;
+        const long long s_rel = (cv->delayed && (long long)cv->switch_at > hb) ? (long long)cv->switch_at - hb : -1;
+        a.nchunk = cv->nchunk;
+        a.pc = cv->pc;
+        a.kn = cv->kp;
+        a.tHa = cv->slot_ptr(cv->cur);
+        a.tSa = cv->scale_ptr(cv->cur);
+        bool new_filter = false;
+        for (int k = 1; k <= cv->kp; k++) {
+            float2 *own = cv->tail_ptrk(cv->wpar, k - 1, 0), *second = cv->tail_ptrk(cv->wpar, k - 1, 1);
+            if (s_rel < 0 || k < s_rel) {
+                a.kTA[k - 1] = own;
+                cv->org[k - 1][0] = hb;
+            } else if (k == s_rel) {
+                a.kTA[k - 1] = own;
+                a.kTB[k - 1] = second;
+                cv->org[k - 1][0] = cv->org[k - 1][1] = hb;
+                new_filter = true;
+            } else {
+                a.kTB[k - 1] = own;  // after the switch the new filter is the block's own
+                cv->org[k - 1][0] = hb;
+                new_filter = true;
+            }
+        }
+        if (new_filter) {
+            a.tHb = cv->slot_ptr(cv->upd);
+            a.tSb = cv->scale_ptr(cv->upd);
+        }
+    } else {
+        a.nchunk = cv->wchunk[cv->wpar];
+        a.pc = cv->wpc[cv->wpar];
+        a.kn = (int)(cv->wo + cv->kp - hb);
+        a.tHa = cv->slot_ptr(cv->upd);
+        a.tSa = cv->scale_ptr(cv->upd);
+        for (int k = 1; k <= a.kn; k++) {
+            const long long blk = hb + k;
+            const int kw = (int)(blk - cv->wo);
+            const int f = blk == (long long)cv->switch_at ? 1 : 0;
+            a.kTA[k - 1] = cv->tail_ptrk(cv->wpar, kw - 1, f);
+            cv->org[kw - 1][f] = hb;
+        }
+    }
+    // a one-output pass runs the single-block tail role, which writes tTa/tTb
+    a.tTa = a.kTA[0];
+    a.tTb = a.kTB[0];
+    return a.tHb ? 2 : 1;
 }
 
 // After a tail has been enqueued: the next head reads its buffer, and the
@@ -864,6 +995,14 @@ static uint64_t tail_bytes(const tanx_conv *cv, int nf)
     return (uint64_t)cv->C * parts * (cv->B * (sizeof(float2) + (uint64_t)ir_elem_bytes(cv->irtype) * nf) + scales);
 }
 
+// partitions 2..d the head adds beyond partitions 0 and 1 (multi-block
+// passes): an IR row and an FDL row each
+static uint64_t near_bytes(const tanx_conv *cv, const TickArgs &a)
+{
+    const int extra = std::max(0, a.dA - 1) + (a.xfade ? std::max(0, a.dB - 1) : 0);
+    return (uint64_t)extra * cv->C * cv->B * (sizeof(float2) + (uint64_t)ir_elem_bytes(cv->irtype));
+}
+
 // frame roles of a non-uniform block: the multiply-accumulate slice (nf
 // filters) and, at the end of a frame, the frame kernel
 static uint64_t frame_mac_bytes(const tanx_conv *cv, int nf)
@@ -890,9 +1029,13 @@ static int32_t enqueue_tail(tanx_conv *cv)
     TickArgs a = tick_common(cv);
     a.nhead = 0;
     const int nf = fill_tail(cv, a);
-    if (nf == 2) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
-    int32_t rc = launch_timed(cv, cv->t_tail, nf, a, tail_bytes(cv, nf), cv->tail_alg_bytes);
-    if (rc != TANX_OK) return rc;
+    if (a.nchunk > 0) {
+        // a pass reads the update slot when it computes the new filter's sums
+        if (nf == 2 || (cv->kp > 1 && a.tHa == cv->slot_ptr(cv->upd)))
+            TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
+        int32_t rc = launch_timed(cv, cv->t_tail, nf, a, tail_bytes(cv, nf), cv->tail_alg_bytes);
+        if (rc != TANX_OK) return rc;
+    }
     tail_done(cv);
     return TANX_OK;
 }
@@ -1040,6 +1183,24 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     a.Tb = xfade ? cv->tail_ptr(cv->tread, 1) : cv->tail_ptr(cv->tread, 0);
     a.tchunk = cv->tail_nchunk;
     a.xfade = xfade ? 1 : 0;
+    if (cv->kp > 1) {
+        // this block's slot of the current window, and the partitions its
+        // head adds: those after the pass that computed the slot
+        const long long bb = (long long)b;
+        const int k = cv->wo >= 0 ? (int)(bb - cv->wo) : 0;
+        if (k >= 1 && k <= cv->kp) {
+            a.Ta = cv->tail_ptrk(cv->wpar, k - 1, 0);
+            a.Tb = xfade ? cv->tail_ptrk(cv->wpar, k - 1, 1) : a.Ta;
+            a.dA = (int)(bb - cv->org[k - 1][0]);
+            a.dB = xfade ? (int)(bb - cv->org[k - 1][1]) : 1;
+            a.tchunk = cv->wchunk[cv->wpar];
+        } else {
+            // the first block: no pass has run, and the history is silent
+            // (the other parity's first slot is still zero)
+            a.Ta = a.Tb = cv->tail_ptrk(cv->wpar, 0, 0);
+            a.tchunk = cv->wchunk[cv->wpar];
+        }
+    }
 
     cv->pos = (cv->pos + 1) % cv->P;
     cv->crossfading = xfade;
@@ -1058,9 +1219,13 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     cv->tail_due = true;
 
     int nf = 1;
-    if (with_tail) nf = fill_tail(cv, a);
+    bool pass_upd = false;
+    if (with_tail) {
+        nf = fill_tail(cv, a);
+        pass_upd = cv->kp > 1 && a.nchunk > 0 && a.tHa == cv->slot_ptr(cv->upd);
+    }
     *tail_nf = nf;
-    *needs_update = xfade || nf == 2 || nu_upd;
+    *needs_update = xfade || nf == 2 || nu_upd || pass_upd;
     *xfade_out = xfade;
     return TANX_OK;
 }
@@ -1139,11 +1304,16 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
                                &needs_update, &xfade, &fa, &frame_due);
     if (rc != TANX_OK) return rc;
     if (needs_update) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
-    uint64_t hb = head_bytes(cv, xfade ? 2 : 1) + frame_mac_bytes(cv, a.bHb ? 2 : 1);
+    uint64_t hb = head_bytes(cv, xfade ? 2 : 1) + frame_mac_bytes(cv, a.bHb ? 2 : 1) + near_bytes(cv, a);
     if (frame_due) hb += frame_kernel_bytes(cv, fa.Hb ? 2 : 1);
     const FrameArgs *fp = frame_due ? &fa : nullptr;
     if (with_tail) {
-        rc = launch_timed(cv, cv->t_head, tail_nf, a, hb + tail_bytes(cv, tail_nf), cv->head_alg_bytes, fp);
+        const uint64_t tb = a.nchunk > 0 ? tail_bytes(cv, tail_nf) : 0;
+        // with multi-block passes, the launches that run a pass are timed as
+        // tail launches and the head-only ones as head launches
+        const bool pass = cv->kp > 1 && a.nchunk > 0;
+        rc = launch_timed(cv, pass ? cv->t_tail : cv->t_head, tail_nf, a, hb + tb,
+                          pass ? cv->tail_alg_bytes : cv->head_alg_bytes, fp);
         if (rc != TANX_OK) return rc;
         tail_done(cv);
     } else {
@@ -1278,7 +1448,8 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (nb == 0) return TANX_OK;
     if (in_stride < nb * B || out_stride < nb * B) return TANX_INVALID_ARG;
     DeviceGuard g(cv->ctx->device);
-    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !cv->nu && !cv->td && !flags && nb >= 2)
+    if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !cv->nu && !cv->td && cv->kp == 1 && !flags &&
+        nb >= 2)
         return process_device_graph(cv, d_in, in_stride, d_out, out_stride, nb, processed);
     for (size_t b = 0; b < nb; b++) {
         const bool fuse = cv->fused && !cv->fmode;
@@ -1699,6 +1870,7 @@ extern "C" int32_t tanx_conv_get_info(const tanx_conv *cv, tanx_conv_info *info)
     info->ticks = cv->ticks;
     info->crossfades = cv->xfades;
     info->nu_multiple = (uint32_t)cv->M;
+    info->pass_blocks = (uint32_t)cv->kp;
     return TANX_OK;
 }
 
@@ -1797,6 +1969,21 @@ extern "C" int32_t tanx_conv_autotune(tanx_conv *cv, uint32_t *chosen, double *b
             a.tSb = nullptr;
             a.tTa = cv->tail_ptr((t + 1) & 1, 0);
             a.tTb = nullptr;
+            if (cv->kp > 1) {
+                // a pass every kp blocks, head-only launches between
+                const int k = t % cv->kp, par = (t / cv->kp) & 1;
+                a.Ta = a.Tb = cv->tail_ptrk(par ^ 1, (k + cv->kp - 1) % cv->kp, 0);
+                a.dA = k == 0 ? cv->kp : k;
+                if (k == 0) {
+                    a.kn = cv->kp;
+                    for (int q = 0; q < kPassMax; q++) {
+                        a.kTA[q] = q < cv->kp ? cv->tail_ptrk(par, q, 0) : nullptr;
+                        a.kTB[q] = nullptr;
+                    }
+                } else {
+                    a.nchunk = 0;
+                }
+            }
             return a;
         };
         const int reps = cv->C >= 4096 ? 4 : 32;
@@ -2213,8 +2400,11 @@ static int32_t dft_tables(tanx_ctx *ctx, int n)
     }
     TANX_CHECK(hipMalloc(&ctx->dft_hi, count * 2));
     TANX_CHECK(hipMalloc(&ctx->dft_lo, count * 2));
-    TANX_CHECK(hipMemcpy(ctx->dft_hi, hi.data(), count * 2, hipMemcpyHostToDevice));
-    TANX_CHECK(hipMemcpy(ctx->dft_lo, lo.data(), count * 2, hipMemcpyHostToDevice));
+    // ordered on the process stream (the kernels' stream) and complete before
+    // the host vectors go away
+    TANX_CHECK(hipMemcpyAsync(ctx->dft_hi, hi.data(), count * 2, hipMemcpyHostToDevice, ctx->proc));
+    TANX_CHECK(hipMemcpyAsync(ctx->dft_lo, lo.data(), count * 2, hipMemcpyHostToDevice, ctx->proc));
+    TANX_CHECK(hipStreamSynchronize(ctx->proc));
     ctx->dft_n = n;
     return TANX_OK;
 }

@@ -43,6 +43,8 @@ namespace tanx {
 enum IrType { IR_F32 = 0, IR_F16 = 1, IR_BF16 = 2 };
 inline int ir_elem_bytes(int t) { return t == IR_F32 ? 8 : 4; }
 
+constexpr int kPassMax = 4;  // blocks one tail pass computes partial sums for
+
 struct TickArgs {
     int B;                 // block size (group launches dispatch on it)
     int C;
@@ -80,6 +82,16 @@ struct TickArgs {
     const float *tSb;
     float2 *tTa;           // partial sums written [C][nchunk][B]
     float2 *tTb;
+    // multi-block pass (kn > 1): output k (0-based) holds the partial sums
+    // for the block k + 1 after this one over partitions i >= k + 2 (with
+    // FDL row (tbase + k - i) & (P - 1)); filter tHa's sums go to kTA[k],
+    // filter tHb's to kTB[k] (nullptr: not computed).  tTa/tTb unused.
+    int kn;
+    float2 *kTA[4];
+    float2 *kTB[4];
+    // head: partitions 0..dA (filter A) and 0..dB (filter B) are added by
+    // the head itself (1 without multi-block passes); the rest come from Ta/Tb
+    int dA, dB;
 
     // non-uniform objects (nu = 0 for uniform ones)
     int nu;

@@ -252,6 +252,7 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
     }
     const int prevslot = (a.slot - 1) & (a.P - 1);
     const float2 *Fprev = a.fdl + ((size_t)c * a.P + prevslot) * M;
+    const int dmax = max(a.dA, nf == 2 ? a.dB : 1);  // 1: partition 1 only
     float2 acc[2][KPT], h0[2][KPT];
 #pragma unroll
     for (int f = 0; f < 2; f++) {
@@ -281,9 +282,23 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                         s.y += t[u].y;
                     }
                 }
-                if (a.P > 1) {
-                    const float2 h1 = IrRow<HT>::get1(H1row, k);
-                    mac2(s, Fprev[k], make_float2(h1.x * s1, h1.y * s1), k == 0);
+                if (dmax == 1) {
+                    if (a.P > 1) {
+                        const float2 h1 = IrRow<HT>::get1(H1row, k);
+                        mac2(s, Fprev[k], make_float2(h1.x * s1, h1.y * s1), k == 0);
+                    }
+                } else {
+                    // partitions 1..d: the blocks since the pass that computed the partials
+                    const int d = min(f == 0 ? a.dA : a.dB, a.P - 1);
+#pragma unroll
+                    for (int jn = 1; jn <= kPassMax; jn++) {
+                        if (jn <= d) {
+                            const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
+                            const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
+                            const float2 hv = IrRow<HT>::get1(ir_row<HT>(Hbase, c, a.P, jn, M), k);
+                            mac2(s, Fj[k], make_float2(hv.x * sj, hv.y * sj), k == 0);
+                        }
+                    }
                 }
                 const float2 h0v = IrRow<HT>::get1(H0row, k);
                 h0[f][q] = make_float2(h0v.x * s0, h0v.y * s0);
@@ -494,6 +509,134 @@ __device__ void mac_rows(const MacView &w, float4 *red)
     }
 }
 
+// Multi-block pass (kn > 1 outputs): partitions [2 + j*pc, 2 + (j+1)*pc)
+// of channel c for each of the kn blocks after this one.  Output k (0-based,
+// block t+1+k) takes partition i >= k + 2 with FDL row (tbase + k - i), the
+// row partition i - k reads for output 0: each lane group walks its
+// partitions in order and keeps the last kPassMax rows it loaded, so every
+// IR row and FDL row is read once and used for every output.
+template <int M, int NF, int HT>
+__device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
+{
+    constexpr int H = M / 2;                     // float4 per row
+    constexpr int V = H > 256 ? H / 256 : 1;     // float4 per lane per row
+    constexpr int R = H / V;                     // lanes per row
+    constexpr int G = kTickThreads / R;          // lane groups
+    constexpr int U = 8;                         // partitions loaded together
+    constexpr int KM = kPassMax;
+    constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
+    const int tid = threadIdx.x;
+    const int g = tid / R, r = tid % R;
+    const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
+    const char *HA = reinterpret_cast<const char *>(a.tHa) + (size_t)c * a.P * kRowBytes;
+    const char *HB = NF == 2 ? reinterpret_cast<const char *>(a.tHb) + (size_t)c * a.P * kRowBytes : nullptr;
+    const float *SA = HT == IR_F32 ? nullptr : a.tSa + (size_t)c * a.P;
+    const float *SB = (HT == IR_F32 || NF == 1) ? nullptr : a.tSb + (size_t)c * a.P;
+    const int mask = a.P - 1, base = a.tbase, kn = a.kn;
+    const int i0 = 2 + j * a.pc, i1 = min(a.P, i0 + a.pc);
+    const int span = (i1 - i0 + G - 1) / G;
+    const int p0 = min(i1, i0 + g * span), p1 = min(i1, p0 + span);
+    bool wantA[KM], wantB[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        wantA[k] = k < kn && a.kTA[k] != nullptr;
+        wantB[k] = NF == 2 && k < kn && a.kTB[k] != nullptr;
+    }
+    const bool dc0 = (r == 0);
+    const float s0 = dc0 ? 0.f : 1.f;
+    auto frow = [&](int row, int v) { return ld_stream(F + (size_t)(row & mask) * H + v * R + r); };
+
+    float4 accA[KM][V], accB[KM][V], xw[KM][V];
+#pragma unroll
+    for (int k = 0; k < KM; k++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            accA[k][v] = make_float4(0.f, 0.f, 0.f, 0.f);
+            accB[k][v] = make_float4(0.f, 0.f, 0.f, 0.f);
+            // the window as it stands before partition p0: xw[k] = row (base + k + 1 - p0)
+            // (output k + 1 reads it at p0), loaded only where some output uses it
+            xw[k][v] = (k + 1 < KM && p0 < p1 && p0 >= k + 3) ? frow(base + k + 1 - p0, v)
+                                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    for (int i = p0; i < p1; i += U) {
+        float4 xn[U][V], ha[U][V], hb[U][V];
+        float sa[U], sb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u;
+            sa[u] = 1.f;
+            sb[u] = 1.f;
+            if (ii < p1) {
+                sa[u] = row_scale<HT>(SA, ii);
+                if (NF == 2) sb[u] = row_scale<HT>(SB, ii);
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                if (ii < p1) {
+                    xn[u][v] = frow(base - ii, v);
+                    ha[u][v] = IrRow<HT>::get2(HA + (size_t)ii * kRowBytes, v * R + r);
+                    if (NF == 2) hb[u][v] = IrRow<HT>::get2(HB + (size_t)ii * kRowBytes, v * R + r);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u;
+            if (ii >= p1) break;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+#pragma unroll
+                for (int k = KM - 1; k > 0; k--) xw[k][v] = xw[k - 1][v];
+                xw[0][v] = xn[u][v];
+                const bool dc = dc0 && v == 0;
+                const float sv = v == 0 ? s0 : 1.f;
+                const float4 hA = HT == IR_F32 ? ha[u][v] : scale4(ha[u][v], sa[u]);
+                float4 hB = hA;
+                if (NF == 2) hB = HT == IR_F32 ? hb[u][v] : scale4(hb[u][v], sb[u]);
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (ii >= k + 2) {
+                        if (wantA[k]) mac4(accA[k][v], xw[k][v], hA, sv, dc);
+                        if (NF == 2 && wantB[k]) mac4(accB[k][v], xw[k][v], hB, sv, dc);
+                    }
+                }
+            }
+        }
+    }
+    // the lane groups' sums, one output at a time through red: [NF][G][H]
+    const size_t orow = ((size_t)c * a.nchunk + j) * H;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        if (!(wantA[k] || wantB[k])) continue;  // uniform over the workgroup
+        if constexpr (G == 1) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                if (wantA[k]) reinterpret_cast<float4 *>(a.kTA[k])[orow + v * R + r] = accA[k][v];
+                if (NF == 2 && wantB[k]) reinterpret_cast<float4 *>(a.kTB[k])[orow + v * R + r] = accB[k][v];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                red[(0 * G + g) * H + v * R + r] = accA[k][v];
+                if (NF == 2) red[(1 * G + g) * H + v * R + r] = accB[k][v];
+            }
+            __syncthreads();
+            for (int e = tid; e < NF * H; e += kTickThreads) {
+                const int f = e / H, q = e % H;
+                if (f == 0 ? !wantA[k] : !wantB[k]) continue;
+                float4 sum = red[(f * G + 0) * H + q];
+#pragma unroll
+                for (int gg = 1; gg < G; gg++) {
+                    const float4 t = red[(f * G + gg) * H + q];
+                    sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
+                }
+                reinterpret_cast<float4 *>(f == 0 ? a.kTA[k] : a.kTB[k])[orow + q] = sum;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Uniform tail: partitions [2 + j*pc, 2 + (j+1)*pc) of channel c for the next block.
 template <int M, int NF, int HT>
 __device__ void tail_role(const TickArgs &a, int c, int j, float4 *red)
@@ -555,7 +698,11 @@ __device__ __forceinline__ void frame_mac_block(const TickArgs &a, int t, float4
     else frame_mac_role<M, 1, HT>(a, c, k, smem);
 }
 
-template <int M, int NF, int HT>
+// TK: 0 = a launch without tail blocks (head and frame roles only),
+// 1 = the next block's tail, 2 = a multi-block pass.  Separate
+// instantiations keep head-only launches light and let profiles tell the
+// launch kinds apart.
+template <int M, int NF, int HT, int TK>
 __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
 {
     constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
@@ -566,11 +713,12 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
     if (b < a.nhead) {
         if (!a.run[b]) return;
         head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
-    } else if (b < a.nhead + a.C * a.nchunk) {
+    } else if (TK != 0 && b < a.nhead + a.C * a.nchunk) {
         const int t = b - a.nhead;
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        tail_role<M, NF, HT>(a, c, j, smem);
+        if constexpr (TK == 2) tailk_role<M, NF, HT>(a, c, j, smem);
+        else tail_role<M, NF, HT>(a, c, j, smem);
     } else {
         frame_mac_block<M, HT>(a, b - a.nhead - a.C * a.nchunk, smem);
     }
@@ -591,8 +739,13 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
         }
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
-        else tail_role<M, 1, HT>(a, c, j, smem);
+        if (a.kn > 1) {
+            if (a.tHb) tailk_role<M, 2, HT>(a, c, j, smem);
+            else tailk_role<M, 1, HT>(a, c, j, smem);
+        } else {
+            if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
+            else tail_role<M, 1, HT>(a, c, j, smem);
+        }
     }
 }
 
@@ -1124,10 +1277,21 @@ template <int HT>
 static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream_t s, unsigned blocks,
                                 hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (tail_nf == 2) {
-        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+    const int tk = a.nchunk == 0 ? 0 : (a.kn > 1 ? 2 : 1);
+    if (tk == 0) {
+        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 0>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+    } else if (tk == 1) {
+        if (tail_nf == 2) {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 1>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        } else {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 1>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        }
     } else {
-        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        if (tail_nf == 2) {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 2>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        } else {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 2>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        }
     }
     return hipGetLastError();
 }

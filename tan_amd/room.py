@@ -79,6 +79,7 @@ def GenerateRoomResponses(context, rooms, sources, listeners, sample_rate, respo
         raise ValueError("rooms, sources and listeners differ in count")
     if out.dim() != 2 or out.shape[0] < 2 * n or out.shape[1] < response_length or out.stride(1) != 1:
         raise ValueError("out must be a (>= 2n, >= response_length) row-major tensor")
+    context.order_after_torch()
     ra = (RoomDefinition * n)(*rooms)
     sa = (MonoSource * n)(*sources)
     la = (StereoListener * n)(*listeners)
@@ -99,9 +100,11 @@ def generateRoomResponse(context, room, source, listener, sample_rate, response_
         rc = GenerateRoomResponses(context, [room], [source], [listener], sample_rate, response_length, tmp, flags,
                                    max_bounces)
         if rc == AMF_OK:
-            with torch.cuda.stream(torch.cuda.ExternalStream(lib.tanx_ctx_process_stream(context.handle))):
+            proc = torch.cuda.ExternalStream(lib.tanx_ctx_process_stream(context.handle))
+            with torch.cuda.stream(proc):
                 responseL[:response_length].copy_(tmp[0])
                 responseR[:response_length].copy_(tmp[1])
+            tmp.record_stream(proc)  # the allocator keeps tmp until the copies ran
         return rc
     for a in (responseL, responseR):
         if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous

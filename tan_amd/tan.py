@@ -124,9 +124,27 @@ class TANContext:
             raise RuntimeError(f"TANCreateContext failed: AMF_RESULT {rc}")
         self.handle = h
         self.device = device
+        self._streams = None
+
+    def order_after_torch(self, update=False):
+        """Makes the work this context queues next (on its process stream,
+        and with update=True on its update stream too) start after the work
+        queued so far on torch's current stream, which may be writing the
+        tensors the next call reads or writes.  The library's calls are
+        asynchronous on its own streams; the device-tensor methods of this
+        module call this first."""
+        import torch
+        if self._streams is None:
+            self._streams = (torch.cuda.ExternalStream(self._lib.tanx_ctx_process_stream(self.handle)),
+                             torch.cuda.ExternalStream(self._lib.tanx_ctx_update_stream(self.handle)))
+        cur = torch.cuda.current_stream()
+        self._streams[0].wait_stream(cur)
+        if update:
+            self._streams[1].wait_stream(cur)
 
     def Terminate(self):
         if self.handle:
+            self._streams = None
             rc = self._lib.tanx_ctx_destroy(self.handle)
             self.handle = None
             return rc
@@ -142,13 +160,15 @@ class TANContext:
 class TANConvolution:
     """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
 
-    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0, nu_phase=0):
+    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0, nu_phase=0, pass_blocks=0):
         """ir_precision: storage of the IR spectra (TAN_IR_PRECISION_*);
         nu_multiple: blocks per frame for the non-uniform methods (0: chosen
         by the library, 1: uniform partitions); nu_phase: frame phase in
         blocks, modulo the multiple (the first frame ends at block
         multiple - 1 - phase), so that objects advanced together can spread
-        their frame work over the ticks."""
+        their frame work over the ticks; pass_blocks: blocks whose partial
+        sums one tail pass computes for uniform objects (1..4, 0: chosen by
+        the library, 1: a tail every block)."""
         self._ctx = context
         self._lib = context._lib
         self.handle = None
@@ -157,6 +177,7 @@ class TANConvolution:
         self.ir_precision = ir_precision
         self.nu_multiple = nu_multiple
         self.nu_phase = nu_phase
+        self.pass_blocks = pass_blocks
 
     # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
     def Init(self, method, response_length, buffer_size, channels):
@@ -170,6 +191,7 @@ class TANConvolution:
         opt.ir_precision = self.ir_precision
         opt.nu_multiple = self.nu_multiple
         opt.nu_phase = self.nu_phase
+        opt.pass_blocks = self.pass_blocks
         rc = self._lib.tanx_conv_create_ex(self._ctx.handle, method, response_length, buffer_size,
                                            channels, ctypes.addressof(opt), ctypes.byref(h))
         if rc == AMF_OK:
@@ -207,6 +229,7 @@ class TANConvolution:
             t = buffers
             if num_samples is None:
                 num_samples = t.shape[1]
+            self._ctx.order_after_torch(update=True)
             return self._lib.tanx_conv_update_td_device(self.handle, t.data_ptr(), t.stride(0),
                                                         num_samples, fp, operation_flags)
         if num_samples is None:
@@ -229,6 +252,7 @@ class TANConvolution:
         if _is_device_tensor(inputs):
             if num_samples is None:
                 num_samples = inputs.shape[1]
+            self._ctx.order_after_torch()
             rc = self._lib.tanx_conv_process_device(
                 self.handle, inputs.data_ptr(), inputs.stride(0), outputs.data_ptr(),
                 outputs.stride(0), num_samples, fp, ctypes.byref(done))
@@ -342,6 +366,7 @@ class TANFFT:
         if not self._init:
             return AMF_NOT_INITIALIZED
         if _is_device_tensor(inputs):
+            self._ctx.order_after_torch(update=True)
             return self._lib.tanx_fft_transform_device(self._ctx.handle, direction, log2len, channels,
                                                        inputs.data_ptr(), outputs.data_ptr(),
                                                        inputs.stride(0))
@@ -355,6 +380,7 @@ class TANFFT:
         the matrix cores (split = 3: compensated high/low products).  torch
         CUDA tensors (channels, >= n) in and (channels, >= n + 2) out, bins as
         (re, im) pairs; asynchronous on the context's process stream."""
+        self._ctx.order_after_torch()
         return self._lib.tanx_dft_r2c_mfma_device(self._ctx.handle, log2len, channels, inputs.data_ptr(),
                                                   inputs.stride(0), outputs.data_ptr(), outputs.stride(0), split)
 
@@ -388,6 +414,7 @@ def GroupProcess(convs, inputs, outputs, flag_masks=None):
     if n == 0:
         return AMF_OK
     lib = convs[0]._lib
+    convs[0]._ctx.order_after_torch()
     hs = (ctypes.c_void_p * n)(*[c.handle for c in convs])
     ins = (ctypes.c_void_p * n)(*[t.data_ptr() for t in inputs])
     outs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in outputs])
@@ -444,6 +471,7 @@ class TANConverter:
         clipped = ctypes.c_int32(0)
         cp = ctypes.byref(clipped) if want_clipped else None
         if _is_device_tensor(inputs):
+            self._ctx.order_after_torch()
             rc = self._lib.tanx_convert_device(
                 self._ctx.handle, inputs.data_ptr(), input_step, input_offset, _sample_type(inputs.dtype),
                 outputs.data_ptr(), output_step, output_offset, _sample_type(outputs.dtype), num_samples,
@@ -503,6 +531,7 @@ class TANMixer:
         if not self.handle:
             return AMF_NOT_INITIALIZED
         if _is_device_tensor(output):
+            self._ctx.order_after_torch()
             if _is_device_tensor(inputs):
                 stride = inputs.stride(0) if input_stride is None else input_stride
                 return self._lib.tanx_mix_device_strided(self.handle, inputs.data_ptr(), stride, output.data_ptr())

@@ -1,0 +1,151 @@
+"""Multi-block tail passes (tanx_conv_options.pass_blocks): one pass
+computes the partial sums of the next K blocks, each later head adds the
+partitions since that pass, and a switch armed inside a window gets a
+catch-up pass for its new filter.  The output is the reference's schedule:
+equal to the oracle (and the FP64 model) for every K, under dense updates
+that arm switches at every offset of the window, updates while a switch is
+armed, channel flushes and the host-memory path."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import tan_amd as T
+from helpers import crossfade_model, snr_db
+
+pytestmark = pytest.mark.gpu
+
+UNIFORM = T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+BLOCKING = T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY
+
+
+def _run(ctx, L, B, C, nb, updates, K, device=True, flags_at=None, seed=500):
+    import torch
+    print(f"passes L={L} B={B} C={C} K={K} device={device}", flush=True)
+    conv = T.TANConvolution(ctx, pass_blocks=K)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    assert conv.Info()["pass_blocks"] == K
+    orc = O.OracleConvolution(L, B, C)
+    x = T.signals.input_bank(C, nb * B, base_seed=seed)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    y = np.zeros_like(x)
+    yo = np.zeros_like(x)
+    xf_gpu, xf_orc = [], []
+    for b in range(nb):
+        if b in updates:
+            ir = updates[b]
+            assert conv.UpdateResponseTD(ir, L, None, BLOCKING) == T.AMF_OK
+            assert orc.update_td(ir) == O.AMF_OK
+        fl = flags_at(b) if flags_at else None
+        blk = np.ascontiguousarray(x[:, b * B:(b + 1) * B])
+        if device:
+            rc, done = conv.Process(xd[:, b * B:], yd[:, b * B:], B, fl)
+        else:
+            out = np.zeros((C, B), np.float32)
+            rc, done = conv.Process(blk, out, B, fl)
+            y[:, b * B:(b + 1) * B] = out
+        assert rc == T.AMF_OK and done == B
+        rco, out_o, _ = orc.process(blk, flags=fl)
+        yo[:, b * B:(b + 1) * B] = out_o
+        if conv.LastWasCrossfade():
+            xf_gpu.append(b)
+        if orc.last_was_crossfade():
+            xf_orc.append(b)
+    conv.Synchronize()
+    if device:
+        y = yd.cpu().numpy()
+    conv.Terminate()
+    assert xf_gpu == xf_orc
+    return x, y, yo, xf_gpu
+
+
+def uniform_schedule(updates, nb):
+    """Crossfades as (block, update index): an update issued before block b
+    is handed off at the end of the first block >= b that does not
+    crossfade, armed at the next block with no switch armed, and crossfades
+    one block later; an update issued while a switch is armed is held for
+    the next one, and one not yet armed is replaced by a later one."""
+    pending = ready = armed = False
+    waiting = held = cur = None
+    switch_at = None
+    out = []
+    k = 0
+    for b in range(nb):
+        while k < len(updates) and updates[k] == b:
+            if armed:
+                held = k
+            else:
+                waiting = k
+            pending = True
+            k += 1
+        xfade = False
+        if not pending and ready and not armed:
+            ready, armed, cur = False, True, waiting
+            switch_at = b + 1
+        elif armed and b == switch_at:
+            xfade = True
+            out.append((b, cur))
+            armed = False
+            waiting, held = held, None
+        if not xfade and pending:
+            pending, ready = False, True
+    return out
+
+
+def _updates(C, L, blocks, seed=600):
+    return {b: T.signals.ir_bank(C, L, update=i, base_seed=seed) for i, b in enumerate(blocks)}
+
+
+DENSE = [0, 7, 8, 20, 21, 22, 40, 44, 45, 61, 62, 63, 64, 90]
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 4])
+def test_dense_updates_every_pass_length(gpu_ctx, K):
+    L, B, C, nb = 8192, 256, 3, 110
+    ups = _updates(C, L, DENSE)
+    x, y, yo, xf = _run(gpu_ctx, L, B, C, nb, ups, K)
+    err = np.abs(y.astype(np.float64) - yo).max()
+    assert err <= 1e-5, err
+    for c in range(C):
+        assert snr_db(yo[c], y[c]) >= 120.0
+    # and the FP64 model of the reference's schedule
+    order = sorted(ups)
+    sched = uniform_schedule(order, nb)
+    assert [s for s, _ in sched] == xf
+    for c in range(C):
+        ref = crossfade_model(x[c], [ups[order[k]][c] for _, k in sched], xf, B)
+        assert np.abs(y[c] - ref).max() <= 1e-5
+
+
+@pytest.mark.parametrize("K", [2, 4])
+def test_flushes_and_host_path(gpu_ctx, K):
+    L, B, C, nb = 4096, 128, 2, 70
+
+    def flags_at(b):
+        if b in (13, 30, 31, 46):
+            return np.array([T.TAN_CONVOLUTION_CHANNEL_FLAG_FLUSH_STREAM, 0], np.uint32)
+        return None
+
+    ups = _updates(C, L, [0, 17, 33, 34, 50])
+    for device in (True, False):
+        x, y, yo, _ = _run(gpu_ctx, L, B, C, nb, ups, K, device=device, flags_at=flags_at, seed=501)
+        err = np.abs(y.astype(np.float64) - yo).max()
+        assert err <= 1e-5, (device, err)
+
+
+def test_default_pass_length(gpu_ctx):
+    """Uniform objects with 16 or more partitions whose pass streams at most
+    4.5 GB use 4-block passes by default; shorter IRs, larger objects and
+    non-uniform objects keep a tail every block."""
+    for L, B, C, want in ((65536, 256, 2, 4), (2048, 256, 2, 1), (16384, 1024, 2, 4), (65536, 256, 8192, 1),
+                          (65536, 256, 4096, 4)):
+        conv = T.TANConvolution(gpu_ctx)
+        assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        assert conv.Info()["pass_blocks"] == want, (L, B, C)
+        conv.Terminate()
+    conv = T.TANConvolution(gpu_ctx)
+    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, 65536, 256, 2) == T.AMF_OK
+    assert conv.Info()["pass_blocks"] == 1
+    conv.Terminate()
+    conv = T.TANConvolution(gpu_ctx, pass_blocks=5)
+    assert conv.InitGpu(UNIFORM, 65536, 256, 2) == T.AMF_INVALID_ARG
