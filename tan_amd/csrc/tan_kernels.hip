@@ -522,7 +522,7 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
     constexpr int V = H > 256 ? H / 256 : 1;     // float4 per lane per row
     constexpr int R = H / V;                     // lanes per row
     constexpr int G = kTickThreads / R;          // lane groups
-    constexpr int U = 8;                         // partitions loaded together
+    constexpr int U = 4;                         // partitions loaded together (8 measured slower)
     constexpr int KM = kPassMax;
     constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
     const int tid = threadIdx.x;
@@ -603,36 +603,35 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
             }
         }
     }
-    // the lane groups' sums, one output at a time through red: [NF][G][H]
+    // the lane groups' sums of every output through red at once: [NF][KM][G][H]
     const size_t orow = ((size_t)c * a.nchunk + j) * H;
+    if constexpr (G == 1) {
 #pragma unroll
-    for (int k = 0; k < KM; k++) {
-        if (!(wantA[k] || wantB[k])) continue;  // uniform over the workgroup
-        if constexpr (G == 1) {
+        for (int k = 0; k < KM; k++)
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 if (wantA[k]) reinterpret_cast<float4 *>(a.kTA[k])[orow + v * R + r] = accA[k][v];
                 if (NF == 2 && wantB[k]) reinterpret_cast<float4 *>(a.kTB[k])[orow + v * R + r] = accB[k][v];
             }
-        } else {
+    } else {
+#pragma unroll
+        for (int k = 0; k < KM; k++)
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                red[(0 * G + g) * H + v * R + r] = accA[k][v];
-                if (NF == 2) red[(1 * G + g) * H + v * R + r] = accB[k][v];
+                if (wantA[k]) red[((0 * KM + k) * G + g) * H + v * R + r] = accA[k][v];
+                if (NF == 2 && wantB[k]) red[((1 * KM + k) * G + g) * H + v * R + r] = accB[k][v];
             }
-            __syncthreads();
-            for (int e = tid; e < NF * H; e += kTickThreads) {
-                const int f = e / H, q = e % H;
-                if (f == 0 ? !wantA[k] : !wantB[k]) continue;
-                float4 sum = red[(f * G + 0) * H + q];
+        __syncthreads();
+        for (int e = tid; e < NF * KM * H; e += kTickThreads) {
+            const int f = e / (KM * H), k = (e / H) % KM, q = e % H;
+            if (f == 0 ? !wantA[k] : !wantB[k]) continue;
+            float4 sum = red[((f * KM + k) * G + 0) * H + q];
 #pragma unroll
-                for (int gg = 1; gg < G; gg++) {
-                    const float4 t = red[(f * G + gg) * H + q];
-                    sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
-                }
-                reinterpret_cast<float4 *>(f == 0 ? a.kTA[k] : a.kTB[k])[orow + q] = sum;
+            for (int gg = 1; gg < G; gg++) {
+                const float4 t = red[((f * KM + k) * G + gg) * H + q];
+                sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
             }
-            __syncthreads();
+            reinterpret_cast<float4 *>(f == 0 ? a.kTA[k] : a.kTB[k])[orow + q] = sum;
         }
     }
 }
@@ -705,7 +704,8 @@ __device__ __forceinline__ void frame_mac_block(const TickArgs &a, int t, float4
 template <int M, int NF, int HT, int TK>
 __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
 {
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
+    // tail reduction buffer (float4): [NF][G][H], and kPassMax outputs for a pass
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * (TK == 2 ? kPassMax : 1);
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
@@ -756,7 +756,8 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
 template <int M, int HT>
 __global__ __launch_bounds__(kTickThreads) void k_tick_desc(const TickArgs *__restrict__ descs, int k)
 {
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
+    // tick_body may run a multi-block pass: room for its kPassMax outputs
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * kPassMax;  // float4
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
@@ -769,7 +770,8 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
                                                              const int *__restrict__ starts, int nobj)
 {
     constexpr int M = kGroupMaxB;
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);
+    // tick_body may run a multi-block pass: room for its kPassMax outputs
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * kPassMax;
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
