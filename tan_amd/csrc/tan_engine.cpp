@@ -1761,6 +1761,7 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     int *fstarts = reinterpret_cast<int *>(host + desc_bytes + start_bytes + fdesc_bytes);
     uint32_t done = 0;
     int total = 0, nframe = 0, ftotal = 0;
+    bool passes = false;
     for (; done < n; done++) {
         tanx_conv *cv = convs[done];
         int nf = 1;
@@ -1771,6 +1772,7 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
         if (rc != TANX_OK) break;
         if (upd) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
         tail_done(cv);
+        passes = passes || descs[done].kn > 1;
         starts[done] = total;
         total += cv->C + cv->C * descs[done].nchunk + cv->C * descs[done].nchB;
         if (fdue) {
@@ -1784,7 +1786,7 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
         const size_t bytes = desc_bytes + start_bytes + fdesc_bytes + fstart_bytes;
         TANX_CHECK(hipMemcpyAsync(ctx->g_dev, host, bytes, hipMemcpyHostToDevice, ctx->proc));
         TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
-        TANX_CHECK(launch_group_tick(irtype, reinterpret_cast<const TickArgs *>(ctx->g_dev),
+        TANX_CHECK(launch_group_tick(irtype, passes, reinterpret_cast<const TickArgs *>(ctx->g_dev),
                                      reinterpret_cast<const int *>(ctx->g_dev + desc_bytes), (int)done,
                                      (unsigned)total, ctx->proc));
         if (nframe > 0)

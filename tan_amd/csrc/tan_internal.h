@@ -196,11 +196,12 @@ hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hip
                        hipEvent_t ev1 = nullptr);
 // One launch for several objects: block b belongs to the object o with
 // starts[o] <= b < starts[o + 1] and runs its roles from d_descs[o].  Every
-// object must use irtype and a block size in {64, 128, 256, 512}.
-hipError_t launch_group_tick(int irtype, const TickArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
-                             hipStream_t s);
+// object must use irtype and a block size in {64, 128, 256, 512}; passes:
+// some descriptor has kn > 1.
+hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, const int *d_starts, int nobj,
+                             unsigned blocks, hipStream_t s);
 constexpr int kGroupMaxB = 512;
-// k_tick reading its arguments from d_descs[k] (graph replay)
+// k_tick reading its arguments from d_descs[k] (graph replay; kn must be 1)
 hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, unsigned blocks, hipStream_t s);
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);
 // frame kernels: N in {128, ..., 4096}

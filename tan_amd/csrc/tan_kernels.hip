@@ -724,8 +724,10 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
     }
 }
 
-// Roles of one tick for block index b of one object's launch.
-template <int M, int HT>
+// Roles of one tick for block index b of one object's launch; KP compiles
+// in the multi-block pass (its larger reduction buffer and registers are
+// then paid by every block of the launch).
+template <int M, int HT, bool KP>
 __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem)
 {
     if (b < a.nhead) {
@@ -739,10 +741,14 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
         }
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        if (a.kn > 1) {
-            if (a.tHb) tailk_role<M, 2, HT>(a, c, j, smem);
-            else tailk_role<M, 1, HT>(a, c, j, smem);
-        } else {
+        if constexpr (KP) {
+            if (a.kn > 1) {
+                if (a.tHb) tailk_role<M, 2, HT>(a, c, j, smem);
+                else tailk_role<M, 1, HT>(a, c, j, smem);
+                return;
+            }
+        }
+        {
             if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
             else tail_role<M, 1, HT>(a, c, j, smem);
         }
@@ -752,26 +758,27 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
 // Graph-replayed variant: the arguments of launch k come from a descriptor
 // array in device memory (written by one copy per batch), so a captured
 // graph of k nodes can be replayed with new arguments; the tail's filter
-// count is read from the descriptor.
+// count is read from the descriptor.  Captured objects run a tail every
+// block (no multi-block passes).
 template <int M, int HT>
 __global__ __launch_bounds__(kTickThreads) void k_tick_desc(const TickArgs *__restrict__ descs, int k)
 {
-    // tick_body may run a multi-block pass: room for its kPassMax outputs
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * kPassMax;  // float4
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
     const TickArgs a = descs[k];
-    tick_body<M, HT>(a, blockIdx.x, smem);
+    tick_body<M, HT, false>(a, blockIdx.x, smem);
 }
 
-template <int HT>
+// KP: some object of the group runs a multi-block pass in this launch
+template <int HT, bool KP>
 __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__restrict__ descs,
                                                              const int *__restrict__ starts, int nobj)
 {
     constexpr int M = kGroupMaxB;
-    // tick_body may run a multi-block pass: room for its kPassMax outputs
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * kPassMax;
+    // room for kPassMax outputs when a pass may run
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * (KP ? kPassMax : 1);
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
@@ -785,10 +792,10 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
     const TickArgs a = descs[lo];
     const int lb = b - starts[lo];
     switch (a.B) {
-    case 64: tick_body<64, HT>(a, lb, smem); break;
-    case 128: tick_body<128, HT>(a, lb, smem); break;
-    case 256: tick_body<256, HT>(a, lb, smem); break;
-    case 512: tick_body<512, HT>(a, lb, smem); break;
+    case 64: tick_body<64, HT, KP>(a, lb, smem); break;
+    case 128: tick_body<128, HT, KP>(a, lb, smem); break;
+    case 256: tick_body<256, HT, KP>(a, lb, smem); break;
+    case 512: tick_body<512, HT, KP>(a, lb, smem); break;
     default: break;
     }
 }
@@ -1327,14 +1334,21 @@ hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, u
     }
 }
 
-hipError_t launch_group_tick(int irtype, const TickArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
-                             hipStream_t s)
+template <int HT>
+static void launch_group_t(bool passes, const TickArgs *d, const int *st, int nobj, unsigned blocks, hipStream_t s)
+{
+    if (passes) hipLaunchKernelGGL((k_group_tick<HT, true>), dim3(blocks), dim3(kTickThreads), 0, s, d, st, nobj);
+    else hipLaunchKernelGGL((k_group_tick<HT, false>), dim3(blocks), dim3(kTickThreads), 0, s, d, st, nobj);
+}
+
+hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, const int *d_starts, int nobj,
+                             unsigned blocks, hipStream_t s)
 {
     if (blocks == 0) return hipSuccess;
     switch (irtype) {
-    case IR_F32: hipLaunchKernelGGL((k_group_tick<IR_F32>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
-    case IR_F16: hipLaunchKernelGGL((k_group_tick<IR_F16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
-    case IR_BF16: hipLaunchKernelGGL((k_group_tick<IR_BF16>), dim3(blocks), dim3(kTickThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_F32: launch_group_t<IR_F32>(passes, d_descs, d_starts, nobj, blocks, s); break;
+    case IR_F16: launch_group_t<IR_F16>(passes, d_descs, d_starts, nobj, blocks, s); break;
+    case IR_BF16: launch_group_t<IR_BF16>(passes, d_descs, d_starts, nobj, blocks, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
