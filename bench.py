@@ -183,9 +183,9 @@ def bench_sweep(T, torch, ctx, counts, ticks=96, method=UNIFORM_METHOD):
         torch.cuda.empty_cache()
         lat = np.array(lat)
         p99 = float(np.percentile(lat, 99))
-        # every launch of the device path (with passes: pass and head-only launches)
+        # every launch of the device path (with passes: pass and head-only launches), per block
         ms_all = kt["head_ms"] + kt["tail_ms"]
-        tick_us = ms_all / max(1, kt["head_launches"] + kt["tail_launches"]) * 1e3
+        tick_us = ms_all / max(1, kt["blocks"]) * 1e3
         gbps = (kt["head_alg_bytes"] + kt["tail_alg_bytes"]) / (ms_all * 1e-3) / 1e9 if ms_all > 0 else 0.0
         row = {"channels": ch, "method": "nonuniform" if method == NONUNIFORM_METHOD else "uniform",
                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": p99,
@@ -291,9 +291,9 @@ def bench_precision(T, torch, ctx, channels=1024, taps=131072, warm=8, blocks=32
         conv.Process(x[:, warm * B:], y[:, warm * B:], blocks * B)
         kt = conv.ReadTiming()
         conv.Terminate()
-        # every launch (with multi-block passes: pass and head-only launches)
+        # every launch (with multi-block passes: pass and batched head launches), per block
         ms_all = kt["head_ms"] + kt["tail_ms"]
-        tick_us = ms_all / (kt["head_launches"] + kt["tail_launches"]) * 1e3
+        tick_us = ms_all / kt["blocks"] * 1e3
         gbps = (kt["head_alg_bytes"] + kt["tail_alg_bytes"]) / (ms_all * 1e-3) / 1e9
         outs[name] = y[:2].cpu().numpy().astype(np.float64)
         rows.append({"precision": name, "tick_us": tick_us, "alg_GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
@@ -737,16 +737,18 @@ def main():
     log("kernel timing pass")
     dt2, kt, info2, _ = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=True)
     kp = info2.get("pass_blocks", 1)
+    head_blocks = None
     if kp > 1:
-        # one launch per block: every kp-th runs the block's head and a pass
-        # computing the next kp blocks' partial sums (the dominant kernel,
-        # k_tick<256, 1, 0, 2>); the others run the head alone
+        # every kp-th block's launch runs its head and a pass computing the
+        # next kp blocks' partial sums (the dominant kernel, k_tick<256, 1, 0,
+        # 2>); the kp - 1 blocks between run their heads in one k_heads launch
         tick_us = kt["tail_ms"] / kt["tail_launches"] * 1e3
         achieved = kt["tail_alg_bytes"] / (kt["tail_ms"] * 1e-3) / 1e9
         alg_per_launch = kt["tail_alg_bytes"] / kt["tail_launches"]
         launches = kt["tail_launches"]
         kernel = f"k_tick (head + {kp}-block pass)"
         head_only_us = kt["head_ms"] / max(1, kt["head_launches"]) * 1e3
+        head_blocks = (kt["blocks"] - kt["tail_launches"]) / max(1, kt["head_launches"])
         prefix = "void tanx::k_tick<256, 1, 0, 2>"
     else:
         # one k_tick launch per block: the block's head and the next block's tail
@@ -757,14 +759,15 @@ def main():
         kernel = "k_tick (head + next tail)"
         head_only_us = None
         prefix = "void tanx::k_tick<256, 1, 0, 1>"
-    per_block_us = (kt["head_ms"] + kt["tail_ms"]) * 1e3 / (kt["head_launches"] + kt["tail_launches"])
+    per_block_us = (kt["head_ms"] + kt["tail_ms"]) * 1e3 / kt["blocks"]
     # the launch's grid: 64 head blocks + 64 x chunks tail blocks of 256 threads
     traffic = pmc_traffic((C + C * info2["tail_chunks"]) * 256, prefix)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": kernel,
                 "traffic_source": "profiles/pmc_traffic_k_tick.json" if traffic else None,
                 "alg_bytes_per_launch": alg_per_launch, "avg_launch_us": tick_us, "launches": launches,
-                "pass_blocks": kp, "head_only_launch_us": head_only_us, "avg_block_us": per_block_us,
+                "pass_blocks": kp, "head_launch_us": head_only_us, "head_blocks_per_launch": head_blocks,
+                "avg_block_us": per_block_us,
                 "note": ("64-channel working set (~67 MB) is resident in the 256 MB Infinity "
                          "Cache; see sweep for HBM-bound channel counts")}
     log(f"  tick kernel {tick_us:.2f} us/launch, {achieved:.0f} GB/s algorithmic; "

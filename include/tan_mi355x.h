@@ -261,6 +261,7 @@ typedef struct tanx_kernel_timing {
     uint64_t tail_launches;
     uint64_t head_alg_bytes;
     uint64_t tail_alg_bytes;
+    uint64_t blocks;          /* blocks whose heads the timed launches ran */
 } tanx_kernel_timing;
 int32_t tanx_conv_enable_timing(tanx_conv *conv, uint32_t enable);
 int32_t tanx_conv_read_timing(tanx_conv *conv, tanx_kernel_timing *out, uint32_t reset);

@@ -41,7 +41,8 @@ class WavInfo(ctypes.Structure):
 class KernelTiming(ctypes.Structure):
     _fields_ = [("head_ms", ctypes.c_double), ("tail_ms", ctypes.c_double),
                 ("head_launches", ctypes.c_uint64), ("tail_launches", ctypes.c_uint64),
-                ("head_alg_bytes", ctypes.c_uint64), ("tail_alg_bytes", ctypes.c_uint64)]
+                ("head_alg_bytes", ctypes.c_uint64), ("tail_alg_bytes", ctypes.c_uint64),
+                ("blocks", ctypes.c_uint64)]
 
 
 _SIGS = {

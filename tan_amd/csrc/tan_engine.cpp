@@ -324,6 +324,7 @@ struct tanx_conv {
     hipEvent_t desc_ev[kDescRing] = {};
     int desc_ring = 0;
     uint64_t tail_alg_bytes = 0, head_alg_bytes = 0;
+    uint64_t timed_blocks = 0;  // blocks whose heads the timed launches ran
 
     // time-domain objects: IR slots [3][C][len] and history [C][len] floats
     bool td = false;
@@ -964,6 +965,7 @@ static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const T
         if (fa) TANX_CHECK(launch_frame(cv->irtype, *fa, s, e1));
         kt.n++;
         alg_sum += alg_bytes;
+        if (a.nhead > 0) cv->timed_blocks++;
         return TANX_OK;
     }
     TANX_CHECK(launch_tick(cv->B, tail_nf, a, s));
@@ -1272,6 +1274,7 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
         kt.n++;
         // history window, filters, input and output
         cv->head_alg_bytes += (uint64_t)cv->C * (cv->len * sizeof(float) * (xfade ? 3 : 2) + 3 * cv->B * sizeof(float));
+        cv->timed_blocks++;
     } else {
         TANX_CHECK(launch_td(t, s));
     }
@@ -1290,10 +1293,67 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
     return TANX_OK;
 }
 
+// Head-only blocks of a multi-block pass window prepared but not launched
+// yet: consecutive blocks of one device Process call, launched together by
+// flush_heads before the next launch of the object.
+struct HeadQueue {
+    TickArgs a[kHeadBatch];
+    int n = 0;
+};
+
+static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
+{
+    const int n = q.n;
+    if (n == 0) return TANX_OK;
+    q.n = 0;
+    uint64_t hb = 0;
+    for (int j = 0; j < n; j++) hb += head_bytes(cv, 1) + near_bytes(cv, q.a[j]);
+    if (n == 1) return launch_timed(cv, cv->t_head, 1, q.a[0], hb, cv->head_alg_bytes);
+    const TickArgs &a0 = q.a[0];
+    HeadBatchArgs h{};
+    h.C = a0.C;
+    h.P = a0.P;
+    h.run = a0.run;
+    h.twM = a0.twM;
+    h.twN = a0.twN;
+    h.fdl = a0.fdl;
+    h.xprev = a0.xprev;
+    h.in_stride = a0.in_stride;
+    h.out_stride = a0.out_stride;
+    h.n = n;
+    // entries past the batch repeat its last block (the kernel loads them unused)
+    for (int j = 0; j < kHeadBatch; j++) {
+        const TickArgs &a = q.a[std::min(j, n - 1)];
+        h.xin[j] = a.xin;
+        h.yout[j] = a.yout;
+        h.H[j] = a.Ha;
+        h.S[j] = a.Sa;
+        h.T[j] = a.Ta;
+        h.slot[j] = a.slot;
+        h.tchunk[j] = a.tchunk;
+        h.d[j] = a.dA;
+    }
+    hipStream_t s = cv->ctx->proc;
+    if (cv->timing) {
+        KernelTimer &kt = cv->t_head;
+        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
+        TANX_CHECK(launch_heads(cv->B, cv->irtype, h, s, kt.start[kt.n], kt.stop[kt.n]));
+        kt.n++;
+        cv->head_alg_bytes += hb;
+        cv->timed_blocks += (uint64_t)n;
+        return TANX_OK;
+    }
+    TANX_CHECK(launch_heads(cv->B, cv->irtype, h, s));
+    return TANX_OK;
+}
+
 // One block launched directly; with_tail also runs the next block's tail in
-// the same launch.
+// the same launch.  With hq, a head-only block (multi-block passes, no
+// crossfade) joins the queue instead; any other block launches the queue
+// first.
 static int32_t process_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
-                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
+                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail,
+                             HeadQueue *hq = nullptr)
 {
     if (cv->td) return process_td_block(cv, din, in_stride, dout, out_stride, n, flags);
     TickArgs a;
@@ -1302,6 +1362,17 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
     bool needs_update = false, xfade = false, frame_due = false;
     int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, with_tail, a, &tail_nf,
                                &needs_update, &xfade, &fa, &frame_due);
+    if (hq) {
+        if (rc == TANX_OK && with_tail && cv->kp > 1 && cv->B <= kHeadBatchMaxB && !flags && a.nchunk == 0 &&
+            a.nchB == 0 && !a.nu && !xfade && !frame_due && !needs_update) {
+            hq->a[hq->n++] = a;
+            tail_done(cv);
+            return hq->n == kHeadBatch ? flush_heads(cv, *hq) : TANX_OK;
+        }
+        // blocks already advanced are launched before this one (or its error)
+        const int32_t frc = flush_heads(cv, *hq);
+        if (frc != TANX_OK) return frc;
+    }
     if (rc != TANX_OK) return rc;
     if (needs_update) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     uint64_t hb = head_bytes(cv, xfade ? 2 : 1) + frame_mac_bytes(cv, a.bHb ? 2 : 1) + near_bytes(cv, a);
@@ -1451,10 +1522,11 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     if (cv->use_graph && cv->fused && !cv->fmode && !cv->timing && !cv->nu && !cv->td && cv->kp == 1 && !flags &&
         nb >= 2)
         return process_device_graph(cv, d_in, in_stride, d_out, out_stride, nb, processed);
+    HeadQueue hq;
+    const bool fuse = cv->fused && !cv->fmode;
     for (size_t b = 0; b < nb; b++) {
-        const bool fuse = cv->fused && !cv->fmode;
         int32_t rc = process_block(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
-                                   (long long)out_stride, B, flags, fuse);
+                                   (long long)out_stride, B, flags, fuse, fuse ? &hq : nullptr);
         if (rc != TANX_OK) return rc;
         if (!cv->fmode && !fuse) {
             rc = enqueue_tail(cv);
@@ -1462,7 +1534,7 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
         }
         if (processed) *processed = (b + 1) * B;
     }
-    return TANX_OK;
+    return flush_heads(cv, hq);
 }
 
 /* ProcessDirect (ConvolutionImpl.cpp:719-762): time-domain objects only.
@@ -1913,10 +1985,12 @@ extern "C" int32_t tanx_conv_read_timing(tanx_conv *cv, tanx_kernel_timing *out,
     out->tail_ms = cv->t_tail.ms;
     out->tail_launches = cv->t_tail.count;
     out->tail_alg_bytes = cv->tail_alg_bytes;
+    out->blocks = cv->timed_blocks;
     if (reset) {
         cv->t_head.ms = cv->t_tail.ms = 0.0;
         cv->t_head.count = cv->t_tail.count = 0;
         cv->head_alg_bytes = cv->tail_alg_bytes = 0;
+        cv->timed_blocks = 0;
     }
     return TANX_OK;
 }

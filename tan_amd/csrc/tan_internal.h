@@ -201,6 +201,31 @@ hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hip
 hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, const int *d_starts, int nobj,
                              unsigned blocks, hipStream_t s);
 constexpr int kGroupMaxB = 512;
+// Consecutive head-only blocks of a uniform object with multi-block passes,
+// launched together (k_heads, one workgroup per channel): block j of the
+// batch follows block j-1; it adds its partial sums T[j] (tchunk[j] rows per
+// channel) and partitions 0..d[j] of filter H[j] itself, writes its spectrum
+// to FDL slot[j], and the last block's input becomes xprev.  No crossfade,
+// tail or frame role; B <= kHeadBatchMaxB.
+constexpr int kHeadBatch = kPassMax;
+constexpr int kHeadBatchMaxB = 512;
+struct HeadBatchArgs {
+    int C, P;
+    const uint8_t *run;
+    const float2 *twM, *twN;
+    float2 *fdl;
+    float *xprev;
+    long long in_stride, out_stride;
+    int n;
+    const float *xin[kHeadBatch];
+    float *yout[kHeadBatch];
+    const void *H[kHeadBatch];
+    const float *S[kHeadBatch];
+    const float2 *T[kHeadBatch];
+    int slot[kHeadBatch], tchunk[kHeadBatch], d[kHeadBatch];
+};
+hipError_t launch_heads(int B, int irtype, const HeadBatchArgs &h, hipStream_t s, hipEvent_t ev0 = nullptr,
+                        hipEvent_t ev1 = nullptr);
 // k_tick reading its arguments from d_descs[k] (graph replay; kn must be 1)
 hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, unsigned blocks, hipStream_t s);
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);

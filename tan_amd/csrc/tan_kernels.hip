@@ -32,23 +32,32 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)  // a * conj(b)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// Stockham autosort FFT of M points held in LDS buffer `a`; `b` is scratch of
-// the same size.  Returns the buffer that holds the result (natural order).
-// Radix-4 stages with a trailing radix-2 stage when log2(M) is odd.  Each
-// stage: butterfly q reads a[q + m*M/R], m < R, multiplies by W_{pR}^{m*k}
-// (k = q mod p), and writes b[(q - k)*R + k + m*p].  INV selects the
-// unscaled inverse.  All threads of the block must call it.
+// Stockham autosort FFTs of nb independent M-point sequences held in LDS:
+// sequence j at a + j*M, with scratch b + j*M.  Returns the base (a or b)
+// that holds the results (natural order).  Radix-4 stages with a trailing
+// radix-2 stage when log2(M) is odd.  Each stage: butterfly q of a sequence
+// reads a[q + m*M/R], m < R, multiplies by W_{pR}^{m*k} (k = q mod p), and
+// writes b[(q - k)*R + k + m*p]; the butterflies of all nb sequences are
+// spread over the block's threads, so nb sequences take the stages (and
+// barriers) of one.  INV selects the unscaled inverse.  All threads of the
+// block must call it.
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
 template <int M, bool INV>
-__device__ float2 *fft_lds(float2 *a, float2 *b, const float2 *__restrict__ tw)
+__device__ float2 *fft_lds_n(float2 *a, float2 *b, int nb, const float2 *__restrict__ tw)
 {
+    constexpr int LQ = ilog2(M / 4), LH = ilog2(M / 2);  // butterflies per sequence: radix 4, radix 2
     const int tid = threadIdx.x, nt = blockDim.x;
     int p = 1;
 #pragma unroll 1
     for (; p * 4 <= M; p *= 4) {
         const int tstride = M / (4 * p);
-        for (int q = tid; q < M / 4; q += nt) {
+        for (int e = tid; e < (nb << LQ); e += nt) {
+            const int q = e & (M / 4 - 1);
+            const float2 *aj = a + (e >> LQ) * M;
+            float2 *bj = b + (e >> LQ) * M;
             const int k = q & (p - 1);
-            float2 x0 = a[q], x1 = a[q + M / 4], x2 = a[q + M / 2], x3 = a[q + 3 * M / 4];
+            float2 x0 = aj[q], x1 = aj[q + M / 4], x2 = aj[q + M / 2], x3 = aj[q + 3 * M / 4];
             if (p > 1) {
                 const int t = k * tstride;
                 float2 w1 = tw[t], w2 = tw[2 * t], w3 = tw[3 * t];
@@ -62,27 +71,37 @@ __device__ float2 *fft_lds(float2 *a, float2 *b, const float2 *__restrict__ tw)
             // forward: W_4 = -i ; inverse: +i
             const float2 jd13 = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
             const int j = (q - k) * 4 + k;
-            b[j] = cadd(s02, s13);
-            b[j + p] = cadd(d02, jd13);
-            b[j + 2 * p] = csub(s02, s13);
-            b[j + 3 * p] = csub(d02, jd13);
+            bj[j] = cadd(s02, s13);
+            bj[j + p] = cadd(d02, jd13);
+            bj[j + 2 * p] = csub(s02, s13);
+            bj[j + 3 * p] = csub(d02, jd13);
         }
         __syncthreads();
         float2 *t = a; a = b; b = t;
     }
     if (p < M) {  // radix-2, p = M/2
-        for (int q = tid; q < M / 2; q += nt) {
-            float2 x0 = a[q], x1 = a[q + M / 2];
+        for (int e = tid; e < (nb << LH); e += nt) {
+            const int q = e & (M / 2 - 1);
+            const float2 *aj = a + (e >> LH) * M;
+            float2 *bj = b + (e >> LH) * M;
+            float2 x0 = aj[q], x1 = aj[q + M / 2];
             float2 w = tw[q];
             if (INV) w.y = -w.y;
             x1 = cmul(x1, w);
-            b[q] = cadd(x0, x1);
-            b[q + M / 2] = csub(x0, x1);
+            bj[q] = cadd(x0, x1);
+            bj[q + M / 2] = csub(x0, x1);
         }
         __syncthreads();
         a = b;
     }
     return a;
+}
+
+// One M-point FFT (see fft_lds_n).
+template <int M, bool INV>
+__device__ __forceinline__ float2 *fft_lds(float2 *a, float2 *b, const float2 *__restrict__ tw)
+{
+    return fft_lds_n<M, INV>(a, b, 1, tw);
 }
 
 // Real-FFT post-processing: Z = FFT_M(z), z[n] = x[2n] + i x[2n+1].
@@ -226,6 +245,26 @@ __device__ __forceinline__ void mac4(float4 &acc, float4 x, float4 h, float s, b
     acc.w = fmaf(x.w, h.z, acc.w);
 }
 
+// Sum of a head's partial-sum rows T[j*M + k], j < tchunk, loaded
+// kHeadChunkBatch rows at a time (every load of a batch in flight at once).
+template <int M>
+__device__ __forceinline__ float2 sum_partials(const float2 *T, int tchunk, int k)
+{
+    float2 s = make_float2(0.f, 0.f);
+    for (int j = 0; j < tchunk; j += kHeadChunkBatch) {
+        float2 t[kHeadChunkBatch];
+#pragma unroll
+        for (int u = 0; u < kHeadChunkBatch; u++)
+            t[u] = (j + u < tchunk) ? T[(size_t)(j + u) * M + k] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kHeadChunkBatch; u++) {
+            s.x += t[u].x;
+            s.y += t[u].y;
+        }
+    }
+    return s;
+}
+
 template <int M>
 struct HeadSmem {
     float2 s0[M], s1[M], sx[M], sa[M], tw[M], twn[M];
@@ -269,29 +308,18 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
             const int k = tid + q * NT;
             float2 s = make_float2(0.f, 0.f);
             if (k < M) {
-                // every partial-sum load in flight at once (one memory round trip)
-                int j = 0;
-                for (; j < a.tchunk; j += kHeadChunkBatch) {
-                    float2 t[kHeadChunkBatch];
-#pragma unroll
-                    for (int u = 0; u < kHeadChunkBatch; u++)
-                        t[u] = (j + u < a.tchunk) ? T[(size_t)(j + u) * M + k] : make_float2(0.f, 0.f);
-#pragma unroll
-                    for (int u = 0; u < kHeadChunkBatch; u++) {
-                        s.x += t[u].x;
-                        s.y += t[u].y;
-                    }
-                }
+                s = sum_partials<M>(T, a.tchunk, k);
                 if (dmax == 1) {
                     if (a.P > 1) {
                         const float2 h1 = IrRow<HT>::get1(H1row, k);
                         mac2(s, Fprev[k], make_float2(h1.x * s1, h1.y * s1), k == 0);
                     }
                 } else {
-                    // partitions 1..d: the blocks since the pass that computed the partials
+                    // partitions d..1 (descending, the order k_heads keeps): the
+                    // blocks since the pass that computed the partials
                     const int d = min(f == 0 ? a.dA : a.dB, a.P - 1);
 #pragma unroll
-                    for (int jn = 1; jn <= kPassMax; jn++) {
+                    for (int jn = kPassMax; jn >= 1; jn--) {
                         if (jn <= d) {
                             const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
                             const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
@@ -797,6 +825,161 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
     case 256: tick_body<256, HT, KP>(a, lb, smem); break;
     case 512: tick_body<512, HT, KP>(a, lb, smem); break;
     default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched head-only blocks (HeadBatchArgs): one workgroup per channel runs
+// the heads of n consecutive blocks.  Their forward and inverse FFTs are
+// independent and share the stages (and barriers) of one; only block j's
+// multiply-accumulate reads the spectra of blocks j-1, ..., 0 of the batch
+// (from LDS) and of the blocks before the batch (from the FDL).  Partitions
+// are added in the order of head_role (partials, d..1, then 0), so a batch
+// gives the same bits as n single-block launches.
+// ---------------------------------------------------------------------------
+constexpr int kHeadsChunkBatch = 8;  // partial-sum rows per block loaded together by k_heads
+
+template <int M>
+struct HeadsSmem {
+    float2 a[kHeadBatch][M], b[kHeadBatch][M], x[kHeadBatch][M], tw[M], twn[M];
+};
+
+template <int M, int HT>
+__global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
+{
+    constexpr int NT = kTickThreads;
+    constexpr int KPT = M >= NT ? M / NT : 1;  // bins per thread
+    constexpr int LM = ilog2(M);
+    constexpr float kScale = 1.0f / (2.0f * (float)M);
+    __shared__ HeadsSmem<M> sm;
+    const int c = blockIdx.x, tid = threadIdx.x, n = h.n;
+    if (!h.run[c]) return;
+
+    // frames [x_{t-1}, x_t] of every block as complex pairs, and the twiddles
+    for (int e = tid; e < (n << LM); e += NT) {
+        const int j = e >> LM, m = e & (M - 1);
+        const float *prev = j == 0 ? h.xprev + (size_t)c * M : h.xin[j - 1] + (size_t)c * h.in_stride;
+        const float *src = m < M / 2 ? prev + 2 * m : h.xin[j] + (size_t)c * h.in_stride + 2 * (m - M / 2);
+        sm.a[j][m] = make_float2(src[0], src[1]);
+    }
+    for (int m = tid; m < M; m += NT) {
+        sm.tw[m] = h.twM[m];
+        sm.twn[m] = h.twN[m];
+    }
+    // Every global load of the batch is issued before any is used (one
+    // memory round trip): entries j >= n repeat block n-1 (valid addresses,
+    // results unused), and the near partitions load from clamped indices.
+    // Block j: the partial sums and the partitions whose FDL rows precede
+    // the batch (i = d_j .. j+1) go to acc; the IR rows of partitions 0..j
+    // (rows of the batch, d_j >= j + 1) to hn.
+    constexpr int KP = kHeadBatch;
+    float2 acc[KP][KPT], hn[KP][KP][KPT];
+    int tmax = 0;
+#pragma unroll
+    for (int j = 0; j < KP; j++) tmax = max(tmax, h.tchunk[j]);
+#pragma unroll
+    for (int q = 0; q < KPT; q++)
+#pragma unroll
+        for (int j = 0; j < KP; j++) acc[j][q] = make_float2(0.f, 0.f);
+    for (int u0 = 0; u0 < tmax; u0 += kHeadsChunkBatch) {
+#pragma unroll
+        for (int q = 0; q < KPT; q++) {
+            const int k = tid + q * NT;
+            if (k >= M) continue;
+            float2 t[KP][kHeadsChunkBatch];
+#pragma unroll
+            for (int j = 0; j < KP; j++) {
+                const float2 *T = h.T[j] + ((size_t)c * h.tchunk[j] + u0) * M + k;
+#pragma unroll
+                for (int u = 0; u < kHeadsChunkBatch; u++)
+                    t[j][u] = u0 + u < h.tchunk[j] ? T[(size_t)u * M] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < KP; j++)
+#pragma unroll
+                for (int u = 0; u < kHeadsChunkBatch; u++) {
+                    acc[j][q].x += t[j][u].x;
+                    acc[j][q].y += t[j][u].y;
+                }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+        const int k = tid + q * NT;
+        if (k >= M) continue;
+        float2 fv[KP][KP + 1], hv[KP][KP + 1];
+#pragma unroll
+        for (int j = 0; j < KP; j++) {
+            const int d = min(h.d[j], h.P - 1);
+#pragma unroll
+            for (int i = KP; i > j; i--) {
+                const int ii = max(1, min(i, d));
+                const float si = ir_scale<HT>(h.S[j], c, h.P, ii);
+                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, ii, M), k);
+                fv[j][i] = h.fdl[((size_t)c * h.P + ((h.slot[j] - ii) & (h.P - 1))) * M + k];
+                hv[j][i] = make_float2(hr.x * si, hr.y * si);
+            }
+#pragma unroll
+            for (int i = 0; i <= j; i++) {
+                const float si = ir_scale<HT>(h.S[j], c, h.P, i);
+                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, i, M), k);
+                hn[j][i][q] = make_float2(hr.x * si, hr.y * si);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KP; j++) {
+            const int d = min(h.d[j], h.P - 1);
+#pragma unroll
+            for (int i = KP; i > j; i--)
+                if (i <= d) mac2(acc[j][q], fv[j][i], hv[j][i], k == 0);
+        }
+    }
+    __syncthreads();
+    {
+        // the batch's last block becomes the previous one of the next launch
+        const float *xl = h.xin[n - 1] + (size_t)c * h.in_stride;
+        float *xpw = h.xprev + (size_t)c * M;
+        for (int m = tid; m < M; m += NT) xpw[m] = xl[m];
+    }
+
+    const float2 *Z = fft_lds_n<M, false>(&sm.a[0][0], &sm.b[0][0], n, sm.tw);
+    for (int e = tid; e < (n << LM); e += NT) {
+        const int j = e >> LM, k = e & (M - 1);
+        const float2 X = r2c_post<M>(Z + j * M, k, sm.twn);
+        sm.x[j][k] = X;
+        h.fdl[((size_t)c * h.P + h.slot[j]) * M + k] = X;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kHeadBatch; j++) {
+        if (j >= n) continue;
+        const int d = min(h.d[j], h.P - 1);
+#pragma unroll
+        for (int q = 0; q < KPT; q++) {
+            const int k = tid + q * NT;
+            if (k < M) {
+                float2 v = acc[j][q];
+#pragma unroll
+                for (int i = KP - 1; i >= 0; i--)
+                    if (i <= j && i <= d) mac2(v, sm.x[j - i][k], hn[j][i][q], k == 0);
+                sm.a[j][k] = v;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < (n << LM); e += NT) {
+        const int j = e >> LM, k = e & (M - 1);
+        sm.b[j][k] = c2r_pre<M>(&sm.a[j][0], k, sm.twn);
+    }
+    __syncthreads();
+    const float2 *z = fft_lds_n<M, true>(&sm.b[0][0], &sm.a[0][0], n, sm.tw);
+    // second half of each block's 2M output samples: z[M/2 .. M)
+    for (int e = tid; e < (n << (LM - 1)); e += NT) {
+        const int j = e >> (LM - 1), m = e & (M / 2 - 1);
+        const float2 v = z[j * M + M / 2 + m];
+        float *yo = h.yout[j] + (size_t)c * h.out_stride;
+        yo[2 * m] = v.x * kScale;
+        yo[2 * m + 1] = v.y * kScale;
     }
 }
 
@@ -1352,6 +1535,30 @@ hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, c
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int HT>
+static hipError_t launch_heads_t(int B, const HeadBatchArgs &h, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    switch (B) {
+    case 64: hipExtLaunchKernelGGL((k_heads<64, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
+    case 128: hipExtLaunchKernelGGL((k_heads<128, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
+    case 256: hipExtLaunchKernelGGL((k_heads<256, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
+    case 512: hipExtLaunchKernelGGL((k_heads<512, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_heads(int B, int irtype, const HeadBatchArgs &h, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (h.n < 1 || h.n > kHeadBatch || B > kHeadBatchMaxB) return hipErrorInvalidValue;
+    switch (irtype) {
+    case IR_F32: return launch_heads_t<IR_F32>(B, h, s, ev0, ev1);
+    case IR_F16: return launch_heads_t<IR_F16>(B, h, s, ev0, ev1);
+    case IR_BF16: return launch_heads_t<IR_BF16>(B, h, s, ev0, ev1);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // partition sizes of uniform blocks (64..1024) and of frames (128..4096)

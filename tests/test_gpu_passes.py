@@ -149,3 +149,80 @@ def test_default_pass_length(gpu_ctx):
     conv.Terminate()
     conv = T.TANConvolution(gpu_ctx, pass_blocks=5)
     assert conv.InitGpu(UNIFORM, 65536, 256, 2) == T.AMF_INVALID_ARG
+
+
+def _run_calls(ctx, L, B, C, calls, updates, K, precision=T.TAN_IR_PRECISION_FP32, seed=700):
+    """Process in calls of calls[i] blocks (device memory), with
+    updates[b] issued before the call that starts at block b."""
+    import torch
+    conv = T.TANConvolution(ctx, ir_precision=precision, pass_blocks=K)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    nb = sum(calls)
+    x = T.signals.input_bank(C, nb * B, base_seed=seed)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    b = 0
+    for n in calls:
+        if b in updates:
+            assert conv.UpdateResponseTD(updates[b], L, None, BLOCKING) == T.AMF_OK
+        rc, done = conv.Process(xd[:, b * B:], yd[:, b * B:], n * B)
+        assert rc == T.AMF_OK and done == n * B
+        b += n
+    conv.Synchronize()
+    y = yd.cpu().numpy()
+    conv.Terminate()
+    return x, y
+
+
+CALLS = [1, 3, 7, 2, 13, 5, 1, 1, 9, 4, 6, 11, 2, 8]
+
+
+@pytest.mark.parametrize("K,B,precision", [(4, 256, T.TAN_IR_PRECISION_FP32), (3, 128, T.TAN_IR_PRECISION_FP32),
+                                           (2, 512, T.TAN_IR_PRECISION_FP32), (4, 64, T.TAN_IR_PRECISION_FP32),
+                                           (4, 256, T.TAN_IR_PRECISION_FP16)])
+def test_multi_block_calls_batch_heads(gpu_ctx, K, B, precision):
+    """Calls of several blocks launch the head-only blocks of a pass window
+    together: the output is bit for bit that of one block per call, and the
+    oracle's, with updates (crossfades, catch-up passes) between calls."""
+    L, C = 32 * B, 3
+    nb = sum(CALLS)
+    starts = np.cumsum([0] + CALLS[:-1])
+    ups = _updates(C, L, [int(starts[i]) for i in (0, 2, 3, 5, 6, 7, 10, 12)], seed=601)
+    print(f"batched heads K={K} B={B} precision={precision}", flush=True)
+    x, y = _run_calls(gpu_ctx, L, B, C, CALLS, ups, K, precision)
+    _, y1 = _run_calls(gpu_ctx, L, B, C, [1] * nb, ups, K, precision)
+    np.testing.assert_array_equal(y, y1)
+    if precision == T.TAN_IR_PRECISION_FP32:
+        orc = O.OracleConvolution(L, B, C)
+        yo = np.zeros_like(x)
+        for b in range(nb):
+            if b in ups:
+                assert orc.update_td(ups[b]) == O.AMF_OK
+            rco, out_o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
+            yo[:, b * B:(b + 1) * B] = out_o
+        err = np.abs(y.astype(np.float64) - yo).max()
+        assert err <= 1e-5, err
+        for c in range(C):
+            assert snr_db(yo[c], y[c]) >= 120.0
+
+
+def test_kernel_timing_counts_blocks(gpu_ctx):
+    """With kernel timing on, a batched call is timed too: the timed
+    launches cover every block (pass launches plus head launches, one of
+    which may carry several blocks) and their algorithmic bytes."""
+    import torch
+    L, B, C, nb = 16384, 256, 4, 40
+    conv = T.TANConvolution(gpu_ctx, pass_blocks=4)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    assert conv.UpdateResponseTD(T.signals.ir_bank(C, L, base_seed=9), L, None, BLOCKING) == T.AMF_OK
+    x = torch.from_numpy(T.signals.input_bank(C, nb * B, base_seed=10)).cuda()
+    y = torch.zeros_like(x)
+    assert conv.EnableTiming(True) == T.AMF_OK
+    conv.ReadTiming(reset=True)
+    rc, done = conv.Process(x, y, nb * B)
+    assert rc == T.AMF_OK and done == nb * B
+    kt = conv.ReadTiming(reset=True)
+    conv.Terminate()
+    assert kt["blocks"] == nb
+    assert kt["head_launches"] + kt["tail_launches"] < nb  # head-only blocks share launches
+    assert kt["head_ms"] > 0 and kt["tail_ms"] > 0 and kt["head_alg_bytes"] > 0
