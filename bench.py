@@ -739,17 +739,18 @@ def main():
     kp = info2.get("pass_blocks", 1)
     head_blocks = None
     if kp > 1:
-        # every kp-th block's launch runs its head and a pass computing the
-        # next kp blocks' partial sums (the dominant kernel, k_tick<256, 1, 0,
-        # 2>); the kp - 1 blocks between run their heads in one k_heads launch
+        # a launch runs up to kp heads and the pass computing the next kp
+        # blocks' partial sums (the dominant kernel, k_window<256, 1, 0>;
+        # k_tick<256, 1, 0, 2> where a launch has a single head); launches
+        # with heads alone are timed as head launches
         tick_us = kt["tail_ms"] / kt["tail_launches"] * 1e3
         achieved = kt["tail_alg_bytes"] / (kt["tail_ms"] * 1e-3) / 1e9
         alg_per_launch = kt["tail_alg_bytes"] / kt["tail_launches"]
         launches = kt["tail_launches"]
-        kernel = f"k_tick (head + {kp}-block pass)"
-        head_only_us = kt["head_ms"] / max(1, kt["head_launches"]) * 1e3
-        head_blocks = (kt["blocks"] - kt["tail_launches"]) / max(1, kt["head_launches"])
-        prefix = "void tanx::k_tick<256, 1, 0, 2>"
+        kernel = f"k_window (heads + {kp}-block pass)"
+        head_only_us = kt["head_ms"] / kt["head_launches"] * 1e3 if kt["head_launches"] else None
+        head_blocks = kt["blocks"] / (kt["head_launches"] + kt["tail_launches"])
+        prefix = "void tanx::k_window<256, 1, 0>"
     else:
         # one k_tick launch per block: the block's head and the next block's tail
         tick_us = kt["head_ms"] / kt["head_launches"] * 1e3
@@ -766,7 +767,7 @@ def main():
                 "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": kernel,
                 "traffic_source": "profiles/pmc_traffic_k_tick.json" if traffic else None,
                 "alg_bytes_per_launch": alg_per_launch, "avg_launch_us": tick_us, "launches": launches,
-                "pass_blocks": kp, "head_launch_us": head_only_us, "head_blocks_per_launch": head_blocks,
+                "pass_blocks": kp, "head_launch_us": head_only_us, "blocks_per_launch": head_blocks,
                 "avg_block_us": per_block_us,
                 "note": ("64-channel working set (~67 MB) is resident in the 256 MB Infinity "
                          "Cache; see sweep for HBM-bound channel counts")}

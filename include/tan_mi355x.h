@@ -88,7 +88,11 @@ enum {
  * convolution (process) queues.  `version` must have major 1
  * (TANContextImpl.cpp:114-120), else TANX_TAN_UNSUPPORTED_VERSION. */
 int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out);
-/* TANContext::Terminate (TANContextImpl.cpp:158-179). */
+/* TANContext::Terminate (TANContextImpl.cpp:158-179): releases the caller's
+ * handle.  Convolution objects and mixers made on the context hold it too,
+ * as the reference's reference-counted objects do, so the context's streams
+ * and buffers go when the last of them is destroyed; the handle itself must
+ * not be used again. */
 int32_t tanx_ctx_destroy(tanx_ctx *ctx);
 int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
 /* The context's process stream (a hipStream_t), on which every Process and

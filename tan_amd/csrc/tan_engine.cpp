@@ -6,6 +6,7 @@
 #include "../../include/tan_mi355x.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -59,6 +60,11 @@ inline bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
 // context
 // ---------------------------------------------------------------------------
 struct tanx_ctx {
+    // the caller's handle and every object made on the context hold a
+    // reference (TANContext is reference counted in the reference's AMF
+    // object model): tanx_ctx_destroy drops the caller's, and the context
+    // goes with the last one
+    std::atomic<int> refs{1};
     int device = 0;
     // group launches: descriptors + block offsets, pinned ring and device copy
     std::mutex group_mtx;
@@ -149,9 +155,8 @@ extern "C" int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **
     return TANX_OK;
 }
 
-extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
+static void ctx_free(tanx_ctx *ctx)
 {
-    if (!ctx) return TANX_INVALID_POINTER;
     {
         DeviceGuard g(ctx->device);
         if (ctx->proc) { (void)hipStreamSynchronize(ctx->proc); (void)hipStreamDestroy(ctx->proc); }
@@ -176,6 +181,18 @@ extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
         if (ctx->room_out_host) (void)hipHostFree(ctx->room_out_host);
     }
     delete ctx;
+}
+
+static void ctx_retain(tanx_ctx *ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
+static void ctx_release(tanx_ctx *ctx)
+{
+    if (ctx->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) ctx_free(ctx);
+}
+
+extern "C" int32_t tanx_ctx_destroy(tanx_ctx *ctx)
+{
+    if (!ctx) return TANX_INVALID_POINTER;
+    ctx_release(ctx);
     return TANX_OK;
 }
 
@@ -247,18 +264,19 @@ struct tanx_conv {
     int tail_nchunk = 1;  // chunking of the partial sums the next head reads
     int tread = 0;        // partial-sum buffer the next head reads
     bool tail_due = false;  // a head ran since the last tail
-    // Multi-block tail passes (kp > 1): the launch of block wo computed the
-    // partial sums of blocks wo+1 .. wo+kp (the window, parity wpar), each
-    // over partitions k+1.. for block wo+k, and block wo+k's head adds
-    // partitions 0..k itself.  A switch armed inside a window gets a
-    // catch-up pass for its new filter; org[k-1][f] is the block whose
-    // launch computed window slot k's sums for filter slot f (0: the
-    // block's filter, 1: the crossfade's second filter).
+    // Multi-block tail passes (kp > 1): a launch running the heads of blocks
+    // [w, e) also runs, when block e has no partial sums yet (or a switch
+    // armed at block e-1 made the old filter's stale), a pass computing the
+    // sums of blocks e .. e+kp-1 from FDL rows before w.  The sums are kept
+    // per block in a ring of 2*kp slots (block b: slot b mod 2kp), per
+    // filter f (0: the block's filter, 1: the crossfade's second filter);
+    // porg is the first head block of the launch that computed them (block
+    // b's head adds partitions 0..b-porg itself), pchunk their chunk count,
+    // covered the last block with sums.
     int kp = 1;
-    long long wo = -1;
-    int wpar = 1;
-    int wchunk[2] = {1, 1}, wpc[2] = {1, 1};
-    long long org[kPassMax][2] = {};
+    long long covered = 0;
+    long long porg[2 * kPassMax][2] = {};
+    int pchunk[2 * kPassMax][2] = {};
     bool fused = true;    // device path: head and next tail in one launch
     bool fixed_chunks = false;
 
@@ -361,10 +379,9 @@ struct tanx_conv {
     float *scale_ptr(int s) const { return S + (size_t)s * C * P; }
     // partial sums of buffer `par` for filter f (0 = current, 1 = next)
     float2 *tail_ptr(int par, int f) const { return T + ((size_t)par * 2 + f) * C * maxchunk * B; }
-    float2 *tail_ptrk(int par, int k, int f) const
-    {
-        return T + (((size_t)par * kp + k) * 2 + f) * C * maxchunk * B;
-    }
+    // multi-block passes: ring slot of block b's partial sums
+    int pring(long long b) const { return (int)(b % (2 * kp)); }
+    float2 *tail_slot(long long b, int f) const { return T + ((size_t)pring(b) * 2 + f) * C * maxchunk * B; }
 };
 
 static void conv_free(tanx_conv *cv)
@@ -393,7 +410,9 @@ static void conv_free(tanx_conv *cv)
         if (cv->h_desc) (void)hipHostFree(cv->h_desc);
         for (auto &e : cv->desc_ev) if (e) (void)hipEventDestroy(e);
     }
+    tanx_ctx *ctx = cv->ctx;
     delete cv;
+    if (ctx) ctx_release(ctx);
 }
 
 static int auto_chunks(int C, int P)
@@ -627,6 +646,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     tanx_conv *cv = new (std::nothrow) tanx_conv();
     if (!cv) return TANX_OUT_OF_MEMORY;
     cv->ctx = ctx;
+    ctx_retain(ctx);
     cv->method = method;
     cv->fmode = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
     cv->irtype = sh.irtype;
@@ -649,8 +669,10 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
     cv->kp = sh.kp;
-    cv->wchunk[0] = cv->wchunk[1] = cv->nchunk;
-    cv->wpc[0] = cv->wpc[1] = cv->pc;
+    for (auto &r : cv->pchunk) r[0] = r[1] = cv->nchunk;
+    // block 0: a silent history, its (zero) sums start at partition 2
+    cv->covered = 0;
+    cv->porg[0][0] = -1;
 
     const size_t C = cv->C, B = cv->B;
     const ConvLayout l = conv_layout(sh);
@@ -734,12 +756,11 @@ static int32_t flush_channel(tanx_conv *cv, int ch)
     if (e == hipSuccess) e = launch_zero(cv->xprev + (size_t)ch * B, B * sizeof(float), s);
     // partial sums are laid out [C][tail_nchunk][B]: the next head reads them
     if (cv->kp > 1) {
-        // every window slot: sums computed ahead hold the history being dropped
-        for (int par = 0; par < 2; par++)
-            for (int k = 0; k < cv->kp; k++)
-                for (int f = 0; f < 2 && e == hipSuccess; f++)
-                    e = launch_zero(cv->tail_ptrk(par, k, f) + (size_t)ch * cv->wchunk[par] * B,
-                                    (size_t)cv->wchunk[par] * B * sizeof(float2), s);
+        // every ring slot: sums computed ahead hold the history being dropped
+        for (int r = 0; r < 2 * cv->kp; r++)
+            for (int f = 0; f < 2 && e == hipSuccess; f++)
+                e = launch_zero(cv->tail_slot(r, f) + (size_t)ch * cv->pchunk[r][f] * B,
+                                (size_t)cv->pchunk[r][f] * B * sizeof(float2), s);
     } else {
         for (int f = 0; f < 2 && e == hipSuccess; f++)
             e = launch_zero(cv->tail_ptr(cv->tread, f) + (size_t)ch * cv->tail_nchunk * B,
@@ -836,17 +857,23 @@ static TickArgs tick_common(tanx_conv *cv)
     a.twN = tw_table(cv->ctx->tw, lB + 1);
     a.fdl = cv->F;
     a.kn = 1;
+    a.kgap = 1;
     a.dA = a.dB = 1;
     return a;
 }
 
 // Tail role arguments for the block after the current one: partitions
-// 2..P-1, with both filters when that block is the crossfade block.
-static int fill_tail_passes(tanx_conv *cv, TickArgs &a);
+// 2..P-1, with both filters when that block is the crossfade block.  With
+// multi-block passes, the pass (if any) of a launch whose heads are the
+// current block alone.
+static int fill_pass(tanx_conv *cv, long long w, long long e, TickArgs &a);
 
 static int fill_tail(tanx_conv *cv, TickArgs &a)
 {
-    if (cv->kp > 1) return fill_tail_passes(cv, a);
+    if (cv->kp > 1) {
+        const long long e = (long long)cv->ticks;
+        return std::max(1, fill_pass(cv, e - 1, e, a));
+    }
     // cv->ticks is the index of the block this tail is for
     const int nf = (cv->delayed && cv->ticks == cv->switch_at) ? 2 : 1;
     a.nchunk = cv->nchunk;
@@ -861,81 +888,60 @@ static int fill_tail(tanx_conv *cv, TickArgs &a)
     return nf;
 }
 
-// Tail role of a launch of an object with multi-block passes (cv->ticks is
-// the block after the one whose head the launch runs):
-//  - a pass when the current window is used up: the sums of the next kp
-//    blocks with the filter each of them will use (an armed switch inside
-//    the window: the old filter up to the crossfade block, the new one from
-//    it on, both for that block);
-//  - a catch-up when a switch was armed at this block inside the window:
-//    the new filter's sums for the window's remaining blocks (the
-//    crossfade block's second filter, and the following blocks' only one);
-//  - otherwise none (a head-only launch).
-// Returns the number of filters the launch's tail blocks read.
-static int fill_tail_passes(tanx_conv *cv, TickArgs &a)
+// Pass of a launch running the heads of blocks [w, e) (cv->ticks == e, the
+// switch state that after block e-1): when block e has no partial sums yet,
+// or a switch armed at block e-1 made the sums computed with the old filter
+// stale, the sums of blocks e .. e+kp-1 from FDL rows before w (output k:
+// partitions >= e - w + k + 1), with the filter each block will use: the
+// old one before an armed switch, both at its crossfade block, the new one
+// after it.  Returns the number of filters the pass reads (0: no pass).
+static int fill_pass(tanx_conv *cv, long long w, long long e, TickArgs &a)
 {
-    const long long hb = (long long)cv->ticks - 1;  // the block whose head this launch runs
-    const bool pass = cv->wo < 0 || hb == cv->wo + cv->kp;
-    const bool armed = cv->delayed && (long long)cv->switch_at == hb + 1;
-    a.tbase = cv->pos;
+    const long long sw = cv->delayed ? (long long)cv->switch_at : -1;
+    const bool stale = sw >= e && sw <= cv->covered;
     a.tHb = nullptr;
     a.tSb = nullptr;
     for (int k = 0; k < kPassMax; k++) a.kTA[k] = a.kTB[k] = nullptr;
-    if (!pass && !armed) {
+    a.tTa = a.tTb = nullptr;
+    if (cv->covered >= e && !stale) {
         a.nchunk = 0;
         a.kn = 1;
-        return 1;
+        return 0;
     }
-    if (pass) {
-        cv->wo = hb;
-        cv->wpar ^= 1;
-        cv->wchunk[cv->wpar] = cv->nchunk;
-        cv->wpc[cv->wpar] = cv->pc;
-        const long long s_rel = (cv->delayed && (long long)cv->switch_at > hb) ? (long long)cv->switch_at - hb : -1;
-        a.nchunk = cv->nchunk;
-        a.pc = cv->pc;
-        a.kn = cv->kp;
-        a.tHa = cv->slot_ptr(cv->cur);
-        a.tSa = cv->scale_ptr(cv->cur);
-        bool new_filter = false;
-        for (int k = 1; k <= cv->kp; k++) {
-            float2 *own = cv->tail_ptrk(cv->wpar, k - 1, 0), *second = cv->tail_ptrk(cv->wpar, k - 1, 1);
-            if (s_rel < 0 || k < s_rel) {
-                a.kTA[k - 1] = own;
-                cv->org[k - 1][0] = hb;
-            } else if (k == s_rel) {
-                a.kTA[k - 1] = own;
-                a.kTB[k - 1] = second;
-                cv->org[k - 1][0] = cv->org[k - 1][1] = hb;
-                new_filter = true;
-            } else {
-                a.kTB[k - 1] = own;  // after the switch the new filter is the block's own
-                cv->org[k - 1][0] = hb;
-                new_filter = true;
-            }
+    const long long s_rel = (sw >= e && sw < e + cv->kp) ? sw - e : -1;
+    a.nchunk = cv->nchunk;
+    a.pc = cv->pc;
+    a.kn = cv->kp;
+    a.kgap = (int)(e - w);
+    a.tbase = cv->pos;  // ring slot of block e
+    a.tHa = cv->slot_ptr(cv->cur);
+    a.tSa = cv->scale_ptr(cv->cur);
+    bool second = false;
+    for (int k = 0; k < cv->kp; k++) {
+        const long long blk = e + k;
+        const int r = cv->pring(blk);
+        float2 *own = cv->tail_slot(blk, 0);
+        if (s_rel < 0 || k < s_rel) {
+            a.kTA[k] = own;
+        } else if (k == s_rel) {
+            a.kTA[k] = own;
+            a.kTB[k] = cv->tail_slot(blk, 1);
+            cv->porg[r][1] = w;
+            cv->pchunk[r][1] = cv->nchunk;
+            second = true;
+        } else {
+            a.kTB[k] = own;  // after the switch the new filter is the block's own
+            second = true;
         }
-        if (new_filter) {
-            a.tHb = cv->slot_ptr(cv->upd);
-            a.tSb = cv->scale_ptr(cv->upd);
-        }
-    } else {
-        a.nchunk = cv->wchunk[cv->wpar];
-        a.pc = cv->wpc[cv->wpar];
-        a.kn = (int)(cv->wo + cv->kp - hb);
-        a.tHa = cv->slot_ptr(cv->upd);
-        a.tSa = cv->scale_ptr(cv->upd);
-        for (int k = 1; k <= a.kn; k++) {
-            const long long blk = hb + k;
-            const int kw = (int)(blk - cv->wo);
-            const int f = blk == (long long)cv->switch_at ? 1 : 0;
-            a.kTA[k - 1] = cv->tail_ptrk(cv->wpar, kw - 1, f);
-            cv->org[kw - 1][f] = hb;
-        }
+        cv->porg[r][0] = w;
+        cv->pchunk[r][0] = cv->nchunk;
     }
-    // a one-output pass runs the single-block tail role, which writes tTa/tTb
-    a.tTa = a.kTA[0];
-    a.tTb = a.kTB[0];
-    return a.tHb ? 2 : 1;
+    if (second) {
+        a.tHb = cv->slot_ptr(cv->upd);
+        a.tSb = cv->scale_ptr(cv->upd);
+    }
+    cv->covered = e + cv->kp - 1;
+    return second ? 2 : 1;
 }
 
 // After a tail has been enqueued: the next head reads its buffer, and the
@@ -1032,9 +1038,8 @@ static int32_t enqueue_tail(tanx_conv *cv)
     a.nhead = 0;
     const int nf = fill_tail(cv, a);
     if (a.nchunk > 0) {
-        // a pass reads the update slot when it computes the new filter's sums
-        if (nf == 2 || (cv->kp > 1 && a.tHa == cv->slot_ptr(cv->upd)))
-            TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
+        // the tail (or a pass) reads the update slot with a second filter
+        if (nf == 2) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
         int32_t rc = launch_timed(cv, cv->t_tail, nf, a, tail_bytes(cv, nf), cv->tail_alg_bytes);
         if (rc != TANX_OK) return rc;
     }
@@ -1186,22 +1191,19 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     a.tchunk = cv->tail_nchunk;
     a.xfade = xfade ? 1 : 0;
     if (cv->kp > 1) {
-        // this block's slot of the current window, and the partitions its
-        // head adds: those after the pass that computed the slot
+        // this block's partial sums, and the partitions its head adds: those
+        // after the launch whose pass computed the sums
         const long long bb = (long long)b;
-        const int k = cv->wo >= 0 ? (int)(bb - cv->wo) : 0;
-        if (k >= 1 && k <= cv->kp) {
-            a.Ta = cv->tail_ptrk(cv->wpar, k - 1, 0);
-            a.Tb = xfade ? cv->tail_ptrk(cv->wpar, k - 1, 1) : a.Ta;
-            a.dA = (int)(bb - cv->org[k - 1][0]);
-            a.dB = xfade ? (int)(bb - cv->org[k - 1][1]) : 1;
-            a.tchunk = cv->wchunk[cv->wpar];
-        } else {
-            // the first block: no pass has run, and the history is silent
-            // (the other parity's first slot is still zero)
-            a.Ta = a.Tb = cv->tail_ptrk(cv->wpar, 0, 0);
-            a.tchunk = cv->wchunk[cv->wpar];
-        }
+        const int r = cv->pring(bb);
+        if (bb > cv->covered) return TANX_UNEXPECTED;  // a launch before it computes its sums
+        a.Ta = cv->tail_slot(bb, 0);
+        a.Tb = xfade ? cv->tail_slot(bb, 1) : a.Ta;
+        a.dA = (int)(bb - cv->porg[r][0]);
+        a.dB = xfade ? (int)(bb - cv->porg[r][1]) : 1;
+        a.tchunk = cv->pchunk[r][0];
+        if (a.dA < 1 || a.dA > kNearMax || a.dB < 1 || a.dB > kNearMax ||
+            (xfade && cv->pchunk[r][1] != a.tchunk))
+            return TANX_UNEXPECTED;
     }
 
     cv->pos = (cv->pos + 1) % cv->P;
@@ -1224,7 +1226,7 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
     bool pass_upd = false;
     if (with_tail) {
         nf = fill_tail(cv, a);
-        pass_upd = cv->kp > 1 && a.nchunk > 0 && a.tHa == cv->slot_ptr(cv->upd);
+        pass_upd = cv->kp > 1 && a.nchunk > 0 && a.tHb != nullptr;
     }
     *tail_nf = nf;
     *needs_update = xfade || nf == 2 || nu_upd || pass_upd;
@@ -1293,22 +1295,39 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
     return TANX_OK;
 }
 
-// Head-only blocks of a multi-block pass window prepared but not launched
-// yet: consecutive blocks of one device Process call, launched together by
-// flush_heads before the next launch of the object.
+// Blocks of an object with multi-block passes prepared but not launched
+// yet: consecutive heads of one launch (flush_heads), within one Process
+// call.
 struct HeadQueue {
     TickArgs a[kHeadBatch];
     int n = 0;
+    bool needs_update = false;  // some head reads the update slot
 };
 
+// Launches the queued heads [w, e) (e = cv->ticks) with the pass fill_pass
+// chooses: one head with k_tick (head role, pass role), several with
+// k_window (batched heads, pass role) or k_heads (no pass).
 static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
 {
     const int n = q.n;
     if (n == 0) return TANX_OK;
     q.n = 0;
+    const long long e = (long long)cv->ticks, w = e - n;
+    TickArgs p = q.a[0];
+    const int pnf = fill_pass(cv, w, e, p);
+    const bool upd = q.needs_update || pnf == 2;
+    q.needs_update = false;
+    if (upd) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     uint64_t hb = 0;
-    for (int j = 0; j < n; j++) hb += head_bytes(cv, 1) + near_bytes(cv, q.a[j]);
-    if (n == 1) return launch_timed(cv, cv->t_head, 1, q.a[0], hb, cv->head_alg_bytes);
+    for (int j = 0; j < n; j++) hb += head_bytes(cv, q.a[j].xfade ? 2 : 1) + near_bytes(cv, q.a[j]);
+    const uint64_t tb = pnf > 0 ? tail_bytes(cv, pnf) : 0;
+    KernelTimer &kt = pnf > 0 ? cv->t_tail : cv->t_head;
+    uint64_t &alg = pnf > 0 ? cv->tail_alg_bytes : cv->head_alg_bytes;
+    if (n == 1) {
+        const int32_t rc = launch_timed(cv, kt, std::max(1, pnf), p, hb + tb, alg);
+        if (rc != TANX_OK) return rc;
+        return mark_xfade(cv);
+    }
     const TickArgs &a0 = q.a[0];
     HeadBatchArgs h{};
     h.C = a0.C;
@@ -1324,6 +1343,7 @@ static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
     // entries past the batch repeat its last block (the kernel loads them unused)
     for (int j = 0; j < kHeadBatch; j++) {
         const TickArgs &a = q.a[std::min(j, n - 1)];
+        if (a.xfade || a.dA > kNearMax) return TANX_UNEXPECTED;
         h.xin[j] = a.xin;
         h.yout[j] = a.yout;
         h.H[j] = a.Ha;
@@ -1334,26 +1354,57 @@ static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
         h.d[j] = a.dA;
     }
     hipStream_t s = cv->ctx->proc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (cv->timing) {
-        KernelTimer &kt = cv->t_head;
         if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
-        TANX_CHECK(launch_heads(cv->B, cv->irtype, h, s, kt.start[kt.n], kt.stop[kt.n]));
-        kt.n++;
-        cv->head_alg_bytes += hb;
-        cv->timed_blocks += (uint64_t)n;
-        return TANX_OK;
+        e0 = kt.start[kt.n];
+        e1 = kt.stop[kt.n];
     }
-    TANX_CHECK(launch_heads(cv->B, cv->irtype, h, s));
+    if (pnf > 0) TANX_CHECK(launch_window(cv->B, pnf, h, p, s, e0, e1));
+    else TANX_CHECK(launch_heads(cv->B, cv->irtype, h, s, e0, e1));
+    if (cv->timing) {
+        kt.n++;
+        alg += hb + tb;
+        cv->timed_blocks += (uint64_t)n;
+    }
+    return TANX_OK;
+}
+
+// One block of the fused device path of an object with multi-block passes:
+// joins the queued heads unless it must start a launch (no partial sums
+// yet, a crossfade block, channel flags, a full queue), and ends the
+// launch when it crossfades or arms a switch (its pass then computes the
+// new filter's sums) or when it must run alone.
+static int32_t process_block_kp(tanx_conv *cv, const float *din, long long in_stride, float *dout,
+                                long long out_stride, size_t n, const uint32_t *flags, HeadQueue &q)
+{
+    const long long b = (long long)cv->ticks;
+    const bool xf_next = cv->delayed && (long long)cv->switch_at == b;
+    if (q.n > 0 && (b > cv->covered || xf_next || flags || q.n == kHeadBatch)) {
+        const int32_t rc = flush_heads(cv, q);
+        if (rc != TANX_OK) return rc;
+    }
+    TickArgs a;
+    int nf = 1;
+    bool upd = false, xf = false;
+    const int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, false, a, &nf, &upd, &xf);
+    if (rc != TANX_OK) {
+        // the blocks already advanced are launched before the error returns
+        const int32_t frc = flush_heads(cv, q);
+        return frc != TANX_OK ? frc : rc;
+    }
+    q.a[q.n++] = a;
+    q.needs_update = q.needs_update || upd;
+    tail_done(cv);
+    const bool armed = cv->delayed && (long long)cv->switch_at == b + 1;
+    if (xf || armed || flags || q.n == kHeadBatch || cv->B > kHeadBatchMaxB) return flush_heads(cv, q);
     return TANX_OK;
 }
 
 // One block launched directly; with_tail also runs the next block's tail in
-// the same launch.  With hq, a head-only block (multi-block passes, no
-// crossfade) joins the queue instead; any other block launches the queue
-// first.
+// the same launch.
 static int32_t process_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
-                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail,
-                             HeadQueue *hq = nullptr)
+                             long long out_stride, size_t n, const uint32_t *flags, bool with_tail)
 {
     if (cv->td) return process_td_block(cv, din, in_stride, dout, out_stride, n, flags);
     TickArgs a;
@@ -1362,17 +1413,6 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
     bool needs_update = false, xfade = false, frame_due = false;
     int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, with_tail, a, &tail_nf,
                                &needs_update, &xfade, &fa, &frame_due);
-    if (hq) {
-        if (rc == TANX_OK && with_tail && cv->kp > 1 && cv->B <= kHeadBatchMaxB && !flags && a.nchunk == 0 &&
-            a.nchB == 0 && !a.nu && !xfade && !frame_due && !needs_update) {
-            hq->a[hq->n++] = a;
-            tail_done(cv);
-            return hq->n == kHeadBatch ? flush_heads(cv, *hq) : TANX_OK;
-        }
-        // blocks already advanced are launched before this one (or its error)
-        const int32_t frc = flush_heads(cv, *hq);
-        if (frc != TANX_OK) return frc;
-    }
     if (rc != TANX_OK) return rc;
     if (needs_update) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     uint64_t hb = head_bytes(cv, xfade ? 2 : 1) + frame_mac_bytes(cv, a.bHb ? 2 : 1) + near_bytes(cv, a);
@@ -1525,8 +1565,11 @@ extern "C" int32_t tanx_conv_process_device(tanx_conv *cv, const float *d_in, si
     HeadQueue hq;
     const bool fuse = cv->fused && !cv->fmode;
     for (size_t b = 0; b < nb; b++) {
-        int32_t rc = process_block(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
-                                   (long long)out_stride, B, flags, fuse, fuse ? &hq : nullptr);
+        int32_t rc = (fuse && cv->kp > 1)
+                         ? process_block_kp(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
+                                            (long long)out_stride, B, flags, hq)
+                         : process_block(cv, d_in + b * B, (long long)in_stride, d_out + b * B,
+                                         (long long)out_stride, B, flags, fuse);
         if (rc != TANX_OK) return rc;
         if (!cv->fmode && !fuse) {
             rc = enqueue_tail(cv);
@@ -2046,14 +2089,15 @@ extern "C" int32_t tanx_conv_autotune(tanx_conv *cv, uint32_t *chosen, double *b
             a.tTa = cv->tail_ptr((t + 1) & 1, 0);
             a.tTb = nullptr;
             if (cv->kp > 1) {
-                // a pass every kp blocks, head-only launches between
-                const int k = t % cv->kp, par = (t / cv->kp) & 1;
-                a.Ta = a.Tb = cv->tail_ptrk(par ^ 1, (k + cv->kp - 1) % cv->kp, 0);
+                // a pass every kp blocks (with that block's head), head-only launches between
+                const int k = t % cv->kp;
+                a.Ta = a.Tb = cv->tail_slot(t, 0);
                 a.dA = k == 0 ? cv->kp : k;
                 if (k == 0) {
                     a.kn = cv->kp;
+                    a.kgap = 1;
                     for (int q = 0; q < kPassMax; q++) {
-                        a.kTA[q] = q < cv->kp ? cv->tail_ptrk(par, q, 0) : nullptr;
+                        a.kTA[q] = q < cv->kp ? cv->tail_slot(t + 1 + q, 0) : nullptr;
                         a.kTB[q] = nullptr;
                     }
                 } else {
@@ -2266,6 +2310,7 @@ extern "C" int32_t tanx_mixer_create(tanx_ctx *ctx, size_t buffer_size, int32_t 
     tanx_mixer *m = new (std::nothrow) tanx_mixer();
     if (!m) return TANX_OUT_OF_MEMORY;
     m->ctx = ctx;
+    ctx_retain(ctx);
     m->n = buffer_size;
     m->C = channels;
     hipError_t e = hipMalloc(&m->d_ptrs, sizeof(float *) * channels * kRunRing);
@@ -2279,6 +2324,7 @@ extern "C" int32_t tanx_mixer_create(tanx_ctx *ctx, size_t buffer_size, int32_t 
         if (m->h_ptrs) (void)hipHostFree(m->h_ptrs);
         for (auto &ev : m->ev) if (ev) (void)hipEventDestroy(ev);
         delete m;
+        ctx_release(ctx);
         return map_hip(e);
     }
     *out = m;
@@ -2293,7 +2339,9 @@ extern "C" int32_t tanx_mixer_destroy(tanx_mixer *m)
     if (m->d_ptrs) (void)hipFree(m->d_ptrs);
     if (m->h_ptrs) (void)hipHostFree(m->h_ptrs);
     for (auto &ev : m->ev) if (ev) (void)hipEventDestroy(ev);
+    tanx_ctx *ctx = m->ctx;
     delete m;
+    ctx_release(ctx);
     return TANX_OK;
 }
 

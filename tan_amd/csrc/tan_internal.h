@@ -44,6 +44,9 @@ enum IrType { IR_F32 = 0, IR_F16 = 1, IR_BF16 = 2 };
 inline int ir_elem_bytes(int t) { return t == IR_F32 ? 8 : 4; }
 
 constexpr int kPassMax = 4;  // blocks one tail pass computes partial sums for
+// partitions a head adds itself at most: a pass of the launch with heads
+// [w, e) serves blocks up to e + kPassMax - 1 from rows <= w - 1, e - w <= kPassMax
+constexpr int kNearMax = 2 * kPassMax - 1;
 
 struct TickArgs {
     int B;                 // block size (group launches dispatch on it)
@@ -82,15 +85,18 @@ struct TickArgs {
     const float *tSb;
     float2 *tTa;           // partial sums written [C][nchunk][B]
     float2 *tTb;
-    // multi-block pass (kn > 1): output k (0-based) holds the partial sums
-    // for the block k + 1 after this one over partitions i >= k + 2 (with
-    // FDL row (tbase + k - i) & (P - 1)); filter tHa's sums go to kTA[k],
-    // filter tHb's to kTB[k] (nullptr: not computed).  tTa/tTb unused.
+    // multi-block pass (kn > 1) from FDL rows before the launch's first head
+    // block o: output k (0-based) holds the partial sums of block o + kgap + k
+    // over partitions i >= kgap + k + 1 (FDL row (tbase + k - i) & (P - 1),
+    // tbase the ring slot of block o + kgap); filter tHa's sums go to
+    // kTA[k], filter tHb's to kTB[k] (nullptr: not computed).  tTa/tTb unused.
     int kn;
+    int kgap;
     float2 *kTA[4];
     float2 *kTB[4];
-    // head: partitions 0..dA (filter A) and 0..dB (filter B) are added by
-    // the head itself (1 without multi-block passes); the rest come from Ta/Tb
+    // head: partitions 0..dA (filter A) and 0..dB (filter B), at most
+    // kNearMax, are added by the head itself (1 without multi-block passes);
+    // the rest come from Ta/Tb
     int dA, dB;
 
     // non-uniform objects (nu = 0 for uniform ones)
@@ -202,11 +208,14 @@ hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, c
                              unsigned blocks, hipStream_t s);
 constexpr int kGroupMaxB = 512;
 // Consecutive head-only blocks of a uniform object with multi-block passes,
-// launched together (k_heads, one workgroup per channel): block j of the
-// batch follows block j-1; it adds its partial sums T[j] (tchunk[j] rows per
-// channel) and partitions 0..d[j] of filter H[j] itself, writes its spectrum
-// to FDL slot[j], and the last block's input becomes xprev.  No crossfade,
-// tail or frame role; B <= kHeadBatchMaxB.
+// launched together (one workgroup per channel): block j of the batch
+// follows block j-1; it adds its partial sums T[j] (tchunk[j] rows per
+// channel) and partitions 0..d[j] (d[j] <= kNearMax) of filter H[j] itself,
+// writes its spectrum to FDL slot[j], and the last block's input becomes
+// xprev.  No crossfade or frame role; B <= kHeadBatchMaxB.  launch_heads
+// runs the batch alone; with a pass (TickArgs a with kn > 1 and its gap
+// after the batch), launch_window runs the batch and the pass in one
+// launch: the pass reads FDL rows before the batch only.
 constexpr int kHeadBatch = kPassMax;
 constexpr int kHeadBatchMaxB = 512;
 struct HeadBatchArgs {
@@ -226,6 +235,8 @@ struct HeadBatchArgs {
 };
 hipError_t launch_heads(int B, int irtype, const HeadBatchArgs &h, hipStream_t s, hipEvent_t ev0 = nullptr,
                         hipEvent_t ev1 = nullptr);
+hipError_t launch_window(int B, int tail_nf, const HeadBatchArgs &h, const TickArgs &a, hipStream_t s,
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // k_tick reading its arguments from d_descs[k] (graph replay; kn must be 1)
 hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, unsigned blocks, hipStream_t s);
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s);

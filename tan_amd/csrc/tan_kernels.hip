@@ -319,7 +319,7 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                     // blocks since the pass that computed the partials
                     const int d = min(f == 0 ? a.dA : a.dB, a.P - 1);
 #pragma unroll
-                    for (int jn = kPassMax; jn >= 1; jn--) {
+                    for (int jn = kNearMax; jn >= 1; jn--) {
                         if (jn <= d) {
                             const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
                             const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
@@ -538,11 +538,12 @@ __device__ void mac_rows(const MacView &w, float4 *red)
 }
 
 // Multi-block pass (kn > 1 outputs): partitions [2 + j*pc, 2 + (j+1)*pc)
-// of channel c for each of the kn blocks after this one.  Output k (0-based,
-// block t+1+k) takes partition i >= k + 2 with FDL row (tbase + k - i), the
-// row partition i - k reads for output 0: each lane group walks its
-// partitions in order and keeps the last kPassMax rows it loaded, so every
-// IR row and FDL row is read once and used for every output.
+// of channel c for kn consecutive blocks.  Output k (0-based, block
+// o + g + k, o the launch's first head block, g = kgap) takes partition
+// i >= g + k + 1 with FDL row (tbase + k - i), the row partition i - k reads
+// for output 0: each lane group walks its partitions in order and keeps the
+// last kPassMax rows it loaded, so every IR row and FDL row is read once and
+// used for every output.
 template <int M, int NF, int HT>
 __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
 {
@@ -560,7 +561,7 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
     const char *HB = NF == 2 ? reinterpret_cast<const char *>(a.tHb) + (size_t)c * a.P * kRowBytes : nullptr;
     const float *SA = HT == IR_F32 ? nullptr : a.tSa + (size_t)c * a.P;
     const float *SB = (HT == IR_F32 || NF == 1) ? nullptr : a.tSb + (size_t)c * a.P;
-    const int mask = a.P - 1, base = a.tbase, kn = a.kn;
+    const int mask = a.P - 1, base = a.tbase, kn = a.kn, gap = a.kgap;
     const int i0 = 2 + j * a.pc, i1 = min(a.P, i0 + a.pc);
     const int span = (i1 - i0 + G - 1) / G;
     const int p0 = min(i1, i0 + g * span), p1 = min(i1, p0 + span);
@@ -583,8 +584,8 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
             accB[k][v] = make_float4(0.f, 0.f, 0.f, 0.f);
             // the window as it stands before partition p0: xw[k] = row (base + k + 1 - p0)
             // (output k + 1 reads it at p0), loaded only where some output uses it
-            xw[k][v] = (k + 1 < KM && p0 < p1 && p0 >= k + 3) ? frow(base + k + 1 - p0, v)
-                                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+            xw[k][v] = (k + 1 < KM && p0 < p1 && p0 >= k + 2 + gap) ? frow(base + k + 1 - p0, v)
+                                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     for (int i = p0; i < p1; i += U) {
         float4 xn[U][V], ha[U][V], hb[U][V];
@@ -623,7 +624,7 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
                 if (NF == 2) hB = HT == IR_F32 ? hb[u][v] : scale4(hb[u][v], sb[u]);
 #pragma unroll
                 for (int k = 0; k < KM; k++) {
-                    if (ii >= k + 2) {
+                    if (ii >= k + 1 + gap) {
                         if (wantA[k]) mac4(accA[k][v], xw[k][v], hA, sv, dc);
                         if (NF == 2 && wantB[k]) mac4(accB[k][v], xw[k][v], hB, sv, dc);
                     }
@@ -845,14 +846,13 @@ struct HeadsSmem {
 };
 
 template <int M, int HT>
-__global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
+__device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
 {
     constexpr int NT = kTickThreads;
     constexpr int KPT = M >= NT ? M / NT : 1;  // bins per thread
     constexpr int LM = ilog2(M);
     constexpr float kScale = 1.0f / (2.0f * (float)M);
-    __shared__ HeadsSmem<M> sm;
-    const int c = blockIdx.x, tid = threadIdx.x, n = h.n;
+    const int tid = threadIdx.x, n = h.n;
     if (!h.run[c]) return;
 
     // frames [x_{t-1}, x_t] of every block as complex pairs, and the twiddles
@@ -907,12 +907,26 @@ __global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
     for (int q = 0; q < KPT; q++) {
         const int k = tid + q * NT;
         if (k >= M) continue;
-        float2 fv[KP][KP + 1], hv[KP][KP + 1];
+        // rows before the batch beyond partition j + KP (a batch after
+        // launches without a pass): loaded as needed, the larger first
+#pragma unroll
+        for (int j = 0; j < KP; j++) {
+            const int d = min(h.d[j], h.P - 1);
+            for (int i = kNearMax; i > j + KP; i--) {
+                if (i > d) continue;
+                const float si = ir_scale<HT>(h.S[j], c, h.P, i);
+                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, i, M), k);
+                const float2 fr = h.fdl[((size_t)c * h.P + ((h.slot[j] - i) & (h.P - 1))) * M + k];
+                mac2(acc[j][q], fr, make_float2(hr.x * si, hr.y * si), k == 0);
+            }
+        }
+        // partitions j + KP .. j + 1, all loads issued together
+        float2 fv[KP][2 * KP + 1], hv[KP][2 * KP + 1];
 #pragma unroll
         for (int j = 0; j < KP; j++) {
             const int d = min(h.d[j], h.P - 1);
 #pragma unroll
-            for (int i = KP; i > j; i--) {
+            for (int i = j + KP; i > j; i--) {
                 const int ii = max(1, min(i, d));
                 const float si = ir_scale<HT>(h.S[j], c, h.P, ii);
                 const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, ii, M), k);
@@ -930,7 +944,7 @@ __global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
         for (int j = 0; j < KP; j++) {
             const int d = min(h.d[j], h.P - 1);
 #pragma unroll
-            for (int i = KP; i > j; i--)
+            for (int i = j + KP; i > j; i--)
                 if (i <= d) mac2(acc[j][q], fv[j][i], hv[j][i], k == 0);
         }
     }
@@ -980,6 +994,33 @@ __global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
         float *yo = h.yout[j] + (size_t)c * h.out_stride;
         yo[2 * m] = v.x * kScale;
         yo[2 * m + 1] = v.y * kScale;
+    }
+}
+
+template <int M, int HT>
+__global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
+{
+    __shared__ HeadsSmem<M> sm;
+    heads_role<M, HT>(h, blockIdx.x, sm);
+}
+
+// A head batch and the pass for the blocks after it in one launch: blocks
+// [0, C) run the heads (started first: their chain of FFTs is the longer
+// one), the rest the pass, which reads FDL rows before the batch only.
+template <int M, int NF, int HT>
+__global__ __launch_bounds__(kTickThreads) void k_window(HeadBatchArgs h, TickArgs a)
+{
+    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * kPassMax;  // float4
+    constexpr int kBytes = sizeof(HeadsSmem<M>) > kTailRed * 16 ? sizeof(HeadsSmem<M>) : kTailRed * 16;
+    __shared__ float4 smem[(kBytes + 15) / 16];
+    const int b = blockIdx.x;
+    if (b < h.C) {
+        heads_role<M, HT>(h, b, *reinterpret_cast<HeadsSmem<M> *>(smem));
+    } else {
+        const int t = b - h.C;
+        const int c = t / a.nchunk, j = t % a.nchunk;
+        if (!a.run[c]) return;
+        tailk_role<M, NF, HT>(a, c, j, smem);
     }
 }
 
@@ -1548,6 +1589,38 @@ static hipError_t launch_heads_t(int B, const HeadBatchArgs &h, hipStream_t s, h
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int HT>
+static hipError_t launch_window_t(int B, int tail_nf, const HeadBatchArgs &h, const TickArgs &a, hipStream_t s,
+                                  hipEvent_t ev0, hipEvent_t ev1)
+{
+    const dim3 grid((unsigned)(h.C + a.C * a.nchunk));
+#define TANX_WINDOW(MM)                                                                                       \
+    if (tail_nf == 2) hipExtLaunchKernelGGL((k_window<MM, 2, HT>), grid, dim3(kTickThreads), 0, s, ev0, ev1, 0, h, a); \
+    else hipExtLaunchKernelGGL((k_window<MM, 1, HT>), grid, dim3(kTickThreads), 0, s, ev0, ev1, 0, h, a)
+    switch (B) {
+    case 64: TANX_WINDOW(64); break;
+    case 128: TANX_WINDOW(128); break;
+    case 256: TANX_WINDOW(256); break;
+    case 512: TANX_WINDOW(512); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef TANX_WINDOW
+    return hipGetLastError();
+}
+
+hipError_t launch_window(int B, int tail_nf, const HeadBatchArgs &h, const TickArgs &a, hipStream_t s,
+                         hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (h.n < 1 || h.n > kHeadBatch || B > kHeadBatchMaxB || a.nchunk < 1 || a.kn < 2 || a.C != h.C)
+        return hipErrorInvalidValue;
+    switch (a.irtype) {
+    case IR_F32: return launch_window_t<IR_F32>(B, tail_nf, h, a, s, ev0, ev1);
+    case IR_F16: return launch_window_t<IR_F16>(B, tail_nf, h, a, s, ev0, ev1);
+    case IR_BF16: return launch_window_t<IR_BF16>(B, tail_nf, h, a, s, ev0, ev1);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_heads(int B, int irtype, const HeadBatchArgs &h, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)

@@ -33,6 +33,8 @@ def _drive(batch, nticks, seed):
 
 
 def _alone(ctx, spec, x, ir):
+    """The stream's object alone, one block per call as in the batch (a call
+    of several blocks would run its heads together, with other rounding)."""
     import torch
     L, B, C = spec
     conv = T.TANConvolution(ctx)
@@ -40,8 +42,9 @@ def _alone(ctx, spec, x, ir):
     assert conv.UpdateResponseTD(ir, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY) == T.AMF_OK
     xd = torch.from_numpy(x).cuda()
     yd = torch.zeros_like(xd)
-    rc, done = conv.Process(xd, yd, x.shape[1])
-    assert rc == T.AMF_OK and done == x.shape[1]
+    for t in range(x.shape[1] // B):
+        rc, done = conv.Process(xd[:, t * B:], yd[:, t * B:], B)
+        assert rc == T.AMF_OK and done == B
     conv.Synchronize()
     y = yd.cpu().numpy()
     conv.Terminate()

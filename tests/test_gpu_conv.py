@@ -272,26 +272,13 @@ def test_device_memory_variants(gpu_ctx, fused):
 
 def test_multiblock_device_process_with_updates(gpu_ctx):
     """One device Process call over many blocks equals block-by-block calls,
-    including a crossfade inside the call."""
+    including a crossfade inside the call: bit for bit with a tail every
+    block, up to rounding with multi-block passes (a call's heads run
+    together, with the pass for the blocks after them), and the oracle's."""
     import torch
     L, B, C, nb = 4096, 128, 2, 40
     x = T.signals.input_bank(C, nb * B)
     dx = torch.from_numpy(x).cuda()
-    outs = []
-    for per_call in (1, 20):
-        conv = T.TANConvolution(gpu_ctx)
-        conv.InitGpu(UNIFORM, L, B, C)
-        dy = torch.zeros_like(dx)
-        torch.cuda.synchronize()
-        for b0 in range(0, nb, per_call):
-            if b0 in (0, 20):
-                conv.UpdateResponseTD(_irs(C, L, b0), L, None, BLOCKING)
-            rc, done = conv.Process(dx[:, b0 * B:], dy[:, b0 * B:], per_call * B)
-            assert rc == T.AMF_OK and done == per_call * B
-        conv.Synchronize()
-        outs.append(dy.cpu().numpy())
-        conv.Terminate()
-    np.testing.assert_array_equal(outs[0], outs[1])
     orc = O.OracleConvolution(L, B, C)
     yo = np.zeros_like(x)
     for b in range(nb):
@@ -299,7 +286,27 @@ def test_multiblock_device_process_with_updates(gpu_ctx):
             orc.update_td(_irs(C, L, b))
         _, o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
         yo[:, b * B:(b + 1) * B] = o
-    assert_parity(outs[0], yo, "multiblock")
+    for kp in (1, 4):
+        outs = []
+        for per_call in (1, 20):
+            conv = T.TANConvolution(gpu_ctx, pass_blocks=kp)
+            conv.InitGpu(UNIFORM, L, B, C)
+            dy = torch.zeros_like(dx)
+            torch.cuda.synchronize()
+            for b0 in range(0, nb, per_call):
+                if b0 in (0, 20):
+                    conv.UpdateResponseTD(_irs(C, L, b0), L, None, BLOCKING)
+                rc, done = conv.Process(dx[:, b0 * B:], dy[:, b0 * B:], per_call * B)
+                assert rc == T.AMF_OK and done == per_call * B
+            conv.Synchronize()
+            outs.append(dy.cpu().numpy())
+            conv.Terminate()
+        if kp == 1:
+            np.testing.assert_array_equal(outs[0], outs[1])
+        else:
+            assert np.abs(outs[0].astype(np.float64) - outs[1]).max() <= 1e-6
+        for y in outs:
+            assert_parity(y, yo, f"multiblock kp={kp}")
 
 
 def test_autotune_keeps_state_and_parity(gpu_ctx):
@@ -348,3 +355,22 @@ def test_graph_replay_matches_direct_launches(gpu_ctx):
         outs.append(dy.cpu().numpy())
         conv.Terminate()
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_objects_keep_their_context():
+    """A context terminated while objects made on it still exist stays alive
+    until the last of them goes (the objects hold references)."""
+    import tan_amd as TT
+    ctx = TT.TANContext(device=0)
+    L, B, C = 4096, 256, 2
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    assert conv.UpdateResponseTD(_irs(C, L, 3), L, None, BLOCKING) == T.AMF_OK
+    assert ctx.Terminate() == T.AMF_OK
+    x = T.signals.input_bank(C, 4 * B)
+    for b in range(4):
+        out = np.zeros((C, B), np.float32)
+        rc, done = conv.Process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]), out)
+        assert rc == T.AMF_OK and done == B
+    assert np.isfinite(out).all()
+    assert conv.Terminate() == T.AMF_OK

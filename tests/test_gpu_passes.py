@@ -1,10 +1,12 @@
-"""Multi-block tail passes (tanx_conv_options.pass_blocks): one pass
-computes the partial sums of the next K blocks, each later head adds the
-partitions since that pass, and a switch armed inside a window gets a
-catch-up pass for its new filter.  The output is the reference's schedule:
-equal to the oracle (and the FP64 model) for every K, under dense updates
-that arm switches at every offset of the window, updates while a switch is
-armed, channel flushes and the host-memory path."""
+"""Multi-block tail passes (tanx_conv_options.pass_blocks): a launch runs
+up to K heads and, when the next block has no partial sums yet, a pass that
+computes the next K blocks' sums from the rows before its heads; each later
+head adds the partitions since that launch, and a switch armed at a launch's
+last block makes its pass compute the new filter's sums.  The output is the
+reference's schedule: equal to the oracle (and the FP64 model) for every K,
+under dense updates that arm switches at every offset, updates while a
+switch is armed, channel flushes, the host-memory path and calls of any
+number of blocks."""
 import numpy as np
 import pytest
 
@@ -181,9 +183,11 @@ CALLS = [1, 3, 7, 2, 13, 5, 1, 1, 9, 4, 6, 11, 2, 8]
                                            (2, 512, T.TAN_IR_PRECISION_FP32), (4, 64, T.TAN_IR_PRECISION_FP32),
                                            (4, 256, T.TAN_IR_PRECISION_FP16)])
 def test_multi_block_calls_batch_heads(gpu_ctx, K, B, precision):
-    """Calls of several blocks launch the head-only blocks of a pass window
-    together: the output is bit for bit that of one block per call, and the
-    oracle's, with updates (crossfades, catch-up passes) between calls."""
+    """Calls of several blocks launch up to K heads together with the pass
+    for the blocks after them (which reads the rows before the batch): the
+    output is the oracle's, with updates (crossfades, catch-up passes)
+    between calls, and that of one block per call up to rounding (the two
+    split each block's sum between pass and head differently)."""
     L, C = 32 * B, 3
     nb = sum(CALLS)
     starts = np.cumsum([0] + CALLS[:-1])
@@ -191,7 +195,7 @@ def test_multi_block_calls_batch_heads(gpu_ctx, K, B, precision):
     print(f"batched heads K={K} B={B} precision={precision}", flush=True)
     x, y = _run_calls(gpu_ctx, L, B, C, CALLS, ups, K, precision)
     _, y1 = _run_calls(gpu_ctx, L, B, C, [1] * nb, ups, K, precision)
-    np.testing.assert_array_equal(y, y1)
+    assert np.abs(y.astype(np.float64) - y1).max() <= 1e-6
     if precision == T.TAN_IR_PRECISION_FP32:
         orc = O.OracleConvolution(L, B, C)
         yo = np.zeros_like(x)
@@ -208,8 +212,8 @@ def test_multi_block_calls_batch_heads(gpu_ctx, K, B, precision):
 
 def test_kernel_timing_counts_blocks(gpu_ctx):
     """With kernel timing on, a batched call is timed too: the timed
-    launches cover every block (pass launches plus head launches, one of
-    which may carry several blocks) and their algorithmic bytes."""
+    launches (heads batched with the next blocks' pass) cover every block
+    and count their algorithmic bytes."""
     import torch
     L, B, C, nb = 16384, 256, 4, 40
     conv = T.TANConvolution(gpu_ctx, pass_blocks=4)
@@ -224,5 +228,5 @@ def test_kernel_timing_counts_blocks(gpu_ctx):
     kt = conv.ReadTiming(reset=True)
     conv.Terminate()
     assert kt["blocks"] == nb
-    assert kt["head_launches"] + kt["tail_launches"] < nb  # head-only blocks share launches
-    assert kt["head_ms"] > 0 and kt["tail_ms"] > 0 and kt["head_alg_bytes"] > 0
+    assert kt["head_launches"] + kt["tail_launches"] < nb  # several heads share a launch
+    assert kt["tail_ms"] > 0 and kt["tail_alg_bytes"] > 0
