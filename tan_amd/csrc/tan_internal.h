@@ -217,7 +217,7 @@ constexpr int kGroupMaxB = 512;
 // after the batch), launch_window runs the batch and the pass in one
 // launch: the pass reads FDL rows before the batch only.
 constexpr int kHeadBatch = kPassMax;
-constexpr int kHeadBatchMaxB = 512;
+constexpr int kHeadBatchMaxB = 256;  // larger blocks: one head per launch (registers)
 struct HeadBatchArgs {
     int C, P;
     const uint8_t *run;

@@ -279,8 +279,11 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
     constexpr float kScale = 1.0f / (2.0f * (float)M);
     const int tid = threadIdx.x;
     const int nf = a.xfade ? 2 : 1;
+    // the head is the launch's longest chain: its waves issue before the
+    // tail's on a shared compute unit
+    __builtin_amdgcn_s_setprio(3);
 
-    // ---- issue every global load first: frame, twiddles, H0/H1, partials
+    // ---- issue every global load first: frame, twiddles, near rows, partials
     const float *xp = a.xprev + (size_t)c * M;
     const float *xi = a.xin + (size_t)c * a.in_stride;
     for (int n = tid; n < M; n += NT) {
@@ -289,8 +292,6 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
         sm.tw[n] = a.twM[n];
         sm.twn[n] = a.twN[n];
     }
-    const int prevslot = (a.slot - 1) & (a.P - 1);
-    const float2 *Fprev = a.fdl + ((size_t)c * a.P + prevslot) * M;
     const int dmax = max(a.dA, nf == 2 ? a.dB : 1);  // 1: partition 1 only
     float2 acc[2][KPT], h0[2][KPT];
 #pragma unroll
@@ -299,36 +300,40 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
         const void *Hbase = f == 0 ? a.Ha : a.Hb;
         const float *Sbase = f == 0 ? a.Sa : a.Sb;
         const void *H0row = ir_row<HT>(Hbase, c, a.P, 0, M);
-        const void *H1row = a.P > 1 ? ir_row<HT>(Hbase, c, a.P, 1, M) : H0row;
         const float s0 = ir_scale<HT>(Sbase, c, a.P, 0);
-        const float s1 = a.P > 1 ? ir_scale<HT>(Sbase, c, a.P, 1) : 0.f;
         const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.tchunk * M;
 #pragma unroll
         for (int q = 0; q < KPT; q++) {
             const int k = tid + q * NT;
             float2 s = make_float2(0.f, 0.f);
             if (k < M) {
-                s = sum_partials<M>(T, a.tchunk, k);
-                if (dmax == 1) {
-                    if (a.P > 1) {
-                        const float2 h1 = IrRow<HT>::get1(H1row, k);
-                        mac2(s, Fprev[k], make_float2(h1.x * s1, h1.y * s1), k == 0);
-                    }
-                } else {
-                    // partitions d..1 (descending, the order k_heads keeps): the
-                    // blocks since the pass that computed the partials
-                    const int d = min(f == 0 ? a.dA : a.dB, a.P - 1);
+                // the near rows' loads go out with the partial sums' (one
+                // memory round trip): partitions 1..kPassMax from clamped
+                // indices, used up to d
+                const int d = dmax == 1 ? 1 : min(f == 0 ? a.dA : a.dB, a.P - 1);
+                float2 fn[kPassMax], hn[kPassMax];
 #pragma unroll
-                    for (int jn = kNearMax; jn >= 1; jn--) {
-                        if (jn <= d) {
-                            const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
-                            const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
-                            const float2 hv = IrRow<HT>::get1(ir_row<HT>(Hbase, c, a.P, jn, M), k);
-                            mac2(s, Fj[k], make_float2(hv.x * sj, hv.y * sj), k == 0);
-                        }
-                    }
+                for (int jn = 1; jn <= kPassMax; jn++) {
+                    const int jj = a.P > 1 ? max(1, min(jn, min(d, a.P - 1))) : 0;  // in range, P = 1 too
+                    fn[jn - 1] = a.fdl[((size_t)c * a.P + ((a.slot - jj) & (a.P - 1))) * M + k];
+                    const float sj = ir_scale<HT>(Sbase, c, a.P, jj);
+                    const float2 hv = IrRow<HT>::get1(ir_row<HT>(Hbase, c, a.P, jj, M), k);
+                    hn[jn - 1] = make_float2(hv.x * sj, hv.y * sj);
                 }
                 const float2 h0v = IrRow<HT>::get1(H0row, k);
+                s = sum_partials<M>(T, a.tchunk, k);
+                // partitions d..1, descending (the order heads_role keeps):
+                // the blocks since the launch whose pass computed the partials
+                for (int jn = kNearMax; jn > kPassMax; jn--) {
+                    if (jn > d) continue;
+                    const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
+                    const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
+                    const float2 hv = IrRow<HT>::get1(ir_row<HT>(Hbase, c, a.P, jn, M), k);
+                    mac2(s, Fj[k], make_float2(hv.x * sj, hv.y * sj), k == 0);
+                }
+#pragma unroll
+                for (int jn = kPassMax; jn >= 1; jn--)
+                    if (jn <= d && a.P > 1) mac2(s, fn[jn - 1], hn[jn - 1], k == 0);
                 h0[f][q] = make_float2(h0v.x * s0, h0v.y * s0);
             }
             acc[f][q] = s;
@@ -854,13 +859,22 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
     constexpr float kScale = 1.0f / (2.0f * (float)M);
     const int tid = threadIdx.x, n = h.n;
     if (!h.run[c]) return;
+    // the batch is the launch's longest chain: its waves issue before the
+    // pass's on a shared compute unit
+    __builtin_amdgcn_s_setprio(3);
 
-    // frames [x_{t-1}, x_t] of every block as complex pairs, and the twiddles
-    for (int e = tid; e < (n << LM); e += NT) {
-        const int j = e >> LM, m = e & (M - 1);
+    // frames [x_{t-1}, x_t] of every entry as complex pairs (entries past
+    // the batch repeat its last block: every load is in range and they all
+    // go out together), and the twiddles
+#pragma unroll
+    for (int j = 0; j < kHeadBatch; j++) {
         const float *prev = j == 0 ? h.xprev + (size_t)c * M : h.xin[j - 1] + (size_t)c * h.in_stride;
-        const float *src = m < M / 2 ? prev + 2 * m : h.xin[j] + (size_t)c * h.in_stride + 2 * (m - M / 2);
-        sm.a[j][m] = make_float2(src[0], src[1]);
+        const float *cur = h.xin[j] + (size_t)c * h.in_stride;
+#pragma unroll
+        for (int m = tid; m < M; m += NT) {
+            const float *src = m < M / 2 ? prev + 2 * m : cur + 2 * (m - M / 2);
+            sm.a[j][m] = make_float2(src[0], src[1]);
+        }
     }
     for (int m = tid; m < M; m += NT) {
         sm.tw[m] = h.twM[m];
@@ -873,7 +887,31 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
     // the batch (i = d_j .. j+1) go to acc; the IR rows of partitions 0..j
     // (rows of the batch, d_j >= j + 1) to hn.
     constexpr int KP = kHeadBatch;
-    float2 acc[KP][KPT], hn[KP][KP][KPT];
+    float2 acc[KP][KPT], hn[KP][KP][KPT], fv[KPT][KP][KP], hv[KPT][KP][KP];
+    // near rows first: partitions j+1 .. j+KP before the batch (clamped
+    // indices, used up to d_j) and the IR rows of partitions 0..j
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+        const int k = min(tid + q * NT, M - 1);
+#pragma unroll
+        for (int j = 0; j < KP; j++) {
+            const int d = min(h.d[j], h.P - 1);
+#pragma unroll
+            for (int r = 0; r < KP; r++) {
+                const int ii = max(1, min(j + 1 + r, d));
+                const float si = ir_scale<HT>(h.S[j], c, h.P, ii);
+                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, ii, M), k);
+                fv[q][j][r] = h.fdl[((size_t)c * h.P + ((h.slot[j] - ii) & (h.P - 1))) * M + k];
+                hv[q][j][r] = make_float2(hr.x * si, hr.y * si);
+            }
+#pragma unroll
+            for (int i = 0; i <= j; i++) {
+                const float si = ir_scale<HT>(h.S[j], c, h.P, i);
+                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, i, M), k);
+                hn[j][i][q] = make_float2(hr.x * si, hr.y * si);
+            }
+        }
+    }
     int tmax = 0;
 #pragma unroll
     for (int j = 0; j < KP; j++) tmax = max(tmax, h.tchunk[j]);
@@ -907,11 +945,11 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
     for (int q = 0; q < KPT; q++) {
         const int k = tid + q * NT;
         if (k >= M) continue;
-        // rows before the batch beyond partition j + KP (a batch after
-        // launches without a pass): loaded as needed, the larger first
 #pragma unroll
         for (int j = 0; j < KP; j++) {
             const int d = min(h.d[j], h.P - 1);
+            // rows before the batch beyond partition j + KP (a batch after
+            // launches without a pass): loaded as needed, the larger first
             for (int i = kNearMax; i > j + KP; i--) {
                 if (i > d) continue;
                 const float si = ir_scale<HT>(h.S[j], c, h.P, i);
@@ -919,33 +957,9 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
                 const float2 fr = h.fdl[((size_t)c * h.P + ((h.slot[j] - i) & (h.P - 1))) * M + k];
                 mac2(acc[j][q], fr, make_float2(hr.x * si, hr.y * si), k == 0);
             }
-        }
-        // partitions j + KP .. j + 1, all loads issued together
-        float2 fv[KP][2 * KP + 1], hv[KP][2 * KP + 1];
 #pragma unroll
-        for (int j = 0; j < KP; j++) {
-            const int d = min(h.d[j], h.P - 1);
-#pragma unroll
-            for (int i = j + KP; i > j; i--) {
-                const int ii = max(1, min(i, d));
-                const float si = ir_scale<HT>(h.S[j], c, h.P, ii);
-                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, ii, M), k);
-                fv[j][i] = h.fdl[((size_t)c * h.P + ((h.slot[j] - ii) & (h.P - 1))) * M + k];
-                hv[j][i] = make_float2(hr.x * si, hr.y * si);
-            }
-#pragma unroll
-            for (int i = 0; i <= j; i++) {
-                const float si = ir_scale<HT>(h.S[j], c, h.P, i);
-                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, i, M), k);
-                hn[j][i][q] = make_float2(hr.x * si, hr.y * si);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < KP; j++) {
-            const int d = min(h.d[j], h.P - 1);
-#pragma unroll
-            for (int i = j + KP; i > j; i--)
-                if (i <= d) mac2(acc[j][q], fv[j][i], hv[j][i], k == 0);
+            for (int r = KP - 1; r >= 0; r--)
+                if (j + 1 + r <= d) mac2(acc[j][q], fv[q][j][r], hv[q][j][r], k == 0);
         }
     }
     __syncthreads();
@@ -1585,7 +1599,6 @@ static hipError_t launch_heads_t(int B, const HeadBatchArgs &h, hipStream_t s, h
     case 64: hipExtLaunchKernelGGL((k_heads<64, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
     case 128: hipExtLaunchKernelGGL((k_heads<128, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
     case 256: hipExtLaunchKernelGGL((k_heads<256, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
-    case 512: hipExtLaunchKernelGGL((k_heads<512, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1603,7 +1616,6 @@ static hipError_t launch_window_t(int B, int tail_nf, const HeadBatchArgs &h, co
     case 64: TANX_WINDOW(64); break;
     case 128: TANX_WINDOW(128); break;
     case 256: TANX_WINDOW(256); break;
-    case 512: TANX_WINDOW(512); break;
     default: return hipErrorInvalidValue;
     }
 #undef TANX_WINDOW
