@@ -46,10 +46,13 @@ class GroupCall:
         self._keep = (list(convs), list(inputs), list(outputs))
         self._ctx = convs[0]._ctx if n else None
 
-    def launch(self):
+    def launch(self, order=True):
+        """One group launch; order: first make the process stream wait for
+        torch's current stream (False when the caller has ordered it)."""
         if self.n == 0:
             return T.AMF_OK
-        self._ctx.order_after_torch()
+        if order:
+            self._ctx.order_after_torch()
         return self._lib.tanx_group_process_device(self.n, self.h, self.i, self.ist, self.o, self.ost, None)
 
 

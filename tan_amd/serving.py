@@ -320,7 +320,7 @@ class SessionServer:
             ev = torch.cuda.Event()
             ev.record()
             s.wait_event(ev)
-        rc = self._call.launch()
+        rc = self._call.launch(order=False)
         if rc != T.AMF_OK:
             raise RuntimeError(f"group launch failed: AMF_RESULT {rc}")
         t1 = time.perf_counter()
