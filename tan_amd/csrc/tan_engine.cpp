@@ -1343,7 +1343,8 @@ static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
     // entries past the batch repeat its last block (the kernel loads them unused)
     for (int j = 0; j < kHeadBatch; j++) {
         const TickArgs &a = q.a[std::min(j, n - 1)];
-        if (a.xfade || a.dA > kNearMax) return TANX_UNEXPECTED;
+        // one filter for the whole batch (a crossfade block runs alone)
+        if (a.xfade || a.dA > kNearMax || a.Ha != a0.Ha || a.Sa != a0.Sa) return TANX_UNEXPECTED;
         h.xin[j] = a.xin;
         h.yout[j] = a.yout;
         h.H[j] = a.Ha;

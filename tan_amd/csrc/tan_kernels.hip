@@ -881,35 +881,27 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
         sm.twn[m] = h.twN[m];
     }
     // Every global load of the batch is issued before any is used (one
-    // memory round trip): entries j >= n repeat block n-1 (valid addresses,
-    // results unused), and the near partitions load from clamped indices.
-    // Block j: the partial sums and the partitions whose FDL rows precede
-    // the batch (i = d_j .. j+1) go to acc; the IR rows of partitions 0..j
-    // (rows of the batch, d_j >= j + 1) to hn.
+    // memory round trip).  The blocks of a batch share one filter (no
+    // crossfade inside a batch) and the FDL rows before it, so a thread
+    // holds, for its bins, IR partitions 0..kNearMax and the rows of blocks
+    // w-1 .. w-kNearMax once (clamped indices, used where in range); block
+    // j adds partitions d_j..j+1 from those rows before its FFT and
+    // partitions j..0 from the batch's spectra after it.  Entries j >= n
+    // repeat block n-1 (valid addresses, results unused).
     constexpr int KP = kHeadBatch;
-    float2 acc[KP][KPT], hn[KP][KP][KPT], fv[KPT][KP][KP], hv[KPT][KP][KP];
-    // near rows first: partitions j+1 .. j+KP before the batch (clamped
-    // indices, used up to d_j) and the IR rows of partitions 0..j
+    float2 acc[KP][KPT], hr[kNearMax + 1][KPT], fr[kNearMax + 1][KPT];
+    const int dlim = min(kNearMax, h.P - 1);
 #pragma unroll
     for (int q = 0; q < KPT; q++) {
         const int k = min(tid + q * NT, M - 1);
 #pragma unroll
-        for (int j = 0; j < KP; j++) {
-            const int d = min(h.d[j], h.P - 1);
-#pragma unroll
-            for (int r = 0; r < KP; r++) {
-                const int ii = max(1, min(j + 1 + r, d));
-                const float si = ir_scale<HT>(h.S[j], c, h.P, ii);
-                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, ii, M), k);
-                fv[q][j][r] = h.fdl[((size_t)c * h.P + ((h.slot[j] - ii) & (h.P - 1))) * M + k];
-                hv[q][j][r] = make_float2(hr.x * si, hr.y * si);
-            }
-#pragma unroll
-            for (int i = 0; i <= j; i++) {
-                const float si = ir_scale<HT>(h.S[j], c, h.P, i);
-                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, i, M), k);
-                hn[j][i][q] = make_float2(hr.x * si, hr.y * si);
-            }
+        for (int i = 0; i <= kNearMax; i++) {
+            const int ii = min(i, dlim);
+            const float si = ir_scale<HT>(h.S[0], c, h.P, ii);
+            const float2 hv = IrRow<HT>::get1(ir_row<HT>(h.H[0], c, h.P, ii, M), k);
+            hr[i][q] = make_float2(hv.x * si, hv.y * si);
+            // row of block w - i (i >= 1), at ring slot slot_0 - i
+            fr[i][q] = h.fdl[((size_t)c * h.P + ((h.slot[0] - max(ii, 1)) & (h.P - 1))) * M + k];
         }
     }
     int tmax = 0;
@@ -947,19 +939,11 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
         if (k >= M) continue;
 #pragma unroll
         for (int j = 0; j < KP; j++) {
-            const int d = min(h.d[j], h.P - 1);
-            // rows before the batch beyond partition j + KP (a batch after
-            // launches without a pass): loaded as needed, the larger first
-            for (int i = kNearMax; i > j + KP; i--) {
-                if (i > d) continue;
-                const float si = ir_scale<HT>(h.S[j], c, h.P, i);
-                const float2 hr = IrRow<HT>::get1(ir_row<HT>(h.H[j], c, h.P, i, M), k);
-                const float2 fr = h.fdl[((size_t)c * h.P + ((h.slot[j] - i) & (h.P - 1))) * M + k];
-                mac2(acc[j][q], fr, make_float2(hr.x * si, hr.y * si), k == 0);
-            }
+            const int d = min(h.d[j], dlim);
+            // partitions d..j+1: rows of blocks w-1 .. (rows before the batch), the larger first
 #pragma unroll
-            for (int r = KP - 1; r >= 0; r--)
-                if (j + 1 + r <= d) mac2(acc[j][q], fv[q][j][r], hv[q][j][r], k == 0);
+            for (int i = kNearMax; i > j; i--)
+                if (i <= d) mac2(acc[j][q], fr[i - j][q], hr[i][q], k == 0);
         }
     }
     __syncthreads();
@@ -989,7 +973,7 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
                 float2 v = acc[j][q];
 #pragma unroll
                 for (int i = KP - 1; i >= 0; i--)
-                    if (i <= j && i <= d) mac2(v, sm.x[j - i][k], hn[j][i][q], k == 0);
+                    if (i <= j && i <= d) mac2(v, sm.x[j - i][k], hr[i][q], k == 0);
                 sm.a[j][k] = v;
             }
         }
