@@ -230,3 +230,41 @@ def test_kernel_timing_counts_blocks(gpu_ctx):
     assert kt["blocks"] == nb
     assert kt["head_launches"] + kt["tail_launches"] < nb  # several heads share a launch
     assert kt["tail_ms"] > 0 and kt["tail_alg_bytes"] > 0
+
+
+@pytest.mark.parametrize("chunks", [1, 3, 16, 30])
+def test_pass_chunk_counts(gpu_ctx, chunks):
+    """A pass splits a channel's partitions into chunks whose rows the
+    heads sum, whatever their number: the oracle's output at every chunk
+    count, with updates and uneven calls."""
+    import torch
+    L, B, C = 8192, 256, 3
+    calls = [1, 5, 2, 9, 4, 1, 1, 7, 3, 8]
+    nb = sum(calls)
+    starts = np.cumsum([0] + calls[:-1])
+    ups = _updates(C, L, [int(starts[i]) for i in (0, 2, 5, 7)], seed=602)
+    conv = T.TANConvolution(gpu_ctx, pass_blocks=4)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    assert conv.SetTailChunks(chunks) == T.AMF_OK
+    print(f"pass chunks={chunks}", flush=True)
+    x = T.signals.input_bank(C, nb * B, base_seed=703)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    b = 0
+    for n in calls:
+        if b in ups:
+            assert conv.UpdateResponseTD(ups[b], L, None, BLOCKING) == T.AMF_OK
+        rc, done = conv.Process(xd[:, b * B:], yd[:, b * B:], n * B)
+        assert rc == T.AMF_OK and done == n * B
+        b += n
+    conv.Synchronize()
+    y = yd.cpu().numpy()
+    conv.Terminate()
+    orc = O.OracleConvolution(L, B, C)
+    yo = np.zeros_like(x)
+    for b in range(nb):
+        if b in ups:
+            assert orc.update_td(ups[b]) == O.AMF_OK
+        _, out_o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
+        yo[:, b * B:(b + 1) * B] = out_o
+    assert np.abs(y.astype(np.float64) - yo).max() <= 1e-5
