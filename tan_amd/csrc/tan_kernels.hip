@@ -1210,6 +1210,36 @@ __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs
     }
 }
 
+// FP32 IR spectra of NR partitions per workgroup (256 threads): the rows'
+// FFTs share the Stockham stages, so a transform of many short partitions
+// runs as a quarter (or half) as many workgroups.  Same arithmetic per row
+// as k_ir_transform.
+template <int M, int NR>
+__global__ __launch_bounds__(256) void k_ir_transform_rows(IrArgs a)
+{
+    __shared__ float2 s0[NR * M], s1[NR * M];
+    const int i0 = blockIdx.x * NR, ch = a.chan[blockIdx.y], tid = threadIdx.x, nt = blockDim.x;
+    const int nr = min(NR, a.P - i0);
+    const float *h = a.ir + (size_t)ch * a.stride;
+    for (int e = tid; e < NR * M; e += nt) {
+        const int r = e / M, n = e - r * M;
+        float2 v = make_float2(0.f, 0.f);
+        if (r < nr && n < M / 2) {
+            const long long k = (long long)(i0 + r) * M + 2 * n;
+            v.x = k < a.n ? h[k] : 0.f;
+            v.y = k + 1 < a.n ? h[k + 1] : 0.f;
+        }
+        s0[e] = v;
+    }
+    __syncthreads();
+    const float2 *Z = fft_lds_n<M, false>(s0, s1, nr, a.twM);
+    float2 *rows = reinterpret_cast<float2 *>(a.H) + ((size_t)ch * a.P + i0) * M;
+    for (int e = tid; e < nr * M; e += nt) {
+        const int r = e / M, k = e - r * M;
+        rows[e] = r2c_post<M>(Z + r * M, k, a.twN);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // TANFFT transforms, one workgroup per channel.  M is the complex FFT size
 // (N for complex directions, N/2 for real ones).
@@ -1648,7 +1678,21 @@ hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
     if (a.P == 0 || nchan == 0) return hipSuccess;
     switch (a.irtype) {
     case IR_F32:
-        TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F32>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        // short partitions: several rows per workgroup
+        if (B <= 256) {
+            const dim3 g((unsigned)((a.P + 3) / 4), (unsigned)nchan);
+            switch (B) {
+            case 64: hipLaunchKernelGGL((k_ir_transform_rows<64, 4>), g, dim3(256), 0, s, a); break;
+            case 128: hipLaunchKernelGGL((k_ir_transform_rows<128, 4>), g, dim3(256), 0, s, a); break;
+            case 256: hipLaunchKernelGGL((k_ir_transform_rows<256, 4>), g, dim3(256), 0, s, a); break;
+            default: return hipErrorInvalidValue;
+            }
+        } else if (B == 512) {
+            hipLaunchKernelGGL((k_ir_transform_rows<512, 2>), dim3((unsigned)((a.P + 1) / 2), (unsigned)nchan), dim3(256),
+                               0, s, a);
+        } else {
+            TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F32>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
+        }
         break;
     case IR_F16:
         TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform<M, IR_F16>), dim3(a.P, nchan), dim3(fft_threads(M)), 0, s, a));
