@@ -153,6 +153,10 @@ public:
     virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
     // replaces InitOpenCL: bind a HIP device (default: device 0 at first use)
     virtual AMF_RESULT AMF_STD_CALL InitHIP(int device = 0) = 0;
+    // replaces InitOpenCL(cl_command_queue convolutionQueue, cl_command_queue
+    // generalQueue): run on the caller's hipStream_t (nullptr: a stream of the
+    // context's own); they must outlive the context
+    virtual AMF_RESULT AMF_STD_CALL InitHIPStreams(void *convStream, void *generalStream, int device = 0) = 0;
     // accepted for source compatibility; the GPU path has no OpenMP threads
     virtual AMF_RESULT AMF_STD_CALL InitOpenMP(int nThreads) = 0;
     virtual int AMF_STD_CALL GetHIPDevice() = 0;

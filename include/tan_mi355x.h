@@ -88,6 +88,18 @@ enum {
  * convolution (process) queues.  `version` must have major 1
  * (TANContextImpl.cpp:114-120), else TANX_TAN_UNSUPPORTED_VERSION. */
 int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out);
+/* TANContext::InitOpenCL(cl_command_queue pConvolutionQueue, cl_command_queue
+ * pGeneralQueue) (TANContextImpl.cpp:316-330; the header,
+ * TrueAudioNext.h:589-591, names the two parameters the other way round and
+ * the implementation wins): like tanx_ctx_create, but Process, group
+ * launches and the output chain run on conv_stream and IR updates and
+ * transforms on general_stream, hipStream_t of the caller on `device` (the
+ * same stream may be given twice; NULL: a stream of the context's own).
+ * TANX_INVALID_ARG when a stream belongs to another device.  The context
+ * drains the caller's streams when it goes but does not destroy them; they
+ * must outlive it. */
+int32_t tanx_ctx_create_on_streams(uint64_t version, int32_t device, void *conv_stream, void *general_stream,
+                                   tanx_ctx **out);
 /* TANContext::Terminate (TANContextImpl.cpp:158-179): releases the caller's
  * handle.  Convolution objects and mixers made on the context hold it too,
  * as the reference's reference-counted objects do, so the context's streams

@@ -47,6 +47,7 @@ class KernelTiming(ctypes.Structure):
 
 _SIGS = {
     "tanx_ctx_create": (_i32, [ctypes.c_uint64, _i32, ctypes.POINTER(_vp)]),
+    "tanx_ctx_create_on_streams": (_i32, [ctypes.c_uint64, _i32, _vp, _vp, ctypes.POINTER(_vp)]),
     "tanx_ctx_destroy": (_i32, [_vp]),
     "tanx_ctx_device": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "tanx_ctx_process_stream": (_vp, [_vp]),

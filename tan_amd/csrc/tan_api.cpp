@@ -51,6 +51,12 @@ public:
         if (ctx_) return AMF_ALREADY_INITIALIZED;
         return (AMF_RESULT)tanx_ctx_create(TAN_FULL_VERSION, device, &ctx_);
     }
+    AMF_RESULT AMF_STD_CALL InitHIPStreams(void *convStream, void *generalStream, int device) override
+    {
+        std::lock_guard<std::mutex> lk(mtx_);
+        if (ctx_) return AMF_ALREADY_INITIALIZED;
+        return (AMF_RESULT)tanx_ctx_create_on_streams(TAN_FULL_VERSION, device, convStream, generalStream, &ctx_);
+    }
     AMF_RESULT AMF_STD_CALL InitOpenMP(int) override { return AMF_OK; }
     int AMF_STD_CALL GetHIPDevice() override
     {
@@ -59,8 +65,17 @@ public:
         if (c) tanx_ctx_device(c, &d);
         return d;
     }
-    void *AMF_STD_CALL GetHIPGeneralStream() override { return nullptr; }
-    void *AMF_STD_CALL GetHIPConvStream() override { return nullptr; }
+    // GetOpenCLGeneralQueue / GetOpenCLConvQueue (TrueAudioNext.h:595-596)
+    void *AMF_STD_CALL GetHIPGeneralStream() override
+    {
+        tanx_ctx *c = handle();
+        return c ? tanx_ctx_update_stream(c) : nullptr;
+    }
+    void *AMF_STD_CALL GetHIPConvStream() override
+    {
+        tanx_ctx *c = handle();
+        return c ? tanx_ctx_process_stream(c) : nullptr;
+    }
 
     // the HIP context, created on device 0 at first use when InitHIP was not called
     tanx_ctx *handle()

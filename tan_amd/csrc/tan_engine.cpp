@@ -75,6 +75,7 @@ struct tanx_ctx {
     int g_ring = 0;
     hipStream_t proc = nullptr;  // convolution queue equivalent
     hipStream_t upd = nullptr;   // general queue equivalent (IR updates, FFT)
+    bool own_proc = true, own_upd = true;  // created here (else the caller's)
     float2 *tw = nullptr;        // twiddle tables, see tw_table()
     std::mutex fft_mtx;
     float *fft_dev = nullptr;
@@ -127,7 +128,9 @@ static int32_t ctx_upload_twiddles(tanx_ctx *ctx)
     return TANX_OK;
 }
 
-extern "C" int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out)
+// A context on `device` whose process and update streams are the caller's
+// (conv, general) or, where NULL, streams of its own.
+static int32_t ctx_create(uint64_t version, int32_t device, hipStream_t conv, hipStream_t general, tanx_ctx **out)
 {
     if (!out) return TANX_INVALID_POINTER;
     *out = nullptr;
@@ -137,13 +140,21 @@ extern "C" int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **
     if (device < 0 || device >= ndev) return TANX_INVALID_ARG;
     DeviceGuard g(device);
     if (!g.ok) return TANX_NO_DEVICE;
+    for (hipStream_t st : {conv, general}) {
+        hipDevice_t d = -1;
+        if (st && (hipStreamGetDevice(st, &d) != hipSuccess || d != device)) return TANX_INVALID_ARG;
+    }
     tanx_ctx *ctx = new (std::nothrow) tanx_ctx();
     if (!ctx) return TANX_OUT_OF_MEMORY;
     ctx->device = device;
+    ctx->proc = conv;
+    ctx->upd = general;
+    ctx->own_proc = conv == nullptr;
+    ctx->own_upd = general == nullptr;
     int32_t rc = TANX_OK;
     // default-priority streams only
-    if (hipStreamCreateWithFlags(&ctx->proc, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->upd, hipStreamNonBlocking) != hipSuccess) {
+    if ((ctx->own_proc && hipStreamCreateWithFlags(&ctx->proc, hipStreamNonBlocking) != hipSuccess) ||
+        (ctx->own_upd && hipStreamCreateWithFlags(&ctx->upd, hipStreamNonBlocking) != hipSuccess)) {
         rc = TANX_UNEXPECTED;
     }
     if (rc == TANX_OK) rc = ctx_upload_twiddles(ctx);
@@ -155,12 +166,31 @@ extern "C" int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **
     return TANX_OK;
 }
 
+extern "C" int32_t tanx_ctx_create(uint64_t version, int32_t device, tanx_ctx **out)
+{
+    return ctx_create(version, device, nullptr, nullptr, out);
+}
+
+extern "C" int32_t tanx_ctx_create_on_streams(uint64_t version, int32_t device, void *conv_stream,
+                                              void *general_stream, tanx_ctx **out)
+{
+    return ctx_create(version, device, static_cast<hipStream_t>(conv_stream),
+                      static_cast<hipStream_t>(general_stream), out);
+}
+
 static void ctx_free(tanx_ctx *ctx)
 {
     {
         DeviceGuard g(ctx->device);
-        if (ctx->proc) { (void)hipStreamSynchronize(ctx->proc); (void)hipStreamDestroy(ctx->proc); }
-        if (ctx->upd) { (void)hipStreamSynchronize(ctx->upd); (void)hipStreamDestroy(ctx->upd); }
+        // the caller's streams are only drained, the context's own also destroyed
+        if (ctx->proc) {
+            (void)hipStreamSynchronize(ctx->proc);
+            if (ctx->own_proc) (void)hipStreamDestroy(ctx->proc);
+        }
+        if (ctx->upd) {
+            (void)hipStreamSynchronize(ctx->upd);
+            if (ctx->own_upd) (void)hipStreamDestroy(ctx->upd);
+        }
         if (ctx->tw) (void)hipFree(ctx->tw);
         if (ctx->fft_dev) (void)hipFree(ctx->fft_dev);
         if (ctx->fft_host) (void)hipHostFree(ctx->fft_host);

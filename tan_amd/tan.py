@@ -116,15 +116,28 @@ class TANContext:
     """TANCreateContext + InitOpenCL equivalent: a HIP device with a process
     stream and an update stream (TANContextImpl.cpp:87-132, 264-328)."""
 
-    def __init__(self, device=0, version=TAN_FULL_VERSION):
+    def __init__(self, device=0, version=TAN_FULL_VERSION, conv_stream=None, general_stream=None):
+        """conv_stream / general_stream: the caller's streams (torch.cuda.Stream
+        or hipStream_t as int) for Process and for IR updates, as the
+        reference's InitOpenCL(convolutionQueue, generalQueue) takes the
+        application's queues; None: a stream of the context's own.  The
+        context keeps references to the stream objects it is given."""
         self._lib = _lib.load()
         h = ctypes.c_void_p()
-        rc = self._lib.tanx_ctx_create(version, device, ctypes.byref(h))
+        self._ext = (conv_stream, general_stream)
+        if conv_stream is None and general_stream is None:
+            rc = self._lib.tanx_ctx_create(version, device, ctypes.byref(h))
+        else:
+            def raw(st):
+                return None if st is None else int(getattr(st, "cuda_stream", st))
+            rc = self._lib.tanx_ctx_create_on_streams(version, device, raw(conv_stream), raw(general_stream),
+                                                      ctypes.byref(h))
         if rc != AMF_OK:
             raise RuntimeError(f"TANCreateContext failed: AMF_RESULT {rc}")
         self.handle = h
         self.device = device
         self._streams = None
+        self._raw = (self._lib.tanx_ctx_process_stream(h) or 0, self._lib.tanx_ctx_update_stream(h) or 0)
 
     def order_after_torch(self, update=False):
         """Makes the work this context queues next (on its process stream,
@@ -135,11 +148,12 @@ class TANContext:
         module call this first."""
         import torch
         if self._streams is None:
-            self._streams = (torch.cuda.ExternalStream(self._lib.tanx_ctx_process_stream(self.handle)),
-                             torch.cuda.ExternalStream(self._lib.tanx_ctx_update_stream(self.handle)))
+            self._streams = (torch.cuda.ExternalStream(self._raw[0]), torch.cuda.ExternalStream(self._raw[1]))
         cur = torch.cuda.current_stream()
-        self._streams[0].wait_stream(cur)
-        if update:
+        # work queued on the same stream is ordered already
+        if cur.cuda_stream != self._raw[0]:
+            self._streams[0].wait_stream(cur)
+        if update and cur.cuda_stream != self._raw[1]:
             self._streams[1].wait_stream(cur)
 
     def Terminate(self):

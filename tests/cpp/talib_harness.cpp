@@ -63,6 +63,12 @@ int main(int argc, char **argv)
     AMF_RESULT r = TANCreateContext(TAN_FULL_VERSION, &ctx);
     if (r != AMF_OK) return fail("TANCreateContext", r);
     if ((r = ctx->InitHIP(0)) != AMF_OK) return fail("InitHIP", r);
+    // GetOpenCLConvQueue / GetOpenCLGeneralQueue: the context's streams
+    if (!ctx->GetHIPConvStream() || !ctx->GetHIPGeneralStream() ||
+        ctx->GetHIPConvStream() == ctx->GetHIPGeneralStream())
+        return fail("GetHIPConvStream/GetHIPGeneralStream", -1);
+    if (ctx->InitHIPStreams(nullptr, nullptr, 0) != AMF_ALREADY_INITIALIZED)
+        return fail("InitHIPStreams after InitHIP", -1);
     TANConvolutionPtr conv;
     if ((r = TANCreateConvolution(ctx, &conv)) != AMF_OK) return fail("TANCreateConvolution", r);
     // the driver's GPU-UN method (TALibTestConvolution.cpp:78-85)

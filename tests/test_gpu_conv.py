@@ -374,3 +374,42 @@ def test_objects_keep_their_context():
         assert rc == T.AMF_OK and done == B
     assert np.isfinite(out).all()
     assert conv.Terminate() == T.AMF_OK
+
+
+def test_context_on_caller_streams(gpu_ctx):
+    """A context on the caller's streams (InitOpenCL with the application's
+    queues): Process and updates run on them, work the caller queued on the
+    same stream needs no ordering, and the output is the oracle's; the
+    context drains but keeps the caller's stream."""
+    import torch
+    L, B, C, nb = 8192, 256, 2, 30
+    s = torch.cuda.Stream()
+    ctx = T.TANContext(device=0, conv_stream=s, general_stream=s)
+    lib = ctx._lib
+    assert lib.tanx_ctx_process_stream(ctx.handle) == s.cuda_stream
+    assert lib.tanx_ctx_update_stream(ctx.handle) == s.cuda_stream
+    ir = _irs(C, L, 5)
+    x = T.signals.input_bank(C, nb * B, base_seed=21)
+    with torch.cuda.stream(s):
+        irs = torch.from_numpy(ir).cuda()
+        xd = torch.from_numpy(x).cuda()
+        yd = torch.zeros_like(xd)
+        conv = T.TANConvolution(ctx)
+        assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        assert conv.UpdateResponseTD(irs, L, None, BLOCKING) == T.AMF_OK
+        for b0, n in ((0, 7), (7, 1), (8, 22)):
+            rc, done = conv.Process(xd[:, b0 * B:], yd[:, b0 * B:], n * B)
+            assert rc == T.AMF_OK and done == n * B
+        y = yd.cpu().numpy()  # ordered on s
+    conv.Terminate()
+    ctx.Terminate()
+    s.synchronize()  # still a valid stream
+    orc = O.OracleConvolution(L, B, C)
+    orc.update_td(ir)
+    yo = np.zeros_like(x, dtype=np.float64)
+    for b in range(nb):
+        _, o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
+        yo[:, b * B:(b + 1) * B] = o
+    assert_parity(y, yo, "caller streams")
+    # no place for the handle
+    assert lib.tanx_ctx_create_on_streams(T.TAN_FULL_VERSION, 0, None, None, None) != T.AMF_OK
