@@ -317,7 +317,8 @@ struct tanx_conv {
     float *S = nullptr;      // [3][C][P] row scales (reduced-precision spectra)
     int irtype = IR_F32;
     float2 *F = nullptr;     // [C][P][B]
-    float2 *T = nullptr;     // [2 parity][2 filter][C][maxchunk][B] tail partial sums
+    float2 *T = nullptr;     // tail partial sums [2 parity][2 filter][C][maxchunk][B]; with
+                             // multi-block passes [2kp slots][2 filter][C][maxchunk][B]
     float *xprev = nullptr;  // [C][B]
     float *din = nullptr;    // [C][B]
     float *dout = nullptr;   // [C][B]
