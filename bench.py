@@ -183,7 +183,7 @@ def bench_sweep(T, torch, ctx, counts, ticks=96, method=UNIFORM_METHOD):
         torch.cuda.empty_cache()
         lat = np.array(lat)
         p99 = float(np.percentile(lat, 99))
-        # every launch of the device path (with passes: pass and head-only launches), per block
+        # every launch of the device path (single-block ticks, batched heads, fused head+pass), per block
         ms_all = kt["head_ms"] + kt["tail_ms"]
         tick_us = ms_all / max(1, kt["blocks"]) * 1e3
         gbps = (kt["head_alg_bytes"] + kt["tail_alg_bytes"]) / (ms_all * 1e-3) / 1e9 if ms_all > 0 else 0.0
