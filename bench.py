@@ -84,7 +84,12 @@ UNIFORM_METHOD = 3  # TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
 NONUNIFORM_METHOD = 2  # TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
 
 
-def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD):
+def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD, stream=None):
+    """stream: the torch stream ctx was created on (its process stream); the
+    inputs are written and Process is called on it."""
+    if stream is not None:
+        with torch.cuda.stream(stream):
+            return bench_throughput(T, torch, ctx, steps, warmup, timing, method)
     conv = T.TANConvolution(ctx)
     assert conv.InitGpu(method, L, B, C) == T.AMF_OK
     rc, chunks, us = conv.Autotune()
@@ -723,19 +728,26 @@ def main():
     import tan_amd as T
     T.load_library()
     ctx = T.TANContext(device=0)
+    # The throughput passes use a context created on a torch stream (the
+    # caller-stream form of the reference's InitOpenCL(queue, queue)): the
+    # inputs are written on that stream, so Process needs no cross-stream
+    # wait per call, as a C++ caller on its own stream would not.
+    hs = torch.cuda.Stream()
+    hctx = T.TANContext(device=0, conv_stream=hs, general_stream=hs)
     if args.profile_run:
         profile_run(T, torch, ctx)
+        hctx.Terminate()
         ctx.Terminate()
         return
 
     log("throughput pass")
-    dt, _, info, finite = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=False)
+    dt, _, info, finite = bench_throughput(T, torch, hctx, args.steps, args.warmup, timing=False, stream=hs)
     assert finite, "non-finite output"
     value = C * args.steps * B / FS / dt
     log(f"  {args.steps} blocks in {dt * 1e3:.1f} ms -> {value:.0f} real-time channels")
 
     log("kernel timing pass")
-    dt2, kt, info2, _ = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=True)
+    dt2, kt, info2, _ = bench_throughput(T, torch, hctx, args.steps, args.warmup, timing=True, stream=hs)
     kp = info2.get("pass_blocks", 1)
     head_blocks = None
     if kp > 1:
@@ -793,8 +805,8 @@ def main():
     nonuniform = None
     if not args.no_nonuniform:
         log("non-uniform pass")
-        dtn, _, infon, finite_n = bench_throughput(T, torch, ctx, args.steps, args.warmup, timing=False,
-                                                   method=NONUNIFORM_METHOD)
+        dtn, _, infon, finite_n = bench_throughput(T, torch, hctx, args.steps, args.warmup, timing=False,
+                                                   method=NONUNIFORM_METHOD, stream=hs)
         assert finite_n, "non-finite output (non-uniform)"
         nu_value = C * args.steps * B / FS / dtn
         log(f"  non-uniform (M = {infon['nu_multiple']}): {nu_value:.0f} real-time channels")
@@ -839,6 +851,7 @@ def main():
             log(f"  room generator: {cpu_room}")
             for r in room_rows:
                 r["cpu_responses_per_s"] = cpu_room["responses_per_s"] if cpu_room else None
+    hctx.Terminate()
     ctx.Terminate()
 
     big = max(rows, key=lambda r: r["channels"]) if rows else None

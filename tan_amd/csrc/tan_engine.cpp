@@ -333,14 +333,12 @@ struct tanx_conv {
     hipEvent_t run_ev[kRunRing] = {};
     int run_ring = 0;
 
-    hipEvent_t ev_upd = nullptr;   // last IR update finished
-    // Process work up to the latest crossfade: every launch that reads an IR
-    // slot other than the current one precedes it (crossfades read the
-    // previous and update slots, the blocks before a switch the armed one),
-    // so an update's transform, which writes a slot the current filter does
-    // not use, waits only for this point, not for all queued process work.
-    hipEvent_t ev_xf = nullptr;
-    bool xf_mark = false;  // a crossfade block was prepared: record ev_xf after its launch
+    // IR updates run on the process stream, in order with every block
+    // enqueued before and after them, so no event orders the two.  ev_upd
+    // marks an update only when it used the pinned staging (host IRs, a
+    // channel subset) or the caller blocks until it is ready.
+    hipEvent_t ev_upd = nullptr;
+    bool stage_busy = false;  // the last update's staging copies may still be in flight
     hipEvent_t ev_out = nullptr;   // host-path output copied
 
     // IR switch state (ConvolutionImpl.cpp:115-117, 869-927)
@@ -431,7 +429,6 @@ static void conv_free(tanx_conv *cv)
         if (cv->hrun) (void)hipHostFree(cv->hrun);
         for (auto &e : cv->run_ev) if (e) (void)hipEventDestroy(e);
         if (cv->ev_upd) (void)hipEventDestroy(cv->ev_upd);
-        if (cv->ev_xf) (void)hipEventDestroy(cv->ev_xf);
         if (cv->ev_out) (void)hipEventDestroy(cv->ev_out);
         cv->t_head.destroy();
         cv->t_tail.destroy();
@@ -744,7 +741,6 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     for (auto &ev : cv->desc_ev)
         if (hipEventRecord(ev, ctx->proc) != hipSuccess) return fail(TANX_UNEXPECTED);
     if (hipEventCreateWithFlags(&cv->ev_upd, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&cv->ev_xf, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&cv->ev_out, hipEventDisableTiming) != hipSuccess)
         return fail(TANX_UNEXPECTED);
 
@@ -755,8 +751,6 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
         std::memset(cv->hrun, 1, C);
         e = hipMemcpyAsync(cv->drun, cv->hrun, C, hipMemcpyHostToDevice, ctx->proc);
     }
-    if (e == hipSuccess) e = hipEventRecord(cv->ev_upd, ctx->upd);
-    if (e == hipSuccess) e = hipEventRecord(cv->ev_xf, ctx->proc);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->proc);
     if (e != hipSuccess) return fail(map_hip(e));
 
@@ -1010,15 +1004,6 @@ static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const T
     return TANX_OK;
 }
 
-// After the launch of a prepared crossfade block: ev_xf marks it.
-static int32_t mark_xfade(tanx_conv *cv)
-{
-    if (!cv->xf_mark) return TANX_OK;
-    TANX_CHECK(hipEventRecord(cv->ev_xf, cv->ctx->proc));
-    cv->xf_mark = false;
-    return TANX_OK;
-}
-
 // algorithmic bytes of the roles (regular blocks count one filter)
 static uint64_t head_bytes(const tanx_conv *cv, int nf)
 {
@@ -1069,8 +1054,6 @@ static int32_t enqueue_tail(tanx_conv *cv)
     a.nhead = 0;
     const int nf = fill_tail(cv, a);
     if (a.nchunk > 0) {
-        // the tail (or a pass) reads the update slot with a second filter
-        if (nf == 2) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
         int32_t rc = launch_timed(cv, cv->t_tail, nf, a, tail_bytes(cv, nf), cv->tail_alg_bytes);
         if (rc != TANX_OK) return rc;
     }
@@ -1080,8 +1063,7 @@ static int32_t enqueue_tail(tanx_conv *cv)
 
 // Process state machine (ConvolutionImpl.cpp:848-1043) for one block: flag
 // bookkeeping, switch schedule, slot rotation, and the launch arguments of
-// the block's head (and, with_tail, the next block's tail).  Sets *tail_nf
-// and *needs_update when the launch reads a freshly transformed IR slot.
+// the block's head (and, with_tail, the next block's tail).  Sets *tail_nf.
 // Switch schedule of block b (ConvolutionImpl.cpp:869-927): arms a handed-off
 // update, and at the armed block crossfades and rotates the slots (current ->
 // previous, update -> current).  Returns whether block b crossfades.
@@ -1102,7 +1084,6 @@ static bool advance_switch(tanx_conv *cv, uint64_t b)
         cv->prev = cv->cur;
         cv->cur = (cv->cur + 1) % 3;
         cv->upd = (cv->upd + 1) % 3;
-        cv->xf_mark = true;
     }
     return xfade;
 }
@@ -1112,15 +1093,14 @@ static bool advance_switch(tanx_conv *cv, uint64_t b)
 // uses the filters in effect there: both while a switch at (gm+1)M is
 // armed, the armed one alone at the block before an armed switch (gm's
 // blocks all follow it), else the current one.  The frame kernel at the last
-// block of frame g feeds block b+1.  Returns whether the update slot is read.
-static bool nu_block_args(tanx_conv *cv, uint64_t b_obj, bool xfade, TickArgs &a, FrameArgs &fa, bool *frame_due)
+// block of frame g feeds block b+1.
+static void nu_block_args(tanx_conv *cv, uint64_t b_obj, bool xfade, TickArgs &a, FrameArgs &fa, bool *frame_due)
 {
     const uint64_t M = (uint64_t)cv->M;
     // the object's frame clock (a frame phase shifts it); b and the switch
     // block are read in it below, ring slots and parities follow it
     const uint64_t b = b_obj + (uint64_t)cv->nu_shift;
     const uint64_t switch_at = cv->switch_at + (uint64_t)cv->nu_shift;
-    bool reads_upd = false;
     a.nu = 1;
     a.M = cv->M;
     a.N = cv->N;
@@ -1153,7 +1133,6 @@ static bool nu_block_args(tanx_conv *cv, uint64_t b_obj, bool xfade, TickArgs &a
         a.bSb = both ? cv->bscale_ptr(cv->upd) : nullptr;
         a.bTa = cv->acc_ptr((int)(gm & 1), 0);
         a.bTb = both ? cv->acc_ptr((int)(gm & 1), 1) : nullptr;
-        reads_upd = both || next_only;
     }
     *frame_due = ((b + 1) % M) == 0;
     if (*frame_due) {
@@ -1183,17 +1162,13 @@ static bool nu_block_args(tanx_conv *cv, uint64_t b_obj, bool xfade, TickArgs &a
         const int lN = __builtin_ctz((unsigned)cv->N);
         fa.twN = tw_table(cv->ctx->tw, lN);
         fa.tw2N = tw_table(cv->ctx->tw, lN + 1);
-        if (two) {
-            cv->zsel_frame = (long long)g;
-            reads_upd = true;
-        }
+        if (two) cv->zsel_frame = (long long)g;
     }
-    return reads_upd;
 }
 
 static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_stride, float *dout,
                              long long out_stride, size_t n, const uint32_t *flags, bool with_tail,
-                             TickArgs &a, int *tail_nf, bool *needs_update, bool *xfade_out,
+                             TickArgs &a, int *tail_nf, bool *xfade_out,
                              FrameArgs *fa = nullptr, bool *frame_due = nullptr)
 {
     int nrun = 0;
@@ -1244,23 +1219,18 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
         cv->delayed = false;
         cv->xfades++;
     }
-    bool nu_upd = false, fdue = false;
+    bool fdue = false;
     if (cv->nu) {
         FrameArgs tmp;
-        nu_upd = nu_block_args(cv, b, xfade, a, fa ? *fa : tmp, &fdue);
+        nu_block_args(cv, b, xfade, a, fa ? *fa : tmp, &fdue);
     }
     if (frame_due) *frame_due = fdue;
     cv->ticks++;
     cv->tail_due = true;
 
     int nf = 1;
-    bool pass_upd = false;
-    if (with_tail) {
-        nf = fill_tail(cv, a);
-        pass_upd = cv->kp > 1 && a.nchunk > 0 && a.tHb != nullptr;
-    }
+    if (with_tail) nf = fill_tail(cv, a);
     *tail_nf = nf;
-    *needs_update = xfade || nf == 2 || nu_upd || pass_upd;
     *xfade_out = xfade;
     return TANX_OK;
 }
@@ -1298,7 +1268,6 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
     t.Ha = cv->td_slot(xfade ? cv->prev : cv->cur);
     t.Hb = xfade ? cv->td_slot(cv->cur) : nullptr;
     t.xfade = xfade ? 1 : 0;
-    if (xfade) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     hipStream_t s = cv->ctx->proc;
     KernelTimer &kt = cv->t_head;
     if (cv->timing) {
@@ -1311,8 +1280,6 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
     } else {
         TANX_CHECK(launch_td(t, s));
     }
-    rc = mark_xfade(cv);
-    if (rc != TANX_OK) return rc;
     cv->tdpos = (cv->tdpos + cv->B) & (cv->len - 1);
     cv->crossfading = xfade;
     cv->last_xfade = xfade;
@@ -1332,7 +1299,6 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
 struct HeadQueue {
     TickArgs a[kHeadBatch];
     int n = 0;
-    bool needs_update = false;  // some head reads the update slot
 };
 
 // Launches the queued heads [w, e) (e = cv->ticks) with the pass fill_pass
@@ -1346,19 +1312,12 @@ static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
     const long long e = (long long)cv->ticks, w = e - n;
     TickArgs p = q.a[0];
     const int pnf = fill_pass(cv, w, e, p);
-    const bool upd = q.needs_update || pnf == 2;
-    q.needs_update = false;
-    if (upd) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     uint64_t hb = 0;
     for (int j = 0; j < n; j++) hb += head_bytes(cv, q.a[j].xfade ? 2 : 1) + near_bytes(cv, q.a[j]);
     const uint64_t tb = pnf > 0 ? tail_bytes(cv, pnf) : 0;
     KernelTimer &kt = pnf > 0 ? cv->t_tail : cv->t_head;
     uint64_t &alg = pnf > 0 ? cv->tail_alg_bytes : cv->head_alg_bytes;
-    if (n == 1) {
-        const int32_t rc = launch_timed(cv, kt, std::max(1, pnf), p, hb + tb, alg);
-        if (rc != TANX_OK) return rc;
-        return mark_xfade(cv);
-    }
+    if (n == 1) return launch_timed(cv, kt, std::max(1, pnf), p, hb + tb, alg);
     const TickArgs &a0 = q.a[0];
     HeadBatchArgs h{};
     h.C = a0.C;
@@ -1418,15 +1377,14 @@ static int32_t process_block_kp(tanx_conv *cv, const float *din, long long in_st
     }
     TickArgs a;
     int nf = 1;
-    bool upd = false, xf = false;
-    const int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, false, a, &nf, &upd, &xf);
+    bool xf = false;
+    const int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, false, a, &nf, &xf);
     if (rc != TANX_OK) {
         // the blocks already advanced are launched before the error returns
         const int32_t frc = flush_heads(cv, q);
         return frc != TANX_OK ? frc : rc;
     }
     q.a[q.n++] = a;
-    q.needs_update = q.needs_update || upd;
     tail_done(cv);
     const bool armed = cv->delayed && (long long)cv->switch_at == b + 1;
     if (xf || armed || flags || q.n == kHeadBatch || cv->B > kHeadBatchMaxB) return flush_heads(cv, q);
@@ -1442,11 +1400,10 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
     TickArgs a;
     FrameArgs fa;
     int tail_nf = 1;
-    bool needs_update = false, xfade = false, frame_due = false;
+    bool xfade = false, frame_due = false;
     int32_t rc = prepare_block(cv, din, in_stride, dout, out_stride, n, flags, with_tail, a, &tail_nf,
-                               &needs_update, &xfade, &fa, &frame_due);
+                               &xfade, &fa, &frame_due);
     if (rc != TANX_OK) return rc;
-    if (needs_update) TANX_CHECK(hipStreamWaitEvent(cv->ctx->proc, cv->ev_upd, 0));
     uint64_t hb = head_bytes(cv, xfade ? 2 : 1) + frame_mac_bytes(cv, a.bHb ? 2 : 1) + near_bytes(cv, a);
     if (frame_due) hb += frame_kernel_bytes(cv, fa.Hb ? 2 : 1);
     const FrameArgs *fp = frame_due ? &fa : nullptr;
@@ -1463,7 +1420,7 @@ static int32_t process_block(tanx_conv *cv, const float *din, long long in_strid
         rc = launch_timed(cv, cv->t_head, 1, a, hb, cv->head_alg_bytes, fp);
         if (rc != TANX_OK) return rc;
     }
-    return mark_xfade(cv);
+    return TANX_OK;
 }
 
 // Instantiated graph of k descriptor-driven tick launches (one per block).
@@ -1509,25 +1466,20 @@ static int32_t process_device_graph(tanx_conv *cv, const float *d_in, size_t in_
         cv->desc_ring = (cv->desc_ring + 1) % kDescRing;
         TANX_CHECK(hipEventSynchronize(cv->desc_ev[slot]));  // pinned entry free
         TickArgs *h = cv->h_desc + (size_t)slot * kGraphBlocks;
-        bool wait_update = false;
         for (int j = 0; j < k; j++) {
             int nf = 1;
-            bool nu = false, xf = false;
+            bool xf = false;
             int32_t rc = prepare_block(cv, d_in + (b + j) * B, (long long)in_stride, d_out + (b + j) * B,
-                                       (long long)out_stride, B, nullptr, true, h[j], &nf, &nu, &xf);
+                                       (long long)out_stride, B, nullptr, true, h[j], &nf, &xf);
             if (rc != TANX_OK) return rc;
-            wait_update = wait_update || nu;
             tail_done(cv);
         }
         hipGraphExec_t exec = nullptr;
         int32_t rc = graph_for(cv, k, &exec);
         if (rc != TANX_OK) return rc;
-        if (wait_update) TANX_CHECK(hipStreamWaitEvent(s, cv->ev_upd, 0));
         TANX_CHECK(hipMemcpyAsync(cv->d_desc, h, sizeof(TickArgs) * k, hipMemcpyHostToDevice, s));
         TANX_CHECK(hipEventRecord(cv->desc_ev[slot], s));
         TANX_CHECK(hipGraphLaunch(exec, s));
-        int32_t mrc = mark_xfade(cv);
-        if (mrc != TANX_OK) return mrc;
         b += k;
         if (processed) *processed = b * B;
     }
@@ -1712,9 +1664,17 @@ extern "C" int32_t tanx_conv_finalize(tanx_conv *cv)
 static int32_t update_common(tanx_conv *cv, const float *const *hir, const float *dir, size_t dstride,
                              size_t n, const uint32_t *flags, uint32_t opflags)
 {
+    // the process stream: blocks enqueued before the update finish reading
+    // the target slot before it is written, and blocks after it see the
+    // new spectra, with no cross-stream wait (a wait costs the process
+    // stream an idle gap of several microseconds)
     tanx_ctx *ctx = cv->ctx;
+    hipStream_t s = ctx->proc;
     if (n > (size_t)cv->L) return TANX_INVALID_ARG;
-    TANX_CHECK(hipEventSynchronize(cv->ev_upd));  // staging buffers free
+    if (cv->stage_busy) {
+        TANX_CHECK(hipEventSynchronize(cv->ev_upd));  // staging buffers free
+        cv->stage_busy = false;
+    }
     for (int ch = 0; ch < cv->C; ch++) {
         if (flags && (flags[ch] & TANX_CHANNEL_FLAG_FLUSH_STREAM)) {
             int32_t rc = flush_channel(cv, ch);
@@ -1732,10 +1692,9 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         }
         cv->hchan[nupd++] = ch;
     }
-    // the target slot must not be overwritten while queued process work
-    // reads it: its last reader is at or before the latest crossfade
-    TANX_CHECK(hipStreamWaitEvent(ctx->upd, cv->ev_xf, 0));
     const size_t chanBytes = (size_t)cv->P * cv->row_bytes();
+    bool staged = hir != nullptr && nupd > 0;  // pinned staging copies enqueued
+    const int *chan = nullptr;
     if (nupd > 0) {
         const float *src = dir;
         long long stride = (long long)dstride;
@@ -1758,15 +1717,20 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
                 if (!hir[ch]) return TANX_INVALID_ARG;
                 std::memcpy(cv->hir + (size_t)ch * cv->len, hir[ch], n * sizeof(float));
                 TANX_CHECK(hipMemcpyAsync(cv->dir + (size_t)ch * cv->len, cv->hir + (size_t)ch * cv->len,
-                                          n * sizeof(float), hipMemcpyHostToDevice, ctx->upd));
+                                          n * sizeof(float), hipMemcpyHostToDevice, s));
             }
             src = cv->dir;
             stride = cv->len;
         }
-        TANX_CHECK(hipMemcpyAsync(cv->dchan, cv->hchan, nupd * sizeof(int), hipMemcpyHostToDevice, ctx->upd));
+        // every channel: the kernels take row y as channel y, no list is copied
+        if (nupd < cv->C) {
+            TANX_CHECK(hipMemcpyAsync(cv->dchan, cv->hchan, nupd * sizeof(int), hipMemcpyHostToDevice, s));
+            staged = true;
+        }
+        chan = nupd < cv->C ? cv->dchan : nullptr;
         if (cv->td) {
-            TANX_CHECK(launch_td_load(src, stride, (long long)n, cv->dchan, nupd, cv->td_slot(target), cv->len,
-                                      ctx->upd));
+            TANX_CHECK(launch_td_load(src, stride, (long long)n, chan, nupd, cv->td_slot(target), cv->len,
+                                      s));
             for (int k = 0; k < nupd; k++) cv->staged[cv->hchan[k]] = 1;
         }
     }
@@ -1781,7 +1745,7 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         a.ir = src;
         a.stride = stride;
         a.n = (long long)n;
-        a.chan = cv->dchan;
+        a.chan = chan;
         a.H = cv->slot_ptr(target);
         a.S = cv->scale_ptr(target);
         a.irtype = cv->irtype;
@@ -1789,7 +1753,7 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         const int lB = __builtin_ctz((unsigned)cv->B);
         a.twM = tw_table(ctx->tw, lB);
         a.twN = tw_table(ctx->tw, lB + 1);
-        TANX_CHECK(launch_ir_transform(cv->B, nupd, a, ctx->upd));
+        TANX_CHECK(launch_ir_transform(cv->B, nupd, a, s));
         if (cv->nu) {
             // taps [N, len): frame partitions of N
             IrArgs b = a;
@@ -1801,7 +1765,7 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
             const int lN = __builtin_ctz((unsigned)cv->N);
             b.twM = tw_table(ctx->tw, lN);
             b.twN = tw_table(ctx->tw, lN + 1);
-            TANX_CHECK(launch_ir_transform(cv->N, nupd, b, ctx->upd));
+            TANX_CHECK(launch_ir_transform(cv->N, nupd, b, s));
         }
         for (int k = 0; k < nupd; k++) cv->staged[cv->hchan[k]] = 1;
     }
@@ -1810,27 +1774,32 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
     for (int ch : keep_list) {
         if (cv->td) {
             TANX_CHECK(hipMemcpyAsync(cv->td_slot(target) + (size_t)ch * cv->len, cv->td_slot(keep) + (size_t)ch * cv->len,
-                                      (size_t)cv->len * sizeof(float), hipMemcpyDeviceToDevice, ctx->upd));
+                                      (size_t)cv->len * sizeof(float), hipMemcpyDeviceToDevice, s));
             continue;
         }
         TANX_CHECK(hipMemcpyAsync(static_cast<char *>(cv->slot_ptr(target)) + (size_t)ch * chanBytes,
                                   static_cast<char *>(cv->slot_ptr(keep)) + (size_t)ch * chanBytes, chanBytes,
-                                  hipMemcpyDeviceToDevice, ctx->upd));
+                                  hipMemcpyDeviceToDevice, s));
         TANX_CHECK(hipMemcpyAsync(cv->scale_ptr(target) + (size_t)ch * cv->P, cv->scale_ptr(keep) + (size_t)ch * cv->P,
-                                  (size_t)cv->P * sizeof(float), hipMemcpyDeviceToDevice, ctx->upd));
+                                  (size_t)cv->P * sizeof(float), hipMemcpyDeviceToDevice, s));
         if (cv->nu) {
             const size_t cb = (size_t)cv->PB * cv->brow_bytes();
             TANX_CHECK(hipMemcpyAsync(static_cast<char *>(cv->bslot_ptr(target)) + (size_t)ch * cb,
                                       static_cast<char *>(cv->bslot_ptr(keep)) + (size_t)ch * cb, cb,
-                                      hipMemcpyDeviceToDevice, ctx->upd));
+                                      hipMemcpyDeviceToDevice, s));
             TANX_CHECK(hipMemcpyAsync(cv->bscale_ptr(target) + (size_t)ch * cv->PB,
                                       cv->bscale_ptr(keep) + (size_t)ch * cv->PB, (size_t)cv->PB * sizeof(float),
-                                      hipMemcpyDeviceToDevice, ctx->upd));
+                                      hipMemcpyDeviceToDevice, s));
         }
     }
-    TANX_CHECK(hipEventRecord(cv->ev_upd, ctx->upd));
+    const bool block = (opflags & TANX_OPERATION_FLAG_BLOCK_UNTIL_READY) != 0;
+    if (staged || block) TANX_CHECK(hipEventRecord(cv->ev_upd, s));
+    cv->stage_busy = staged;
     cv->pending = true;
-    if (opflags & TANX_OPERATION_FLAG_BLOCK_UNTIL_READY) TANX_CHECK(hipEventSynchronize(cv->ev_upd));
+    if (block) {
+        TANX_CHECK(hipEventSynchronize(cv->ev_upd));
+        cv->stage_busy = false;
+    }
     return TANX_OK;
 }
 
@@ -1912,12 +1881,11 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     for (; done < n; done++) {
         tanx_conv *cv = convs[done];
         int nf = 1;
-        bool upd = false, xf = false, fdue = false;
+        bool xf = false, fdue = false;
         rc = prepare_block(cv, d_in[done], (long long)in_stride[done], d_out[done], (long long)out_stride[done],
-                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &upd, &xf,
+                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &xf,
                            &fdescs[nframe], &fdue);
         if (rc != TANX_OK) break;
-        if (upd) TANX_CHECK(hipStreamWaitEvent(ctx->proc, cv->ev_upd, 0));
         tail_done(cv);
         passes = passes || descs[done].kn > 1;
         starts[done] = total;
@@ -1941,10 +1909,6 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
                 irtype, reinterpret_cast<const FrameArgs *>(ctx->g_dev + desc_bytes + start_bytes),
                 reinterpret_cast<const int *>(ctx->g_dev + desc_bytes + start_bytes + fdesc_bytes), nframe,
                 (unsigned)ftotal, ctx->proc));
-        for (uint32_t i = 0; i < done; i++) {
-            const int32_t mrc = mark_xfade(convs[i]);
-            if (mrc != TANX_OK) return mrc;
-        }
     } else {
         TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
     }

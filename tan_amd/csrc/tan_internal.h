@@ -177,7 +177,7 @@ struct IrArgs {
     const float *ir;       // channel ch IR at ir + ch * stride, n valid samples
     long long stride;
     long long n;
-    const int *chan;       // list of channels to transform (blockIdx.y indexes it)
+    const int *chan;       // list of channels to transform (blockIdx.y indexes it); null: channel blockIdx.y
     void *H;               // destination slot base [C][P][B] of IrType
     float *S;              // destination row scales [C][P]
     int irtype;

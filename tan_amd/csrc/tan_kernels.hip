@@ -1160,7 +1160,8 @@ __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs
 {
     __shared__ float2 s0[M], s1[M];
     __shared__ float smax[32];
-    const int i = blockIdx.x, ch = a.chan[blockIdx.y], tid = threadIdx.x, nt = blockDim.x;
+    const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int ch = a.chan ? a.chan[blockIdx.y] : (int)blockIdx.y;
     const float *h = a.ir + (size_t)ch * a.stride;
     const long long base = (long long)i * M;
     for (int n = tid; n < M; n += nt) {
@@ -1212,31 +1213,51 @@ __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs
 
 // FP32 IR spectra of NR partitions per workgroup (256 threads): the rows'
 // FFTs share the Stockham stages, so a transform of many short partitions
-// runs as a quarter (or half) as many workgroups.  Same arithmetic per row
-// as k_ir_transform.
+// runs as a quarter (or half) as many workgroups.  The twiddles are staged
+// in LDS with the samples (one memory round trip before the stages), and
+// aligned rows load four samples per lane.  Same arithmetic per row as
+// k_ir_transform.
 template <int M, int NR>
 __global__ __launch_bounds__(256) void k_ir_transform_rows(IrArgs a)
 {
-    __shared__ float2 s0[NR * M], s1[NR * M];
-    const int i0 = blockIdx.x * NR, ch = a.chan[blockIdx.y], tid = threadIdx.x, nt = blockDim.x;
+    __shared__ float2 s0[NR * M], s1[NR * M], tw[M], twn[M];
+    const int i0 = blockIdx.x * NR, tid = threadIdx.x, nt = blockDim.x;
+    const int ch = a.chan ? a.chan[blockIdx.y] : (int)blockIdx.y;
     const int nr = min(NR, a.P - i0);
     const float *h = a.ir + (size_t)ch * a.stride;
-    for (int e = tid; e < NR * M; e += nt) {
-        const int r = e / M, n = e - r * M;
-        float2 v = make_float2(0.f, 0.f);
-        if (r < nr && n < M / 2) {
-            const long long k = (long long)(i0 + r) * M + 2 * n;
-            v.x = k < a.n ? h[k] : 0.f;
-            v.y = k + 1 < a.n ? h[k + 1] : 0.f;
+    const bool al = (reinterpret_cast<uintptr_t>(h) & 15) == 0;
+    // row r holds [h[(i0+r)M .. +M), 0_M] as M complex pairs: lane e loads
+    // samples 4q..4q+3 of row r and zeroes the matching pairs of the upper half
+    for (int e = tid; e < NR * (M / 4); e += nt) {
+        const int r = e / (M / 4), q = e - r * (M / 4);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < nr) {
+            const long long k = (long long)(i0 + r) * M + 4 * q;
+            if (al && k + 3 < a.n) {
+                v = *reinterpret_cast<const float4 *>(h + k);
+            } else {
+                v.x = k < a.n ? h[k] : 0.f;
+                v.y = k + 1 < a.n ? h[k + 1] : 0.f;
+                v.z = k + 2 < a.n ? h[k + 2] : 0.f;
+                v.w = k + 3 < a.n ? h[k + 3] : 0.f;
+            }
         }
-        s0[e] = v;
+        float2 *row = s0 + r * M;
+        row[2 * q] = make_float2(v.x, v.y);
+        row[2 * q + 1] = make_float2(v.z, v.w);
+        row[M / 2 + 2 * q] = make_float2(0.f, 0.f);
+        row[M / 2 + 2 * q + 1] = make_float2(0.f, 0.f);
+    }
+    for (int m = tid; m < M; m += nt) {
+        tw[m] = a.twM[m];
+        twn[m] = a.twN[m];
     }
     __syncthreads();
-    const float2 *Z = fft_lds_n<M, false>(s0, s1, nr, a.twM);
+    const float2 *Z = fft_lds_n<M, false>(s0, s1, nr, tw);
     float2 *rows = reinterpret_cast<float2 *>(a.H) + ((size_t)ch * a.P + i0) * M;
     for (int e = tid; e < nr * M; e += nt) {
         const int r = e / M, k = e - r * M;
-        rows[e] = r2c_post<M>(Z + r * M, k, a.twN);
+        rows[e] = r2c_post<M>(Z + r * M, k, twn);
     }
 }
 
@@ -1358,7 +1379,7 @@ __global__ __launch_bounds__(256) void k_td(TdArgs a)
 __global__ void k_td_load(const float *__restrict__ ir, long long stride, long long n, const int *__restrict__ chan,
                           float *__restrict__ H, int len)
 {
-    const int ch = chan[blockIdx.y];
+    const int ch = chan ? chan[blockIdx.y] : (int)blockIdx.y;
     const float *src = ir + (size_t)ch * stride;
     float *dst = H + (size_t)ch * len;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < len; k += gridDim.x * blockDim.x)

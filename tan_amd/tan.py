@@ -118,8 +118,8 @@ class TANContext:
 
     def __init__(self, device=0, version=TAN_FULL_VERSION, conv_stream=None, general_stream=None):
         """conv_stream / general_stream: the caller's streams (torch.cuda.Stream
-        or hipStream_t as int) for Process and for IR updates, as the
-        reference's InitOpenCL(convolutionQueue, generalQueue) takes the
+        or hipStream_t as int) for Process and IR updates, and for TANFFT, as
+        the reference's InitOpenCL(convolutionQueue, generalQueue) takes the
         application's queues; None: a stream of the context's own.  The
         context keeps references to the stream objects it is given."""
         self._lib = _lib.load()
@@ -141,7 +141,7 @@ class TANContext:
 
     def order_after_torch(self, update=False):
         """Makes the work this context queues next (on its process stream,
-        and with update=True on its update stream too) start after the work
+        and with update=True on its general stream too) start after the work
         queued so far on torch's current stream, which may be writing the
         tensors the next call reads or writes.  The library's calls are
         asynchronous on its own streams; the device-tensor methods of this
@@ -243,7 +243,8 @@ class TANConvolution:
             t = buffers
             if num_samples is None:
                 num_samples = t.shape[1]
-            self._ctx.order_after_torch(update=True)
+            # the transform runs on the process stream
+            self._ctx.order_after_torch()
             return self._lib.tanx_conv_update_td_device(self.handle, t.data_ptr(), t.stride(0),
                                                         num_samples, fp, operation_flags)
         if num_samples is None:
