@@ -107,15 +107,15 @@ int32_t tanx_ctx_create_on_streams(uint64_t version, int32_t device, void *conv_
  * not be used again. */
 int32_t tanx_ctx_destroy(tanx_ctx *ctx);
 int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device);
-/* The context's process stream (a hipStream_t), on which every Process and
- * group launch of its objects runs; for callers that order their own work or
- * timing events against it (the reference's counterpart is
- * TANContext::GetOpenCLConvQueue, TrueAudioNext.h:596). */
+/* The context's process stream (a hipStream_t), on which every Process,
+ * IR update (copies and transform) and group launch of its objects runs, in
+ * call order; for callers that order their own work or timing events against
+ * it (the reference's counterpart is TANContext::GetOpenCLConvQueue,
+ * TrueAudioNext.h:596).  A caller that writes inputs or a device IR on its own
+ * stream orders this stream after that write. */
 void *tanx_ctx_process_stream(tanx_ctx *ctx);
-/* The context's update stream (a hipStream_t), on which IR updates read the
- * caller's IR and transform it (TANContext::GetOpenCLGeneralQueue,
- * TrueAudioNext.h:597).  A caller that writes a device IR on its own stream
- * orders this stream after that write before UpdateResponseTD. */
+/* The context's general stream (a hipStream_t), on which TANFFT transforms
+ * run (TANContext::GetOpenCLGeneralQueue, TrueAudioNext.h:597). */
 void *tanx_ctx_update_stream(tanx_ctx *ctx);
 
 /* ---- convolution ----------------------------------------------------------- */
@@ -225,7 +225,7 @@ int32_t tanx_conv_flush(tanx_conv *conv, uint32_t channel);
 /* GetNextFreeChannel (ConvolutionImpl.cpp:1084-1110). */
 int32_t tanx_conv_get_next_free_channel(tanx_conv *conv, uint32_t *channel,
                                         const uint32_t *flags);
-/* Waits for all work of this object (process and update streams). */
+/* Waits for all work of this object (on the context's process stream). */
 int32_t tanx_conv_synchronize(tanx_conv *conv);
 
 typedef struct tanx_conv_info {

@@ -1,5 +1,5 @@
 // Host engine of the MI355X TAN convolution library: contexts (device +
-// process/update streams), convolution objects (device-resident IR slots,
+// process/general streams), convolution objects (device-resident IR slots,
 // FDL and tail partial sums; the IR-switch state machine of the reference's
 // TANConvolutionImpl::Process) and the C ABI declared in tan_mi355x.h.
 #include "tan_internal.h"
@@ -128,7 +128,7 @@ static int32_t ctx_upload_twiddles(tanx_ctx *ctx)
     return TANX_OK;
 }
 
-// A context on `device` whose process and update streams are the caller's
+// A context on `device` whose process and general streams are the caller's
 // (conv, general) or, where NULL, streams of its own.
 static int32_t ctx_create(uint64_t version, int32_t device, hipStream_t conv, hipStream_t general, tanx_ctx **out)
 {
