@@ -81,6 +81,7 @@ def run_blocks(T, conv, x, y, banks, t0, t1):
 
 
 UNIFORM_METHOD = 3  # TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+TAIL_CHUNKS = 0  # --tail-chunks: 0 lets the autotuner choose
 NONUNIFORM_METHOD = 2  # TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
 
 
@@ -92,9 +93,14 @@ def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD
             return bench_throughput(T, torch, ctx, steps, warmup, timing, method)
     conv = T.TANConvolution(ctx)
     assert conv.InitGpu(method, L, B, C) == T.AMF_OK
-    rc, chunks, us = conv.Autotune()
-    assert rc == T.AMF_OK, rc
-    log(f"  autotune: {chunks} tail chunks, {us:.2f} us per block")
+    if TAIL_CHUNKS:
+        # pinned (a profiler slows the autotuner's dry runs and shifts its choice)
+        assert conv.SetTailChunks(TAIL_CHUNKS) == T.AMF_OK
+        log(f"  tail chunks pinned to {TAIL_CHUNKS}")
+    else:
+        rc, chunks, us = conv.Autotune()
+        assert rc == T.AMF_OK, rc
+        log(f"  autotune: {chunks} tail chunks, {us:.2f} us per block")
     total = warmup + steps
     nbank = total // UPDATE_EVERY + 1
     banks = [ir_bank_device(torch, C, L, 1000 + k) for k in range(nbank)]
@@ -705,6 +711,8 @@ def main():
     ap.add_argument("--latency-blocks", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tail-chunks", type=int, default=0,
+                    help="pin the throughput passes' tail chunks instead of autotuning (for profiled runs)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
@@ -719,6 +727,8 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
+    global TAIL_CHUNKS
+    TAIL_CHUNKS = args.tail_chunks
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (channels and "
               "sessions shard across GPUs without exchange; see the design notes)", file=sys.stderr)
