@@ -284,6 +284,13 @@ struct KernelTimer {
     }
 };
 
+// Objects whose pass rows total at most this (half the 256 MB Infinity Cache
+// of an MI355X) read them in k_window's multi-block passes with ordinary
+// loads, which keep them resident between launches (about 8% more
+// throughput at 64 channels x 65,536 taps); larger ones, and every other
+// launch, stream them with nontemporal loads.
+constexpr size_t kCachedRowsMax = (size_t)128 << 20;
+
 struct tanx_conv {
     tanx_ctx *ctx = nullptr;
     std::mutex mtx;
@@ -304,6 +311,9 @@ struct tanx_conv {
     // b's head adds partitions 0..b-porg itself), pchunk their chunk count,
     // covered the last block with sums.
     int kp = 1;
+    // the rows a pass streams (FDL, both filters of a crossfade, frame rows)
+    // fit in the Infinity Cache: k_window passes read them with ordinary loads
+    bool cached_rows = false;
     long long covered = 0;
     long long porg[2 * kPassMax][2] = {};
     int pchunk[2 * kPassMax][2] = {};
@@ -697,6 +707,11 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
     cv->kp = sh.kp;
+    {
+        const size_t rows = (size_t)cv->C * cv->P * cv->B * sizeof(float2) + 2 * cv->slot_bytes() +
+                            (cv->nu ? (size_t)cv->C * cv->RB * cv->N * sizeof(float2) + 2 * cv->bslot_bytes() : 0);
+        cv->cached_rows = !cv->td && rows <= kCachedRowsMax;
+    }
     for (auto &r : cv->pchunk) r[0] = r[1] = cv->nchunk;
     // block 0: a silent history, its (zero) sums start at partition 2
     cv->covered = 0;
@@ -881,6 +896,7 @@ static TickArgs tick_common(tanx_conv *cv)
     a.twM = tw_table(cv->ctx->tw, lB);
     a.twN = tw_table(cv->ctx->tw, lB + 1);
     a.fdl = cv->F;
+    a.cached = cv->cached_rows ? 1 : 0;
     a.kn = 1;
     a.kgap = 1;
     a.dA = a.dB = 1;
@@ -1886,6 +1902,8 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
                            cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &xf,
                            &fdescs[nframe], &fdue);
         if (rc != TANX_OK) break;
+        // a group streams every member's rows (group kernels ignore the flag)
+        descs[done].cached = 0;
         tail_done(cv);
         passes = passes || descs[done].kn > 1;
         starts[done] = total;

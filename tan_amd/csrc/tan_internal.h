@@ -57,6 +57,8 @@ struct TickArgs {
     const float2 *twM;     // W_B^t, t < B
     const float2 *twN;     // W_2B^t, t < 2B
     float2 *fdl;           // [C][P][B] ring of input spectra
+    int cached;            // nonzero: the streamed rows (FDL, IR) fit in the Infinity Cache and
+                           // k_window's pass reads them with ordinary loads; 0: nontemporal loads
 
     // head role
     int nhead;             // C or 0

@@ -142,12 +142,21 @@ constexpr int kTickThreads = 256;
 constexpr int kHeadChunkBatch = 16;  // partial-sum rows loaded together by the head
 
 // Streamed rows (IR spectra, FDL) are read once per block: nontemporal loads
-// keep them from displacing reused data in the caches.
+// keep them from displacing reused data in the caches.  A multi-block pass
+// over an object whose rows fit in the Infinity Cache reads them with
+// ordinary loads instead (ld_row<false>), so they stay resident from one
+// launch to the next.
 typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ld_stream(const float4 *p)
 {
     const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
     return make_float4(v.x, v.y, v.z, v.w);
+}
+template <bool NT>
+__device__ __forceinline__ float4 ld_row(const float4 *p)
+{
+    if constexpr (NT) return ld_stream(p);
+    return *p;
 }
 
 typedef unsigned int v2u __attribute__((ext_vector_type(2)));
@@ -165,8 +174,9 @@ template <>
 struct IrRow<IR_F32> {
     static constexpr int kBytes = 8;
     __device__ static float2 get1(const void *row, int k) { return reinterpret_cast<const float2 *>(row)[k]; }
-    // complex 2q and 2q+1, streamed
+    // complex 2q and 2q+1, streamed (getc: ordinary load)
     __device__ static float4 get2(const void *row, int q) { return ld_stream(reinterpret_cast<const float4 *>(row) + q); }
+    __device__ static float4 getc(const void *row, int q) { return reinterpret_cast<const float4 *>(row)[q]; }
 };
 
 template <>
@@ -182,6 +192,11 @@ struct IrRow<IR_F16> {
         const v2u w = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(row) + q);
         return make_float4(half_lo(w.x), half_hi(w.x), half_lo(w.y), half_hi(w.y));
     }
+    __device__ static float4 getc(const void *row, int q)
+    {
+        const v2u w = reinterpret_cast<const v2u *>(row)[q];
+        return make_float4(half_lo(w.x), half_hi(w.x), half_lo(w.y), half_hi(w.y));
+    }
 };
 
 template <>
@@ -195,6 +210,11 @@ struct IrRow<IR_BF16> {
     __device__ static float4 get2(const void *row, int q)
     {
         const v2u w = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(row) + q);
+        return make_float4(bf_lo(w.x), bf_hi(w.x), bf_lo(w.y), bf_hi(w.y));
+    }
+    __device__ static float4 getc(const void *row, int q)
+    {
+        const v2u w = reinterpret_cast<const v2u *>(row)[q];
         return make_float4(bf_lo(w.x), bf_hi(w.x), bf_lo(w.y), bf_hi(w.y));
     }
 };
@@ -549,7 +569,7 @@ __device__ void mac_rows(const MacView &w, float4 *red)
 // for output 0: each lane group walks its partitions in order and keeps the
 // last kPassMax rows it loaded, so every IR row and FDL row is read once and
 // used for every output.
-template <int M, int NF, int HT>
+template <int M, int NF, int HT, bool NT = true>
 __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
 {
     constexpr int H = M / 2;                     // float4 per row
@@ -578,7 +598,8 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
     }
     const bool dc0 = (r == 0);
     const float s0 = dc0 ? 0.f : 1.f;
-    auto frow = [&](int row, int v) { return ld_stream(F + (size_t)(row & mask) * H + v * R + r); };
+    auto frow = [&](int row, int v) { return ld_row<NT>(F + (size_t)(row & mask) * H + v * R + r); };
+    auto hrow2 = [&](const char *row, int q) { return NT ? IrRow<HT>::get2(row, q) : IrRow<HT>::getc(row, q); };
 
     float4 accA[KM][V], accB[KM][V], xw[KM][V];
 #pragma unroll
@@ -608,8 +629,8 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
             for (int v = 0; v < V; v++) {
                 if (ii < p1) {
                     xn[u][v] = frow(base - ii, v);
-                    ha[u][v] = IrRow<HT>::get2(HA + (size_t)ii * kRowBytes, v * R + r);
-                    if (NF == 2) hb[u][v] = IrRow<HT>::get2(HB + (size_t)ii * kRowBytes, v * R + r);
+                    ha[u][v] = hrow2(HA + (size_t)ii * kRowBytes, v * R + r);
+                    if (NF == 2) hb[u][v] = hrow2(HB + (size_t)ii * kRowBytes, v * R + r);
                 }
             }
         }
@@ -1018,7 +1039,8 @@ __global__ __launch_bounds__(kTickThreads) void k_window(HeadBatchArgs h, TickAr
         const int t = b - h.C;
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        tailk_role<M, NF, HT>(a, c, j, smem);
+        if (a.cached) tailk_role<M, NF, HT, false>(a, c, j, smem);
+        else tailk_role<M, NF, HT, true>(a, c, j, smem);
     }
 }
 
