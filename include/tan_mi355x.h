@@ -241,6 +241,9 @@ typedef struct tanx_conv_info {
     uint32_t ir_precision; /* TANX_IR_* */
     uint32_t nu_multiple;  /* blocks per frame of non-uniform objects, 1 = uniform */
     uint32_t pass_blocks;  /* blocks per tail pass (1 = a tail every block) */
+    uint32_t cached_rows;  /* 1: passes read the FDL and IR rows with ordinary loads, because the
+                              pass rows of every live object on the context total at most 128 MB
+                              (half the Infinity Cache); 0: nontemporal loads */
 } tanx_conv_info;
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */

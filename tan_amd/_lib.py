@@ -23,7 +23,8 @@ class ConvInfo(ctypes.Structure):
                 ("tail_chunks", ctypes.c_uint32), ("finalize_mode", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64),
                 ("crossfades", ctypes.c_uint64), ("ir_precision", ctypes.c_uint32),
-                ("nu_multiple", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32)]
+                ("nu_multiple", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32),
+                ("cached_rows", ctypes.c_uint32)]
 
 
 class ConvOptions(ctypes.Structure):

@@ -35,6 +35,11 @@ class GroupCall:
 
     def __init__(self, convs, inputs, outputs):
         n = len(convs)
+        for c, i, o in zip(convs, inputs, outputs):
+            # one block per row is read and written through pointer and row stride
+            for t in (i, o):
+                if t.dim() != 2 or t.shape[0] < c.channels or t.shape[1] < c.block:
+                    raise ValueError("group input/output views must hold (channels, >= block) samples")
         self.n = n
         self._lib = convs[0]._lib if n else None
         self.h = (ctypes.c_void_p * n)(*[c.handle for c in convs])

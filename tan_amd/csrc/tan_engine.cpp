@@ -66,6 +66,10 @@ struct tanx_ctx {
     // goes with the last one
     std::atomic<int> refs{1};
     int device = 0;
+    // pass rows (FDL and two IR slots) of every live object on the context:
+    // multi-block passes read them with ordinary loads only while the total
+    // fits the cached bound, so objects do not evict one another
+    std::atomic<uint64_t> pass_rows{0};
     // group launches: descriptors + block offsets, pinned ring and device copy
     std::mutex group_mtx;
     char *g_host = nullptr;     // [kGroupRing][g_cap]
@@ -284,11 +288,11 @@ struct KernelTimer {
     }
 };
 
-// Objects whose pass rows total at most this (half the 256 MB Infinity Cache
-// of an MI355X) read them in k_window's multi-block passes with ordinary
-// loads, which keep them resident between launches (about 8% more
-// throughput at 64 channels x 65,536 taps); larger ones, and every other
-// launch, stream them with nontemporal loads.
+// While the pass rows of all live objects on a context total at most this
+// (half the 256 MB Infinity Cache of an MI355X), k_window's multi-block
+// passes read them with ordinary loads, which keep them resident between
+// launches (about 8% more throughput at 64 channels x 65,536 taps); beyond
+// it, and in every other launch, rows are streamed with nontemporal loads.
 constexpr size_t kCachedRowsMax = (size_t)128 << 20;
 
 struct tanx_conv {
@@ -311,9 +315,9 @@ struct tanx_conv {
     // b's head adds partitions 0..b-porg itself), pchunk their chunk count,
     // covered the last block with sums.
     int kp = 1;
-    // the rows a pass streams (FDL, both filters of a crossfade, frame rows)
-    // fit in the Infinity Cache: k_window passes read them with ordinary loads
-    bool cached_rows = false;
+    // bytes of the rows a pass streams (FDL, both filters of a crossfade,
+    // frame rows), counted in the context's pass_rows while the object lives
+    uint64_t rows_bytes = 0;
     long long covered = 0;
     long long porg[2 * kPassMax][2] = {};
     int pchunk[2 * kPassMax][2] = {};
@@ -427,6 +431,7 @@ static void conv_free(tanx_conv *cv)
 {
     if (!cv) return;
     if (cv->ctx) {
+        cv->ctx->pass_rows.fetch_sub(cv->rows_bytes, std::memory_order_relaxed);
         DeviceGuard g(cv->ctx->device);
         (void)hipStreamSynchronize(cv->ctx->proc);
         (void)hipStreamSynchronize(cv->ctx->upd);
@@ -707,10 +712,10 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
     cv->kp = sh.kp;
-    {
-        const size_t rows = (size_t)cv->C * cv->P * cv->B * sizeof(float2) + 2 * cv->slot_bytes() +
-                            (cv->nu ? (size_t)cv->C * cv->RB * cv->N * sizeof(float2) + 2 * cv->bslot_bytes() : 0);
-        cv->cached_rows = !cv->td && rows <= kCachedRowsMax;
+    if (!cv->td) {
+        cv->rows_bytes = (uint64_t)cv->C * cv->P * cv->B * sizeof(float2) + 2 * cv->slot_bytes() +
+                         (cv->nu ? (uint64_t)cv->C * cv->RB * cv->N * sizeof(float2) + 2 * cv->bslot_bytes() : 0);
+        ctx->pass_rows.fetch_add(cv->rows_bytes, std::memory_order_relaxed);
     }
     for (auto &r : cv->pchunk) r[0] = r[1] = cv->nchunk;
     // block 0: a silent history, its (zero) sums start at partition 2
@@ -884,6 +889,12 @@ static int32_t sync_run_mask(tanx_conv *cv)
     return TANX_OK;
 }
 
+// ordinary row loads for this object's passes: the context's live pass rows fit the cached bound
+static bool cached_rows(const tanx_conv *cv)
+{
+    return !cv->td && cv->ctx->pass_rows.load(std::memory_order_relaxed) <= kCachedRowsMax;
+}
+
 static TickArgs tick_common(tanx_conv *cv)
 {
     TickArgs a{};
@@ -896,7 +907,7 @@ static TickArgs tick_common(tanx_conv *cv)
     a.twM = tw_table(cv->ctx->tw, lB);
     a.twN = tw_table(cv->ctx->tw, lB + 1);
     a.fdl = cv->F;
-    a.cached = cv->cached_rows ? 1 : 0;
+    a.cached = cached_rows(cv) ? 1 : 0;
     a.kn = 1;
     a.kgap = 1;
     a.dA = a.dB = 1;
@@ -2002,6 +2013,7 @@ extern "C" int32_t tanx_conv_get_info(const tanx_conv *cv, tanx_conv_info *info)
     info->crossfades = cv->xfades;
     info->nu_multiple = (uint32_t)cv->M;
     info->pass_blocks = (uint32_t)cv->kp;
+    info->cached_rows = cached_rows(cv) ? 1u : 0u;
     return TANX_OK;
 }
 

@@ -1344,7 +1344,7 @@ __global__ __launch_bounds__(M / 4 > 64 ? (M / 4 < 1024 ? M / 4 : 1024) : 64) vo
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_td(TdArgs a)
 {
-    extern __shared__ float td_smem[];
+    HIP_DYNAMIC_SHARED(float, td_smem)
     const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (!a.run[c]) return;
     const int len = a.len, n = a.n, mask = len - 1;

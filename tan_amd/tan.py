@@ -243,6 +243,10 @@ class TANConvolution:
             t = buffers
             if num_samples is None:
                 num_samples = t.shape[1]
+            # the library reads num_samples per row through the pointer and
+            # row stride: a shorter view would be read past its end
+            if t.dim() != 2 or t.shape[0] < self.channels or t.shape[1] < num_samples:
+                return AMF_INVALID_ARG
             # the transform runs on the process stream
             self._ctx.order_after_torch()
             return self._lib.tanx_conv_update_td_device(self.handle, t.data_ptr(), t.stride(0),
@@ -267,6 +271,12 @@ class TANConvolution:
         if _is_device_tensor(inputs):
             if num_samples is None:
                 num_samples = inputs.shape[1]
+            # whole blocks are read from and written to every row through the
+            # pointer and row stride: views shorter than that would be overrun
+            nwhole = (num_samples // self.block) * self.block
+            for t in (inputs, outputs):
+                if not _is_device_tensor(t) or t.dim() != 2 or t.shape[0] < self.channels or t.shape[1] < nwhole:
+                    return AMF_INVALID_ARG, 0
             self._ctx.order_after_torch()
             rc = self._lib.tanx_conv_process_device(
                 self.handle, inputs.data_ptr(), inputs.stride(0), outputs.data_ptr(),
@@ -428,6 +438,10 @@ def GroupProcess(convs, inputs, outputs, flag_masks=None):
     n = len(convs)
     if n == 0:
         return AMF_OK
+    for c, i, o in zip(convs, inputs, outputs):
+        for t in (i, o):
+            if not _is_device_tensor(t) or t.dim() != 2 or t.shape[0] < c.channels or t.shape[1] < c.block:
+                return AMF_INVALID_ARG
     lib = convs[0]._lib
     convs[0]._ctx.order_after_torch()
     hs = (ctypes.c_void_p * n)(*[c.handle for c in convs])

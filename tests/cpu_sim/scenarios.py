@@ -1,0 +1,206 @@
+"""Launch sequences of the GPU tests replayed on the CPU simulation build
+(memory safety and barrier checks before they reach a GPU).
+
+Run one scenario per process: `python scenarios.py <name>`; exit status 0
+when it completes, 86 when an access leaves its allocation (the simulator's
+trap), 87 on a barrier deadlock.  Numerics are checked against the oracle
+where the scenario is small enough.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(HERE))
+import simlib  # noqa: E402
+
+lib = simlib.load()
+import oracle_lib as O  # noqa: E402
+import tan_amd as T  # noqa: E402
+from tan_amd import batching as Bt  # noqa: E402
+
+UNIFORM = T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+_done = ctypes.c_size_t(0)
+
+
+def process(conv, xd, yd, t, nblocks, B):
+    rc = lib.tanx_conv_process_device(conv.handle, xd.at(t * B), xd.stride, yd.at(t * B), yd.stride, nblocks * B,
+                                      None, ctypes.byref(_done))
+    assert rc == T.AMF_OK and _done.value == nblocks * B, (rc, _done.value)
+
+
+def oracle(L, B, x, updates, nb):
+    C = x.shape[0]
+    orc = O.OracleConvolution(L, B, C)
+    yo = np.zeros((C, nb * B), np.float32)
+    for b in range(nb):
+        if b in updates:
+            assert orc.update_td(np.ascontiguousarray(updates[b])) == O.AMF_OK
+        yo[:, b * B:(b + 1) * B] = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))[1]
+    return yo
+
+
+def check(y, yo, label, tol=1e-5):
+    err = float(np.abs(y.astype(np.float64) - yo).max())
+    print(f"{label}: max|sim-oracle| = {err:.3e}", flush=True)
+    assert err <= tol, (label, err)
+
+
+def headline_calls(nb=120, C=64, L=65536, B=256, every=50):
+    """Calls of up to 50 blocks on a 64-channel object with an IR update
+    before each call (the last call is shorter), batched heads beside the
+    multi-block passes; against the oracle on 4 channels."""
+    ctx = T.TANContext(device=0)
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    x = T.signals.input_bank(C, nb * B, base_seed=501)
+    xd = simlib.Dev(C, nb * B, x)
+    yd = simlib.Dev(C, nb * B, 0.0)
+    ups = {}
+    for t in range(0, nb, every):
+        ups[t] = T.signals.ir_bank(C, L, t // every, base_seed=502)
+        bank = simlib.Dev(C, L, ups[t])
+        assert lib.tanx_conv_update_td_device(conv.handle, bank.ptr, bank.stride, L, None, 0) == T.AMF_OK
+        process(conv, xd, yd, t, min(every, nb - t), B)
+    check(yd.a[:4].copy(), oracle(L, B, x[:4], {b: h[:4] for b, h in ups.items()}, nb), "headline calls")
+
+
+def headline_overrun(C=4, L=4096, B=256):
+    """A planted mistake: a call asking for more blocks than the buffers
+    hold.  The simulator must trap it (exit status 86)."""
+    ctx = T.TANContext(device=0)
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    xd = simlib.Dev(C, 20 * B, 0.5)
+    yd = simlib.Dev(C, 20 * B, 0.0)
+    process(conv, xd, yd, 0, 16, B)
+    process(conv, xd, yd, 16, 8, B)  # 4 blocks past the end of the last row
+
+
+def streaming_pass(nb=60, C=256, L=65536, B=256):
+    """An object above the cached bound (nontemporal pass loads), calls of
+    uneven size and updates at uneven offsets, one while a switch is armed."""
+    ctx = T.TANContext(device=0)
+    conv = T.TANConvolution(ctx)
+    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    info = conv.Info()
+    assert info["pass_blocks"] == 4 and info["cached_rows"] == 0, info
+    rng = np.random.default_rng(611)
+    x = rng.uniform(-1, 1, (C, nb * B)).astype(np.float32)
+    xd = simlib.Dev(C, nb * B, x)
+    yd = simlib.Dev(C, nb * B, 0.0)
+    upd = [0, 23, 37, 38]
+    banks = {b: simlib.Dev(C, L, T.signals.ir_bank(C, L, k, base_seed=612)) for k, b in enumerate(upd)}
+    sizes = [1, 3, 7, 2, 13, 4, 1, 1, 9, 5]
+    cuts = sorted(set(upd + [nb]))
+    t = k = 0
+    while t < nb:
+        if t in banks:
+            assert lib.tanx_conv_update_td_device(conv.handle, banks[t].ptr, L, L, None, 0) == T.AMF_OK
+        n = min(sizes[k % len(sizes)], min(c for c in cuts if c > t) - t)
+        k += 1
+        process(conv, xd, yd, t, n, B)
+        t += n
+    check(yd.a[:4].copy(), oracle(L, B, x[:4], {b: banks[b].a[:4].copy() for b in upd}, nb), "streaming pass")
+
+
+def c3_group(nticks=300, nstreams=64):
+    """Group launches of the C3 draw (512 streams; every eighth one here,
+    phases from the full set) with inputs and outputs resident, as the C3
+    test drives them; two streams against the oracle."""
+    specs = [(L, B, 1) for L, B in Bt.draw_streams(512, seed=7)]
+    base, periods, phases, cycle = Bt.phase_plan([b for _, b, _ in specs])
+    sel = list(range(0, 512, 512 // nstreams))
+    ctx = T.TANContext(device=0)
+    convs, irs, xs, ys = {}, {}, {}, {}
+    for j in sel:
+        L, B, C = specs[j]
+        c = T.TANConvolution(ctx)
+        assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        irs[j] = T.signals.ir_bank(C, L, 0, base_seed=900 + j)
+        dev_ir = simlib.Dev(C, L, irs[j])
+        assert lib.tanx_conv_update_td_device(c.handle, dev_ir.ptr, L, L, None, 1) == T.AMF_OK
+        convs[j] = c
+        n = len(range(phases[j], nticks, periods[j]))
+        xs[j] = simlib.Dev(C, n * B, np.random.default_rng(j).uniform(-1, 1, (C, n * B)))
+        ys[j] = simlib.Dev(C, n * B, 0.0)
+    due = [[j for j in sel if t % periods[j] == phases[j]] for t in range(cycle)]
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    for t in range(nticks):
+        d = due[t % cycle]
+        blk = [(t - phases[j]) // periods[j] for j in d]
+        hs = (vp * len(d))(*[convs[j].handle.value for j in d])
+        ins = (vp * len(d))(*[xs[j].at(specs[j][1] * k) for j, k in zip(d, blk)])
+        outs = (vp * len(d))(*[ys[j].at(specs[j][1] * k) for j, k in zip(d, blk)])
+        st = (sz * len(d))(*[xs[j].stride for j in d])
+        assert lib.tanx_group_process_device(len(d), hs, ins, st, outs, st, None) == T.AMF_OK
+    for j in (sel[1], sel[-1]):
+        L, B, C = specs[j]
+        nb = xs[j].shape[1] // B
+        check(ys[j].a.copy(), oracle(L, B, xs[j].a.copy(), {0: irs[j]}, nb), f"C3 stream {j} (L={L}, B={B})")
+
+
+# sessions of the long serving test: (channels, IR taps, first tick, last tick + 1, update ticks)
+LONG_PLAN = {
+    0: (2, 131072, 0, 2200, (500, 1300)),
+    1: (1, 524288, 4, 2200, (700,)),
+    2: (3, 200000, 40, 1600, (90, 900)),
+    3: (1, 8192, 0, 300, (10,)),
+    4: (5, 300000, 900, 2200, (37,)),
+}
+
+
+def _serving_long(method, nticks, B=256):
+    """The long serving test's sessions arriving, updating and leaving, all
+    active ones advanced in one group launch per tick."""
+    ctx = T.TANContext(device=0)
+    live, xs, ys, irs, done_ticks = {}, {}, {}, {}, {}
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    for t in range(nticks):
+        for sid, (C, L, t0, t1, ups) in LONG_PLAN.items():
+            if t == t1 and sid in live:
+                live.pop(sid).Terminate()
+            if t == t0:
+                c = T.TANConvolution(ctx)
+                assert c.InitGpu(method, L, B, C) == T.AMF_OK
+                nb = min(t1, nticks) - t0
+                xs[sid] = simlib.Dev(C, nb * B, np.random.default_rng(sid).uniform(-1, 1, (C, nb * B)))
+                ys[sid] = simlib.Dev(C, nb * B, 0.0)
+                irs[sid] = [T.signals.ir_bank(C, L, k, base_seed=300 + sid) for k in range(len(ups) + 1)]
+                live[sid] = c
+            if sid in live and (t - t0 == 0 or (t - t0) in ups):
+                k = 0 if t == t0 else 1 + list(ups).index(t - t0)
+                dev = simlib.Dev(C, L, irs[sid][k])
+                assert lib.tanx_conv_update_td_device(live[sid].handle, dev.ptr, L, L, None, 0) == T.AMF_OK
+                dev.free()
+        act = list(live)
+        hs = (vp * len(act))(*[live[s].handle.value for s in act])
+        ins = (vp * len(act))(*[xs[s].at(B * (t - LONG_PLAN[s][2])) for s in act])
+        outs = (vp * len(act))(*[ys[s].at(B * (t - LONG_PLAN[s][2])) for s in act])
+        ist = (sz * len(act))(*[xs[s].stride for s in act])
+        assert lib.tanx_group_process_device(len(act), hs, ins, ist, outs, ist, None) == T.AMF_OK
+        if t % 200 == 0:
+            print(f"tick {t}: {len(act)} sessions", flush=True)
+    return xs, ys, irs
+
+
+def serving_long_uniform(nticks=2200):
+    xs, ys, irs = _serving_long(UNIFORM, nticks)
+    for sid in (1, 3):
+        C, L, t0, t1, ups = LONG_PLAN[sid]
+        nb = xs[sid].shape[1] // 256
+        upd = {0: irs[sid][0]}
+        upd.update({b: irs[sid][k + 1] for k, b in enumerate(ups)})
+        check(ys[sid].a.copy(), oracle(L, 256, xs[sid].a.copy(), upd, nb), f"serving session {sid} (L={L})")
+
+
+def serving_long_nonuniform(nticks=2200):
+    _serving_long(T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, nticks)
+
+
+if __name__ == "__main__":
+    globals()[sys.argv[1]]()
+    print("scenario complete", flush=True)

@@ -1,0 +1,53 @@
+"""Launch sequences of the GPU tests replayed on the CPU simulation build of
+the library (tests/cpu_sim): the library's own kernel and engine source runs
+on the host with every device allocation fenced by guard pages, so an access
+outside its buffer or a barrier that cannot be passed ends the replay
+(exit status 86 or 87) before the sequence reaches a GPU.  SIM_GUARD=start
+puts the guard before each allocation (underruns), SIM_FILL fills fresh
+allocations with garbage as a GPU leaves them."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SIM = os.path.join(HERE, "cpu_sim")
+
+
+@pytest.fixture(scope="module")
+def sim_built():
+    subprocess.check_call(["make", "-s", "-C", SIM])
+
+
+def _run(name, guard="end", fill=None, timeout=600):
+    env = dict(os.environ, SIM_GUARD=guard)
+    if fill is not None:
+        env["SIM_FILL"] = fill
+    p = subprocess.run([sys.executable, os.path.join(SIM, "scenarios.py"), name], env=env, capture_output=True,
+                       text=True, timeout=timeout)
+    print(p.stdout[-2000:], p.stderr[-2000:])
+    return p
+
+
+def test_planted_overrun_traps(sim_built):
+    """A call for more blocks than its buffers hold must trap in the replay."""
+    p = _run("headline_overrun")
+    assert p.returncode == 86 and "FAULT" in p.stderr, p.returncode
+
+
+@pytest.mark.parametrize("guard,fill", [("end", None), ("start", "0xff")])
+def test_headline_calls(sim_built, guard, fill):
+    p = _run("headline_calls", guard, fill)
+    assert p.returncode == 0, p.returncode
+
+
+def test_streaming_pass(sim_built):
+    p = _run("streaming_pass")
+    assert p.returncode == 0, p.returncode
+
+
+@pytest.mark.parametrize("guard,fill", [("end", None), ("start", "0xff")])
+def test_c3_group(sim_built, guard, fill):
+    p = _run("c3_group", guard, fill)
+    assert p.returncode == 0, p.returncode
