@@ -208,6 +208,21 @@ int32_t tanx_conv_process_device(tanx_conv *conv, const float *d_in, size_t in_s
 int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
                                   const size_t *in_stride, float *const *d_out, const size_t *out_stride,
                                   const uint32_t *const *flags);
+/* The same group step with no allocation on any call: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the launch
+ * descriptors go to a fixed device scratch array (one per GPU, about 0.6 MB,
+ * resolved when the context is created and held until the process ends)
+ * through a static host buffer, instead of the staging
+ * tanx_group_process_device allocates and grows.  At most 1,024 objects per
+ * call (TANX_INVALID_ARG beyond that).  Results are bit-identical to
+ * tanx_group_process_device.  Every call on a GPU shares that GPU's scratch
+ * array and its contents persist from call to call: calls that overlap on
+ * one GPU (concurrent streams or contexts) race on it, so issue them from
+ * one stream, one at a time. */
+int32_t tanx_group_process_device_noalloc(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
+                                          const size_t *in_stride, float *const *d_out, const size_t *out_stride,
+                                          const uint32_t *const *flags);
 /* ProcessDirect (ConvolutionImpl.cpp:719-762; TrueAudioNext.h ProcessDirect):
  * time-domain objects (TANX_METHOD_TIME_DOMAIN, IR length <= 4096) only,
  * AMF_NOT_SUPPORTED otherwise.  The IR of channel c is ir[c], taps

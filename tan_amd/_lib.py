@@ -90,6 +90,9 @@ _SIGS = {
     "tanx_group_process_device": (_i32, [_u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(_sz), ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                                          ctypes.c_void_p]),
+    "tanx_group_process_device_noalloc": (_i32, [_u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                 ctypes.POINTER(_sz), ctypes.POINTER(_vp), ctypes.POINTER(_sz),
+                                                 ctypes.c_void_p]),
     "tanx_fft_transform": (_i32, [_vp, _u32, _u32, _u32, ctypes.POINTER(_f32p), ctypes.POINTER(_f32p)]),
     "tanx_fft_transform_device": (_i32, [_vp, _u32, _u32, _u32, _vp, _vp, _sz]),
     "tanx_version": (ctypes.c_char_p, []),

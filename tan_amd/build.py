@@ -15,15 +15,31 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include")]
 
-SOURCES = ["tan_kernels.hip", "tan_room.hip", "tan_engine.cpp", "tan_api.cpp", "tan_wav.cpp"]
+SOURCES = ["tan_kernels.hip", "tan_room.hip", "tan_noalloc.hip", "tan_engine.cpp", "tan_api.cpp", "tan_wav.cpp"]
 
 
-def build_library(verbose=False):
+def build_library(verbose=False, jobs=None):
+    """Compiles every source to an object (in parallel) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
     out = os.path.join(LIBDIR, "libtan_mi355x.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+
+    def compile_one(pair):
+        src, obj = pair
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+
+    workers = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(compile_one, zip(srcs, objs)))
+    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -162,6 +162,9 @@ static int32_t ctx_create(uint64_t version, int32_t device, hipStream_t conv, hi
         rc = TANX_UNEXPECTED;
     }
     if (rc == TANX_OK) rc = ctx_upload_twiddles(ctx);
+    // the device address of the allocation-free group entry point's scratch
+    // array on this device (the device is current here)
+    if (rc == TANX_OK) rc = map_hip(noalloc_scratch_init(device));
     if (rc != TANX_OK) {
         tanx_ctx_destroy(ctx);
         return rc;
@@ -1862,11 +1865,10 @@ static int32_t group_staging(tanx_ctx *ctx, size_t bytes, char **host, int *slot
     return TANX_OK;
 }
 
-extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
-                                             const size_t *in_stride, float *const *d_out,
-                                             const size_t *out_stride, const uint32_t *const *flags)
+// Checks the arguments of a group call; returns the context and IR type.
+static int32_t group_check(uint32_t n, tanx_conv *const *convs, const float *const *d_in, const size_t *in_stride,
+                           float *const *d_out, const size_t *out_stride, tanx_ctx **ctx_out, int *irtype_out)
 {
-    if (n == 0) return TANX_OK;
     if (!convs || !d_in || !d_out || !in_stride || !out_stride) return TANX_INVALID_ARG;
     tanx_ctx *ctx = convs[0] ? convs[0]->ctx : nullptr;
     if (!ctx) return TANX_NOT_INITIALIZED;
@@ -1879,39 +1881,56 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
         if (!d_in[i] || !d_out[i] || in_stride[i] < (size_t)cv->B || out_stride[i] < (size_t)cv->B)
             return TANX_INVALID_ARG;
     }
-    // lock every object (address order) and the context's group staging
-    std::vector<tanx_conv *> order(convs, convs + n);
-    std::sort(order.begin(), order.end());
-    if (std::adjacent_find(order.begin(), order.end()) != order.end()) return TANX_INVALID_ARG;
-    std::lock_guard<std::mutex> glk(ctx->group_mtx);
-    std::vector<std::unique_lock<std::mutex>> locks;
-    locks.reserve(n);
-    for (tanx_conv *cv : order) locks.emplace_back(cv->mtx);
-    DeviceGuard g(ctx->device);
+    *ctx_out = ctx;
+    *irtype_out = irtype;
+    return TANX_OK;
+}
 
-    // staging: tick descriptors, their block offsets, frame descriptors and
-    // theirs (every piece 16-byte aligned)
-    auto al16 = [](size_t b) { return (b + 15) & ~size_t(15); };
-    const size_t desc_bytes = al16(sizeof(TickArgs) * n), start_bytes = al16(sizeof(int) * n);
-    const size_t fdesc_bytes = al16(sizeof(FrameArgs) * n), fstart_bytes = al16(sizeof(int) * n);
-    char *host = nullptr;
-    int slot = 0;
-    int32_t rc = group_staging(ctx, desc_bytes + start_bytes + fdesc_bytes + fstart_bytes, &host, &slot);
-    if (rc != TANX_OK) return rc;
-    TickArgs *descs = reinterpret_cast<TickArgs *>(host);
-    int *starts = reinterpret_cast<int *>(host + desc_bytes);
-    FrameArgs *fdescs = reinterpret_cast<FrameArgs *>(host + desc_bytes + start_bytes);
-    int *fstarts = reinterpret_cast<int *>(host + desc_bytes + start_bytes + fdesc_bytes);
+// Where one group call's descriptors go: host staging and the device copy,
+// both laid out [TickArgs x cap][int x cap][FrameArgs x cap][int x cap]
+// (16-byte aligned pieces) for up to `cap` objects.
+struct GroupStage {
+    char *host;
+    char *dev;
+    size_t desc_bytes, start_bytes, fdesc_bytes, fstart_bytes;
+    size_t total() const { return desc_bytes + start_bytes + fdesc_bytes + fstart_bytes; }
+};
+
+static GroupStage group_layout(size_t cap)
+{
+    GroupStage g{};
+    g.desc_bytes = align16(sizeof(TickArgs) * cap);
+    g.start_bytes = align16(sizeof(int) * cap);
+    g.fdesc_bytes = align16(sizeof(FrameArgs) * cap);
+    g.fstart_bytes = align16(sizeof(int) * cap);
+    return g;
+}
+
+// Advances every object (locked by the caller) one block and writes its
+// launch descriptors into g.host; then copies them to g.dev on the process
+// stream and launches the group kernels.  Objects already advanced are
+// launched even if a later one fails (its error is returned).  staged: the
+// event to record after the copy (the host staging is free again once it
+// completes), or nullptr.
+static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const *convs, const float *const *d_in,
+                         const size_t *in_stride, float *const *d_out, const size_t *out_stride,
+                         const uint32_t *const *flags, const GroupStage &g, hipEvent_t staged)
+{
+    TickArgs *descs = reinterpret_cast<TickArgs *>(g.host);
+    int *starts = reinterpret_cast<int *>(g.host + g.desc_bytes);
+    FrameArgs *fdescs = reinterpret_cast<FrameArgs *>(g.host + g.desc_bytes + g.start_bytes);
+    int *fstarts = reinterpret_cast<int *>(g.host + g.desc_bytes + g.start_bytes + g.fdesc_bytes);
     uint32_t done = 0;
     int total = 0, nframe = 0, ftotal = 0;
     bool passes = false;
+    int32_t rc = TANX_OK;
     for (; done < n; done++) {
         tanx_conv *cv = convs[done];
         int nf = 1;
         bool xf = false, fdue = false;
         rc = prepare_block(cv, d_in[done], (long long)in_stride[done], d_out[done], (long long)out_stride[done],
-                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &xf,
-                           &fdescs[nframe], &fdue);
+                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &xf, &fdescs[nframe],
+                           &fdue);
         if (rc != TANX_OK) break;
         // a group streams every member's rows (group kernels ignore the flag)
         descs[done].cached = 0;
@@ -1925,23 +1944,85 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
         }
     }
     if (done > 0) {
-        // objects already advanced are launched even if a later one failed;
         // the staging pieces are copied in one transfer at their offsets
-        const size_t bytes = desc_bytes + start_bytes + fdesc_bytes + fstart_bytes;
-        TANX_CHECK(hipMemcpyAsync(ctx->g_dev, host, bytes, hipMemcpyHostToDevice, ctx->proc));
-        TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
-        TANX_CHECK(launch_group_tick(irtype, passes, reinterpret_cast<const TickArgs *>(ctx->g_dev),
-                                     reinterpret_cast<const int *>(ctx->g_dev + desc_bytes), (int)done,
-                                     (unsigned)total, ctx->proc));
+        TANX_CHECK(hipMemcpyAsync(g.dev, g.host, g.total(), hipMemcpyHostToDevice, ctx->proc));
+        if (staged) TANX_CHECK(hipEventRecord(staged, ctx->proc));
+        TANX_CHECK(launch_group_tick(irtype, passes, reinterpret_cast<const TickArgs *>(g.dev),
+                                     reinterpret_cast<const int *>(g.dev + g.desc_bytes), (int)done, (unsigned)total,
+                                     ctx->proc));
         if (nframe > 0)
             TANX_CHECK(launch_group_frame(
-                irtype, reinterpret_cast<const FrameArgs *>(ctx->g_dev + desc_bytes + start_bytes),
-                reinterpret_cast<const int *>(ctx->g_dev + desc_bytes + start_bytes + fdesc_bytes), nframe,
+                irtype, reinterpret_cast<const FrameArgs *>(g.dev + g.desc_bytes + g.start_bytes),
+                reinterpret_cast<const int *>(g.dev + g.desc_bytes + g.start_bytes + g.fdesc_bytes), nframe,
                 (unsigned)ftotal, ctx->proc));
-    } else {
-        TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
+    } else if (staged) {
+        TANX_CHECK(hipEventRecord(staged, ctx->proc));
     }
     return rc;
+}
+
+extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
+                                             const size_t *in_stride, float *const *d_out,
+                                             const size_t *out_stride, const uint32_t *const *flags)
+{
+    if (n == 0) return TANX_OK;
+    tanx_ctx *ctx = nullptr;
+    int irtype = 0;
+    int32_t rc = group_check(n, convs, d_in, in_stride, d_out, out_stride, &ctx, &irtype);
+    if (rc != TANX_OK) return rc;
+    // lock every object (address order) and the context's group staging
+    std::vector<tanx_conv *> order(convs, convs + n);
+    std::sort(order.begin(), order.end());
+    if (std::adjacent_find(order.begin(), order.end()) != order.end()) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(n);
+    for (tanx_conv *cv : order) locks.emplace_back(cv->mtx);
+    DeviceGuard g(ctx->device);
+    GroupStage st = group_layout(n);
+    int slot = 0;
+    rc = group_staging(ctx, st.total(), &st.host, &slot);
+    if (rc != TANX_OK) return rc;
+    st.dev = ctx->g_dev;
+    return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, ctx->g_ev[slot]);
+}
+
+// Host staging of the allocation-free variant (pageable: a copy from it has
+// consumed it when hipMemcpyAsync returns), its lock order array and its
+// object locks, all static: the entry point allocates nothing.
+namespace {
+std::mutex g_noalloc_mtx;
+alignas(16) char g_noalloc_host[kNoAllocScratchBytes];
+tanx_conv *g_noalloc_order[kNoAllocGroupMax];
+}  // namespace
+
+extern "C" int32_t tanx_group_process_device_noalloc(uint32_t n, tanx_conv *const *convs,
+                                                     const float *const *d_in, const size_t *in_stride,
+                                                     float *const *d_out, const size_t *out_stride,
+                                                     const uint32_t *const *flags)
+{
+    if (n == 0) return TANX_OK;
+    if (n > (uint32_t)kNoAllocGroupMax) return TANX_INVALID_ARG;  // beyond the scratch array
+    tanx_ctx *ctx = nullptr;
+    int irtype = 0;
+    int32_t rc = group_check(n, convs, d_in, in_stride, d_out, out_stride, &ctx, &irtype);
+    if (rc != TANX_OK) return rc;
+    char *dev = static_cast<char *>(noalloc_scratch(ctx->device));
+    if (!dev) return TANX_NOT_INITIALIZED;  // the scratch address was not resolved at context creation
+    // one call at a time per process: the static staging, order and locks
+    std::lock_guard<std::mutex> slk(g_noalloc_mtx);
+    std::copy(convs, convs + n, g_noalloc_order);
+    std::sort(g_noalloc_order, g_noalloc_order + n);
+    if (std::adjacent_find(g_noalloc_order, g_noalloc_order + n) != g_noalloc_order + n) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    std::unique_lock<std::mutex> locks[kNoAllocGroupMax];
+    for (uint32_t i = 0; i < n; i++) locks[i] = std::unique_lock<std::mutex>(g_noalloc_order[i]->mtx);
+    DeviceGuard g(ctx->device);
+    // packed for n objects: within the scratch array, laid out for kNoAllocGroupMax
+    GroupStage st = group_layout(n);
+    st.host = g_noalloc_host;
+    st.dev = dev;
+    return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, nullptr);
 }
 
 extern "C" int32_t tanx_conv_update_td(tanx_conv *cv, const float *const *ir, size_t n,

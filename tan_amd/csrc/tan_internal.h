@@ -301,6 +301,24 @@ int32_t room_ears(const tanx_room *room, const tanx_source *source, const tanx_l
 hipError_t launch_room(const RoomEar *d_ears, int nears, long long max_early, int max_windows, int len,
                        unsigned *acc, float *out, long long out_stride, hipStream_t s);
 
+// Allocation-free group launches (tanx_group_process_device_noalloc): the
+// descriptors of at most kNoAllocGroupMax objects per call are staged in a
+// file-scope device array of kNoAllocScratchBytes (tan_noalloc.hip), laid out
+// as [TickArgs x max][int x max][FrameArgs x max][int x max], each piece
+// 16-byte aligned.
+constexpr int kNoAllocGroupMax = 1024;
+constexpr int kNoAllocMaxDevices = 64;
+constexpr size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
+constexpr size_t kNoAllocDescBytes = align16(sizeof(TickArgs) * kNoAllocGroupMax);
+constexpr size_t kNoAllocStartBytes = align16(sizeof(int) * kNoAllocGroupMax);
+constexpr size_t kNoAllocFrameBytes = align16(sizeof(FrameArgs) * kNoAllocGroupMax);
+constexpr size_t kNoAllocScratchBytes = kNoAllocDescBytes + 2 * kNoAllocStartBytes + kNoAllocFrameBytes;
+// Resolves the scratch array's address on `device`, which must be current;
+// once per device (later calls return at once).
+hipError_t noalloc_scratch_init(int device);
+// The address noalloc_scratch_init resolved for `device`, or nullptr.
+void *noalloc_scratch(int device);
+
 // Twiddle tables: tw + (2^l - 1) holds W_{2^l}^t = exp(-2 pi i t / 2^l), t < 2^l,
 // for l = 0..kMaxTwLog2.
 constexpr int kMaxTwLog2 = 13;

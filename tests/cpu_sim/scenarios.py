@@ -201,6 +201,50 @@ def serving_long_nonuniform(nticks=2200):
     _serving_long(T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, nticks)
 
 
+def noalloc_group():
+    """The allocation-free group entry point beside the allocating one on twin
+    objects (the sequence of tests/test_zz_no_alloc.py): bit-identical
+    outputs, calls whose object count drops and rises."""
+    specs = [(65536, 256, 4, UNIFORM), (5000, 128, 2, UNIFORM), (131072, 512, 1, UNIFORM),
+             (32768, 256, 2, T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM), (4096, 64, 3, UNIFORM)]
+    calls = [[0, 1, 2, 3, 4], [1, 3, 4]] * 24
+    ctx = T.TANContext(device=0)
+    irs = [simlib.Dev(C, L, T.signals.ir_bank(C, L, 0, base_seed=77 + i)) for i, (L, B, C, m) in enumerate(specs)]
+    sets = []
+    for _ in range(2):
+        objs = []
+        for (L, B, C, m), h in zip(specs, irs):
+            c = T.TANConvolution(ctx)
+            assert c.InitGpu(m, L, B, C) == T.AMF_OK
+            assert lib.tanx_conv_update_td_device(c.handle, h.ptr, L, L, None, 1) == T.AMF_OK
+            objs.append(c)
+        sets.append(objs)
+    nblk = [sum(1 for c in calls if i in c) for i in range(len(specs))]
+    xs = [simlib.Dev(C, n * B, np.random.default_rng(i).uniform(-1, 1, (C, n * B)))
+          for i, ((L, B, C, m), n) in enumerate(zip(specs, nblk))]
+    ys = [[simlib.Dev(C, n * B, 0.0) for (L, B, C, m), n in zip(specs, nblk)] for _ in range(2)]
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    blk = [0] * len(specs)
+    for idx in calls:
+        for fn, objs, y in ((lib.tanx_group_process_device, sets[0], ys[0]),
+                            (lib.tanx_group_process_device_noalloc, sets[1], ys[1])):
+            n = len(idx)
+            hs = (vp * n)(*[objs[i].handle.value for i in idx])
+            ins = (vp * n)(*[xs[i].at(specs[i][1] * blk[i]) for i in idx])
+            outs = (vp * n)(*[y[i].at(specs[i][1] * blk[i]) for i in idx])
+            st = (sz * n)(*[xs[i].stride for i in idx])
+            assert fn(n, hs, ins, st, outs, st, None) == T.AMF_OK
+        for i in idx:
+            blk[i] += 1
+    for i in range(len(specs)):
+        assert np.array_equal(ys[0][i].a, ys[1][i].a), i
+        assert float(np.abs(ys[1][i].a).max()) > 0.01, i
+    many = (vp * 1025)()
+    szs = (sz * 1025)()
+    assert lib.tanx_group_process_device_noalloc(1025, many, many, szs, many, szs, None) == T.AMF_INVALID_ARG
+    print("noalloc group: outputs bit-identical", flush=True)
+
+
 if __name__ == "__main__":
     globals()[sys.argv[1]]()
     print("scenario complete", flush=True)

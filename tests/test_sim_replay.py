@@ -51,3 +51,9 @@ def test_streaming_pass(sim_built):
 def test_c3_group(sim_built, guard, fill):
     p = _run("c3_group", guard, fill)
     assert p.returncode == 0, p.returncode
+
+
+@pytest.mark.parametrize("guard,fill", [("end", None), ("start", "0xff")])
+def test_noalloc_group(sim_built, guard, fill):
+    p = _run("noalloc_group", guard, fill)
+    assert p.returncode == 0, p.returncode
