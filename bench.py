@@ -1,27 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the time-varying multichannel convolution path on one MI355X.
 
-Workload (the headline configuration): 64 channels, FP32, 48 kHz, block 256,
-IR 65,536 taps of decaying noise (unit L2), white-noise input in [-1, 1],
-2,000 blocks, a fresh IR for every channel every 50 blocks (UpdateResponseTD)
-with a one-block linear crossfade.  Inputs, outputs and IR banks are resident
-in HBM before the timed region; the timed region is exactly --steps blocks.
-
-value = real-time channels sustained = channels * audio seconds / wall seconds.
+value: the real-time channel count at a 65,536-tap IR and 256-sample blocks
+(48 kHz): the most channels for which every one of `--steps` x 100
+consecutive ticks finishes within the 5.333 ms block period, one block in
+flight, with a fresh IR for every channel every 50 blocks and a one-block
+crossfade.  Channels are grouped in objects of 2,048 whose multi-block
+passes run on staggered blocks (pass phases) and whose IR updates are spread
+over the 50-block period; all groups advance in one group launch per tick.
+Inputs, outputs and IR banks are resident in HBM before the timed region.
+A step is 100 ticks (two update periods), so the default 20 steps are 2,000
+ticks.
 
 Also reported (one JSON line):
-  roofline      -- the dominant kernel (tail multiply-accumulate over the
-                   FDL), algorithmic bytes / average launch time from HIP
-                   events recorded on the process stream, against 8 TB/s;
-  latency       -- p50/p99/max block latency of the real-time loop
-                   (host buffers in, host buffers out, one block in flight)
-                   and deadline misses against 5.333 ms;
-  sweep         -- the largest channel count whose every block (one in
-                   flight, device-resident) finishes within the deadline;
-  cpu_baseline  -- the CPU restatement of the reference's CPU path
-                   (uniform partitioned FFT convolution, planar FMA CMAC,
-                   OpenMP threads = cores/4 as the reference sets them) on a
-                   bounded sample of the same workload.
+  roofline      -- the dominant kernel of the timed run (the group tick
+                   kernel): algorithmic bytes of the roles it ran / its
+                   launch-stamped duration, against 8 TB/s, with the HBM
+                   traffic per launch from the committed PMC summary;
+  c2_64ch       -- the 64-channel workload as configured (2,000 blocks, IR
+                   updates every 50 inside the timed window);
+  latency_64ch  -- one block per call with host buffers (PCIe included);
+  precision, mfma_dft, nonuniform, batching, serving, room_responses;
+  cpu_baseline  -- the oracle's restatement of the reference CPU path on a
+                   bounded sample of the 64-channel workload, with the
+                   reference's thread policy and with all cores.
 """
 import argparse
 import ctypes
@@ -44,6 +46,11 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "real-time 48kHz channels @64k-tap IR, 256 blk; p99 block latency vs plain FP32"
 WORKLOAD = ("plain GPU method: 64 ch FP32, block 256, IR 65536 taps (decaying noise), "
             "2000 blocks, fresh IR every 50 blocks, 1-block linear crossfade")
+RT_WORKLOAD = ("real-time channel count of the plain FP32 method: IR 65,536 taps (decaying noise), block 256 "
+               "at 48 kHz, a fresh IR for every channel every 50 blocks with a one-block linear crossfade, "
+               "every tick (one block in flight, device-resident data) within 5.333 ms over steps x 100 "
+               "consecutive ticks; channels in groups of 2,048 with staggered passes and updates, one group "
+               "launch per tick")
 
 
 def log(msg):
@@ -129,6 +136,168 @@ def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD
     return dt, kt, info, ok
 
 
+RT_GROUP = 2048        # channels per object (channel group) of the real-time count
+RT_STEP_TICKS = 100    # ticks per timed step: two IR update periods
+RT_RING = 8            # input/output blocks kept on the device
+RT_BANKS = 4           # IR banks the updates cycle through
+
+
+class RealTimeRig:
+    """The real-time channel count: channel groups of RT_GROUP channels (one
+    object each, 65,536 taps, B = 256, 4-block passes with phases g % 4), a
+    fresh IR for every channel every 50 blocks (group g's update at the ticks
+    t with (t + 50 g / k) % 50 == 0, so updates and their crossfades are
+    staggered over the period), all k active groups advanced by one group
+    launch per tick with one block in flight.  Everything runs on one torch
+    stream that is the context's process stream."""
+
+    def __init__(self, T, torch, ngroups):
+        self.T, self.torch = T, torch
+        self.hs = torch.cuda.Stream()
+        self.ctx = T.TANContext(device=0, conv_stream=self.hs, general_stream=self.hs)
+        self.k = ngroups
+        self.t = 0
+        with torch.cuda.stream(self.hs):
+            self.banks = [ir_bank_device(torch, RT_GROUP, L, 5000 + i) for i in range(RT_BANKS)]
+            self.x = noise_device(torch, RT_RING * ngroups * RT_GROUP, B, 11).view(RT_RING, ngroups * RT_GROUP, B)
+            self.y = torch.zeros_like(self.x)
+            self.objs = []
+            for g in range(ngroups):
+                conv = T.TANConvolution(self.ctx, pass_blocks=4, pass_phase=g % 4)
+                rc = conv.InitGpu(UNIFORM_METHOD, L, B, RT_GROUP)
+                if rc != T.AMF_OK:
+                    log(f"  group {g}: InitGpu returned {rc}; {g} groups")
+                    break
+                assert conv.UpdateResponseTD(self.banks[g % RT_BANKS], L, None, 0) == T.AMF_OK
+                self.objs.append(conv)
+        self.kmax = len(self.objs)
+        self.updates = [0] * self.kmax
+        self.calls = {}
+        self.ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(2)]
+        torch.cuda.synchronize()
+
+    def _call(self, k, r):
+        from tan_amd.batching import GroupCall
+        key = (k, r)
+        if key not in self.calls:
+            rows = lambda a, g: a[r, g * RT_GROUP:(g + 1) * RT_GROUP]  # noqa: E731
+            self.calls[key] = GroupCall(self.objs[:k], [rows(self.x, g) for g in range(k)],
+                                        [rows(self.y, g) for g in range(k)])
+        return self.calls[key]
+
+    def tick(self, k):
+        """One tick of the first k groups: their due IR updates, then one group
+        launch; returns (host ms, device ms) from the tick's start to the end
+        of its kernels (one block in flight)."""
+        T = self.T
+        e0, e1 = self.ev[self.t & 1]
+        t0 = time.perf_counter()
+        e0.record(self.hs)
+        for g in range(k):
+            if (self.t + (g * UPDATE_EVERY) // k) % UPDATE_EVERY == 0:
+                self.updates[g] += 1
+                bank = self.banks[(self.updates[g] + g) % RT_BANKS]
+                # the transform runs on the process stream, ahead of this tick's launch
+                assert self.objs[g].UpdateResponseTD(bank, L, None, 0) == T.AMF_OK
+        rc = self._call(k, self.t % RT_RING).launch(order=False)
+        assert rc == T.AMF_OK, rc
+        e1.record(self.hs)
+        e1.synchronize()
+        host = (time.perf_counter() - t0) * 1e3
+        self.t += 1
+        return host, e0.elapsed_time(e1)
+
+    def run(self, k, n):
+        hosts, devs = [], []
+        with self.torch.cuda.stream(self.hs):
+            for _ in range(n):
+                h, d = self.tick(k)
+                hosts.append(h)
+                devs.append(d)
+        return np.asarray(hosts), np.asarray(devs)
+
+    def close(self):
+        for o in self.objs:
+            o.Terminate()
+        self.ctx.Terminate()
+        self.objs, self.calls, self.banks = [], {}, []
+        self.x = self.y = None
+        self.torch.cuda.synchronize()
+        self.torch.cuda.empty_cache()
+
+
+def bench_realtime(T, torch, steps, warmup, margin=0.97):
+    """Largest channel count whose every tick of `steps` x 100 consecutive
+    ticks (IR updates and crossfades included) finishes within the 5.333 ms
+    block period with one block in flight.
+
+    Groups are added while HBM holds them; probes then lower the number of
+    active groups from that maximum (never raising it, so the active groups
+    keep equal block counts and their pass phases stay staggered) until a
+    probe of 100 ticks stays under margin x deadline; the timed run of
+    exactly `steps` steps follows, one group fewer after any miss."""
+    from tan_amd import serving as S
+    gc_ = __import__("gc")
+    per_group = S.estimate_bytes(T._lib.load(), L, B, RT_GROUP)
+    free, _ = torch.cuda.mem_get_info()
+    reserve = RT_BANKS * RT_GROUP * L * 4 + (6 << 30)
+    kmax = int((free - reserve) // (per_group + 2 * RT_RING * RT_GROUP * B * 4))
+    log(f"real-time count: {free / 2**30:.0f} GiB free, {per_group / 2**20:.0f} MiB per group of {RT_GROUP} "
+        f"channels: up to {kmax} groups")
+    rig = RealTimeRig(T, torch, kmax)
+    try:
+        k = rig.kmax
+        # a real-time loop: no garbage-collector pauses inside ticks
+        gc_.collect()
+        gc_.disable()
+        rig.run(k, 2 * UPDATE_EVERY)  # warm-up: every group updated and crossfaded once
+        probes = []
+        while k > 0:
+            h, d = rig.run(k, 2 * UPDATE_EVERY)
+            p99 = float(np.percentile(h, 99))
+            probes.append({"groups": k, "channels": k * RT_GROUP, "max_host_ms": float(h.max()),
+                           "p99_host_ms": p99, "max_device_ms": float(d.max())})
+            log(f"  probe {probes[-1]}")
+            if h.max() <= margin * DEADLINE_MS:
+                break
+            # tick time grows with the active groups: scale by the p99 tick
+            # (a lone outlier does not steer the search), at least one group
+            # and at most 30% down per probe
+            k = max(1, max(int(k * 0.7), min(k - 1, int(k * margin * DEADLINE_MS / p99))))
+        gc_.enable()
+        result = None
+        for attempt in range(4):
+            rig.run(k, warmup * RT_STEP_TICKS)
+            rig.ctx.EnableGroupTiming(True)
+            rig.ctx.ReadGroupTiming(reset=True)
+            gc_.collect()
+            gc_.disable()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            h, d = rig.run(k, steps * RT_STEP_TICKS)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            gc_.enable()
+            gt = rig.ctx.ReadGroupTiming(reset=True)
+            rig.ctx.EnableGroupTiming(False)
+            misses = int((h > DEADLINE_MS).sum())
+            result = {"groups": k, "channels": k * RT_GROUP, "ticks": int(h.size), "wall_s": wall,
+                      "misses": misses, "p50_host_ms": float(np.percentile(h, 50)),
+                      "p99_host_ms": float(np.percentile(h, 99)), "max_host_ms": float(h.max()),
+                      "p50_device_ms": float(np.percentile(d, 50)), "p99_device_ms": float(np.percentile(d, 99)),
+                      "max_device_ms": float(d.max()), "group_timing": gt, "attempt": attempt,
+                      "probes": probes}
+            log(f"  timed run: {k} groups, {misses} misses, max {h.max():.3f} ms, p99 {result['p99_host_ms']:.3f} ms")
+            if misses == 0 or k == 1:
+                break
+            k -= 1
+        finite = bool(torch.isfinite(rig.y).all().item())
+        result["outputs_finite"] = finite
+        return result
+    finally:
+        rig.close()
+
+
 def bench_latency(T, ctx, nblocks):
     """Real-time loop: host buffers, one block per Process call."""
     conv = T.TANConvolution(ctx)
@@ -154,61 +323,6 @@ def bench_latency(T, ctx, nblocks):
             "max_ms": float(lat.max()), "blocks": int(lat.size),
             "deadline_misses": int((lat > DEADLINE_MS).sum()),
             "worst": [[int(i) + 8, float(lat[i])] for i in worst]}
-
-
-def bench_sweep(T, torch, ctx, counts, ticks=96, method=UNIFORM_METHOD):
-    """Per-block latency with one block in flight, device-resident data."""
-    from tan_amd import serving as S
-    rows = []
-    best = 0
-    for ch in counts:
-        free, _ = torch.cuda.mem_get_info()
-        # the object, its IR staging and 2 x 64 blocks of input and output
-        need = (S.estimate_bytes(ctx._lib, L, B, ch, method=method) + ch * L * 4 + 2 * ch * 64 * B * 4) * 1.05
-        if need > free:
-            log(f"sweep: skip {ch} channels (needs {need / 2**30:.1f} GiB)")
-            break
-        conv = T.TANConvolution(ctx)
-        assert conv.InitGpu(method, L, B, ch) == T.AMF_OK
-        h = ir_bank_device(torch, ch, L, 77)
-        torch.cuda.synchronize()
-        assert conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY) == 0
-        del h
-        x = noise_device(torch, ch, 64 * B, 5)
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        for t in range(16):
-            conv.Process(x[:, (t % 64) * B:], y[:, (t % 64) * B:], B)
-        conv.Synchronize()
-        conv.EnableTiming(True)
-        conv.ReadTiming(reset=True)
-        lat = []
-        for t in range(ticks):
-            t0 = time.perf_counter()
-            conv.Process(x[:, (t % 64) * B:], y[:, (t % 64) * B:], B)
-            conv.Synchronize()
-            lat.append((time.perf_counter() - t0) * 1e3)
-        kt = conv.ReadTiming()
-        conv.Terminate()
-        del x, y
-        torch.cuda.empty_cache()
-        lat = np.array(lat)
-        p99 = float(np.percentile(lat, 99))
-        # every launch of the device path (single-block ticks, batched heads, fused head+pass), per block
-        ms_all = kt["head_ms"] + kt["tail_ms"]
-        tick_us = ms_all / max(1, kt["blocks"]) * 1e3
-        gbps = (kt["head_alg_bytes"] + kt["tail_alg_bytes"]) / (ms_all * 1e-3) / 1e9 if ms_all > 0 else 0.0
-        row = {"channels": ch, "method": "nonuniform" if method == NONUNIFORM_METHOD else "uniform",
-               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": p99,
-               "max_ms": float(lat.max()), "tick_kernel_us": tick_us,
-               "tick_GBps": gbps, "meets_deadline": bool(lat.max() <= DEADLINE_MS)}
-        log(f"sweep: {row}")
-        rows.append(row)
-        if row["meets_deadline"]:
-            best = ch
-        else:
-            break
-    return best, rows
 
 
 def bench_sweep_phased(T, torch, ctx, counts, ticks=96):
@@ -353,18 +467,25 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
         chosen = batch.tune(cycles=8)
         tune_s = time.perf_counter() - t0
         after = batch.time_ticks(cyc * 64) / (cyc * 64)
-        # steady stream of ticks: device time over the whole run
+        # steady stream of ticks: device time over the whole run, and the
+        # launch-stamped group kernels with the bytes their roles streamed
+        ctx.EnableGroupTiming(True)
+        ctx.ReadGroupTiming(reset=True)
         total_ms = batch.time_ticks(ticks)
+        gt = ctx.ReadGroupTiming(reset=True)
+        ctx.EnableGroupTiming(False)
         per = batch.time_ticks(min(ticks, 1000), per_tick=True)
-        gb = sum(batch.bytes_per_tick) / cyc / (total_ms / ticks * 1e-3) / 1e9
+        kms = gt["tick_ms"] + gt["frame_ms"]
+        gb = (gt["tick_alg_bytes"] + gt["frame_alg_bytes"]) / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
         row = {"streams": n, "base_block": batch.base, "deadline_ms": period,
                "objects_per_launch": [len(d) for d in batch.due],
                "tick_ms_default_chunks": before, "tick_ms_tuned": after, "tune_seconds": tune_s,
                "tuned_chunks": {f"B{b}_L{ln}": c for (b, ln), c in sorted(chosen.items())},
                "mean_tick_ms": total_ms / ticks, "p50_tick_ms": _pct(per, 50), "p99_tick_ms": _pct(per, 99),
                "max_tick_ms": float(max(per)), "deadline_misses": int(sum(1 for v in per if v > period)),
-               "alg_GBps": gb, "hbm_frac": gb / HBM_PEAK_GBPS,
-               "realtime_factor": period / (total_ms / ticks)}
+               "kernel_alg_GBps": gb, "hbm_frac": gb / HBM_PEAK_GBPS,
+               "kernel_ms_per_tick": kms / ticks, "alg_MB_per_tick": (gt["tick_alg_bytes"] + gt["frame_alg_bytes"])
+               / ticks / 1e6, "realtime_factor": period / (total_ms / ticks)}
         log(f"  batching {n} streams: tick {row['mean_tick_ms'] * 1e3:.1f} us (default chunks "
             f"{before * 1e3:.1f} us), p99 {row['p99_tick_ms'] * 1e3:.1f} us, {gb:.0f} GB/s, "
             f"tuned {row['tuned_chunks']}")
@@ -425,13 +546,16 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
         srv.finish(srv.tick())
     setup_s = time.perf_counter() - t0
     ticks = int(seconds * FS / B)
-    lat, enq, ctl, chans, sess, byts, waits = [], [], [], [], [], [], []
+    lat, enq, ctl, chans, sess, waits = [], [], [], [], [], []
     admitted = departed = updates = 0
     pend = None
     # a real-time control loop: no garbage-collector pauses inside ticks
     import gc
     gc.collect()
     gc.disable()
+    # launch-stamped group kernels and the bytes their roles streamed
+    ctx.EnableGroupTiming(True)
+    ctx.ReadGroupTiming(reset=True)
     wall0 = time.perf_counter()
     for t in range(ticks):
         c0 = time.perf_counter()
@@ -458,9 +582,10 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
         enq.append(rec.enqueue_ms)
         chans.append(rec.channels)
         sess.append(rec.sessions)
-        byts.append(rec.alg_bytes)
     wall = time.perf_counter() - wall0
     gc.enable()
+    gt = ctx.ReadGroupTiming(reset=True)
+    ctx.EnableGroupTiming(False)
     finite = bool(torch.isfinite(srv.out_arena).all().item())
     queued = srv.admission.waiting()
     srv.shutdown()
@@ -468,7 +593,8 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
     torch.cuda.synchronize()
     period = srv.period_ms
     lat_a = np.asarray(lat)
-    gbps = np.asarray(byts) / (lat_a * 1e-3) / 1e9
+    kms = gt["tick_ms"] + gt["frame_ms"]
+    gbps = (gt["tick_alg_bytes"] + gt["frame_alg_bytes"]) / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     return {"method": "nonuniform" if method == NONUNIFORM_METHOD else "uniform",
             "block": B, "deadline_ms": period, "audio_seconds": ticks * B / FS, "ticks": ticks,
             "ir_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[prec],
@@ -481,7 +607,8 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
             "deadline_misses": int((lat_a > period).sum()), "miss_rate": float((lat_a > period).mean()),
             "p50_enqueue_ms": _pct(enq, 50), "p99_enqueue_ms": _pct(enq, 99),
             "p50_control_ms": _pct(ctl, 50), "p99_control_ms": _pct(ctl, 99),
-            "alg_GBps_p50": _pct(gbps, 50), "hbm_frac_p50": _pct(gbps, 50) / HBM_PEAK_GBPS,
+            "kernel_alg_GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBPS,
+            "kernel_ms_per_tick": kms / ticks,
             "realtime_factor": ticks * B / FS / wall, "outputs_finite": finite}
 
 
@@ -618,22 +745,25 @@ def cpu_room_baseline():
             "sample": f"one {ROOM_LEN}-tap response (both ears) in the same room, scalar C"}
 
 
-def pmc_traffic(grid_threads, kernel_prefix="void tanx::k_tick<256, 1, 0, 1>"):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_k_tick.json")
+def cpu_model():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    name, n = None, 0
     try:
-        data = json.load(open(path))
-    except (OSError, ValueError):
-        return None
-    for k in data.get("kernels", []):
-        if k["kernel"].startswith(kernel_prefix) and k["grid_threads"] == grid_threads:
-            return float(k["traffic_bytes"])
-    return None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                n += 1
+                name = name or line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return name, n
 
 
 def cpu_baseline(seconds):
-    """The oracle's restatement of the reference CPU path, timed on this host."""
+    """The oracle's restatement of the reference CPU path (UPOLS with an FDL,
+    planar FMA multiply-accumulate, FP32), timed on this host on a bounded
+    sample of the 64-channel workload: with the reference's thread policy
+    (OpenMP threads = procs / 4, FFTImpl.cpp:245-250) as the reported value,
+    and with every core as a stronger baseline beside it."""
     path = os.path.join(ROOT, "oracle", "build", "libtan_oracle.so")
     if not os.path.exists(path):
         return None
@@ -642,59 +772,41 @@ def cpu_baseline(seconds):
     lib.orc_bench_upols.argtypes = [ctypes.c_int] * 5 + [ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     procs = lib.orc_num_procs()
-    # reference thread policy: omp_get_num_procs()/4 (FFTImpl.cpp:245-250)
-    threads = lib.orc_set_threads(max(1, procs // 4))
-    probe = 8
-    t = lib.orc_bench_upols(C, L, B, probe, 4, 11, None)
-    per_block = t / probe
-    nblocks = int(max(16, min(20000, seconds / max(per_block, 1e-6))))
-    times = (ctypes.c_double * nblocks)()
-    total = lib.orc_bench_upols(C, L, B, nblocks, 8, 12, times)
-    arr = np.array(times[:])
-    rt = C * nblocks * B / FS / total
-    return {"value": rt, "unit": "real-time channels", "cores": int(threads), "kind": "port",
-            "sample": (f"{nblocks} blocks of the same 64ch/B=256/65536-tap workload (one IR, no "
-                       f"updates), {threads} OpenMP threads of {procs} procs (reference policy "
-                       f"procs/4), own radix-2 FFT (FFTW absent)"),
-            "p99_block_ms": float(np.percentile(arr, 99) * 1e3),
-            "mean_block_ms": float(arr.mean() * 1e3)}
+    model, logical = cpu_model()
+
+    def run(nthreads, budget):
+        threads = lib.orc_set_threads(nthreads)
+        probe = 8
+        t = lib.orc_bench_upols(C, L, B, probe, 4, 11, None)
+        nblocks = int(max(16, min(20000, budget / max(t / probe, 1e-6))))
+        times = (ctypes.c_double * nblocks)()
+        total = lib.orc_bench_upols(C, L, B, nblocks, 8, 12, times)
+        arr = np.array(times[:])
+        return threads, nblocks, C * nblocks * B / FS / total, arr
+
+    th, nb, rt, arr = run(max(1, procs // 4), seconds)
+    tha, nba, rta, arra = run(procs, seconds / 2)
+    fft = "the oracle's radix-2 FFT (FFTW 3.3.5 and MKL absent)"
+    return {"value": rt, "unit": "real-time channels", "cores": int(th), "kind": "port",
+            "sample": (f"{nb} blocks of the 64-channel, B=256, 65,536-tap workload (one IR, no updates), "
+                       f"{th} OpenMP threads of {procs} processors (reference policy procs/4), {fft}"),
+            "p99_block_ms": float(np.percentile(arr, 99) * 1e3), "mean_block_ms": float(arr.mean() * 1e3),
+            "cpu_model": model, "logical_cpus": logical, "omp_procs": procs, "fft": fft,
+            "all_cores": {"value": rta, "threads": int(tha), "blocks": nba,
+                          "p99_block_ms": float(np.percentile(arra, 99) * 1e3),
+                          "mean_block_ms": float(arra.mean() * 1e3)}}
 
 
-def profile_run(T, torch, ctx):
-    """Short fixed workload for counter collection (rocprofv3 --pmc): 100
-    blocks of the headline configuration, 10 blocks at 8,192 channels
-    (HBM-bound), and a plain device copy of 512 MiB (3 launches) whose
-    FETCH_SIZE/WRITE_SIZE calibrate the counters against known bytes."""
-    bench_throughput(T, torch, ctx, 100, 10, timing=False)
-    # the headline shape at every tail chunk count the autotuner may pick
-    conv = T.TANConvolution(ctx)
-    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, C) == T.AMF_OK
-    h = ir_bank_device(torch, C, L, 78)
-    torch.cuda.synchronize()
-    conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY)
-    x = noise_device(torch, C, 20 * B, 6)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    for chunks in (1, 2, 4, 8, 16, 32, 64):
-        conv.SetTailChunks(chunks)
-        conv.Process(x, y, 20 * B)
-    conv.Synchronize()
-    conv.Terminate()
-    del h, x, y
-    ch = 8192
-    conv = T.TANConvolution(ctx)
-    assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, L, B, ch) == T.AMF_OK
-    h = ir_bank_device(torch, ch, L, 77)
-    torch.cuda.synchronize()
-    conv.UpdateResponseTD(h, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY)
-    del h
-    x = noise_device(torch, ch, 16 * B, 5)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    conv.Process(x, y, 10 * B)
-    conv.Synchronize()
-    conv.Terminate()
-    del x, y
+def profile_run(T, torch, ngroups):
+    """Short fixed workload for counter collection (rocprofv3 --pmc): the
+    real-time rig with `ngroups` groups of 2,048 channels for 200 ticks (its
+    updates and crossfades included), and a plain device copy of 512 MiB (3
+    launches) whose FETCH_SIZE / WRITE_SIZE calibrate the counters against
+    known bytes."""
+    rig = RealTimeRig(T, torch, ngroups)
+    h, d = rig.run(rig.kmax, 200)
+    log(f"profile run: {rig.kmax} groups ({rig.kmax * RT_GROUP} channels), 200 ticks, max {h.max():.3f} ms")
+    rig.close()
     a = torch.ones(512 << 18, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
@@ -703,17 +815,31 @@ def profile_run(T, torch, ctx):
     log("profile run done")
 
 
+def pmc_traffic_per_channel(kernel_prefix="void tanx::k_group_tick<0, true>"):
+    """HBM bytes per channel and launch of the real-time group kernel from the
+    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a
+    512 MiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_k_group_tick_rt.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    for k in data.get("kernels", []):
+        if k["kernel"].startswith(kernel_prefix):
+            return float(k["traffic_bytes_per_channel"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of 100 ticks (2,000 ticks at 20)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps of 100 ticks before them")
     ap.add_argument("--latency-blocks", type=int, default=300)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tail-chunks", type=int, default=0,
-                    help="pin the throughput passes' tail chunks instead of autotuning (for profiled runs)")
-    ap.add_argument("--no-sweep", action="store_true")
+                    help="pin the 64-channel passes' tail chunks instead of autotuning (for profiled runs)")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -721,9 +847,11 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--no-nonuniform", action="store_true")
     ap.add_argument("--no-room", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--serving-methods", type=str, default="uniform,nonuniform")
-    ap.add_argument("--nu-sweep", type=str, default="32768,65536,98304,131072")
-    ap.add_argument("--sweep", type=str, default="1024,4096,8192,16384,24576,28672,32768,36864,40960")
+    ap.add_argument("--nu-sweep", type=str, default="65536,98304")
+    ap.add_argument("--rt-groups", type=int, default=0,
+                    help="profile run: real-time rig with this many groups of 2,048 channels")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
@@ -737,73 +865,55 @@ def main():
     import torch
     import tan_amd as T
     T.load_library()
+    if args.profile_run:
+        profile_run(T, torch, args.rt_groups or 24)
+        return
+
+    log("real-time channel count (value)")
+    rt = bench_realtime(T, torch, args.steps, args.warmup)
+    value = rt["channels"]
+    gt = rt["group_timing"]
+    launches = max(1, gt["tick_launches"])
+    tick_us = gt["tick_ms"] / launches * 1e3
+    achieved = gt["tick_alg_bytes"] / (gt["tick_ms"] * 1e-3) / 1e9 if gt["tick_ms"] > 0 else 0.0
+    per_ch = pmc_traffic_per_channel()
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBPS,
+                "traffic": per_ch * value if per_ch else None,
+                "kernel": "k_group_tick<0, true> (heads of every channel, the passes of one group in four, "
+                          "crossfade catch-up passes)",
+                "traffic_source": "profiles/pmc_traffic_k_group_tick_rt.json" if per_ch else None,
+                "alg_bytes_per_launch": gt["tick_alg_bytes"] / launches, "avg_launch_us": tick_us,
+                "launches": gt["tick_launches"], "channels": value}
+    log(f"  {value} real-time channels; group kernel {tick_us:.1f} us per launch, {achieved:.0f} GB/s algorithmic")
+
     ctx = T.TANContext(device=0)
-    # The throughput passes use a context created on a torch stream (the
+    # The 64-channel pass uses a context created on a torch stream (the
     # caller-stream form of the reference's InitOpenCL(queue, queue)): the
     # inputs are written on that stream, so Process needs no cross-stream
     # wait per call, as a C++ caller on its own stream would not.
     hs = torch.cuda.Stream()
     hctx = T.TANContext(device=0, conv_stream=hs, general_stream=hs)
-    if args.profile_run:
-        profile_run(T, torch, ctx)
-        hctx.Terminate()
-        ctx.Terminate()
-        return
 
-    log("throughput pass")
-    dt, _, info, finite = bench_throughput(T, torch, hctx, args.steps, args.warmup, timing=False, stream=hs)
-    assert finite, "non-finite output"
-    value = C * args.steps * B / FS / dt
-    log(f"  {args.steps} blocks in {dt * 1e3:.1f} ms -> {value:.0f} real-time channels")
-
-    log("kernel timing pass")
-    dt2, kt, info2, _ = bench_throughput(T, torch, hctx, args.steps, args.warmup, timing=True, stream=hs)
-    kp = info2.get("pass_blocks", 1)
-    head_blocks = None
-    if kp > 1:
-        # a launch runs up to kp heads and the pass computing the next kp
-        # blocks' partial sums (the dominant kernel, k_window<256, 1, 0>;
-        # k_tick<256, 1, 0, 2> where a launch has a single head); launches
-        # with heads alone are timed as head launches
-        tick_us = kt["tail_ms"] / kt["tail_launches"] * 1e3
-        achieved = kt["tail_alg_bytes"] / (kt["tail_ms"] * 1e-3) / 1e9
-        alg_per_launch = kt["tail_alg_bytes"] / kt["tail_launches"]
-        launches = kt["tail_launches"]
-        kernel = f"k_window (heads + {kp}-block pass)"
-        head_only_us = kt["head_ms"] / kt["head_launches"] * 1e3 if kt["head_launches"] else None
-        head_blocks = kt["blocks"] / (kt["head_launches"] + kt["tail_launches"])
-        prefix = "void tanx::k_window<256, 1, 0>"
-    else:
-        # one k_tick launch per block: the block's head and the next block's tail
-        tick_us = kt["head_ms"] / kt["head_launches"] * 1e3
-        achieved = kt["head_alg_bytes"] / (kt["head_ms"] * 1e-3) / 1e9
-        alg_per_launch = kt["head_alg_bytes"] / kt["head_launches"]
-        launches = kt["head_launches"]
-        kernel = "k_tick (head + next tail)"
-        head_only_us = None
-        prefix = "void tanx::k_tick<256, 1, 0, 1>"
-    per_block_us = (kt["head_ms"] + kt["tail_ms"]) * 1e3 / kt["blocks"]
-    # the launch's grid: 64 head blocks + 64 x chunks tail blocks of 256 threads
-    traffic = pmc_traffic((C + C * info2["tail_chunks"]) * 256, prefix)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": kernel,
-                "traffic_source": "profiles/pmc_traffic_k_tick.json" if traffic else None,
-                "alg_bytes_per_launch": alg_per_launch, "avg_launch_us": tick_us, "launches": launches,
-                "pass_blocks": kp, "head_launch_us": head_only_us, "blocks_per_launch": head_blocks,
-                "avg_block_us": per_block_us,
-                "note": ("64-channel working set (~67 MB) is resident in the 256 MB Infinity "
-                         "Cache; see sweep for HBM-bound channel counts")}
-    log(f"  tick kernel {tick_us:.2f} us/launch, {achieved:.0f} GB/s algorithmic; "
-        f"{per_block_us:.2f} us per block on average")
+    c2 = None
+    if not args.no_c2:
+        log("C2 workload (64 channels, 2,000 blocks, updates every 50)")
+        dt, _, info, finite = bench_throughput(T, torch, hctx, 2000, 100, timing=False, stream=hs)
+        assert finite, "non-finite output"
+        dt2, kt, info2, _ = bench_throughput(T, torch, hctx, 2000, 100, timing=True, stream=hs)
+        pass_us = kt["tail_ms"] / max(1, kt["tail_launches"]) * 1e3
+        c2 = {"channels": C, "blocks": 2000, "ir_updates": 40, "wall_ms": dt * 1e3,
+              "rt_channels_throughput": C * 2000 * B / FS / dt, "ms_per_update_period": dt / 40 * 1e3,
+              "pass_blocks": info.get("pass_blocks"), "tail_chunks": info.get("tail_chunks"),
+              "k_window_avg_us": pass_us,
+              "k_window_alg_GBps": kt["tail_alg_bytes"] / (kt["tail_ms"] * 1e-3) / 1e9 if kt["tail_ms"] else None,
+              "kernel_us_per_block": (kt["head_ms"] + kt["tail_ms"]) * 1e3 / kt["blocks"],
+              "note": "64-channel rows (~100 MB) stay in the Infinity Cache; not an HBM figure"}
+        log(f"  {c2['rt_channels_throughput']:.0f} real-time channels of throughput at 64 channels")
 
     log("latency pass")
     lat = bench_latency(T, ctx, args.latency_blocks)
     log(f"  {lat}")
-
-    best, rows = (0, [])
-    if not args.no_sweep:
-        log("deadline sweep")
-        best, rows = bench_sweep(T, torch, ctx, [int(v) for v in args.sweep.split(",") if v])
 
     prec_rows, dft_rows = [], []
     if not args.no_precision:
@@ -815,22 +925,16 @@ def main():
     nonuniform = None
     if not args.no_nonuniform:
         log("non-uniform pass")
-        dtn, _, infon, finite_n = bench_throughput(T, torch, hctx, args.steps, args.warmup, timing=False,
+        dtn, _, infon, finite_n = bench_throughput(T, torch, hctx, 2000, 100, timing=False,
                                                    method=NONUNIFORM_METHOD, stream=hs)
         assert finite_n, "non-finite output (non-uniform)"
-        nu_value = C * args.steps * B / FS / dtn
-        log(f"  non-uniform (M = {infon['nu_multiple']}): {nu_value:.0f} real-time channels")
-        nu_best, nu_rows = (0, [])
-        ph_best, ph_rows = (0, [])
-        if not args.no_sweep:
-            counts = [int(v) for v in args.nu_sweep.split(",") if v]
-            nu_best, nu_rows = bench_sweep(T, torch, ctx, counts, method=NONUNIFORM_METHOD)
-            ph_best, ph_rows = bench_sweep_phased(T, torch, ctx, counts)
+        nu_value = C * 2000 * B / FS / dtn
+        log(f"  non-uniform (M = {infon['nu_multiple']}): {nu_value:.0f} real-time channels of throughput")
+        counts = [int(v) for v in args.nu_sweep.split(",") if v]
+        ph_best, ph_rows = bench_sweep_phased(T, torch, ctx, counts)
         nonuniform = {"method": "TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM",
-                      "nu_multiple": infon["nu_multiple"], "value": nu_value,
-                      "ms_per_step": dtn / args.steps * 1e3, "rt_channels_at_deadline": nu_best,
-                      "sweep": nu_rows, "rt_channels_at_deadline_phased": ph_best,
-                      "sweep_phased": ph_rows}
+                      "nu_multiple": infon["nu_multiple"], "rt_channels_throughput_64ch": nu_value,
+                      "rt_channels_at_deadline_phased_no_updates": ph_best, "sweep_phased": ph_rows}
 
     room_rows = []
     if not args.no_room:
@@ -864,22 +968,24 @@ def main():
     hctx.Terminate()
     ctx.Terminate()
 
-    big = max(rows, key=lambda r: r["channels"]) if rows else None
     out = {
         "metric": METRIC, "value": value, "unit": "real-time channels", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": rt["wall_s"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (decaying-noise IRs, U[-1,1] input, generated on device)",
-        "config": {"workload": WORKLOAD, "channels": C, "block": B, "ir_taps": L,
-                   "blocks": args.steps, "update_every": UPDATE_EVERY, "sample_rate": FS},
-        "p50_block_ms": lat["p50_ms"], "p99_block_ms": lat["p99_ms"], "max_block_ms": lat["max_ms"],
-        "deadline_ms": DEADLINE_MS, "deadline_misses": lat["deadline_misses"],
-        "latency_worst_blocks": lat["worst"],
+        "config": {"workload": RT_WORKLOAD, "channels": value, "channel_groups": rt["groups"],
+                   "group_channels": RT_GROUP, "block": B, "ir_taps": L, "ticks": rt["ticks"],
+                   "ticks_per_step": RT_STEP_TICKS, "update_every": UPDATE_EVERY, "pass_blocks": 4,
+                   "sample_rate": FS, "deadline_ms": DEADLINE_MS},
+        "deadline_misses": rt["misses"],
+        "p50_block_ms": rt["p50_host_ms"], "p99_block_ms": rt["p99_host_ms"], "max_block_ms": rt["max_host_ms"],
+        "p50_device_ms": rt["p50_device_ms"], "p99_device_ms": rt["p99_device_ms"],
+        "max_device_ms": rt["max_device_ms"],
+        "rt_throughput_channels": value * rt["ticks"] * B / FS / rt["wall_s"],
         "roofline": roofline,
-        "rt_channels_at_deadline": best,
-        "sweep": rows,
-        "roofline_hbm": ({"channels": big["channels"], "achieved": big["tick_GBps"],
-                          "frac": big["tick_GBps"] / HBM_PEAK_GBPS} if big else None),
+        "realtime_search": rt["probes"], "realtime_outputs_finite": rt["outputs_finite"],
+        "c2_64ch": c2,
+        "latency_64ch_host_buffers": lat,
         "precision": prec_rows,
         "mfma_dft": dft_rows,
         "nonuniform": nonuniform,

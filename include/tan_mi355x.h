@@ -154,9 +154,16 @@ typedef struct tanx_conv_options {
     uint32_t pass_blocks;   /* uniform objects: blocks whose partial sums one
                                tail pass computes (1..4; 1 = a tail every
                                block; 0 = chosen by the library: 4 when the
-                               IR has 16 or more partitions and one pass
-                               streams at most 4.5 GB, else 1) */
-    uint32_t reserved[4];   /* zero */
+                               IR has 16 or more partitions, else 1) */
+    uint32_t pass_phase;    /* uniform objects with multi-block passes: pass
+                               phase in blocks, taken modulo pass_blocks K.
+                               After block 0, passes run in the launches of
+                               blocks b with (b + phase) % K == 0 (a catch-up
+                               pass for an IR switch ends on the same grid).
+                               Objects of one group with phases 0..K-1 each
+                               stream their rows on a different block, so no
+                               block carries every object's pass. */
+    uint32_t reserved[3];   /* zero */
 } tanx_conv_options;
 /* tanx_conv_create with options (no counterpart in the reference, which
  * stores FP32 spectra only); opt may be NULL. */
@@ -299,6 +306,16 @@ typedef struct tanx_kernel_timing {
 } tanx_kernel_timing;
 int32_t tanx_conv_enable_timing(tanx_conv *conv, uint32_t enable);
 int32_t tanx_conv_read_timing(tanx_conv *conv, tanx_kernel_timing *out, uint32_t reset);
+/* Timing of the group launches of a context (no counterpart in the
+ * reference): when enabled, every group call's kernels stamp HIP events with
+ * their own start and end.  read_group_timing synchronizes and returns, in
+ * the same struct: head_* = the group tick kernels (duration, launches, and
+ * the algorithmic bytes of the roles they ran: each object's head, the
+ * further partitions its head added, its tail or multi-block pass and its
+ * frame slice), tail_* = the frame kernels of non-uniform objects, blocks =
+ * objects advanced. */
+int32_t tanx_ctx_enable_group_timing(tanx_ctx *ctx, uint32_t enable);
+int32_t tanx_ctx_read_group_timing(tanx_ctx *ctx, tanx_kernel_timing *out, uint32_t reset);
 /* 1 if the last processed block was a crossfade block. */
 int32_t tanx_conv_last_was_crossfade(const tanx_conv *conv);
 /* Stream on which process work is ordered (a hipStream_t). */

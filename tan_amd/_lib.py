@@ -30,7 +30,7 @@ class ConvInfo(ctypes.Structure):
 class ConvOptions(ctypes.Structure):
     _fields_ = [("ir_precision", ctypes.c_uint32), ("nu_multiple", ctypes.c_uint32),
                 ("nu_phase", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("pass_phase", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class WavInfo(ctypes.Structure):
@@ -87,6 +87,8 @@ _SIGS = {
     "tanx_conv_autotune": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(ctypes.c_double)]),
     "tanx_conv_read_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_conv_process_stream": (_vp, [_vp]),
+    "tanx_ctx_enable_group_timing": (_i32, [_vp, _u32]),
+    "tanx_ctx_read_group_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_group_process_device": (_i32, [_u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(_sz), ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                                          ctypes.c_void_p]),

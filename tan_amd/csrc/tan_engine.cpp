@@ -56,6 +56,47 @@ inline bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
 
 }  // namespace
 
+static constexpr int kTimerRing = 1024;
+
+// Per-kernel durations measured with HIP events recorded on the process
+// stream around each launch; drained into running sums when the ring fills
+// or when read.
+struct KernelTimer {
+    std::vector<hipEvent_t> start, stop;
+    int n = 0;
+    double ms = 0.0;
+    uint64_t count = 0;
+    bool init()
+    {
+        start.resize(kTimerRing);
+        stop.resize(kTimerRing);
+        for (int i = 0; i < kTimerRing; i++)
+            if (hipEventCreate(&start[i]) != hipSuccess || hipEventCreate(&stop[i]) != hipSuccess) return false;
+        return true;
+    }
+    void destroy()
+    {
+        for (auto e : start) if (e) (void)hipEventDestroy(e);
+        for (auto e : stop) if (e) (void)hipEventDestroy(e);
+        start.clear();
+        stop.clear();
+    }
+    hipError_t drain()
+    {
+        for (int i = 0; i < n; i++) {
+            hipError_t e = hipEventSynchronize(stop[i]);
+            if (e != hipSuccess) return e;
+            float t = 0.f;
+            e = hipEventElapsedTime(&t, start[i], stop[i]);
+            if (e != hipSuccess) return e;
+            ms += t;
+            count++;
+        }
+        n = 0;
+        return hipSuccess;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -77,6 +118,12 @@ struct tanx_ctx {
     size_t g_cap = 0;
     hipEvent_t g_ev[4] = {};
     int g_ring = 0;
+    // group-launch timing (tanx_ctx_enable_group_timing): launch-stamped
+    // events around the tick and frame kernels, and the algorithmic bytes of
+    // the roles they ran
+    bool gtiming = false;
+    KernelTimer gt_tick, gt_frame;
+    uint64_t g_tick_bytes = 0, g_frame_bytes = 0, g_objects = 0;
     hipStream_t proc = nullptr;  // convolution queue equivalent
     hipStream_t upd = nullptr;   // general queue equivalent (IR updates, FFT)
     bool own_proc = true, own_upd = true;  // created here (else the caller's)
@@ -204,6 +251,8 @@ static void ctx_free(tanx_ctx *ctx)
         if (ctx->g_host) (void)hipHostFree(ctx->g_host);
         if (ctx->g_dev) (void)hipFree(ctx->g_dev);
         for (auto &e : ctx->g_ev) if (e) (void)hipEventDestroy(e);
+        ctx->gt_tick.destroy();
+        ctx->gt_frame.destroy();
         if (ctx->dft_hi) (void)hipFree(ctx->dft_hi);
         if (ctx->dft_lo) (void)hipFree(ctx->dft_lo);
         if (ctx->io_flag) (void)hipFree(ctx->io_flag);
@@ -248,48 +297,9 @@ extern "C" int32_t tanx_ctx_device(const tanx_ctx *ctx, int32_t *device)
 // ---------------------------------------------------------------------------
 static constexpr int kMaxChunks = 64;
 static constexpr int kRunRing = 8;
-static constexpr int kTimerRing = 1024;
 static constexpr int kGraphBlocks = 64;
 static constexpr int kDescRing = 4;
 
-// Per-kernel durations measured with HIP events recorded on the process
-// stream around each launch; drained into running sums when the ring fills
-// or when read.
-struct KernelTimer {
-    std::vector<hipEvent_t> start, stop;
-    int n = 0;
-    double ms = 0.0;
-    uint64_t count = 0;
-    bool init()
-    {
-        start.resize(kTimerRing);
-        stop.resize(kTimerRing);
-        for (int i = 0; i < kTimerRing; i++)
-            if (hipEventCreate(&start[i]) != hipSuccess || hipEventCreate(&stop[i]) != hipSuccess) return false;
-        return true;
-    }
-    void destroy()
-    {
-        for (auto e : start) if (e) (void)hipEventDestroy(e);
-        for (auto e : stop) if (e) (void)hipEventDestroy(e);
-        start.clear();
-        stop.clear();
-    }
-    hipError_t drain()
-    {
-        for (int i = 0; i < n; i++) {
-            hipError_t e = hipEventSynchronize(stop[i]);
-            if (e != hipSuccess) return e;
-            float t = 0.f;
-            e = hipEventElapsedTime(&t, start[i], stop[i]);
-            if (e != hipSuccess) return e;
-            ms += t;
-            count++;
-        }
-        n = 0;
-        return hipSuccess;
-    }
-};
 
 // While the pass rows of all live objects on a context total at most this
 // (half the 256 MB Infinity Cache of an MI355X), k_window's multi-block
@@ -318,6 +328,7 @@ struct tanx_conv {
     // b's head adds partitions 0..b-porg itself), pchunk their chunk count,
     // covered the last block with sums.
     int kp = 1;
+    int kphase = 0;  // passes end on blocks b with (b + kphase) % kp == 0
     // bytes of the rows a pass streams (FDL, both filters of a crossfade,
     // frame rows), counted in the context's pass_rows while the object lives
     uint64_t rows_bytes = 0;
@@ -480,15 +491,17 @@ static void set_chunks(tanx_conv *cv, int n)
     cv->nchunk = (parts + cv->pc - 1) / cv->pc;
 }
 
-constexpr double kPassDefaultBytes = 4.5e9;
-
-static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(kMaxChunks, P - 2), 16384 / C)); }
+// Most tail chunks an object may use (the partial-sum ring is sized for it):
+// chunks exist to give small channel counts enough workgroups, so beyond
+// 4,096 chunk rows per object more of them only cost memory.
+static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(kMaxChunks, P - 2), 4096 / C)); }
 
 // Shape of an object: uniform partitions of B over the first `head` taps
 // (P = head / B), then (non-uniform objects) PB frame partitions of N = M * B.
 struct ConvShape {
     int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1, shift = 0;
     int kp = 1;  // blocks per tail pass
+    int kphase = 0;  // pass phase: passes in the launches of blocks b with (b + kphase) % kp == 0
     int irtype = IR_F32;
     bool td = false;  // time-domain object: IR slots and history in samples
 };
@@ -591,18 +604,19 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
         sh.P = sh.len / sh.B;
     }
     sh.maxchunk = max_chunks_for(sh.C, sh.P);
-    // multi-block tail passes: uniform objects outside finalize mode.  A
-    // pass reads every partition once for kp blocks, but its launch takes
-    // longer than one block's tail; by default it is used while one pass
-    // streams at most kPassDefaultBytes (about 0.75 ms of HBM time), so the
-    // block carrying it stays far inside any audio deadline, and the IR has
-    // partitions enough to stream.
+    // multi-block tail passes: uniform objects outside finalize mode whose
+    // IR has partitions enough to stream.  A pass reads every partition once
+    // for kp blocks; the block that carries it streams the whole object (as
+    // a tail every block would), the blocks between stream none, so the
+    // worst block is no slower and the rows cost a quarter as much.  Objects
+    // advanced together with pass phases 0..kp-1 take turns carrying their
+    // pass, so no block streams every object's rows.
     const uint32_t want_kp = opt ? opt->pass_blocks : 0;
     if (want_kp > (uint32_t)kPassMax) return TANX_INVALID_ARG;
     const bool finalize = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
     if (M == 1 && !finalize) {
-        const double pass_bytes = (double)sh.C * (sh.P - 2) * sh.B * (sizeof(float2) + ir_elem_bytes(sh.irtype));
-        sh.kp = want_kp ? (int)want_kp : ((sh.P >= 16 && pass_bytes <= kPassDefaultBytes) ? kPassMax : 1);
+        sh.kp = want_kp ? (int)want_kp : (sh.P >= 16 ? kPassMax : 1);
+        sh.kphase = (int)((opt ? opt->pass_phase : 0) % (uint32_t)sh.kp);
     }
     return TANX_OK;
 }
@@ -715,6 +729,7 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     set_chunks(cv, auto_chunks(cv->C, cv->P));
     cv->tail_nchunk = cv->nchunk;
     cv->kp = sh.kp;
+    cv->kphase = sh.kphase;
     if (!cv->td) {
         cv->rows_bytes = (uint64_t)cv->C * cv->P * cv->B * sizeof(float2) + 2 * cv->slot_bytes() +
                          (cv->nu ? (uint64_t)cv->C * cv->RB * cv->N * sizeof(float2) + 2 * cv->bslot_bytes() : 0);
@@ -963,16 +978,19 @@ static int fill_pass(tanx_conv *cv, long long w, long long e, TickArgs &a)
         a.kn = 1;
         return 0;
     }
-    const long long s_rel = (sw >= e && sw < e + cv->kp) ? sw - e : -1;
+    // the pass ends on the phase grid: its last block c satisfies
+    // (c + kphase) % kp == 0, so the next pass runs in that block's launch
+    const int kn = (int)(((-(long long)cv->kphase - e) % cv->kp + cv->kp) % cv->kp) + 1;
+    const long long s_rel = (sw >= e && sw < e + kn) ? sw - e : -1;
     a.nchunk = cv->nchunk;
     a.pc = cv->pc;
-    a.kn = cv->kp;
+    a.kn = kn;
     a.kgap = (int)(e - w);
     a.tbase = cv->pos;  // ring slot of block e
     a.tHa = cv->slot_ptr(cv->cur);
     a.tSa = cv->scale_ptr(cv->cur);
     bool second = false;
-    for (int k = 0; k < cv->kp; k++) {
+    for (int k = 0; k < kn; k++) {
         const long long blk = e + k;
         const int r = cv->pring(blk);
         float2 *own = cv->tail_slot(blk, 0);
@@ -995,7 +1013,7 @@ static int fill_pass(tanx_conv *cv, long long w, long long e, TickArgs &a)
         a.tHb = cv->slot_ptr(cv->upd);
         a.tSb = cv->scale_ptr(cv->upd);
     }
-    cv->covered = e + cv->kp - 1;
+    cv->covered = e + kn - 1;
     return second ? 2 : 1;
 }
 
@@ -1906,12 +1924,14 @@ static GroupStage group_layout(size_t cap)
     return g;
 }
 
-// Advances every object (locked by the caller) one block and writes its
-// launch descriptors into g.host; then copies them to g.dev on the process
-// stream and launches the group kernels.  Objects already advanced are
-// launched even if a later one fails (its error is returned).  staged: the
-// event to record after the copy (the host staging is free again once it
-// completes), or nullptr.
+// Advances every object (locked by the caller) one block, in call order, and
+// writes its launch descriptors into g.host, grouped by block size (the
+// objects of one block size contiguous, in call order); then copies them to
+// g.dev on the process stream and launches one group kernel per block size
+// present and the frame kernels.  Objects already advanced are launched even
+// if a later one fails (its error is returned).  staged: the event to record
+// after the copy (the host staging is free again once it completes), or
+// nullptr.
 static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const *convs, const float *const *d_in,
                          const size_t *in_stride, float *const *d_out, const size_t *out_stride,
                          const uint32_t *const *flags, const GroupStage &g, hipEvent_t staged)
@@ -1920,41 +1940,115 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
     int *starts = reinterpret_cast<int *>(g.host + g.desc_bytes);
     FrameArgs *fdescs = reinterpret_cast<FrameArgs *>(g.host + g.desc_bytes + g.start_bytes);
     int *fstarts = reinterpret_cast<int *>(g.host + g.desc_bytes + g.start_bytes + g.fdesc_bytes);
+    // block-size classes: descriptor range [first, first + placed) of each
+    constexpr int kClasses = 4;
+    struct Class {
+        uint32_t first = 0, size = 0, placed = 0;
+        int total = 0;
+    } cls[kClasses];
+    auto class_of = [](int B) { return B == 64 ? 0 : B == 128 ? 1 : B == 256 ? 2 : 3; };
+    for (uint32_t i = 0; i < n; i++) cls[class_of(convs[i]->B)].size++;
+    for (int c = 1; c < kClasses; c++) cls[c].first = cls[c - 1].first + cls[c - 1].size;
     uint32_t done = 0;
-    int total = 0, nframe = 0, ftotal = 0;
-    bool passes = false;
+    int nframe = 0, ftotal = 0;
     int32_t rc = TANX_OK;
+    uint64_t tick_bytes = 0, frame_bytes = 0;
     for (; done < n; done++) {
         tanx_conv *cv = convs[done];
+        Class &k = cls[class_of(cv->B)];
+        const uint32_t at = k.first + k.placed;
         int nf = 1;
         bool xf = false, fdue = false;
         rc = prepare_block(cv, d_in[done], (long long)in_stride[done], d_out[done], (long long)out_stride[done],
-                           cv->B, flags ? flags[done] : nullptr, true, descs[done], &nf, &xf, &fdescs[nframe],
-                           &fdue);
+                           cv->B, flags ? flags[done] : nullptr, true, descs[at], &nf, &xf, &fdescs[nframe], &fdue);
         if (rc != TANX_OK) break;
+        const TickArgs &a = descs[at];
+        if (ctx->gtiming) {
+            tick_bytes += head_bytes(cv, xf ? 2 : 1) + near_bytes(cv, a) + frame_mac_bytes(cv, a.bHb ? 2 : 1) +
+                          (a.nchunk > 0 ? tail_bytes(cv, nf) : 0);
+            if (fdue) frame_bytes += frame_kernel_bytes(cv, fdescs[nframe].Hb ? 2 : 1);
+        }
         // a group streams every member's rows (group kernels ignore the flag)
-        descs[done].cached = 0;
+        descs[at].cached = 0;
         tail_done(cv);
-        passes = passes || descs[done].kn > 1;
-        starts[done] = total;
-        total += cv->C + cv->C * descs[done].nchunk + cv->C * descs[done].nchB;
+        k.placed++;
         if (fdue) {
             fstarts[nframe++] = ftotal;
             ftotal += cv->C;
         }
     }
     if (done > 0) {
+        // each class's range splits into the lean descriptors first and the
+        // general ones after them (swapped in place), each sub-range one
+        // launch with block offsets of its own
+        struct Launch {
+            uint32_t first, count;
+            int B, total;
+            bool lean, passes;
+        } launches[2 * kClasses];
+        int nl = 0;
+        for (int c = 0; c < kClasses; c++) {
+            const Class &k = cls[c];
+            if (k.placed == 0) continue;
+            uint32_t lo = k.first, hi = k.first + k.placed;
+            while (lo < hi) {
+                if (is_lean(descs[lo])) {
+                    lo++;
+                } else {
+                    hi--;
+                    std::swap(descs[lo], descs[hi]);
+                }
+            }
+            for (int part = 0; part < 2; part++) {
+                const uint32_t a0 = part == 0 ? k.first : lo, a1 = part == 0 ? lo : k.first + k.placed;
+                if (a0 == a1) continue;
+                Launch &l = launches[nl++];
+                l = Launch{a0, a1 - a0, 64 << c, 0, part == 0, false};
+                for (uint32_t i = a0; i < a1; i++) {
+                    starts[i] = l.total;
+                    l.total += descs[i].C * (1 + descs[i].nchunk + descs[i].nchB);
+                    l.passes = l.passes || is_pass(descs[i]);
+                }
+            }
+        }
         // the staging pieces are copied in one transfer at their offsets
         TANX_CHECK(hipMemcpyAsync(g.dev, g.host, g.total(), hipMemcpyHostToDevice, ctx->proc));
         if (staged) TANX_CHECK(hipEventRecord(staged, ctx->proc));
-        TANX_CHECK(launch_group_tick(irtype, passes, reinterpret_cast<const TickArgs *>(g.dev),
-                                     reinterpret_cast<const int *>(g.dev + g.desc_bytes), (int)done, (unsigned)total,
-                                     ctx->proc));
-        if (nframe > 0)
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (ctx->gtiming) {
+            if (ctx->gt_tick.n == kTimerRing) TANX_CHECK(ctx->gt_tick.drain());
+            t0 = ctx->gt_tick.start[ctx->gt_tick.n];
+            t1 = ctx->gt_tick.stop[ctx->gt_tick.n];
+        }
+        for (int i = 0; i < nl; i++) {
+            const Launch &l = launches[i];
+            const TickArgs *dd = reinterpret_cast<const TickArgs *>(g.dev) + l.first;
+            const int *ds = reinterpret_cast<const int *>(g.dev + g.desc_bytes) + l.first;
+            // the first launch stamps the start event, the last one the end
+            TANX_CHECK(launch_group_tick(irtype, l.B, l.lean, l.passes, dd, ds, (int)l.count, (unsigned)l.total,
+                                         ctx->proc, i == 0 ? t0 : nullptr, i == nl - 1 ? t1 : nullptr));
+        }
+        if (ctx->gtiming) {
+            ctx->gt_tick.n++;
+            ctx->g_tick_bytes += tick_bytes;
+            ctx->g_objects += done;
+        }
+        if (nframe > 0) {
+            hipEvent_t f0 = nullptr, f1 = nullptr;
+            if (ctx->gtiming) {
+                if (ctx->gt_frame.n == kTimerRing) TANX_CHECK(ctx->gt_frame.drain());
+                f0 = ctx->gt_frame.start[ctx->gt_frame.n];
+                f1 = ctx->gt_frame.stop[ctx->gt_frame.n];
+            }
             TANX_CHECK(launch_group_frame(
                 irtype, reinterpret_cast<const FrameArgs *>(g.dev + g.desc_bytes + g.start_bytes),
                 reinterpret_cast<const int *>(g.dev + g.desc_bytes + g.start_bytes + g.fdesc_bytes), nframe,
-                (unsigned)ftotal, ctx->proc));
+                (unsigned)ftotal, ctx->proc, f0, f1));
+            if (ctx->gtiming) {
+                ctx->gt_frame.n++;
+                ctx->g_frame_bytes += frame_bytes;
+            }
+        }
     } else if (staged) {
         TANX_CHECK(hipEventRecord(staged, ctx->proc));
     }
@@ -1985,6 +2079,40 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     if (rc != TANX_OK) return rc;
     st.dev = ctx->g_dev;
     return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, ctx->g_ev[slot]);
+}
+
+extern "C" int32_t tanx_ctx_enable_group_timing(tanx_ctx *ctx, uint32_t enable)
+{
+    if (!ctx) return TANX_INVALID_POINTER;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    DeviceGuard g(ctx->device);
+    if (enable && ctx->gt_tick.start.empty()) {
+        if (!ctx->gt_tick.init() || !ctx->gt_frame.init()) return TANX_UNEXPECTED;
+    }
+    ctx->gtiming = enable != 0;
+    return TANX_OK;
+}
+
+extern "C" int32_t tanx_ctx_read_group_timing(tanx_ctx *ctx, tanx_kernel_timing *out, uint32_t reset)
+{
+    if (!ctx || !out) return TANX_INVALID_POINTER;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    DeviceGuard g(ctx->device);
+    TANX_CHECK(ctx->gt_tick.drain());
+    TANX_CHECK(ctx->gt_frame.drain());
+    out->head_ms = ctx->gt_tick.ms;
+    out->tail_ms = ctx->gt_frame.ms;
+    out->head_launches = ctx->gt_tick.count;
+    out->tail_launches = ctx->gt_frame.count;
+    out->head_alg_bytes = ctx->g_tick_bytes;
+    out->tail_alg_bytes = ctx->g_frame_bytes;
+    out->blocks = ctx->g_objects;
+    if (reset) {
+        ctx->gt_tick.ms = ctx->gt_frame.ms = 0.0;
+        ctx->gt_tick.count = ctx->gt_frame.count = 0;
+        ctx->g_tick_bytes = ctx->g_frame_bytes = ctx->g_objects = 0;
+    }
+    return TANX_OK;
 }
 
 // Host staging of the allocation-free variant (pageable: a copy from it has

@@ -127,6 +127,22 @@ struct TickArgs {
     float2 *bTb;
 };
 
+// The tail role of a launch is a multi-block pass (tailk_role: outputs in
+// kTA/kTB, any kn >= 1, partitions from kgap + k + 1) rather than a single
+// tail (tail_role: outputs in tTa/tTb, partitions from 2).  A pass may be one
+// block long: one that ends on its object's pass-phase grid.
+__host__ __device__ inline bool is_pass(const TickArgs &a)
+{
+    return a.kn > 1 || a.kTA[0] != nullptr || a.kTB[0] != nullptr;
+}
+
+// A descriptor the lean group kernel can run: heads, and a tail that is a
+// single-filter pass or none; no frame slice.
+__host__ __device__ inline bool is_lean(const TickArgs &a)
+{
+    return a.nchB == 0 && (a.nchunk == 0 || (is_pass(a) && a.tHb == nullptr));
+}
+
 // Time-domain convolution (TAN_CONVOLUTION_METHOD_TIME_DOMAIN): one
 // workgroup per channel computes n <= B outputs
 // y[j] = sum_{k=first}^{last-1} hist[(pos + j - k) mod len] h[k] over the
@@ -202,12 +218,14 @@ struct FftArgs {
 // ev0/ev1 (optional): events stamped with the kernel's own start and end
 hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hipEvent_t ev0 = nullptr,
                        hipEvent_t ev1 = nullptr);
-// One launch for several objects: block b belongs to the object o with
-// starts[o] <= b < starts[o + 1] and runs its roles from d_descs[o].  Every
-// object must use irtype and a block size in {64, 128, 256, 512}; passes:
-// some descriptor has kn > 1.
-hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, const int *d_starts, int nobj,
-                             unsigned blocks, hipStream_t s);
+// One launch for several objects of block size B: block b belongs to the
+// object o with starts[o] <= b < starts[o + 1] and runs its roles from
+// d_descs[o].  Every object must use irtype and block size B (64, 128, 256
+// or 512); lean: every descriptor is_lean (the kernel without the other
+// roles); else passes: some descriptor carries a pass (is_pass).
+hipError_t launch_group_tick(int irtype, int B, bool lean, bool passes, const TickArgs *d_descs, const int *d_starts,
+                             int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr);
 constexpr int kGroupMaxB = 512;
 // Consecutive head-only blocks of a uniform object with multi-block passes,
 // launched together (one workgroup per channel): block j of the batch
@@ -250,7 +268,7 @@ hipError_t launch_td(const TdArgs &a, hipStream_t s, hipEvent_t ev0 = nullptr, h
 hipError_t launch_td_load(const float *ir, long long stride, long long n, const int *chan, int nchan, float *H,
                           int len, hipStream_t s);
 hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
-                              hipStream_t s);
+                              hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s);
 // batched real DFT of n-sample frames on the matrix cores (n a multiple of
 // 32): y[c][2k], y[c][2k+1] = Re, Im of bin k <= n/2; fh/fl: [n+2][n] bf16

@@ -562,7 +562,7 @@ __device__ void mac_rows(const MacView &w, float4 *red)
     }
 }
 
-// Multi-block pass (kn > 1 outputs): partitions [2 + j*pc, 2 + (j+1)*pc)
+// Multi-block pass (kn >= 1 outputs): partitions [2 + j*pc, 2 + (j+1)*pc)
 // of channel c for kn consecutive blocks.  Output k (0-based, block
 // o + g + k, o the launch's first head block, g = kgap) takes partition
 // i >= g + k + 1 with FDL row (tbase + k - i), the row partition i - k reads
@@ -677,17 +677,25 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
                 if (NF == 2 && wantB[k]) red[((1 * KM + k) * G + g) * H + v * R + r] = accB[k][v];
             }
         __syncthreads();
-        for (int e = tid; e < NF * KM * H; e += kTickThreads) {
-            const int f = e / (KM * H), k = (e / H) % KM, q = e % H;
-            if (f == 0 ? !wantA[k] : !wantB[k]) continue;
-            float4 sum = red[((f * KM + k) * G + 0) * H + q];
+        // outputs by unrolled (filter, output) so every pointer is picked
+        // statically (an output pointer indexed at run time would put the
+        // whole argument struct in private memory)
 #pragma unroll
-            for (int gg = 1; gg < G; gg++) {
-                const float4 t = red[((f * KM + k) * G + gg) * H + q];
-                sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                if (f == 0 ? !wantA[k] : !wantB[k]) continue;
+                float4 *dst = reinterpret_cast<float4 *>(f == 0 ? a.kTA[k] : a.kTB[k]) + orow;
+                for (int q = tid; q < H; q += kTickThreads) {
+                    float4 sum = red[((f * KM + k) * G + 0) * H + q];
+#pragma unroll
+                    for (int gg = 1; gg < G; gg++) {
+                        const float4 t = red[((f * KM + k) * G + gg) * H + q];
+                        sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
+                    }
+                    dst[q] = sum;
+                }
             }
-            reinterpret_cast<float4 *>(f == 0 ? a.kTA[k] : a.kTB[k])[orow + q] = sum;
-        }
     }
 }
 
@@ -782,7 +790,11 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
 // Roles of one tick for block index b of one object's launch; KP compiles
 // in the multi-block pass (its larger reduction buffer and registers are
 // then paid by every block of the launch).
-template <int M, int HT, bool KP>
+// GEN = false compiles only the lean roles (heads, and single-filter
+// passes), for objects whose tail is such a pass or absent and that have
+// no frame slice (is_lean); the registers of the other roles would cut the
+// occupancy of every launch that includes them.
+template <int M, int HT, bool KP, bool GEN = true>
 __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem)
 {
     if (b < a.nhead) {
@@ -791,19 +803,21 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
     } else {
         const int t = b - a.nhead;
         if (t >= a.C * a.nchunk) {
-            frame_mac_block<M, HT>(a, t - a.C * a.nchunk, smem);
+            if constexpr (GEN) frame_mac_block<M, HT>(a, t - a.C * a.nchunk, smem);
             return;
         }
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        if constexpr (KP) {
-            if (a.kn > 1) {
-                if (a.tHb) tailk_role<M, 2, HT>(a, c, j, smem);
-                else tailk_role<M, 1, HT>(a, c, j, smem);
-                return;
+        if constexpr (!GEN) {
+            tailk_role<M, 1, HT>(a, c, j, smem);
+        } else {
+            if constexpr (KP) {
+                if (is_pass(a)) {
+                    if (a.tHb) tailk_role<M, 2, HT>(a, c, j, smem);
+                    else tailk_role<M, 1, HT>(a, c, j, smem);
+                    return;
+                }
             }
-        }
-        {
             if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
             else tail_role<M, 1, HT>(a, c, j, smem);
         }
@@ -822,18 +836,21 @@ __global__ __launch_bounds__(kTickThreads) void k_tick_desc(const TickArgs *__re
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
-    const TickArgs a = descs[k];
-    tick_body<M, HT, false>(a, blockIdx.x, smem);
+    tick_body<M, HT, false>(descs[k], blockIdx.x, smem);
 }
 
-// KP: some object of the group runs a multi-block pass in this launch
-template <int HT, bool KP>
+// One launch for the objects of one block size M (a group call launches
+// one per block size present, lean and general descriptors apart).  KP:
+// some object runs a multi-block pass in this launch; GEN: the general
+// roles are compiled in.  A workgroup finds its object by binary search
+// over the block offsets.
+template <int HT, bool KP, int M, bool GEN = true>
 __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__restrict__ descs,
                                                              const int *__restrict__ starts, int nobj)
 {
-    constexpr int M = kGroupMaxB;
-    // room for kPassMax outputs when a pass may run
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * (KP ? kPassMax : 1);
+    // room for kPassMax outputs (of one filter in the lean variant) when a pass may run
+    constexpr int kTailRed =
+        (GEN ? 2 : 1) * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * (KP ? kPassMax : 1);
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
@@ -844,15 +861,11 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
         if (starts[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
+    // a copy: its fields load once, before any store, into scalar registers
+    // (read in place after the roles' stores, they would be vector loads)
     const TickArgs a = descs[lo];
-    const int lb = b - starts[lo];
-    switch (a.B) {
-    case 64: tick_body<64, HT, KP>(a, lb, smem); break;
-    case 128: tick_body<128, HT, KP>(a, lb, smem); break;
-    case 256: tick_body<256, HT, KP>(a, lb, smem); break;
-    case 512: tick_body<512, HT, KP>(a, lb, smem); break;
-    default: break;
-    }
+    if (a.B != M) return;  // the host launches each object with its own block size
+    tick_body<M, HT, KP, GEN>(a, b - starts[lo], smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1581,7 +1594,7 @@ template <int HT>
 static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream_t s, unsigned blocks,
                                 hipEvent_t ev0, hipEvent_t ev1)
 {
-    const int tk = a.nchunk == 0 ? 0 : (a.kn > 1 ? 2 : 1);
+    const int tk = a.nchunk == 0 ? 0 : (is_pass(a) ? 2 : 1);
     if (tk == 0) {
         TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 0>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
     } else if (tk == 1) {
@@ -1629,23 +1642,45 @@ hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, u
     }
 }
 
-template <int HT>
-static void launch_group_t(bool passes, const TickArgs *d, const int *st, int nobj, unsigned blocks, hipStream_t s)
+template <int HT, int M>
+static void launch_group_m(bool lean, bool passes, const TickArgs *d, const int *st, int nobj, unsigned blocks,
+                           hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (passes) hipLaunchKernelGGL((k_group_tick<HT, true>), dim3(blocks), dim3(kTickThreads), 0, s, d, st, nobj);
-    else hipLaunchKernelGGL((k_group_tick<HT, false>), dim3(blocks), dim3(kTickThreads), 0, s, d, st, nobj);
+    const dim3 g(blocks), t(kTickThreads);
+    if (lean)
+        hipExtLaunchKernelGGL((k_group_tick<HT, true, M, false>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+    else if (passes)
+        hipExtLaunchKernelGGL((k_group_tick<HT, true, M>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+    else
+        hipExtLaunchKernelGGL((k_group_tick<HT, false, M>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
 }
 
-hipError_t launch_group_tick(int irtype, bool passes, const TickArgs *d_descs, const int *d_starts, int nobj,
-                             unsigned blocks, hipStream_t s)
+template <int HT>
+static hipError_t launch_group_t(int B, bool lean, bool passes, const TickArgs *d, const int *st, int nobj,
+                                 unsigned blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (blocks == 0) return hipSuccess;
-    switch (irtype) {
-    case IR_F32: launch_group_t<IR_F32>(passes, d_descs, d_starts, nobj, blocks, s); break;
-    case IR_F16: launch_group_t<IR_F16>(passes, d_descs, d_starts, nobj, blocks, s); break;
-    case IR_BF16: launch_group_t<IR_BF16>(passes, d_descs, d_starts, nobj, blocks, s); break;
+    switch (B) {
+    case 64: launch_group_m<HT, 64>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
+    case 128: launch_group_m<HT, 128>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
+    case 256: launch_group_m<HT, 256>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
+    case 512: launch_group_m<HT, 512>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
     default: return hipErrorInvalidValue;
     }
+    return hipSuccess;
+}
+
+hipError_t launch_group_tick(int irtype, int B, bool lean, bool passes, const TickArgs *d_descs, const int *d_starts,
+                             int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (blocks == 0) return hipSuccess;
+    hipError_t e;
+    switch (irtype) {
+    case IR_F32: e = launch_group_t<IR_F32>(B, lean, passes, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    case IR_F16: e = launch_group_t<IR_F16>(B, lean, passes, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    case IR_BF16: e = launch_group_t<IR_BF16>(B, lean, passes, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -1682,7 +1717,7 @@ static hipError_t launch_window_t(int B, int tail_nf, const HeadBatchArgs &h, co
 hipError_t launch_window(int B, int tail_nf, const HeadBatchArgs &h, const TickArgs &a, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (h.n < 1 || h.n > kHeadBatch || B > kHeadBatchMaxB || a.nchunk < 1 || a.kn < 2 || a.C != h.C)
+    if (h.n < 1 || h.n > kHeadBatch || B > kHeadBatchMaxB || a.nchunk < 1 || !is_pass(a) || a.C != h.C)
         return hipErrorInvalidValue;
     switch (a.irtype) {
     case IR_F32: return launch_window_t<IR_F32>(B, tail_nf, h, a, s, ev0, ev1);
@@ -1773,13 +1808,14 @@ hipError_t launch_frame(int irtype, const FrameArgs &a, hipStream_t s, hipEvent_
 }
 
 hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d_starts, int nobj, unsigned blocks,
-                              hipStream_t s)
+                              hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0 || nobj == 0) return hipSuccess;
+    const dim3 g(blocks), t(kGroupFrameThreads);
     switch (irtype) {
-    case IR_F32: hipLaunchKernelGGL((k_group_frame<IR_F32>), dim3(blocks), dim3(kGroupFrameThreads), 0, s, d_descs, d_starts, nobj); break;
-    case IR_F16: hipLaunchKernelGGL((k_group_frame<IR_F16>), dim3(blocks), dim3(kGroupFrameThreads), 0, s, d_descs, d_starts, nobj); break;
-    case IR_BF16: hipLaunchKernelGGL((k_group_frame<IR_BF16>), dim3(blocks), dim3(kGroupFrameThreads), 0, s, d_descs, d_starts, nobj); break;
+    case IR_F32: hipExtLaunchKernelGGL((k_group_frame<IR_F32>), g, t, 0, s, ev0, ev1, 0, d_descs, d_starts, nobj); break;
+    case IR_F16: hipExtLaunchKernelGGL((k_group_frame<IR_F16>), g, t, 0, s, ev0, ev1, 0, d_descs, d_starts, nobj); break;
+    case IR_BF16: hipExtLaunchKernelGGL((k_group_frame<IR_BF16>), g, t, 0, s, ev0, ev1, 0, d_descs, d_starts, nobj); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

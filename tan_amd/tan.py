@@ -156,6 +156,21 @@ class TANContext:
         if update and cur.cuda_stream != self._raw[1]:
             self._streams[1].wait_stream(cur)
 
+    def EnableGroupTiming(self, enable=True):
+        """Launch-stamped timing of this context's group launches."""
+        return self._lib.tanx_ctx_enable_group_timing(self.handle, int(bool(enable)))
+
+    def ReadGroupTiming(self, reset=False):
+        """{tick_ms, tick_launches, tick_alg_bytes, frame_ms, frame_launches,
+        frame_alg_bytes, objects} of the group launches since the last reset."""
+        t = _lib.KernelTiming()
+        rc = self._lib.tanx_ctx_read_group_timing(self.handle, ctypes.byref(t), int(bool(reset)))
+        if rc != AMF_OK:
+            raise RuntimeError(f"read_group_timing failed: {rc}")
+        return {"tick_ms": t.head_ms, "tick_launches": t.head_launches, "tick_alg_bytes": t.head_alg_bytes,
+                "frame_ms": t.tail_ms, "frame_launches": t.tail_launches, "frame_alg_bytes": t.tail_alg_bytes,
+                "objects": t.blocks}
+
     def Terminate(self):
         if self.handle:
             self._streams = None
@@ -174,7 +189,8 @@ class TANContext:
 class TANConvolution:
     """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
 
-    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0, nu_phase=0, pass_blocks=0):
+    def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0, nu_phase=0, pass_blocks=0,
+                 pass_phase=0):
         """ir_precision: storage of the IR spectra (TAN_IR_PRECISION_*);
         nu_multiple: blocks per frame for the non-uniform methods (0: chosen
         by the library, 1: uniform partitions); nu_phase: frame phase in
@@ -182,7 +198,11 @@ class TANConvolution:
         multiple - 1 - phase), so that objects advanced together can spread
         their frame work over the ticks; pass_blocks: blocks whose partial
         sums one tail pass computes for uniform objects (1..4, 0: chosen by
-        the library, 1: a tail every block)."""
+        the library, 1: a tail every block); pass_phase: uniform objects with
+        multi-block passes run their passes in the launches of blocks b with
+        (b + pass_phase) % pass_blocks == 0 (after block 0), so objects
+        advanced together with phases 0..pass_blocks-1 spread their passes
+        over the ticks."""
         self._ctx = context
         self._lib = context._lib
         self.handle = None
@@ -192,6 +212,7 @@ class TANConvolution:
         self.nu_multiple = nu_multiple
         self.nu_phase = nu_phase
         self.pass_blocks = pass_blocks
+        self.pass_phase = pass_phase
 
     # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
     def Init(self, method, response_length, buffer_size, channels):
@@ -206,6 +227,7 @@ class TANConvolution:
         opt.nu_multiple = self.nu_multiple
         opt.nu_phase = self.nu_phase
         opt.pass_blocks = self.pass_blocks
+        opt.pass_phase = self.pass_phase
         rc = self._lib.tanx_conv_create_ex(self._ctx.handle, method, response_length, buffer_size,
                                            channels, ctypes.addressof(opt), ctypes.byref(h))
         if rc == AMF_OK:

@@ -245,6 +245,104 @@ def noalloc_group():
     print("noalloc group: outputs bit-identical", flush=True)
 
 
+def pass_phases(C=2, L=16384, B=128, nb=80):
+    """Objects with pass phases 0..3: alone with calls of uneven size and
+    dense updates, and together in group launches (one block each per
+    call); every channel against the oracle."""
+    ctx = T.TANContext(device=0)
+    x = T.signals.input_bank(C, nb * B, base_seed=41)
+    ups = {0: 0, 9: 1, 10: 2, 23: 3, 24: 4, 25: 5, 50: 6}
+    irs = {b: T.signals.ir_bank(C, L, k, base_seed=42) for b, k in ups.items()}
+    yo = oracle(L, B, x, irs, nb)
+    sizes = [1, 2, 5, 3, 7, 1, 4]
+    for phase in range(4):
+        conv = T.TANConvolution(ctx, pass_blocks=4, pass_phase=phase)
+        assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        xd = simlib.Dev(C, nb * B, x)
+        yd = simlib.Dev(C, nb * B, 0.0)
+        t = k = 0
+        cuts = sorted(set(list(irs) + [nb]))
+        while t < nb:
+            if t in irs:
+                dev = simlib.Dev(C, L, irs[t])
+                assert lib.tanx_conv_update_td_device(conv.handle, dev.ptr, L, L, None, 0) == T.AMF_OK
+            n = min(sizes[k % len(sizes)], min(c for c in cuts if c > t) - t)
+            k += 1
+            process(conv, xd, yd, t, n, B)
+            t += n
+        check(yd.a.copy(), yo, f"phase {phase} alone")
+    # four phased objects in one group launch per block
+    objs = []
+    for phase in range(4):
+        c = T.TANConvolution(ctx, pass_blocks=4, pass_phase=phase)
+        assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        objs.append(c)
+    xd = simlib.Dev(C, nb * B, x)
+    ys = [simlib.Dev(C, nb * B, 0.0) for _ in objs]
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    assert ctx.EnableGroupTiming(True) == T.AMF_OK
+    P = L // B
+    pass_bytes = C * (P - 2) * B * 16          # FDL + IR rows of partitions 2..P-1
+    head_max = 4 * (C * B * 44 + 3 * C * B * 16)  # four heads with up to 3 further partitions
+    for t in range(nb):
+        if t in irs:
+            dev = simlib.Dev(C, L, irs[t])
+            for c in objs:
+                assert lib.tanx_conv_update_td_device(c.handle, dev.ptr, L, L, None, 0) == T.AMF_OK
+        n = len(objs)
+        hs = (vp * n)(*[c.handle.value for c in objs])
+        ins = (vp * n)(*[xd.at(t * B)] * n)
+        outs = (vp * n)(*[y.at(t * B) for y in ys])
+        st = (sz * n)(*[xd.stride] * n)
+        assert lib.tanx_group_process_device(n, hs, ins, st, outs, st, None) == T.AMF_OK
+        tb = ctx.ReadGroupTiming(reset=True)["tick_alg_bytes"]
+        if 30 <= t < 50:  # no switch in flight: exactly one object's pass per tick
+            assert pass_bytes <= tb <= pass_bytes + head_max, (t, tb)
+    for phase, y in enumerate(ys):
+        check(y.a.copy(), yo, f"phase {phase} in group")
+
+
+def rt_rig(groups=8, C=16, L=65536, B=256, nticks=130):
+    """The benchmark's real-time rig at a small size: groups with pass phases
+    g % 4, each group's IR update at its own offset in the 50-block period
+    (transform on the process stream before that tick's launch), one group
+    launch per tick with group timing on; two groups against the oracle."""
+    ctx = T.TANContext(device=0)
+    assert ctx.EnableGroupTiming(True) == T.AMF_OK
+    banks = [simlib.Dev(C, L, T.signals.ir_bank(C, L, i, base_seed=5000)) for i in range(4)]
+    objs = []
+    for g in range(groups):
+        c = T.TANConvolution(ctx, pass_blocks=4, pass_phase=g % 4)
+        assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        assert lib.tanx_conv_update_td_device(c.handle, banks[g % 4].ptr, L, L, None, 0) == T.AMF_OK
+        objs.append(c)
+    x = np.random.default_rng(11).uniform(-1, 1, (groups * C, nticks * B)).astype(np.float32)
+    xd = simlib.Dev(groups * C, nticks * B, x)
+    yd = simlib.Dev(groups * C, nticks * B, 0.0)
+    ups = [dict() for _ in range(groups)]
+    for g in range(groups):
+        ups[g][0] = banks[g % 4].a.copy()
+    nupd = [0] * groups
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    for t in range(nticks):
+        for g in range(groups):
+            if (t + (g * 50) // groups) % 50 == 0:
+                nupd[g] += 1
+                bank = banks[(nupd[g] + g) % 4]
+                assert lib.tanx_conv_update_td_device(objs[g].handle, bank.ptr, L, L, None, 0) == T.AMF_OK
+                ups[g][t] = bank.a.copy()
+        hs = (vp * groups)(*[o.handle.value for o in objs])
+        ins = (vp * groups)(*[xd.at(t * B) + 4 * g * C * xd.stride for g in range(groups)])
+        outs = (vp * groups)(*[yd.at(t * B) + 4 * g * C * yd.stride for g in range(groups)])
+        st = (sz * groups)(*[xd.stride] * groups)
+        assert lib.tanx_group_process_device(groups, hs, ins, st, outs, st, None) == T.AMF_OK
+    gt = ctx.ReadGroupTiming(reset=True)
+    assert gt["tick_launches"] == nticks and gt["objects"] == nticks * groups, gt
+    for g in (1, groups - 1):
+        sl = slice(g * C, g * C + 2)
+        check(yd.a[sl].copy(), oracle(L, B, x[sl], {b: h[:2] for b, h in ups[g].items()}, nticks), f"rt group {g}")
+
+
 if __name__ == "__main__":
     globals()[sys.argv[1]]()
     print("scenario complete", flush=True)

@@ -136,10 +136,10 @@ def test_flushes_and_host_path(gpu_ctx, K):
 
 
 def test_default_pass_length(gpu_ctx):
-    """Uniform objects with 16 or more partitions whose pass streams at most
-    4.5 GB use 4-block passes by default; shorter IRs, larger objects and
-    non-uniform objects keep a tail every block."""
-    for L, B, C, want in ((65536, 256, 2, 4), (2048, 256, 2, 1), (16384, 1024, 2, 4), (65536, 256, 8192, 1),
+    """Uniform objects with 16 or more partitions use 4-block passes by
+    default, at any channel count; shorter IRs and non-uniform objects keep a
+    tail every block."""
+    for L, B, C, want in ((65536, 256, 2, 4), (2048, 256, 2, 1), (16384, 1024, 2, 4), (65536, 256, 8192, 4),
                           (65536, 256, 4096, 4)):
         conv = T.TANConvolution(gpu_ctx)
         assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
@@ -268,3 +268,75 @@ def test_pass_chunk_counts(gpu_ctx, chunks):
         _, out_o, _ = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))
         yo[:, b * B:(b + 1) * B] = out_o
     assert np.abs(y.astype(np.float64) - yo).max() <= 1e-5
+
+
+def _oracle_run(L, B, x, irs, nb):
+    C = x.shape[0]
+    orc = O.OracleConvolution(L, B, C)
+    yo = np.zeros_like(x)
+    for b in range(nb):
+        if b in irs:
+            assert orc.update_td(irs[b]) == O.AMF_OK
+        yo[:, b * B:(b + 1) * B] = orc.process(np.ascontiguousarray(x[:, b * B:(b + 1) * B]))[1]
+    return yo
+
+
+def test_pass_phases_alone_and_grouped(gpu_ctx):
+    """Pass phases 0..3 (tanx_conv_options.pass_phase): each object alone
+    with calls of uneven size and dense updates, and the four together in
+    one group launch per block, all equal to the oracle; in the group, a tick
+    with no switch in flight streams exactly one object's pass."""
+    import torch
+    L, B, C, nb = 16384, 128, 2, 80
+    x = T.signals.input_bank(C, nb * B, base_seed=41)
+    ups = {0: 0, 9: 1, 10: 2, 23: 3, 24: 4, 25: 5, 50: 6}
+    irs = {b: T.signals.ir_bank(C, L, k, base_seed=42) for b, k in ups.items()}
+    yo = _oracle_run(L, B, x, irs, nb)
+    xd = torch.from_numpy(x).cuda()
+    sizes = [1, 2, 5, 3, 7, 1, 4]
+    cuts = sorted(set(list(irs) + [nb]))
+    for phase in range(4):
+        conv = T.TANConvolution(gpu_ctx, pass_blocks=4, pass_phase=phase)
+        assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        yd = torch.zeros_like(xd)
+        t = k = 0
+        while t < nb:
+            if t in irs:
+                assert conv.UpdateResponseTD(torch.from_numpy(irs[t]).cuda(), L, None, BLOCKING) == T.AMF_OK
+            n = min(sizes[k % len(sizes)], min(c for c in cuts if c > t) - t)
+            k += 1
+            rc, done = conv.Process(xd[:, t * B:], yd[:, t * B:], n * B)
+            assert rc == T.AMF_OK and done == n * B
+            t += n
+        conv.Synchronize()
+        conv.Terminate()
+        err = np.abs(yd.cpu().numpy().astype(np.float64) - yo).max()
+        print(f"phase {phase} alone: max|gpu-oracle| = {err:.3e}", flush=True)
+        assert err <= 1e-5
+    ctx = T.TANContext(device=0)
+    objs = []
+    for phase in range(4):
+        c = T.TANConvolution(ctx, pass_blocks=4, pass_phase=phase)
+        assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        objs.append(c)
+    ys = [torch.zeros_like(xd) for _ in objs]
+    assert ctx.EnableGroupTiming(True) == T.AMF_OK
+    pass_bytes = C * (L // B - 2) * B * 16
+    head_max = 4 * (C * B * 44 + 3 * C * B * 16)
+    for t in range(nb):
+        if t in irs:
+            h = torch.from_numpy(irs[t]).cuda()
+            for c in objs:
+                assert c.UpdateResponseTD(h, L, None, BLOCKING) == T.AMF_OK
+        assert T.GroupProcess(objs, [xd[:, t * B:]] * 4, [y[:, t * B:] for y in ys]) == T.AMF_OK
+        tb = ctx.ReadGroupTiming(reset=True)["tick_alg_bytes"]
+        if 30 <= t < 50:
+            assert pass_bytes <= tb <= pass_bytes + head_max, (t, tb)
+    torch.cuda.synchronize()
+    for phase, y in enumerate(ys):
+        err = np.abs(y.cpu().numpy().astype(np.float64) - yo).max()
+        print(f"phase {phase} in group: max|gpu-oracle| = {err:.3e}", flush=True)
+        assert err <= 1e-5
+    for c in objs:
+        c.Terminate()
+    ctx.Terminate()

@@ -57,3 +57,13 @@ def test_c3_group(sim_built, guard, fill):
 def test_noalloc_group(sim_built, guard, fill):
     p = _run("noalloc_group", guard, fill)
     assert p.returncode == 0, p.returncode
+
+
+def test_pass_phases(sim_built):
+    p = _run("pass_phases")
+    assert p.returncode == 0, p.returncode
+
+
+def test_rt_rig(sim_built):
+    p = _run("rt_rig", "start", "0xff")
+    assert p.returncode == 0, p.returncode
