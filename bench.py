@@ -815,7 +815,7 @@ def profile_run(T, torch, ngroups):
     log("profile run done")
 
 
-def pmc_traffic_per_channel(kernel_prefix="void tanx::k_group_tick<0, true>"):
+def pmc_traffic_per_channel(kernel_prefix="void tanx::k_group_tick<0, true, 256, false>"):
     """HBM bytes per channel and launch of the real-time group kernel from the
     committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a
     512 MiB copy), or None."""
@@ -880,8 +880,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": per_ch * value if per_ch else None,
-                "kernel": "k_group_tick<0, true> (heads of every channel, the passes of one group in four, "
-                          "crossfade catch-up passes)",
+                "kernel": "k_group_tick<0, true, 256, false> + <0, true, 256, true> (one tick's launches: the "
+                          "heads of every channel, the passes of one group in four, a crossfading group's "
+                          "two-filter catch-up pass)",
+                "trace": "profiles/k_group_tick_realtime_81920ch_65536taps_b256.txt",
                 "traffic_source": "profiles/pmc_traffic_k_group_tick_rt.json" if per_ch else None,
                 "alg_bytes_per_launch": gt["tick_alg_bytes"] / launches, "avg_launch_us": tick_us,
                 "launches": gt["tick_launches"], "channels": value}
