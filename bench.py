@@ -140,16 +140,18 @@ RT_GROUP = 2048        # channels per object (channel group) of the real-time co
 RT_STEP_TICKS = 100    # ticks per timed step: two IR update periods
 RT_RING = 8            # input/output blocks kept on the device
 RT_BANKS = 4           # IR banks the updates cycle through
+RT_PASS = 8            # blocks per tail pass of the real-time groups (phases g % RT_PASS)
 
 
 class RealTimeRig:
     """The real-time channel count: channel groups of RT_GROUP channels (one
-    object each, 65,536 taps, B = 256, 4-block passes with phases g % 4), a
-    fresh IR for every channel every 50 blocks (group g's update at the ticks
-    t with (t + 50 g / k) % 50 == 0, so updates and their crossfades are
-    staggered over the period), all k active groups advanced by one group
-    launch per tick with one block in flight.  Everything runs on one torch
-    stream that is the context's process stream."""
+    object each, 65,536 taps, B = 256, RT_PASS-block passes with phases
+    g % RT_PASS), a fresh IR for every channel every 50 blocks (group g's
+    update at the ticks t with (t + 50 g / k) % 50 == 0, so updates and
+    their crossfades are staggered over the period), all k active groups
+    advanced by one group launch per tick with one block in flight.
+    Everything runs on one torch stream that is the context's process
+    stream."""
 
     def __init__(self, T, torch, ngroups):
         self.T, self.torch = T, torch
@@ -163,7 +165,7 @@ class RealTimeRig:
             self.y = torch.zeros_like(self.x)
             self.objs = []
             for g in range(ngroups):
-                conv = T.TANConvolution(self.ctx, pass_blocks=4, pass_phase=g % 4)
+                conv = T.TANConvolution(self.ctx, pass_blocks=RT_PASS, pass_phase=g % RT_PASS)
                 rc = conv.InitGpu(UNIFORM_METHOD, L, B, RT_GROUP)
                 if rc != T.AMF_OK:
                     log(f"  group {g}: InitGpu returned {rc}; {g} groups")
@@ -240,7 +242,7 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
     gc_ = __import__("gc")
     per_group = S.estimate_bytes(T._lib.load(), L, B, RT_GROUP)
     free, _ = torch.cuda.mem_get_info()
-    reserve = RT_BANKS * RT_GROUP * L * 4 + (6 << 30)
+    reserve = RT_BANKS * RT_GROUP * L * 4 + (10 << 30)
     kmax = int((free - reserve) // (per_group + 2 * RT_RING * RT_GROUP * B * 4))
     log(f"real-time count: {free / 2**30:.0f} GiB free, {per_group / 2**20:.0f} MiB per group of {RT_GROUP} "
         f"channels: up to {kmax} groups")
@@ -291,7 +293,9 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
             if misses == 0 or k == 1:
                 break
             k -= 1
-        finite = bool(torch.isfinite(rig.y).all().item())
+        # in slices: HBM may be nearly full with the groups' state
+        yv = rig.y.view(-1)
+        finite = all(bool(torch.isfinite(yv[i:i + (1 << 24)]).all().item()) for i in range(0, yv.numel(), 1 << 24))
         result["outputs_finite"] = finite
         return result
     finally:
@@ -831,6 +835,7 @@ def pmc_traffic_per_channel(kernel_prefix="void tanx::k_group_tick<0, true, 256,
 
 
 def main():
+    global TAIL_CHUNKS, RT_PASS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps of 100 ticks (2,000 ticks at 20)")
@@ -852,11 +857,13 @@ def main():
     ap.add_argument("--nu-sweep", type=str, default="65536,98304")
     ap.add_argument("--rt-groups", type=int, default=0,
                     help="profile run: real-time rig with this many groups of 2,048 channels")
+    ap.add_argument("--rt-pass-blocks", type=int, default=RT_PASS,
+                    help="blocks per tail pass of the real-time groups (1..8)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
-    global TAIL_CHUNKS
     TAIL_CHUNKS = args.tail_chunks
+    RT_PASS = args.rt_pass_blocks
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (channels and "
               "sessions shard across GPUs without exchange; see the design notes)", file=sys.stderr)
@@ -880,9 +887,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": per_ch * value if per_ch else None,
-                "kernel": "k_group_tick<0, true, 256, false> + <0, true, 256, true> (one tick's launches: the "
-                          "heads of every channel, the passes of one group in four, a crossfading group's "
-                          "two-filter catch-up pass)",
+                "kernel": f"k_group_tick<0, 256, 1, {RT_PASS}> (the heads of every channel, the "
+                          f"single-filter passes of one group in {RT_PASS}, the new filter's catch-up pass "
+                          f"of each group whose IR switched) + k_group_tick<0, 256, 2, {RT_PASS}> when a "
+                          "switch falls on a group's pass block (its two-filter pass)",
                 "trace": "profiles/k_group_tick_realtime_81920ch_65536taps_b256.txt",
                 "traffic_source": "profiles/pmc_traffic_k_group_tick_rt.json" if per_ch else None,
                 "alg_bytes_per_launch": gt["tick_alg_bytes"] / launches, "avg_launch_us": tick_us,
@@ -977,7 +985,7 @@ def main():
         "data": "synthetic (decaying-noise IRs, U[-1,1] input, generated on device)",
         "config": {"workload": RT_WORKLOAD, "channels": value, "channel_groups": rt["groups"],
                    "group_channels": RT_GROUP, "block": B, "ir_taps": L, "ticks": rt["ticks"],
-                   "ticks_per_step": RT_STEP_TICKS, "update_every": UPDATE_EVERY, "pass_blocks": 4,
+                   "ticks_per_step": RT_STEP_TICKS, "update_every": UPDATE_EVERY, "pass_blocks": RT_PASS,
                    "sample_rate": FS, "deadline_ms": DEADLINE_MS},
         "deadline_misses": rt["misses"],
         "p50_block_ms": rt["p50_host_ms"], "p99_block_ms": rt["p99_host_ms"], "max_block_ms": rt["max_host_ms"],

@@ -152,9 +152,12 @@ typedef struct tanx_conv_options {
                                Objects of one group with different phases
                                spread their frame work over the ticks. */
     uint32_t pass_blocks;   /* uniform objects: blocks whose partial sums one
-                               tail pass computes (1..4; 1 = a tail every
+                               tail pass computes (1..8; 1 = a tail every
                                block; 0 = chosen by the library: 4 when the
-                               IR has 16 or more partitions, else 1) */
+                               IR has 16 or more partitions, else 1).  A
+                               pass streams the IR and FDL rows once for K
+                               blocks; passes of 5..8 blocks run a kernel
+                               with more registers per lane. */
     uint32_t pass_phase;    /* uniform objects with multi-block passes: pass
                                phase in blocks, taken modulo pass_blocks K.
                                After block 0, passes run in the launches of

@@ -615,7 +615,7 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
     if (want_kp > (uint32_t)kPassMax) return TANX_INVALID_ARG;
     const bool finalize = (method & TANX_METHOD_USE_PROCESS_TAILTHREAD) == TANX_METHOD_USE_PROCESS_FINALIZE;
     if (M == 1 && !finalize) {
-        sh.kp = want_kp ? (int)want_kp : (sh.P >= 16 ? kPassMax : 1);
+        sh.kp = want_kp ? (int)want_kp : (sh.P >= 16 ? kPassDefault : 1);
         sh.kphase = (int)((opt ? opt->pass_phase : 0) % (uint32_t)sh.kp);
     }
     return TANX_OK;
@@ -926,6 +926,7 @@ static TickArgs tick_common(tanx_conv *cv)
     a.twN = tw_table(cv->ctx->tw, lB + 1);
     a.fdl = cv->F;
     a.cached = cached_rows(cv) ? 1 : 0;
+    a.kp = cv->kp;
     a.kn = 1;
     a.kgap = 1;
     a.dA = a.dB = 1;
@@ -960,11 +961,14 @@ static int fill_tail(tanx_conv *cv, TickArgs &a)
 
 // Pass of a launch running the heads of blocks [w, e) (cv->ticks == e, the
 // switch state that after block e-1): when block e has no partial sums yet,
-// or a switch armed at block e-1 made the sums computed with the old filter
-// stale, the sums of blocks e .. e+kp-1 from FDL rows before w (output k:
-// partitions >= e - w + k + 1), with the filter each block will use: the
-// old one before an armed switch, both at its crossfade block, the new one
-// after it.  Returns the number of filters the pass reads (0: no pass).
+// the sums of blocks e .. c (c the next block on the pass grid) from FDL
+// rows before w (output k: partitions >= e - w + k + 1), with the filter
+// each block will use: the old one before an armed switch, both at its
+// crossfade block, the new one after it.  When a switch armed at block e-1
+// falls in blocks an earlier pass covered, the old filter's sums of those
+// blocks stay valid and a catch-up pass computes the new filter's alone:
+// the crossfade block's second sums and the sums of the blocks after it.
+// Returns the number of filters the pass reads (0: no pass).
 static int fill_pass(tanx_conv *cv, long long w, long long e, TickArgs &a)
 {
     const long long sw = cv->delayed ? (long long)cv->switch_at : -1;
@@ -987,6 +991,22 @@ static int fill_pass(tanx_conv *cv, long long w, long long e, TickArgs &a)
     a.kn = kn;
     a.kgap = (int)(e - w);
     a.tbase = cv->pos;  // ring slot of block e
+    // an earlier pass's grid ends at covered, so this pass ends there too;
+    // its old-filter sums of the crossfade block are kept when they were
+    // computed with the current chunking (the head reads both with one)
+    if (stale && e + kn - 1 == cv->covered && cv->pchunk[cv->pring(sw)][0] == cv->nchunk) {
+        a.tHa = cv->slot_ptr(cv->upd);
+        a.tSa = cv->scale_ptr(cv->upd);
+        for (int k = (int)s_rel; k < kn; k++) {
+            const long long blk = e + k;
+            const int r = cv->pring(blk);
+            const int f = k == s_rel ? 1 : 0;  // after the switch the new filter is the block's own
+            a.kTA[k] = cv->tail_slot(blk, f);
+            cv->porg[r][f] = w;
+            cv->pchunk[r][f] = cv->nchunk;
+        }
+        return 1;
+    }
     a.tHa = cv->slot_ptr(cv->cur);
     a.tSa = cv->scale_ptr(cv->cur);
     bool second = false;
@@ -1255,7 +1275,8 @@ static int32_t prepare_block(tanx_conv *cv, const float *din, long long in_strid
         a.dA = (int)(bb - cv->porg[r][0]);
         a.dB = xfade ? (int)(bb - cv->porg[r][1]) : 1;
         a.tchunk = cv->pchunk[r][0];
-        if (a.dA < 1 || a.dA > kNearMax || a.dB < 1 || a.dB > kNearMax ||
+        const int near = kHeadBatch + pass_km(cv->kp) - 1;  // what a batched head adds, at most
+        if (a.dA < 1 || a.dA > near || a.dB < 1 || a.dB > near ||
             (xfade && cv->pchunk[r][1] != a.tchunk))
             return TANX_UNEXPECTED;
     }
@@ -1378,11 +1399,13 @@ static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
     h.in_stride = a0.in_stride;
     h.out_stride = a0.out_stride;
     h.n = n;
+    h.kp = cv->kp;
     // entries past the batch repeat its last block (the kernel loads them unused)
     for (int j = 0; j < kHeadBatch; j++) {
         const TickArgs &a = q.a[std::min(j, n - 1)];
         // one filter for the whole batch (a crossfade block runs alone)
-        if (a.xfade || a.dA > kNearMax || a.Ha != a0.Ha || a.Sa != a0.Sa) return TANX_UNEXPECTED;
+        if (a.xfade || a.dA > kHeadBatch + pass_km(cv->kp) - 1 || a.Ha != a0.Ha || a.Sa != a0.Sa)
+            return TANX_UNEXPECTED;
         h.xin[j] = a.xin;
         h.yout[j] = a.yout;
         h.H[j] = a.Ha;
@@ -1978,37 +2001,37 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
         }
     }
     if (done > 0) {
-        // each class's range splits into the lean descriptors first and the
-        // general ones after them (swapped in place), each sub-range one
-        // launch with block offsets of its own
+        // each class's range splits by launch kind (role set, then pass
+        // length) in place, each sub-range one launch with block offsets of
+        // its own: lean 4, lean 8, crossfade 4, crossfade 8, general
+        constexpr int kKinds = 5;
+        auto kind_of = [](const TickArgs &a) {
+            const int r = group_roles(a);
+            return r == kRolesGeneral ? 4 : 2 * (r == kRolesXfade) + (pass_km(a.kp) == 8);
+        };
         struct Launch {
             uint32_t first, count;
-            int B, total;
-            bool lean, passes;
-        } launches[2 * kClasses];
+            int B, total, roles, km;
+        } launches[kKinds * kClasses];
         int nl = 0;
         for (int c = 0; c < kClasses; c++) {
             const Class &k = cls[c];
-            if (k.placed == 0) continue;
-            uint32_t lo = k.first, hi = k.first + k.placed;
-            while (lo < hi) {
-                if (is_lean(descs[lo])) {
-                    lo++;
-                } else {
-                    hi--;
-                    std::swap(descs[lo], descs[hi]);
-                }
-            }
-            for (int part = 0; part < 2; part++) {
-                const uint32_t a0 = part == 0 ? k.first : lo, a1 = part == 0 ? lo : k.first + k.placed;
-                if (a0 == a1) continue;
+            uint32_t lo = k.first;
+            const uint32_t end = k.first + k.placed;
+            for (int kind = 0; kind < kKinds && lo < end; kind++) {
+                // move this kind's descriptors to the front of [lo, end)
+                uint32_t mid = lo;
+                for (uint32_t i = lo; i < end; i++)
+                    if (kind_of(descs[i]) == kind) std::swap(descs[i], descs[mid++]);
+                if (mid == lo) continue;
                 Launch &l = launches[nl++];
-                l = Launch{a0, a1 - a0, 64 << c, 0, part == 0, false};
-                for (uint32_t i = a0; i < a1; i++) {
+                l = Launch{lo, mid - lo, 64 << c, 0, kind == 4 ? kRolesGeneral : (kind >= 2 ? kRolesXfade : kRolesLean),
+                           (kind & 1) ? 8 : 4};
+                for (uint32_t i = lo; i < mid; i++) {
                     starts[i] = l.total;
                     l.total += descs[i].C * (1 + descs[i].nchunk + descs[i].nchB);
-                    l.passes = l.passes || is_pass(descs[i]);
                 }
+                lo = mid;
             }
         }
         // the staging pieces are copied in one transfer at their offsets
@@ -2025,7 +2048,7 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
             const TickArgs *dd = reinterpret_cast<const TickArgs *>(g.dev) + l.first;
             const int *ds = reinterpret_cast<const int *>(g.dev + g.desc_bytes) + l.first;
             // the first launch stamps the start event, the last one the end
-            TANX_CHECK(launch_group_tick(irtype, l.B, l.lean, l.passes, dd, ds, (int)l.count, (unsigned)l.total,
+            TANX_CHECK(launch_group_tick(irtype, l.B, l.roles, l.km, dd, ds, (int)l.count, (unsigned)l.total,
                                          ctx->proc, i == 0 ? t0 : nullptr, i == nl - 1 ? t1 : nullptr));
         }
         if (ctx->gtiming) {

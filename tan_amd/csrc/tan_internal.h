@@ -43,10 +43,12 @@ namespace tanx {
 enum IrType { IR_F32 = 0, IR_F16 = 1, IR_BF16 = 2 };
 inline int ir_elem_bytes(int t) { return t == IR_F32 ? 8 : 4; }
 
-constexpr int kPassMax = 4;  // blocks one tail pass computes partial sums for
+constexpr int kPassMax = 8;      // blocks one tail pass computes partial sums for, at most
+constexpr int kPassDefault = 4;  // the library's choice for IRs of 16 or more partitions
+constexpr int kHeadBatch = 4;    // heads one launch runs together, at most
 // partitions a head adds itself at most: a pass of the launch with heads
-// [w, e) serves blocks up to e + kPassMax - 1 from rows <= w - 1, e - w <= kPassMax
-constexpr int kNearMax = 2 * kPassMax - 1;
+// [w, e) serves blocks up to e + kPassMax - 1 from rows <= w - 1, e - w <= kHeadBatch
+constexpr int kNearMax = kHeadBatch + kPassMax - 1;
 
 struct TickArgs {
     int B;                 // block size (group launches dispatch on it)
@@ -92,10 +94,11 @@ struct TickArgs {
     // over partitions i >= kgap + k + 1 (FDL row (tbase + k - i) & (P - 1),
     // tbase the ring slot of block o + kgap); filter tHa's sums go to
     // kTA[k], filter tHb's to kTB[k] (nullptr: not computed).  tTa/tTb unused.
+    int kp;                // the object's pass length (the launch compiles passes of at most 4 or 8 outputs)
     int kn;
     int kgap;
-    float2 *kTA[4];
-    float2 *kTB[4];
+    float2 *kTA[kPassMax];
+    float2 *kTB[kPassMax];
     // head: partitions 0..dA (filter A) and 0..dB (filter B), at most
     // kNearMax, are added by the head itself (1 without multi-block passes);
     // the rest come from Ta/Tb
@@ -136,12 +139,9 @@ __host__ __device__ inline bool is_pass(const TickArgs &a)
     return a.kn > 1 || a.kTA[0] != nullptr || a.kTB[0] != nullptr;
 }
 
-// A descriptor the lean group kernel can run: heads, and a tail that is a
-// single-filter pass or none; no frame slice.
-__host__ __device__ inline bool is_lean(const TickArgs &a)
-{
-    return a.nchB == 0 && (a.nchunk == 0 || (is_pass(a) && a.tHb == nullptr));
-}
+// Pass length a launch compiles for an object of pass length kp (registers
+// grow with it): 4 or 8.
+__host__ __device__ inline int pass_km(int kp) { return kp > 4 ? 8 : 4; }
 
 // Time-domain convolution (TAN_CONVOLUTION_METHOD_TIME_DOMAIN): one
 // workgroup per channel computes n <= B outputs
@@ -220,12 +220,20 @@ hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hip
                        hipEvent_t ev1 = nullptr);
 // One launch for several objects of block size B: block b belongs to the
 // object o with starts[o] <= b < starts[o + 1] and runs its roles from
-// d_descs[o].  Every object must use irtype and block size B (64, 128, 256
-// or 512); lean: every descriptor is_lean (the kernel without the other
-// roles); else passes: some descriptor carries a pass (is_pass).
-hipError_t launch_group_tick(int irtype, int B, bool lean, bool passes, const TickArgs *d_descs, const int *d_starts,
+// d_descs[o].  Every object must use irtype, block size B (64, 128, 256 or
+// 512), role set `roles` (group_roles) and, for passes, pass_km(kp) == km.
+hipError_t launch_group_tick(int irtype, int B, int roles, int km, const TickArgs *d_descs, const int *d_starts,
                              int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);
+// Role set of a group descriptor (kRoles*): heads with a single-filter pass
+// or none (lean), with a two-filter pass (crossfade), or with a single tail
+// or a frame slice (general).
+constexpr int kRolesGeneral = 0, kRolesLean = 1, kRolesXfade = 2;
+__host__ __device__ inline int group_roles(const TickArgs &a)
+{
+    if (a.nchB > 0 || (a.nchunk > 0 && !is_pass(a))) return kRolesGeneral;
+    return (a.nchunk > 0 && a.tHb != nullptr) ? kRolesXfade : kRolesLean;
+}
 constexpr int kGroupMaxB = 512;
 // Consecutive head-only blocks of a uniform object with multi-block passes,
 // launched together (one workgroup per channel): block j of the batch
@@ -236,10 +244,10 @@ constexpr int kGroupMaxB = 512;
 // runs the batch alone; with a pass (TickArgs a with kn > 1 and its gap
 // after the batch), launch_window runs the batch and the pass in one
 // launch: the pass reads FDL rows before the batch only.
-constexpr int kHeadBatch = kPassMax;
 constexpr int kHeadBatchMaxB = 256;  // larger blocks: one head per launch (registers)
 struct HeadBatchArgs {
     int C, P;
+    int kp;                // the object's pass length: d[j] <= kHeadBatch + kp - 1
     const uint8_t *run;
     const float2 *twM, *twN;
     float2 *fdl;

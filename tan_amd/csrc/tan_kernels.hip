@@ -140,6 +140,12 @@ __device__ __forceinline__ float2 c2r_pre(const float2 *A, int k, const float2 *
 // ---------------------------------------------------------------------------
 constexpr int kTickThreads = 256;
 constexpr int kHeadChunkBatch = 16;  // partial-sum rows loaded together by the head
+constexpr int kHeadPreload = 4;      // near partitions a head loads with its partial sums
+
+// float4 of LDS a tail or pass role reduces its lane groups' sums through:
+// [filter (<= 2)][lane group G][row float4 H] per round, G * H = 256 * V
+template <int M>
+constexpr int tail_red_float4() { return 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1); }
 
 // Streamed rows (IR spectra, FDL) are read once per block: nontemporal loads
 // keep them from displacing reused data in the caches.  A multi-block pass
@@ -328,12 +334,12 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
             float2 s = make_float2(0.f, 0.f);
             if (k < M) {
                 // the near rows' loads go out with the partial sums' (one
-                // memory round trip): partitions 1..kPassMax from clamped
+                // memory round trip): partitions 1..kHeadPreload from clamped
                 // indices, used up to d
                 const int d = dmax == 1 ? 1 : min(f == 0 ? a.dA : a.dB, a.P - 1);
-                float2 fn[kPassMax], hn[kPassMax];
+                float2 fn[kHeadPreload], hn[kHeadPreload];
 #pragma unroll
-                for (int jn = 1; jn <= kPassMax; jn++) {
+                for (int jn = 1; jn <= kHeadPreload; jn++) {
                     const int jj = a.P > 1 ? max(1, min(jn, min(d, a.P - 1))) : 0;  // in range, P = 1 too
                     fn[jn - 1] = a.fdl[((size_t)c * a.P + ((a.slot - jj) & (a.P - 1))) * M + k];
                     const float sj = ir_scale<HT>(Sbase, c, a.P, jj);
@@ -344,7 +350,7 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                 s = sum_partials<M>(T, a.tchunk, k);
                 // partitions d..1, descending (the order heads_role keeps):
                 // the blocks since the launch whose pass computed the partials
-                for (int jn = kNearMax; jn > kPassMax; jn--) {
+                for (int jn = kNearMax; jn > kHeadPreload; jn--) {
                     if (jn > d) continue;
                     const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
                     const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
@@ -352,7 +358,7 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                     mac2(s, Fj[k], make_float2(hv.x * sj, hv.y * sj), k == 0);
                 }
 #pragma unroll
-                for (int jn = kPassMax; jn >= 1; jn--)
+                for (int jn = kHeadPreload; jn >= 1; jn--)
                     if (jn <= d && a.P > 1) mac2(s, fn[jn - 1], hn[jn - 1], k == 0);
                 h0[f][q] = make_float2(h0v.x * s0, h0v.y * s0);
             }
@@ -543,6 +549,7 @@ __device__ void mac_rows(const MacView &w, float4 *red)
         }
     } else {
         // red: [NF][G][H] float4
+        static_assert(NF * G * H <= tail_red_float4<M>(), "tail reduction buffer");
 #pragma unroll
         for (int v = 0; v < V; v++) {
             red[(0 * G + g) * H + v * R + r] = accA[v];
@@ -569,7 +576,7 @@ __device__ void mac_rows(const MacView &w, float4 *red)
 // for output 0: each lane group walks its partitions in order and keeps the
 // last kPassMax rows it loaded, so every IR row and FDL row is read once and
 // used for every output.
-template <int M, int NF, int HT, bool NT = true>
+template <int M, int NF, int HT, bool NT = true, int KM = kPassMax>
 __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
 {
     constexpr int H = M / 2;                     // float4 per row
@@ -577,7 +584,7 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
     constexpr int R = H / V;                     // lanes per row
     constexpr int G = kTickThreads / R;          // lane groups
     constexpr int U = 4;                         // partitions loaded together (8 measured slower)
-    constexpr int KM = kPassMax;
+    static_assert(KM >= 1 && KM <= kPassMax, "pass length");
     constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
     const int tid = threadIdx.x;
     const int g = tid / R, r = tid % R;
@@ -658,7 +665,8 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
             }
         }
     }
-    // the lane groups' sums of every output through red at once: [NF][KM][G][H]
+    // the lane groups' sums reduced through red one output at a time:
+    // [NF][G][H] float4 per round
     const size_t orow = ((size_t)c * a.nchunk + j) * H;
     if constexpr (G == 1) {
 #pragma unroll
@@ -669,33 +677,36 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
                 if (NF == 2 && wantB[k]) reinterpret_cast<float4 *>(a.kTB[k])[orow + v * R + r] = accB[k][v];
             }
     } else {
+        static_assert(NF * G * H <= tail_red_float4<M>(), "pass reduction buffer");
+        // unrolled over the outputs so that every pointer is picked
+        // statically (an output pointer indexed at run time would put the
+        // whole argument struct in private memory); the skip is uniform
 #pragma unroll
-        for (int k = 0; k < KM; k++)
+        for (int k = 0; k < KM; k++) {
+            if (!wantA[k] && !(NF == 2 && wantB[k])) continue;
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                if (wantA[k]) red[((0 * KM + k) * G + g) * H + v * R + r] = accA[k][v];
-                if (NF == 2 && wantB[k]) red[((1 * KM + k) * G + g) * H + v * R + r] = accB[k][v];
+                red[(0 * G + g) * H + v * R + r] = accA[k][v];
+                if (NF == 2) red[(1 * G + g) * H + v * R + r] = accB[k][v];
             }
-        __syncthreads();
-        // outputs by unrolled (filter, output) so every pointer is picked
-        // statically (an output pointer indexed at run time would put the
-        // whole argument struct in private memory)
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-#pragma unroll
-            for (int k = 0; k < KM; k++) {
-                if (f == 0 ? !wantA[k] : !wantB[k]) continue;
-                float4 *dst = reinterpret_cast<float4 *>(f == 0 ? a.kTA[k] : a.kTB[k]) + orow;
+            __syncthreads();
+            // one loop per filter: a pointer chosen between the two at run
+            // time would also put the argument copy in private memory
+            auto reduce_to = [&](int f, float2 *out) {
                 for (int q = tid; q < H; q += kTickThreads) {
-                    float4 sum = red[((f * KM + k) * G + 0) * H + q];
+                    float4 sum = red[(f * G + 0) * H + q];
 #pragma unroll
                     for (int gg = 1; gg < G; gg++) {
-                        const float4 t = red[((f * KM + k) * G + gg) * H + q];
+                        const float4 t = red[(f * G + gg) * H + q];
                         sum.x += t.x; sum.y += t.y; sum.z += t.z; sum.w += t.w;
                     }
-                    dst[q] = sum;
+                    reinterpret_cast<float4 *>(out)[orow + q] = sum;
                 }
-            }
+            };
+            if (wantA[k]) reduce_to(0, a.kTA[k]);
+            if (NF == 2 && wantB[k]) reduce_to(1, a.kTB[k]);
+            __syncthreads();
+        }
     }
 }
 
@@ -764,11 +775,11 @@ __device__ __forceinline__ void frame_mac_block(const TickArgs &a, int t, float4
 // 1 = the next block's tail, 2 = a multi-block pass.  Separate
 // instantiations keep head-only launches light and let profiles tell the
 // launch kinds apart.
-template <int M, int NF, int HT, int TK>
+template <int M, int NF, int HT, int TK, int KM = kPassMax>
 __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
 {
-    // tail reduction buffer (float4): [NF][G][H], and kPassMax outputs for a pass
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * (TK == 2 ? kPassMax : 1);
+    // tail or pass reduction buffer (float4): [NF][G][H]
+    constexpr int kTailRed = tail_red_float4<M>();
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
@@ -780,21 +791,20 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
         const int t = b - a.nhead;
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        if constexpr (TK == 2) tailk_role<M, NF, HT>(a, c, j, smem);
+        if constexpr (TK == 2) tailk_role<M, NF, HT, true, KM>(a, c, j, smem);
         else tail_role<M, NF, HT>(a, c, j, smem);
     } else {
         frame_mac_block<M, HT>(a, b - a.nhead - a.C * a.nchunk, smem);
     }
 }
 
-// Roles of one tick for block index b of one object's launch; KP compiles
-// in the multi-block pass (its larger reduction buffer and registers are
-// then paid by every block of the launch).
-// GEN = false compiles only the lean roles (heads, and single-filter
-// passes), for objects whose tail is such a pass or absent and that have
-// no frame slice (is_lean); the registers of the other roles would cut the
-// occupancy of every launch that includes them.
-template <int M, int HT, bool KP, bool GEN = true>
+// Roles of one tick for block index b of one object's launch.  Role sets of
+// a tick launch (each compiles only its roles, so a launch carries the
+// registers of what it runs): kRolesGeneral: heads, single tails and frame
+// slices (objects without multi-block passes); kRolesLean: heads and
+// single-filter passes of at most KM outputs; kRolesXfade: heads and
+// two-filter passes (a switch in flight).
+template <int M, int HT, int ROLES, int KM = kPassMax>
 __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem)
 {
     if (b < a.nhead) {
@@ -803,21 +813,16 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
     } else {
         const int t = b - a.nhead;
         if (t >= a.C * a.nchunk) {
-            if constexpr (GEN) frame_mac_block<M, HT>(a, t - a.C * a.nchunk, smem);
+            if constexpr (ROLES == kRolesGeneral) frame_mac_block<M, HT>(a, t - a.C * a.nchunk, smem);
             return;
         }
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        if constexpr (!GEN) {
-            tailk_role<M, 1, HT>(a, c, j, smem);
+        if constexpr (ROLES == kRolesLean) {
+            tailk_role<M, 1, HT, true, KM>(a, c, j, smem);
+        } else if constexpr (ROLES == kRolesXfade) {
+            tailk_role<M, 2, HT, true, KM>(a, c, j, smem);
         } else {
-            if constexpr (KP) {
-                if (is_pass(a)) {
-                    if (a.tHb) tailk_role<M, 2, HT>(a, c, j, smem);
-                    else tailk_role<M, 1, HT>(a, c, j, smem);
-                    return;
-                }
-            }
             if (a.tHb) tail_role<M, 2, HT>(a, c, j, smem);
             else tail_role<M, 1, HT>(a, c, j, smem);
         }
@@ -832,25 +837,22 @@ __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem
 template <int M, int HT>
 __global__ __launch_bounds__(kTickThreads) void k_tick_desc(const TickArgs *__restrict__ descs, int k)
 {
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1);  // float4
+    constexpr int kTailRed = tail_red_float4<M>();  // float4
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
-    tick_body<M, HT, false>(descs[k], blockIdx.x, smem);
+    tick_body<M, HT, kRolesGeneral>(descs[k], blockIdx.x, smem);
 }
 
-// One launch for the objects of one block size M (a group call launches
-// one per block size present, lean and general descriptors apart).  KP:
-// some object runs a multi-block pass in this launch; GEN: the general
-// roles are compiled in.  A workgroup finds its object by binary search
-// over the block offsets.
-template <int HT, bool KP, int M, bool GEN = true>
+// One launch for the objects of one block size M and role set (a group
+// call launches one per block size, role set and pass length present).  A
+// workgroup finds its object by binary search over the block offsets.
+template <int HT, int M, int ROLES, int KM>
 __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__restrict__ descs,
                                                              const int *__restrict__ starts, int nobj)
 {
-    // room for kPassMax outputs (of one filter in the lean variant) when a pass may run
-    constexpr int kTailRed =
-        (GEN ? 2 : 1) * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * (KP ? kPassMax : 1);
+    // tail or pass reduction buffer (float4): [NF][G][H]
+    constexpr int kTailRed = tail_red_float4<M>();
     constexpr int kHeadBytes = sizeof(HeadSmem<M>);
     constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
@@ -865,7 +867,7 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
     // (read in place after the roles' stores, they would be vector loads)
     const TickArgs a = descs[lo];
     if (a.B != M) return;  // the host launches each object with its own block size
-    tick_body<M, HT, KP, GEN>(a, b - starts[lo], smem);
+    tick_body<M, HT, ROLES, KM>(a, b - starts[lo], smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -884,7 +886,7 @@ struct HeadsSmem {
     float2 a[kHeadBatch][M], b[kHeadBatch][M], x[kHeadBatch][M], tw[M], twn[M];
 };
 
-template <int M, int HT>
+template <int M, int HT, int NEAR = kNearMax>
 __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
 {
     constexpr int NT = kTickThreads;
@@ -917,19 +919,19 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
     // Every global load of the batch is issued before any is used (one
     // memory round trip).  The blocks of a batch share one filter (no
     // crossfade inside a batch) and the FDL rows before it, so a thread
-    // holds, for its bins, IR partitions 0..kNearMax and the rows of blocks
-    // w-1 .. w-kNearMax once (clamped indices, used where in range); block
+    // holds, for its bins, IR partitions 0..NEAR and the rows of blocks
+    // w-1 .. w-NEAR once (clamped indices, used where in range); block
     // j adds partitions d_j..j+1 from those rows before its FFT and
     // partitions j..0 from the batch's spectra after it.  Entries j >= n
     // repeat block n-1 (valid addresses, results unused).
     constexpr int KP = kHeadBatch;
-    float2 acc[KP][KPT], hr[kNearMax + 1][KPT], fr[kNearMax + 1][KPT];
-    const int dlim = min(kNearMax, h.P - 1);
+    float2 acc[KP][KPT], hr[NEAR + 1][KPT], fr[NEAR + 1][KPT];
+    const int dlim = min(NEAR, h.P - 1);
 #pragma unroll
     for (int q = 0; q < KPT; q++) {
         const int k = min(tid + q * NT, M - 1);
 #pragma unroll
-        for (int i = 0; i <= kNearMax; i++) {
+        for (int i = 0; i <= NEAR; i++) {
             const int ii = min(i, dlim);
             const float si = ir_scale<HT>(h.S[0], c, h.P, ii);
             const float2 hv = IrRow<HT>::get1(ir_row<HT>(h.H[0], c, h.P, ii, M), k);
@@ -976,7 +978,7 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
             const int d = min(h.d[j], dlim);
             // partitions d..j+1: rows of blocks w-1 .. (rows before the batch), the larger first
 #pragma unroll
-            for (int i = kNearMax; i > j; i--)
+            for (int i = NEAR; i > j; i--)
                 if (i <= d) mac2(acc[j][q], fr[i - j][q], hr[i][q], k == 0);
         }
     }
@@ -1029,31 +1031,33 @@ __device__ void heads_role(const HeadBatchArgs &h, int c, HeadsSmem<M> &sm)
     }
 }
 
-template <int M, int HT>
+// NEAR: the most partitions a head of the batch adds (kHeadBatch + the
+// object's pass length - 1)
+template <int M, int HT, int NEAR>
 __global__ __launch_bounds__(kTickThreads) void k_heads(HeadBatchArgs h)
 {
     __shared__ HeadsSmem<M> sm;
-    heads_role<M, HT>(h, blockIdx.x, sm);
+    heads_role<M, HT, NEAR>(h, blockIdx.x, sm);
 }
 
 // A head batch and the pass for the blocks after it in one launch: blocks
 // [0, C) run the heads (started first: their chain of FFTs is the longer
 // one), the rest the pass, which reads FDL rows before the batch only.
-template <int M, int NF, int HT>
+template <int M, int NF, int HT, int KM>
 __global__ __launch_bounds__(kTickThreads) void k_window(HeadBatchArgs h, TickArgs a)
 {
-    constexpr int kTailRed = 2 * kTickThreads * ((M / 2 > 256) ? (M / 2) / 256 : 1) * kPassMax;  // float4
+    constexpr int kTailRed = tail_red_float4<M>();  // float4
     constexpr int kBytes = sizeof(HeadsSmem<M>) > kTailRed * 16 ? sizeof(HeadsSmem<M>) : kTailRed * 16;
     __shared__ float4 smem[(kBytes + 15) / 16];
     const int b = blockIdx.x;
     if (b < h.C) {
-        heads_role<M, HT>(h, b, *reinterpret_cast<HeadsSmem<M> *>(smem));
+        heads_role<M, HT, kHeadBatch + KM - 1>(h, b, *reinterpret_cast<HeadsSmem<M> *>(smem));
     } else {
         const int t = b - h.C;
         const int c = t / a.nchunk, j = t % a.nchunk;
         if (!a.run[c]) return;
-        if (a.cached) tailk_role<M, NF, HT, false>(a, c, j, smem);
-        else tailk_role<M, NF, HT, true>(a, c, j, smem);
+        if (a.cached) tailk_role<M, NF, HT, false, KM>(a, c, j, smem);
+        else tailk_role<M, NF, HT, true, KM>(a, c, j, smem);
     }
 }
 
@@ -1596,18 +1600,24 @@ static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream
 {
     const int tk = a.nchunk == 0 ? 0 : (is_pass(a) ? 2 : 1);
     if (tk == 0) {
-        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 0>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 0, 4>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
     } else if (tk == 1) {
         if (tail_nf == 2) {
-            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 1>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 1, 4>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
         } else {
-            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 1>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 1, 4>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        }
+    } else if (pass_km(a.kp) == 4) {
+        if (tail_nf == 2) {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 2, 4>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        } else {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 2, 4>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
         }
     } else {
         if (tail_nf == 2) {
-            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 2>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 2, 8>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
         } else {
-            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 2>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 2, 8>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
         }
     }
     return hipGetLastError();
@@ -1643,41 +1653,50 @@ hipError_t launch_tick_desc(int B, int irtype, const TickArgs *d_descs, int k, u
 }
 
 template <int HT, int M>
-static void launch_group_m(bool lean, bool passes, const TickArgs *d, const int *st, int nobj, unsigned blocks,
-                           hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+static hipError_t launch_group_m(int roles, int km, const TickArgs *d, const int *st, int nobj, unsigned blocks,
+                                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const dim3 g(blocks), t(kTickThreads);
-    if (lean)
-        hipExtLaunchKernelGGL((k_group_tick<HT, true, M, false>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
-    else if (passes)
-        hipExtLaunchKernelGGL((k_group_tick<HT, true, M>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
-    else
-        hipExtLaunchKernelGGL((k_group_tick<HT, false, M>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
-}
-
-template <int HT>
-static hipError_t launch_group_t(int B, bool lean, bool passes, const TickArgs *d, const int *st, int nobj,
-                                 unsigned blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
-{
-    switch (B) {
-    case 64: launch_group_m<HT, 64>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
-    case 128: launch_group_m<HT, 128>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
-    case 256: launch_group_m<HT, 256>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
-    case 512: launch_group_m<HT, 512>(lean, passes, d, st, nobj, blocks, s, ev0, ev1); break;
+    if (km != 4 && km != 8) return hipErrorInvalidValue;
+    switch (roles) {
+    case kRolesGeneral:
+        hipExtLaunchKernelGGL((k_group_tick<HT, M, kRolesGeneral, 4>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+        break;
+    case kRolesLean:
+        if (km == 4) hipExtLaunchKernelGGL((k_group_tick<HT, M, kRolesLean, 4>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+        else hipExtLaunchKernelGGL((k_group_tick<HT, M, kRolesLean, 8>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+        break;
+    case kRolesXfade:
+        if (km == 4) hipExtLaunchKernelGGL((k_group_tick<HT, M, kRolesXfade, 4>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+        else hipExtLaunchKernelGGL((k_group_tick<HT, M, kRolesXfade, 8>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipSuccess;
 }
 
-hipError_t launch_group_tick(int irtype, int B, bool lean, bool passes, const TickArgs *d_descs, const int *d_starts,
+template <int HT>
+static hipError_t launch_group_t(int B, int roles, int km, const TickArgs *d, const int *st, int nobj,
+                                 unsigned blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    switch (B) {
+    case 64: return launch_group_m<HT, 64>(roles, km, d, st, nobj, blocks, s, ev0, ev1);
+    case 128: return launch_group_m<HT, 128>(roles, km, d, st, nobj, blocks, s, ev0, ev1);
+    case 256: return launch_group_m<HT, 256>(roles, km, d, st, nobj, blocks, s, ev0, ev1);
+    case 512: return launch_group_m<HT, 512>(roles, km, d, st, nobj, blocks, s, ev0, ev1);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_group_tick(int irtype, int B, int roles, int km, const TickArgs *d_descs, const int *d_starts,
                              int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0) return hipSuccess;
     hipError_t e;
     switch (irtype) {
-    case IR_F32: e = launch_group_t<IR_F32>(B, lean, passes, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
-    case IR_F16: e = launch_group_t<IR_F16>(B, lean, passes, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
-    case IR_BF16: e = launch_group_t<IR_BF16>(B, lean, passes, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    case IR_F32: e = launch_group_t<IR_F32>(B, roles, km, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    case IR_F16: e = launch_group_t<IR_F16>(B, roles, km, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    case IR_BF16: e = launch_group_t<IR_BF16>(B, roles, km, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -1687,12 +1706,17 @@ hipError_t launch_group_tick(int irtype, int B, bool lean, bool passes, const Ti
 template <int HT>
 static hipError_t launch_heads_t(int B, const HeadBatchArgs &h, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
+    const dim3 g((unsigned)h.C), t(kTickThreads);
+#define TANX_HEADS(MM)                                                                            \
+    if (pass_km(h.kp) == 4) hipExtLaunchKernelGGL((k_heads<MM, HT, kHeadBatch + 3>), g, t, 0, s, ev0, ev1, 0, h); \
+    else hipExtLaunchKernelGGL((k_heads<MM, HT, kHeadBatch + 7>), g, t, 0, s, ev0, ev1, 0, h)
     switch (B) {
-    case 64: hipExtLaunchKernelGGL((k_heads<64, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
-    case 128: hipExtLaunchKernelGGL((k_heads<128, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
-    case 256: hipExtLaunchKernelGGL((k_heads<256, HT>), dim3(h.C), dim3(kTickThreads), 0, s, ev0, ev1, 0, h); break;
+    case 64: TANX_HEADS(64); break;
+    case 128: TANX_HEADS(128); break;
+    case 256: TANX_HEADS(256); break;
     default: return hipErrorInvalidValue;
     }
+#undef TANX_HEADS
     return hipGetLastError();
 }
 
@@ -1701,9 +1725,12 @@ static hipError_t launch_window_t(int B, int tail_nf, const HeadBatchArgs &h, co
                                   hipEvent_t ev0, hipEvent_t ev1)
 {
     const dim3 grid((unsigned)(h.C + a.C * a.nchunk));
-#define TANX_WINDOW(MM)                                                                                       \
-    if (tail_nf == 2) hipExtLaunchKernelGGL((k_window<MM, 2, HT>), grid, dim3(kTickThreads), 0, s, ev0, ev1, 0, h, a); \
-    else hipExtLaunchKernelGGL((k_window<MM, 1, HT>), grid, dim3(kTickThreads), 0, s, ev0, ev1, 0, h, a)
+#define TANX_WINDOW_K(MM, KK)                                                                                     \
+    if (tail_nf == 2) hipExtLaunchKernelGGL((k_window<MM, 2, HT, KK>), grid, dim3(kTickThreads), 0, s, ev0, ev1, 0, h, a); \
+    else hipExtLaunchKernelGGL((k_window<MM, 1, HT, KK>), grid, dim3(kTickThreads), 0, s, ev0, ev1, 0, h, a)
+#define TANX_WINDOW(MM)                         \
+    if (pass_km(a.kp) == 4) { TANX_WINDOW_K(MM, 4); } \
+    else { TANX_WINDOW_K(MM, 8); }
     switch (B) {
     case 64: TANX_WINDOW(64); break;
     case 128: TANX_WINDOW(128); break;
@@ -1711,13 +1738,15 @@ static hipError_t launch_window_t(int B, int tail_nf, const HeadBatchArgs &h, co
     default: return hipErrorInvalidValue;
     }
 #undef TANX_WINDOW
+#undef TANX_WINDOW_K
     return hipGetLastError();
 }
 
 hipError_t launch_window(int B, int tail_nf, const HeadBatchArgs &h, const TickArgs &a, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (h.n < 1 || h.n > kHeadBatch || B > kHeadBatchMaxB || a.nchunk < 1 || !is_pass(a) || a.C != h.C)
+    if (h.n < 1 || h.n > kHeadBatch || B > kHeadBatchMaxB || a.nchunk < 1 || !is_pass(a) || a.C != h.C ||
+        a.kp != h.kp || a.kn > pass_km(a.kp))
         return hipErrorInvalidValue;
     switch (a.irtype) {
     case IR_F32: return launch_window_t<IR_F32>(B, tail_nf, h, a, s, ev0, ev1);

@@ -197,7 +197,7 @@ class TANConvolution:
         blocks, modulo the multiple (the first frame ends at block
         multiple - 1 - phase), so that objects advanced together can spread
         their frame work over the ticks; pass_blocks: blocks whose partial
-        sums one tail pass computes for uniform objects (1..4, 0: chosen by
+        sums one tail pass computes for uniform objects (1..8, 0: chosen by
         the library, 1: a tail every block); pass_phase: uniform objects with
         multi-block passes run their passes in the launches of blocks b with
         (b + pass_phase) % pass_blocks == 0 (after block 0), so objects

@@ -245,18 +245,52 @@ def noalloc_group():
     print("noalloc group: outputs bit-identical", flush=True)
 
 
-def pass_phases(C=2, L=16384, B=128, nb=80):
-    """Objects with pass phases 0..3: alone with calls of uneven size and
-    dense updates, and together in group launches (one block each per
-    call); every channel against the oracle."""
+def batched_calls(kp=8, B=256, precision=0):
+    """Calls of several blocks with K-block passes (heads batched with the
+    next blocks' pass, head-only batches, catch-up passes after updates)
+    and dense updates between calls; against the oracle (fp32 spectra) or
+    against one block per call."""
+    calls = [1, 3, 7, 2, 13, 5, 1, 1, 9, 4, 6, 11, 2, 8]
+    L, C = 32 * B, 3
+    nb = sum(calls)
+    starts = np.cumsum([0] + calls[:-1])
+    ups = {int(starts[i]): T.signals.ir_bank(C, L, update=k, base_seed=601)
+           for k, i in enumerate((0, 2, 3, 5, 6, 7, 10, 12))}
+    x = T.signals.input_bank(C, nb * B, base_seed=700)
+    ctx = T.TANContext(device=0)
+    ys = []
+    for sizes in (calls, [1] * nb):
+        conv = T.TANConvolution(ctx, ir_precision=precision, pass_blocks=kp)
+        assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        xd = simlib.Dev(C, nb * B, x)
+        yd = simlib.Dev(C, nb * B, 0.0)
+        t = 0
+        for n in sizes:
+            if t in ups:
+                dev = simlib.Dev(C, L, ups[t])
+                assert lib.tanx_conv_update_td_device(conv.handle, dev.ptr, L, L, None, 0) == T.AMF_OK
+            process(conv, xd, yd, t, n, B)
+            t += n
+        ys.append(yd.a.copy())
+    err = float(np.abs(ys[0].astype(np.float64) - ys[1]).max())
+    print(f"K={kp} B={B}: calls vs one block per call {err:.3e}", flush=True)
+    assert err <= 1e-6
+    if precision == 0:
+        check(ys[0], oracle(L, B, x, ups, nb), f"K={kp} B={B} calls")
+
+
+def pass_phases(C=2, L=16384, B=128, nb=80, kp=4):
+    """Objects with pass length kp and phases 0..kp-1: alone with calls of
+    uneven size and dense updates, and together in group launches (one block
+    each per call); every channel against the oracle."""
     ctx = T.TANContext(device=0)
     x = T.signals.input_bank(C, nb * B, base_seed=41)
     ups = {0: 0, 9: 1, 10: 2, 23: 3, 24: 4, 25: 5, 50: 6}
     irs = {b: T.signals.ir_bank(C, L, k, base_seed=42) for b, k in ups.items()}
     yo = oracle(L, B, x, irs, nb)
     sizes = [1, 2, 5, 3, 7, 1, 4]
-    for phase in range(4):
-        conv = T.TANConvolution(ctx, pass_blocks=4, pass_phase=phase)
+    for phase in range(kp):
+        conv = T.TANConvolution(ctx, pass_blocks=kp, pass_phase=phase)
         assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
         xd = simlib.Dev(C, nb * B, x)
         yd = simlib.Dev(C, nb * B, 0.0)
@@ -271,10 +305,10 @@ def pass_phases(C=2, L=16384, B=128, nb=80):
             process(conv, xd, yd, t, n, B)
             t += n
         check(yd.a.copy(), yo, f"phase {phase} alone")
-    # four phased objects in one group launch per block
+    # kp phased objects in one group launch per block
     objs = []
-    for phase in range(4):
-        c = T.TANConvolution(ctx, pass_blocks=4, pass_phase=phase)
+    for phase in range(kp):
+        c = T.TANConvolution(ctx, pass_blocks=kp, pass_phase=phase)
         assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
         objs.append(c)
     xd = simlib.Dev(C, nb * B, x)
@@ -283,7 +317,7 @@ def pass_phases(C=2, L=16384, B=128, nb=80):
     assert ctx.EnableGroupTiming(True) == T.AMF_OK
     P = L // B
     pass_bytes = C * (P - 2) * B * 16          # FDL + IR rows of partitions 2..P-1
-    head_max = 4 * (C * B * 44 + 3 * C * B * 16)  # four heads with up to 3 further partitions
+    head_max = kp * (C * B * 44 + (kp - 1) * C * B * 16)  # kp heads with up to kp - 1 further partitions
     for t in range(nb):
         if t in irs:
             dev = simlib.Dev(C, L, irs[t])
@@ -302,9 +336,9 @@ def pass_phases(C=2, L=16384, B=128, nb=80):
         check(y.a.copy(), yo, f"phase {phase} in group")
 
 
-def rt_rig(groups=8, C=16, L=65536, B=256, nticks=130):
-    """The benchmark's real-time rig at a small size: groups with pass phases
-    g % 4, each group's IR update at its own offset in the 50-block period
+def rt_rig(groups=8, C=16, L=65536, B=256, nticks=130, kp=4):
+    """The benchmark's real-time rig at a small size: groups with pass length
+    kp and phases g % kp, each group's IR update at its own offset in the 50-block period
     (transform on the process stream before that tick's launch), one group
     launch per tick with group timing on; two groups against the oracle."""
     ctx = T.TANContext(device=0)
@@ -312,7 +346,7 @@ def rt_rig(groups=8, C=16, L=65536, B=256, nticks=130):
     banks = [simlib.Dev(C, L, T.signals.ir_bank(C, L, i, base_seed=5000)) for i in range(4)]
     objs = []
     for g in range(groups):
-        c = T.TANConvolution(ctx, pass_blocks=4, pass_phase=g % 4)
+        c = T.TANConvolution(ctx, pass_blocks=kp, pass_phase=g % kp)
         assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
         assert lib.tanx_conv_update_td_device(c.handle, banks[g % 4].ptr, L, L, None, 0) == T.AMF_OK
         objs.append(c)
@@ -344,5 +378,6 @@ def rt_rig(groups=8, C=16, L=65536, B=256, nticks=130):
 
 
 if __name__ == "__main__":
-    globals()[sys.argv[1]]()
+    # python scenarios.py NAME [key=int ...]
+    globals()[sys.argv[1]](**{k: int(v) for k, v in (a.split("=") for a in sys.argv[2:])})
     print("scenario complete", flush=True)

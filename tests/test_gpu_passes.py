@@ -101,7 +101,7 @@ def _updates(C, L, blocks, seed=600):
 DENSE = [0, 7, 8, 20, 21, 22, 40, 44, 45, 61, 62, 63, 64, 90]
 
 
-@pytest.mark.parametrize("K", [1, 2, 3, 4])
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 8])
 def test_dense_updates_every_pass_length(gpu_ctx, K):
     L, B, C, nb = 8192, 256, 3, 110
     ups = _updates(C, L, DENSE)
@@ -149,7 +149,7 @@ def test_default_pass_length(gpu_ctx):
     assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, 65536, 256, 2) == T.AMF_OK
     assert conv.Info()["pass_blocks"] == 1
     conv.Terminate()
-    conv = T.TANConvolution(gpu_ctx, pass_blocks=5)
+    conv = T.TANConvolution(gpu_ctx, pass_blocks=9)
     assert conv.InitGpu(UNIFORM, 65536, 256, 2) == T.AMF_INVALID_ARG
 
 
@@ -181,7 +181,8 @@ CALLS = [1, 3, 7, 2, 13, 5, 1, 1, 9, 4, 6, 11, 2, 8]
 
 @pytest.mark.parametrize("K,B,precision", [(4, 256, T.TAN_IR_PRECISION_FP32), (3, 128, T.TAN_IR_PRECISION_FP32),
                                            (2, 512, T.TAN_IR_PRECISION_FP32), (4, 64, T.TAN_IR_PRECISION_FP32),
-                                           (4, 256, T.TAN_IR_PRECISION_FP16)])
+                                           (4, 256, T.TAN_IR_PRECISION_FP16), (8, 256, T.TAN_IR_PRECISION_FP32),
+                                           (6, 128, T.TAN_IR_PRECISION_FP32), (8, 64, T.TAN_IR_PRECISION_BF16)])
 def test_multi_block_calls_batch_heads(gpu_ctx, K, B, precision):
     """Calls of several blocks launch up to K heads together with the pass
     for the blocks after them (which reads the rows before the batch): the
@@ -281,11 +282,13 @@ def _oracle_run(L, B, x, irs, nb):
     return yo
 
 
-def test_pass_phases_alone_and_grouped(gpu_ctx):
-    """Pass phases 0..3 (tanx_conv_options.pass_phase): each object alone
-    with calls of uneven size and dense updates, and the four together in
-    one group launch per block, all equal to the oracle; in the group, a tick
-    with no switch in flight streams exactly one object's pass."""
+@pytest.mark.parametrize("K", [4, 8])
+def test_pass_phases_alone_and_grouped(gpu_ctx, K):
+    """Pass phases 0..K-1 (tanx_conv_options.pass_phase) of K-block passes:
+    each object alone with calls of uneven size and dense updates, and the K
+    together in one group launch per block, all equal to the oracle; in the
+    group, a tick with no switch in flight streams exactly one object's
+    pass."""
     import torch
     L, B, C, nb = 16384, 128, 2, 80
     x = T.signals.input_bank(C, nb * B, base_seed=41)
@@ -295,8 +298,8 @@ def test_pass_phases_alone_and_grouped(gpu_ctx):
     xd = torch.from_numpy(x).cuda()
     sizes = [1, 2, 5, 3, 7, 1, 4]
     cuts = sorted(set(list(irs) + [nb]))
-    for phase in range(4):
-        conv = T.TANConvolution(gpu_ctx, pass_blocks=4, pass_phase=phase)
+    for phase in range(K):
+        conv = T.TANConvolution(gpu_ctx, pass_blocks=K, pass_phase=phase)
         assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
         yd = torch.zeros_like(xd)
         t = k = 0
@@ -311,31 +314,31 @@ def test_pass_phases_alone_and_grouped(gpu_ctx):
         conv.Synchronize()
         conv.Terminate()
         err = np.abs(yd.cpu().numpy().astype(np.float64) - yo).max()
-        print(f"phase {phase} alone: max|gpu-oracle| = {err:.3e}", flush=True)
+        print(f"K={K} phase {phase} alone: max|gpu-oracle| = {err:.3e}", flush=True)
         assert err <= 1e-5
     ctx = T.TANContext(device=0)
     objs = []
-    for phase in range(4):
-        c = T.TANConvolution(ctx, pass_blocks=4, pass_phase=phase)
+    for phase in range(K):
+        c = T.TANConvolution(ctx, pass_blocks=K, pass_phase=phase)
         assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
         objs.append(c)
     ys = [torch.zeros_like(xd) for _ in objs]
     assert ctx.EnableGroupTiming(True) == T.AMF_OK
     pass_bytes = C * (L // B - 2) * B * 16
-    head_max = 4 * (C * B * 44 + 3 * C * B * 16)
+    head_max = K * (C * B * 44 + (K - 1) * C * B * 16)  # K heads, up to K - 1 further partitions
     for t in range(nb):
         if t in irs:
             h = torch.from_numpy(irs[t]).cuda()
             for c in objs:
                 assert c.UpdateResponseTD(h, L, None, BLOCKING) == T.AMF_OK
-        assert T.GroupProcess(objs, [xd[:, t * B:]] * 4, [y[:, t * B:] for y in ys]) == T.AMF_OK
+        assert T.GroupProcess(objs, [xd[:, t * B:]] * K, [y[:, t * B:] for y in ys]) == T.AMF_OK
         tb = ctx.ReadGroupTiming(reset=True)["tick_alg_bytes"]
         if 30 <= t < 50:
             assert pass_bytes <= tb <= pass_bytes + head_max, (t, tb)
     torch.cuda.synchronize()
     for phase, y in enumerate(ys):
         err = np.abs(y.cpu().numpy().astype(np.float64) - yo).max()
-        print(f"phase {phase} in group: max|gpu-oracle| = {err:.3e}", flush=True)
+        print(f"K={K} phase {phase} in group: max|gpu-oracle| = {err:.3e}", flush=True)
         assert err <= 1e-5
     for c in objs:
         c.Terminate()

@@ -20,12 +20,13 @@ def sim_built():
     subprocess.check_call(["make", "-s", "-C", SIM])
 
 
-def _run(name, guard="end", fill=None, timeout=600):
+def _run(name, guard="end", fill=None, timeout=600, **kw):
     env = dict(os.environ, SIM_GUARD=guard)
     if fill is not None:
         env["SIM_FILL"] = fill
-    p = subprocess.run([sys.executable, os.path.join(SIM, "scenarios.py"), name], env=env, capture_output=True,
-                       text=True, timeout=timeout)
+    args = [f"{k}={v}" for k, v in kw.items()]
+    p = subprocess.run([sys.executable, os.path.join(SIM, "scenarios.py"), name] + args, env=env,
+                       capture_output=True, text=True, timeout=timeout)
     print(p.stdout[-2000:], p.stderr[-2000:])
     return p
 
@@ -59,11 +60,19 @@ def test_noalloc_group(sim_built, guard, fill):
     assert p.returncode == 0, p.returncode
 
 
-def test_pass_phases(sim_built):
-    p = _run("pass_phases")
+@pytest.mark.parametrize("kp", [4, 8])
+def test_pass_phases(sim_built, kp):
+    p = _run("pass_phases", "start", "0xff", kp=kp)
     assert p.returncode == 0, p.returncode
 
 
-def test_rt_rig(sim_built):
-    p = _run("rt_rig", "start", "0xff")
+@pytest.mark.parametrize("kp", [4, 8])
+def test_rt_rig(sim_built, kp):
+    p = _run("rt_rig", "start", "0xff", kp=kp)
+    assert p.returncode == 0, p.returncode
+
+
+@pytest.mark.parametrize("kp,B,precision", [(8, 256, 0), (6, 128, 0), (8, 64, 2)])
+def test_batched_calls(sim_built, kp, B, precision):
+    p = _run("batched_calls", "start", "0xff", kp=kp, B=B, precision=precision)
     assert p.returncode == 0, p.returncode
