@@ -819,19 +819,106 @@ def profile_run(T, torch, ngroups):
     log("profile run done")
 
 
-def pmc_traffic_per_channel(kernel_prefix="void tanx::k_group_tick<0, true, 256, false>"):
-    """HBM bytes per channel and launch of the real-time group kernel from the
-    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a
-    512 MiB copy), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_k_group_tick_rt.json")
+PROFILE_CMDS = {
+    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d <repo>/gpurun_out/prof_rt -o run -- "
+             "python3 <repo>/bench.py --profile-run --rt-groups {g}",
+    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <repo>/gpurun_out/pmc_fetch -o run -- "
+             "python3 <repo>/bench.py --profile-run --rt-groups {g}",
+    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <repo>/gpurun_out/pmc_write -o run -- "
+             "python3 <repo>/bench.py --profile-run --rt-groups {g}",
+}
+PROFILE_TICKS = 200
+PROFILE_GROUPS = 60    # groups of the profiled rig runs (about the real-time count)
+PMC_JSON = os.path.join(ROOT, "profiles", "pmc_traffic_k_group_tick_rt.json")
+
+
+def pmc_traffic_per_channel():
+    """HBM bytes per channel and tick of the real-time group launches (every
+    k_group_tick of a tick) from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None when it was
+    taken with another pass length."""
     try:
-        data = json.load(open(path))
+        data = json.load(open(PMC_JSON))
     except (OSError, ValueError):
         return None
-    for k in data.get("kernels", []):
-        if k["kernel"].startswith(kernel_prefix):
-            return float(k["traffic_bytes_per_channel"])
-    return None
+    if data.get("pass_blocks") != RT_PASS:
+        return None
+    return data.get("group_tick_traffic_bytes_per_channel")
+
+
+def summarize_profiles(run_dir, groups):
+    """Writes the committed summaries of the profiled rig runs (the commands
+    in PROFILE_CMDS, outputs under run_dir): the kernel trace statistics as
+    text and the PMC traffic as JSON, under profiles/."""
+    import csv
+    import collections
+    channels = groups * RT_GROUP
+    pre = "cd /tmp && export TMPDIR=/tmp && "
+
+    def counters(path, counter):
+        per = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r.get("Counter_Name") == counter:
+                per[r["Kernel_Name"]].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+        return per
+
+    f = counters(os.path.join(run_dir, "pmc_fetch", "run_counter_collection.csv"), "FETCH_SIZE")
+    w = counters(os.path.join(run_dir, "pmc_write", "run_counter_collection.csv"), "WRITE_SIZE")
+    # the 512 MiB copy's counters (its large dispatches) calibrate KiB per byte
+    cf = [v for g, v in f["__amd_rocclr_copyBuffer"] if g >= 131072]
+    cw = [v for g, v in w["__amd_rocclr_copyBuffer"] if g >= 131072]
+    kib = 512 * 1024
+    ff, wf = kib / (sum(cf) / len(cf)), kib / (sum(cw) / len(cw))
+    rows, tick_bytes = [], 0.0
+    for name in sorted(f, key=lambda k: -sum(v for _, v in f[k])):
+        if not name.startswith("void tanx::"):
+            continue
+        fv, wv = [v for _, v in f[name]], [v for _, v in w.get(name, [])]
+        fk, wk = sum(fv) / len(fv), (sum(wv) / len(wv) if wv else 0.0)
+        tb = fk * 1024 * ff + wk * 1024 * wf
+        rows.append({"kernel": name, "dispatches": len(fv), "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                     "traffic_bytes": round(tb)})
+        if name.startswith("void tanx::k_group_tick"):
+            tick_bytes += tb * len(fv)
+    per_tick = tick_bytes / PROFILE_TICKS
+    pmc = {"source": f"rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, of: python3 bench.py --profile-run "
+                     f"--rt-groups {groups}",
+           "commands": [pre + PROFILE_CMDS[k].format(g=groups) for k in ("fetch", "write")],
+           "workload": f"real-time rig: {groups} groups of 2,048 channels ({channels} channels), 65,536 taps, "
+                       f"B=256, {RT_PASS}-block passes, {PROFILE_TICKS} ticks with staggered passes, IR updates "
+                       "and crossfades",
+           "pass_blocks": RT_PASS, "channels": channels,
+           "calibration": {"kernel": "__amd_rocclr_copyBuffer (torch copy_ of 512 MiB)", "bytes_read": 1 << 29,
+                           "bytes_written": 1 << 29, "fetch_kib": sum(cf) / len(cf), "write_kib": sum(cw) / len(cw),
+                           "fetch_factor": ff, "write_factor": wf},
+           "kernels": rows,
+           "group_tick_traffic_bytes_per_tick": round(per_tick),
+           "group_tick_traffic_bytes_per_channel": per_tick / channels}
+    with open(PMC_JSON, "w") as fh:
+        json.dump(pmc, fh, indent=1)
+    stats = list(csv.DictReader(open(os.path.join(run_dir, "prof_rt", "run_kernel_stats.csv"))))
+    gt_ns = sum(float(r["TotalDurationNs"]) for r in stats if r["Name"].startswith("void tanx::k_group_tick"))
+    lines = [f"# Kernel trace: real-time rig, {groups} groups x 2,048 channels ({channels:,} channels), "
+             f"65,536 taps, B = 256, {RT_PASS}-block passes", "",
+             "Command (from the repository root on the GPU box):",
+             "  " + pre + PROFILE_CMDS["trace"].format(g=groups), "",
+             f"Workload: {PROFILE_TICKS} ticks of the benchmark's real-time rig (one group launch per tick and role "
+             f"set, one block in flight, pass phases g % {RT_PASS}, each group's IR update every 50 ticks, "
+             "staggered, with its crossfade); the first ticks include cold caches (the Max column).", "",
+             f"{'kernel':64s} {'calls':>6s} {'avg us':>10s} {'min us':>10s} {'max us':>10s} {'% time':>7s}"]
+    for r in stats:
+        lines.append(f"{r['Name'].split('(')[0].replace('void ', '')[:64]:64s} {int(r['Calls']):6d} "
+                     f"{float(r['AverageNs']) / 1e3:10.1f} {float(r['MinNs']) / 1e3:10.1f} "
+                     f"{float(r['MaxNs']) / 1e3:10.1f} {float(r['Percentage']):7.2f}")
+    lines += ["", f"Group kernels per tick: {gt_ns / PROFILE_TICKS / 1e3:.1f} us (all k_group_tick launches of "
+              f"the {PROFILE_TICKS} ticks / {PROFILE_TICKS}).",
+              f"HBM traffic of the group kernels ({os.path.relpath(PMC_JSON, ROOT)}): {per_tick / 1e9:.2f} GB per "
+              f"tick, {per_tick / channels / 1e3:.1f} KB per channel (FETCH_SIZE x {ff:.3f} + WRITE_SIZE x "
+              f"{wf:.3f}, the factors of a 512 MiB copy)."]
+    txt = os.path.join(ROOT, "profiles", f"k_group_tick_realtime_{channels}ch_65536taps_b256.txt")
+    with open(txt, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return txt
 
 
 def main():
@@ -859,6 +946,8 @@ def main():
                     help="profile run: real-time rig with this many groups of 2,048 channels")
     ap.add_argument("--rt-pass-blocks", type=int, default=RT_PASS,
                     help="blocks per tail pass of the real-time groups (1..8)")
+    ap.add_argument("--summarize-profiles", type=str, default="",
+                    help="write profiles/ summaries from the rocprofv3 outputs under this folder (no GPU)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
@@ -869,11 +958,14 @@ def main():
               "sessions shard across GPUs without exchange; see the design notes)", file=sys.stderr)
         sys.exit(2)
 
+    if args.summarize_profiles:
+        print(summarize_profiles(args.summarize_profiles, args.rt_groups or PROFILE_GROUPS))
+        return
     import torch
     import tan_amd as T
     T.load_library()
     if args.profile_run:
-        profile_run(T, torch, args.rt_groups or 24)
+        profile_run(T, torch, args.rt_groups or PROFILE_GROUPS)
         return
 
     log("real-time channel count (value)")
@@ -891,7 +983,7 @@ def main():
                           f"single-filter passes of one group in {RT_PASS}, the new filter's catch-up pass "
                           f"of each group whose IR switched) + k_group_tick<0, 256, 2, {RT_PASS}> when a "
                           "switch falls on a group's pass block (its two-filter pass)",
-                "trace": "profiles/k_group_tick_realtime_81920ch_65536taps_b256.txt",
+                "trace": f"profiles/k_group_tick_realtime_{PROFILE_GROUPS * RT_GROUP}ch_65536taps_b256.txt",
                 "traffic_source": "profiles/pmc_traffic_k_group_tick_rt.json" if per_ch else None,
                 "alg_bytes_per_launch": gt["tick_alg_bytes"] / launches, "avg_launch_us": tick_us,
                 "launches": gt["tick_launches"], "channels": value}
