@@ -2,22 +2,31 @@
 //
 // Source-compatible replacement for the convolution path of TAN's public
 // header (tan/tanlibrary/include/TrueAudioNext.h): the same namespace,
-// class names, enums, method names and argument order for TANContext
-// (:580-597), TANConvolution (:125-250) and TANFFT (:493-541), and the same
-// extern "C" factories (:642-681), returning AMF_RESULT codes with the
-// reference's numeric values (amf/public/include/core/Result.h:43-122).
+// class names, interface IDs, enums, method names and argument order for
+// TANContext (:580-597), TANConvolution (:125-250) and TANFFT (:493-541),
+// and the same extern "C" factories (:642-681), returning AMF_RESULT codes
+// with the reference's numeric values (amf/public/include/core/Result.h:43-122).
 //
 // Deliberate differences:
-//   * OpenCL is replaced by HIP.  Every cl_mem overload is an overload on
-//     tan_hip_mem (an opaque pointer type: cast a device float pointer to
-//     it); per-channel device arrays must be equally spaced in one
-//     allocation.  TANContext::InitOpenCL becomes InitHIP.
-//   * No AMF runtime: objects are reference counted (Acquire/Release) and
+//   * OpenCL is replaced by HIP, under the reference's method names.
+//     cl_mem becomes tan_hip_mem (an opaque pointer type: cast a device
+//     float pointer to it; per-channel device arrays must be equally spaced
+//     in one allocation), cl_command_queue becomes hipStream_t and
+//     cl_context becomes hipCtx_t (the context of the device to run on).
+//     InitOpenCL / GetOpenCLContext / GetOpenCLGeneralQueue /
+//     GetOpenCLConvQueue keep their names and argument order; InitHIP,
+//     InitHIPStreams and the GetHIP* getters are the same calls by device
+//     ordinal.
+//   * No AMF runtime: objects are reference counted (Acquire/Release),
+//     QueryInterface answers the reference's interface IDs, and
 //     TANInterfacePtr_T is the smart pointer; property storage is not
 //     provided.
 //   * The CPU engine is not part of this library: InitCpu returns
-//     AMF_NOT_SUPPORTED.  Methods TIME_DOMAIN and the non-uniform methods
-//     return AMF_NOT_SUPPORTED from Init.
+//     AMF_NOT_SUPPORTED.  Every method, TIME_DOMAIN and the non-uniform
+//     ones included, runs on the GPU.
+//   * ProcessDirect's device overload takes one device buffer per channel
+//     (the reference declares cl_mem* arrays and returns AMF_NOT_SUPPORTED,
+//     ConvolutionImpl.cpp:766-778) and runs TIME_DOMAIN objects.
 #pragma once
 
 #include <cstddef>
@@ -36,6 +45,8 @@
 
 typedef uint64_t amf_uint64;
 typedef uint32_t amf_uint32;
+typedef uint16_t amf_uint16;
+typedef uint8_t amf_uint8;
 typedef int64_t amf_int64;
 typedef int32_t amf_int32;
 typedef size_t amf_size;
@@ -68,6 +79,12 @@ typedef enum AMF_RESULT {
 // Opaque device-memory handle standing in for cl_mem: a HIP device pointer
 // to float data, cast to this type.
 typedef struct _tan_hip_mem *tan_hip_mem;
+
+// HIP's handle types (the same typedefs as hip_runtime_api.h, so this header
+// needs no HIP include): hipStream_t stands in for cl_command_queue,
+// hipCtx_t for cl_context.
+typedef struct ihipStream_t *hipStream_t;
+typedef struct ihipCtx_t *hipCtx_t;
 
 namespace amf {
 
@@ -108,12 +125,43 @@ enum TAN_FFT_TRANSFORM_DIRECTION {
     TAN_FFT_C2R_PLANAR_TRANSFORM_DIRECTION_BACKWARD = 5
 };
 
-// Reference-counted base (the AMFInterface contract: a new object starts at
-// count 1; Release at zero deletes it).
+// Interface ID (the layout of AMF's AMFGuid, Platform.h:387-446).
+struct AMFGuid {
+    amf_uint32 data1;
+    amf_uint16 data2, data3;
+    amf_uint8 data41, data42, data43, data44, data45, data46, data47, data48;
+    AMFGuid(amf_uint32 d1, amf_uint16 d2, amf_uint16 d3, amf_uint8 a, amf_uint8 b, amf_uint8 c, amf_uint8 d,
+            amf_uint8 e, amf_uint8 f, amf_uint8 g, amf_uint8 h)
+        : data1(d1), data2(d2), data3(d3), data41(a), data42(b), data43(c), data44(d), data45(e), data46(f),
+          data47(g), data48(h)
+    {
+    }
+    bool operator==(const AMFGuid &o) const
+    {
+        return data1 == o.data1 && data2 == o.data2 && data3 == o.data3 && data41 == o.data41 &&
+               data42 == o.data42 && data43 == o.data43 && data44 == o.data44 && data45 == o.data45 &&
+               data46 == o.data46 && data47 == o.data47 && data48 == o.data48;
+    }
+    bool operator!=(const AMFGuid &o) const { return !(*this == o); }
+};
+
+// static IID() of an interface, with the reference's value
+#define TAN_INTERFACE_ID(d1, d2, d3, a, b, c, d, e, f, g, h)          \
+    static const AMFGuid &IID()                                        \
+    {                                                                  \
+        static const AMFGuid id(d1, d2, d3, a, b, c, d, e, f, g, h);   \
+        return id;                                                     \
+    }
+
+// Reference-counted base (the AMFInterface contract, Interface.h:63-75: a
+// new object starts at count 1; Release at zero deletes it; QueryInterface
+// hands out the object, acquired, under its own IID or this one's).
 class TANInterface {
 public:
+    TAN_INTERFACE_ID(0x9d872f34, 0x90dc, 0x4b93, 0xb6, 0xb2, 0x6c, 0xa3, 0x7c, 0x85, 0x25, 0xdb)
     virtual amf_long AMF_STD_CALL Acquire() = 0;
     virtual amf_long AMF_STD_CALL Release() = 0;
+    virtual AMF_RESULT AMF_STD_CALL QueryInterface(const AMFGuid &interfaceID, void **ppInterface) = 0;
 
 protected:
     virtual ~TANInterface() {}
@@ -125,6 +173,14 @@ public:
     TANInterfacePtr_T() : p_(nullptr) {}
     TANInterfacePtr_T(T *p) : p_(p) { if (p_) p_->Acquire(); }
     TANInterfacePtr_T(const TANInterfacePtr_T &o) : p_(o.p_) { if (p_) p_->Acquire(); }
+    // from a pointer to another interface: QueryInterface for T (null when
+    // the object does not implement it)
+    template <class U>
+    explicit TANInterfacePtr_T(const TANInterfacePtr_T<U> &o) : p_(nullptr)
+    {
+        void *q = nullptr;
+        if (o.GetPtr() && o.GetPtr()->QueryInterface(T::IID(), &q) == AMF_OK) p_ = static_cast<T *>(q);
+    }
     ~TANInterfacePtr_T() { if (p_) p_->Release(); }
     TANInterfacePtr_T &operator=(const TANInterfacePtr_T &o)
     {
@@ -150,15 +206,29 @@ private:
 
 class TANContext : public TANInterface {
 public:
+    TAN_INTERFACE_ID(0x601a0b1e, 0x7659, 0x41f2, 0xb2, 0x6d, 0x6a, 0x16, 0x5f, 0x37, 0xa4, 0xa2)
     virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
-    // replaces InitOpenCL: bind a HIP device (default: device 0 at first use)
+    // InitOpenCL(cl_context) (TrueAudioNext.h:587-588): run on the device of
+    // a HIP context (the primary context of a device, hipCtxGetCurrent after
+    // hipSetDevice; nullptr: device 0), on streams of the context's own
+    virtual AMF_RESULT AMF_STD_CALL InitOpenCL(hipCtx_t pContext) = 0;
+    // InitOpenCL(general queue, convolution queue) (:589-591): run on the
+    // caller's streams (nullptr: a stream of the context's own), on the
+    // current device; they must outlive the context
+    virtual AMF_RESULT AMF_STD_CALL InitOpenCL(hipStream_t pGeneralQueue = nullptr,
+                                               hipStream_t pConvolutionQueue = nullptr) = 0;
+    // the same by device ordinal: InitHIP(device) as InitOpenCL(that
+    // device's context); InitHIPStreams(conv, general, device) as
+    // InitOpenCL(general, conv) on that device
     virtual AMF_RESULT AMF_STD_CALL InitHIP(int device = 0) = 0;
-    // replaces InitOpenCL(cl_command_queue convolutionQueue, cl_command_queue
-    // generalQueue): run on the caller's hipStream_t (nullptr: a stream of the
-    // context's own); they must outlive the context
     virtual AMF_RESULT AMF_STD_CALL InitHIPStreams(void *convStream, void *generalStream, int device = 0) = 0;
     // accepted for source compatibility; the GPU path has no OpenMP threads
     virtual AMF_RESULT AMF_STD_CALL InitOpenMP(int nThreads) = 0;
+    // GetOpenCLContext / GetOpenCLGeneralQueue / GetOpenCLConvQueue
+    // (:593-596): the device's primary context and the context's streams
+    virtual hipCtx_t AMF_STD_CALL GetOpenCLContext() = 0;
+    virtual hipStream_t AMF_STD_CALL GetOpenCLGeneralQueue() = 0;
+    virtual hipStream_t AMF_STD_CALL GetOpenCLConvQueue() = 0;
     virtual int AMF_STD_CALL GetHIPDevice() = 0;
     virtual void *AMF_STD_CALL GetHIPGeneralStream() = 0;  // hipStream_t
     virtual void *AMF_STD_CALL GetHIPConvStream() = 0;     // hipStream_t
@@ -167,6 +237,7 @@ typedef TANInterfacePtr_T<TANContext> TANContextPtr;
 
 class TANConvolution : public TANInterface {
 public:
+    TAN_INTERFACE_ID(0x99d46c18, 0x0c92, 0x40bd, 0x8a, 0xc7, 0x1c, 0x30, 0xf7, 0xe4, 0xeb, 0xcb)
     virtual AMF_RESULT AMF_STD_CALL Init(TAN_CONVOLUTION_METHOD convolutionMethod, amf_uint32 responseLengthInSamples,
                                          amf_uint32 bufferSizeInSamples, amf_uint32 channels) = 0;
     virtual AMF_RESULT AMF_STD_CALL InitCpu(TAN_CONVOLUTION_METHOD convolutionMethod,
@@ -199,6 +270,12 @@ public:
     virtual AMF_RESULT AMF_STD_CALL ProcessDirect(float *ppImpulseResponse[], float *ppBufferInput[],
                                                   float *ppBufferOutput[], amf_size numOfSamplesToProcess,
                                                   amf_size *pNumOfSamplesProcessed, int *nzFirstLast = nullptr) = 0;
+    // device buffers, one per channel (equally spaced in one allocation
+    // each); enqueued on the convolution stream, outputs written when it
+    // reaches them
+    virtual AMF_RESULT AMF_STD_CALL ProcessDirect(tan_hip_mem ppImpulseResponse[], tan_hip_mem ppBufferInput[],
+                                                  tan_hip_mem ppBufferOutput[], amf_size numOfSamplesToProcess,
+                                                  amf_size *pNumOfSamplesProcessed, int *nzFirstLast = nullptr) = 0;
     virtual AMF_RESULT AMF_STD_CALL ProcessFinalize(void) = 0;
     virtual AMF_RESULT AMF_STD_CALL GetNextFreeChannel(amf_uint32 *pChannelIndex, const amf_uint32 flagMasks[]) = 0;
 };
@@ -206,6 +283,7 @@ typedef TANInterfacePtr_T<TANConvolution> TANConvolutionPtr;
 
 class TANFFT : public TANInterface {
 public:
+    TAN_INTERFACE_ID(0x5f3965a1, 0x46c2, 0x4987, 0xb7, 0x1f, 0x3f, 0xf8, 0x1a, 0xe7, 0x5b, 0x6f)
     virtual AMF_RESULT AMF_STD_CALL Init() = 0;
     virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
     virtual TANContext *AMF_STD_CALL GetContext() = 0;
@@ -224,6 +302,7 @@ typedef TANInterfacePtr_T<TANFFT> TANFFTPtr;
 // on the GPU; host overloads stage through device memory.
 class TANConverter : public TANInterface {
 public:
+    TAN_INTERFACE_ID(0x2c3e6c65, 0x1788, 0x4bbf, 0xb0, 0x70, 0xa1, 0x35, 0x8c, 0xbe, 0xe8, 0x9e)
     virtual AMF_RESULT AMF_STD_CALL Init() = 0;
     virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
     virtual TANContext *AMF_STD_CALL GetContext() = 0;
@@ -254,6 +333,7 @@ typedef TANInterfacePtr_T<TANConverter> TANConverterPtr;
 // TANMixer (TrueAudioNext.h:611-635 of the reference).
 class TANMixer : public TANInterface {
 public:
+    TAN_INTERFACE_ID(0xd2432f7f, 0xc646, 0x4764, 0x9d, 0x61, 0x60, 0xd2, 0x7c, 0x7e, 0x20, 0xc7)
     virtual AMF_RESULT AMF_STD_CALL Init(amf_size buffer_size, int num_channels) = 0;
     virtual AMF_RESULT AMF_STD_CALL Terminate() = 0;
     virtual TANContext *AMF_STD_CALL GetContext() = 0;

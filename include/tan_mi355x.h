@@ -241,6 +241,19 @@ int32_t tanx_group_process_device_noalloc(uint32_t n, tanx_conv *const *convs, c
  * advances.  Host buffers; returns with the outputs written. */
 int32_t tanx_conv_process_direct(tanx_conv *conv, const float *const *ir, const float *const *in,
                                  float *const *out, size_t n, size_t *processed, const int32_t *nz_first_last);
+/* Device form of ProcessDirect: the reference declares a cl_mem overload
+ * (TrueAudioNext.h:235-242) whose body returns AMF_NOT_SUPPORTED
+ * (ConvolutionImpl.cpp:766-778); here it runs the time-domain kernel on
+ * device arrays.  Channel c's IR is ir + c*ir_stride (ir_len taps of
+ * tanx_conv_info, or only its nz_first_last range [first, last) is read),
+ * its input in + c*in_stride and its output out + c*out_stride (strides in
+ * floats; outputs of different channels must not overlap); min(n, ir_len)
+ * samples of every channel, enqueued on the context's process stream (the
+ * call returns before the outputs are written; tanx_conv_synchronize
+ * waits).  nz_first_last: host array of 2*C ints, or NULL. */
+int32_t tanx_conv_process_direct_device(tanx_conv *conv, const float *ir, size_t ir_stride, const float *in,
+                                        size_t in_stride, float *out, size_t out_stride, size_t n,
+                                        size_t *processed, const int32_t *nz_first_last);
 /* ProcessFinalize (ConvolutionImpl.cpp:1046-1079): enqueues the tail
  * multiply-accumulate for the next block.  Implicit after each Process unless
  * the object was created with the 0x8000 bit. */

@@ -75,6 +75,8 @@ _SIGS = {
     "tanx_dft_r2c_mfma_device": (_i32, [_vp, _u32, _u32, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _u32]),
     "tanx_conv_process_direct": (_i32, [_vp, _vp, _vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                         ctypes.c_void_p]),
+    "tanx_conv_process_direct_device": (_i32, [_vp, _vp, _sz, _vp, _sz, _vp, _sz, _sz, ctypes.POINTER(_sz),
+                                               ctypes.c_void_p]),
     "tanx_conv_flush": (_i32, [_vp, _u32]),
     "tanx_conv_get_next_free_channel": (_i32, [_vp, _u32p, _u32p]),
     "tanx_conv_synchronize": (_i32, [_vp]),

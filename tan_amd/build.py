@@ -53,10 +53,13 @@ def build_harness(verbose=False):
     out_dir = os.path.join(ROOT, "tests", "cpp", "build")
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "talib_harness")
-    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "oracle"),
+    # host C++ (g++) with HIP's runtime API for the caller-created streams
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROOT, "oracle"), "-I/opt/rocm/include",
            os.path.join(ROOT, "tests", "cpp", "talib_harness.cpp"), "-o", out,
            "-L" + LIBDIR, "-ltan_mi355x", "-L" + os.path.join(ROOT, "oracle", "build"), "-ltan_oracle",
-           "-Wl,-rpath,$ORIGIN/../../../tan_amd/lib:$ORIGIN/../../../oracle/build"]
+           "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../../tan_amd/lib:$ORIGIN/../../../oracle/build:/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

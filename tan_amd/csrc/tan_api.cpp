@@ -23,6 +23,17 @@ public:
         if (r == 0) delete this;
         return r;
     }
+    // the object under its interface's IID or the base interface's
+    // (AMFInterface::QueryInterface, Interface.h:70), acquired once more
+    AMF_RESULT AMF_STD_CALL QueryInterface(const AMFGuid &id, void **pp) override
+    {
+        if (!pp) return AMF_INVALID_POINTER;
+        *pp = nullptr;
+        if (id != Base::IID() && id != TANInterface::IID()) return AMF_NO_INTERFACE;
+        Acquire();
+        *pp = static_cast<Base *>(this);
+        return AMF_OK;
+    }
 
 protected:
     std::atomic<amf_long> refs_{0};
@@ -57,6 +68,57 @@ public:
         if (ctx_) return AMF_ALREADY_INITIALIZED;
         return (AMF_RESULT)tanx_ctx_create_on_streams(TAN_FULL_VERSION, device, convStream, generalStream, &ctx_);
     }
+    AMF_RESULT AMF_STD_CALL InitOpenCL(hipCtx_t pContext) override
+    {
+        int device = 0;
+        if (pContext) {
+            // the device of a HIP context: made current for the query
+            // (HIP's context calls are deprecated, not removed)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+            hipCtx_t prev = nullptr;
+            if (hipCtxPushCurrent(pContext) != hipSuccess) return AMF_INVALID_ARG;
+            const hipError_t e = hipCtxGetDevice(&device);
+            (void)hipCtxPopCurrent(&prev);
+#pragma clang diagnostic pop
+            if (e != hipSuccess) return AMF_INVALID_ARG;
+        }
+        return InitHIP(device);
+    }
+    AMF_RESULT AMF_STD_CALL InitOpenCL(hipStream_t general, hipStream_t conv) override
+    {
+        // the streams' device (the current one for the context's own streams)
+        int device = 0;
+        hipStream_t any = conv ? conv : general;
+        if (any) {
+            if (hipStreamGetDevice(any, &device) != hipSuccess) return AMF_INVALID_ARG;
+            int other = device;
+            if (conv && general && (hipStreamGetDevice(general, &other) != hipSuccess || other != device))
+                return AMF_INVALID_ARG;
+        } else if (hipGetDevice(&device) != hipSuccess) {
+            return AMF_NO_DEVICE;
+        }
+        return InitHIPStreams(conv, general, device);
+    }
+    hipCtx_t AMF_STD_CALL GetOpenCLContext() override
+    {
+        const int d = GetHIPDevice();
+        if (d < 0) return nullptr;
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+        hipCtx_t c = nullptr;
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+        if (hipSetDevice(d) == hipSuccess) (void)hipCtxGetCurrent(&c);
+#pragma clang diagnostic pop
+        (void)hipSetDevice(cur);
+        return c;
+    }
+    hipStream_t AMF_STD_CALL GetOpenCLGeneralQueue() override
+    {
+        return static_cast<hipStream_t>(GetHIPGeneralStream());
+    }
+    hipStream_t AMF_STD_CALL GetOpenCLConvQueue() override { return static_cast<hipStream_t>(GetHIPConvStream()); }
     AMF_RESULT AMF_STD_CALL InitOpenMP(int) override { return AMF_OK; }
     int AMF_STD_CALL GetHIPDevice() override
     {
@@ -238,6 +300,22 @@ public:
         if (!conv_) return AMF_NOT_INITIALIZED;
         size_t done = 0;
         const AMF_RESULT r = (AMF_RESULT)tanx_conv_process_direct(conv_, ir, in, out, n, &done, nzFirstLast);
+        if (processed) *processed = done;
+        return r;
+    }
+    AMF_RESULT AMF_STD_CALL ProcessDirect(tan_hip_mem ir[], tan_hip_mem in[], tan_hip_mem out[], amf_size n,
+                                          amf_size *processed, int *nzFirstLast) override
+    {
+        if (processed) *processed = 0;
+        if (!conv_) return AMF_NOT_INITIALIZED;
+        size_t sh = 0, si = 0, so = 0;
+        if (!uniform_stride(ir, channels_, &sh) || !uniform_stride(in, channels_, &si) ||
+            !uniform_stride(out, channels_, &so))
+            return AMF_INVALID_ARG;
+        size_t done = 0;
+        const AMF_RESULT r = (AMF_RESULT)tanx_conv_process_direct_device(
+            conv_, reinterpret_cast<const float *>(ir[0]), sh, reinterpret_cast<const float *>(in[0]), si,
+            reinterpret_cast<float *>(out[0]), so, n, &done, nzFirstLast);
         if (processed) *processed = done;
         return r;
     }

@@ -1718,6 +1718,60 @@ extern "C" int32_t tanx_conv_process_direct(tanx_conv *cv, const float *const *i
     return TANX_OK;
 }
 
+extern "C" int32_t tanx_conv_process_direct_device(tanx_conv *cv, const float *ir, size_t ir_stride, const float *in,
+                                                   size_t in_stride, float *out, size_t out_stride, size_t n,
+                                                   size_t *processed, const int32_t *nz_first_last)
+{
+    if (!cv) return TANX_NOT_INITIALIZED;
+    if (processed) *processed = 0;
+    if (!cv->td) return TANX_NOT_SUPPORTED;
+    if (!ir || !in || !out) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(cv->mtx);
+    DeviceGuard g(cv->ctx->device);
+    const size_t C = cv->C, len = cv->len, B = cv->B;
+    n = std::min(n, len);
+    if (C > 1 && out_stride < n) return TANX_INVALID_ARG;  // channels' outputs must not overlap
+    if (!cv->d_nz) {
+        TANX_CHECK(hipMalloc(&cv->d_nz, C * 2 * sizeof(int) + C));
+        cv->dbytes += C * 2 * sizeof(int) + C;
+    }
+    hipStream_t s = cv->ctx->proc;
+    std::vector<int> nz(2 * C);
+    for (size_t c = 0; c < C; c++) {
+        int f0 = 0, f1 = (int)len;
+        if (nz_first_last) {
+            f0 = std::max(0, std::min((int)len, (int)nz_first_last[2 * c]));
+            f1 = std::max(f0, std::min((int)len, (int)nz_first_last[2 * c + 1]));
+        }
+        nz[2 * c] = f0;
+        nz[2 * c + 1] = f1;
+    }
+    // stream-ordered after the previous call's kernels read the ranges
+    std::vector<uint8_t> ones(C, 1);
+    uint8_t *d_ones = reinterpret_cast<uint8_t *>(cv->d_nz + 2 * C);
+    TANX_CHECK(hipMemcpyAsync(cv->d_nz, nz.data(), 2 * C * sizeof(int), hipMemcpyHostToDevice, s));
+    TANX_CHECK(hipMemcpyAsync(d_ones, ones.data(), C, hipMemcpyHostToDevice, s));
+    for (size_t off = 0; off < n; off += B) {
+        const size_t m = std::min(B, n - off);
+        TdArgs t;
+        td_common(cv, t);
+        t.run = d_ones;
+        t.n = (int)m;
+        t.xin = in + off;
+        t.in_stride = (long long)in_stride;
+        t.yout = out + off;
+        t.out_stride = (long long)out_stride;
+        t.pos = cv->tdpos;
+        t.Ha = ir;
+        t.h_stride = (long long)ir_stride;
+        t.nz = cv->d_nz;
+        TANX_CHECK(launch_td(t, s));
+        cv->tdpos = (int)((cv->tdpos + m) & (len - 1));
+    }
+    if (processed) *processed = n;
+    return TANX_OK;
+}
+
 extern "C" int32_t tanx_conv_finalize(tanx_conv *cv)
 {
     if (!cv) return TANX_NOT_INITIALIZED;

@@ -1379,11 +1379,12 @@ __global__ __launch_bounds__(256) void k_td(TdArgs a)
     const float *xi = a.xin + (size_t)c * a.in_stride;
     for (int t = tid; t < len - 1; t += nt) w[t] = hist[(a.pos - (len - 1) + t) & mask];
     for (int j = tid; j < n; j += nt) w[len - 1 + j] = xi[j];
+    // the taps the sums use, [f0, f1), only: a caller's IR array may hold no more
     const float *Ha = a.Ha + (size_t)c * a.h_stride;
-    for (int k = tid; k < len; k += nt) ha[k] = Ha[k];
+    for (int k = f0 + tid; k < f1; k += nt) ha[k] = Ha[k];
     if (a.Hb) {
         const float *Hb = a.Hb + (size_t)c * a.h_stride;
-        for (int k = tid; k < len; k += nt) hb[k] = Hb[k];
+        for (int k = f0 + tid; k < f1; k += nt) hb[k] = Hb[k];
     }
     __syncthreads();
     float *yo = a.yout + (size_t)c * a.out_stride;

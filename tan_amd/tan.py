@@ -314,10 +314,31 @@ class TANConvolution:
 
     def ProcessDirect(self, impulse_responses, inputs, outputs, num_samples=None, nz_first_last=None):
         """ProcessDirect (ConvolutionImpl.cpp:719-762): TIME_DOMAIN objects
-        only.  Host arrays (channels, >= n); nz_first_last: None or (channels,
-        2) first / last+1 non-zero taps.  Returns (AMF_RESULT, processed)."""
+        only.  Host arrays (channels, >= n), or CUDA tensors (channels, >= n
+        samples; IRs (channels, ir_len)), which run on the context's process
+        stream; nz_first_last: None or (channels, 2) first / last+1 non-zero
+        taps.  Returns (AMF_RESULT, processed)."""
         if not self.handle:
             return AMF_NOT_INITIALIZED, 0
+        if _is_device_tensor(inputs):
+            if num_samples is None:
+                num_samples = inputs.shape[1]
+            n = min(num_samples, self.Info()["ir_len"])
+            views = ((impulse_responses, self.Info()["ir_len"]), (inputs, n), (outputs, n))
+            for t, need in views:
+                if (not _is_device_tensor(t) or t.dim() != 2 or t.shape[0] < self.channels or t.shape[1] < need
+                        or t.dtype.itemsize != 4 or t.stride(1) != 1):
+                    return AMF_INVALID_ARG, 0
+            nz = None
+            if nz_first_last is not None:
+                nz = np.ascontiguousarray(nz_first_last, dtype=np.int32).reshape(-1)
+            self._ctx.order_after_torch()
+            done = ctypes.c_size_t(0)
+            rc = self._lib.tanx_conv_process_direct_device(
+                self.handle, impulse_responses.data_ptr(), impulse_responses.stride(0), inputs.data_ptr(),
+                inputs.stride(0), outputs.data_ptr(), outputs.stride(0), num_samples, ctypes.byref(done),
+                nz.ctypes.data if nz is not None else None)
+            return rc, done.value
         if num_samples is None:
             num_samples = inputs.shape[1] if isinstance(inputs, np.ndarray) else min(len(b) for b in inputs)
         pir, _k1 = _host_rows(impulse_responses, self.channels)

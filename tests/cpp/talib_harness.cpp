@@ -9,10 +9,16 @@
 // usage: talib_harness <dir with pulsetrainS_0p5s.wav, testresponse.wav, testresponse2.wav> [out.wav]
 //
 // Like the driver, it writes the convolution output with WriteWaveFileF
-// (tanx_wav_write, 16-bit) and checks the file read back.
+// (tanx_wav_write, 16-bit) and checks the file read back.  The context is
+// set up as the driver does (TALibTestConvolution.cpp:166-174: TANCreateContext,
+// then InitOpenCL with two queues of the caller's), with hipStream_t for
+// cl_command_queue; QueryInterface, the OpenCL-named getters and the device
+// ProcessDirect overload are checked too.
 #include "TrueAudioNext.h"
 #include "tan_mi355x.h"
 #include "tan_oracle.h"
+
+#include <hip/hip_runtime_api.h>
 
 #include <cmath>
 #include <cstdio>
@@ -59,18 +65,49 @@ int main(int argc, char **argv)
     const amf_uint32 NSamples = (amf_uint32)samples[0].size();
     const amf_uint32 NResSamples = (amf_uint32)resp1[0].size();
 
+    // the driver's sequence: a context, then InitOpenCL(general queue,
+    // convolution queue) with queues the caller created (they outlive it)
     TANContextPtr ctx;
     AMF_RESULT r = TANCreateContext(TAN_FULL_VERSION, &ctx);
     if (r != AMF_OK) return fail("TANCreateContext", r);
-    if ((r = ctx->InitHIP(0)) != AMF_OK) return fail("InitHIP", r);
-    // GetOpenCLConvQueue / GetOpenCLGeneralQueue: the context's streams
-    if (!ctx->GetHIPConvStream() || !ctx->GetHIPGeneralStream() ||
-        ctx->GetHIPConvStream() == ctx->GetHIPGeneralStream())
-        return fail("GetHIPConvStream/GetHIPGeneralStream", -1);
-    if (ctx->InitHIPStreams(nullptr, nullptr, 0) != AMF_ALREADY_INITIALIZED)
-        return fail("InitHIPStreams after InitHIP", -1);
+    hipStream_t general_q = nullptr, conv_q = nullptr;
+    if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&general_q) != hipSuccess ||
+        hipStreamCreate(&conv_q) != hipSuccess)
+        return fail("hipStreamCreate", -1);
+    if ((r = ctx->InitOpenCL(general_q, conv_q)) != AMF_OK) return fail("InitOpenCL(queues)", r);
+    if (ctx->GetOpenCLGeneralQueue() != general_q || ctx->GetOpenCLConvQueue() != conv_q)
+        return fail("GetOpenCLGeneralQueue/GetOpenCLConvQueue", -1);
+    if (ctx->InitOpenCL(nullptr, nullptr) != AMF_ALREADY_INITIALIZED || ctx->InitHIP(0) != AMF_ALREADY_INITIALIZED)
+        return fail("a second Init", -1);
+    // a context on the first one's device, from its HIP context
+    {
+        TANContextPtr ctx2;
+        if ((r = TANCreateContext(TAN_FULL_VERSION, &ctx2)) != AMF_OK) return fail("TANCreateContext", r);
+        hipCtx_t hc = ctx->GetOpenCLContext();
+        if (!hc) return fail("GetOpenCLContext", -1);
+        if ((r = ctx2->InitOpenCL(hc)) != AMF_OK || ctx2->GetHIPDevice() != 0) return fail("InitOpenCL(context)", r);
+        if (!ctx2->GetOpenCLConvQueue() || ctx2->GetOpenCLConvQueue() == ctx2->GetOpenCLGeneralQueue())
+            return fail("the context's own streams", -1);
+    }
     TANConvolutionPtr conv;
     if ((r = TANCreateConvolution(ctx, &conv)) != AMF_OK) return fail("TANCreateConvolution", r);
+    // QueryInterface: the object under its IID and the base IID, acquired;
+    // no other interface; the smart pointer's conversion uses it
+    {
+        void *q = nullptr;
+        if (conv->QueryInterface(TANConvolution::IID(), &q) != AMF_OK || q != conv.GetPtr())
+            return fail("QueryInterface(TANConvolution)", -1);
+        static_cast<TANConvolution *>(q)->Release();
+        if (conv->QueryInterface(TANInterface::IID(), &q) != AMF_OK || q == nullptr)
+            return fail("QueryInterface(base)", -1);
+        static_cast<TANConvolution *>(q)->Release();
+        if (conv->QueryInterface(TANFFT::IID(), &q) != AMF_NO_INTERFACE || q != nullptr)
+            return fail("QueryInterface(TANFFT) on a convolution", -1);
+        TANInterfacePtr_T<TANInterface> base(conv.GetPtr());
+        TANConvolutionPtr back(base);
+        TANFFTPtr none(base);
+        if (back.GetPtr() != conv.GetPtr() || none.GetPtr() != nullptr) return fail("TANInterfacePtr_T conversion", -1);
+    }
     // the driver's GPU-UN method (TALibTestConvolution.cpp:78-85)
     if ((r = conv->InitGpu(TAN_CONVOLUTION_METHOD_FHT_UNIFORM_PARTITIONED, NResSamples, n_samples, nch)) != AMF_OK)
         return fail("InitGpu", r);
@@ -196,10 +233,61 @@ int main(int argc, char **argv)
     std::printf("output chain: mix %s, convert %s, clipped %d\n", mono == mono_ref ? "bit-exact" : "DIFFERS",
                 pcm == pcm_ref ? "bit-exact" : "DIFFERS", (int)clipped);
 
+    // ProcessDirect, device overload: a TIME_DOMAIN object fed the same IRs,
+    // inputs and non-zero ranges as a twin on the host overload; the outputs
+    // must be the same bits, over two calls (the history carries over)
+    bool direct_ok = false;
+    {
+        const amf_uint32 tdL = 512, tdB = 128, tdC = 2, tdN = 300;
+        TANConvolutionPtr th, td;
+        if ((r = TANCreateConvolution(ctx, &th)) != AMF_OK || (r = TANCreateConvolution(ctx, &td)) != AMF_OK)
+            return fail("TANCreateConvolution (time domain)", r);
+        if ((r = th->InitGpu(TAN_CONVOLUTION_METHOD_TIME_DOMAIN, tdL, tdB, tdC)) != AMF_OK ||
+            (r = td->InitGpu(TAN_CONVOLUTION_METHOD_TIME_DOMAIN, tdL, tdB, tdC)) != AMF_OK)
+            return fail("InitGpu (time domain)", r);
+        std::vector<float> hir(tdC * tdL), hin(tdC * 2 * tdN), hout(tdC * 2 * tdN, 0.f), dres(tdC * 2 * tdN, 0.f);
+        for (amf_uint32 i = 0; i < tdC * tdL; i++) hir[i] = resp1[i / tdL][i % tdL];
+        for (amf_uint32 i = 0; i < tdC * 2 * tdN; i++) hin[i] = samples[i / (2 * tdN)][3000 + i % (2 * tdN)];
+        int nz[] = {0, (int)tdL, 17, 400};
+        float *dmem = nullptr;
+        const size_t nir = tdC * tdL, nio = tdC * 2 * tdN;
+        if (hipMalloc(&dmem, (nir + 2 * nio) * sizeof(float)) != hipSuccess) return fail("hipMalloc", -1);
+        float *dir = dmem, *din = dmem + nir, *dout = din + nio;
+        if (hipMemcpy(dir, hir.data(), nir * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(din, hin.data(), nio * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail("hipMemcpy", -1);
+        for (amf_uint32 call = 0; call < 2; call++) {
+            float *hi[2], *ho[2], *hr[2];
+            tan_hip_mem di[2], dout_m[2], dr[2];
+            for (amf_uint32 c = 0; c < tdC; c++) {
+                hr[c] = hir.data() + c * tdL;
+                hi[c] = hin.data() + c * 2 * tdN + call * tdN;
+                ho[c] = hout.data() + c * 2 * tdN + call * tdN;
+                dr[c] = reinterpret_cast<tan_hip_mem>(dir + c * tdL);
+                di[c] = reinterpret_cast<tan_hip_mem>(din + c * 2 * tdN + call * tdN);
+                dout_m[c] = reinterpret_cast<tan_hip_mem>(dout + c * 2 * tdN + call * tdN);
+            }
+            amf_size dh = 0, dd = 0;
+            if ((r = th->ProcessDirect(hr, hi, ho, tdN, &dh, nz)) != AMF_OK) return fail("ProcessDirect (host)", r);
+            if ((r = td->ProcessDirect(dr, di, dout_m, tdN, &dd, nz)) != AMF_OK)
+                return fail("ProcessDirect (device)", r);
+            if (dh != tdN || dd != tdN) return fail("ProcessDirect count", (int)dd);
+        }
+        if (hipStreamSynchronize(conv_q) != hipSuccess ||
+            hipMemcpy(dres.data(), dout, nio * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail("hipMemcpy back", -1);
+        (void)hipFree(dmem);
+        float peak = 0.f;
+        for (float v : hout) peak = std::fmax(peak, std::fabs(v));
+        direct_ok = dres == hout && peak > 0.f;
+        std::printf("process direct: device overload %s the host overload's output\n",
+                    direct_ok ? "equals" : "DIFFERS from");
+    }
+
     orc_conv_destroy(orc);
     ctx->Terminate();  // while conv/fft are still alive, as the driver does
     const bool ok = maxerr <= 1e-5 && snr >= 120.0 && xfades == expected_xfades && fe <= 1e-4 && re <= 1e-5 && chain_ok &&
-                    wav_ok;
+                    wav_ok && direct_ok;
     std::printf("%s\n", ok ? "PASS" : "FAIL");
     return ok ? 0 : 1;
 }

@@ -79,10 +79,10 @@ enum {
     hipErrorNotSupported = 801,
     hipErrorUnknown = 999,
 };
-struct sim_stream;
+struct ihipStream_t;
 struct sim_event;
 struct sim_graph;
-typedef sim_stream *hipStream_t;
+typedef ihipStream_t *hipStream_t;
 typedef sim_event *hipEvent_t;
 typedef sim_graph *hipGraph_t;
 typedef sim_graph *hipGraphExec_t;
@@ -104,6 +104,13 @@ hipError_t hipGetDeviceCount(int *n);
 hipError_t hipGetDevice(int *d);
 hipError_t hipSetDevice(int d);
 hipError_t hipStreamGetDevice(hipStream_t s, hipDevice_t *d);
+// one device, one context
+struct ihipCtx_t;
+typedef struct ihipCtx_t *hipCtx_t;
+hipError_t hipCtxGetCurrent(hipCtx_t *c);
+hipError_t hipCtxPushCurrent(hipCtx_t c);
+hipError_t hipCtxPopCurrent(hipCtx_t *c);
+hipError_t hipCtxGetDevice(hipDevice_t *d);
 hipError_t hipMalloc(void **p, size_t bytes);
 template <class T>
 inline hipError_t hipMalloc(T **p, size_t bytes) { return hipMalloc(reinterpret_cast<void **>(p), bytes); }

@@ -310,7 +310,7 @@ struct Launch {
 
 }  // namespace
 
-struct sim_stream {
+struct ihipStream_t {
     int device = 0;
     bool capturing = false;
     sim_graph *graph = nullptr;
@@ -429,6 +429,26 @@ hipError_t hipStreamGetDevice(hipStream_t, hipDevice_t *d)
     *d = 0;
     return hipSuccess;
 }
+struct ihipCtx_t {
+    int device = 0;
+};
+static ihipCtx_t g_ctx;
+hipError_t hipCtxGetCurrent(hipCtx_t *c)
+{
+    *c = &g_ctx;
+    return hipSuccess;
+}
+hipError_t hipCtxPushCurrent(hipCtx_t c) { return c == &g_ctx ? hipSuccess : hipErrorInvalidValue; }
+hipError_t hipCtxPopCurrent(hipCtx_t *c)
+{
+    if (c) *c = &g_ctx;
+    return hipSuccess;
+}
+hipError_t hipCtxGetDevice(hipDevice_t *d)
+{
+    *d = 0;
+    return hipSuccess;
+}
 hipError_t hipMalloc(void **p, size_t bytes)
 {
     *p = guarded_alloc(bytes, false);
@@ -458,7 +478,7 @@ hipError_t hipMemsetAsync(void *dst, int v, size_t bytes, hipStream_t)
 }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
 {
-    *s = new sim_stream();
+    *s = new ihipStream_t();
     return hipSuccess;
 }
 hipError_t hipStreamDestroy(hipStream_t s)

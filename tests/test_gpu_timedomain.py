@@ -95,6 +95,40 @@ def test_process_direct_with_tap_ranges(gpu_ctx):
     u.Terminate()
 
 
+def test_process_direct_device_equals_host(gpu_ctx):
+    """ProcessDirect on device tensors (tanx_conv_process_direct_device)
+    gives the host overload's bits, call after call (the history carries
+    over), with tap ranges; the caller's IR rows are read in place."""
+    import torch
+    L, B, C = 1024, 256, 3
+    x = T.signals.input_bank(C, 3000, base_seed=53)
+    ir = T.signals.ir_bank(C, L, 0, base_seed=54)
+    nz = np.array([[10, 700], [0, 1024], [300, 301]], np.int32)
+    host = T.TANConvolution(gpu_ctx)
+    dev = T.TANConvolution(gpu_ctx)
+    assert host.InitGpu(TD, L, B, C) == T.AMF_OK and dev.InitGpu(TD, L, B, C) == T.AMF_OK
+    # IR rows with a stride wider than the IR, inputs and outputs as views
+    ird = torch.zeros((C, L + 64), dtype=torch.float32, device="cuda")
+    ird[:, :L] = torch.from_numpy(ir).cuda()
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    yh = np.zeros_like(x)
+    for off, n in ((0, 1000), (1000, 1024), (2024, 976)):
+        out = np.zeros((C, n), np.float32)
+        rc, done = host.ProcessDirect(ir, np.ascontiguousarray(x[:, off:off + n]), out, n, nz)
+        assert rc == T.AMF_OK and done == n
+        yh[:, off:off + n] = out
+        rc, done = dev.ProcessDirect(ird, xd[:, off:], yd[:, off:], n, nz)
+        assert rc == T.AMF_OK and done == n
+    assert dev.Synchronize() == T.AMF_OK
+    np.testing.assert_array_equal(yd.cpu().numpy(), yh)
+    # views shorter than the IR or the samples are refused before any launch
+    assert dev.ProcessDirect(ird[:, :L - 1], xd, yd, 100)[0] == T.AMF_INVALID_ARG
+    assert dev.ProcessDirect(ird, xd[:, :50], yd, 100)[0] == T.AMF_INVALID_ARG
+    host.Terminate()
+    dev.Terminate()
+
+
 def test_time_domain_flush_and_group_rejection(gpu_ctx):
     import torch
     L, B, C = 256, 64, 2
