@@ -342,7 +342,11 @@ void *tanx_conv_process_stream(tanx_conv *conv);
 /* TANFFT::Transform(direction, log2len, channels, float* in[], float* out[])
  * (fft/FFTImpl.cpp:507-575): each channel buffer holds 2^(log2len+1) floats;
  * complex data interleaved; backward transforms scaled by 1/N; planar
- * layouts keep im at N/2+8.  log2len in [1, 12] (planar: [4, 12]). */
+ * layouts keep im at N/2+8.  log2len in [1, 19], the reference's largest
+ * plan (FFTImpl.h:67; planar: [4, 19]).  Up to 2,048 complex points a
+ * channel is one workgroup's LDS transform; above, a four-step transform
+ * (column FFTs, FP64 twiddles, row FFTs) through scratch the context keeps
+ * (2 x 2^log2len complex per channel, allocated at the first such call). */
 int32_t tanx_fft_transform(tanx_ctx *ctx, uint32_t direction, uint32_t log2len,
                            uint32_t channels, const float *const *in, float *const *out);
 /* Device-memory variant (stands in for Transform(cl_mem[]) and

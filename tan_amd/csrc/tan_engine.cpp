@@ -131,6 +131,8 @@ struct tanx_ctx {
     std::mutex fft_mtx;
     float *fft_dev = nullptr;
     size_t fft_dev_floats = 0;
+    float2 *fft_scratch = nullptr;  // four-step transforms: 2 x M complex per channel
+    size_t fft_scratch_elems = 0;
     float *fft_host = nullptr;
     size_t fft_host_floats = 0;
     // matrix-core DFT tables per size: [n+2][n] bf16 high and low parts
@@ -247,6 +249,7 @@ static void ctx_free(tanx_ctx *ctx)
         }
         if (ctx->tw) (void)hipFree(ctx->tw);
         if (ctx->fft_dev) (void)hipFree(ctx->fft_dev);
+        if (ctx->fft_scratch) (void)hipFree(ctx->fft_scratch);
         if (ctx->fft_host) (void)hipHostFree(ctx->fft_host);
         if (ctx->g_host) (void)hipHostFree(ctx->g_host);
         if (ctx->g_dev) (void)hipFree(ctx->g_dev);
@@ -2476,25 +2479,45 @@ static int32_t fft_check(uint32_t dir, uint32_t log2len, uint32_t channels)
 {
     if (channels == 0 || log2len == 0) return TANX_INVALID_ARG;
     if (dir > TANX_FFT_C2R_PLANAR_BACKWARD) return TANX_INVALID_ARG;
-    if (dir <= TANX_FFT_BACKWARD && log2len > 11) return TANX_NOT_SUPPORTED;
-    if (dir >= TANX_FFT_R2C_FORWARD && log2len > 12) return TANX_NOT_SUPPORTED;
+    if (log2len > (uint32_t)kFftLog2Max) return TANX_NOT_SUPPORTED;
     // planar layout ends at N + 9 floats of a 2N-float channel buffer
     if (dir >= TANX_FFT_R2C_PLANAR_FORWARD && log2len < 4) return TANX_INVALID_ARG;
     return TANX_OK;
 }
 
-static FftArgs fft_args(tanx_ctx *ctx, uint32_t dir, uint32_t log2len, const float *in, float *out, size_t stride)
+// Launch arguments of a transform; sizes above one workgroup's LDS get the
+// four-step scratch (grown as needed, kept by the context; the caller holds
+// fft_mtx) and the tables of its column and row sizes.
+static int32_t fft_args(tanx_ctx *ctx, uint32_t dir, uint32_t log2len, uint32_t channels, const float *in,
+                        float *out, size_t stride, FftArgs &a)
 {
-    FftArgs a{};
+    a = FftArgs{};
     a.in = in;
     a.out = out;
     a.stride = (long long)stride;
     a.mode = (int)dir;
     a.log2len = (int)log2len;
     const int lM = dir >= TANX_FFT_R2C_FORWARD ? (int)log2len - 1 : (int)log2len;
-    a.twM = tw_table(ctx->tw, lM);
-    a.twN = tw_table(ctx->tw, lM + 1);
-    return a;
+    if ((1 << lM) <= kFftLdsMax) {
+        a.twM = tw_table(ctx->tw, lM);
+        a.twN = tw_table(ctx->tw, lM + 1);
+        return TANX_OK;
+    }
+    const size_t need = size_t(2) * channels * (size_t(1) << lM);
+    if (ctx->fft_scratch_elems < need) {
+        // the stream may still run a transform that uses the old scratch
+        TANX_CHECK(hipStreamSynchronize(ctx->upd));
+        if (ctx->fft_scratch) (void)hipFree(ctx->fft_scratch);
+        ctx->fft_scratch = nullptr;
+        ctx->fft_scratch_elems = 0;
+        TANX_CHECK(hipMalloc(&ctx->fft_scratch, need * sizeof(float2)));
+        ctx->fft_scratch_elems = need;
+    }
+    a.scratch = ctx->fft_scratch;
+    const int l1 = lM / 2;
+    a.tw1 = tw_table(ctx->tw, l1);
+    a.tw2 = tw_table(ctx->tw, lM - l1);
+    return TANX_OK;
 }
 
 extern "C" int32_t tanx_fft_transform_device(tanx_ctx *ctx, uint32_t direction, uint32_t log2len,
@@ -2506,7 +2529,10 @@ extern "C" int32_t tanx_fft_transform_device(tanx_ctx *ctx, uint32_t direction, 
     if (!d_in || !d_out || stride < (size_t(2) << log2len)) return TANX_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->fft_mtx);
     DeviceGuard g(ctx->device);
-    TANX_CHECK(launch_fft((int)channels, fft_args(ctx, direction, log2len, d_in, d_out, stride), ctx->upd));
+    FftArgs a;
+    rc = fft_args(ctx, direction, log2len, channels, d_in, d_out, stride, a);
+    if (rc != TANX_OK) return rc;
+    TANX_CHECK(launch_fft((int)channels, a, ctx->upd));
     TANX_CHECK(hipStreamSynchronize(ctx->upd));
     return TANX_OK;
 }
@@ -2558,8 +2584,11 @@ extern "C" int32_t tanx_fft_transform(tanx_ctx *ctx, uint32_t direction, uint32_
         std::memcpy(ctx->fft_host + c * stride, in[c], stride * sizeof(float));
     }
     hipStream_t s = ctx->upd;
+    FftArgs a;
+    rc = fft_args(ctx, direction, log2len, channels, ctx->fft_dev, ctx->fft_dev, stride, a);
+    if (rc != TANX_OK) return rc;
     TANX_CHECK(hipMemcpyAsync(ctx->fft_dev, ctx->fft_host, need * sizeof(float), hipMemcpyHostToDevice, s));
-    TANX_CHECK(launch_fft((int)channels, fft_args(ctx, direction, log2len, ctx->fft_dev, ctx->fft_dev, stride), s));
+    TANX_CHECK(launch_fft((int)channels, a, s));
     TANX_CHECK(hipMemcpyAsync(ctx->fft_host, ctx->fft_dev, need * sizeof(float), hipMemcpyDeviceToHost, s));
     TANX_CHECK(hipStreamSynchronize(s));
     size_t r[2][2];

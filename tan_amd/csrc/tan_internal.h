@@ -212,7 +212,14 @@ struct FftArgs {
     int log2len;
     const float2 *twM;     // W_M table for the complex FFT size M
     const float2 *twN;     // W_2M table (real transforms)
+    // complex sizes M > kFftLdsMax: the four-step transform (M = M1 * M2,
+    // columns of M1 points, then rows of M2 points) through scratch of
+    // 2 * M complex per channel, with the W_M1 and W_M2 tables
+    float2 *scratch;
+    const float2 *tw1, *tw2;
 };
+constexpr int kFftLdsMax = 2048;   // complex points one workgroup transforms in LDS
+constexpr int kFftLog2Max = 19;    // largest TANFFT log2len (complex and real)
 
 // Launchers (return hipError_t).  B is the block size (power of two).
 // ev0/ev1 (optional): events stamped with the kernel's own start and end

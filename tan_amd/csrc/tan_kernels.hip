@@ -1357,6 +1357,146 @@ __global__ __launch_bounds__(M / 4 > 64 ? (M / 4 < 1024 ? M / 4 : 1024) : 64) vo
 }
 
 // ---------------------------------------------------------------------------
+// TANFFT above kFftLdsMax complex points: the four-step transform.  With
+// M = M1 * M2 and n = n1 * M2 + n2, k = k1 + M1 * k2:
+//   Y[k1][n2] = W_M^(n2 k1) * sum_n1 x[n1 M2 + n2] W_M1^(n1 k1)   (columns)
+//   X[k1 + M1 k2] = sum_n2 Y[k1][n2] W_M2^(n2 k2)                  (rows)
+// Each workgroup transforms T = 4096 / size columns or rows in LDS (64 KB),
+// so global reads and writes come in runs of T consecutive elements.  The
+// twiddles between the steps are generated in FP64 and rounded once.
+// ---------------------------------------------------------------------------
+struct Fft4Pass {
+    const float2 *in;
+    float2 *out;
+    long long in_stride, out_stride;  // complex elements between channels
+    int logM, logM1;                   // M and the column size M1 (M2 = M / M1)
+    int inv;
+    float scale;                       // row pass: applied to the outputs
+    const float2 *tw;                  // W_M1 (columns) or W_M2 (rows)
+};
+
+// exp(-+2 pi i r / 2^logn) rounded once to FP32 (sign + for the inverse)
+__device__ __forceinline__ float2 tw_fp64(unsigned long long r, int logn, bool inv)
+{
+    double s, c;
+    sincospi(-2.0 * (double)r / (double)(1ull << logn), &s, &c);
+    return make_float2((float)c, (float)(inv ? -s : s));
+}
+
+constexpr int kFft4Tile = 4096;  // complex elements a four-step workgroup holds
+
+template <int M1>
+__global__ __launch_bounds__(256) void k_fft4_cols(Fft4Pass a)
+{
+    constexpr int T = kFft4Tile / M1;  // columns per workgroup
+    __shared__ float2 s0[kFft4Tile], s1[kFft4Tile];
+    const int c = blockIdx.y, tid = threadIdx.x;
+    const long long M2 = 1ll << (a.logM - a.logM1);
+    const long long n20 = (long long)blockIdx.x * T;
+    const float2 *x = a.in + (size_t)c * a.in_stride;
+    // column t of the tile as LDS row t; runs of T consecutive n2 per n1
+    for (int e = tid; e < kFft4Tile; e += 256) {
+        const int t = e % T, n1 = e / T;
+        s0[t * M1 + n1] = x[(size_t)n1 * M2 + n20 + t];
+    }
+    __syncthreads();
+    const float2 *z = a.inv ? fft_lds_n<M1, true>(s0, s1, T, a.tw) : fft_lds_n<M1, false>(s0, s1, T, a.tw);
+    float2 *y = a.out + (size_t)c * a.out_stride;
+    const unsigned long long mask = (1ull << a.logM) - 1;
+    for (int e = tid; e < kFft4Tile; e += 256) {
+        const int t = e % T, k1 = e / T;
+        const long long n2 = n20 + t;
+        const float2 w = tw_fp64(((unsigned long long)n2 * (unsigned long long)k1) & mask, a.logM, a.inv != 0);
+        y[(size_t)k1 * M2 + n2] = cmul(z[t * M1 + k1], w);
+    }
+}
+
+template <int M2>
+__global__ __launch_bounds__(256) void k_fft4_rows(Fft4Pass a)
+{
+    constexpr int T = kFft4Tile / M2;  // rows per workgroup
+    __shared__ float2 s0[kFft4Tile], s1[kFft4Tile];
+    const int c = blockIdx.y, tid = threadIdx.x;
+    const long long M1 = 1ll << a.logM1;
+    const long long k10 = (long long)blockIdx.x * T;
+    const float2 *y = a.in + (size_t)c * a.in_stride + (size_t)k10 * M2;
+    for (int e = tid; e < kFft4Tile; e += 256) s0[e] = y[e];
+    __syncthreads();
+    const float2 *z = a.inv ? fft_lds_n<M2, true>(s0, s1, T, a.tw) : fft_lds_n<M2, false>(s0, s1, T, a.tw);
+    float2 *X = a.out + (size_t)c * a.out_stride;
+    for (int e = tid; e < kFft4Tile; e += 256) {
+        const int t = e % T, k2 = e / T;
+        const float2 v = z[t * M2 + k2];
+        X[(size_t)k2 * M1 + k10 + t] = make_float2(v.x * a.scale, v.y * a.scale);
+    }
+}
+
+// Real forward transforms of 2M points from Z = FFT_M(z) (z[n] = x[2n] +
+// i x[2n+1]): bins 0..M into the interleaved (mode 2) or planar (mode 4,
+// imaginary parts from M + 8) layout, as k_fft writes them.
+__global__ __launch_bounds__(256) void k_fft4_r2c_post(const float2 *Z, long long zs, float *out, long long os,
+                                                       int logM, int mode)
+{
+    const int c = blockIdx.y;
+    const long long M = 1ll << logM;
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k > M) return;
+    const float2 *z = Z + (size_t)c * zs;
+    float *o = out + (size_t)c * os;
+    float2 X;
+    if (k == 0 || k == M) {
+        const float2 z0 = z[0];
+        X = make_float2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.f);
+    } else {
+        // r2c_post with the W_2M twiddle generated in FP64
+        const float2 z0 = z[k], zc = z[M - k];
+        const float er = 0.5f * (z0.x + zc.x), ei = 0.5f * (z0.y - zc.y);
+        const float dr = z0.x - zc.x, di = z0.y + zc.y;
+        const float orr = 0.5f * di, oi = -0.5f * dr;
+        const float2 w = tw_fp64((unsigned long long)k, logM + 1, false);
+        X = make_float2(er + (w.x * orr - w.y * oi), ei + (w.x * oi + w.y * orr));
+    }
+    if (mode == 2) {
+        o[2 * k] = X.x;
+        o[2 * k + 1] = X.y;
+    } else {
+        o[k] = X.x;
+        o[M + 8 + k] = X.y;
+    }
+}
+
+// Real inverse transforms: the packed M-point input of c2r_pre from the
+// interleaved (mode 3) or planar (mode 5) spectrum, as k_fft reads it.
+__device__ __forceinline__ float2 c2r_bin(const float *in, long long M, long long k, int mode)
+{
+    if (k == 0) return mode == 3 ? make_float2(in[0], in[2 * M]) : make_float2(in[0], in[M]);
+    return mode == 3 ? make_float2(in[2 * k], in[2 * k + 1]) : make_float2(in[k], in[M + 8 + k]);
+}
+
+__global__ __launch_bounds__(256) void k_fft4_c2r_pre(const float *in, long long is, float2 *A, long long as,
+                                                      int logM, int mode)
+{
+    const int c = blockIdx.y;
+    const long long M = 1ll << logM;
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= M) return;
+    const float *x = in + (size_t)c * is;
+    const float2 a = c2r_bin(x, M, k, mode);
+    float2 r;
+    if (k == 0) {
+        r = make_float2(a.x + a.y, a.x - a.y);
+    } else {
+        const float2 b = c2r_bin(x, M, M - k, mode);
+        const float er = a.x + b.x, ei = a.y - b.y;
+        const float dr = a.x - b.x, di = a.y + b.y;
+        const float2 w = tw_fp64((unsigned long long)k, logM + 1, false);
+        const float orr = dr * w.x + di * w.y, oi = di * w.x - dr * w.y;
+        r = make_float2(er - oi, ei + orr);
+    }
+    A[(size_t)c * as + k] = r;
+}
+
+// ---------------------------------------------------------------------------
 // Time-domain convolution (see TdArgs), 256 threads, window and filters in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_td(TdArgs a)
@@ -1851,10 +1991,93 @@ hipError_t launch_group_frame(int irtype, const FrameArgs *d_descs, const int *d
     return hipGetLastError();
 }
 
+static hipError_t launch_fft4_pass(bool rows, int size, int channels, const Fft4Pass &p, hipStream_t s)
+{
+    const unsigned tiles = (unsigned)((1ll << p.logM) / kFft4Tile);
+    const dim3 g(tiles, (unsigned)channels), t(256);
+    switch (size) {
+#define TANX_FFT4_CASE(V)                                                  \
+    case V:                                                                \
+        if (rows) hipLaunchKernelGGL(k_fft4_rows<V>, g, t, 0, s, p);       \
+        else hipLaunchKernelGGL(k_fft4_cols<V>, g, t, 0, s, p);            \
+        break;
+        TANX_FFT4_CASE(64) TANX_FFT4_CASE(128) TANX_FFT4_CASE(256) TANX_FFT4_CASE(512) TANX_FFT4_CASE(1024)
+        TANX_FFT4_CASE(2048)
+#undef TANX_FFT4_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// The four-step transform of M complex points per channel (M > kFftLdsMax):
+// complex directions from/to the caller's buffers; real ones through the
+// r2c/c2r kernels.  Scratch: two arrays of M complex per channel.
+static hipError_t launch_fft4(int channels, const FftArgs &a, hipStream_t s)
+{
+    const bool real = a.mode >= 2;
+    const int logM = real ? a.log2len - 1 : a.log2len;
+    const long long M = 1ll << logM;
+    const int logM1 = logM / 2;  // column size <= row size
+    if (M <= kFftLdsMax || logM > kFftLog2Max || !a.scratch || !a.tw1 || !a.tw2) return hipErrorInvalidValue;
+    const long long cs = a.stride / 2;  // the caller's channel stride in complex elements
+    float2 *A = a.scratch, *Bs = a.scratch + (size_t)channels * M;
+    const bool inv = a.mode == 1 || a.mode == 3 || a.mode == 5;
+    Fft4Pass cols{}, rows{};
+    cols.logM = rows.logM = logM;
+    cols.logM1 = rows.logM1 = logM1;
+    cols.inv = rows.inv = inv ? 1 : 0;
+    cols.tw = a.tw1;
+    rows.tw = a.tw2;
+    rows.scale = a.mode == 1 ? 1.0f / (float)M : (real && inv) ? 1.0f / (2.0f * (float)M) : 1.0f;
+    const unsigned bins = (unsigned)((M + 1 + 255) / 256);
+    hipError_t e;
+    if (!real) {
+        cols.in = reinterpret_cast<const float2 *>(a.in);
+        cols.in_stride = cs;
+        cols.out = A;
+        cols.out_stride = M;
+        rows.in = A;
+        rows.in_stride = M;
+        rows.out = reinterpret_cast<float2 *>(a.out);
+        rows.out_stride = cs;
+    } else if (!inv) {
+        cols.in = reinterpret_cast<const float2 *>(a.in);
+        cols.in_stride = cs;
+        cols.out = A;
+        cols.out_stride = M;
+        rows.in = A;
+        rows.in_stride = M;
+        rows.out = Bs;
+        rows.out_stride = M;
+    } else {
+        hipLaunchKernelGGL(k_fft4_c2r_pre, dim3(bins, (unsigned)channels), dim3(256), 0, s, a.in, a.stride, A, M,
+                           logM, a.mode);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        cols.in = A;
+        cols.in_stride = M;
+        cols.out = Bs;
+        cols.out_stride = M;
+        rows.in = Bs;
+        rows.in_stride = M;
+        rows.out = reinterpret_cast<float2 *>(a.out);
+        rows.out_stride = cs;
+    }
+    if ((e = launch_fft4_pass(false, 1 << logM1, channels, cols, s)) != hipSuccess) return e;
+    if ((e = launch_fft4_pass(true, 1 << (logM - logM1), channels, rows, s)) != hipSuccess) return e;
+    if (real && !inv) {
+        hipLaunchKernelGGL(k_fft4_r2c_post, dim3(bins, (unsigned)channels), dim3(256), 0, s, Bs, M, a.out, a.stride,
+                           logM, a.mode);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_fft(int channels, const FftArgs &a, hipStream_t s)
 {
     const bool real = a.mode >= 2;
     const int M = real ? (1 << (a.log2len - 1)) : (1 << a.log2len);
+    if (M > kFftLdsMax) return launch_fft4(channels, a, s);
     switch (M) {
 #define TANX_FFT_CASE(V) case V: hipLaunchKernelGGL(k_fft<V>, dim3(channels), dim3(fft_threads(V)), 0, s, a); break;
         TANX_FFT_CASE(1) TANX_FFT_CASE(2) TANX_FFT_CASE(4) TANX_FFT_CASE(8) TANX_FFT_CASE(16)

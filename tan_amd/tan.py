@@ -434,6 +434,12 @@ class TANFFT:
         if not self._init:
             return AMF_NOT_INITIALIZED
         if _is_device_tensor(inputs):
+            # one channel stride for both; every row holds the 2^(log2len+1) floats
+            n = 2 << log2len if 0 < log2len < 31 else 0
+            for t in (inputs, outputs):
+                if (not _is_device_tensor(t) or t.dim() != 2 or t.dtype.itemsize != 4 or t.stride(1) != 1
+                        or t.shape[0] < channels or t.shape[1] < n or t.stride(0) != inputs.stride(0)):
+                    return AMF_INVALID_ARG
             self._ctx.order_after_torch(update=True)
             return self._lib.tanx_fft_transform_device(self._ctx.handle, direction, log2len, channels,
                                                        inputs.data_ptr(), outputs.data_ptr(),

@@ -66,6 +66,12 @@ inline auto max(A a, B b) -> decltype(a + b)
 {
     return a < b ? b : a;
 }
+// the device library's FP64 sin/cos of pi * x
+inline void sincospi(double x, double *s, double *c)
+{
+    *s = std::sin(M_PI * x);
+    *c = std::cos(M_PI * x);
+}
 
 // ---- runtime types ----
 typedef int hipError_t;

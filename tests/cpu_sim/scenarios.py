@@ -279,6 +279,48 @@ def batched_calls(kp=8, B=256, precision=0):
         check(ys[0], oracle(L, B, x, ups, nb), f"K={kp} B={B} calls")
 
 
+def fft_large(log2n=13, C=2):
+    """TANFFT's four-step sizes, every direction, host buffers and device
+    buffers in place, against numpy's FP64 FFT."""
+    ctx = T.TANContext(device=0)
+    fft = T.TANFFT(ctx)
+    assert fft.Init() == T.AMF_OK
+    n, m = 1 << log2n, 1 << (log2n - 1)
+    rng = np.random.default_rng(log2n)
+    z = (rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64)
+    buf = np.zeros((C, 2 * n), np.float32)
+    buf[:, 0::2], buf[:, 1::2] = z.real, z.imag
+    out = np.zeros_like(buf)
+    assert fft.Transform(0, log2n, C, buf, out) == T.AMF_OK
+    ref = np.fft.fft(z.astype(np.complex128), axis=1)
+    err = np.abs(out[:, 0::2] + 1j * out[:, 1::2] - ref).max() / np.abs(ref).max()
+    print(f"complex 2^{log2n}: relative error {err:.2e}", flush=True)
+    assert err <= 3e-7 * log2n
+    back = np.zeros_like(buf)
+    assert fft.Transform(1, log2n, C, out, back) == T.AMF_OK
+    assert np.abs(back - buf).max() <= 1.5e-6 * log2n
+    # real directions: interleaved and planar, and back
+    x = rng.standard_normal((C, n)).astype(np.float32)
+    rb = np.zeros((C, 2 * n), np.float32)
+    rb[:, :n] = x
+    rref = np.fft.rfft(x.astype(np.float64), axis=1)
+    for fwd, inv in ((2, 3), (4, 5)):
+        o = np.full_like(rb, 9.0)
+        assert fft.Transform(fwd, log2n, C, rb, o) == T.AMF_OK
+        got = (o[:, 0:n + 2:2] + 1j * o[:, 1:n + 2:2]) if fwd == 2 else (o[:, :m + 1] + 1j * o[:, m + 8:2 * m + 9])
+        err = np.abs(got - rref).max() / np.abs(rref).max()
+        print(f"real 2^{log2n} mode {fwd}: relative error {err:.2e}", flush=True)
+        assert err <= 3e-7 * log2n
+        bk = np.zeros_like(rb)
+        assert fft.Transform(inv, log2n, C, o, bk) == T.AMF_OK
+        assert np.abs(bk[:, :n] - x).max() <= 1e-6 * log2n
+    # device buffer, in place, through the C ABI
+    dev = simlib.Dev(C, 2 * n, buf)
+    assert lib.tanx_fft_transform_device(ctx.handle, 0, log2n, C, dev.ptr, dev.ptr, dev.stride) == T.AMF_OK
+    assert np.array_equal(dev.a, out)
+    print("four-step transforms complete", flush=True)
+
+
 def pass_phases(C=2, L=16384, B=128, nb=80, kp=4):
     """Objects with pass length kp and phases 0..kp-1: alone with calls of
     uneven size and dense updates, and together in group launches (one block

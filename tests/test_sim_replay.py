@@ -76,3 +76,9 @@ def test_rt_rig(sim_built, kp):
 def test_batched_calls(sim_built, kp, B, precision):
     p = _run("batched_calls", "start", "0xff", kp=kp, B=B, precision=precision)
     assert p.returncode == 0, p.returncode
+
+
+@pytest.mark.parametrize("log2n", [12, 14, 19])
+def test_fft_four_step(sim_built, log2n):
+    p = _run("fft_large", "start", "0xff", log2n=log2n)
+    assert p.returncode == 0, p.returncode
