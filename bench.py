@@ -501,7 +501,8 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
     return rows
 
 
-def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precision=None, method=UNIFORM_METHOD):
+def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=96.0, seed=5, precision=None, method=UNIFORM_METHOD,
+                  pass_blocks=8):
     """Poisson session arrivals (20/s, mean life 30 s, 1-16 channels, IR
     log-uniform 8,192-524,288 taps, B = 256, IR update every U{10..100}
     blocks) served with HBM-budget admission and one group launch per tick.
@@ -525,7 +526,7 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
     noise = torch.empty(nx, rows_cap, B, device="cuda").uniform_(-1.0, 1.0, generator=g)
     torch.cuda.synchronize()
     srv = S.SessionServer(ctx, block=B, hbm_budget=int(budget_gib * (1 << 30)), ir_precision=prec,
-                          io_rows=rows_cap, method=method)
+                          io_rows=rows_cap, method=method, pass_blocks=pass_blocks)
     meta = {}
 
     def ir_for(sid, k, c, ln):
@@ -603,6 +604,9 @@ def bench_serving(T, torch, ctx, seconds=20.0, budget_gib=32.0, seed=5, precisio
             "block": B, "deadline_ms": period, "audio_seconds": ticks * B / FS, "ticks": ticks,
             "ir_precision": {0: "fp32", 1: "fp16", 2: "bf16"}[prec],
             "hbm_budget_GiB": budget_gib, "setup_seconds": setup_s,
+            "pass_blocks": pass_blocks if method == UNIFORM_METHOD else None,
+            "pass_phases": "balanced at admission" if method == UNIFORM_METHOD else None,
+            "ir_updates_per_tick_call": "one grouped call (tanx_group_update_td_device)",
             "sessions_mean": float(np.mean(sess)), "sessions_max": int(max(sess)),
             "channels_mean": float(np.mean(chans)), "channels_max": int(max(chans)),
             "admitted_during_run": admitted, "departed": departed, "ir_updates": updates,
@@ -948,6 +952,8 @@ def main():
                     help="blocks per tail pass of the real-time groups (1..8)")
     ap.add_argument("--summarize-profiles", type=str, default="",
                     help="write profiles/ summaries from the rocprofv3 outputs under this folder (no GPU)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run the serving legs alone and print their rows as one JSON line (no headline)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short fixed workload for rocprofv3 counter passes; prints no JSON")
     args = ap.parse_args()
@@ -966,6 +972,16 @@ def main():
     T.load_library()
     if args.profile_run:
         profile_run(T, torch, args.rt_groups or PROFILE_GROUPS)
+        return
+    if args.serving_only:
+        sctx = T.TANContext(device=0)
+        names = {"uniform": UNIFORM_METHOD, "nonuniform": NONUNIFORM_METHOD}
+        rows = []
+        for meth in [names[m] for m in args.serving_methods.split(",") if m]:
+            rows.append(bench_serving(T, torch, sctx, seconds=args.serving_seconds, method=meth))
+            log(f"  {rows[-1]}")
+        sctx.Terminate()
+        print(json.dumps({"serving": rows}))
         return
 
     log("real-time channel count (value)")

@@ -218,6 +218,19 @@ int32_t tanx_conv_process_device(tanx_conv *conv, const float *d_in, size_t in_s
 int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs, const float *const *d_in,
                                   const size_t *in_stride, float *const *d_out, const size_t *out_stride,
                                   const uint32_t *const *flags);
+/* IR updates of several objects of one context in one call (a serving
+ * tick's updates): object i is updated as tanx_conv_update_td_device(
+ * convs[i], d_irs[i], ir_stride[i], ir_len[i], flags ? flags[i] : NULL,
+ * opflags) would update it, in call order, but the IR transforms of all of
+ * them run in one launch per partition size and IR type.  No counterpart in
+ * the reference, whose updates are per object (ConvolutionImpl.cpp:407-530);
+ * for callers that update many objects per audio tick.  Time-domain objects
+ * are not accepted (TANX_NOT_SUPPORTED); an object may appear once.  With
+ * TANX_OPERATION_FLAG_BLOCK_UNTIL_READY the call returns once every
+ * transform has run. */
+int32_t tanx_group_update_td_device(uint32_t n, tanx_conv *const *convs, const float *const *d_irs,
+                                    const size_t *ir_stride, const size_t *ir_len,
+                                    const uint32_t *const *flags, uint32_t opflags);
 /* The same group step with no allocation on any call: no counterpart in the
  * reference.  For callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc): the launch

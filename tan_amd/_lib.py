@@ -77,6 +77,7 @@ _SIGS = {
                                         ctypes.c_void_p]),
     "tanx_conv_process_direct_device": (_i32, [_vp, _vp, _sz, _vp, _sz, _vp, _sz, _sz, ctypes.POINTER(_sz),
                                                ctypes.c_void_p]),
+    "tanx_group_update_td_device": (_i32, [_u32, _vp, _vp, _vp, _vp, _vp, _u32]),
     "tanx_conv_flush": (_i32, [_vp, _u32]),
     "tanx_conv_get_next_free_channel": (_i32, [_vp, _u32p, _u32p]),
     "tanx_conv_synchronize": (_i32, [_vp]),

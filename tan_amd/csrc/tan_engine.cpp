@@ -1789,8 +1789,18 @@ extern "C" int32_t tanx_conv_finalize(tanx_conv *cv)
 // that becomes the update slot after that crossfade, and handed off by the
 // first regular block after it, which is the order the reference produces
 // for a caller on another thread.
+// An IR transform of one object's update: partition size B, nchan listed
+// channels (a grouped update launches many in one kernel).
+struct IrJob {
+    IrArgs a;
+    int B, nchan;
+};
+
+// defer: where the update's IR transforms go instead of being launched
+// (the caller launches them before anything else on the process stream).
 static int32_t update_common(tanx_conv *cv, const float *const *hir, const float *dir, size_t dstride,
-                             size_t n, const uint32_t *flags, uint32_t opflags)
+                             size_t n, const uint32_t *flags, uint32_t opflags,
+                             std::vector<IrJob> *defer = nullptr)
 {
     // the process stream: blocks enqueued before the update finish reading
     // the target slot before it is written, and blocks after it see the
@@ -1881,7 +1891,8 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
         const int lB = __builtin_ctz((unsigned)cv->B);
         a.twM = tw_table(ctx->tw, lB);
         a.twN = tw_table(ctx->tw, lB + 1);
-        TANX_CHECK(launch_ir_transform(cv->B, nupd, a, s));
+        if (defer) defer->push_back(IrJob{a, cv->B, nupd});
+        else TANX_CHECK(launch_ir_transform(cv->B, nupd, a, s));
         if (cv->nu) {
             // taps [N, len): frame partitions of N
             IrArgs b = a;
@@ -1893,7 +1904,8 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
             const int lN = __builtin_ctz((unsigned)cv->N);
             b.twM = tw_table(ctx->tw, lN);
             b.twN = tw_table(ctx->tw, lN + 1);
-            TANX_CHECK(launch_ir_transform(cv->N, nupd, b, s));
+            if (defer) defer->push_back(IrJob{b, cv->N, nupd});
+            else TANX_CHECK(launch_ir_transform(cv->N, nupd, b, s));
         }
         for (int k = 0; k < nupd; k++) cv->staged[cv->hchan[k]] = 1;
     }
@@ -2159,6 +2171,79 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     if (rc != TANX_OK) return rc;
     st.dev = ctx->g_dev;
     return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, ctx->g_ev[slot]);
+}
+
+extern "C" int32_t tanx_group_update_td_device(uint32_t n, tanx_conv *const *convs, const float *const *d_irs,
+                                               const size_t *ir_stride, const size_t *ir_len,
+                                               const uint32_t *const *flags, uint32_t opflags)
+{
+    if (n == 0) return TANX_OK;
+    if (!convs || !d_irs || !ir_stride || !ir_len) return TANX_INVALID_ARG;
+    tanx_ctx *ctx = convs[0] ? convs[0]->ctx : nullptr;
+    if (!ctx) return TANX_NOT_INITIALIZED;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!convs[i]) return TANX_NOT_INITIALIZED;
+        if (convs[i]->ctx != ctx || convs[i]->td) return TANX_NOT_SUPPORTED;
+        if (!d_irs[i] || ir_stride[i] < ir_len[i]) return TANX_INVALID_ARG;
+    }
+    std::vector<tanx_conv *> order(convs, convs + n);
+    std::sort(order.begin(), order.end());
+    if (std::adjacent_find(order.begin(), order.end()) != order.end()) return TANX_INVALID_ARG;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(n);
+    for (tanx_conv *cv : order) locks.emplace_back(cv->mtx);
+    DeviceGuard g(ctx->device);
+    // each object's bookkeeping, slot choice and copies, in call order; its
+    // transforms are collected (a failure leaves the earlier objects'
+    // transforms to be launched below, as the group process call does)
+    const uint32_t opf = opflags & ~uint32_t(TANX_OPERATION_FLAG_BLOCK_UNTIL_READY);
+    std::vector<IrJob> jobs;
+    int32_t rc = TANX_OK;
+    for (uint32_t i = 0; i < n && rc == TANX_OK; i++)
+        rc = update_common(convs[i], nullptr, d_irs[i], ir_stride[i], ir_len[i], flags ? flags[i] : nullptr, opf,
+                           &jobs);
+    if (!jobs.empty()) {
+        // one launch per (partition size, IR type): descriptors and block
+        // offsets staged in one copy
+        std::stable_sort(jobs.begin(), jobs.end(), [](const IrJob &x, const IrJob &y) {
+            return x.B != y.B ? x.B < y.B : x.a.irtype < y.a.irtype;
+        });
+        const size_t nj = jobs.size(), db = align16(sizeof(IrArgs) * nj), sb = align16(sizeof(int) * nj);
+        char *host = nullptr;
+        int slot = 0;
+        const int32_t rs = group_staging(ctx, db + sb, &host, &slot);
+        if (rs != TANX_OK) return rs;
+        IrArgs *descs = reinterpret_cast<IrArgs *>(host);
+        int *starts = reinterpret_cast<int *>(host + db);
+        struct Launch {
+            size_t first, count;
+            int B, irtype;
+            long long blocks;
+        };
+        std::vector<Launch> launches;
+        for (size_t j = 0; j < nj; j++) {
+            const IrJob &jb = jobs[j];
+            if (launches.empty() || launches.back().B != jb.B || launches.back().irtype != jb.a.irtype)
+                launches.push_back(Launch{j, 0, jb.B, jb.a.irtype, 0});
+            Launch &l = launches.back();
+            descs[j] = jb.a;
+            starts[j] = (int)l.blocks;
+            l.blocks += ir_transform_blocks(jb.B, jb.a.irtype, jb.a.P, jb.nchan);
+            l.count++;
+        }
+        for (const Launch &l : launches)
+            if (l.blocks > INT32_MAX) return TANX_INVALID_ARG;
+        TANX_CHECK(hipMemcpyAsync(ctx->g_dev, host, db + sb, hipMemcpyHostToDevice, ctx->proc));
+        TANX_CHECK(hipEventRecord(ctx->g_ev[slot], ctx->proc));
+        for (const Launch &l : launches)
+            TANX_CHECK(launch_ir_transform_group(l.B, l.irtype, reinterpret_cast<const IrArgs *>(ctx->g_dev) + l.first,
+                                                 reinterpret_cast<const int *>(ctx->g_dev + db) + l.first,
+                                                 (int)l.count, (unsigned)l.blocks, ctx->proc));
+    }
+    if (rc == TANX_OK && (opflags & TANX_OPERATION_FLAG_BLOCK_UNTIL_READY))
+        TANX_CHECK(hipStreamSynchronize(ctx->proc));
+    return rc;
 }
 
 extern "C" int32_t tanx_ctx_enable_group_timing(tanx_ctx *ctx, uint32_t enable)

@@ -229,6 +229,13 @@ hipError_t launch_tick(int B, int tail_nf, const TickArgs &a, hipStream_t s, hip
 // object o with starts[o] <= b < starts[o + 1] and runs its roles from
 // d_descs[o].  Every object must use irtype, block size B (64, 128, 256 or
 // 512), role set `roles` (group_roles) and, for passes, pass_km(kp) == km.
+// IR transforms of several objects of partition size B and one IR type in
+// one launch: entry j (d_descs[j], its channel list of nchan_j entries) owns
+// blocks [d_starts[j], d_starts[j] + ir_transform_blocks(B, irtype, P_j,
+// nchan_j)).
+int ir_transform_blocks(int B, int irtype, int P, int nchan);
+hipError_t launch_ir_transform_group(int B, int irtype, const IrArgs *d_descs, const int *d_starts, int n,
+                                     unsigned blocks, hipStream_t s);
 hipError_t launch_group_tick(int irtype, int B, int roles, int km, const TickArgs *d_descs, const int *d_starts,
                              int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);

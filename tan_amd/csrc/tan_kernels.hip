@@ -1194,13 +1194,13 @@ __device__ __forceinline__ unsigned short f2bf_rne(float f)
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// Partition i of list entry y (channel a.chan[y], or y) of one IR.
 template <int M, int HT>
-__global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs a)
+__device__ __forceinline__ void ir_transform_part(const IrArgs &a, int i, int y, float2 *s0, float2 *s1,
+                                                  float *smax)
 {
-    __shared__ float2 s0[M], s1[M];
-    __shared__ float smax[32];
-    const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    const int ch = a.chan ? a.chan[blockIdx.y] : (int)blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ch = a.chan ? a.chan[y] : y;
     const float *h = a.ir + (size_t)ch * a.stride;
     const long long base = (long long)i * M;
     for (int n = tid; n < M; n += nt) {
@@ -1250,6 +1250,40 @@ __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs
     }
 }
 
+template <int M, int HT>
+__global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs a)
+{
+    __shared__ float2 s0[M], s1[M];
+    __shared__ float smax[32];
+    ir_transform_part<M, HT>(a, blockIdx.x, blockIdx.y, s0, s1, smax);
+}
+
+// Entry j of a descriptor list whose block offsets start at starts[j]
+__device__ __forceinline__ int group_entry(const int *__restrict__ starts, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Several IRs' transforms (one entry each: every partition of its listed
+// channels, P blocks per channel) in one launch.
+template <int M, int HT>
+__global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform_group(const IrArgs *__restrict__ descs,
+                                                                                const int *__restrict__ starts, int n)
+{
+    __shared__ float2 s0[M], s1[M];
+    __shared__ float smax[32];
+    const int b = blockIdx.x, j = group_entry(starts, n, b);
+    const IrArgs a = descs[j];
+    const int local = b - starts[j];
+    ir_transform_part<M, HT>(a, local % a.P, local / a.P, s0, s1, smax);
+}
+
 // FP32 IR spectra of NR partitions per workgroup (256 threads): the rows'
 // FFTs share the Stockham stages, so a transform of many short partitions
 // runs as a quarter (or half) as many workgroups.  The twiddles are staged
@@ -1257,11 +1291,11 @@ __global__ __launch_bounds__(M / 4 > 64 ? M / 4 : 64) void k_ir_transform(IrArgs
 // aligned rows load four samples per lane.  Same arithmetic per row as
 // k_ir_transform.
 template <int M, int NR>
-__global__ __launch_bounds__(256) void k_ir_transform_rows(IrArgs a)
+__device__ __forceinline__ void ir_transform_rows_part(const IrArgs &a, int xb, int y, float2 *s0, float2 *s1,
+                                                       float2 *tw, float2 *twn)
 {
-    __shared__ float2 s0[NR * M], s1[NR * M], tw[M], twn[M];
-    const int i0 = blockIdx.x * NR, tid = threadIdx.x, nt = blockDim.x;
-    const int ch = a.chan ? a.chan[blockIdx.y] : (int)blockIdx.y;
+    const int i0 = xb * NR, tid = threadIdx.x, nt = blockDim.x;
+    const int ch = a.chan ? a.chan[y] : y;
     const int nr = min(NR, a.P - i0);
     const float *h = a.ir + (size_t)ch * a.stride;
     const bool al = (reinterpret_cast<uintptr_t>(h) & 15) == 0;
@@ -1298,6 +1332,25 @@ __global__ __launch_bounds__(256) void k_ir_transform_rows(IrArgs a)
         const int r = e / M, k = e - r * M;
         rows[e] = r2c_post<M>(Z + r * M, k, twn);
     }
+}
+
+template <int M, int NR>
+__global__ __launch_bounds__(256) void k_ir_transform_rows(IrArgs a)
+{
+    __shared__ float2 s0[NR * M], s1[NR * M], tw[M], twn[M];
+    ir_transform_rows_part<M, NR>(a, blockIdx.x, blockIdx.y, s0, s1, tw, twn);
+}
+
+// Grouped form of k_ir_transform_rows: ceil(P / NR) blocks per listed channel.
+template <int M, int NR>
+__global__ __launch_bounds__(256) void k_ir_transform_rows_group(const IrArgs *__restrict__ descs,
+                                                                 const int *__restrict__ starts, int n)
+{
+    __shared__ float2 s0[NR * M], s1[NR * M], tw[M], twn[M];
+    const int b = blockIdx.x, j = group_entry(starts, n, b);
+    const IrArgs a = descs[j];
+    const int local = b - starts[j], chunks = (a.P + NR - 1) / NR;
+    ir_transform_rows_part<M, NR>(a, local % chunks, local / chunks, s0, s1, tw, twn);
 }
 
 // ---------------------------------------------------------------------------
@@ -1920,6 +1973,48 @@ hipError_t launch_heads(int B, int irtype, const HeadBatchArgs &h, hipStream_t s
     case 4096: { constexpr int M = 4096; CALL; } break;  \
     default: return hipErrorInvalidValue;                \
     }
+
+int ir_transform_blocks(int B, int irtype, int P, int nchan)
+{
+    if (irtype == IR_F32 && B <= 256) return (P + 3) / 4 * nchan;
+    if (irtype == IR_F32 && B == 512) return (P + 1) / 2 * nchan;
+    return P * nchan;
+}
+
+hipError_t launch_ir_transform_group(int B, int irtype, const IrArgs *d_descs, const int *d_starts, int n,
+                                     unsigned blocks, hipStream_t s)
+{
+    if (n == 0 || blocks == 0) return hipSuccess;
+    const dim3 g(blocks);
+    switch (irtype) {
+    case IR_F32:
+        if (B <= 256) {
+            switch (B) {
+            case 64: hipLaunchKernelGGL((k_ir_transform_rows_group<64, 4>), g, dim3(256), 0, s, d_descs, d_starts, n); break;
+            case 128: hipLaunchKernelGGL((k_ir_transform_rows_group<128, 4>), g, dim3(256), 0, s, d_descs, d_starts, n); break;
+            case 256: hipLaunchKernelGGL((k_ir_transform_rows_group<256, 4>), g, dim3(256), 0, s, d_descs, d_starts, n); break;
+            default: return hipErrorInvalidValue;
+            }
+        } else if (B == 512) {
+            hipLaunchKernelGGL((k_ir_transform_rows_group<512, 2>), g, dim3(256), 0, s, d_descs, d_starts, n);
+        } else {
+            TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform_group<M, IR_F32>), g, dim3(fft_threads(M)), 0, s,
+                                                   d_descs, d_starts, n));
+        }
+        break;
+    case IR_F16:
+        TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform_group<M, IR_F16>), g, dim3(fft_threads(M)), 0, s,
+                                               d_descs, d_starts, n));
+        break;
+    case IR_BF16:
+        TANX_SWITCH_PART(B, hipLaunchKernelGGL((k_ir_transform_group<M, IR_BF16>), g, dim3(fft_threads(M)), 0, s,
+                                               d_descs, d_starts, n));
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_ir_transform(int B, int nchan, const IrArgs &a, hipStream_t s)
 {

@@ -6,9 +6,11 @@ fit the HBM budget; until then it waits in a FIFO queue, and a waiting
 session that fits is admitted even when an older, larger one does not
 (first fit in arrival order).  Admitted sessions keep their FDL, overlap and
 IR slots resident in HBM (one TANConvolution object each); every audio tick,
-all admitted sessions advance one block in ONE group kernel launch, while IR
-transforms of new sessions and of IR updates run on the update stream and
-overlap the ticks.
+all admitted sessions advance one block in ONE group kernel launch.  IR
+transforms of new sessions and of the tick's IR updates run on the process
+stream ahead of the tick's launch, all of one tick in one launch per
+partition size (update_many).  Uniform sessions get pass phases that spread
+their multi-block passes evenly over the ticks.
 
 A session's output is the same as its object processed alone: objects keep
 independent state and the group kernel runs the same per-object block code
@@ -137,7 +139,7 @@ class RowAllocator:
 
 class Session:
     __slots__ = ("sid", "channels", "ir_len", "conv", "inp", "out", "row", "nbytes", "blocks", "arrived_tick",
-                 "admitted_tick", "updates", "multiple")
+                 "admitted_tick", "updates", "multiple", "pass_blocks", "pass_phase", "pass_bytes", "pass_slots")
 
     def __init__(self, sid, channels, ir_len):
         self.sid = sid
@@ -153,7 +155,34 @@ class Session:
         self.arrived_tick = 0
         self.admitted_tick = -1
         self.updates = 0
+        self.pass_blocks = 0   # the object's pass length K (tanx_conv_info.pass_blocks)
+        self.pass_phase = 0    # the pass phase it was created with
+        self.pass_bytes = 0    # bytes one of its passes streams
+        self.pass_slots = ()   # tick residues (mod PASS_CYCLE) its passes land on
 
+    def conv_options(self):
+        """TANConvolution keyword arguments that rebuild this session's object
+        (the same pass length and phase)."""
+        return {"pass_blocks": self.pass_blocks if self.pass_slots else 0, "pass_phase": self.pass_phase}
+
+
+PASS_CYCLE = 8  # common period of pass lengths 1, 2, 4 and 8
+
+
+def pass_residue(load, k):
+    """Tick residue r (mod k) for the passes of a new session of pass length
+    k: the one whose PASS_CYCLE // k residues mod PASS_CYCLE carry the least
+    pass bytes at their busiest (ties: the lowest r)."""
+    cost = [max(load[r + j * k] for j in range(PASS_CYCLE // k)) for r in range(k)]
+    return min(range(k), key=lambda i: (cost[i], i))
+
+
+def pass_bytes(ir_len, block, channels, ir_precision=T.TAN_IR_PRECISION_FP32):
+    """Bytes one multi-block pass of a uniform object streams: the FDL row and
+    the IR row of partitions 2..P-1 of every channel."""
+    parts = max(0, -(-ir_len // block) - 2)
+    ir_elem = 8 if ir_precision == T.TAN_IR_PRECISION_FP32 else 4
+    return channels * parts * block * (8 + ir_elem)
 
 class TickRecord:
     __slots__ = ("tick", "sessions", "channels", "latency_ms", "enqueue_ms", "alg_bytes")
@@ -177,11 +206,26 @@ class SessionServer:
     tick can bring in every session's input; a session that finds no free
     range gets tensors of its own."""
 
-    def __init__(self, ctx, block=256, hbm_budget=32 << 30, ir_precision=T.TAN_IR_PRECISION_FP32,
+    def __init__(self, ctx, block=256, hbm_budget=96 << 30, ir_precision=T.TAN_IR_PRECISION_FP32,
                  device="cuda", io_rows=0, method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED,
-                 nu_multiple=0):
+                 nu_multiple=0, pass_blocks=0, balance_passes=True, group_updates=True):
+        """pass_blocks: pass length K of uniform sessions (0: the library's
+        choice); balance_passes: give each uniform session the pass phase
+        that puts its passes on the tick residue (mod PASS_CYCLE) with the
+        fewest pass bytes so far, so that no tick streams every session's
+        rows (a session's output does not depend on its phase);
+        group_updates: the IR updates issued between two ticks (first IRs
+        of admitted sessions included) go to the device in one grouped call
+        before the next tick (GroupUpdateResponseTD), else one call each."""
         if block & (block - 1) or block < 64 or block > 512:
             raise ValueError("block must be a power of two in [64, 512]")
+        if pass_blocks not in (0, 1, 2, 4, 8):
+            raise ValueError("pass_blocks must be 0, 1, 2, 4 or 8")
+        self.pass_blocks = pass_blocks
+        self.balance_passes = balance_passes
+        self.group_updates = group_updates
+        self._updates = collections.OrderedDict()  # sid -> (session, IR tensor), issued before the next tick
+        self.pass_load = [0] * PASS_CYCLE
         self.ctx = ctx
         self.lib = ctx._lib
         self.block = block
@@ -228,22 +272,15 @@ class SessionServer:
         out = []
         for sid, nb in self.admission.admit():
             s, ir = self.pending.pop(sid)
-            conv = T.TANConvolution(self.ctx, self.ir_precision, self.nu_multiple)
-            rc = conv.InitGpu(self.method, s.ir_len, self.block, s.channels)
-            if rc != T.AMF_OK:
+            conv, phase = self._create(s)
+            if conv is None:
                 self.admission.release(nb)
-                raise RuntimeError(f"InitGpu failed for session {sid}: AMF_RESULT {rc}")
-            # the IR transform runs on the update stream; the session's first
-            # blocks are silent and the IR fades in, as for any first
-            # UpdateResponseTD (ConvolutionImpl.cpp:115-117, 877-991)
-            rc = conv.UpdateResponseTD(ir, s.ir_len, None, T.TAN_CONVOLUTION_OPERATION_FLAG_NONE)
-            if rc != T.AMF_OK:
-                conv.Terminate()
-                self.admission.release(nb)
-                raise RuntimeError(f"UpdateResponseTD failed for session {sid}: AMF_RESULT {rc}")
+                raise RuntimeError(f"InitGpu failed for session {sid}: AMF_RESULT {phase}")
             s.conv = conv
-            s.multiple = conv.Info()["nu_multiple"]
+            info = conv.Info()
+            s.multiple = info["nu_multiple"]
             s.nbytes = nb
+            self._place_passes(s, info["pass_blocks"], phase)
             r = self.rows.alloc(s.channels) if self.rows else None
             if r is not None:
                 s.row = r
@@ -254,19 +291,76 @@ class SessionServer:
                 s.out = torch.zeros(s.channels, self.block, device=self.device)
             s.admitted_tick = self.ticks
             self.active[sid] = s
+            # the first IR (transformed on the process stream before the
+            # session's first block, which is silent while the IR fades in,
+            # as after any first UpdateResponseTD, ConvolutionImpl.cpp:115-117,
+            # 877-991)
+            self._queue_update(s, ir)
             out.append(s)
         if out:
             self._call = None
         return out
 
-    def update(self, sid, ir):
-        """UpdateResponseTD for an admitted session (asynchronous; the
-        crossfade happens two blocks later)."""
-        s = self.active[sid]
-        rc = s.conv.UpdateResponseTD(ir, s.ir_len, None, T.TAN_CONVOLUTION_OPERATION_FLAG_NONE)
+    def _create(self, s):
+        """The session's object, with the pass phase that balances pass
+        bytes over the ticks; returns (conv, phase) or (None, AMF_RESULT)."""
+        uniform = self.method == T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+        parts = -(-s.ir_len // self.block)
+        k = self.pass_blocks or (4 if parts >= 16 else 1)
+        phase = 0
+        if uniform and self.balance_passes and k > 1:
+            # block b of the session runs at tick admitted + b; its passes
+            # end on blocks with (b + phase) % k == 0, i.e. on ticks
+            # t = admitted - phase (mod k)
+            phase = (self.ticks - pass_residue(self.pass_load, k)) % k
+        conv = T.TANConvolution(self.ctx, self.ir_precision, self.nu_multiple,
+                                pass_blocks=self.pass_blocks if uniform else 0, pass_phase=phase)
+        rc = conv.InitGpu(self.method, s.ir_len, self.block, s.channels)
         if rc != T.AMF_OK:
-            raise RuntimeError(f"UpdateResponseTD failed for session {sid}: AMF_RESULT {rc}")
+            return None, rc
+        return conv, phase
+
+    def _place_passes(self, s, k, phase):
+        s.pass_blocks, s.pass_phase = k, phase
+        if k <= 1 or PASS_CYCLE % k:
+            return
+        r = (self.ticks - phase) % k
+        s.pass_bytes = pass_bytes(s.ir_len, self.block, s.channels, self.ir_precision)
+        s.pass_slots = tuple(r + j * k for j in range(PASS_CYCLE // k))
+        for i in s.pass_slots:
+            self.pass_load[i] += s.pass_bytes
+
+    def update(self, sid, ir):
+        """UpdateResponseTD for an admitted session (asynchronous; issued
+        before the next tick, the crossfade happens two blocks later)."""
+        s = self.active[sid]
+        self._queue_update(s, ir)
         s.updates += 1
+
+    def _queue_update(self, s, ir):
+        if not self.group_updates:
+            self._issue([(s, ir)])
+            return
+        if s.sid in self._updates:  # a second update before the tick: keep the order
+            self.flush_updates()
+        self._updates[s.sid] = (s, ir)
+
+    def _issue(self, items):
+        convs = [s.conv for s, _ in items]
+        irs = [ir[:s.channels, :s.ir_len] for s, ir in items]
+        if len(items) == 1:
+            rc = convs[0].UpdateResponseTD(irs[0], items[0][0].ir_len, None, T.TAN_CONVOLUTION_OPERATION_FLAG_NONE)
+        else:
+            rc = T.GroupUpdateResponseTD(convs, irs, [s.ir_len for s, _ in items])
+        if rc != T.AMF_OK:
+            raise RuntimeError(f"IR update failed for sessions {[s.sid for s, _ in items]}: AMF_RESULT {rc}")
+
+    def flush_updates(self):
+        """Issues the queued IR updates (one grouped call)."""
+        if self._updates:
+            items = list(self._updates.values())
+            self._updates.clear()
+            self._issue(items)
 
     def release(self, sid):
         """End a session (queued or admitted) and return its bytes."""
@@ -275,6 +369,10 @@ class SessionServer:
             self.admission.queue = collections.deque((k, nb) for k, nb in self.admission.queue if k != sid)
             return s
         s = self.active.pop(sid)
+        self._updates.pop(sid, None)
+        for i in s.pass_slots:
+            self.pass_load[i] -= s.pass_bytes
+        s.pass_slots = ()
         s.conv.Terminate()
         if s.row is not None:
             self.rows.release(s.row, s.channels)
@@ -309,6 +407,8 @@ class SessionServer:
         # starts: control work, as a server thread prepares it off the audio path
         if self._call is None:
             self._rebuild()
+        # the queued IR updates run ahead of the tick on the process stream
+        self.flush_updates()
         t0 = time.perf_counter()
         s = self.stream()
         if timed:

@@ -511,6 +511,40 @@ def GroupProcess(convs, inputs, outputs, flag_masks=None):
     return lib.tanx_group_process_device(n, hs, ins, ist, outs, ost, fptr)
 
 
+def GroupUpdateResponseTD(convs, irs, lengths=None, flag_masks=None, operation_flags=0):
+    """UpdateResponseTD of several objects of one context in one call
+    (tanx_group_update_td_device): object i's update is the one
+    convs[i].UpdateResponseTD(irs[i], lengths[i]) would make, but the IR
+    transforms of all of them run in one launch per partition size.  irs:
+    torch CUDA tensors (channels, >= length); lengths: None (each tensor's
+    row length, as UpdateResponseTD takes it) or per-object sample counts."""
+    n = len(convs)
+    if n == 0:
+        return AMF_OK
+    lens = [h.shape[1] if lengths is None else int(lengths[i]) for i, h in enumerate(irs)]
+    for c, h, ln in zip(convs, irs, lens):
+        if (not _is_device_tensor(h) or h.dim() != 2 or h.shape[0] < c.channels or h.shape[1] < ln
+                or h.dtype.itemsize != 4 or h.stride(1) != 1):
+            return AMF_INVALID_ARG
+    lib = convs[0]._lib
+    convs[0]._ctx.order_after_torch()
+    hs = (ctypes.c_void_p * n)(*[c.handle for c in convs])
+    ptrs = (ctypes.c_void_p * n)(*[h.data_ptr() for h in irs])
+    strides = (ctypes.c_size_t * n)(*[h.stride(0) for h in irs])
+    ln = (ctypes.c_size_t * n)(*lens)
+    fptr = None
+    keep = []
+    if flag_masks is not None:
+        arr = (ctypes.c_void_p * n)()
+        for i, f in enumerate(flag_masks):
+            if f is not None:
+                a = np.ascontiguousarray(f, dtype=np.uint32)
+                keep.append(a)
+                arr[i] = a.ctypes.data
+        fptr = ctypes.cast(arr, ctypes.c_void_p)
+    return lib.tanx_group_update_td_device(n, hs, ptrs, strides, ln, fptr, operation_flags)
+
+
 # TAN_SAMPLE_TYPE (TrueAudioNext.h:73-77)
 TAN_SAMPLE_TYPE_FLOAT = 0
 TAN_SAMPLE_TYPE_SHORT = 1

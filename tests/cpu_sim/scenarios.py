@@ -153,9 +153,10 @@ LONG_PLAN = {
 }
 
 
-def _serving_long(method, nticks, B=256):
+def _serving_long(method, nticks, B=256, kp=0):
     """The long serving test's sessions arriving, updating and leaving, all
-    active ones advanced in one group launch per tick."""
+    active ones advanced in one group launch per tick (uniform ones with
+    pass length kp and phase sid % kp when kp is given)."""
     ctx = T.TANContext(device=0)
     live, xs, ys, irs, done_ticks = {}, {}, {}, {}, {}
     vp, sz = ctypes.c_void_p, ctypes.c_size_t
@@ -164,7 +165,7 @@ def _serving_long(method, nticks, B=256):
             if t == t1 and sid in live:
                 live.pop(sid).Terminate()
             if t == t0:
-                c = T.TANConvolution(ctx)
+                c = T.TANConvolution(ctx, pass_blocks=kp, pass_phase=sid % kp if kp else 0)
                 assert c.InitGpu(method, L, B, C) == T.AMF_OK
                 nb = min(t1, nticks) - t0
                 xs[sid] = simlib.Dev(C, nb * B, np.random.default_rng(sid).uniform(-1, 1, (C, nb * B)))
@@ -187,8 +188,8 @@ def _serving_long(method, nticks, B=256):
     return xs, ys, irs
 
 
-def serving_long_uniform(nticks=2200):
-    xs, ys, irs = _serving_long(UNIFORM, nticks)
+def serving_long_uniform(nticks=2200, kp=0):
+    xs, ys, irs = _serving_long(UNIFORM, nticks, kp=kp)
     for sid in (1, 3):
         C, L, t0, t1, ups = LONG_PLAN[sid]
         nb = xs[sid].shape[1] // 256
@@ -319,6 +320,63 @@ def fft_large(log2n=13, C=2):
     assert lib.tanx_fft_transform_device(ctx.handle, 0, log2n, C, dev.ptr, dev.ptr, dev.stride) == T.AMF_OK
     assert np.array_equal(dev.a, out)
     print("four-step transforms complete", flush=True)
+
+
+GROUP_UPDATE_SPECS = [  # (ir_len, block, channels, method, ir precision)
+    (65536, 256, 3, UNIFORM, 0),
+    (5000, 128, 2, UNIFORM, 0),
+    (131072, 512, 1, UNIFORM, 0),
+    (32768, 256, 2, T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, 0),
+    (20000, 256, 2, UNIFORM, 1),
+    (4096, 64, 3, UNIFORM, 2),
+]
+
+
+def group_updates(nb=40, every=7):
+    """IR updates of several objects in one grouped call beside the same
+    updates one object at a time on twin objects (block sizes 64-512,
+    non-uniform frames, FP16 and BF16 spectra, a channel held by
+    STOP_INPUT): the outputs are the same bits."""
+    ctx = T.TANContext(device=0)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    twins = []
+    for L, Bk, C, m, prec in GROUP_UPDATE_SPECS:
+        pair = []
+        for _ in range(2):
+            c = T.TANConvolution(ctx, ir_precision=prec)
+            assert c.InitGpu(m, L, Bk, C) == T.AMF_OK
+            pair.append(c)
+        x = simlib.Dev(C, nb * Bk, np.random.default_rng(L).uniform(-1, 1, (C, nb * Bk)))
+        ys = [simlib.Dev(C, nb * Bk, 0.0) for _ in range(2)]
+        twins.append((pair, x, ys, L, Bk, C))
+    irs = [[simlib.Dev(C, L, T.signals.ir_bank(C, L, k, base_seed=70 + i)) for k in range(3)]
+           for i, (_, _, _, L, _, C) in enumerate(twins)]
+    flags = [np.zeros(C, np.uint32) for (_, _, _, _, _, C) in twins]
+    flags[0][1] = T.TAN_CONVOLUTION_CHANNEL_FLAG_STOP_INPUT
+    for t in range(nb):
+        if t % every == 0:
+            k = (t // every) % 3
+            fl = [f if t else None for f in flags]  # the first update sets every channel
+            for i, (pair, x, ys, L, Bk, C) in enumerate(twins):
+                fp = fl[i].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if fl[i] is not None else None
+                assert lib.tanx_conv_update_td_device(pair[0].handle, irs[i][k].ptr, L, L, fp, 0) == T.AMF_OK
+            n = len(twins)
+            hs = (vp * n)(*[p[1].handle.value for p, *_ in twins])
+            ps = (vp * n)(*[irs[i][k].ptr for i in range(n)])
+            st = (sz * n)(*[tw[3] for tw in twins])
+            fa = (vp * n)(*[f.ctypes.data if f is not None else None for f in fl])
+            assert lib.tanx_group_update_td_device(n, hs, ps, st, st, ctypes.cast(fa, vp), 0) == T.AMF_OK
+        for pair, x, ys, L, Bk, C in twins:
+            for j in range(2):
+                process(pair[j], x, ys[j], t, 1, Bk)
+    for i, (pair, x, ys, L, Bk, C) in enumerate(twins):
+        assert np.array_equal(ys[0].a, ys[1].a), i
+        assert float(np.abs(ys[0].a).max()) > 0.01, i
+    bad = (vp * 2)(twins[0][0][1].handle.value, twins[0][0][1].handle.value)
+    st = (sz * 2)(65536, 65536)
+    assert lib.tanx_group_update_td_device(2, bad, (vp * 2)(irs[0][0].ptr, irs[0][0].ptr), st, st, None,
+                                           0) == T.AMF_INVALID_ARG
+    print("grouped updates: outputs bit-identical to per-object updates", flush=True)
 
 
 def pass_phases(C=2, L=16384, B=128, nb=80, kp=4):

@@ -48,12 +48,13 @@ def _x(sid, tick, C):
     return rng.uniform(-1.0, 1.0, (C, B)).astype(np.float32)
 
 
-def _run_mix(ctx, budget, plan=PLAN, nticks=60, method=UNIFORM):
-    """Drive a server through the plan; returns {sid: (admit tick, outputs)}."""
+def _run_mix(ctx, budget, plan=PLAN, nticks=60, method=UNIFORM, **server_kw):
+    """Drive a server through the plan; returns {sid: (admit tick, outputs,
+    the object's options)}."""
     import torch
-    srv = S.SessionServer(ctx, block=B, hbm_budget=budget, method=method)
+    srv = S.SessionServer(ctx, block=B, hbm_budget=budget, method=method, **server_kw)
     outs = {sid: [] for sid in plan}
-    admitted_at = {}
+    admitted_at, opts = {}, {}
     upd_count = {sid: 0 for sid in plan}
     for t in range(nticks):
         for sid, (C, L, t0, t1, ups) in plan.items():
@@ -63,6 +64,7 @@ def _run_mix(ctx, budget, plan=PLAN, nticks=60, method=UNIFORM):
                 assert srv.submit(sid, C, L, torch.from_numpy(_ir(sid, 0, C, L)).cuda())
         for s in srv.admit():
             admitted_at[s.sid] = t
+            opts[s.sid] = s.conv_options()
         for sid, (C, L, t0, t1, ups) in plan.items():
             if sid in srv.active and (t - admitted_at[sid]) in ups:
                 upd_count[sid] += 1
@@ -74,15 +76,17 @@ def _run_mix(ctx, budget, plan=PLAN, nticks=60, method=UNIFORM):
         for sid, s in srv.active.items():
             outs[sid].append(s.out.cpu().numpy().copy())
     srv.shutdown()
-    return {sid: (admitted_at[sid], np.concatenate(o, axis=1)) for sid, o in outs.items() if o}
+    assert not any(srv.pass_load), "released sessions return their pass bytes"
+    return {sid: (admitted_at[sid], np.concatenate(o, axis=1), opts[sid]) for sid, o in outs.items() if o}
 
 
-def _alone(ctx, sid, nblocks, plan=PLAN, method=UNIFORM, xfades=None, info=None):
-    """The same session on its own object, one Process per block; appends
-    the crossfade blocks to xfades and the object's Info() to info."""
+def _alone(ctx, sid, nblocks, plan=PLAN, method=UNIFORM, xfades=None, info=None, opts=None):
+    """The same session on its own object (created with the server's options
+    for it), one Process per block; appends the crossfade blocks to xfades
+    and the object's Info() to info."""
     import torch
     C, L, t0, t1, ups = plan[sid]
-    conv = T.TANConvolution(ctx)
+    conv = T.TANConvolution(ctx, **(opts or {}))
     assert conv.InitGpu(method, L, B, C) == T.AMF_OK
     if info is not None:
         info.update(conv.Info())
@@ -122,12 +126,12 @@ def _oracle(sid, nblocks, plan=PLAN):
 
 def test_session_alone_equals_mix_and_oracle(gpu_ctx):
     mix = _run_mix(gpu_ctx, budget=64 << 30)
-    for sid, (at, y) in mix.items():
+    for sid, (at, y, opts) in mix.items():
         C, L, t0, t1, ups = PLAN[sid]
         assert at == t0
         nb = y.shape[1] // B
         assert nb == t1 - t0
-        ya = _alone(gpu_ctx, sid, nb)
+        ya = _alone(gpu_ctx, sid, nb, opts=opts)
         np.testing.assert_array_equal(y, ya, err_msg=f"session {sid}")
         yo = _oracle(sid, nb)
         assert np.abs(y - yo).max() <= 1e-5, f"session {sid}"
@@ -143,10 +147,10 @@ def test_budget_holds_session_until_release(gpu_ctx):
     assert need[1] > need[4] and need[3] >= need[1]
     budget = need[0] + need[2] + need[3] + need[4]
     mix = _run_mix(gpu_ctx, budget=budget)
-    at3, y3 = mix[3]
+    at3, y3, o3 = mix[3]
     assert at3 == PLAN[1][3], "session 3 is admitted in the tick session 1 leaves"
     nb = y3.shape[1] // B
-    np.testing.assert_array_equal(y3, _alone(gpu_ctx, 3, nb))
+    np.testing.assert_array_equal(y3, _alone(gpu_ctx, 3, nb, opts=o3))
     for sid in (0, 1, 2, 4):
         assert mix[sid][0] == PLAN[sid][2]
 
@@ -162,13 +166,14 @@ def test_admission_rejects_oversized(gpu_ctx):
 def test_long_sessions_uniform_against_oracle(gpu_ctx):
     """Sessions of 131,072-524,288 taps (one arriving mid-run) over 2,200
     ticks: the mix equals each session alone, and the oracle."""
-    mix = _run_mix(gpu_ctx, budget=64 << 30, plan=LONG_PLAN, nticks=2200)
-    for sid, (at, y) in sorted(mix.items()):
+    mix = _run_mix(gpu_ctx, budget=64 << 30, plan=LONG_PLAN, nticks=2200, pass_blocks=8)
+    for sid, (at, y, opts) in sorted(mix.items()):
         C, L, t0, t1, ups = LONG_PLAN[sid]
         nb = y.shape[1] // B
         assert at == t0 and nb == t1 - t0
-        print(f"long uniform session {sid}: C={C} L={L} blocks={nb}", flush=True)
-        np.testing.assert_array_equal(y, _alone(gpu_ctx, sid, nb, plan=LONG_PLAN), err_msg=f"session {sid}")
+        print(f"long uniform session {sid}: C={C} L={L} blocks={nb} {opts}", flush=True)
+        np.testing.assert_array_equal(y, _alone(gpu_ctx, sid, nb, plan=LONG_PLAN, opts=opts),
+                                      err_msg=f"session {sid}")
         yo = _oracle(sid, nb, plan=LONG_PLAN)
         err = float(np.abs(y.astype(np.float64) - yo).max())
         snr = min(snr_db(yo[c], y[c]) for c in range(C))
@@ -181,12 +186,12 @@ def test_long_sessions_nonuniform_against_fp64(gpu_ctx):
     equals each session alone, its crossfades fall on the frame-aligned
     schedule, and the output matches the FP64 time-varying convolution."""
     mix = _run_mix(gpu_ctx, budget=64 << 30, plan=LONG_PLAN, nticks=2200, method=NONUNIFORM)
-    for sid, (at, y) in sorted(mix.items()):
+    for sid, (at, y, opts) in sorted(mix.items()):
         C, L, t0, t1, ups = LONG_PLAN[sid]
         nb = y.shape[1] // B
         assert at == t0 and nb == t1 - t0
         xf, info = [], {}
-        ya = _alone(gpu_ctx, sid, nb, plan=LONG_PLAN, method=NONUNIFORM, xfades=xf, info=info)
+        ya = _alone(gpu_ctx, sid, nb, plan=LONG_PLAN, method=NONUNIFORM, xfades=xf, info=info, opts=opts)
         np.testing.assert_array_equal(y, ya, err_msg=f"session {sid}")
         M = info["nu_multiple"]
         sched = nu_schedule([0] + list(ups), M, 0, nb)

@@ -170,3 +170,31 @@ def test_nonuniform_byte_model():
     assert n == 16 * 16 * k + 8 * k + 16 * 256 + 16 * (pb - 2) * 256 + (8 * N + 16 * N + 16 * N + 4 * N) // 16
     assert u / n > 12
     assert Bt.alg_bytes_per_block(65536, 256, 3, 1) == Bt.alg_bytes_per_block(65536, 256, 3)
+
+
+def test_pass_residue_balances_pass_bytes():
+    """Sessions placed one after another by pass_residue keep the pass bytes
+    of the busiest tick residue within one session's pass of the quietest,
+    for pass lengths 4 and 8 mixed; a session of length k lands on every
+    k-th residue."""
+    rng = np.random.default_rng(3)
+    load = [0] * S.PASS_CYCLE
+    sizes = []
+    for _ in range(200):
+        k = int(rng.choice([4, 8]))
+        nbytes = int(rng.integers(1, 100)) * 1000
+        r = S.pass_residue(load, k)
+        assert 0 <= r < k
+        slots = [r + j * k for j in range(S.PASS_CYCLE // k)]
+        for i in slots:
+            load[i] += nbytes
+        sizes.append(nbytes)
+    assert max(load) - min(load) <= max(sizes)
+    # ties go to the lowest residue; a loaded residue is avoided
+    assert S.pass_residue([0] * 8, 4) == 0
+    assert S.pass_residue([5, 0, 0, 0, 0, 0, 0, 0], 8) == 1
+    assert S.pass_residue([5, 0, 0, 0, 0, 0, 0, 0], 4) == 1
+    assert S.pass_residue([0, 9, 9, 9, 1, 9, 9, 9], 4) == 0
+    # the pass of an FP32 session: FDL and IR rows of partitions 2..P-1
+    assert S.pass_bytes(65536, 256, 2) == 2 * 254 * 256 * 16
+    assert S.pass_bytes(65536, 256, 2, T.TAN_IR_PRECISION_FP16) == 2 * 254 * 256 * 12

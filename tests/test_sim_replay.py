@@ -82,3 +82,9 @@ def test_batched_calls(sim_built, kp, B, precision):
 def test_fft_four_step(sim_built, log2n):
     p = _run("fft_large", "start", "0xff", log2n=log2n)
     assert p.returncode == 0, p.returncode
+
+
+@pytest.mark.parametrize("guard,fill", [("end", None), ("start", "0xff")])
+def test_group_updates(sim_built, guard, fill):
+    p = _run("group_updates", guard, fill)
+    assert p.returncode == 0, p.returncode
