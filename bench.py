@@ -141,6 +141,14 @@ RT_STEP_TICKS = 100    # ticks per timed step: two IR update periods
 RT_RING = 8            # input/output blocks kept on the device
 RT_BANKS = 4           # IR banks the updates cycle through
 RT_PASS = 8            # blocks per tail pass of the real-time groups (phases g % RT_PASS)
+RT_SLOTS = 3           # IR spectrum slots per object (2: a third less HBM per channel)
+
+
+def rt_chunks():
+    """Partition chunks per channel of a real-time group's pass: the
+    library's default for the group size (one launch carries every group,
+    so wider passes buy nothing), which also sizes the partial rows kept."""
+    return max(1, 1024 // RT_GROUP)
 
 
 class RealTimeRig:
@@ -165,7 +173,8 @@ class RealTimeRig:
             self.y = torch.zeros_like(self.x)
             self.objs = []
             for g in range(ngroups):
-                conv = T.TANConvolution(self.ctx, pass_blocks=RT_PASS, pass_phase=g % RT_PASS)
+                conv = T.TANConvolution(self.ctx, pass_blocks=RT_PASS, pass_phase=g % RT_PASS,
+                                        max_tail_chunks=rt_chunks(), ir_slots=RT_SLOTS)
                 rc = conv.InitGpu(UNIFORM_METHOD, L, B, RT_GROUP)
                 if rc != T.AMF_OK:
                     log(f"  group {g}: InitGpu returned {rc}; {g} groups")
@@ -195,12 +204,19 @@ class RealTimeRig:
         e0, e1 = self.ev[self.t & 1]
         t0 = time.perf_counter()
         e0.record(self.hs)
+        due, irs = [], []
         for g in range(k):
-            if (self.t + (g * UPDATE_EVERY) // k) % UPDATE_EVERY == 0:
+            # offsets spread over the groups allocated (fixed while the count
+            # of active groups changes, so every group keeps its 50-block period)
+            if (self.t + (g * UPDATE_EVERY) // self.kmax) % UPDATE_EVERY == 0:
                 self.updates[g] += 1
-                bank = self.banks[(self.updates[g] + g) % RT_BANKS]
-                # the transform runs on the process stream, ahead of this tick's launch
-                assert self.objs[g].UpdateResponseTD(bank, L, None, 0) == T.AMF_OK
+                due.append(self.objs[g])
+                irs.append(self.banks[(self.updates[g] + g) % RT_BANKS])
+        if due:
+            # the tick's updates in one call: their IR transforms run in one
+            # launch on the process stream, ahead of this tick's group launch
+            rc = T.GroupUpdateResponseTD(due, irs, [L] * len(due))
+            assert rc == T.AMF_OK, f"grouped IR update of {len(due)} groups: AMF_RESULT {rc}"
         rc = self._call(k, self.t % RT_RING).launch(order=False)
         assert rc == T.AMF_OK, rc
         e1.record(self.hs)
@@ -228,7 +244,7 @@ class RealTimeRig:
         self.torch.cuda.empty_cache()
 
 
-def bench_realtime(T, torch, steps, warmup, margin=0.97):
+def bench_realtime(T, torch, steps, warmup, margin=0.95):
     """Largest channel count whose every tick of `steps` x 100 consecutive
     ticks (IR updates and crossfades included) finishes within the 5.333 ms
     block period with one block in flight.
@@ -237,10 +253,12 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
     active groups from that maximum (never raising it, so the active groups
     keep equal block counts and their pass phases stay staggered) until a
     probe of 100 ticks stays under margin x deadline; the timed run of
-    exactly `steps` steps follows, one group fewer after any miss."""
+    exactly `steps` steps follows, repeated with 2% fewer groups (at least
+    one) after any miss.  The count is that of a run without a miss."""
     from tan_amd import serving as S
     gc_ = __import__("gc")
-    per_group = S.estimate_bytes(T._lib.load(), L, B, RT_GROUP)
+    per_group = S.estimate_bytes(T._lib.load(), L, B, RT_GROUP, pass_blocks=RT_PASS, max_tail_chunks=rt_chunks(),
+                                 ir_slots=RT_SLOTS)
     free, _ = torch.cuda.mem_get_info()
     reserve = RT_BANKS * RT_GROUP * L * 4 + (10 << 30)
     kmax = int((free - reserve) // (per_group + 2 * RT_RING * RT_GROUP * B * 4))
@@ -268,7 +286,7 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
             k = max(1, max(int(k * 0.7), min(k - 1, int(k * margin * DEADLINE_MS / p99))))
         gc_.enable()
         result = None
-        for attempt in range(4):
+        for attempt in range(12):
             rig.run(k, warmup * RT_STEP_TICKS)
             rig.ctx.EnableGroupTiming(True)
             rig.ctx.ReadGroupTiming(reset=True)
@@ -292,7 +310,10 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
             log(f"  timed run: {k} groups, {misses} misses, max {h.max():.3f} ms, p99 {result['p99_host_ms']:.3f} ms")
             if misses == 0 or k == 1:
                 break
-            k -= 1
+            k = max(1, k - max(1, -(-k // 50)))
+        if result["misses"]:
+            # no run without a miss: no real-time count
+            result["channels"] = 0
         # in slices: HBM may be nearly full with the groups' state
         yv = rig.y.view(-1)
         finite = all(bool(torch.isfinite(yv[i:i + (1 << 24)]).all().item()) for i in range(0, yv.numel(), 1 << 24))
@@ -926,7 +947,7 @@ def summarize_profiles(run_dir, groups):
 
 
 def main():
-    global TAIL_CHUNKS, RT_PASS
+    global TAIL_CHUNKS, RT_PASS, RT_GROUP, RT_SLOTS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps of 100 ticks (2,000 ticks at 20)")
@@ -948,8 +969,12 @@ def main():
     ap.add_argument("--nu-sweep", type=str, default="65536,98304")
     ap.add_argument("--rt-groups", type=int, default=0,
                     help="profile run: real-time rig with this many groups of 2,048 channels")
+    ap.add_argument("--rt-group-channels", type=int, default=RT_GROUP,
+                    help="channels per object of the real-time count")
     ap.add_argument("--rt-pass-blocks", type=int, default=RT_PASS,
-                    help="blocks per tail pass of the real-time groups (1..8)")
+                    help="blocks per tail pass of the real-time groups (1..16)")
+    ap.add_argument("--rt-ir-slots", type=int, default=RT_SLOTS, choices=(2, 3),
+                    help="IR spectrum slots per real-time group")
     ap.add_argument("--summarize-profiles", type=str, default="",
                     help="write profiles/ summaries from the rocprofv3 outputs under this folder (no GPU)")
     ap.add_argument("--serving-only", action="store_true",
@@ -959,6 +984,8 @@ def main():
     args = ap.parse_args()
     TAIL_CHUNKS = args.tail_chunks
     RT_PASS = args.rt_pass_blocks
+    RT_GROUP = args.rt_group_channels
+    RT_SLOTS = args.rt_ir_slots
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (channels and "
               "sessions shard across GPUs without exchange; see the design notes)", file=sys.stderr)
@@ -1094,6 +1121,7 @@ def main():
         "config": {"workload": RT_WORKLOAD, "channels": value, "channel_groups": rt["groups"],
                    "group_channels": RT_GROUP, "block": B, "ir_taps": L, "ticks": rt["ticks"],
                    "ticks_per_step": RT_STEP_TICKS, "update_every": UPDATE_EVERY, "pass_blocks": RT_PASS,
+                   "ir_slots": RT_SLOTS, "tail_chunks": rt_chunks(),
                    "sample_rate": FS, "deadline_ms": DEADLINE_MS},
         "deadline_misses": rt["misses"],
         "p50_block_ms": rt["p50_host_ms"], "p99_block_ms": rt["p99_host_ms"], "max_block_ms": rt["max_host_ms"],

@@ -72,6 +72,7 @@ typedef enum AMF_RESULT {
     AMF_WRONG_STATE = 15,
     AMF_FILE_NOT_OPEN = 16,
     AMF_NO_DEVICE = 17,
+    AMF_INPUT_FULL = 25,
     AMF_TAN_CLIPPING_WAS_REQUIRED = 42,
     AMF_TAN_UNSUPPORTED_VERSION = 43
 } AMF_RESULT;

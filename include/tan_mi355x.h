@@ -41,6 +41,7 @@ enum {
     TANX_WRONG_STATE = 15,
     TANX_FILE_NOT_OPEN = 16,
     TANX_NO_DEVICE = 17,
+    TANX_INPUT_FULL = 25,
     TANX_TAN_UNSUPPORTED_VERSION = 43
 };
 
@@ -152,12 +153,14 @@ typedef struct tanx_conv_options {
                                Objects of one group with different phases
                                spread their frame work over the ticks. */
     uint32_t pass_blocks;   /* uniform objects: blocks whose partial sums one
-                               tail pass computes (1..8; 1 = a tail every
+                               tail pass computes (1..16; 1 = a tail every
                                block; 0 = chosen by the library: 4 when the
                                IR has 16 or more partitions, else 1).  A
                                pass streams the IR and FDL rows once for K
-                               blocks; passes of 5..8 blocks run a kernel
-                               with more registers per lane. */
+                               blocks; passes of 5..8 and 9..16 blocks run
+                               kernels with more registers per lane, and
+                               the heads between passes add up to K + 3
+                               partitions themselves. */
     uint32_t pass_phase;    /* uniform objects with multi-block passes: pass
                                phase in blocks, taken modulo pass_blocks K.
                                After block 0, passes run in the launches of
@@ -166,7 +169,25 @@ typedef struct tanx_conv_options {
                                Objects of one group with phases 0..K-1 each
                                stream their rows on a different block, so no
                                block carries every object's pass. */
-    uint32_t reserved[3];   /* zero */
+    uint32_t max_tail_chunks; /* uniform objects: at most this many partition
+                               chunks per channel in a tail or pass (0 = the
+                               library's choice, min(4096 / C, P - 2, 64)).
+                               Each chunk keeps its own partial-sum row per
+                               block (sized at creation; the next heads read
+                               them all), so objects advanced many at a time,
+                               whose launches are wide anyway, need few. */
+    uint32_t ir_slots;      /* uniform objects: IR spectrum slots, 3 (0 =
+                               3) or 2.  With 3, an update issued while a
+                               switch is armed goes to the third slot; with
+                               2 (a third less HBM per channel) such an
+                               update returns TANX_INPUT_FULL and is not
+                               made: the armed filter and the current one
+                               are both in use until the crossfade block has
+                               run (the reference blocks the caller there,
+                               ConvolutionImpl.cpp:421-425).  Updates issued
+                               before the switch is armed replace the
+                               pending one, as with 3. */
+    uint32_t reserved[1];   /* zero */
 } tanx_conv_options;
 /* tanx_conv_create with options (no counterpart in the reference, which
  * stores FP32 spectra only); opt may be NULL. */
@@ -295,6 +316,7 @@ typedef struct tanx_conv_info {
     uint32_t cached_rows;  /* 1: passes read the FDL and IR rows with ordinary loads, because the
                               pass rows of every live object on the context total at most 128 MB
                               (half the Infinity Cache); 0: nontemporal loads */
+    uint32_t ir_slots;     /* IR spectrum slots (3, or 2 by tanx_conv_options.ir_slots) */
 } tanx_conv_info;
 int32_t tanx_conv_get_info(const tanx_conv *conv, tanx_conv_info *info);
 /* Sets the partition chunking of the tail kernel (0 = automatic). */

@@ -24,13 +24,14 @@ class ConvInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64),
                 ("crossfades", ctypes.c_uint64), ("ir_precision", ctypes.c_uint32),
                 ("nu_multiple", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32),
-                ("cached_rows", ctypes.c_uint32)]
+                ("cached_rows", ctypes.c_uint32), ("ir_slots", ctypes.c_uint32)]
 
 
 class ConvOptions(ctypes.Structure):
     _fields_ = [("ir_precision", ctypes.c_uint32), ("nu_multiple", ctypes.c_uint32),
                 ("nu_phase", ctypes.c_uint32), ("pass_blocks", ctypes.c_uint32),
-                ("pass_phase", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("pass_phase", ctypes.c_uint32), ("max_tail_chunks", ctypes.c_uint32),
+                ("ir_slots", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
 
 
 class WavInfo(ctypes.Structure):

@@ -43,6 +43,10 @@ struct DeviceGuard {
     bool ok = false;
     explicit DeviceGuard(int dev)
     {
+        // an error left by an earlier call (one the library handled, such as
+        // a refused allocation, or the caller's own) must not be taken for
+        // the result of this call's launches
+        (void)hipGetLastError();
         if (hipGetDevice(&prev) != hipSuccess) prev = -1;
         ok = hipSetDevice(dev) == hipSuccess;
     }
@@ -374,6 +378,7 @@ struct tanx_conv {
 
     // IR switch state (ConvolutionImpl.cpp:115-117, 869-927)
     int cur = 1, prev = 0, upd = 2;
+    int nslots = 3;  // 2: the update slot is the previous filter's once a crossfade has run
     bool pending = false, ready = false, delayed = false, crossfading = false, last_xfade = false;
     int pos = 0;  // FDL ring slot of the next block
     std::vector<uint8_t> staged;  // channel staged since the last hand-off
@@ -504,6 +509,7 @@ static int max_chunks_for(int C, int P) { return std::max(1, std::min(std::min(k
 struct ConvShape {
     int len = 0, B = 0, C = 0, M = 1, N = 0, P = 0, PB = 0, RB = 0, nchB = 0, pcB = 1, maxchunk = 1, shift = 0;
     int kp = 1;  // blocks per tail pass
+    int nslots = 3;  // IR spectrum slots
     int kphase = 0;  // pass phase: passes in the launches of blocks b with (b + kphase) % kp == 0
     int irtype = IR_F32;
     bool td = false;  // time-domain object: IR slots and history in samples
@@ -575,6 +581,7 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
     if (base == TANX_METHOD_TIME_DOMAIN) {
         // direct convolution for short (HRTF-length) responses, FP32
         if (sh.len > kMaxTdLen || sh.irtype != IR_F32) return TANX_INVALID_ARG;
+        if (opt && opt->ir_slots == 2) return TANX_NOT_SUPPORTED;
         sh.td = true;
         sh.P = 1;
         sh.maxchunk = 1;
@@ -607,6 +614,7 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
         sh.P = sh.len / sh.B;
     }
     sh.maxchunk = max_chunks_for(sh.C, sh.P);
+    if (opt && opt->max_tail_chunks) sh.maxchunk = std::min(sh.maxchunk, (int)opt->max_tail_chunks);
     // multi-block tail passes: uniform objects outside finalize mode whose
     // IR has partitions enough to stream.  A pass reads every partition once
     // for kp blocks; the block that carries it streams the whole object (as
@@ -620,6 +628,13 @@ static int32_t conv_shape(uint32_t method, uint32_t ir_len, uint32_t block, uint
     if (M == 1 && !finalize) {
         sh.kp = want_kp ? (int)want_kp : (sh.P >= 16 ? kPassDefault : 1);
         sh.kphase = (int)((opt ? opt->pass_phase : 0) % (uint32_t)sh.kp);
+    }
+    const uint32_t slots = opt ? opt->ir_slots : 0;
+    if (slots != 0 && slots != 2 && slots != 3) return TANX_INVALID_ARG;
+    if (slots == 2) {
+        // non-uniform frames keep using the previous filter after a switch
+        if (M != 1) return TANX_NOT_SUPPORTED;
+        sh.nslots = 2;
     }
     return TANX_OK;
 }
@@ -645,8 +660,8 @@ static ConvLayout conv_layout(const ConvShape &sh)
         l.Run = al(C);
         return l;
     }
-    l.H = al(3 * C * P * B * elem);
-    l.S = al(3 * C * P * sizeof(float));
+    l.H = al((size_t)sh.nslots * C * P * B * elem);
+    l.S = al((size_t)sh.nslots * C * P * sizeof(float));
     l.F = al(C * P * B * sizeof(float2));
     l.T = al(4 * (size_t)sh.kp * C * (size_t)sh.maxchunk * B * sizeof(float2));
     l.X = al(C * B * sizeof(float));
@@ -733,6 +748,8 @@ extern "C" int32_t tanx_conv_create_ex(tanx_ctx *ctx, uint32_t method, uint32_t 
     cv->tail_nchunk = cv->nchunk;
     cv->kp = sh.kp;
     cv->kphase = sh.kphase;
+    cv->nslots = sh.nslots;
+    if (cv->nslots == 2) cv->upd = 0;  // slots 1 (current) and 0 (update)
     if (!cv->td) {
         cv->rows_bytes = (uint64_t)cv->C * cv->P * cv->B * sizeof(float2) + 2 * cv->slot_bytes() +
                          (cv->nu ? (uint64_t)cv->C * cv->RB * cv->N * sizeof(float2) + 2 * cv->bslot_bytes() : 0);
@@ -1152,9 +1169,14 @@ static bool advance_switch(tanx_conv *cv, uint64_t b)
         xfade = true;
     }
     if (xfade) {
-        cv->prev = cv->cur;
-        cv->cur = (cv->cur + 1) % 3;
-        cv->upd = (cv->upd + 1) % 3;
+        // the armed filter becomes current; the update slot is the one no
+        // filter uses after this block (with 2 slots: the previous filter's,
+        // which the crossfade block itself still reads, so an update is
+        // written only after this block in stream order)
+        const int old = cv->cur;
+        cv->prev = old;
+        cv->cur = cv->upd;
+        cv->upd = cv->nslots == 3 ? (cv->upd + 1) % 3 : old;
     }
     return xfade;
 }
@@ -1819,6 +1841,9 @@ static int32_t update_common(tanx_conv *cv, const float *const *hir, const float
             if (rc != TANX_OK) return rc;
         }
     }
+    // an armed switch holds the current and the armed filter: with two
+    // slots there is none left for this update
+    if (cv->delayed && cv->nslots == 2) return TANX_INPUT_FULL;
     const int target = cv->delayed ? cv->prev : cv->upd;
     const int keep = cv->delayed ? cv->upd : cv->cur;
     int nupd = 0;
@@ -2072,11 +2097,15 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
     if (done > 0) {
         // each class's range splits by launch kind (role set, then pass
         // length) in place, each sub-range one launch with block offsets of
-        // its own: lean 4, lean 8, crossfade 4, crossfade 8, general
-        constexpr int kKinds = 5;
+        // its own: lean 4, 8, 16, crossfade 4, 8, 16, general
+        constexpr int kKinds = 7;
         auto kind_of = [](const TickArgs &a) {
             const int r = group_roles(a);
-            return r == kRolesGeneral ? 4 : 2 * (r == kRolesXfade) + (pass_km(a.kp) == 8);
+            // an object without a pass this tick runs heads alone: in the
+            // smallest kernel, whatever its pass length (a launch's register
+            // count is paid by all of its workgroups)
+            const int km = a.nchunk > 0 ? pass_km(a.kp) : 4;
+            return r == kRolesGeneral ? 6 : 3 * (r == kRolesXfade) + (km == 4 ? 0 : km == 8 ? 1 : 2);
         };
         struct Launch {
             uint32_t first, count;
@@ -2094,8 +2123,8 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
                     if (kind_of(descs[i]) == kind) std::swap(descs[i], descs[mid++]);
                 if (mid == lo) continue;
                 Launch &l = launches[nl++];
-                l = Launch{lo, mid - lo, 64 << c, 0, kind == 4 ? kRolesGeneral : (kind >= 2 ? kRolesXfade : kRolesLean),
-                           (kind & 1) ? 8 : 4};
+                l = Launch{lo, mid - lo, 64 << c, 0, kind == 6 ? kRolesGeneral : (kind >= 3 ? kRolesXfade : kRolesLean),
+                           4 << (kind % 3)};
                 for (uint32_t i = lo; i < mid; i++) {
                     starts[i] = l.total;
                     l.total += descs[i].C * (1 + descs[i].nchunk + descs[i].nchB);
@@ -2388,6 +2417,7 @@ extern "C" int32_t tanx_conv_get_info(const tanx_conv *cv, tanx_conv_info *info)
     info->nu_multiple = (uint32_t)cv->M;
     info->pass_blocks = (uint32_t)cv->kp;
     info->cached_rows = cached_rows(cv) ? 1u : 0u;
+    info->ir_slots = (uint32_t)cv->nslots;
     return TANX_OK;
 }
 

@@ -43,7 +43,7 @@ namespace tanx {
 enum IrType { IR_F32 = 0, IR_F16 = 1, IR_BF16 = 2 };
 inline int ir_elem_bytes(int t) { return t == IR_F32 ? 8 : 4; }
 
-constexpr int kPassMax = 8;      // blocks one tail pass computes partial sums for, at most
+constexpr int kPassMax = 16;     // blocks one tail pass computes partial sums for, at most
 constexpr int kPassDefault = 4;  // the library's choice for IRs of 16 or more partitions
 constexpr int kHeadBatch = 4;    // heads one launch runs together, at most
 // partitions a head adds itself at most: a pass of the launch with heads
@@ -140,8 +140,8 @@ __host__ __device__ inline bool is_pass(const TickArgs &a)
 }
 
 // Pass length a launch compiles for an object of pass length kp (registers
-// grow with it): 4 or 8.
-__host__ __device__ inline int pass_km(int kp) { return kp > 4 ? 8 : 4; }
+// grow with it): 4, 8 or 16.
+__host__ __device__ inline int pass_km(int kp) { return kp > 8 ? 16 : kp > 4 ? 8 : 4; }
 
 // Time-domain convolution (TAN_CONVOLUTION_METHOD_TIME_DOMAIN): one
 // workgroup per channel computes n <= B outputs

@@ -38,12 +38,16 @@ SAMPLE_RATE = 48000
 
 
 def estimate_bytes(lib, ir_len, block, channels, ir_precision=T.TAN_IR_PRECISION_FP32,
-                   method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, nu_multiple=0):
-    """HBM bytes of one session: the object (tanx_conv_estimate_bytes) plus its
-    input and output block."""
+                   method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED, nu_multiple=0, pass_blocks=0,
+                   max_tail_chunks=0, ir_slots=0):
+    """HBM bytes of one session: the object (tanx_conv_estimate_bytes, with
+    the options it will be created with) plus its input and output block."""
     opt = _lib.ConvOptions()
     opt.ir_precision = ir_precision
     opt.nu_multiple = nu_multiple
+    opt.pass_blocks = pass_blocks
+    opt.max_tail_chunks = max_tail_chunks
+    opt.ir_slots = ir_slots
     out = ctypes.c_uint64(0)
     rc = lib.tanx_conv_estimate_bytes(method, ir_len, block, channels, ctypes.addressof(opt), ctypes.byref(out))
     if rc != T.AMF_OK:
@@ -139,7 +143,8 @@ class RowAllocator:
 
 class Session:
     __slots__ = ("sid", "channels", "ir_len", "conv", "inp", "out", "row", "nbytes", "blocks", "arrived_tick",
-                 "admitted_tick", "updates", "multiple", "pass_blocks", "pass_phase", "pass_bytes", "pass_slots")
+                 "admitted_tick", "updates", "multiple", "pass_blocks", "pass_phase", "pass_bytes", "pass_slots",
+                 "max_tail_chunks")
 
     def __init__(self, sid, channels, ir_len):
         self.sid = sid
@@ -158,12 +163,14 @@ class Session:
         self.pass_blocks = 0   # the object's pass length K (tanx_conv_info.pass_blocks)
         self.pass_phase = 0    # the pass phase it was created with
         self.pass_bytes = 0    # bytes one of its passes streams
+        self.max_tail_chunks = 0
         self.pass_slots = ()   # tick residues (mod PASS_CYCLE) its passes land on
 
     def conv_options(self):
         """TANConvolution keyword arguments that rebuild this session's object
         (the same pass length and phase)."""
-        return {"pass_blocks": self.pass_blocks if self.pass_slots else 0, "pass_phase": self.pass_phase}
+        return {"pass_blocks": self.pass_blocks if self.pass_slots else 0, "pass_phase": self.pass_phase,
+                "max_tail_chunks": self.max_tail_chunks}
 
 
 PASS_CYCLE = 8  # common period of pass lengths 1, 2, 4 and 8
@@ -208,7 +215,7 @@ class SessionServer:
 
     def __init__(self, ctx, block=256, hbm_budget=96 << 30, ir_precision=T.TAN_IR_PRECISION_FP32,
                  device="cuda", io_rows=0, method=T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED,
-                 nu_multiple=0, pass_blocks=0, balance_passes=True, group_updates=True):
+                 nu_multiple=0, pass_blocks=0, balance_passes=True, group_updates=True, max_tail_chunks=4):
         """pass_blocks: pass length K of uniform sessions (0: the library's
         choice); balance_passes: give each uniform session the pass phase
         that puts its passes on the tick residue (mod PASS_CYCLE) with the
@@ -216,7 +223,10 @@ class SessionServer:
         rows (a session's output does not depend on its phase);
         group_updates: the IR updates issued between two ticks (first IRs
         of admitted sessions included) go to the device in one grouped call
-        before the next tick (GroupUpdateResponseTD), else one call each."""
+        before the next tick (GroupUpdateResponseTD), else one call each;
+        max_tail_chunks: partition chunks per channel of a session's pass
+        (every session's pass shares one wide launch, so a few suffice; it
+        also sizes the partial rows each session keeps)."""
         if block & (block - 1) or block < 64 or block > 512:
             raise ValueError("block must be a power of two in [64, 512]")
         if pass_blocks not in (0, 1, 2, 4, 8):
@@ -224,6 +234,7 @@ class SessionServer:
         self.pass_blocks = pass_blocks
         self.balance_passes = balance_passes
         self.group_updates = group_updates
+        self.max_tail_chunks = max_tail_chunks
         self._updates = collections.OrderedDict()  # sid -> (session, IR tensor), issued before the next tick
         self.pass_load = [0] * PASS_CYCLE
         self.ctx = ctx
@@ -253,13 +264,18 @@ class SessionServer:
         return 1e3 * self.block / SAMPLE_RATE
 
     # -- control plane --
+    def session_bytes(self, channels, ir_len):
+        """HBM bytes a session of this server takes (the admission charge)."""
+        uniform = self.method == T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+        return estimate_bytes(self.lib, ir_len, self.block, channels, self.ir_precision, self.method,
+                              self.nu_multiple, self.pass_blocks if uniform else 0, self.max_tail_chunks)
+
     def submit(self, sid, channels, ir_len, first_ir):
         """Queue a session; first_ir is a (channels, >= ir_len) CUDA tensor
         (kept until admission).  Returns False if it can never fit."""
         s = Session(sid, channels, ir_len)
         s.arrived_tick = self.ticks
-        nb = estimate_bytes(self.lib, ir_len, self.block, channels, self.ir_precision, self.method,
-                            self.nu_multiple)
+        nb = self.session_bytes(channels, ir_len)
         if not self.admission.submit(sid, nb):
             return False
         self.pending[sid] = (s, first_ir)
@@ -314,7 +330,8 @@ class SessionServer:
             # t = admitted - phase (mod k)
             phase = (self.ticks - pass_residue(self.pass_load, k)) % k
         conv = T.TANConvolution(self.ctx, self.ir_precision, self.nu_multiple,
-                                pass_blocks=self.pass_blocks if uniform else 0, pass_phase=phase)
+                                pass_blocks=self.pass_blocks if uniform else 0, pass_phase=phase,
+                                max_tail_chunks=self.max_tail_chunks)
         rc = conv.InitGpu(self.method, s.ir_len, self.block, s.channels)
         if rc != T.AMF_OK:
             return None, rc
@@ -322,6 +339,7 @@ class SessionServer:
 
     def _place_passes(self, s, k, phase):
         s.pass_blocks, s.pass_phase = k, phase
+        s.max_tail_chunks = self.max_tail_chunks
         if k <= 1 or PASS_CYCLE % k:
             return
         r = (self.ticks - phase) % k

@@ -27,6 +27,7 @@ AMF_ALREADY_INITIALIZED = 12
 AMF_NOT_INITIALIZED = 13
 AMF_WRONG_STATE = 15
 AMF_NO_DEVICE = 17
+AMF_INPUT_FULL = 25
 AMF_TAN_UNSUPPORTED_VERSION = 43
 
 # TAN_CONVOLUTION_METHOD (TrueAudioNext.h:81-98)
@@ -190,19 +191,23 @@ class TANConvolution:
     """TANConvolution on the GPU engine (ConvolutionImpl.cpp)."""
 
     def __init__(self, context, ir_precision=TAN_IR_PRECISION_FP32, nu_multiple=0, nu_phase=0, pass_blocks=0,
-                 pass_phase=0):
+                 pass_phase=0, max_tail_chunks=0, ir_slots=0):
         """ir_precision: storage of the IR spectra (TAN_IR_PRECISION_*);
         nu_multiple: blocks per frame for the non-uniform methods (0: chosen
         by the library, 1: uniform partitions); nu_phase: frame phase in
         blocks, modulo the multiple (the first frame ends at block
         multiple - 1 - phase), so that objects advanced together can spread
         their frame work over the ticks; pass_blocks: blocks whose partial
-        sums one tail pass computes for uniform objects (1..8, 0: chosen by
+        sums one tail pass computes for uniform objects (1..16, 0: chosen by
         the library, 1: a tail every block); pass_phase: uniform objects with
         multi-block passes run their passes in the launches of blocks b with
         (b + pass_phase) % pass_blocks == 0 (after block 0), so objects
         advanced together with phases 0..pass_blocks-1 spread their passes
-        over the ticks."""
+        over the ticks; max_tail_chunks: cap on the partition chunks per
+        channel of a pass (0: the library's choice), which sets the partial
+        rows kept per block; ir_slots: 2 keeps two IR spectrum slots instead
+        of 3 (uniform objects; an update issued while a switch is armed then
+        returns AMF_INPUT_FULL)."""
         self._ctx = context
         self._lib = context._lib
         self.handle = None
@@ -213,6 +218,8 @@ class TANConvolution:
         self.nu_phase = nu_phase
         self.pass_blocks = pass_blocks
         self.pass_phase = pass_phase
+        self.max_tail_chunks = max_tail_chunks
+        self.ir_slots = ir_slots
 
     # -- Init family (ConvolutionImpl.cpp:179-305, 1113-1259) --
     def Init(self, method, response_length, buffer_size, channels):
@@ -228,6 +235,8 @@ class TANConvolution:
         opt.nu_phase = self.nu_phase
         opt.pass_blocks = self.pass_blocks
         opt.pass_phase = self.pass_phase
+        opt.max_tail_chunks = self.max_tail_chunks
+        opt.ir_slots = self.ir_slots
         rc = self._lib.tanx_conv_create_ex(self._ctx.handle, method, response_length, buffer_size,
                                            channels, ctypes.addressof(opt), ctypes.byref(h))
         if rc == AMF_OK:

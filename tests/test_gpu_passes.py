@@ -101,7 +101,7 @@ def _updates(C, L, blocks, seed=600):
 DENSE = [0, 7, 8, 20, 21, 22, 40, 44, 45, 61, 62, 63, 64, 90]
 
 
-@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 8, 11, 16])
 def test_dense_updates_every_pass_length(gpu_ctx, K):
     L, B, C, nb = 8192, 256, 3, 110
     ups = _updates(C, L, DENSE)
@@ -149,7 +149,7 @@ def test_default_pass_length(gpu_ctx):
     assert conv.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, 65536, 256, 2) == T.AMF_OK
     assert conv.Info()["pass_blocks"] == 1
     conv.Terminate()
-    conv = T.TANConvolution(gpu_ctx, pass_blocks=9)
+    conv = T.TANConvolution(gpu_ctx, pass_blocks=17)
     assert conv.InitGpu(UNIFORM, 65536, 256, 2) == T.AMF_INVALID_ARG
 
 
@@ -182,7 +182,8 @@ CALLS = [1, 3, 7, 2, 13, 5, 1, 1, 9, 4, 6, 11, 2, 8]
 @pytest.mark.parametrize("K,B,precision", [(4, 256, T.TAN_IR_PRECISION_FP32), (3, 128, T.TAN_IR_PRECISION_FP32),
                                            (2, 512, T.TAN_IR_PRECISION_FP32), (4, 64, T.TAN_IR_PRECISION_FP32),
                                            (4, 256, T.TAN_IR_PRECISION_FP16), (8, 256, T.TAN_IR_PRECISION_FP32),
-                                           (6, 128, T.TAN_IR_PRECISION_FP32), (8, 64, T.TAN_IR_PRECISION_BF16)])
+                                           (6, 128, T.TAN_IR_PRECISION_FP32), (8, 64, T.TAN_IR_PRECISION_BF16),
+                                           (16, 256, T.TAN_IR_PRECISION_FP32), (13, 128, T.TAN_IR_PRECISION_FP16)])
 def test_multi_block_calls_batch_heads(gpu_ctx, K, B, precision):
     """Calls of several blocks launch up to K heads together with the pass
     for the blocks after them (which reads the rows before the batch): the
@@ -282,7 +283,7 @@ def _oracle_run(L, B, x, irs, nb):
     return yo
 
 
-@pytest.mark.parametrize("K", [4, 8])
+@pytest.mark.parametrize("K", [4, 8, 16])
 def test_pass_phases_alone_and_grouped(gpu_ctx, K):
     """Pass phases 0..K-1 (tanx_conv_options.pass_phase) of K-block passes:
     each object alone with calls of uneven size and dense updates, and the K
@@ -343,3 +344,65 @@ def test_pass_phases_alone_and_grouped(gpu_ctx, K):
     for c in objs:
         c.Terminate()
     ctx.Terminate()
+
+
+def test_two_ir_slots(gpu_ctx):
+    """ir_slots=2: the same output as three slots over updates that never
+    fall on an armed switch; an update while a switch is armed (after the
+    block that hands the previous one off and the block that arms it) returns
+    AMF_INPUT_FULL and changes nothing; non-uniform and time-domain objects
+    refuse two slots."""
+    import torch
+    L, B, C, nb = 16384, 256, 2, 90
+    x = T.signals.input_bank(C, nb * B, base_seed=81)
+    xd = torch.from_numpy(x).cuda()
+    ups = [0, 3, 9, 10, 30, 31, 47, 60]   # 10 and 31: replace a pending update
+    irs = {b: torch.from_numpy(T.signals.ir_bank(C, L, k, base_seed=82)).cuda() for k, b in enumerate(ups)}
+    outs = []
+    for slots in (3, 2):
+        conv = T.TANConvolution(gpu_ctx, ir_slots=slots, pass_blocks=8)
+        assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+        assert conv.Info()["ir_slots"] == slots
+        yd = torch.zeros_like(xd)
+        for b in range(nb):
+            if b in irs:
+                assert conv.UpdateResponseTD(irs[b], L, None, BLOCKING) == T.AMF_OK
+            rc, done = conv.Process(xd[:, b * B:], yd[:, b * B:], B)
+            assert rc == T.AMF_OK and done == B
+        conv.Synchronize()
+        outs.append(yd.cpu().numpy())
+        conv.Terminate()
+    np.testing.assert_array_equal(outs[1], outs[0])
+    yo = _oracle_run(L, B, x, {b: irs[b].cpu().numpy() for b in irs}, nb)
+    assert np.abs(outs[1].astype(np.float64) - yo).max() <= 1e-5
+    # the armed window: update before block 5, blocks 5 and 6 hand it off and arm it
+    conv = T.TANConvolution(gpu_ctx, ir_slots=2)
+    twin = T.TANConvolution(gpu_ctx)
+    for c in (conv, twin):
+        assert c.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    y2, y3 = torch.zeros_like(xd), torch.zeros_like(xd)
+    for b in range(12):
+        if b == 5:
+            for c in (conv, twin):
+                assert c.UpdateResponseTD(irs[3], L, None, BLOCKING) == T.AMF_OK
+        if b == 7:
+            # three slots take it (into the third), two refuse it
+            assert conv.UpdateResponseTD(irs[9], L, None, BLOCKING) == T.AMF_INPUT_FULL
+            assert twin.UpdateResponseTD(irs[9], L, None, BLOCKING) == T.AMF_OK
+        assert conv.Process(xd[:, b * B:], y2[:, b * B:], B)[0] == T.AMF_OK
+        assert twin.Process(xd[:, b * B:], y3[:, b * B:], B)[0] == T.AMF_OK
+    conv.Synchronize()
+    twin.Synchronize()
+    assert conv.Info()["crossfades"] == 1 and twin.Info()["crossfades"] == 2
+    # up to the twin's second switch the two are the same
+    np.testing.assert_array_equal(y2[:, :9 * B].cpu().numpy(), y3[:, :9 * B].cpu().numpy())
+    # once the crossfade block has run, the slot is free again
+    assert conv.UpdateResponseTD(irs[9], L, None, BLOCKING) == T.AMF_OK
+    for c in (conv, twin):
+        c.Terminate()
+    nu = T.TANConvolution(gpu_ctx, ir_slots=2)
+    assert nu.InitGpu(T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM, 65536, 256, 2) == T.AMF_NOT_SUPPORTED
+    td = T.TANConvolution(gpu_ctx, ir_slots=2)
+    assert td.InitGpu(T.TAN_CONVOLUTION_METHOD_TIME_DOMAIN, 256, 256, 2) == T.AMF_NOT_SUPPORTED
+    bad = T.TANConvolution(gpu_ctx, ir_slots=4)
+    assert bad.InitGpu(UNIFORM, L, B, C) == T.AMF_INVALID_ARG

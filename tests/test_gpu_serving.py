@@ -140,8 +140,8 @@ def test_session_alone_equals_mix_and_oracle(gpu_ctx):
 
 
 def test_budget_holds_session_until_release(gpu_ctx):
-    lib = gpu_ctx._lib
-    need = {sid: S.estimate_bytes(lib, L, B, C) for sid, (C, L, *_r) in PLAN.items()}
+    probe = S.SessionServer(gpu_ctx, block=B)  # the admission charge of the servers below
+    need = {sid: probe.session_bytes(C, L) for sid, (C, L, *_r) in PLAN.items()}
     # 3 (arrives at 10) does not fit beside 0, 1 and 2; 4 (arrives at 25,
     # smaller) does and goes first; 3 fits once 1 leaves at tick 40
     assert need[1] > need[4] and need[3] >= need[1]

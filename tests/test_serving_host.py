@@ -98,6 +98,20 @@ def test_estimate_covers_resident_state():
         assert n - n16 >= 3 * C * ln * 4 - 4096  # half-size IR slots
     with pytest.raises(ValueError):
         S.estimate_bytes(lib, 4096, 300, 1)
+    # the partial-sum ring: 2K slots x 2 filters x chunks rows of B complex
+    # per channel, so pass length and chunk cap both enter the estimate
+    L, B, C = 65536, 256, 512
+    base = S.estimate_bytes(lib, L, B, C, pass_blocks=4, max_tail_chunks=2)
+    assert S.estimate_bytes(lib, L, B, C, pass_blocks=8, max_tail_chunks=2) - base == 4 * 4 * C * 2 * B * 8
+    assert S.estimate_bytes(lib, L, B, C, pass_blocks=4, max_tail_chunks=8) - base == 4 * 4 * C * 6 * B * 8
+    # two IR slots: one slot of spectra and row scales less
+    assert base - S.estimate_bytes(lib, L, B, C, pass_blocks=4, max_tail_chunks=2, ir_slots=2) == \
+        C * (L // B) * (B * 8 + 4)
+    with pytest.raises(ValueError):
+        S.estimate_bytes(lib, L, B, C, ir_slots=5)
+    # uncapped, the library's own bound applies (4096 / C = 8 chunks here)
+    assert S.estimate_bytes(lib, L, B, C, pass_blocks=4) == S.estimate_bytes(lib, L, B, C, pass_blocks=4,
+                                                                              max_tail_chunks=8)
 
 
 def test_draw_streams():

@@ -17,7 +17,11 @@ SIM = os.path.join(HERE, "cpu_sim")
 
 @pytest.fixture(scope="module")
 def sim_built():
-    subprocess.check_call(["make", "-s", "-C", SIM])
+    # one build at a time (parallel test workers share the output)
+    import fcntl
+    with open(os.path.join(HERE, ".cpu_sim_build.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        subprocess.check_call(["make", "-s", "-C", SIM])
 
 
 def _run(name, guard="end", fill=None, timeout=600, **kw):
@@ -60,19 +64,19 @@ def test_noalloc_group(sim_built, guard, fill):
     assert p.returncode == 0, p.returncode
 
 
-@pytest.mark.parametrize("kp", [4, 8])
+@pytest.mark.parametrize("kp", [4, 8, 16])
 def test_pass_phases(sim_built, kp):
     p = _run("pass_phases", "start", "0xff", kp=kp)
     assert p.returncode == 0, p.returncode
 
 
-@pytest.mark.parametrize("kp", [4, 8])
+@pytest.mark.parametrize("kp", [4, 8, 16])
 def test_rt_rig(sim_built, kp):
     p = _run("rt_rig", "start", "0xff", kp=kp)
     assert p.returncode == 0, p.returncode
 
 
-@pytest.mark.parametrize("kp,B,precision", [(8, 256, 0), (6, 128, 0), (8, 64, 2)])
+@pytest.mark.parametrize("kp,B,precision", [(8, 256, 0), (6, 128, 0), (8, 64, 2), (16, 256, 0), (13, 128, 1)])
 def test_batched_calls(sim_built, kp, B, precision):
     p = _run("batched_calls", "start", "0xff", kp=kp, B=B, precision=precision)
     assert p.returncode == 0, p.returncode
