@@ -1022,10 +1022,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": per_ch * value if per_ch else None,
-                "kernel": f"k_group_tick<0, 256, 1, {RT_PASS}> (the heads of every channel, the "
-                          f"single-filter passes of one group in {RT_PASS}, the new filter's catch-up pass "
-                          f"of each group whose IR switched) + k_group_tick<0, 256, 2, {RT_PASS}> when a "
-                          "switch falls on a group's pass block (its two-filter pass)",
+                "kernel": f"one tick's group launches: k_group_tick<0, 256, 1, 4> (the heads of the groups "
+                          f"without a pass this tick), k_group_tick<0, 256, 1, {RT_PASS}> (the groups whose "
+                          f"single-filter pass runs this tick, one in {RT_PASS}, and the new filter's catch-up "
+                          f"pass of each group whose IR switched, with their heads), k_group_tick<0, 256, 2, "
+                          f"{RT_PASS}> when a switch falls on a group's pass block (its two-filter pass)",
                 "trace": f"profiles/k_group_tick_realtime_{PROFILE_GROUPS * RT_GROUP}ch_65536taps_b256.txt",
                 "traffic_source": "profiles/pmc_traffic_k_group_tick_rt.json" if per_ch else None,
                 "alg_bytes_per_launch": gt["tick_alg_bytes"] / launches, "avg_launch_us": tick_us,
