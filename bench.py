@@ -479,7 +479,9 @@ def _pct(a, q):
 def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
     """Uneven streams (IR log-uniform 4,096-262,144 taps, B in {128, 256, 512},
     one channel each) in one group launch per 128-sample base tick, with the
-    tail chunks tuned per (block, rounded length) bucket."""
+    partition layout (uniform with 4- or 8-block passes, or non-uniform
+    frames of M x B) and then the tail chunks tuned per (block, rounded
+    length) bucket, by measured tick time."""
     from tan_amd import batching as Bt
     rows = []
     for n in counts:
@@ -489,6 +491,8 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
         cyc = batch.cycle
         before = batch.time_ticks(cyc * 64) / (cyc * 64)
         t0 = time.perf_counter()
+        layouts = batch.tune_layouts(cycles=8)
+        after_layout = batch.time_ticks(cyc * 64) / (cyc * 64)
         chosen = batch.tune(cycles=8)
         tune_s = time.perf_counter() - t0
         after = batch.time_ticks(cyc * 64) / (cyc * 64)
@@ -504,16 +508,18 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
         gb = (gt["tick_alg_bytes"] + gt["frame_alg_bytes"]) / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
         row = {"streams": n, "base_block": batch.base, "deadline_ms": period,
                "objects_per_launch": [len(d) for d in batch.due],
-               "tick_ms_default_chunks": before, "tick_ms_tuned": after, "tune_seconds": tune_s,
+               "tick_ms_default": before, "tick_ms_layouts_tuned": after_layout, "tick_ms_tuned": after,
+               "tune_seconds": tune_s,
+               "tuned_layouts": {f"B{b}_L{ln}": f"{k} {v}" for (b, ln), (k, v) in sorted(layouts.items())},
                "tuned_chunks": {f"B{b}_L{ln}": c for (b, ln), c in sorted(chosen.items())},
                "mean_tick_ms": total_ms / ticks, "p50_tick_ms": _pct(per, 50), "p99_tick_ms": _pct(per, 99),
                "max_tick_ms": float(max(per)), "deadline_misses": int(sum(1 for v in per if v > period)),
                "kernel_alg_GBps": gb, "hbm_frac": gb / HBM_PEAK_GBPS,
                "kernel_ms_per_tick": kms / ticks, "alg_MB_per_tick": (gt["tick_alg_bytes"] + gt["frame_alg_bytes"])
                / ticks / 1e6, "realtime_factor": period / (total_ms / ticks)}
-        log(f"  batching {n} streams: tick {row['mean_tick_ms'] * 1e3:.1f} us (default chunks "
-            f"{before * 1e3:.1f} us), p99 {row['p99_tick_ms'] * 1e3:.1f} us, {gb:.0f} GB/s, "
-            f"tuned {row['tuned_chunks']}")
+        log(f"  batching {n} streams: tick {row['mean_tick_ms'] * 1e3:.1f} us (default "
+            f"{before * 1e3:.1f} us, layouts tuned {after_layout * 1e3:.1f} us), p99 "
+            f"{row['p99_tick_ms'] * 1e3:.1f} us, {gb:.0f} GB/s, layouts {row['tuned_layouts']}")
         rows.append(row)
         for c in batch.convs:
             c.Terminate()
@@ -857,6 +863,66 @@ PROFILE_GROUPS = 60    # groups of the profiled rig runs (about the real-time co
 PMC_JSON = os.path.join(ROOT, "profiles", "pmc_traffic_k_group_tick_rt.json")
 
 
+PROFILE_LEGS = {
+    "c2": ("k_window_64ch_65536taps_b256.txt",
+           "one 64-channel object x 65,536 taps, B = 256, 2,000 blocks in calls of 50 blocks, a fresh IR "
+           "every 50 blocks (the c2_64ch leg): fused launches of up to 4 heads with the 4-block pass after "
+           "them, tail chunks pinned to 8; the object's rows (100 MB) stay in the Infinity Cache"),
+    "serving": ("k_group_tick_serving.txt",
+                "the served mixed workload (Poisson sessions, 20/s, mean life 30 s, 1-16 channels, IR "
+                "8,192-524,288 taps, B = 256, an IR update every 10-100 blocks; about 620 sessions / 5,300 "
+                "channels), 4 s of audio after a 64-tick warm-up, uniform then non-uniform: one grouped IR "
+                "update call and one group launch per kind per tick"),
+    "room": ("room_responses_and_block_dft.txt",
+             "the room-response generator (1 and 64 responses of 65,536 taps, both ears, per launch) and the "
+             "block DFT on the matrix cores (1,024 frames of 256 and 512 samples, plain and split-bf16, beside "
+             "the FP32 LDS FFT)"),
+}
+
+
+def profile_leg(T, torch, leg):
+    """The workload of one profile summary other than the real-time rig's."""
+    global TAIL_CHUNKS
+    ctx = T.TANContext(device=0)
+    if leg == "c2":
+        TAIL_CHUNKS = 8  # the autotuner's choice in the unprofiled run (the tracer slows its dry runs)
+        hs = torch.cuda.Stream()
+        hctx = T.TANContext(device=0, conv_stream=hs, general_stream=hs)
+        bench_throughput(T, torch, hctx, 2000, 100, timing=False, stream=hs)
+        hctx.Terminate()
+    elif leg == "serving":
+        for meth in (UNIFORM_METHOD, NONUNIFORM_METHOD):
+            bench_serving(T, torch, ctx, seconds=4.0, method=meth)
+    elif leg == "room":
+        bench_room(T, torch, ctx)
+        bench_dft_mfma(T, torch, ctx, log2n=8)
+        bench_dft_mfma(T, torch, ctx, log2n=9)
+    ctx.Terminate()
+    log(f"profile leg {leg} done")
+
+
+def summarize_trace(run_dir, leg):
+    """Writes profiles/<leg's file> from the kernel statistics of a traced
+    `bench.py --profile-run --profile-leg <leg>` under run_dir/prof_<leg>."""
+    import csv
+    name, what = PROFILE_LEGS[leg]
+    stats = list(csv.DictReader(open(os.path.join(run_dir, f"prof_{leg}", "run_kernel_stats.csv"))))
+    cmd = (f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv -d "
+           f"<repo>/gpurun_out/prof_{leg} -o run -- python3 <repo>/bench.py --profile-run --profile-leg {leg}")
+    lines = [f"# Kernel trace: {what}", "", "Command (from the repository root on the GPU box):", "  " + cmd, "",
+             f"{'kernel':64s} {'calls':>6s} {'avg us':>10s} {'min us':>10s} {'max us':>10s} {'% time':>7s}"]
+    for r in stats:
+        if float(r["Percentage"]) < 0.05:
+            continue
+        lines.append(f"{r['Name'].split('(')[0].replace('void ', '')[:64]:64s} {int(r['Calls']):6d} "
+                     f"{float(r['AverageNs']) / 1e3:10.1f} {float(r['MinNs']) / 1e3:10.1f} "
+                     f"{float(r['MaxNs']) / 1e3:10.1f} {float(r['Percentage']):7.2f}")
+    out = os.path.join(ROOT, "profiles", name)
+    with open(out, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return out
+
+
 def pmc_traffic_per_channel():
     """HBM bytes per channel and tick of the real-time group launches (every
     k_group_tick of a tick) from the committed rocprofv3 PMC summary
@@ -977,6 +1043,9 @@ def main():
                     help="IR spectrum slots per real-time group")
     ap.add_argument("--summarize-profiles", type=str, default="",
                     help="write profiles/ summaries from the rocprofv3 outputs under this folder (no GPU)")
+    ap.add_argument("--profile-leg", type=str, default="rt", choices=["rt"] + sorted(PROFILE_LEGS),
+                    help="with --profile-run: the real-time rig (rt) or another leg's workload; with "
+                         "--summarize-profiles: which summary to write")
     ap.add_argument("--serving-only", action="store_true",
                     help="run the serving legs alone and print their rows as one JSON line (no headline)")
     ap.add_argument("--profile-run", action="store_true",
@@ -992,13 +1061,19 @@ def main():
         sys.exit(2)
 
     if args.summarize_profiles:
-        print(summarize_profiles(args.summarize_profiles, args.rt_groups or PROFILE_GROUPS))
+        if args.profile_leg == "rt":
+            print(summarize_profiles(args.summarize_profiles, args.rt_groups or PROFILE_GROUPS))
+        else:
+            print(summarize_trace(args.summarize_profiles, args.profile_leg))
         return
     import torch
     import tan_amd as T
     T.load_library()
     if args.profile_run:
-        profile_run(T, torch, args.rt_groups or PROFILE_GROUPS)
+        if args.profile_leg == "rt":
+            profile_run(T, torch, args.rt_groups or PROFILE_GROUPS)
+        else:
+            profile_leg(T, torch, args.profile_leg)
         return
     if args.serving_only:
         sctx = T.TANContext(device=0)

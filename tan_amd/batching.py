@@ -119,6 +119,26 @@ def phase_plan(blocks):
     return base, periods, phases, cycle
 
 
+NONUNIFORM = T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
+UNIFORM = T.TAN_CONVOLUTION_METHOD_FFT_UNIFORM_PARTITIONED
+
+
+def layout_candidates(block, ir_len):
+    """Partition layouts the tuner tries for streams of this block size and IR
+    length: uniform partitions of B with 4- or 8-block passes, and frames of
+    M x B for the non-uniform method (M = 2..16, a frame of at most 4,096
+    samples and at most half the IR), as the reference's bestNUMultiple
+    (ConvolutionImpl.cpp:224-246) chooses among multiples, here by
+    measurement instead of a cost model."""
+    ln = rounded_length(ir_len, block)
+    out = [("uniform", 4), ("uniform", 8)] if ln // block >= 16 else [("uniform", 1)]
+    m = 2
+    while m <= 16 and m * block <= 4096 and 2 * m * block <= ln:
+        out.append(("nonuniform", m))
+        m *= 2
+    return out
+
+
 class StreamBatch:
     """Streams of uneven (ir_len, block) advancing together on one device.
 
@@ -140,7 +160,11 @@ class StreamBatch:
         self.convs = []
         self.inputs = []
         self.outputs = []
+        self.irs = []
+        self.layouts = []  # per stream: (kind, pass length or frame multiple); None = the library's choice
         self.buckets = {}
+        self.ir_precision = ir_precision
+        self.method = method
         gen = torch.Generator(device=device)
         gen.manual_seed(ir_seed)
         for j, (ln, b, c) in enumerate(self.specs):
@@ -158,16 +182,62 @@ class StreamBatch:
             if rc != T.AMF_OK:
                 raise RuntimeError(f"UpdateResponseTD failed: AMF_RESULT {rc}")
             self.convs.append(conv)
+            self.irs.append(h)
+            self.layouts.append(None)
             self.inputs.append(torch.empty(c, b, device=device).uniform_(-1.0, 1.0, generator=gen))
             self.outputs.append(torch.zeros(c, b, device=device))
             self.buckets.setdefault((b, rounded_length(ln, b)), []).append(j)
         self.due = [[j for j in range(len(self.specs)) if t % self.periods[j] == self.phases[j]]
                     for t in range(self.cycle)]
-        self.calls = [GroupCall([self.convs[j] for j in d], [self.inputs[j] for j in d],
-                                [self.outputs[j] for j in d]) for d in self.due]
+        self._build_calls()
         self.bytes_per_tick = [sum(alg_bytes_per_block(*self.specs[j]) for j in d) for d in self.due]
         self.ticks = 0
         self._stream = None
+
+    def _build_calls(self):
+        self.calls = [GroupCall([self.convs[j] for j in d], [self.inputs[j] for j in d],
+                                [self.outputs[j] for j in d]) for d in self.due]
+
+    def stream_options(self, j):
+        """(method, TANConvolution keyword arguments) that rebuild stream j's
+        object as the batch has it."""
+        lay = self.layouts[j]
+        if lay is None:
+            return self.method, {"ir_precision": self.ir_precision}
+        kind, v = lay
+        if kind == "uniform":
+            # the bucket's streams take pass phases 0..v-1 in turn, so a
+            # bucket's passes are spread over its blocks
+            phase = self.buckets[(self.specs[j][1], rounded_length(self.specs[j][0], self.specs[j][1]))].index(j) % v
+            return UNIFORM, {"ir_precision": self.ir_precision, "pass_blocks": v, "pass_phase": phase}
+        return NONUNIFORM, {"ir_precision": self.ir_precision, "nu_multiple": v}
+
+    def set_bucket_layout(self, key, layout):
+        """Rebuilds the objects of one (block, rounded length) bucket with a
+        partition layout of layout_candidates (their state starts over: the
+        IR is transformed again and fades in)."""
+        import torch
+        for j in self.buckets[key]:
+            ln, b, c = self.specs[j]
+            self.layouts[j] = layout
+            method, kw = self.stream_options(j)
+            conv = T.TANConvolution(self.ctx, **kw)
+            rc = conv.InitGpu(method, ln, b, c)
+            if rc != T.AMF_OK:
+                raise RuntimeError(f"InitGpu(L={ln}, B={b}, C={c}, {layout}) failed: AMF_RESULT {rc}")
+            torch.cuda.current_stream().synchronize()
+            rc = conv.UpdateResponseTD(self.irs[j], ln, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY)
+            if rc != T.AMF_OK:
+                raise RuntimeError(f"UpdateResponseTD failed: AMF_RESULT {rc}")
+            self.convs[j].Terminate()
+            self.convs[j] = conv
+        self._build_calls()
+
+    def reset(self):
+        """Rebuilds every object with its current layout: every stream starts
+        over from block 0 (after tuning, whose timed ticks advanced them)."""
+        for key, js in self.buckets.items():
+            self.set_bucket_layout(key, self.layouts[js[0]])
 
     @property
     def base_period_ms(self):
@@ -224,6 +294,28 @@ class StreamBatch:
             rc = self.convs[j].SetTailChunks(chunks)
             if rc != T.AMF_OK:
                 raise RuntimeError(f"SetTailChunks failed: AMF_RESULT {rc}")
+
+    def tune_layouts(self, cycles=4):
+        """Coordinate descent over buckets (largest byte share first): each
+        bucket tries every partition layout of layout_candidates with the
+        others fixed and keeps the one with the least device time over
+        `cycles` full tick cycles.  Returns {bucket: layout}."""
+        n = cycles * self.cycle
+        share = {k: sum(alg_bytes_per_block(*self.specs[j]) // self.periods[j] for j in v)
+                 for k, v in self.buckets.items()}
+        chosen = {}
+        self.time_ticks(n)  # warm-up
+        for k in sorted(self.buckets, key=lambda k: -share[k]):
+            best = None
+            for lay in layout_candidates(k[0], k[1]):
+                self.set_bucket_layout(k, lay)
+                self.time_ticks(self.cycle)  # past the rebuilt objects' first pass
+                ms = min(self.time_ticks(n) for _ in range(2))
+                if best is None or ms < best[0]:
+                    best = (ms, lay)
+            chosen[k] = best[1]
+            self.set_bucket_layout(k, best[1])
+        return chosen
 
     def tune(self, cycles=4, candidates=(1, 2, 4, 8, 16, 32, 64), passes=1):
         """Coordinate descent over buckets (largest byte share first): each

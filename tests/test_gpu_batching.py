@@ -38,13 +38,14 @@ def _drive(batch, nticks, seed):
     return [np.concatenate(x, 1) for x in xs], [np.concatenate(y, 1) for y in ys]
 
 
-def _alone(ctx, spec, x, ir):
-    """The stream's object alone, one block per call as in the batch (a call
-    of several blocks would run its heads together, with other rounding)."""
+def _alone(ctx, spec, x, ir, method=UNIFORM, kw=None):
+    """The stream's object alone (created as the batch created it), one block
+    per call as in the batch (a call of several blocks would run its heads
+    together, with other rounding)."""
     import torch
     L, B, C = spec
-    conv = T.TANConvolution(ctx)
-    assert conv.InitGpu(UNIFORM, L, B, C) == T.AMF_OK
+    conv = T.TANConvolution(ctx, **(kw or {}))
+    assert conv.InitGpu(method, L, B, C) == T.AMF_OK
     assert conv.UpdateResponseTD(ir, L, None, T.TAN_CONVOLUTION_OPERATION_FLAG_BLOCK_UNTIL_READY) == T.AMF_OK
     xd = torch.from_numpy(x).cuda()
     yd = torch.zeros_like(xd)
@@ -99,6 +100,27 @@ def test_tuned_batch_within_tolerance(gpu_ctx):
     for j, spec in enumerate(tuned.specs):
         ya = _alone(gpu_ctx, spec, xs[j], irs[j])
         assert np.abs(ys[j] - ya).max() <= 1e-5, f"stream {j}"
+
+
+def test_layout_tuner_streams_equal_alone(gpu_ctx):
+    """The partition-layout tuner (uniform with 4- or 8-block passes, or
+    non-uniform frames of M x B, per length bucket, by measured tick time)
+    picks a candidate for every bucket; rebuilt from block 0 with the chosen
+    layouts, every stream equals its object alone created with the same
+    options, bit for bit, and long ones match the oracle."""
+    specs = Bt.draw_streams(24, seed=4, max_len=131072)
+    batch = Bt.StreamBatch(gpu_ctx, specs)
+    chosen = batch.tune_layouts(cycles=1)
+    assert set(chosen) == set(batch.buckets)
+    for (b, ln), lay in chosen.items():
+        assert lay in Bt.layout_candidates(b, ln), (b, ln, lay)
+    print("tuned layouts:", {f"{b}/{ln}": lay for (b, ln), lay in sorted(chosen.items())}, flush=True)
+    batch.reset()
+    xs, ys = _drive(batch, 48, seed=5)
+    for j, spec in enumerate(batch.specs):
+        method, kw = batch.stream_options(j)
+        ya = _alone(gpu_ctx, spec, xs[j], batch.irs[j], method, kw)
+        np.testing.assert_array_equal(ys[j], ya, err_msg=f"stream {j} {spec} {batch.layouts[j]}")
 
 
 def _drive_resident(batch, nticks, seed):

@@ -212,3 +212,20 @@ def test_pass_residue_balances_pass_bytes():
     # the pass of an FP32 session: FDL and IR rows of partitions 2..P-1
     assert S.pass_bytes(65536, 256, 2) == 2 * 254 * 256 * 16
     assert S.pass_bytes(65536, 256, 2, T.TAN_IR_PRECISION_FP16) == 2 * 254 * 256 * 12
+
+
+def test_layout_candidates():
+    """Uniform layouts with 4- and 8-block passes for IRs of 16 or more
+    partitions, one-block tails below; non-uniform multiples M = 2..16 with
+    frames of at most 4,096 samples and at most half the IR."""
+    assert Bt.layout_candidates(128, 4096) == [("uniform", 4), ("uniform", 8), ("nonuniform", 2),
+                                               ("nonuniform", 4), ("nonuniform", 8), ("nonuniform", 16)]
+    assert Bt.layout_candidates(512, 262144) == [("uniform", 4), ("uniform", 8), ("nonuniform", 2),
+                                                 ("nonuniform", 4), ("nonuniform", 8)]
+    assert Bt.layout_candidates(512, 4096) == [("uniform", 1), ("nonuniform", 2), ("nonuniform", 4)]
+    assert Bt.layout_candidates(256, 700) == [("uniform", 1), ("nonuniform", 2)]
+    for b in (128, 256, 512):
+        for ln in (4096, 9000, 65536, 262144):
+            for kind, v in Bt.layout_candidates(b, ln):
+                if kind == "nonuniform":
+                    assert v * b <= 4096 and 2 * v * b <= Bt.rounded_length(ln, b)
