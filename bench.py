@@ -140,8 +140,9 @@ RT_GROUP = 2048        # channels per object (channel group) of the real-time co
 RT_STEP_TICKS = 100    # ticks per timed step: two IR update periods
 RT_RING = 8            # input/output blocks kept on the device
 RT_BANKS = 4           # IR banks the updates cycle through
-RT_PASS = 8            # blocks per tail pass of the real-time groups (phases g % RT_PASS)
-RT_SLOTS = 3           # IR spectrum slots per object (2: a third less HBM per channel)
+RT_PASS = 12           # blocks per tail pass of the real-time groups (phases rt_phase(g))
+RT_SLOTS = 2           # IR spectrum slots per object (2: a third less HBM per channel; the
+                       # schedule never updates a group while its switch is armed)
 
 
 def rt_chunks():
@@ -151,15 +152,32 @@ def rt_chunks():
     return max(1, 1024 // RT_GROUP)
 
 
+def rt_phase(g):
+    """Pass phase of group g: phases in the order 0, 2, 4, ..., 1, 3, 5, ...
+    round robin, so the phases that carry one group more than the others
+    (when the count is not a multiple of RT_PASS) pass on even ticks."""
+    order = list(range(0, RT_PASS, 2)) + list(range(1, RT_PASS, 2))
+    return order[g % RT_PASS]
+
+
+def rt_update_offset(g):
+    """Update offset of group g in the 50-tick period: one group per offset
+    for the first 50, then the odd offsets 1, 3, 5, ... so that ticks with two
+    updates are odd ticks, never the even ticks of the heavier passes."""
+    if g < UPDATE_EVERY:
+        return g
+    return (2 * (g - UPDATE_EVERY) + 1) % UPDATE_EVERY
+
+
 class RealTimeRig:
     """The real-time channel count: channel groups of RT_GROUP channels (one
     object each, 65,536 taps, B = 256, RT_PASS-block passes with phases
-    g % RT_PASS), a fresh IR for every channel every 50 blocks (group g's
-    update at the ticks t with (t + 50 g / k) % 50 == 0, so updates and
-    their crossfades are staggered over the period), all k active groups
-    advanced by one group launch per tick with one block in flight.
-    Everything runs on one torch stream that is the context's process
-    stream."""
+    rt_phase(g)), a fresh IR for every channel every 50 blocks (group g's
+    update at the ticks t with (t + rt_update_offset(g)) % 50 == 0, so
+    updates and their crossfades are staggered over the period), all k
+    active groups advanced by one group launch per tick with one block in
+    flight.  Everything runs on one torch stream that is the context's
+    process stream."""
 
     def __init__(self, T, torch, ngroups):
         self.T, self.torch = T, torch
@@ -173,13 +191,16 @@ class RealTimeRig:
             self.y = torch.zeros_like(self.x)
             self.objs = []
             for g in range(ngroups):
-                conv = T.TANConvolution(self.ctx, pass_blocks=RT_PASS, pass_phase=g % RT_PASS,
+                conv = T.TANConvolution(self.ctx, pass_blocks=RT_PASS, pass_phase=rt_phase(g),
                                         max_tail_chunks=rt_chunks(), ir_slots=RT_SLOTS)
                 rc = conv.InitGpu(UNIFORM_METHOD, L, B, RT_GROUP)
                 if rc != T.AMF_OK:
                     log(f"  group {g}: InitGpu returned {rc}; {g} groups")
                     break
-                assert conv.UpdateResponseTD(self.banks[g % RT_BANKS], L, None, 0) == T.AMF_OK
+                # no IR yet: each group's first comes from its own update
+                # schedule (within the warm-up), so every update is 50 blocks
+                # after the previous one (two IR slots refuse one that finds
+                # a switch still armed)
                 self.objs.append(conv)
         self.kmax = len(self.objs)
         self.updates = [0] * self.kmax
@@ -206,9 +227,9 @@ class RealTimeRig:
         e0.record(self.hs)
         due, irs = [], []
         for g in range(k):
-            # offsets spread over the groups allocated (fixed while the count
-            # of active groups changes, so every group keeps its 50-block period)
-            if (self.t + (g * UPDATE_EVERY) // self.kmax) % UPDATE_EVERY == 0:
+            # fixed offsets (a group keeps its 50-block period while the count
+            # of active groups changes)
+            if (self.t + rt_update_offset(g)) % UPDATE_EVERY == 0:
                 self.updates[g] += 1
                 due.append(self.objs[g])
                 irs.append(self.banks[(self.updates[g] + g) % RT_BANKS])
@@ -288,10 +309,14 @@ def bench_realtime(T, torch, steps, warmup, margin=0.95):
         result = None
         for attempt in range(12):
             rig.run(k, warmup * RT_STEP_TICKS)
-            rig.ctx.EnableGroupTiming(True)
-            rig.ctx.ReadGroupTiming(reset=True)
             gc_.collect()
             gc_.disable()
+            rig.ctx.EnableGroupTiming(True)
+            # a few ticks after the pauses above (collection, set-up), as in a
+            # loop that keeps a tick every 5.333 ms: the first tick after the
+            # device idles runs slower while it returns to full clock
+            rig.run(k, 8)
+            rig.ctx.ReadGroupTiming(reset=True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             h, d = rig.run(k, steps * RT_STEP_TICKS)
@@ -305,9 +330,11 @@ def bench_realtime(T, torch, steps, warmup, margin=0.95):
                       "misses": misses, "p50_host_ms": float(np.percentile(h, 50)),
                       "p99_host_ms": float(np.percentile(h, 99)), "max_host_ms": float(h.max()),
                       "p50_device_ms": float(np.percentile(d, 50)), "p99_device_ms": float(np.percentile(d, 99)),
-                      "max_device_ms": float(d.max()), "group_timing": gt, "attempt": attempt,
+                      "max_device_ms": float(d.max()), "max_tick_index": int(np.argmax(h)),
+                      "group_timing": gt, "attempt": attempt,
                       "probes": probes}
-            log(f"  timed run: {k} groups, {misses} misses, max {h.max():.3f} ms, p99 {result['p99_host_ms']:.3f} ms")
+            log(f"  timed run: {k} groups, {misses} misses, max {h.max():.3f} ms (tick {int(np.argmax(h))}), "
+                f"p99 {result['p99_host_ms']:.3f} ms")
             if misses == 0 or k == 1:
                 break
             k = max(1, k - max(1, -(-k // 50)))
@@ -859,7 +886,7 @@ PROFILE_CMDS = {
              "python3 <repo>/bench.py --profile-run --rt-groups {g}",
 }
 PROFILE_TICKS = 200
-PROFILE_GROUPS = 60    # groups of the profiled rig runs (about the real-time count)
+PROFILE_GROUPS = 72    # groups of the profiled rig runs (about the real-time count)
 PMC_JSON = os.path.join(ROOT, "profiles", "pmc_traffic_k_group_tick_rt.json")
 
 

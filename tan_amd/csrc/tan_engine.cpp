@@ -67,7 +67,8 @@ static constexpr int kTimerRing = 1024;
 // or when read.
 struct KernelTimer {
     std::vector<hipEvent_t> start, stop;
-    int n = 0;
+    int head = 0;     // slot the next launch records into
+    int pending = 0;  // slots recorded and not read yet; the oldest is head - pending
     double ms = 0.0;
     uint64_t count = 0;
     bool init()
@@ -85,18 +86,54 @@ struct KernelTimer {
         start.clear();
         stop.clear();
     }
+    // reads the oldest recorded slot: waiting for it, or only if it is done
+    // (hipErrorNotReady otherwise)
+    hipError_t read_oldest(bool wait)
+    {
+        const int i = (head - pending + kTimerRing) % kTimerRing;
+        hipError_t e = wait ? hipEventSynchronize(stop[i]) : hipEventQuery(stop[i]);
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();  // not an error of the launches that follow
+            return e;
+        }
+        if (e != hipSuccess) return e;
+        float t = 0.f;
+        e = hipEventElapsedTime(&t, start[i], stop[i]);
+        if (e != hipSuccess) return e;
+        ms += t;
+        count++;
+        pending--;
+        return hipSuccess;
+    }
+    // events for the next launch.  A few completed slots are read first (in a
+    // stream of launches the oldest are long done: a few queries per launch),
+    // so no launch has to read a whole ring; only a full ring waits.
+    hipError_t acquire(hipEvent_t *e0, hipEvent_t *e1)
+    {
+        for (int k = 0; k < 4 && pending > 0; k++) {
+            const hipError_t e = read_oldest(false);
+            if (e == hipErrorNotReady) break;
+            if (e != hipSuccess) return e;
+        }
+        if (pending == kTimerRing) {
+            const hipError_t e = read_oldest(true);
+            if (e != hipSuccess) return e;
+        }
+        *e0 = start[head];
+        *e1 = stop[head];
+        return hipSuccess;
+    }
+    void commit()
+    {
+        head = (head + 1) % kTimerRing;
+        pending++;
+    }
     hipError_t drain()
     {
-        for (int i = 0; i < n; i++) {
-            hipError_t e = hipEventSynchronize(stop[i]);
+        while (pending > 0) {
+            const hipError_t e = read_oldest(true);
             if (e != hipSuccess) return e;
-            float t = 0.f;
-            e = hipEventElapsedTime(&t, start[i], stop[i]);
-            if (e != hipSuccess) return e;
-            ms += t;
-            count++;
         }
-        n = 0;
         return hipSuccess;
     }
 };
@@ -1077,12 +1114,12 @@ static int32_t launch_timed(tanx_conv *cv, KernelTimer &kt, int tail_nf, const T
 {
     hipStream_t s = cv->ctx->proc;
     if (cv->timing) {
-        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
         // the launches stamp the events themselves: the time of the kernels alone
-        hipEvent_t e0 = kt.start[kt.n], e1 = kt.stop[kt.n];
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        TANX_CHECK(kt.acquire(&e0, &e1));
         TANX_CHECK(launch_tick(cv->B, tail_nf, a, s, e0, fa ? nullptr : e1));
         if (fa) TANX_CHECK(launch_frame(cv->irtype, *fa, s, e1));
-        kt.n++;
+        kt.commit();
         alg_sum += alg_bytes;
         if (a.nhead > 0) cv->timed_blocks++;
         return TANX_OK;
@@ -1365,9 +1402,10 @@ static int32_t process_td_block(tanx_conv *cv, const float *din, long long in_st
     hipStream_t s = cv->ctx->proc;
     KernelTimer &kt = cv->t_head;
     if (cv->timing) {
-        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
-        TANX_CHECK(launch_td(t, s, kt.start[kt.n], kt.stop[kt.n]));
-        kt.n++;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        TANX_CHECK(kt.acquire(&e0, &e1));
+        TANX_CHECK(launch_td(t, s, e0, e1));
+        kt.commit();
         // history window, filters, input and output
         cv->head_alg_bytes += (uint64_t)cv->C * (cv->len * sizeof(float) * (xfade ? 3 : 2) + 3 * cv->B * sizeof(float));
         cv->timed_blocks++;
@@ -1443,14 +1481,12 @@ static int32_t flush_heads(tanx_conv *cv, HeadQueue &q)
     hipStream_t s = cv->ctx->proc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (cv->timing) {
-        if (kt.n == kTimerRing) TANX_CHECK(kt.drain());
-        e0 = kt.start[kt.n];
-        e1 = kt.stop[kt.n];
+        TANX_CHECK(kt.acquire(&e0, &e1));
     }
     if (pnf > 0) TANX_CHECK(launch_window(cv->B, pnf, h, p, s, e0, e1));
     else TANX_CHECK(launch_heads(cv->B, cv->irtype, h, s, e0, e1));
     if (cv->timing) {
-        kt.n++;
+        kt.commit();
         alg += hb + tb;
         cv->timed_blocks += (uint64_t)n;
     }
@@ -2097,15 +2133,15 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
     if (done > 0) {
         // each class's range splits by launch kind (role set, then pass
         // length) in place, each sub-range one launch with block offsets of
-        // its own: lean 4, 8, 16, crossfade 4, 8, 16, general
-        constexpr int kKinds = 7;
+        // its own: lean 4, 8, 12, 16, crossfade 4, 8, 12, 16, general
+        constexpr int kKinds = 9;
         auto kind_of = [](const TickArgs &a) {
             const int r = group_roles(a);
             // an object without a pass this tick runs heads alone: in the
             // smallest kernel, whatever its pass length (a launch's register
             // count is paid by all of its workgroups)
             const int km = a.nchunk > 0 ? pass_km(a.kp) : 4;
-            return r == kRolesGeneral ? 6 : 3 * (r == kRolesXfade) + (km == 4 ? 0 : km == 8 ? 1 : 2);
+            return r == kRolesGeneral ? 8 : 4 * (r == kRolesXfade) + km / 4 - 1;
         };
         struct Launch {
             uint32_t first, count;
@@ -2123,8 +2159,8 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
                     if (kind_of(descs[i]) == kind) std::swap(descs[i], descs[mid++]);
                 if (mid == lo) continue;
                 Launch &l = launches[nl++];
-                l = Launch{lo, mid - lo, 64 << c, 0, kind == 6 ? kRolesGeneral : (kind >= 3 ? kRolesXfade : kRolesLean),
-                           4 << (kind % 3)};
+                l = Launch{lo, mid - lo, 64 << c, 0, kind == 8 ? kRolesGeneral : (kind >= 4 ? kRolesXfade : kRolesLean),
+                           4 * (kind % 4 + 1)};
                 for (uint32_t i = lo; i < mid; i++) {
                     starts[i] = l.total;
                     l.total += descs[i].C * (1 + descs[i].nchunk + descs[i].nchB);
@@ -2137,9 +2173,7 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
         if (staged) TANX_CHECK(hipEventRecord(staged, ctx->proc));
         hipEvent_t t0 = nullptr, t1 = nullptr;
         if (ctx->gtiming) {
-            if (ctx->gt_tick.n == kTimerRing) TANX_CHECK(ctx->gt_tick.drain());
-            t0 = ctx->gt_tick.start[ctx->gt_tick.n];
-            t1 = ctx->gt_tick.stop[ctx->gt_tick.n];
+            TANX_CHECK(ctx->gt_tick.acquire(&t0, &t1));
         }
         for (int i = 0; i < nl; i++) {
             const Launch &l = launches[i];
@@ -2150,23 +2184,21 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
                                          ctx->proc, i == 0 ? t0 : nullptr, i == nl - 1 ? t1 : nullptr));
         }
         if (ctx->gtiming) {
-            ctx->gt_tick.n++;
+            ctx->gt_tick.commit();
             ctx->g_tick_bytes += tick_bytes;
             ctx->g_objects += done;
         }
         if (nframe > 0) {
             hipEvent_t f0 = nullptr, f1 = nullptr;
             if (ctx->gtiming) {
-                if (ctx->gt_frame.n == kTimerRing) TANX_CHECK(ctx->gt_frame.drain());
-                f0 = ctx->gt_frame.start[ctx->gt_frame.n];
-                f1 = ctx->gt_frame.stop[ctx->gt_frame.n];
+                TANX_CHECK(ctx->gt_frame.acquire(&f0, &f1));
             }
             TANX_CHECK(launch_group_frame(
                 irtype, reinterpret_cast<const FrameArgs *>(g.dev + g.desc_bytes + g.start_bytes),
                 reinterpret_cast<const int *>(g.dev + g.desc_bytes + g.start_bytes + g.fdesc_bytes), nframe,
                 (unsigned)ftotal, ctx->proc, f0, f1));
             if (ctx->gtiming) {
-                ctx->gt_frame.n++;
+                ctx->gt_frame.commit();
                 ctx->g_frame_bytes += frame_bytes;
             }
         }

@@ -140,8 +140,8 @@ __host__ __device__ inline bool is_pass(const TickArgs &a)
 }
 
 // Pass length a launch compiles for an object of pass length kp (registers
-// grow with it): 4, 8 or 16.
-__host__ __device__ inline int pass_km(int kp) { return kp > 8 ? 16 : kp > 4 ? 8 : 4; }
+// grow with it): 4, 8, 12 or 16.
+__host__ __device__ inline int pass_km(int kp) { return kp > 12 ? 16 : kp > 8 ? 12 : kp > 4 ? 8 : 4; }
 
 // Time-domain convolution (TAN_CONVOLUTION_METHOD_TIME_DOMAIN): one
 // workgroup per channel computes n <= B outputs

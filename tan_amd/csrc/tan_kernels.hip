@@ -1813,6 +1813,12 @@ static hipError_t launch_tick_t(int B, int tail_nf, const TickArgs &a, hipStream
         } else {
             TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 2, 8>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
         }
+    } else if (pass_km(a.kp) == 12) {
+        if (tail_nf == 2) {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 2, 12>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        } else {
+            TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 1, HT, 2, 12>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
+        }
     } else {
         if (tail_nf == 2) {
             TANX_SWITCH_B(B, hipExtLaunchKernelGGL((k_tick<M, 2, HT, 2, 16>), dim3(blocks), dim3(kTickThreads), 0, s, ev0, ev1, 0, a));
@@ -1857,10 +1863,11 @@ static hipError_t launch_group_m(int roles, int km, const TickArgs *d, const int
                                  hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const dim3 g(blocks), t(kTickThreads);
-    if (km != 4 && km != 8 && km != 16) return hipErrorInvalidValue;
-#define TANX_GROUP_KM(R)                                                                                   \
-    if (km == 4) hipExtLaunchKernelGGL((k_group_tick<HT, M, R, 4>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);      \
-    else if (km == 8) hipExtLaunchKernelGGL((k_group_tick<HT, M, R, 8>), g, t, 0, s, ev0, ev1, 0, d, st, nobj); \
+    if (km != 4 && km != 8 && km != 12 && km != 16) return hipErrorInvalidValue;
+#define TANX_GROUP_KM(R)                                                                                    \
+    if (km == 4) hipExtLaunchKernelGGL((k_group_tick<HT, M, R, 4>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);       \
+    else if (km == 8) hipExtLaunchKernelGGL((k_group_tick<HT, M, R, 8>), g, t, 0, s, ev0, ev1, 0, d, st, nobj);  \
+    else if (km == 12) hipExtLaunchKernelGGL((k_group_tick<HT, M, R, 12>), g, t, 0, s, ev0, ev1, 0, d, st, nobj); \
     else hipExtLaunchKernelGGL((k_group_tick<HT, M, R, 16>), g, t, 0, s, ev0, ev1, 0, d, st, nobj)
     switch (roles) {
     case kRolesGeneral:
@@ -1909,6 +1916,7 @@ static hipError_t launch_heads_t(int B, const HeadBatchArgs &h, hipStream_t s, h
 #define TANX_HEADS(MM)                                                                            \
     if (pass_km(h.kp) == 4) hipExtLaunchKernelGGL((k_heads<MM, HT, kHeadBatch + 3>), g, t, 0, s, ev0, ev1, 0, h); \
     else if (pass_km(h.kp) == 8) hipExtLaunchKernelGGL((k_heads<MM, HT, kHeadBatch + 7>), g, t, 0, s, ev0, ev1, 0, h); \
+    else if (pass_km(h.kp) == 12) hipExtLaunchKernelGGL((k_heads<MM, HT, kHeadBatch + 11>), g, t, 0, s, ev0, ev1, 0, h); \
     else hipExtLaunchKernelGGL((k_heads<MM, HT, kHeadBatch + 15>), g, t, 0, s, ev0, ev1, 0, h)
     switch (B) {
     case 64: TANX_HEADS(64); break;
@@ -1931,6 +1939,7 @@ static hipError_t launch_window_t(int B, int tail_nf, const HeadBatchArgs &h, co
 #define TANX_WINDOW(MM)                              \
     if (pass_km(a.kp) == 4) { TANX_WINDOW_K(MM, 4); }      \
     else if (pass_km(a.kp) == 8) { TANX_WINDOW_K(MM, 8); } \
+    else if (pass_km(a.kp) == 12) { TANX_WINDOW_K(MM, 12); } \
     else { TANX_WINDOW_K(MM, 16); }
     switch (B) {
     case 64: TANX_WINDOW(64); break;

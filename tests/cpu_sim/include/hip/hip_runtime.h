@@ -78,6 +78,7 @@ typedef int hipError_t;
 enum {
     hipSuccess = 0,
     hipErrorInvalidValue = 1,
+    hipErrorNotReady = 600,
     hipErrorOutOfMemory = 2,
     hipErrorMemoryAllocation = 2,
     hipErrorInvalidDevice = 101,
@@ -140,6 +141,7 @@ hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t e);

@@ -101,7 +101,7 @@ def _updates(C, L, blocks, seed=600):
 DENSE = [0, 7, 8, 20, 21, 22, 40, 44, 45, 61, 62, 63, 64, 90]
 
 
-@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 8, 11, 16])
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 8, 11, 12, 16])
 def test_dense_updates_every_pass_length(gpu_ctx, K):
     L, B, C, nb = 8192, 256, 3, 110
     ups = _updates(C, L, DENSE)
@@ -283,7 +283,7 @@ def _oracle_run(L, B, x, irs, nb):
     return yo
 
 
-@pytest.mark.parametrize("K", [4, 8, 16])
+@pytest.mark.parametrize("K", [4, 8, 12, 16])
 def test_pass_phases_alone_and_grouped(gpu_ctx, K):
     """Pass phases 0..K-1 (tanx_conv_options.pass_phase) of K-block passes:
     each object alone with calls of uneven size and dense updates, and the K

@@ -70,7 +70,7 @@ def test_pass_phases(sim_built, kp):
     assert p.returncode == 0, p.returncode
 
 
-@pytest.mark.parametrize("kp", [4, 8, 16])
+@pytest.mark.parametrize("kp", [4, 8, 12, 16])
 def test_rt_rig(sim_built, kp):
     p = _run("rt_rig", "start", "0xff", kp=kp)
     assert p.returncode == 0, p.returncode
