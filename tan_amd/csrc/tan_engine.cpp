@@ -2133,15 +2133,18 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
     if (done > 0) {
         // each class's range splits by launch kind (role set, then pass
         // length) in place, each sub-range one launch with block offsets of
-        // its own: lean 4, 8, 12, 16, crossfade 4, 8, 12, 16, general
-        constexpr int kKinds = 9;
+        // its own: lean 4, 8, 12, 16, crossfade 4, 8, 12, 16, general, heads
+        // (objects without a pass this tick, whatever their pass length: a
+        // launch's register count is paid by all of its workgroups)
+        constexpr int kKinds = 10;
         auto kind_of = [](const TickArgs &a) {
             const int r = group_roles(a);
-            // an object without a pass this tick runs heads alone: in the
-            // smallest kernel, whatever its pass length (a launch's register
-            // count is paid by all of its workgroups)
-            const int km = a.nchunk > 0 ? pass_km(a.kp) : 4;
-            return r == kRolesGeneral ? 8 : 4 * (r == kRolesXfade) + km / 4 - 1;
+            if (r == kRolesHeads) return 9;
+            if (r == kRolesGeneral) return 8;
+            return 4 * (r == kRolesXfade) + pass_km(a.kp) / 4 - 1;
+        };
+        auto kind_roles = [](int kind) {
+            return kind == 9 ? kRolesHeads : kind == 8 ? kRolesGeneral : kind >= 4 ? kRolesXfade : kRolesLean;
         };
         struct Launch {
             uint32_t first, count;
@@ -2159,11 +2162,10 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
                     if (kind_of(descs[i]) == kind) std::swap(descs[i], descs[mid++]);
                 if (mid == lo) continue;
                 Launch &l = launches[nl++];
-                l = Launch{lo, mid - lo, 64 << c, 0, kind == 8 ? kRolesGeneral : (kind >= 4 ? kRolesXfade : kRolesLean),
-                           4 * (kind % 4 + 1)};
+                l = Launch{lo, mid - lo, 64 << c, 0, kind_roles(kind), kind < 8 ? 4 * (kind % 4 + 1) : 4};
                 for (uint32_t i = lo; i < mid; i++) {
                     starts[i] = l.total;
-                    l.total += descs[i].C * (1 + descs[i].nchunk + descs[i].nchB);
+                    l.total += group_blocks(descs[i], l.roles);
                 }
                 lo = mid;
             }

@@ -18,9 +18,10 @@ namespace tanx {
 // row is therefore exactly B complex floats (B*8 bytes).
 
 // One launch of k_tick runs up to two roles:
-//   head (blocks [0, nhead)): channel c = block; R2C of [x_{t-1}, x_t], FDL
-//     insert, A_f = sum of tail partials + X_{t-1} H_{f,1} + X_t H_{f,0},
-//     C2R, crossfade, output;
+//   head (blocks [0, nhead)): channel c = block (heads-only group launches:
+//     c = wave_heads(B) * block + wave); R2C of [x_{t-1}, x_t], FDL insert,
+//     A_f = sum of tail partials + X_{t-1} H_{f,1} + X_t H_{f,0}, C2R,
+//     crossfade, output;
 //   tail (blocks [nhead, nhead + C * nchunk)): partial sums over partitions
 //     [2 + j*pc, 2 + (j+1)*pc) of X_{t+1-i} H_i for the NEXT block;
 //   frame multiply-accumulate (blocks after those, non-uniform objects only,
@@ -49,6 +50,12 @@ constexpr int kHeadBatch = 4;    // heads one launch runs together, at most
 // partitions a head adds itself at most: a pass of the launch with heads
 // [w, e) serves blocks up to e + kPassMax - 1 from rows <= w - 1, e - w <= kHeadBatch
 constexpr int kNearMax = kHeadBatch + kPassMax - 1;
+// Heads per workgroup of a heads-only group launch (objects without a pass,
+// tail or frame slice in a tick): one per wave for blocks of at most 256
+// samples (an FFT stage of the head is at most 64 butterflies, one wave's
+// worth), one per workgroup above.  Other tick launches run a head per
+// workgroup.
+constexpr int wave_heads(int B) { return B <= 256 ? 4 : 1; }
 
 struct TickArgs {
     int B;                 // block size (group launches dispatch on it)
@@ -240,13 +247,20 @@ hipError_t launch_group_tick(int irtype, int B, int roles, int km, const TickArg
                              int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);
 // Role set of a group descriptor (kRoles*): heads with a single-filter pass
-// or none (lean), with a two-filter pass (crossfade), or with a single tail
-// or a frame slice (general).
-constexpr int kRolesGeneral = 0, kRolesLean = 1, kRolesXfade = 2;
+// (lean), with a two-filter pass (crossfade), with a single tail or a frame
+// slice (general), or heads alone (heads: wave_heads(B) per workgroup).
+constexpr int kRolesGeneral = 0, kRolesLean = 1, kRolesXfade = 2, kRolesHeads = 3;
 __host__ __device__ inline int group_roles(const TickArgs &a)
 {
     if (a.nchB > 0 || (a.nchunk > 0 && !is_pass(a))) return kRolesGeneral;
-    return (a.nchunk > 0 && a.tHb != nullptr) ? kRolesXfade : kRolesLean;
+    if (a.nchunk == 0) return kRolesHeads;
+    return a.tHb != nullptr ? kRolesXfade : kRolesLean;
+}
+// Workgroups of a group descriptor's roles in a launch of role set `roles`
+inline int group_blocks(const TickArgs &a, int roles)
+{
+    const int heads = roles == kRolesHeads ? (a.nhead + wave_heads(a.B) - 1) / wave_heads(a.B) : a.nhead;
+    return heads + a.C * (a.nchunk + a.nchB);
 }
 constexpr int kGroupMaxB = 512;
 // Consecutive head-only blocks of a uniform object with multi-block passes,

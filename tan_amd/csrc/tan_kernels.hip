@@ -21,13 +21,64 @@
 
 namespace tanx {
 
+// Loads and stores through the global address space.  A pointer read from
+// device memory (a group descriptor) is generic to the compiler, and plain
+// accesses through it become flat instructions: 64-bit per-lane addresses
+// (two VGPRs each) and waits shared with LDS.  Through these helpers they
+// are global instructions with a scalar base and a 32-bit lane offset.
+template <class T>
+using gmem_t = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ T gld(const T *p)
+{
+    return *(const gmem_t<T> *)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T *p, T v)
+{
+    *(gmem_t<T> *)p = v;
+}
+// float2 / float4 go through native vectors: the HIP vector classes copy
+// through a generic reference, which would make the access flat again
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 gld(const float2 *p)
+{
+    const v2f v = *(const gmem_t<v2f> *)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 gld(const float4 *p)
+{
+    const v4f v = *(const gmem_t<v4f> *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(float2 *p, float2 v)
+{
+    v2f w;
+    w.x = v.x;
+    w.y = v.y;
+    *(gmem_t<v2f> *)p = w;
+}
+__device__ __forceinline__ void gst(float4 *p, float4 v)
+{
+    v4f w;
+    w.x = v.x;
+    w.y = v.y;
+    w.z = v.z;
+    w.w = v.w;
+    *(gmem_t<v4f> *)p = w;
+}
+
+// Products written with explicit fused multiply-adds: the rounding is then
+// the same in every kernel that inlines them (a contracted a*b + c*d could
+// fuse either product, and the choice can differ between instantiations).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b)  // a * conj(b)
 {
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -40,14 +91,36 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // writes b[(q - k)*R + k + m*p]; the butterflies of all nb sequences are
 // spread over the block's threads, so nb sequences take the stages (and
 // barriers) of one.  INV selects the unscaled inverse.  All threads of the
-// block must call it.
+// block must call it; with NT = 64 the calling wave runs the sequences alone
+// (its own buffers, a wave barrier between stages) and only its 64 lanes
+// call it.
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
-template <int M, bool INV>
+// Orders one wave's LDS accesses: its lanes' stores before the barrier are
+// seen by its lanes' loads after it (a wave's LDS operations complete in
+// issue order; the fences keep the compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Barrier of the threads that share a piece of work: the wave (NT = 64) or
+// the workgroup.
+template <int NT>
+__device__ __forceinline__ void group_sync()
+{
+    if constexpr (NT == 64) wave_sync();
+    else __syncthreads();
+}
+
+template <int M, bool INV, int NT = 0>
 __device__ float2 *fft_lds_n(float2 *a, float2 *b, int nb, const float2 *__restrict__ tw)
 {
     constexpr int LQ = ilog2(M / 4), LH = ilog2(M / 2);  // butterflies per sequence: radix 4, radix 2
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = NT == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int nt = NT == 64 ? 64 : (int)blockDim.x;
     int p = 1;
 #pragma unroll 1
     for (; p * 4 <= M; p *= 4) {
@@ -76,7 +149,7 @@ __device__ float2 *fft_lds_n(float2 *a, float2 *b, int nb, const float2 *__restr
             bj[j + 2 * p] = csub(s02, s13);
             bj[j + 3 * p] = csub(d02, jd13);
         }
-        __syncthreads();
+        group_sync<NT>();
         float2 *t = a; a = b; b = t;
     }
     if (p < M) {  // radix-2, p = M/2
@@ -91,23 +164,24 @@ __device__ float2 *fft_lds_n(float2 *a, float2 *b, int nb, const float2 *__restr
             bj[q] = cadd(x0, x1);
             bj[q + M / 2] = csub(x0, x1);
         }
-        __syncthreads();
+        group_sync<NT>();
         a = b;
     }
     return a;
 }
 
 // One M-point FFT (see fft_lds_n).
-template <int M, bool INV>
+template <int M, bool INV, int NT = 0>
 __device__ __forceinline__ float2 *fft_lds(float2 *a, float2 *b, const float2 *__restrict__ tw)
 {
-    return fft_lds_n<M, INV>(a, b, 1, tw);
+    return fft_lds_n<M, INV, NT>(a, b, 1, tw);
 }
 
 // Real-FFT post-processing: Z = FFT_M(z), z[n] = x[2n] + i x[2n+1].
-// Returns packed bin k of the 2M-point real FFT (k = 0: (X[0], X[M])).
+// Returns packed bin k of the 2M-point real FFT (k = 0: (X[0], X[M])); w is
+// the 2M-point twiddle of bin k (unused for k = 0).
 template <int M>
-__device__ __forceinline__ float2 r2c_post(const float2 *Z, int k, const float2 *__restrict__ twN)
+__device__ __forceinline__ float2 r2c_post_w(const float2 *Z, int k, float2 w)
 {
     const float2 z0 = Z[k];
     if (k == 0) return make_float2(z0.x + z0.y, z0.x - z0.y);
@@ -115,24 +189,35 @@ __device__ __forceinline__ float2 r2c_post(const float2 *Z, int k, const float2 
     const float er = 0.5f * (z0.x + zc.x), ei = 0.5f * (z0.y - zc.y);
     const float dr = z0.x - zc.x, di = z0.y + zc.y;   // Z[k] - conj(Z[M-k])
     const float orr = 0.5f * di, oi = -0.5f * dr;     // (Z - conj)/(2i)
-    const float2 w = twN[k];
-    return make_float2(er + (w.x * orr - w.y * oi), ei + (w.x * oi + w.y * orr));
+    return make_float2(er + fmaf(w.x, orr, -(w.y * oi)), ei + fmaf(w.x, oi, w.y * orr));
+}
+
+template <int M>
+__device__ __forceinline__ float2 r2c_post(const float2 *Z, int k, const float2 *__restrict__ twN)
+{
+    return k == 0 ? r2c_post_w<M>(Z, 0, make_float2(1.f, 0.f)) : r2c_post_w<M>(Z, k, twN[k]);
 }
 
 // Inverse pre-processing from a packed spectrum A (element 0 = (DC, Nyquist)):
 // returns Z''[k] = 2 (E[k] + i O[k]) so that IFFT_M(Z'')/(2M) interleaves the
-// even/odd output samples.
+// even/odd output samples; w as in r2c_post_w.
 template <int M>
-__device__ __forceinline__ float2 c2r_pre(const float2 *A, int k, const float2 *__restrict__ twN)
+__device__ __forceinline__ float2 c2r_pre_w(const float2 *A, int k, float2 w)
 {
     const float2 a = A[k];
     if (k == 0) return make_float2(a.x + a.y, a.x - a.y);
     const float2 c = A[M - k];                        // conj(c) used
     const float er = a.x + c.x, ei = a.y - c.y;
     const float dr = a.x - c.x, di = a.y + c.y;       // X[k] - conj(X[M-k])
-    const float2 w = twN[k];                          // multiply by conj(w)
-    const float orr = dr * w.x + di * w.y, oi = di * w.x - dr * w.y;
+    // multiply by conj(w)
+    const float orr = fmaf(dr, w.x, di * w.y), oi = fmaf(di, w.x, -(dr * w.y));
     return make_float2(er - oi, ei + orr);
+}
+
+template <int M>
+__device__ __forceinline__ float2 c2r_pre(const float2 *A, int k, const float2 *__restrict__ twN)
+{
+    return k == 0 ? c2r_pre_w<M>(A, 0, make_float2(1.f, 0.f)) : c2r_pre_w<M>(A, k, twN[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -140,7 +225,7 @@ __device__ __forceinline__ float2 c2r_pre(const float2 *A, int k, const float2 *
 // ---------------------------------------------------------------------------
 constexpr int kTickThreads = 256;
 constexpr int kHeadChunkBatch = 16;  // partial-sum rows loaded together by the head
-constexpr int kHeadPreload = 4;      // near partitions a head loads with its partial sums
+constexpr int kHeadPreload = 4;      // near partitions a head loads together
 
 // float4 of LDS a tail or pass role reduces its lane groups' sums through:
 // [filter (<= 2)][lane group G][row float4 H] per round, G * H = 256 * V
@@ -152,17 +237,16 @@ constexpr int tail_red_float4() { return 2 * kTickThreads * ((M / 2 > 256) ? (M 
 // over an object whose rows fit in the Infinity Cache reads them with
 // ordinary loads instead (ld_row<false>), so they stay resident from one
 // launch to the next.
-typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ld_stream(const float4 *p)
 {
-    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+    const v4f v = __builtin_nontemporal_load((const gmem_t<v4f> *)p);
     return make_float4(v.x, v.y, v.z, v.w);
 }
 template <bool NT>
 __device__ __forceinline__ float4 ld_row(const float4 *p)
 {
     if constexpr (NT) return ld_stream(p);
-    return *p;
+    return gld(p);
 }
 
 typedef unsigned int v2u __attribute__((ext_vector_type(2)));
@@ -179,10 +263,10 @@ struct IrRow;
 template <>
 struct IrRow<IR_F32> {
     static constexpr int kBytes = 8;
-    __device__ static float2 get1(const void *row, int k) { return reinterpret_cast<const float2 *>(row)[k]; }
+    __device__ static float2 get1(const void *row, int k) { return gld(reinterpret_cast<const float2 *>(row) + k); }
     // complex 2q and 2q+1, streamed (getc: ordinary load)
     __device__ static float4 get2(const void *row, int q) { return ld_stream(reinterpret_cast<const float4 *>(row) + q); }
-    __device__ static float4 getc(const void *row, int q) { return reinterpret_cast<const float4 *>(row)[q]; }
+    __device__ static float4 getc(const void *row, int q) { return gld(reinterpret_cast<const float4 *>(row) + q); }
 };
 
 template <>
@@ -190,17 +274,17 @@ struct IrRow<IR_F16> {
     static constexpr int kBytes = 4;
     __device__ static float2 get1(const void *row, int k)
     {
-        const unsigned int w = reinterpret_cast<const unsigned int *>(row)[k];
+        const unsigned int w = gld(reinterpret_cast<const unsigned int *>(row) + k);
         return make_float2(half_lo(w), half_hi(w));
     }
     __device__ static float4 get2(const void *row, int q)
     {
-        const v2u w = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(row) + q);
+        const v2u w = __builtin_nontemporal_load((const gmem_t<v2u> *)row + q);
         return make_float4(half_lo(w.x), half_hi(w.x), half_lo(w.y), half_hi(w.y));
     }
     __device__ static float4 getc(const void *row, int q)
     {
-        const v2u w = reinterpret_cast<const v2u *>(row)[q];
+        const v2u w = gld(reinterpret_cast<const v2u *>(row) + q);
         return make_float4(half_lo(w.x), half_hi(w.x), half_lo(w.y), half_hi(w.y));
     }
 };
@@ -210,17 +294,17 @@ struct IrRow<IR_BF16> {
     static constexpr int kBytes = 4;
     __device__ static float2 get1(const void *row, int k)
     {
-        const unsigned int w = reinterpret_cast<const unsigned int *>(row)[k];
+        const unsigned int w = gld(reinterpret_cast<const unsigned int *>(row) + k);
         return make_float2(bf_lo(w), bf_hi(w));
     }
     __device__ static float4 get2(const void *row, int q)
     {
-        const v2u w = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(row) + q);
+        const v2u w = __builtin_nontemporal_load((const gmem_t<v2u> *)row + q);
         return make_float4(bf_lo(w.x), bf_hi(w.x), bf_lo(w.y), bf_hi(w.y));
     }
     __device__ static float4 getc(const void *row, int q)
     {
-        const v2u w = reinterpret_cast<const v2u *>(row)[q];
+        const v2u w = gld(reinterpret_cast<const v2u *>(row) + q);
         return make_float4(bf_lo(w.x), bf_hi(w.x), bf_lo(w.y), bf_hi(w.y));
     }
 };
@@ -238,7 +322,7 @@ __device__ __forceinline__ float ir_scale(const float *S, int c, int P, int i)
     if constexpr (HT == IR_F32) {
         return 1.0f;
     } else {
-        return S[(size_t)c * P + i];
+        return gld(S + (size_t)c * P + i);
     }
 }
 
@@ -271,39 +355,28 @@ __device__ __forceinline__ void mac4(float4 &acc, float4 x, float4 h, float s, b
     acc.w = fmaf(x.w, h.z, acc.w);
 }
 
-// Sum of a head's partial-sum rows T[j*M + k], j < tchunk, loaded
-// kHeadChunkBatch rows at a time (every load of a batch in flight at once).
-template <int M>
-__device__ __forceinline__ float2 sum_partials(const float2 *T, int tchunk, int k)
-{
-    float2 s = make_float2(0.f, 0.f);
-    for (int j = 0; j < tchunk; j += kHeadChunkBatch) {
-        float2 t[kHeadChunkBatch];
-#pragma unroll
-        for (int u = 0; u < kHeadChunkBatch; u++)
-            t[u] = (j + u < tchunk) ? T[(size_t)(j + u) * M + k] : make_float2(0.f, 0.f);
-#pragma unroll
-        for (int u = 0; u < kHeadChunkBatch; u++) {
-            s.x += t[u].x;
-            s.y += t[u].y;
-        }
-    }
-    return s;
-}
-
+// LDS of one head: the FFT's two buffers and the M-point twiddles.  The
+// bins a thread owns (k = tid + q * NT) stay in its registers between the
+// transforms: the forward spectrum, the 2M-point twiddles and the output.
 template <int M>
 struct HeadSmem {
-    float2 s0[M], s1[M], sx[M], sa[M], tw[M], twn[M];
-    float ya[M];
+    float2 s0[M], s1[M], tw[M];
 };
 
-template <int M, int HT>
+// Head of channel c, run by NT threads: one wave (NT = 64, with its own
+// HeadSmem and wave barriers) or the whole workgroup (NT = kTickThreads).
+template <int M, int HT, int NT>
 __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
 {
-    constexpr int NT = kTickThreads;
     constexpr int KPT = M >= NT ? M / NT : 1;  // bins per thread
+    constexpr int ZP = (M / 2 + NT - 1) / NT;  // output sample pairs per thread
     constexpr float kScale = 1.0f / (2.0f * (float)M);
-    const int tid = threadIdx.x;
+    // rows per memory round trip: partial sums (CB) and near partitions
+    // (NB); a wave's lane holds KPT bins of each row
+    constexpr int CB = NT == 64 ? 2 : kHeadChunkBatch;
+    constexpr int NB = NT == 64 ? 2 : kHeadPreload;
+    static_assert(M % NT == 0 || NT > M, "bins per thread");
+    const int tid = NT == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
     const int nf = a.xfade ? 2 : 1;
     // the head is the launch's longest chain: its waves issue before the
     // tail's on a shared compute unit
@@ -314,12 +387,17 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
     const float *xi = a.xin + (size_t)c * a.in_stride;
     for (int n = tid; n < M; n += NT) {
         const float *src = n < M / 2 ? xp + 2 * n : xi + 2 * (n - M / 2);
-        sm.s0[n] = make_float2(src[0], src[1]);
-        sm.tw[n] = a.twM[n];
-        sm.twn[n] = a.twN[n];
+        sm.s0[n] = make_float2(gld(src), gld(src + 1));
+        sm.tw[n] = gld(a.twM + n);
     }
+    float2 wn[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; q++) wn[q] = gld(a.twN + min(tid + q * NT, M - 1));
     const int dmax = max(a.dA, nf == 2 ? a.dB : 1);  // 1: partition 1 only
     float2 acc[2][KPT], h0[2][KPT];
+    // bin of slot q, clamped for the loads of threads past M (NT > M:
+    // their values are never used)
+    auto bin = [&](int q) { return min(tid + q * NT, M - 1); };
 #pragma unroll
     for (int f = 0; f < 2; f++) {
         if (f >= nf) continue;
@@ -328,45 +406,65 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
         const void *H0row = ir_row<HT>(Hbase, c, a.P, 0, M);
         const float s0 = ir_scale<HT>(Sbase, c, a.P, 0);
         const float2 *T = (f == 0 ? a.Ta : a.Tb) + (size_t)c * a.tchunk * M;
+        // partition 0 and the partial sums: CB rows per memory round trip
+        // (clamped row indices keep every load in range; rows past tchunk
+        // add zero)
+        float2 s[KPT];
 #pragma unroll
         for (int q = 0; q < KPT; q++) {
-            const int k = tid + q * NT;
-            float2 s = make_float2(0.f, 0.f);
-            if (k < M) {
-                // the near rows' loads go out with the partial sums' (one
-                // memory round trip): partitions 1..kHeadPreload from clamped
-                // indices, used up to d
-                const int d = dmax == 1 ? 1 : min(f == 0 ? a.dA : a.dB, a.P - 1);
-                float2 fn[kHeadPreload], hn[kHeadPreload];
-#pragma unroll
-                for (int jn = 1; jn <= kHeadPreload; jn++) {
-                    const int jj = a.P > 1 ? max(1, min(jn, min(d, a.P - 1))) : 0;  // in range, P = 1 too
-                    fn[jn - 1] = a.fdl[((size_t)c * a.P + ((a.slot - jj) & (a.P - 1))) * M + k];
-                    const float sj = ir_scale<HT>(Sbase, c, a.P, jj);
-                    const float2 hv = IrRow<HT>::get1(ir_row<HT>(Hbase, c, a.P, jj, M), k);
-                    hn[jn - 1] = make_float2(hv.x * sj, hv.y * sj);
-                }
-                const float2 h0v = IrRow<HT>::get1(H0row, k);
-                s = sum_partials<M>(T, a.tchunk, k);
-                // partitions d..1, descending (the order heads_role keeps):
-                // the blocks since the launch whose pass computed the partials
-                for (int jn = kNearMax; jn > kHeadPreload; jn--) {
-                    if (jn > d) continue;
-                    const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
-                    const float sj = ir_scale<HT>(Sbase, c, a.P, jn);
-                    const float2 hv = IrRow<HT>::get1(ir_row<HT>(Hbase, c, a.P, jn, M), k);
-                    mac2(s, Fj[k], make_float2(hv.x * sj, hv.y * sj), k == 0);
-                }
-#pragma unroll
-                for (int jn = kHeadPreload; jn >= 1; jn--)
-                    if (jn <= d && a.P > 1) mac2(s, fn[jn - 1], hn[jn - 1], k == 0);
-                h0[f][q] = make_float2(h0v.x * s0, h0v.y * s0);
-            }
-            acc[f][q] = s;
+            const float2 h0v = IrRow<HT>::get1(H0row, bin(q));
+            h0[f][q] = make_float2(h0v.x * s0, h0v.y * s0);
+            s[q] = make_float2(0.f, 0.f);
         }
+        for (int j = 0; j < a.tchunk; j += CB) {
+            float2 t[CB][KPT];
+#pragma unroll
+            for (int u = 0; u < CB; u++)
+#pragma unroll
+                for (int q = 0; q < KPT; q++) t[u][q] = gld(T + (size_t)min(j + u, a.tchunk - 1) * M + bin(q));
+#pragma unroll
+            for (int u = 0; u < CB; u++)
+#pragma unroll
+                for (int q = 0; q < KPT; q++) {
+                    const bool in = j + u < a.tchunk;
+                    s[q].x += in ? t[u][q].x : 0.f;
+                    s[q].y += in ? t[u][q].y : 0.f;
+                }
+        }
+        // partitions d..1, descending (the order heads_role keeps): the
+        // blocks since the launch whose pass computed the partials, NB rows
+        // per memory round trip
+        const int d = a.P > 1 ? (dmax == 1 ? 1 : min(f == 0 ? a.dA : a.dB, a.P - 1)) : 0;
+        for (int jb = d; jb >= 1; jb -= NB) {
+            float2 fr[NB][KPT], hr[NB][KPT];
+            float sj[NB];
+#pragma unroll
+            for (int u = 0; u < NB; u++) {
+                const int jn = max(jb - u, 1);
+                const float2 *Fj = a.fdl + ((size_t)c * a.P + ((a.slot - jn) & (a.P - 1))) * M;
+                const void *Hj = ir_row<HT>(Hbase, c, a.P, jn, M);
+                sj[u] = ir_scale<HT>(Sbase, c, a.P, jn);
+#pragma unroll
+                for (int q = 0; q < KPT; q++) {
+                    fr[u][q] = gld(Fj + bin(q));
+                    hr[u][q] = IrRow<HT>::get1(Hj, bin(q));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NB; u++) {
+                if (jb - u >= 1) {
+#pragma unroll
+                    for (int q = 0; q < KPT; q++) {
+                        const int k = tid + q * NT;
+                        mac2(s[q], fr[u][q], make_float2(hr[u][q].x * sj[u], hr[u][q].y * sj[u]), k == 0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KPT; q++) acc[f][q] = s[q];
     }
     // non-uniform objects: the frame outputs for this block (read early)
-    constexpr int ZP = (M / 2 + NT - 1) / NT;  // sample pairs per thread
     float2 za[ZP], zb[ZP];
 #pragma unroll
     for (int q = 0; q < ZP; q++) {
@@ -374,46 +472,57 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
         zb[q] = make_float2(0.f, 0.f);
         const int n = tid + q * NT;
         if (n < M / 2) {
-            if (a.zA) za[q] = reinterpret_cast<const float2 *>(a.zA + (size_t)c * a.N + a.zoff)[n];
-            if (a.zB) zb[q] = reinterpret_cast<const float2 *>(a.zB + (size_t)c * a.N + a.zoff)[n];
+            if (a.zA) za[q] = gld(reinterpret_cast<const float2 *>(a.zA + (size_t)c * a.N + a.zoff) + n);
+            if (a.zB) zb[q] = gld(reinterpret_cast<const float2 *>(a.zB + (size_t)c * a.N + a.zoff) + n);
         }
     }
-    __syncthreads();
+    group_sync<NT>();
     if (tid < M) {
         // this block becomes the previous one of the next frame
         float *xpw = a.xprev + (size_t)c * M;
-        for (int j = tid; j < M; j += NT) xpw[j] = xi[j];
+        for (int j = tid; j < M; j += NT) gst(xpw + j, gld(xi + j));
         if (a.nu) {
             float *xr = a.xring + ((size_t)c * 2 * a.M + a.xslot) * M;
-            for (int j = tid; j < M; j += NT) xr[j] = xi[j];
+            for (int j = tid; j < M; j += NT) gst(xr + j, gld(xi + j));
         }
     }
 
-    const float2 *Z = fft_lds<M, false>(sm.s0, sm.s1, sm.tw);
+    const float2 *Z = fft_lds<M, false, NT>(sm.s0, sm.s1, sm.tw);
     float2 *row = a.fdl + ((size_t)c * a.P + a.slot) * M;
-    for (int k = tid; k < M; k += NT) {
-        const float2 X = r2c_post<M>(Z, k, sm.twn);
-        sm.sx[k] = X;
-        row[k] = X;
+    float2 X[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+        const int k = tid + q * NT;
+        if (k < M) {
+            X[q] = r2c_post_w<M>(Z, k, wn[q]);
+            gst(row + k, X[q]);
+        }
     }
-    __syncthreads();
+    group_sync<NT>();
 
+    // output samples 2n, 2n + 1 for n = tid + q * NT < M / 2
+    float2 y[ZP];
 #pragma unroll
     for (int f = 0; f < 2; f++) {
         if (f >= nf) continue;
+        // the spectrum to invert, into s1; its C2R input into s0
 #pragma unroll
         for (int q = 0; q < KPT; q++) {
             const int k = tid + q * NT;
             if (k < M) {
                 float2 v = acc[f][q];
-                mac2(v, sm.sx[k], h0[f][q], k == 0);
-                sm.sa[k] = v;
+                mac2(v, X[q], h0[f][q], k == 0);
+                sm.s1[k] = v;
             }
         }
-        __syncthreads();
-        for (int k = tid; k < M; k += NT) sm.s0[k] = c2r_pre<M>(sm.sa, k, sm.twn);
-        __syncthreads();
-        const float2 *z = fft_lds<M, true>(sm.s0, sm.s1, sm.tw);
+        group_sync<NT>();
+#pragma unroll
+        for (int q = 0; q < KPT; q++) {
+            const int k = tid + q * NT;
+            if (k < M) sm.s0[k] = c2r_pre_w<M>(sm.s1, k, wn[q]);
+        }
+        group_sync<NT>();
+        const float2 *z = fft_lds<M, true, NT>(sm.s0, sm.s1, sm.tw);
         // second half of the 2M output samples: z[M/2 .. M)
         if (f == 0) {
 #pragma unroll
@@ -421,8 +530,8 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                 const int n = tid + q * NT;
                 if (n < M / 2) {
                     const float2 v = z[M / 2 + n];
-                    sm.ya[2 * n] = v.x * kScale + za[q].x;
-                    sm.ya[2 * n + 1] = v.y * kScale + za[q].y;
+                    y[q].x = fmaf(v.x, kScale, za[q].x);
+                    y[q].y = fmaf(v.y, kScale, za[q].y);
                 }
             }
         } else {
@@ -433,16 +542,43 @@ __device__ void head_role(const TickArgs &a, int c, HeadSmem<M> &sm)
                 if (n < M / 2) {
                     const float2 v = z[M / 2 + n];
                     const int j0 = 2 * n, j1 = 2 * n + 1;
-                    const float y0 = v.x * kScale + zb[q].x, y1 = v.y * kScale + zb[q].y;
-                    sm.ya[j0] = y0 * (step * (float)j0) + sm.ya[j0] * (step * (float)(M - j0));
-                    sm.ya[j1] = y1 * (step * (float)j1) + sm.ya[j1] * (step * (float)(M - j1));
+                    const float y0 = fmaf(v.x, kScale, zb[q].x), y1 = fmaf(v.y, kScale, zb[q].y);
+                    y[q].x = fmaf(y0, step * (float)j0, y[q].x * (step * (float)(M - j0)));
+                    y[q].y = fmaf(y1, step * (float)j1, y[q].y * (step * (float)(M - j1)));
                 }
             }
         }
-        __syncthreads();
+        // the next filter's stores into s0 / s1 come after every lane's reads
+        group_sync<NT>();
     }
     float *yo = a.yout + (size_t)c * a.out_stride;
-    for (int j = tid; j < M; j += NT) yo[j] = sm.ya[j];
+#pragma unroll
+    for (int q = 0; q < ZP; q++) {
+        const int n = tid + q * NT;
+        if (n < M / 2) {
+            gst(yo + 2 * n, y[q].x);
+            gst(yo + 2 * n + 1, y[q].y);
+        }
+    }
+}
+
+// Head workgroup b: channel b, or with WAVE the wave_heads(M) channels from
+// wave_heads(M) * b, one per wave on its own HeadSmem.
+template <int M, int HT, bool WAVE>
+__device__ __forceinline__ void head_block(const TickArgs &a, int b, HeadSmem<M> *hs)
+{
+    constexpr int HPB = WAVE ? wave_heads(M) : 1;
+    if constexpr (HPB > 1) {
+        static_assert(HPB == kTickThreads / 64, "one head per wave");
+        // the wave index as a uniform value: the channel's row addresses
+        // stay in scalar registers
+        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), c = b * HPB + w;
+        if (c >= a.nhead || !a.run[c]) return;  // wave-uniform: no workgroup barrier follows
+        head_role<M, HT, 64>(a, c, hs[w]);
+    } else {
+        if (!a.run[b]) return;
+        head_role<M, HT, kTickThreads>(a, b, *hs);
+    }
 }
 
 // Rows of one channel for a partial sum over partitions [i0, i1): partition
@@ -467,7 +603,7 @@ template <int HT>
 __device__ __forceinline__ float row_scale(const float *S, int i)
 {
     if constexpr (HT == IR_F32) return 1.f;
-    else return S[i];
+    else return gld(S + i);
 }
 
 // 256 threads; a row of B complex is read as float4 (two complex) per lane,
@@ -785,8 +921,7 @@ __global__ __launch_bounds__(kTickThreads) void k_tick(TickArgs a)
     __shared__ float4 smem[(kBytes + 15) / 16];
     const int b = blockIdx.x;
     if (b < a.nhead) {
-        if (!a.run[b]) return;
-        head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
+        head_block<M, HT, false>(a, b, reinterpret_cast<HeadSmem<M> *>(smem));
     } else if (TK != 0 && b < a.nhead + a.C * a.nchunk) {
         const int t = b - a.nhead;
         const int c = t / a.nchunk, j = t % a.nchunk;
@@ -808,8 +943,7 @@ template <int M, int HT, int ROLES, int KM = kPassMax>
 __device__ __forceinline__ void tick_body(const TickArgs &a, int b, float4 *smem)
 {
     if (b < a.nhead) {
-        if (!a.run[b]) return;
-        head_role<M, HT>(a, b, *reinterpret_cast<HeadSmem<M> *>(smem));
+        head_block<M, HT, false>(a, b, reinterpret_cast<HeadSmem<M> *>(smem));
     } else {
         const int t = b - a.nhead;
         if (t >= a.C * a.nchunk) {
@@ -868,6 +1002,29 @@ __global__ __launch_bounds__(kTickThreads) void k_group_tick(const TickArgs *__r
     const TickArgs a = descs[lo];
     if (a.B != M) return;  // the host launches each object with its own block size
     tick_body<M, HT, ROLES, KM>(a, b - starts[lo], smem);
+}
+
+// Heads-only group launch: the objects of block size M without a pass, tail
+// or frame slice this tick, wave_heads(M) heads per workgroup.  Without the
+// pass roles' registers and with four heads per workgroup the heads' FFT
+// chains overlap four times as many memory round trips per compute unit;
+// at least 4 waves per SIMD (at most 128 VGPRs) keep 16 heads per compute
+// unit in flight.
+template <int HT, int M>
+__global__ __launch_bounds__(kTickThreads, 4) void k_group_heads(const TickArgs *__restrict__ descs,
+                                                                 const int *__restrict__ starts, int nobj)
+{
+    __shared__ HeadSmem<M> sm[wave_heads(M)];
+    const int b = blockIdx.x;
+    int lo = 0, hi = nobj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TickArgs a = descs[lo];
+    if (a.B != M) return;
+    head_block<M, HT, true>(a, b - starts[lo], sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1507,7 +1664,7 @@ __global__ __launch_bounds__(256) void k_fft4_r2c_post(const float2 *Z, long lon
         const float dr = z0.x - zc.x, di = z0.y + zc.y;
         const float orr = 0.5f * di, oi = -0.5f * dr;
         const float2 w = tw_fp64((unsigned long long)k, logM + 1, false);
-        X = make_float2(er + (w.x * orr - w.y * oi), ei + (w.x * oi + w.y * orr));
+        X = make_float2(er + fmaf(w.x, orr, -(w.y * oi)), ei + fmaf(w.x, oi, w.y * orr));
     }
     if (mode == 2) {
         o[2 * k] = X.x;
@@ -1543,7 +1700,7 @@ __global__ __launch_bounds__(256) void k_fft4_c2r_pre(const float *in, long long
         const float er = a.x + b.x, ei = a.y - b.y;
         const float dr = a.x - b.x, di = a.y + b.y;
         const float2 w = tw_fp64((unsigned long long)k, logM + 1, false);
-        const float orr = dr * w.x + di * w.y, oi = di * w.x - dr * w.y;
+        const float orr = fmaf(dr, w.x, di * w.y), oi = fmaf(di, w.x, -(dr * w.y));
         r = make_float2(er - oi, ei + orr);
     }
     A[(size_t)c * as + k] = r;
@@ -1875,6 +2032,7 @@ static hipError_t launch_group_m(int roles, int km, const TickArgs *d, const int
         break;
     case kRolesLean: TANX_GROUP_KM(kRolesLean); break;
     case kRolesXfade: TANX_GROUP_KM(kRolesXfade); break;
+    case kRolesHeads: hipExtLaunchKernelGGL((k_group_heads<HT, M>), g, t, 0, s, ev0, ev1, 0, d, st, nobj); break;
     default: return hipErrorInvalidValue;
     }
 #undef TANX_GROUP_KM

@@ -167,6 +167,8 @@ void *dynamic_smem();
 // wave collectives: every live lane of the wave contributes `in`; returns
 // the inputs of all 64 lanes (index by lane; lanes that have exited hold 0)
 const void *wave_exchange(const void *in, size_t bytes);
+// parks the lane until every live lane of its wave has arrived
+void wave_barrier();
 int lane_id();
 }  // namespace sim
 
@@ -213,6 +215,11 @@ inline T atomicAdd(T *p, T v)
 }
 
 #define __builtin_amdgcn_s_setprio(x) ((void)0)
+#define __builtin_amdgcn_wave_barrier() ::sim::wave_barrier()
+// used on wave-uniform values only
+#define __builtin_amdgcn_readfirstlane(x) (x)
+// fibers of one OS thread see each other's stores at once
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
 // v_mfma_f32_16x16x32_bf16 on the wave (lane layout as the library documents it)
 #define __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, C, cbsz, abid, blgp) sim_mfma_16x16x32_bf16(A, B, C)
 typedef __bf16 sim_bf16x8 __attribute__((ext_vector_type(8)));

@@ -10,7 +10,7 @@
 //
 // Launches run synchronously: workgroup by workgroup, each thread a fiber on
 // the calling OS thread.  __syncthreads() parks a fiber until every live
-// fiber of the block has arrived; a wave collective parks it until every
+// fiber of the block has arrived; a wave collective or wave barrier parks it until every
 // live lane of its 64-lane wave has.  A barrier no fiber can pass (threads
 // of a block waiting at different barriers) is reported as a deadlock
 // (exit status 87).  Streams and events are trivial; graph capture records
@@ -387,6 +387,15 @@ void syncthreads()
 void *dynamic_smem() { return g_dyn_smem; }
 
 int lane_id() { return g_cur & 63; }
+
+void wave_barrier()
+{
+    if (g_cur < 0) {
+        fprintf(stderr, "sim: wave barrier outside a kernel\n");
+        _exit(88);
+    }
+    park(BAR_WAVE);
+}
 
 const void *wave_exchange(const void *in, size_t bytes)
 {
