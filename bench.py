@@ -301,10 +301,12 @@ def bench_realtime(T, torch, steps, warmup, margin=0.95):
             log(f"  probe {probes[-1]}")
             if h.max() <= margin * DEADLINE_MS:
                 break
-            # tick time grows with the active groups: scale by the p99 tick
-            # (a lone outlier does not steer the search), at least one group
-            # and at most 30% down per probe
-            k = max(1, max(int(k * 0.7), min(k - 1, int(k * margin * DEADLINE_MS / p99))))
+            # tick time grows with the active groups, less than in proportion
+            # (a tick has a fixed part): step half way to the count the p99
+            # tick scales to (a lone outlier does not steer the search), at
+            # least one group and at most 30% down per probe
+            k_lin = int(k * margin * DEADLINE_MS / p99)
+            k = max(1, max(int(k * 0.7), min(k - 1, k - -(-(k - k_lin) // 2))))
         gc_.enable()
         result = None
         for attempt in range(12):
@@ -950,9 +952,15 @@ def summarize_trace(run_dir, leg):
     return out
 
 
+def is_group_kernel(name):
+    """A tick's group launch in a rocprofv3 kernel name: the pass and tail
+    kernels and the heads-only kernel."""
+    return name.startswith("void tanx::k_group_tick") or name.startswith("void tanx::k_group_heads")
+
+
 def pmc_traffic_per_channel():
     """HBM bytes per channel and tick of the real-time group launches (every
-    k_group_tick of a tick) from the committed rocprofv3 PMC summary
+    k_group_tick and k_group_heads of a tick) from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a 512 MiB copy), or None when it was
     taken with another pass length."""
     try:
@@ -996,7 +1004,7 @@ def summarize_profiles(run_dir, groups):
         tb = fk * 1024 * ff + wk * 1024 * wf
         rows.append({"kernel": name, "dispatches": len(fv), "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
                      "traffic_bytes": round(tb)})
-        if name.startswith("void tanx::k_group_tick"):
+        if is_group_kernel(name):
             tick_bytes += tb * len(fv)
     per_tick = tick_bytes / PROFILE_TICKS
     pmc = {"source": f"rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, of: python3 bench.py --profile-run "
@@ -1015,7 +1023,7 @@ def summarize_profiles(run_dir, groups):
     with open(PMC_JSON, "w") as fh:
         json.dump(pmc, fh, indent=1)
     stats = list(csv.DictReader(open(os.path.join(run_dir, "prof_rt", "run_kernel_stats.csv"))))
-    gt_ns = sum(float(r["TotalDurationNs"]) for r in stats if r["Name"].startswith("void tanx::k_group_tick"))
+    gt_ns = sum(float(r["TotalDurationNs"]) for r in stats if is_group_kernel(r["Name"]))
     lines = [f"# Kernel trace: real-time rig, {groups} groups x 2,048 channels ({channels:,} channels), "
              f"65,536 taps, B = 256, {RT_PASS}-block passes", "",
              "Command (from the repository root on the GPU box):",
@@ -1028,8 +1036,8 @@ def summarize_profiles(run_dir, groups):
         lines.append(f"{r['Name'].split('(')[0].replace('void ', '')[:64]:64s} {int(r['Calls']):6d} "
                      f"{float(r['AverageNs']) / 1e3:10.1f} {float(r['MinNs']) / 1e3:10.1f} "
                      f"{float(r['MaxNs']) / 1e3:10.1f} {float(r['Percentage']):7.2f}")
-    lines += ["", f"Group kernels per tick: {gt_ns / PROFILE_TICKS / 1e3:.1f} us (all k_group_tick launches of "
-              f"the {PROFILE_TICKS} ticks / {PROFILE_TICKS}).",
+    lines += ["", f"Group kernels per tick: {gt_ns / PROFILE_TICKS / 1e3:.1f} us (all k_group_tick and "
+              f"k_group_heads launches of the {PROFILE_TICKS} ticks / {PROFILE_TICKS}).",
               f"HBM traffic of the group kernels ({os.path.relpath(PMC_JSON, ROOT)}): {per_tick / 1e9:.2f} GB per "
               f"tick, {per_tick / channels / 1e3:.1f} KB per channel (FETCH_SIZE x {ff:.3f} + WRITE_SIZE x "
               f"{wf:.3f}, the factors of a 512 MiB copy)."]
@@ -1124,8 +1132,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": per_ch * value if per_ch else None,
-                "kernel": f"one tick's group launches: k_group_tick<0, 256, 1, 4> (the heads of the groups "
-                          f"without a pass this tick), k_group_tick<0, 256, 1, {RT_PASS}> (the groups whose "
+                "kernel": f"one tick's group launches: k_group_heads<0, 256> (the heads of the groups "
+                          f"without a pass this tick, one per wave), k_group_tick<0, 256, 1, {RT_PASS}> (the groups whose "
                           f"single-filter pass runs this tick, one in {RT_PASS}, and the new filter's catch-up "
                           f"pass of each group whose IR switched, with their heads), k_group_tick<0, 256, 2, "
                           f"{RT_PASS}> when a switch falls on a group's pass block (its two-filter pass)",

@@ -1357,15 +1357,15 @@ __device__ __forceinline__ void ir_transform_part(const IrArgs &a, int i, int y,
                                                   float *smax)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int ch = a.chan ? a.chan[y] : y;
+    const int ch = a.chan ? gld(a.chan + y) : y;
     const float *h = a.ir + (size_t)ch * a.stride;
     const long long base = (long long)i * M;
     for (int n = tid; n < M; n += nt) {
         float2 v = make_float2(0.f, 0.f);
         if (n < M / 2) {
             const long long k = base + 2 * n;
-            v.x = k < a.n ? h[k] : 0.f;
-            v.y = k + 1 < a.n ? h[k + 1] : 0.f;
+            v.x = k < a.n ? gld(h + k) : 0.f;
+            v.y = k + 1 < a.n ? gld(h + k + 1) : 0.f;
         }
         s0[n] = v;
     }
@@ -1373,7 +1373,7 @@ __device__ __forceinline__ void ir_transform_part(const IrArgs &a, int i, int y,
     const float2 *Z = fft_lds<M, false>(s0, s1, a.twM);
     if constexpr (HT == IR_F32) {
         float2 *row = reinterpret_cast<float2 *>(a.H) + ((size_t)ch * a.P + i) * M;
-        for (int k = tid; k < M; k += nt) row[k] = r2c_post<M>(Z, k, a.twN);
+        for (int k = tid; k < M; k += nt) gst(row + k, r2c_post<M>(Z, k, a.twN));
     } else {
         // spectrum into the free buffer, then the row's max |re|,|im|
         float2 *X = (Z == s0) ? s1 : s0;
@@ -1401,9 +1401,9 @@ __device__ __forceinline__ void ir_transform_part(const IrArgs &a, int i, int y,
                 lo = f2bf_rne(v.x * inv);
                 hi = f2bf_rne(v.y * inv);
             }
-            row[k] = lo | (hi << 16);
+            gst(row + k, lo | (hi << 16));
         }
-        if (tid == 0) a.S[(size_t)ch * a.P + i] = sc;
+        if (tid == 0) gst(a.S + (size_t)ch * a.P + i, sc);
     }
 }
 
@@ -1452,7 +1452,7 @@ __device__ __forceinline__ void ir_transform_rows_part(const IrArgs &a, int xb, 
                                                        float2 *tw, float2 *twn)
 {
     const int i0 = xb * NR, tid = threadIdx.x, nt = blockDim.x;
-    const int ch = a.chan ? a.chan[y] : y;
+    const int ch = a.chan ? gld(a.chan + y) : y;
     const int nr = min(NR, a.P - i0);
     const float *h = a.ir + (size_t)ch * a.stride;
     const bool al = (reinterpret_cast<uintptr_t>(h) & 15) == 0;
@@ -1464,12 +1464,12 @@ __device__ __forceinline__ void ir_transform_rows_part(const IrArgs &a, int xb, 
         if (r < nr) {
             const long long k = (long long)(i0 + r) * M + 4 * q;
             if (al && k + 3 < a.n) {
-                v = *reinterpret_cast<const float4 *>(h + k);
+                v = gld(reinterpret_cast<const float4 *>(h + k));
             } else {
-                v.x = k < a.n ? h[k] : 0.f;
-                v.y = k + 1 < a.n ? h[k + 1] : 0.f;
-                v.z = k + 2 < a.n ? h[k + 2] : 0.f;
-                v.w = k + 3 < a.n ? h[k + 3] : 0.f;
+                v.x = k < a.n ? gld(h + k) : 0.f;
+                v.y = k + 1 < a.n ? gld(h + k + 1) : 0.f;
+                v.z = k + 2 < a.n ? gld(h + k + 2) : 0.f;
+                v.w = k + 3 < a.n ? gld(h + k + 3) : 0.f;
             }
         }
         float2 *row = s0 + r * M;
@@ -1479,15 +1479,15 @@ __device__ __forceinline__ void ir_transform_rows_part(const IrArgs &a, int xb, 
         row[M / 2 + 2 * q + 1] = make_float2(0.f, 0.f);
     }
     for (int m = tid; m < M; m += nt) {
-        tw[m] = a.twM[m];
-        twn[m] = a.twN[m];
+        tw[m] = gld(a.twM + m);
+        twn[m] = gld(a.twN + m);
     }
     __syncthreads();
     const float2 *Z = fft_lds_n<M, false>(s0, s1, nr, tw);
     float2 *rows = reinterpret_cast<float2 *>(a.H) + ((size_t)ch * a.P + i0) * M;
     for (int e = tid; e < nr * M; e += nt) {
         const int r = e / M, k = e - r * M;
-        rows[e] = r2c_post<M>(Z + r * M, k, twn);
+        gst(rows + e, r2c_post<M>(Z + r * M, k, twn));
     }
 }
 
