@@ -265,7 +265,7 @@ class RealTimeRig:
         self.torch.cuda.empty_cache()
 
 
-def bench_realtime(T, torch, steps, warmup, margin=0.95):
+def bench_realtime(T, torch, steps, warmup, margin=0.97):
     """Largest channel count whose every tick of `steps` x 100 consecutive
     ticks (IR updates and crossfades included) finishes within the 5.333 ms
     block period with one block in flight.
@@ -518,13 +518,24 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
         batch = Bt.StreamBatch(ctx, specs)
         period = batch.base_period_ms
         cyc = batch.cycle
+        batch.time_ticks(cyc * 16)  # past the first passes and the IRs' fade-in
         before = batch.time_ticks(cyc * 64) / (cyc * 64)
         t0 = time.perf_counter()
         layouts = batch.tune_layouts(cycles=8)
         after_layout = batch.time_ticks(cyc * 64) / (cyc * 64)
         chosen = batch.tune(cycles=8)
-        tune_s = time.perf_counter() - t0
         after = batch.time_ticks(cyc * 64) / (cyc * 64)
+        kept_default = False
+        if after > before:
+            # the tuned batch measured slower than the library's choices
+            # (ticks near their host enqueue time): keep the defaults
+            batch.restore_defaults()
+            batch.time_ticks(cyc * 8)
+            after = batch.time_ticks(cyc * 64) / (cyc * 64)
+            layouts = {k: None for k in layouts}
+            chosen = {k: 0 for k in chosen}
+            kept_default = True
+        tune_s = time.perf_counter() - t0
         # steady stream of ticks: device time over the whole run, and the
         # launch-stamped group kernels with the bytes their roles streamed
         ctx.EnableGroupTiming(True)
@@ -539,7 +550,9 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
                "objects_per_launch": [len(d) for d in batch.due],
                "tick_ms_default": before, "tick_ms_layouts_tuned": after_layout, "tick_ms_tuned": after,
                "tune_seconds": tune_s,
-               "tuned_layouts": {f"B{b}_L{ln}": f"{k} {v}" for (b, ln), (k, v) in sorted(layouts.items())},
+               "kept_default": kept_default,
+               "tuned_layouts": {f"B{b}_L{ln}": (f"{v[0]} {v[1]}" if v else "default")
+                                 for (b, ln), v in sorted(layouts.items())},
                "tuned_chunks": {f"B{b}_L{ln}": c for (b, ln), c in sorted(chosen.items())},
                "mean_tick_ms": total_ms / ticks, "p50_tick_ms": _pct(per, 50), "p99_tick_ms": _pct(per, 99),
                "max_tick_ms": float(max(per)), "deadline_misses": int(sum(1 for v in per if v > period)),
@@ -547,7 +560,8 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
                "kernel_ms_per_tick": kms / ticks, "alg_MB_per_tick": (gt["tick_alg_bytes"] + gt["frame_alg_bytes"])
                / ticks / 1e6, "realtime_factor": period / (total_ms / ticks)}
         log(f"  batching {n} streams: tick {row['mean_tick_ms'] * 1e3:.1f} us (default "
-            f"{before * 1e3:.1f} us, layouts tuned {after_layout * 1e3:.1f} us), p99 "
+            f"{before * 1e3:.1f} us, layouts tuned {after_layout * 1e3:.1f} us, tuned {after * 1e3:.1f} us"
+            f"{', defaults kept' if kept_default else ''}), p99 "
             f"{row['p99_tick_ms'] * 1e3:.1f} us, {gb:.0f} GB/s, layouts {row['tuned_layouts']}")
         rows.append(row)
         for c in batch.convs:

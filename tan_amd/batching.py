@@ -295,53 +295,69 @@ class StreamBatch:
             if rc != T.AMF_OK:
                 raise RuntimeError(f"SetTailChunks failed: AMF_RESULT {rc}")
 
-    def tune_layouts(self, cycles=4):
+    def _best_of(self, n, reps=3):
+        """Least device time of `reps` runs of n ticks (ms)."""
+        return min(self.time_ticks(n) for _ in range(reps))
+
+    def tune_layouts(self, cycles=4, min_gain=0.03):
         """Coordinate descent over buckets (largest byte share first): each
-        bucket tries every partition layout of layout_candidates with the
-        others fixed and keeps the one with the least device time over
-        `cycles` full tick cycles.  Returns {bucket: layout}."""
+        bucket measures its current layout, then every partition layout of
+        layout_candidates with the others fixed, and switches only to one
+        whose device time over `cycles` full tick cycles is at least
+        `min_gain` lower (a tick near its host enqueue time measures with
+        noise of a few percent).  Returns {bucket: layout} (None: the
+        library's choice)."""
         n = cycles * self.cycle
         share = {k: sum(alg_bytes_per_block(*self.specs[j]) // self.periods[j] for j in v)
                  for k, v in self.buckets.items()}
         chosen = {}
         self.time_ticks(n)  # warm-up
         for k in sorted(self.buckets, key=lambda k: -share[k]):
-            best = None
+            cur = self.layouts[self.buckets[k][0]]
+            best = (self._best_of(n), cur)
             for lay in layout_candidates(k[0], k[1]):
+                if lay == cur:
+                    continue
                 self.set_bucket_layout(k, lay)
                 self.time_ticks(self.cycle)  # past the rebuilt objects' first pass
-                ms = min(self.time_ticks(n) for _ in range(2))
-                if best is None or ms < best[0]:
+                ms = self._best_of(n)
+                if ms < best[0] * (1.0 - min_gain):
                     best = (ms, lay)
             chosen[k] = best[1]
-            self.set_bucket_layout(k, best[1])
+            if self.layouts[self.buckets[k][0]] != best[1]:
+                self.set_bucket_layout(k, best[1])
         return chosen
 
-    def tune(self, cycles=4, candidates=(1, 2, 4, 8, 16, 32, 64), passes=1):
+    def tune(self, cycles=4, candidates=(1, 2, 4, 8, 16, 32, 64), passes=1, min_gain=0.03):
         """Coordinate descent over buckets (largest byte share first): each
-        bucket tries every candidate chunk count with the others fixed and
-        keeps the one with the least device time over `cycles` full tick
-        cycles.  Returns {bucket: chunks}."""
+        bucket measures its current chunk count, then every candidate with
+        the others fixed, and switches only to one whose device time over
+        `cycles` full tick cycles is at least `min_gain` lower.  Returns
+        {bucket: chunks} (0: the library's choice)."""
         n = cycles * self.cycle
         share = {k: sum(alg_bytes_per_block(*self.specs[j]) // self.periods[j] for j in v)
                  for k, v in self.buckets.items()}
         order = sorted(self.buckets, key=lambda k: -share[k])
-        chosen = {}
-        for k in order:
-            chosen[k] = 0  # library default
+        chosen = {k: 0 for k in order}  # library default
         self.time_ticks(n)  # warm-up
         for _ in range(passes):
             for k in order:
                 b, ln = k
                 parts = max(1, ln // b - 2)
-                best = None
+                best = (self._best_of(n), chosen[k])
                 for c in candidates:
-                    if c > parts:
+                    if c > parts or c == chosen[k]:
                         continue
                     self.set_bucket_chunks(k, c)
-                    ms = min(self.time_ticks(n) for _ in range(2))
-                    if best is None or ms < best[0]:
+                    ms = self._best_of(n)
+                    if ms < best[0] * (1.0 - min_gain):
                         best = (ms, c)
                 chosen[k] = best[1]
                 self.set_bucket_chunks(k, best[1])
         return chosen
+
+    def restore_defaults(self):
+        """Every bucket back to the library's layout and chunk choice (the
+        objects are rebuilt and start over)."""
+        for key in self.buckets:
+            self.set_bucket_layout(key, None)

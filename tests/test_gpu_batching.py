@@ -91,7 +91,7 @@ def test_tuned_batch_within_tolerance(gpu_ctx):
     chosen = batch.tune(cycles=2, candidates=(1, 4, 16))
     assert set(chosen) == set(batch.buckets)
     for (b, ln), c in chosen.items():
-        assert 1 <= c <= max(1, ln // b - 2)
+        assert c == 0 or 1 <= c <= max(1, ln // b - 2)  # 0: the library's choice kept
     # a fresh batch of the same streams with the tuned chunks, from block 0
     tuned = Bt.StreamBatch(gpu_ctx, specs)
     for k, c in chosen.items():
@@ -105,7 +105,7 @@ def test_tuned_batch_within_tolerance(gpu_ctx):
 def test_layout_tuner_streams_equal_alone(gpu_ctx):
     """The partition-layout tuner (uniform with 4- or 8-block passes, or
     non-uniform frames of M x B, per length bucket, by measured tick time)
-    picks a candidate for every bucket; rebuilt from block 0 with the chosen
+    picks a candidate or keeps the library's choice for every bucket; rebuilt from block 0 with the chosen
     layouts, every stream equals its object alone created with the same
     options, bit for bit, and long ones match the oracle."""
     specs = Bt.draw_streams(24, seed=4, max_len=131072)
@@ -113,7 +113,7 @@ def test_layout_tuner_streams_equal_alone(gpu_ctx):
     chosen = batch.tune_layouts(cycles=1)
     assert set(chosen) == set(batch.buckets)
     for (b, ln), lay in chosen.items():
-        assert lay in Bt.layout_candidates(b, ln), (b, ln, lay)
+        assert lay is None or lay in Bt.layout_candidates(b, ln), (b, ln, lay)  # None: the library's choice kept
     print("tuned layouts:", {f"{b}/{ln}": lay for (b, ln), lay in sorted(chosen.items())}, flush=True)
     batch.reset()
     xs, ys = _drive(batch, 48, seed=5)
