@@ -137,6 +137,7 @@ def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD
 
 
 RT_GROUP = 2048        # channels per object (channel group) of the real-time count
+RT_BALANCED = 6        # groups of RT_GROUP channels per pass phase before the smaller ones
 RT_STEP_TICKS = 100    # ticks per timed step: two IR update periods
 RT_RING = 8            # input/output blocks kept on the device
 RT_BANKS = 4           # IR banks the updates cycle through
@@ -152,27 +153,52 @@ def rt_chunks():
     return max(1, 1024 // RT_GROUP)
 
 
+def rt_large():
+    """Groups of RT_GROUP channels: RT_BALANCED per pass phase; the groups
+    after them have half as many channels, one per phase in turn, so a
+    count between two multiples of RT_PASS large groups adds half a large
+    group's pass to some phases instead of a whole one."""
+    return RT_BALANCED * RT_PASS
+
+
+def rt_group_channels(g):
+    """Channels of group g."""
+    return RT_GROUP if g < rt_large() else RT_GROUP // 2
+
+
+def rt_channels(k):
+    """Channels of the first k groups."""
+    return sum(rt_group_channels(g) for g in range(k))
+
+
 def rt_phase(g):
     """Pass phase of group g: phases in the order 0, 2, 4, ..., 1, 3, 5, ...
-    round robin, so the phases that carry one group more than the others
-    (when the count is not a multiple of RT_PASS) pass on even ticks."""
+    round robin over the large groups, then again over the smaller ones, so
+    the phases that carry one group more than others pass on even ticks."""
     order = list(range(0, RT_PASS, 2)) + list(range(1, RT_PASS, 2))
-    return order[g % RT_PASS]
+    return order[(g if g < rt_large() else g - rt_large()) % RT_PASS]
 
 
 def rt_update_offset(g):
     """Update offset of group g in the 50-tick period: one group per offset
-    for the first 50, then the odd offsets 1, 3, 5, ... so that ticks with two
-    updates are odd ticks, never the even ticks of the heavier passes."""
+    for the first 50, then the odd offsets 1, 3, 5, ... for the other large
+    groups, so that ticks with two large updates are odd ticks; the smaller
+    groups take the even offsets (ticks with one large update), then the
+    odd offsets the large groups left."""
     if g < UPDATE_EVERY:
         return g
-    return (2 * (g - UPDATE_EVERY) + 1) % UPDATE_EVERY
+    big = rt_large()
+    if g < big:
+        return (2 * (g - UPDATE_EVERY) + 1) % UPDATE_EVERY
+    used = {(2 * (j - UPDATE_EVERY) + 1) % UPDATE_EVERY for j in range(UPDATE_EVERY, big)}
+    seq = list(range(0, UPDATE_EVERY, 2)) + [o for o in range(1, UPDATE_EVERY, 2) if o not in used]
+    return seq[(g - big) % len(seq)]
 
 
 class RealTimeRig:
-    """The real-time channel count: channel groups of RT_GROUP channels (one
-    object each, 65,536 taps, B = 256, RT_PASS-block passes with phases
-    rt_phase(g)), a fresh IR for every channel every 50 blocks (group g's
+    """The real-time channel count: channel groups of rt_group_channels(g)
+    channels (one object each, 65,536 taps, B = 256, RT_PASS-block passes
+    with phases rt_phase(g)), a fresh IR for every channel every 50 blocks (group g's
     update at the ticks t with (t + rt_update_offset(g)) % 50 == 0, so
     updates and their crossfades are staggered over the period), all k
     active groups advanced by one group launch per tick with one block in
@@ -185,15 +211,17 @@ class RealTimeRig:
         self.ctx = T.TANContext(device=0, conv_stream=self.hs, general_stream=self.hs)
         self.k = ngroups
         self.t = 0
+        self.off = [rt_channels(g) for g in range(ngroups + 1)]  # first channel of group g
         with torch.cuda.stream(self.hs):
             self.banks = [ir_bank_device(torch, RT_GROUP, L, 5000 + i) for i in range(RT_BANKS)]
-            self.x = noise_device(torch, RT_RING * ngroups * RT_GROUP, B, 11).view(RT_RING, ngroups * RT_GROUP, B)
+            total = self.off[ngroups]
+            self.x = noise_device(torch, RT_RING * total, B, 11).view(RT_RING, total, B)
             self.y = torch.zeros_like(self.x)
             self.objs = []
             for g in range(ngroups):
                 conv = T.TANConvolution(self.ctx, pass_blocks=RT_PASS, pass_phase=rt_phase(g),
                                         max_tail_chunks=rt_chunks(), ir_slots=RT_SLOTS)
-                rc = conv.InitGpu(UNIFORM_METHOD, L, B, RT_GROUP)
+                rc = conv.InitGpu(UNIFORM_METHOD, L, B, rt_group_channels(g))
                 if rc != T.AMF_OK:
                     log(f"  group {g}: InitGpu returned {rc}; {g} groups")
                     break
@@ -212,7 +240,7 @@ class RealTimeRig:
         from tan_amd.batching import GroupCall
         key = (k, r)
         if key not in self.calls:
-            rows = lambda a, g: a[r, g * RT_GROUP:(g + 1) * RT_GROUP]  # noqa: E731
+            rows = lambda a, g: a[r, self.off[g]:self.off[g + 1]]  # noqa: E731
             self.calls[key] = GroupCall(self.objs[:k], [rows(self.x, g) for g in range(k)],
                                         [rows(self.y, g) for g in range(k)])
         return self.calls[key]
@@ -232,7 +260,7 @@ class RealTimeRig:
             if (self.t + rt_update_offset(g)) % UPDATE_EVERY == 0:
                 self.updates[g] += 1
                 due.append(self.objs[g])
-                irs.append(self.banks[(self.updates[g] + g) % RT_BANKS])
+                irs.append(self.banks[(self.updates[g] + g) % RT_BANKS][:rt_group_channels(g)])
         if due:
             # the tick's updates in one call: their IR transforms run in one
             # launch on the process stream, ahead of this tick's group launch
@@ -265,7 +293,7 @@ class RealTimeRig:
         self.torch.cuda.empty_cache()
 
 
-def bench_realtime(T, torch, steps, warmup, margin=0.97):
+def bench_realtime(T, torch, steps, warmup, margin=0.98):
     """Largest channel count whose every tick of `steps` x 100 consecutive
     ticks (IR updates and crossfades included) finishes within the 5.333 ms
     block period with one block in flight.
@@ -278,13 +306,24 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
     one) after any miss.  The count is that of a run without a miss."""
     from tan_amd import serving as S
     gc_ = __import__("gc")
-    per_group = S.estimate_bytes(T._lib.load(), L, B, RT_GROUP, pass_blocks=RT_PASS, max_tail_chunks=rt_chunks(),
-                                 ir_slots=RT_SLOTS)
+    sizes = {}
+
+    def group_bytes(g):
+        c = rt_group_channels(g)
+        if c not in sizes:
+            sizes[c] = S.estimate_bytes(T._lib.load(), L, B, c, pass_blocks=RT_PASS, max_tail_chunks=rt_chunks(),
+                                        ir_slots=RT_SLOTS) + 2 * RT_RING * c * B * 4
+        return sizes[c]
+
     free, _ = torch.cuda.mem_get_info()
     reserve = RT_BANKS * RT_GROUP * L * 4 + (10 << 30)
-    kmax = int((free - reserve) // (per_group + 2 * RT_RING * RT_GROUP * B * 4))
-    log(f"real-time count: {free / 2**30:.0f} GiB free, {per_group / 2**20:.0f} MiB per group of {RT_GROUP} "
-        f"channels: up to {kmax} groups")
+    kmax, used = 0, 0
+    while used + group_bytes(kmax) <= free - reserve:
+        used += group_bytes(kmax)
+        kmax += 1
+    log(f"real-time count: {free / 2**30:.0f} GiB free, {group_bytes(0) / 2**20:.0f} MiB per group of {RT_GROUP} "
+        f"channels ({RT_BALANCED} per pass phase), then groups of {RT_GROUP // 2}: up to {kmax} groups, "
+        f"{rt_channels(kmax)} channels")
     rig = RealTimeRig(T, torch, kmax)
     try:
         k = rig.kmax
@@ -296,7 +335,7 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
         while k > 0:
             h, d = rig.run(k, 2 * UPDATE_EVERY)
             p99 = float(np.percentile(h, 99))
-            probes.append({"groups": k, "channels": k * RT_GROUP, "max_host_ms": float(h.max()),
+            probes.append({"groups": k, "channels": rt_channels(k), "max_host_ms": float(h.max()),
                            "p99_host_ms": p99, "max_device_ms": float(d.max())})
             log(f"  probe {probes[-1]}")
             if h.max() <= margin * DEADLINE_MS:
@@ -328,7 +367,7 @@ def bench_realtime(T, torch, steps, warmup, margin=0.97):
             gt = rig.ctx.ReadGroupTiming(reset=True)
             rig.ctx.EnableGroupTiming(False)
             misses = int((h > DEADLINE_MS).sum())
-            result = {"groups": k, "channels": k * RT_GROUP, "ticks": int(h.size), "wall_s": wall,
+            result = {"groups": k, "channels": rt_channels(k), "ticks": int(h.size), "wall_s": wall,
                       "misses": misses, "p50_host_ms": float(np.percentile(h, 50)),
                       "p99_host_ms": float(np.percentile(h, 99)), "max_host_ms": float(h.max()),
                       "p50_device_ms": float(np.percentile(d, 50)), "p99_device_ms": float(np.percentile(d, 99)),
@@ -883,7 +922,7 @@ def profile_run(T, torch, ngroups):
     known bytes."""
     rig = RealTimeRig(T, torch, ngroups)
     h, d = rig.run(rig.kmax, 200)
-    log(f"profile run: {rig.kmax} groups ({rig.kmax * RT_GROUP} channels), 200 ticks, max {h.max():.3f} ms")
+    log(f"profile run: {rig.kmax} groups ({rt_channels(rig.kmax)} channels), 200 ticks, max {h.max():.3f} ms")
     rig.close()
     a = torch.ones(512 << 18, device="cuda")
     b = torch.empty_like(a)
@@ -992,7 +1031,7 @@ def summarize_profiles(run_dir, groups):
     text and the PMC traffic as JSON, under profiles/."""
     import csv
     import collections
-    channels = groups * RT_GROUP
+    channels = rt_channels(groups)
     pre = "cd /tmp && export TMPDIR=/tmp && "
 
     def counters(path, counter):
@@ -1151,7 +1190,7 @@ def main():
                           f"single-filter pass runs this tick, one in {RT_PASS}, and the new filter's catch-up "
                           f"pass of each group whose IR switched, with their heads), k_group_tick<0, 256, 2, "
                           f"{RT_PASS}> when a switch falls on a group's pass block (its two-filter pass)",
-                "trace": f"profiles/k_group_tick_realtime_{PROFILE_GROUPS * RT_GROUP}ch_65536taps_b256.txt",
+                "trace": f"profiles/k_group_tick_realtime_{rt_channels(PROFILE_GROUPS)}ch_65536taps_b256.txt",
                 "traffic_source": "profiles/pmc_traffic_k_group_tick_rt.json" if per_ch else None,
                 "alg_bytes_per_launch": gt["tick_alg_bytes"] / launches, "avg_launch_us": tick_us,
                 "launches": gt["tick_launches"], "channels": value}
@@ -1244,7 +1283,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (decaying-noise IRs, U[-1,1] input, generated on device)",
         "config": {"workload": RT_WORKLOAD, "channels": value, "channel_groups": rt["groups"],
-                   "group_channels": RT_GROUP, "block": B, "ir_taps": L, "ticks": rt["ticks"],
+                   "group_channels": RT_GROUP, "large_groups": min(rt["groups"], rt_large()),
+                   "small_group_channels": RT_GROUP // 2, "block": B, "ir_taps": L, "ticks": rt["ticks"],
                    "ticks_per_step": RT_STEP_TICKS, "update_every": UPDATE_EVERY, "pass_blocks": RT_PASS,
                    "ir_slots": RT_SLOTS, "tail_chunks": rt_chunks(),
                    "sample_rate": FS, "deadline_ms": DEADLINE_MS},
