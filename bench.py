@@ -575,6 +575,13 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
             chosen = {k: 0 for k in chosen}
             kept_default = True
         tune_s = time.perf_counter() - t0
+        # the same ticks as work-queue launches (the device's resident
+        # workgroups take each launch's items from a counter)
+        ctx.SetGroupQueue(True)
+        batch.time_ticks(cyc * 8)
+        queue_ms = batch.time_ticks(cyc * 64) / (cyc * 64)
+        ctx.SetGroupQueue(False)
+        batch.time_ticks(cyc * 8)
         # steady stream of ticks: device time over the whole run, and the
         # launch-stamped group kernels with the bytes their roles streamed
         ctx.EnableGroupTiming(True)
@@ -588,6 +595,7 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
         row = {"streams": n, "base_block": batch.base, "deadline_ms": period,
                "objects_per_launch": [len(d) for d in batch.due],
                "tick_ms_default": before, "tick_ms_layouts_tuned": after_layout, "tick_ms_tuned": after,
+               "tick_ms_queue": queue_ms,
                "tune_seconds": tune_s,
                "kept_default": kept_default,
                "tuned_layouts": {f"B{b}_L{ln}": (f"{v[0]} {v[1]}" if v else "default")
@@ -600,7 +608,7 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
                / ticks / 1e6, "realtime_factor": period / (total_ms / ticks)}
         log(f"  batching {n} streams: tick {row['mean_tick_ms'] * 1e3:.1f} us (default "
             f"{before * 1e3:.1f} us, layouts tuned {after_layout * 1e3:.1f} us, tuned {after * 1e3:.1f} us"
-            f"{', defaults kept' if kept_default else ''}), p99 "
+            f"{', defaults kept' if kept_default else ''}, work-queue launches {queue_ms * 1e3:.1f} us), p99 "
             f"{row['p99_tick_ms'] * 1e3:.1f} us, {gb:.0f} GB/s, layouts {row['tuned_layouts']}")
         rows.append(row)
         for c in batch.convs:

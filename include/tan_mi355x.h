@@ -367,6 +367,15 @@ int32_t tanx_conv_read_timing(tanx_conv *conv, tanx_kernel_timing *out, uint32_t
  * objects advanced. */
 int32_t tanx_ctx_enable_group_timing(tanx_ctx *ctx, uint32_t enable);
 int32_t tanx_ctx_read_group_timing(tanx_ctx *ctx, tanx_kernel_timing *out, uint32_t reset);
+/* Work-queue group launches (no counterpart in the reference, which has no
+ * group launches; for batching callers that advance many uneven streams per
+ * tick): with enable, tanx_group_process_device runs each of its launches as
+ * a persistent kernel of the device's resident workgroups that take the
+ * launch's workgroup items from a counter in order (results identical to the
+ * plain launches).  The counters are an allocation of the context, made at
+ * the first such call; tanx_group_process_device_noalloc always uses the
+ * plain launches. */
+int32_t tanx_ctx_set_group_queue(tanx_ctx *ctx, uint32_t enable);
 /* 1 if the last processed block was a crossfade block. */
 int32_t tanx_conv_last_was_crossfade(const tanx_conv *conv);
 /* Stream on which process work is ordered (a hipStream_t). */

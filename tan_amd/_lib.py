@@ -92,6 +92,7 @@ _SIGS = {
     "tanx_conv_read_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_conv_process_stream": (_vp, [_vp]),
     "tanx_ctx_enable_group_timing": (_i32, [_vp, _u32]),
+    "tanx_ctx_set_group_queue": (_i32, [_vp, _u32]),
     "tanx_ctx_read_group_timing": (_i32, [_vp, ctypes.POINTER(KernelTiming), _u32]),
     "tanx_group_process_device": (_i32, [_u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(_sz), ctypes.POINTER(_vp), ctypes.POINTER(_sz),

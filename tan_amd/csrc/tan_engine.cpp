@@ -165,6 +165,10 @@ struct tanx_ctx {
     bool gtiming = false;
     KernelTimer gt_tick, gt_frame;
     uint64_t g_tick_bytes = 0, g_frame_bytes = 0, g_objects = 0;
+    // group calls as work-queue launches (tanx_ctx_set_group_queue): one
+    // counter per launch of a call, zeroed on the process stream before it
+    bool gqueue = false;
+    int *q_next = nullptr;
     hipStream_t proc = nullptr;  // convolution queue equivalent
     hipStream_t upd = nullptr;   // general queue equivalent (IR updates, FFT)
     bool own_proc = true, own_upd = true;  // created here (else the caller's)
@@ -289,6 +293,7 @@ static void ctx_free(tanx_ctx *ctx)
             if (ctx->own_upd) (void)hipStreamDestroy(ctx->upd);
         }
         if (ctx->tw) (void)hipFree(ctx->tw);
+        if (ctx->q_next) (void)hipFree(ctx->q_next);
         if (ctx->fft_dev) (void)hipFree(ctx->fft_dev);
         if (ctx->fft_scratch) (void)hipFree(ctx->fft_scratch);
         if (ctx->fft_host) (void)hipHostFree(ctx->fft_host);
@@ -2084,10 +2089,10 @@ static GroupStage group_layout(size_t cap)
 // present and the frame kernels.  Objects already advanced are launched even
 // if a later one fails (its error is returned).  staged: the event to record
 // after the copy (the host staging is free again once it completes), or
-// nullptr.
+// nullptr.  may_queue: the launches may take the context's work-queue form.
 static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const *convs, const float *const *d_in,
                          const size_t *in_stride, float *const *d_out, const size_t *out_stride,
-                         const uint32_t *const *flags, const GroupStage &g, hipEvent_t staged)
+                         const uint32_t *const *flags, const GroupStage &g, hipEvent_t staged, bool may_queue)
 {
     TickArgs *descs = reinterpret_cast<TickArgs *>(g.host);
     int *starts = reinterpret_cast<int *>(g.host + g.desc_bytes);
@@ -2177,13 +2182,24 @@ static int32_t group_run(tanx_ctx *ctx, int irtype, uint32_t n, tanx_conv *const
         if (ctx->gtiming) {
             TANX_CHECK(ctx->gt_tick.acquire(&t0, &t1));
         }
+        const bool queue = may_queue && ctx->gqueue;
+        if (queue) {
+            static_assert(kKinds * kClasses * sizeof(int) % 16 == 0, "counters in whole float4");
+            if (!ctx->q_next) TANX_CHECK(hipMalloc(&ctx->q_next, kKinds * kClasses * sizeof(int)));
+            TANX_CHECK(launch_zero(ctx->q_next, align16((size_t)nl * sizeof(int)), ctx->proc));
+        }
         for (int i = 0; i < nl; i++) {
             const Launch &l = launches[i];
             const TickArgs *dd = reinterpret_cast<const TickArgs *>(g.dev) + l.first;
             const int *ds = reinterpret_cast<const int *>(g.dev + g.desc_bytes) + l.first;
             // the first launch stamps the start event, the last one the end
-            TANX_CHECK(launch_group_tick(irtype, l.B, l.roles, l.km, dd, ds, (int)l.count, (unsigned)l.total,
-                                         ctx->proc, i == 0 ? t0 : nullptr, i == nl - 1 ? t1 : nullptr));
+            if (queue)
+                TANX_CHECK(launch_group_queue(irtype, l.B, l.roles, l.km, dd, ds, (int)l.count, l.total,
+                                              ctx->q_next + i, ctx->proc, i == 0 ? t0 : nullptr,
+                                              i == nl - 1 ? t1 : nullptr));
+            else
+                TANX_CHECK(launch_group_tick(irtype, l.B, l.roles, l.km, dd, ds, (int)l.count, (unsigned)l.total,
+                                             ctx->proc, i == 0 ? t0 : nullptr, i == nl - 1 ? t1 : nullptr));
         }
         if (ctx->gtiming) {
             ctx->gt_tick.commit();
@@ -2233,7 +2249,7 @@ extern "C" int32_t tanx_group_process_device(uint32_t n, tanx_conv *const *convs
     rc = group_staging(ctx, st.total(), &st.host, &slot);
     if (rc != TANX_OK) return rc;
     st.dev = ctx->g_dev;
-    return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, ctx->g_ev[slot]);
+    return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, ctx->g_ev[slot], true);
 }
 
 extern "C" int32_t tanx_group_update_td_device(uint32_t n, tanx_conv *const *convs, const float *const *d_irs,
@@ -2321,6 +2337,14 @@ extern "C" int32_t tanx_ctx_enable_group_timing(tanx_ctx *ctx, uint32_t enable)
     return TANX_OK;
 }
 
+extern "C" int32_t tanx_ctx_set_group_queue(tanx_ctx *ctx, uint32_t enable)
+{
+    if (!ctx) return TANX_INVALID_POINTER;
+    std::lock_guard<std::mutex> glk(ctx->group_mtx);
+    ctx->gqueue = enable != 0;
+    return TANX_OK;
+}
+
 extern "C" int32_t tanx_ctx_read_group_timing(tanx_ctx *ctx, tanx_kernel_timing *out, uint32_t reset)
 {
     if (!ctx || !out) return TANX_INVALID_POINTER;
@@ -2378,7 +2402,9 @@ extern "C" int32_t tanx_group_process_device_noalloc(uint32_t n, tanx_conv *cons
     GroupStage st = group_layout(n);
     st.host = g_noalloc_host;
     st.dev = dev;
-    return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, nullptr);
+    // the plain group launches: the work-queue form keeps its counters in an
+    // allocation of the context
+    return group_run(ctx, irtype, n, convs, d_in, in_stride, d_out, out_stride, flags, st, nullptr, false);
 }
 
 extern "C" int32_t tanx_conv_update_td(tanx_conv *cv, const float *const *ir, size_t n,

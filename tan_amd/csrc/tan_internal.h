@@ -246,6 +246,12 @@ hipError_t launch_ir_transform_group(int B, int irtype, const IrArgs *d_descs, c
 hipError_t launch_group_tick(int irtype, int B, int roles, int km, const TickArgs *d_descs, const int *d_starts,
                              int nobj, unsigned blocks, hipStream_t s, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);
+// Work-queue form of launch_group_tick: `total` workgroup items taken in
+// order from *next (zero before the launch) by the device's resident
+// workgroups of the kernel.
+hipError_t launch_group_queue(int irtype, int B, int roles, int km, const TickArgs *d_descs, const int *d_starts,
+                              int nobj, int total, int *next, hipStream_t s, hipEvent_t ev0 = nullptr,
+                              hipEvent_t ev1 = nullptr);
 // Role set of a group descriptor (kRoles*): heads with a single-filter pass
 // (lean), with a two-filter pass (crossfade), with a single tail or a frame
 // slice (general), or heads alone (heads: wave_heads(B) per workgroup).

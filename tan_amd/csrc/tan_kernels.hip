@@ -19,6 +19,10 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 namespace tanx {
 
 // Loads and stores through the global address space.  A pointer read from
@@ -1025,6 +1029,45 @@ __global__ __launch_bounds__(kTickThreads, 4) void k_group_heads(const TickArgs 
     const TickArgs a = descs[lo];
     if (a.B != M) return;
     head_block<M, HT, true>(a, b - starts[lo], sm);
+}
+
+// Work-queue form of a group launch: a grid of the device's resident
+// workgroups takes the launch's workgroup items [0, total) in order from
+// *next (zero before the launch) and runs each one as k_group_tick or, for
+// the heads kind, k_group_heads would run that workgroup (the same roles,
+// so the same bits).  Every item is in the descriptors before the launch:
+// a workgroup leaves once the counter has passed total, waiting on nothing.
+template <int HT, int M, int ROLES, int KM>
+__global__ __launch_bounds__(kTickThreads) void k_group_queue(const TickArgs *__restrict__ descs,
+                                                              const int *__restrict__ starts, int nobj, int total,
+                                                              int *__restrict__ next)
+{
+    constexpr int kTailRed = tail_red_float4<M>();
+    constexpr int kHeadBytes = (ROLES == kRolesHeads ? wave_heads(M) : 1) * (int)sizeof(HeadSmem<M>);
+    constexpr int kBytes = kHeadBytes > kTailRed * 16 ? kHeadBytes : kTailRed * 16;
+    __shared__ float4 smem[(kBytes + 15) / 16];
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(next, 1);
+        __syncthreads();
+        const int b = item;
+        __syncthreads();  // every thread has read item before the next one is taken
+        if (b >= total) break;
+        int lo = 0, hi = nobj - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (starts[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const TickArgs a = descs[lo];
+        if (a.B == M) {
+            if constexpr (ROLES == kRolesHeads)
+                head_block<M, HT, true>(a, b - starts[lo], reinterpret_cast<HeadSmem<M> *>(smem));
+            else
+                tick_body<M, HT, ROLES, KM>(a, b - starts[lo], smem);
+        }
+        __syncthreads();  // the item's LDS use is over before the next item
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2061,6 +2104,88 @@ hipError_t launch_group_tick(int irtype, int B, int roles, int km, const TickArg
     case IR_F32: e = launch_group_t<IR_F32>(B, roles, km, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
     case IR_F16: e = launch_group_t<IR_F16>(B, roles, km, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
     case IR_BF16: e = launch_group_t<IR_BF16>(B, roles, km, d_descs, d_starts, nobj, blocks, s, ev0, ev1); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+// Resident workgroups of one kernel on the current device (cached per kernel)
+static unsigned resident_blocks(const void *kernel)
+{
+    static std::mutex mtx;
+    static std::map<std::pair<int, const void *>, unsigned> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = cache.find({dev, kernel});
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTickThreads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const unsigned n = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus);
+    cache[{dev, kernel}] = n;
+    return n;
+}
+
+template <int HT, int M, int ROLES, int KM>
+static hipError_t launch_queue_k(const TickArgs *d, const int *st, int nobj, int total, int *next, hipStream_t s,
+                                 hipEvent_t ev0, hipEvent_t ev1)
+{
+    const void *k = reinterpret_cast<const void *>(&k_group_queue<HT, M, ROLES, KM>);
+    const unsigned res = resident_blocks(k);
+    if (res == 0) return hipErrorInvalidValue;
+    const unsigned grid = std::min(res, (unsigned)total);
+    hipExtLaunchKernelGGL((k_group_queue<HT, M, ROLES, KM>), dim3(grid), dim3(kTickThreads), 0, s, ev0, ev1, 0, d, st,
+                          nobj, total, next);
+    return hipSuccess;
+}
+
+template <int HT, int M>
+static hipError_t launch_queue_m(int roles, int km, const TickArgs *d, const int *st, int nobj, int total, int *next,
+                                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (km != 4 && km != 8 && km != 12 && km != 16) return hipErrorInvalidValue;
+#define TANX_QUEUE_KM(R)                                                                  \
+    (km == 4 ? launch_queue_k<HT, M, R, 4>(d, st, nobj, total, next, s, ev0, ev1)        \
+     : km == 8 ? launch_queue_k<HT, M, R, 8>(d, st, nobj, total, next, s, ev0, ev1)      \
+     : km == 12 ? launch_queue_k<HT, M, R, 12>(d, st, nobj, total, next, s, ev0, ev1)    \
+                : launch_queue_k<HT, M, R, 16>(d, st, nobj, total, next, s, ev0, ev1))
+    hipError_t e;
+    switch (roles) {
+    case kRolesGeneral: e = launch_queue_k<HT, M, kRolesGeneral, 4>(d, st, nobj, total, next, s, ev0, ev1); break;
+    case kRolesLean: e = TANX_QUEUE_KM(kRolesLean); break;
+    case kRolesXfade: e = TANX_QUEUE_KM(kRolesXfade); break;
+    case kRolesHeads: e = launch_queue_k<HT, M, kRolesHeads, 4>(d, st, nobj, total, next, s, ev0, ev1); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef TANX_QUEUE_KM
+    return e;
+}
+
+template <int HT>
+static hipError_t launch_queue_t(int B, int roles, int km, const TickArgs *d, const int *st, int nobj, int total,
+                                 int *next, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    switch (B) {
+    case 64: return launch_queue_m<HT, 64>(roles, km, d, st, nobj, total, next, s, ev0, ev1);
+    case 128: return launch_queue_m<HT, 128>(roles, km, d, st, nobj, total, next, s, ev0, ev1);
+    case 256: return launch_queue_m<HT, 256>(roles, km, d, st, nobj, total, next, s, ev0, ev1);
+    case 512: return launch_queue_m<HT, 512>(roles, km, d, st, nobj, total, next, s, ev0, ev1);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_group_queue(int irtype, int B, int roles, int km, const TickArgs *d_descs, const int *d_starts,
+                              int nobj, int total, int *next, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (total <= 0) return hipSuccess;
+    hipError_t e;
+    switch (irtype) {
+    case IR_F32: e = launch_queue_t<IR_F32>(B, roles, km, d_descs, d_starts, nobj, total, next, s, ev0, ev1); break;
+    case IR_F16: e = launch_queue_t<IR_F16>(B, roles, km, d_descs, d_starts, nobj, total, next, s, ev0, ev1); break;
+    case IR_BF16: e = launch_queue_t<IR_BF16>(B, roles, km, d_descs, d_starts, nobj, total, next, s, ev0, ev1); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;

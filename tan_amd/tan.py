@@ -157,6 +157,11 @@ class TANContext:
         if update and cur.cuda_stream != self._raw[1]:
             self._streams[1].wait_stream(cur)
 
+    def SetGroupQueue(self, enable=True):
+        """Group calls of this context as work-queue launches (the device's
+        resident workgroups take the launch's items from a counter)."""
+        return self._lib.tanx_ctx_set_group_queue(self.handle, int(bool(enable)))
+
     def EnableGroupTiming(self, enable=True):
         """Launch-stamped timing of this context's group launches."""
         return self._lib.tanx_ctx_enable_group_timing(self.handle, int(bool(enable)))

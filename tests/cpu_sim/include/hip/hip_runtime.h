@@ -136,6 +136,10 @@ hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags = 0);
+// a small simulated device: 4 compute units, 2 resident workgroups each
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 63 };
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t attr, int device);
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *kernel, int threads, size_t smem);
 hipError_t hipEventCreate(hipEvent_t *e);
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);

@@ -246,6 +246,57 @@ def noalloc_group():
     print("noalloc group: outputs bit-identical", flush=True)
 
 
+def queue_group(ncalls=40):
+    """Work-queue group launches (tanx_ctx_set_group_queue) beside the plain
+    ones on twin objects of two contexts: multi-block passes of several
+    lengths, a non-uniform object, IR updates with their crossfades, calls
+    whose object count drops and rises; bit-identical outputs."""
+    NU = T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
+    specs = [(65536, 256, 4, UNIFORM, 4), (5000, 128, 2, UNIFORM, 0), (131072, 512, 1, UNIFORM, 8),
+             (32768, 256, 2, NU, 0), (16384, 64, 3, UNIFORM, 12), (65536, 256, 3, UNIFORM, 12)]
+    calls = [[0, 1, 2, 3, 4, 5], [1, 3, 4, 5]] * (ncalls // 2)
+    irs = [[simlib.Dev(C, L, T.signals.ir_bank(C, L, u, base_seed=91 + i)) for u in range(2)]
+           for i, (L, B, C, m, kp) in enumerate(specs)]
+    sets = []
+    for q in range(2):
+        ctx = T.TANContext(device=0)
+        assert ctx.SetGroupQueue(q == 1) == T.AMF_OK
+        objs = []
+        for i, (L, B, C, m, kp) in enumerate(specs):
+            kw = {"pass_blocks": kp, "pass_phase": i % kp} if kp else {}
+            c = T.TANConvolution(ctx, **kw)
+            assert c.InitGpu(m, L, B, C) == T.AMF_OK
+            assert lib.tanx_conv_update_td_device(c.handle, irs[i][0].ptr, L, L, None, 1) == T.AMF_OK
+            objs.append(c)
+        sets.append(objs)
+    nblk = [sum(1 for c in calls if i in c) for i in range(len(specs))]
+    xs = [simlib.Dev(C, n * B, np.random.default_rng(i).uniform(-1, 1, (C, n * B)))
+          for i, ((L, B, C, m, kp), n) in enumerate(zip(specs, nblk))]
+    ys = [[simlib.Dev(C, n * B, 0.0) for (L, B, C, m, kp), n in zip(specs, nblk)] for _ in range(2)]
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    blk = [0] * len(specs)
+    for t, idx in enumerate(calls):
+        for objs, y in zip(sets, ys):
+            if t % 9 == 4:
+                # a fresh IR for two objects (crossfades in the next calls)
+                for i in (0, 5):
+                    L = specs[i][0]
+                    rc = lib.tanx_conv_update_td_device(objs[i].handle, irs[i][(t // 9) % 2 ^ 1].ptr, L, L, None, 0)
+                    assert rc == T.AMF_OK, rc
+            n = len(idx)
+            hs = (vp * n)(*[objs[i].handle.value for i in idx])
+            ins = (vp * n)(*[xs[i].at(specs[i][1] * blk[i]) for i in idx])
+            outs = (vp * n)(*[y[i].at(specs[i][1] * blk[i]) for i in idx])
+            st = (sz * n)(*[xs[i].stride for i in idx])
+            assert lib.tanx_group_process_device(n, hs, ins, st, outs, st, None) == T.AMF_OK
+        for i in idx:
+            blk[i] += 1
+    for i in range(len(specs)):
+        assert np.array_equal(ys[0][i].a, ys[1][i].a), i
+        assert float(np.abs(ys[1][i].a).max()) > 0.01, i
+    print("queue group: outputs bit-identical", flush=True)
+
+
 def batched_calls(kp=8, B=256, precision=0):
     """Calls of several blocks with K-block passes (heads batched with the
     next blocks' pass, head-only batches, catch-up passes after updates)

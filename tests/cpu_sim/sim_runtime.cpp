@@ -497,6 +497,17 @@ hipError_t hipStreamDestroy(hipStream_t s)
 }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t attr, int)
+{
+    if (attr != hipDeviceAttributeMultiprocessorCount) return hipErrorInvalidValue;
+    *v = 4;
+    return hipSuccess;
+}
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int, size_t)
+{
+    *n = 2;
+    return hipSuccess;
+}
 hipError_t hipEventCreate(hipEvent_t *e)
 {
     *e = new sim_event();

@@ -87,3 +87,54 @@ def test_group_rejects_mixed_objects(gpu_ctx):
     assert T.GroupProcess([a, a], [x, x], [y, y]) == T.AMF_INVALID_ARG
     a.Terminate()
     b.Terminate()
+
+
+def test_queue_launches_equal_plain_launches():
+    """Work-queue group launches (tanx_ctx_set_group_queue: the device's
+    resident workgroups take the launch's items from a counter) give the
+    plain group launches' outputs bit for bit: objects with multi-block
+    passes of 4, 8 and 12 blocks on several phases, a non-uniform object and
+    IR switches with their crossfades, over uneven calls; long enough that
+    every kind (heads only, lean and crossfade passes, general) runs."""
+    import torch
+    NU = T.TAN_CONVOLUTION_METHOD_FFT_PARTITIONED_NONUNIFORM
+    specs = [(65536, 256, 8, UNIFORM, 12), (8000, 128, 3, UNIFORM, 0), (65536, 512, 2, UNIFORM, 8),
+             (32768, 256, 4, NU, 0), (16384, 64, 5, UNIFORM, 4), (65536, 256, 16, UNIFORM, 12)]
+    nb = 96
+    outs = []
+    for q in (False, True):
+        ctx = T.TANContext(device=0)
+        assert ctx.SetGroupQueue(q) == T.AMF_OK
+        objs, xs, ys, irs = [], [], [], []
+        for i, (L, B, C, m, kp) in enumerate(specs):
+            kw = {"pass_blocks": kp, "pass_phase": (3 * i) % kp} if kp else {}
+            c = T.TANConvolution(ctx, **kw)
+            assert c.InitGpu(m, L, B, C) == T.AMF_OK
+            irs.append([T.signals.ir_bank(C, L, u, base_seed=300 + i) for u in range(2)])
+            assert c.UpdateResponseTD(irs[i][0], L, None, BLOCKING) == T.AMF_OK
+            objs.append(c)
+            xs.append(torch.from_numpy(T.signals.input_bank(C, nb * B, base_seed=400 + i)).cuda())
+            ys.append(torch.zeros(C, nb * B, device="cuda"))
+        torch.cuda.synchronize()
+        blk = [0] * len(specs)
+        for t in range(nb):
+            idx = [i for i in range(len(specs)) if t % 3 or i % 2 == 0]
+            if t % 17 == 5:
+                for i in (0, 4, 5):
+                    L = specs[i][0]
+                    assert objs[i].UpdateResponseTD(irs[i][(t // 17) % 2 ^ 1], L, None, BLOCKING) == T.AMF_OK
+            rc = T.GroupProcess([objs[i] for i in idx],
+                                [xs[i][:, blk[i] * specs[i][1]:] for i in idx],
+                                [ys[i][:, blk[i] * specs[i][1]:] for i in idx])
+            assert rc == T.AMF_OK
+            for i in idx:
+                blk[i] += 1
+        for c in objs:
+            c.Synchronize()
+        outs.append([y[:, :blk[i] * specs[i][1]].cpu().numpy() for i, y in enumerate(ys)])
+        for c in objs:
+            c.Terminate()
+        ctx.Terminate()
+    for i in range(len(specs)):
+        np.testing.assert_array_equal(outs[1][i], outs[0][i], err_msg=f"object {i} {specs[i]}")
+        assert float(np.abs(outs[1][i]).max()) > 0.01, i

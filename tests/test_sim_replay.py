@@ -64,6 +64,14 @@ def test_noalloc_group(sim_built, guard, fill):
     assert p.returncode == 0, p.returncode
 
 
+@pytest.mark.parametrize("guard,fill", [("end", None), ("start", "0xff")])
+def test_queue_group(sim_built, guard, fill):
+    """Work-queue group launches: a grid of 8 simulated resident workgroups
+    takes every launch's items; outputs equal the plain launches'."""
+    p = _run("queue_group", guard, fill)
+    assert p.returncode == 0, p.returncode
+
+
 @pytest.mark.parametrize("kp", [4, 8, 16])
 def test_pass_phases(sim_built, kp):
     p = _run("pass_phases", "start", "0xff", kp=kp)
