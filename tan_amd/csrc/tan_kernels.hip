@@ -1526,11 +1526,19 @@ __device__ __forceinline__ void ir_transform_rows_part(const IrArgs &a, int xb, 
         twn[m] = gld(a.twN + m);
     }
     __syncthreads();
-    const float2 *Z = fft_lds_n<M, false>(s0, s1, nr, tw);
     float2 *rows = reinterpret_cast<float2 *>(a.H) + ((size_t)ch * a.P + i0) * M;
-    for (int e = tid; e < nr * M; e += nt) {
-        const int r = e / M, k = e - r * M;
-        gst(rows + e, r2c_post<M>(Z + r * M, k, twn));
+    if constexpr (NR == kTickThreads / 64 && M / 4 <= 64) {
+        // one row per wave: its stages wait on the wave alone
+        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = tid & 63;
+        if (w >= nr) return;
+        const float2 *Z = fft_lds<M, false, 64>(s0 + w * M, s1 + w * M, tw);
+        for (int k = lane; k < M; k += 64) gst(rows + (size_t)w * M + k, r2c_post<M>(Z, k, twn));
+    } else {
+        const float2 *Z = fft_lds_n<M, false>(s0, s1, nr, tw);
+        for (int e = tid; e < nr * M; e += nt) {
+            const int r = e / M, k = e - r * M;
+            gst(rows + e, r2c_post<M>(Z + r * M, k, twn));
+        }
     }
 }
 
