@@ -569,7 +569,7 @@ def bench_batching(T, torch, ctx, counts=(512,), ticks=2000, seed=7):
             # the tuned batch measured slower than the library's choices
             # (ticks near their host enqueue time): keep the defaults
             batch.restore_defaults()
-            batch.time_ticks(cyc * 8)
+            batch.time_ticks(cyc * 16)  # rebuilt from block 0: past the first passes again
             after = batch.time_ticks(cyc * 64) / (cyc * 64)
             layouts = {k: None for k in layouts}
             chosen = {k: 0 for k in chosen}
