@@ -137,7 +137,6 @@ def bench_throughput(T, torch, ctx, steps, warmup, timing, method=UNIFORM_METHOD
 
 
 RT_GROUP = 2048        # channels per object (channel group) of the real-time count
-RT_BALANCED = 6        # groups of RT_GROUP channels per pass phase before the smaller ones
 RT_STEP_TICKS = 100    # ticks per timed step: two IR update periods
 RT_RING = 8            # input/output blocks kept on the device
 RT_BANKS = 4           # IR banks the updates cycle through
@@ -153,12 +152,18 @@ def rt_chunks():
     return max(1, 1024 // RT_GROUP)
 
 
+def rt_balanced():
+    """Groups of RT_GROUP channels per pass phase before the smaller ones
+    (72 large groups in all at 8- or 12-block passes, 64 at 16)."""
+    return max(1, 72 // RT_PASS)
+
+
 def rt_large():
-    """Groups of RT_GROUP channels: RT_BALANCED per pass phase; the groups
+    """Groups of RT_GROUP channels: rt_balanced() per pass phase; the groups
     after them have half as many channels, one per phase in turn, so a
     count between two multiples of RT_PASS large groups adds half a large
     group's pass to some phases instead of a whole one."""
-    return RT_BALANCED * RT_PASS
+    return rt_balanced() * RT_PASS
 
 
 def rt_group_channels(g):
@@ -322,7 +327,7 @@ def bench_realtime(T, torch, steps, warmup, margin=0.98):
         used += group_bytes(kmax)
         kmax += 1
     log(f"real-time count: {free / 2**30:.0f} GiB free, {group_bytes(0) / 2**20:.0f} MiB per group of {RT_GROUP} "
-        f"channels ({RT_BALANCED} per pass phase), then groups of {RT_GROUP // 2}: up to {kmax} groups, "
+        f"channels ({rt_balanced()} per pass phase), then groups of {RT_GROUP // 2}: up to {kmax} groups, "
         f"{rt_channels(kmax)} channels")
     rig = RealTimeRig(T, torch, kmax)
     try:

@@ -709,6 +709,108 @@ __device__ void mac_rows(const MacView &w, float4 *red)
     }
 }
 
+// One lane group's share of a split pass: outputs [OFF, OFF + KH) over every
+// partition of the chunk, the window and accumulators of KH outputs per lane;
+// output OFF + k reads FDL row (tbase + OFF + k - i), so the loop is the
+// pass's own with base and gap moved by OFF.  Sums go out directly.
+template <int M, int NF, int HT, bool NT, int KH, int OFF>
+__device__ void tailk_half(const TickArgs &a, int c, int j)
+{
+    constexpr int H = M / 2;                     // float4 per row
+    constexpr int V = H > 256 ? H / 256 : 1;     // float4 per lane per row
+    constexpr int R = H / V;                     // lanes per row
+    constexpr int U = 4;                         // partitions loaded together (8 measured slower)
+    static_assert(KH >= 1 && KH <= kPassMax, "pass length");
+    constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
+    const int tid = threadIdx.x;
+    const int r = tid % R;
+    const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
+    const char *HA = reinterpret_cast<const char *>(a.tHa) + (size_t)c * a.P * kRowBytes;
+    const char *HB = NF == 2 ? reinterpret_cast<const char *>(a.tHb) + (size_t)c * a.P * kRowBytes : nullptr;
+    const float *SA = HT == IR_F32 ? nullptr : a.tSa + (size_t)c * a.P;
+    const float *SB = (HT == IR_F32 || NF == 1) ? nullptr : a.tSb + (size_t)c * a.P;
+    const int mask = a.P - 1, base = a.tbase + OFF, kn = a.kn - OFF, gap = a.kgap + OFF;
+    const int i0 = 2 + j * a.pc, i1 = min(a.P, i0 + a.pc);
+    const int p0 = i0, p1 = i1;
+    bool wantA[KH], wantB[KH];
+#pragma unroll
+    for (int k = 0; k < KH; k++) {
+        wantA[k] = k < kn && a.kTA[OFF + k] != nullptr;
+        wantB[k] = NF == 2 && k < kn && a.kTB[OFF + k] != nullptr;
+    }
+    if (kn <= 0) return;  // a pass of at most OFF outputs: none of this group's
+    const bool dc0 = (r == 0);
+    const float s0 = dc0 ? 0.f : 1.f;
+    auto frow = [&](int row, int v) { return ld_row<NT>(F + (size_t)(row & mask) * H + v * R + r); };
+    auto hrow2 = [&](const char *row, int q) { return NT ? IrRow<HT>::get2(row, q) : IrRow<HT>::getc(row, q); };
+
+    float4 accA[KH][V], accB[KH][V], xw[KH][V];
+#pragma unroll
+    for (int k = 0; k < KH; k++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            accA[k][v] = make_float4(0.f, 0.f, 0.f, 0.f);
+            accB[k][v] = make_float4(0.f, 0.f, 0.f, 0.f);
+            // the window as it stands before partition p0: xw[k] = row (base + k + 1 - p0)
+            // (output k + 1 reads it at p0), loaded only where some output uses it
+            xw[k][v] = (k + 1 < KH && p0 < p1 && p0 >= k + 2 + gap) ? frow(base + k + 1 - p0, v)
+                                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    for (int i = p0; i < p1; i += U) {
+        float4 xn[U][V], ha[U][V], hb[U][V];
+        float sa[U], sb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u;
+            sa[u] = 1.f;
+            sb[u] = 1.f;
+            if (ii < p1) {
+                sa[u] = row_scale<HT>(SA, ii);
+                if (NF == 2) sb[u] = row_scale<HT>(SB, ii);
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                if (ii < p1) {
+                    xn[u][v] = frow(base - ii, v);
+                    ha[u][v] = hrow2(HA + (size_t)ii * kRowBytes, v * R + r);
+                    if (NF == 2) hb[u][v] = hrow2(HB + (size_t)ii * kRowBytes, v * R + r);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u;
+            if (ii >= p1) break;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+#pragma unroll
+                for (int k = KH - 1; k > 0; k--) xw[k][v] = xw[k - 1][v];
+                xw[0][v] = xn[u][v];
+                const bool dc = dc0 && v == 0;
+                const float sv = v == 0 ? s0 : 1.f;
+                const float4 hA = HT == IR_F32 ? ha[u][v] : scale4(ha[u][v], sa[u]);
+                float4 hB = hA;
+                if (NF == 2) hB = HT == IR_F32 ? hb[u][v] : scale4(hb[u][v], sb[u]);
+#pragma unroll
+                for (int k = 0; k < KH; k++) {
+                    if (ii >= k + 1 + gap) {
+                        if (wantA[k]) mac4(accA[k][v], xw[k][v], hA, sv, dc);
+                        if (NF == 2 && wantB[k]) mac4(accB[k][v], xw[k][v], hB, sv, dc);
+                    }
+                }
+            }
+        }
+    }
+    const size_t orow = ((size_t)c * a.nchunk + j) * H;
+#pragma unroll
+    for (int k = 0; k < KH; k++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            if (wantA[k]) reinterpret_cast<float4 *>(a.kTA[OFF + k])[orow + v * R + r] = accA[k][v];
+            if (NF == 2 && wantB[k]) reinterpret_cast<float4 *>(a.kTB[OFF + k])[orow + v * R + r] = accB[k][v];
+        }
+}
+
 // Multi-block pass (kn >= 1 outputs): partitions [2 + j*pc, 2 + (j+1)*pc)
 // of channel c for kn consecutive blocks.  Output k (0-based, block
 // o + g + k, o the launch's first head block, g = kgap) takes partition
@@ -728,6 +830,16 @@ __device__ void tailk_role(const TickArgs &a, int c, int j, float4 *red)
     constexpr size_t kRowBytes = (size_t)M * IrRow<HT>::kBytes;
     const int tid = threadIdx.x;
     const int g = tid / R, r = tid % R;
+    if constexpr (KM == kPassMax && G == 2 && R % 64 == 0) {
+        // 16 outputs: each lane group takes 8 of them over every partition
+        // (8 window rows and 8 accumulators per lane instead of 16, and no
+        // reduction between the groups); the second group's FDL rows are the
+        // first group's of 8 partitions before, its IR rows the same ones
+        // (ordinary loads: a row must stay in the caches for the other group)
+        if (__builtin_amdgcn_readfirstlane(g) == 0) tailk_half<M, NF, HT, false, kPassMax / 2, 0>(a, c, j);
+        else tailk_half<M, NF, HT, false, kPassMax / 2, kPassMax / 2>(a, c, j);
+        return;
+    }
     const float4 *F = reinterpret_cast<const float4 *>(a.fdl) + (size_t)c * a.P * H;
     const char *HA = reinterpret_cast<const char *>(a.tHa) + (size_t)c * a.P * kRowBytes;
     const char *HB = NF == 2 ? reinterpret_cast<const char *>(a.tHb) + (size_t)c * a.P * kRowBytes : nullptr;

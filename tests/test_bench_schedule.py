@@ -17,7 +17,7 @@ def _phase_channels(k):
 
 def test_group_sizes_and_channels():
     big = bench.rt_large()
-    assert big == bench.RT_BALANCED * bench.RT_PASS
+    assert big == bench.rt_balanced() * bench.RT_PASS
     assert all(bench.rt_group_channels(g) == bench.RT_GROUP for g in range(big))
     assert bench.rt_group_channels(big) == bench.RT_GROUP // 2
     assert bench.rt_channels(big) == big * bench.RT_GROUP
